@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark: ms per p' solve to 1e-8 L1 relative residual at 256^3.
+
+Workload: the 256^3-cell log-normal (sigma = 1) Neumann p' operator on a
+uniform grid with the reference-pressure cell pinned, RHS N(0,1), fi0 = 0
+(generated by the library's seeded generator, resident in HBM before the
+timed region).  One step = one complete p' solve from fi = 0 to
+sum|r| / sum|r0| < 1e-8 (the reference's stop test, pcg-sip.f90:186).
+
+The headline `value` is the plain FP64 drop-in, PCG with the SIP
+preconditioner (alfa = 0: with the reference's alfa = 0.92 CG does not reach
+1e-8 on this symmetric operator).  The other FP64 solvers and the
+mixed-precision path are timed beside it under "solvers".
+
+roofline: the dominant kernel of one PCG-SIP iteration, timed live on the
+library's stream with HIP events; algorithmic bytes per launch = array
+streams x 8 B x interior cells.  cpu_baseline: the single-threaded C
+restatement of pcgsip (the oracle) on the host, on a bounded number of
+iterations of the same 256^3 system, scaled to the GPU's iteration count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("ms per p' solve to 1e-8 rel. residual at 256^3 (plain FP64 vs learned+mixed); %HBM")
+HBM_PEAK_GBS = 8000.0
+# algorithmic FP64 bytes per interior cell of the kernels of one PCG-SIP iteration
+KERNEL_BYTES = {"sip_fwd": 48, "sip_bwd_dot": 48, "pk_update": 24, "spmv_dot": 72,
+                "update_l1": 48}
+KERNEL_NAMES = list(KERNEL_BYTES)
+ITER_BYTES = sum(KERNEL_BYTES.values())   # 240
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256, help="interior cells per direction")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-iters", type=int, default=24,
+                    help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
+    ap.add_argument("--others", default="cgstab_sip,iccg,mixed",
+                    help="comma list of extra solves to time once each")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the p' path runs on one MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import __graft_entry__ as GE
+    cap = GE.load_package()
+    torch.cuda.set_device(0)
+    n = args.n
+    dims = (n + 4, n + 4, n + 4)
+    ncell = n ** 3
+    npad = dims[0] * dims[1] * dims[2]
+
+    t0 = time.time()
+    arrs = {f: torch.zeros(npad, dtype=torch.float64, device="cuda")
+            for f in ("ae", "aw", "an", "as_", "at", "ab", "ap", "su")}
+    coef = cap.Coef(**arrs)
+    ctx = cap.Context(*dims, device_pointers=True)
+    ctx.build_lognormal(coef, args.seed, 1.0)
+    torch.cuda.synchronize()
+    log(f"[bench] generated {n}^3 system in {time.time() - t0:.1f}s")
+    fi = torch.zeros(npad, dtype=torch.float64, device="cuda")
+
+    def solve_pcg():
+        fi.zero_()
+        return ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=20000)
+
+    for w in range(args.warmup):
+        rep = solve_pcg()
+        log(f"[bench] warmup {w}: iters {rep['iters']} conv {rep['converged']} {rep['ms']:.1f} ms")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    reps = []
+    for k in range(args.steps):
+        reps.append(solve_pcg())
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t1) * 1e3 / args.steps
+    rep = reps[-1]
+    log(f"[bench] pcgsip: {ms_step:.1f} ms/solve, iters {rep['iters']}, conv {rep['converged']}")
+    iters = rep["iters"]
+
+    # live per-kernel timing on the library's stream (HIP events)
+    kms = ctx.time_kernels(fp32=False, reps=10)
+    kern = {}
+    for name, t in zip(KERNEL_NAMES, kms):
+        gbs = KERNEL_BYTES[name] * ncell / (t * 1e-3) / 1e9
+        kern[name] = {"ms": round(t, 4), "GBps": round(gbs, 1),
+                      "bytes_per_launch": KERNEL_BYTES[name] * ncell}
+    iter_ms = sum(kms)
+    dom = max(KERNEL_NAMES, key=lambda q: kern[q]["ms"])
+    achieved = kern[dom]["GBps"]
+    log(f"[bench] kernels: {json.dumps(kern)}")
+
+    solvers = {"pcgsip_fp64_alfa0": {"ms": round(ms_step, 2), "iters": iters,
+                                     "converged": rep["converged"],
+                                     "ms_per_iter": round(ms_step / max(iters, 1), 4),
+                                     "effective_GBps": round(ITER_BYTES * ncell * iters /
+                                                             (ms_step * 1e-3) / 1e9, 1)}}
+    for other in [o for o in args.others.split(",") if o]:
+        fi.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        if other == "cgstab_sip":
+            r = ctx.cgstab_sip(fi, coef, alfa=0.92, sor=1e-8, nsw=20000)
+        elif other == "iccg":
+            r = ctx.iccg(fi, coef, sor=1e-8, nsw=20000)
+        elif other == "mixed":
+            r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=1e-3)
+        else:
+            continue
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) * 1e3
+        solvers[other] = {"ms": round(dt, 2), "iters": r["iters"], "converged": r["converged"]}
+        log(f"[bench] {other}: {dt:.1f} ms, iters {r['iters']}, conv {r['converged']}")
+
+    cpu = None
+    if args.cpu_iters > 0:
+        O = GE.load_oracle()
+        t = time.time()
+        s = O.build_lognormal(dims, args.seed, 1.0)
+        fi_c = np.zeros(s.n)
+        sec = O.pcgsip_time_iters(s, fi_c, args.cpu_iters, alfa=0.0)
+        per_it = sec / args.cpu_iters
+        cpu = {"value": round(per_it * iters * 1e3, 1), "unit": "ms per p' solve",
+               "cores": 1, "kind": "port",
+               "sample": (f"{args.cpu_iters} PCG-SIP iterations of the same {n}^3 system "
+                          f"({sec:.1f} s, {per_it * 1e3:.1f} ms/iteration, single thread, "
+                          f"C restatement -O2), scaled to the GPU's {iters} iterations"),
+               "cpu": _cpu_model(), "nproc": os.cpu_count()}
+        log(f"[bench] cpu baseline {per_it * 1e3:.1f} ms/iter (setup {time.time() - t:.1f}s)")
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms_step, 2),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"{n}^3 log-normal (sigma=1) Neumann p' operator, uniform "
+                                "grid, reference cell pinned, N(0,1) RHS, fi0=0; "
+                                "PCG-SIP (alfa=0) to 1e-8 L1 relative residual"),
+                   "cells": ncell, "padded": list(dims), "seed": args.seed},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": KERNEL_BYTES[dom] * ncell,
+                     "avg_launch_ms": kern[dom]["ms"]},
+        "pct_hbm_iteration": round(100 * ITER_BYTES * ncell / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
+        "cell_updates_per_s": round(ncell * iters / (ms_step * 1e-3), 1),
+        "kernels": kern,
+        "solvers": solvers,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+if __name__ == "__main__":
+    main()

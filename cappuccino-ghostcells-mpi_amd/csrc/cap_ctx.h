@@ -1,0 +1,58 @@
+// cap_ctx.h -- the library context behind the opaque cap_ctx handle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace cap { struct DevState; }
+
+// solver work vectors in the tile-sheared layout
+enum {
+    T_FI, T_RES, T_ZK, T_PK, T_W, T_RESO, T_UK, T_VK, T_D,
+    T_AE, T_AW, T_AN, T_AS, T_AT, T_AB, T_AP, T_SU,
+    T_BB, T_BW, T_BS, T_BP, T_BN, T_BE, T_BT,
+    T_COUNT
+};
+
+// staging copies of caller arrays (reference layout) in host-pointer mode;
+// S_AE..S_SU follow the order ae, aw, an, as, at, ab, ap, su
+enum {
+    S_FI, S_AE, S_AW, S_AN, S_AS, S_AT, S_AB, S_AP, S_SU, S_RES,
+    S_BB, S_BS, S_BW, S_BP, S_BN, S_BE, S_BT,
+    S_COUNT
+};
+
+// FP32 vectors of the mixed-precision inner solver (tile-sheared layout)
+enum {
+    F_R, F_Z, F_P, F_W, F_X,
+    F_AE, F_AW, F_AN, F_AS, F_AT, F_AB, F_AP,
+    F_BB, F_BW, F_BS, F_BP, F_BN, F_BE, F_BT,
+    F_COUNT
+};
+
+struct cap_ctx {
+    int ni, nj, nk;
+    size_t nstd;
+    int flags;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    double *T[T_COUNT];
+    size_t tcap;
+    double *S[S_COUNT];
+    float *F[F_COUNT];
+    size_t fcap;
+    double *G[20];             // stencil staging (reference layout)
+    uint64_t *progress;
+    int ntiles_cap;
+    unsigned long long *claim;
+    unsigned long long claim_base;
+    uint64_t epoch;
+    cap::DevState *st;
+    cap::DevState *st2;            // inner solve of the mixed-precision solver
+    cap::DevState *st_host;
+    double *partials;
+    unsigned *ticket;
+    char err[256];
+};
+
+void cap_free_extra(cap_ctx *ctx);

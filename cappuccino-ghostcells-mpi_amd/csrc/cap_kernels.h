@@ -1,0 +1,173 @@
+// cap_kernels.h -- device-side building blocks shared by the solver kernels:
+// device scalar state, deterministic grid reductions and the finalisers that
+// turn reduced sums into the Krylov scalars of the reference solvers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "cap_wave.h"
+
+namespace cap {
+
+constexpr int BLOCK = 256;
+constexpr int MAX_RED_BLOCKS = 4096;
+
+// Scalars of one solve, resident on the device.  The reference keeps them as
+// locals of pcgsip / iccg / cgstab_sip / sipsol; the names follow it.
+struct DevState {
+    double res0, resor, resl, rsm;
+    double s0, sk, bet, pkapk, alf;          // pcg-sip.f90:34-35, iccg.f90:37-38
+    double beto, gam, om, ukreso, vres, vv;  // cgstab_sip.f90:36-37
+    double sor, small;
+    double red[4];                           // raw sums (tests / mixed solver)
+    int l;          // iterations completed
+    int nsw;
+    int done;       // 1: later kernels of this solve do nothing
+    int conv;       // 1: stopped on the relative-residual test
+    int stop;       // sipsol: stop after this sweep's update
+    int err;        // CAP_ERR_TIMEOUT from a wavefront wait, CAP_NAN
+    int pad[2];
+};
+
+enum Fin {
+    F_NONE = 0,
+    F_RES0,        // res0 = sum|r|                          pcg-sip.f90:56
+    F_PCG_SK,      // sk, bet = sk/s0                        pcg-sip.f90:118,127
+    F_PCG_PKAPK,   // pkapk, alf = sk/pkapk                  pcg-sip.f90:150,157
+    F_PCG_RESL,    // resl, s0 = sk, stop test               pcg-sip.f90:171-188
+    F_SIP_RES1,    // RES1, RESOR, RSM                       sipsol.f90:69,77-78
+    F_SIP_STEP,    // end of a sipsol sweep                  sipsol.f90:96
+    F_BIC_BET,     // bet, om, beto                          cgstab_sip.f90:111-116
+    F_BIC_GAM,     // ukreso, gam                            cgstab_sip.f90:175-179
+    F_BIC_ALF,     // vres, vv, alf                          cgstab_sip.f90:239-245
+    F_BIC_RESL,    // resl, stop test                        cgstab_sip.f90:256-266
+    F_RAW,         // keep the sums in red[]
+};
+
+__device__ __forceinline__ void fin_resl(DevState *st, double resl)
+{
+    st->resl = resl;
+    st->l += 1;
+    if (st->l == 1) st->resor = st->res0;
+    st->rsm = resl / (st->resor + st->small);
+    if (!(resl == resl) || resl > 1.0e300) { st->err = 2; st->done = 1; return; }
+    if (st->rsm < st->sor) { st->conv = 1; st->done = 1; }
+    else if (st->l >= st->nsw) st->done = 1;
+}
+
+template <int FIN>
+__device__ __forceinline__ void finalize(DevState *st, const double *v)
+{
+    if constexpr (FIN == F_RES0) {
+        st->res0 = v[0];
+    } else if constexpr (FIN == F_PCG_SK) {
+        st->sk = v[0];
+        st->bet = v[0] / st->s0;
+    } else if constexpr (FIN == F_PCG_PKAPK) {
+        st->pkapk = v[0];
+        st->alf = st->sk / v[0];
+    } else if constexpr (FIN == F_PCG_RESL) {
+        st->s0 = st->sk;
+        fin_resl(st, v[0]);
+    } else if constexpr (FIN == F_SIP_RES1) {
+        st->resl = v[0];
+        if (st->l == 0) st->resor = v[0];
+        st->rsm = v[0] / (st->resor + st->small);
+        st->stop = st->rsm < st->sor;
+        if (!(v[0] == v[0]) || v[0] > 1.0e300) st->err = 2;
+    } else if constexpr (FIN == F_SIP_STEP) {
+        st->l += 1;
+        if (st->err) st->done = 1;
+        else if (st->stop) { st->conv = 1; st->done = 1; }
+        else if (st->l >= st->nsw) st->done = 1;
+    } else if constexpr (FIN == F_BIC_BET) {
+        st->om = v[0] * st->gam / (st->alf * st->beto + st->small);
+        st->beto = v[0];
+    } else if constexpr (FIN == F_BIC_GAM) {
+        st->ukreso = v[0];
+        st->gam = st->beto / v[0];
+    } else if constexpr (FIN == F_BIC_ALF) {
+        st->vres = v[0];
+        st->vv = v[1];
+        st->alf = v[0] / (v[1] + st->small);
+    } else if constexpr (FIN == F_BIC_RESL) {
+        fin_resl(st, v[0]);
+    } else if constexpr (FIN == F_RAW) {
+        for (int q = 0; q < 4; ++q) st->red[q] = v[q];
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Deterministic two-level reduction of NV per-thread sums over the grid.
+// Partial sums go to partials[slot*NV + q] (slot fixed per block), the last
+// block to arrive adds the partials in slot order and runs the finaliser.
+template <int NV, int FIN>
+__device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double *partials,
+                            unsigned *ticket, DevState *st)
+{
+    __shared__ double sh[NV][BLOCK / 64];
+    __shared__ int last;
+    const int w = threadIdx.x >> 6;
+    for (int q = 0; q < NV; ++q) {
+        double x = wave_sum(v[q]);
+        if ((threadIdx.x & 63) == 0) sh[q][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NV; ++q)
+            partials[(size_t)slot * NV + q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc[NV];
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (int s = threadIdx.x; s < nslots; s += BLOCK)
+        for (int q = 0; q < NV; ++q)
+            acc[q] += __hip_atomic_load(&partials[(size_t)s * NV + q], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    for (int q = 0; q < NV; ++q) {
+        double x = wave_sum(acc[q]);
+        if ((threadIdx.x & 63) == 0) sh[q][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot[NV > 4 ? NV : 4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < NV; ++q) tot[q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+        finalize<FIN>(st, tot);
+        *ticket = 0;
+        __threadfence();
+    }
+}
+
+// element / cell decode for kernels that sweep the tiled vectors
+struct Cell {
+    size_t e;
+    int ti, tj, slot, b, a, kk;
+    bool in;   // interior cell (solve range)
+};
+
+__device__ __forceinline__ Cell cell_of(const Lay &L, size_t ts, int lane)
+{
+    Cell c;
+    const int t = (int)(ts / (size_t)L.S);
+    c.slot = (int)(ts % (size_t)L.S);
+    c.ti = t / L.nTJ;
+    c.tj = t % L.nTJ;
+    c.b = lane / TW;
+    c.a = lane % TW;
+    c.kk = posmod(c.slot - c.a - c.b, L.S);
+    c.e = ts * TCOL + lane;
+    const int ci = c.ti * TW + c.b, cj = c.tj * TW + c.a;
+    c.in = col_interior(L, ci, cj) && c.kk >= 1 && c.kk <= L.nks;
+    return c;
+}
+
+}  // namespace cap

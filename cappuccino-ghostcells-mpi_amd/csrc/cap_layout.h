@@ -1,0 +1,138 @@
+// cap_layout.h -- tile-sheared layout of the solver's work vectors.
+//
+// The reference keeps every field in the padded j-fastest layout
+//   ijk = (k-1)*ni*nj + (i-1)*nj + j         (setind.f90:34-40)
+// and its SIP / IC(0) sweeps are recursions over (i-1, j-1, k-1) (forward) or
+// (i+1, j+1, k+1) (backward), e.g. pcg-sip.f90:98-121.  On the GPU the sweeps
+// run as a hyperplane wavefront.  In a hyperplane the cells of one cache line
+// belong to different planes, so the solver keeps its work vectors (fi, res,
+// zk, pk, ...) and its copies of the coefficients in a layout built for the
+// wavefront:
+//
+//  * columns (i,j) of the solve box plus a one-column halo ring are grouped
+//    into tiles of 16 (j) x 16 (i) columns;
+//  * inside a tile, element (slot s, b, a) holds column (i0+b, j0+a) at the
+//    k-plane kk = (s - a - b) mod S, where S = nks + 2 k-planes (solve range
+//    plus one halo plane at each end);
+//  * element address = ((ti*nTJ + tj)*S + s)*256 + b*16 + a.
+//
+// A wavefront lane (a,b) handles its column; at step t it works on
+// kk = t - a - b, so every lane of the tile touches slot t mod S at step t:
+// 4 waves x 64 lanes read 256 contiguous doubles.  The recursion's j-1 value
+// comes from lane a-1 one step earlier, i-1 from lane b-1 one step earlier,
+// k-1 from the lane itself one step earlier -- the reference's dependency
+// order, so each value is computed from the same operands as the sequential
+// loop.
+//
+// Everything here is plain index arithmetic shared by the GPU kernels and the
+// host-side schedule check.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+
+#if defined(__HIPCC__)
+#define CAP_HD __host__ __device__ __forceinline__
+#else
+#define CAP_HD inline
+#endif
+
+namespace cap {
+
+constexpr int TW = 16;          // tile width in a (j) and b (i)
+constexpr int TCOL = TW * TW;   // columns per tile = lanes per workgroup
+
+struct Lay {
+    int ni, nj, nk, nij;      // padded extents of the reference arrays
+    int ia, ja, ka;           // first solve cell (1-based)
+    int nis, njs, nks;        // solve-box extents
+    int nci, ncj;             // column space incl. halo ring: nis+2, njs+2
+    int nTI, nTJ;             // tiles along i and j
+    int S;                    // slots per tile = nks + 2
+    int ntiles;
+    size_t n;                 // elements per tiled vector
+};
+
+CAP_HD Lay make_lay(int ni, int nj, int nk, int ia, int ib, int ja, int jb, int ka, int kb)
+{
+    Lay L;
+    L.ni = ni; L.nj = nj; L.nk = nk; L.nij = ni * nj;
+    L.ia = ia; L.ja = ja; L.ka = ka;
+    L.nis = ib - ia + 1; L.njs = jb - ja + 1; L.nks = kb - ka + 1;
+    L.nci = L.nis + 2; L.ncj = L.njs + 2;
+    L.nTI = (L.nci + TW - 1) / TW;
+    L.nTJ = (L.ncj + TW - 1) / TW;
+    L.S = L.nks + 2;
+    L.ntiles = L.nTI * L.nTJ;
+    L.n = (size_t)L.ntiles * (size_t)L.S * TCOL;
+    return L;
+}
+
+CAP_HD int posmod(int x, int m) { int r = x % m; return r < 0 ? r + m : r; }
+
+CAP_HD size_t tile_base(const Lay &L, int ti, int tj)
+{
+    return ((size_t)(ti * L.nTJ + tj) * (size_t)L.S) * TCOL;
+}
+
+CAP_HD size_t elem(const Lay &L, int ti, int tj, int slot, int b, int a)
+{
+    return tile_base(L, ti, tj) + (size_t)slot * TCOL + b * TW + a;
+}
+
+// element of column (ci, cj) at k-plane kk (all 0-based in the column space)
+CAP_HD size_t elem_of(const Lay &L, int ci, int cj, int kk)
+{
+    int ti = ci / TW, b = ci % TW, tj = cj / TW, a = cj % TW;
+    return elem(L, ti, tj, posmod(kk + a + b, L.S), b, a);
+}
+
+// offset into the reference array of column (ci, cj), plane kk
+CAP_HD size_t std_index(const Lay &L, int ci, int cj, int kk)
+{
+    int i = L.ia - 1 + ci, j = L.ja - 1 + cj, k = L.ka - 1 + kk;
+    return (size_t)(k - 1) * L.nij + (size_t)(i - 1) * L.nj + (size_t)(j - 1);
+}
+
+CAP_HD bool col_valid(const Lay &L, int ci, int cj) { return ci < L.nci && cj < L.ncj; }
+CAP_HD bool col_interior(const Lay &L, int ci, int cj)
+{
+    return ci >= 1 && ci <= L.nis && cj >= 1 && cj <= L.njs;
+}
+
+// Neighbour elements of interior cell (ti,tj,slot,b,a) at plane kk.  The
+// reference's names: east = i+1 (b+1), west = i-1, north = j+1 (a+1),
+// south = j-1, top = k+1, bottom = k-1.
+struct Nb { size_t e, w, n, s, t, bt; };
+
+CAP_HD Nb neighbours(const Lay &L, int ti, int tj, int slot, int b, int a, int kk)
+{
+    const int S = L.S;
+    const int sp = slot + 1 == S ? 0 : slot + 1;
+    const int sm = slot == 0 ? S - 1 : slot - 1;
+    Nb r;
+    r.t = elem(L, ti, tj, sp, b, a);
+    r.bt = elem(L, ti, tj, sm, b, a);
+    r.n = a < TW - 1 ? elem(L, ti, tj, sp, b, a + 1) : elem(L, ti, tj + 1, posmod(kk + b, S), b, 0);
+    r.s = a > 0 ? elem(L, ti, tj, sm, b, a - 1) : elem(L, ti, tj - 1, posmod(kk + TW - 1 + b, S), b, TW - 1);
+    r.e = b < TW - 1 ? elem(L, ti, tj, sp, b + 1, a) : elem(L, ti + 1, tj, posmod(kk + a, S), 0, a);
+    r.w = b > 0 ? elem(L, ti, tj, sm, b - 1, a) : elem(L, ti - 1, tj, posmod(kk + a + TW - 1, S), TW - 1, a);
+    return r;
+}
+
+// Decompose an element index into (ci, cj, kk); returns false for slots of
+// columns beyond the ring.
+CAP_HD bool decode(const Lay &L, size_t e, int &ci, int &cj, int &kk)
+{
+    size_t tslot = e / TCOL;
+    int lane = (int)(e % TCOL);
+    int b = lane / TW, a = lane % TW;
+    int slot = (int)(tslot % (size_t)L.S);
+    int t = (int)(tslot / (size_t)L.S);
+    int ti = t / L.nTJ, tj = t % L.nTJ;
+    ci = ti * TW + b;
+    cj = tj * TW + a;
+    kk = posmod(slot - a - b, L.S);
+    return col_valid(L, ci, cj);
+}
+
+}  // namespace cap

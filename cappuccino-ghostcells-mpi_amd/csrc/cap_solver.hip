@@ -1,0 +1,1176 @@
+// cap_solver.hip -- gfx950 kernels and host drivers of the p' linear solvers
+// (sipsol, pcgsip, iccg, cgstab_sip of the reference) behind the C ABI of
+// include/cap_amd.h.
+//
+// Data flow of one solve:
+//   reference arrays (padded, j-fastest)  --gather-->  tile-sheared vectors
+//   --> Krylov iterations entirely on the device (wavefront sweeps, SpMV,
+//       vector updates, deterministic FP64 reductions; the scalars live in a
+//       DevState in device memory, and every kernel is a no-op once the
+//       stop test has fired)  --scatter-->  reference arrays.
+// The host enqueues iterations in batches and reads the 100-byte DevState
+// back once per batch to learn whether the solve has stopped.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <algorithm>
+#include <vector>
+#include "cap_kernels.h"
+#include "cap_amd.h"
+#include "cap_ctx.h"
+
+using namespace cap;
+
+#define WAVE_G 8
+#define SPIN_LIMIT (1 << 22)
+
+// ============================================================ layout moves
+__global__ __launch_bounds__(BLOCK) void k_gather(Lay L, const double *__restrict__ src,
+                                                  double *__restrict__ dst)
+{
+    const size_t nts = (size_t)L.ntiles * L.S;
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        size_t e = ts * TCOL + threadIdx.x;
+        int ci, cj, kk;
+        double v = 0.0;
+        if (decode(L, e, ci, cj, kk)) v = src[std_index(L, ci, cj, kk)];
+        dst[e] = v;
+    }
+}
+
+// tiled -> reference layout, solve-range cells only
+__global__ __launch_bounds__(BLOCK) void k_scatter(Lay L, const double *__restrict__ src,
+                                                   double *__restrict__ dst)
+{
+    const size_t nts = (size_t)L.ntiles * L.S;
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        Cell c = cell_of(L, ts, threadIdx.x);
+        if (!c.in) continue;
+        dst[std_index(L, c.ti * TW + c.b, c.tj * TW + c.a, c.kk)] = src[c.e];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill(double *p, size_t n, double v)
+{
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK)
+        p[i] = v;
+}
+
+// ========================================================== vector kernels
+template <class V>
+struct Coef7T { const V *ae, *aw, *an, *as, *at, *ab, *ap; };
+using Coef7 = Coef7T<double>;
+
+// r = ae fi_e + aw fi_w + an fi_n + as fi_s + at fi_t + ab fi_b + su - ap fi
+// (pcg-sip.f90:53-55); sum|r| -> FIN
+template <int FIN>
+__global__ __launch_bounds__(BLOCK) void k_resid(Lay L, Coef7 A, const double *__restrict__ su,
+                                                 const double *__restrict__ fi,
+                                                 double *__restrict__ r, DevState *st,
+                                                 double *partials, unsigned *ticket)
+{
+    if (st->done) return;
+    const size_t nts = (size_t)L.ntiles * L.S;
+    double acc[1] = {0.0};
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        Cell c = cell_of(L, ts, threadIdx.x);
+        if (!c.in) continue;
+        Nb nb = neighbours(L, c.ti, c.tj, c.slot, c.b, c.a, c.kk);
+        const size_t e = c.e;
+        double v = A.ae[e] * fi[nb.e] + A.aw[e] * fi[nb.w] + A.an[e] * fi[nb.n] +
+                   A.as[e] * fi[nb.s] + A.at[e] * fi[nb.t] + A.ab[e] * fi[nb.bt] + su[e] -
+                   A.ap[e] * fi[e];
+        r[e] = v;
+        acc[0] += fabs(v);
+    }
+    grid_reduce<1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
+}
+
+// z = ap p - ae p_e - aw p_w - an p_n - as p_s - at p_t - ab p_b
+// (pcg-sip.f90:147-149).  MODE 0: sum p.z   MODE 1: sum z.y   MODE 2: sum z.y, sum z.z
+template <class V, int MODE, int FIN>
+__global__ __launch_bounds__(BLOCK) void k_spmv(Lay L, Coef7T<V> A, const V *__restrict__ p,
+                                                V *__restrict__ z, const V *__restrict__ y,
+                                                DevState *st, double *partials,
+                                                unsigned *ticket)
+{
+    if (st->done) return;
+    const size_t nts = (size_t)L.ntiles * L.S;
+    double acc[MODE == 2 ? 2 : 1] = {};
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        Cell c = cell_of(L, ts, threadIdx.x);
+        if (!c.in) continue;
+        Nb nb = neighbours(L, c.ti, c.tj, c.slot, c.b, c.a, c.kk);
+        const size_t e = c.e;
+        V v = A.ap[e] * p[e] - A.ae[e] * p[nb.e] - A.aw[e] * p[nb.w] - A.an[e] * p[nb.n] -
+              A.as[e] * p[nb.s] - A.at[e] * p[nb.t] - A.ab[e] * p[nb.bt];
+        z[e] = v;
+        if constexpr (MODE == 0) acc[0] += (double)p[e] * (double)v;
+        if constexpr (MODE == 1) acc[0] += (double)v * (double)y[e];
+        if constexpr (MODE == 2) { acc[0] += (double)v * (double)y[e]; acc[1] += (double)v * (double)v; }
+    }
+    grid_reduce<MODE == 2 ? 2 : 1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
+}
+
+// pk = zk + bet pk   (pcg-sip.f90:135)
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_pupdate(Lay L, const V *__restrict__ z,
+                                                   V *__restrict__ p, DevState *st)
+{
+    if (st->done) return;
+    const V bet = (V)st->bet;
+    const size_t nts = (size_t)L.ntiles * L.S;
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        Cell c = cell_of(L, ts, threadIdx.x);
+        if (!c.in) continue;
+        p[c.e] = z[c.e] + bet * p[c.e];
+    }
+}
+
+// pk = res + om (pk - alf uk)   (cgstab_sip.f90:124)
+__global__ __launch_bounds__(BLOCK) void k_bic_p(Lay L, const double *__restrict__ res,
+                                                 const double *__restrict__ uk,
+                                                 double *__restrict__ p, DevState *st)
+{
+    if (st->done) return;
+    const double om = st->om, alf = st->alf;
+    const size_t nts = (size_t)L.ntiles * L.S;
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        Cell c = cell_of(L, ts, threadIdx.x);
+        if (!c.in) continue;
+        p[c.e] = res[c.e] + om * (p[c.e] - alf * uk[c.e]);
+    }
+}
+
+// fi += a p; res -= a z; with a = alf (which 0) or gam (which 1);
+// L1 of res -> FIN when FIN != F_NONE   (pcg-sip.f90:167-171, cgstab_sip.f90:187-188, 254-256)
+template <class V, int WHICH, int FIN>
+__global__ __launch_bounds__(BLOCK) void k_update(Lay L, V *__restrict__ fi, V *__restrict__ res,
+                                                  const V *__restrict__ p,
+                                                  const V *__restrict__ z, DevState *st,
+                                                  double *partials, unsigned *ticket)
+{
+    if (st->done) return;
+    const V a = (V)(WHICH == 0 ? st->alf : st->gam);
+    const size_t nts = (size_t)L.ntiles * L.S;
+    double acc[1] = {0.0};
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        Cell c = cell_of(L, ts, threadIdx.x);
+        if (!c.in) continue;
+        fi[c.e] = fi[c.e] + a * p[c.e];
+        V r = res[c.e] - a * z[c.e];
+        res[c.e] = r;
+        acc[0] += fabs((double)r);
+    }
+    if constexpr (FIN != F_NONE)
+        grid_reduce<1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
+}
+
+// sum x.y -> FIN; optionally y := x first (COPY: reso = res, cgstab_sip.f90:87)
+template <int FIN, bool COPY>
+__global__ __launch_bounds__(BLOCK) void k_dot(Lay L, const double *__restrict__ x,
+                                               double *__restrict__ y, DevState *st,
+                                               double *partials, unsigned *ticket)
+{
+    if (st->done) return;
+    const size_t nts = (size_t)L.ntiles * L.S;
+    double acc[1] = {0.0};
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        Cell c = cell_of(L, ts, threadIdx.x);
+        if (!c.in) continue;
+        if constexpr (COPY) y[c.e] = x[c.e];
+        acc[0] += x[c.e] * y[c.e];
+    }
+    grid_reduce<1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
+}
+
+// ======================================================= wavefront sweeps
+template <class Op>
+struct WaveIO {
+    using V = typename Op::T;
+    const V *in[Op::NIN];
+    V *dep[Op::ND];
+    V *xo[Op::NX > 0 ? Op::NX : 1];
+};
+
+struct WaveSync {
+    uint64_t *progress;             // per tile: (epoch << 20) | steps completed
+    unsigned long long *claim;      // tile claim counter
+    unsigned long long claim_base;
+    uint64_t epoch;
+    DevState *st;
+    double *partials;
+    unsigned *ticket;
+};
+
+template <class V>
+__device__ __forceinline__ V coh_load(const V *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class V>
+__device__ __forceinline__ void coh_store(V *p, V v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool wait_ge(uint64_t *p, uint64_t target)
+{
+    for (int it = 0; it < SPIN_LIMIT; ++it) {
+        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+
+// One workgroup = one tile (256 lanes = 16x16 columns), marching through the
+// tile's steps in groups of WAVE_G.  Per group: wait until the (one or two)
+// upstream tiles have published enough steps, fetch their boundary values
+// (device-coherent loads), prefetch the next group's inputs, run the steps
+// (one LDS exchange + barrier per step), then publish progress.  The tile a
+// workgroup works on is claimed from a counter in a topological order, so a
+// workgroup only ever waits for tiles already held by running workgroups.
+template <class Op, bool FWD, int FIN>
+__global__ __launch_bounds__(BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io, WaveSync sy)
+{
+    using V = typename Op::T;
+    constexpr int ND = Op::ND, NIN = Op::NIN, G = WAVE_G;
+    constexpr int NXS = Op::NX > 0 ? Op::NX : 1, NRS = Op::NR > 0 ? Op::NR : 1;
+    __shared__ V lds[2][ND][TCOL];
+    __shared__ int sh_tile, sh_flag;
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        const unsigned long long c = atomicAdd(sy.claim, 1ull) - sy.claim_base;
+        sh_flag = sy.st->done;
+        if (c >= (unsigned long long)L.ntiles) {   // launch bookkeeping broken
+            sh_flag = 1;
+            sy.st->err = CAP_ERR_ARG;
+            sy.st->done = 1;
+        }
+        sh_tile = (int)c;
+    }
+    __syncthreads();
+    if (sh_flag) return;
+    const int claim = sh_tile;
+    int ti, tj;
+    claim_tile<FWD>(L, claim, ti, tj);
+    LaneInfo li;
+    lane_setup<FWD>(L, ti, tj, lane, li);
+    const size_t tb = tile_base(L, ti, tj);
+    const int NST = nsteps(L);
+    const int ngroups = (NST + G - 1) / G;
+    const bool has_pi = FWD ? ti > 0 : ti < L.nTI - 1;
+    const bool has_pj = FWD ? tj > 0 : tj < L.nTJ - 1;
+    const int pid_i = (FWD ? ti - 1 : ti + 1) * L.nTJ + tj;
+    const int pid_j = ti * L.nTJ + (FWD ? tj - 1 : tj + 1);
+    const int my_id = ti * L.nTJ + tj;
+    const uint64_t ep = sy.epoch << 20;
+    const bool bnd = is_boundary<FWD>(li);
+
+    V bufA[G][NIN], bufB[G][NIN];
+    V prev[ND];
+    double red[NRS];
+    for (int q = 0; q < ND; ++q) prev[q] = V(0);
+    for (int q = 0; q < NRS; ++q) red[q] = 0.0;
+
+    // Branch-free prefetch: every lane issues the same loads (all addresses
+    // lie inside the tile); a LOAD cell takes its first ND slots from the
+    // dependency arrays instead of the inputs.  Keeping the loads
+    // unconditional lets them stay in flight across the steps.
+    auto prefetch = [&](int g, V (&buf)[G][NIN]) {
+#pragma unroll
+        for (int d = 0; d < G; ++d) {
+            const int t = g * G + d;
+            int kk;
+            const int kind = step_kind<FWD>(L, li, t, kk);
+            const size_t e = tb + (size_t)step_slot<FWD>(L, t) * TCOL + lane;
+            const bool ld = kind != K_COMPUTE;
+#pragma unroll
+            for (int q = 0; q < NIN; ++q) {
+                const V *src = (q < ND && ld) ? io.dep[q < ND ? q : 0] : io.in[q];
+                buf[d][q] = src[e];
+            }
+        }
+    };
+
+    auto group = [&](int g, V (&cur)[G][NIN], V (&nxt)[G][NIN]) -> bool {
+        const int s0 = g * G;
+        if (lane == 0) {
+            const uint64_t target = ep | (uint64_t)need_steps(L, s0 + G - 1);
+            bool ok = true;
+            if (has_pi) ok = wait_ge(&sy.progress[pid_i], target);
+            if (ok && has_pj) ok = wait_ge(&sy.progress[pid_j], target);
+            if (!ok) {
+                sh_flag = 1;
+                sy.st->err = CAP_ERR_TIMEOUT;
+                sy.st->done = 1;
+            }
+        }
+        __syncthreads();
+        if (sh_flag) return false;
+        V xa[G][ND], xb[G][ND];
+#pragma unroll
+        for (int d = 0; d < G; ++d) {
+            int kk;
+            const int kind = step_kind<FWD>(L, li, s0 + d, kk);
+#pragma unroll
+            for (int q = 0; q < ND; ++q) { xa[d][q] = V(0); xb[d][q] = V(0); }
+            if (kind == K_COMPUTE && li.pa == 0) {
+                const size_t x = xa_elem<FWD>(L, ti, tj, li, kk);
+#pragma unroll
+                for (int q = 0; q < ND; ++q) xa[d][q] = coh_load(io.dep[q] + x);
+            }
+            if (kind == K_COMPUTE && li.pb == 0) {
+                const size_t x = xb_elem<FWD>(L, ti, tj, li, kk);
+#pragma unroll
+                for (int q = 0; q < ND; ++q) xb[d][q] = coh_load(io.dep[q] + x);
+            }
+        }
+        if (g + 1 < ngroups) prefetch(g + 1, nxt);
+#pragma unroll
+        for (int d = 0; d < G; ++d) {
+            const int t = s0 + d;
+            if (t >= NST) break;
+            int kk;
+            const int kind = step_kind<FWD>(L, li, t, kk);
+            const size_t e = tb + (size_t)step_slot<FWD>(L, t) * TCOL + lane;
+            const int rp = (t + 1) & 1, wp = t & 1;
+            V dk[ND], di[ND], dj[ND], out[ND];
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                dk[q] = prev[q];
+                const V la = lds[rp][q][li.nb_a];
+                const V lb = lds[rp][q][li.nb_b];
+                dj[q] = li.pa == 0 ? xa[d][q] : la;
+                di[q] = li.pb == 0 ? xb[d][q] : lb;
+            }
+            if (kind == K_COMPUTE) {
+                V xv[NXS];
+                double r[NRS];
+                op.compute(cur[d], dk, di, dj, out, xv, r);
+#pragma unroll
+                for (int q = 0; q < ND; ++q) {
+                    if (bnd) coh_store(io.dep[q] + e, out[q]);
+                    else io.dep[q][e] = out[q];
+                }
+#pragma unroll
+                for (int q = 0; q < Op::NX; ++q) io.xo[q][e] = xv[q];
+#pragma unroll
+                for (int q = 0; q < Op::NR; ++q) red[q] += r[q];
+            } else if (kind == K_LOAD) {
+#pragma unroll
+                for (int q = 0; q < ND; ++q) out[q] = cur[d][q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < ND; ++q) out[q] = V(0);
+            }
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                prev[q] = out[q];
+                lds[wp][q][lane] = out[q];
+            }
+            __syncthreads();
+        }
+        // every wave's boundary stores must be complete before the flag
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        __syncthreads();
+        if (lane == 0) {
+            const int done_steps = s0 + G < NST ? s0 + G : NST;
+            __hip_atomic_store(&sy.progress[my_id], ep | (uint64_t)done_steps, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return true;
+    };
+
+    prefetch(0, bufA);
+    for (int g = 0; g < ngroups; g += 2) {
+        if (!group(g, bufA, bufB)) return;
+        if (g + 1 < ngroups)
+            if (!group(g + 1, bufB, bufA)) return;
+    }
+    if constexpr (FIN != F_NONE) {
+        double v[NRS];
+        for (int q = 0; q < NRS; ++q) v[q] = Op::NR ? red[q] : 0.0;
+        grid_reduce<NRS, FIN>(v, my_id, L.ntiles, sy.partials, sy.ticket, sy.st);
+    }
+}
+
+// ============================================================ host helpers
+static int grid_for(const Lay &L)
+{
+    size_t nts = (size_t)L.ntiles * L.S;
+    return (int)std::min<size_t>(nts, 2048);
+}
+
+static int fail(cap_ctx *c, int code, const char *what)
+{
+    snprintf(c->err, sizeof c->err, "%s", what);
+    return code;
+}
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (call);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            snprintf(ctx->err, sizeof ctx->err, "%s failed: %s (%s:%d)", #call,           \
+                     hipGetErrorString(_e), __FILE__, __LINE__);                          \
+            return CAP_ERR_HIP;                                                           \
+        }                                                                                 \
+    } while (0)
+
+static int ensure_tiled(cap_ctx *ctx, size_t n)
+{
+    if (ctx->tcap >= n) return CAP_OK;
+    for (int v = 0; v < T_COUNT; ++v)
+        if (ctx->T[v]) { (void)hipFree(ctx->T[v]); ctx->T[v] = nullptr; }
+    for (int v = 0; v < T_COUNT; ++v) HIPCHK(hipMalloc(&ctx->T[v], n * sizeof(double)));
+    ctx->tcap = n;
+    return CAP_OK;
+}
+
+static int ensure_std(cap_ctx *ctx)
+{
+    if (ctx->S[0]) return CAP_OK;
+    for (int v = 0; v < S_COUNT; ++v) HIPCHK(hipMalloc(&ctx->S[v], ctx->nstd * sizeof(double)));
+    return CAP_OK;
+}
+
+// device pointer for a caller array (uploading it in host-pointer mode)
+static int dev_in(cap_ctx *ctx, int slot, const double *p, double **out)
+{
+    if (!p) { *out = nullptr; return CAP_OK; }
+    if (ctx->flags & CAP_DEVICE_POINTERS) { *out = (double *)p; return CAP_OK; }
+    int rc = ensure_std(ctx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->S[slot], p, ctx->nstd * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    *out = ctx->S[slot];
+    return CAP_OK;
+}
+
+static int dev_out(cap_ctx *ctx, double *host, const double *dev)
+{
+    if (!host || (ctx->flags & CAP_DEVICE_POINTERS)) return CAP_OK;
+    HIPCHK(hipMemcpyAsync(host, dev, ctx->nstd * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    return CAP_OK;
+}
+
+template <class Op, bool FWD, int FIN>
+static void launch_wave(cap_ctx *ctx, const Lay &L, const Op &op, const WaveIO<Op> &io,
+                        DevState *st = nullptr)
+{
+    WaveSync sy;
+    sy.progress = ctx->progress;
+    sy.claim = ctx->claim;
+    sy.claim_base = ctx->claim_base;
+    sy.epoch = ++ctx->epoch;
+    sy.st = st ? st : ctx->st;
+    sy.partials = ctx->partials;
+    sy.ticket = ctx->ticket;
+    ctx->claim_base += (unsigned long long)L.ntiles;
+    hipLaunchKernelGGL((k_wave<Op, FWD, FIN>), dim3(L.ntiles), dim3(BLOCK), 0, ctx->stream, L, op,
+                       io, sy);
+}
+
+static Coef7 coef7(cap_ctx *ctx)
+{
+    Coef7 A;
+    A.ae = ctx->T[T_AE]; A.aw = ctx->T[T_AW]; A.an = ctx->T[T_AN]; A.as = ctx->T[T_AS];
+    A.at = ctx->T[T_AT]; A.ab = ctx->T[T_AB]; A.ap = ctx->T[T_AP];
+    return A;
+}
+
+// Everything a solve needs in device memory, in both layouts.
+struct SolveIn {
+    double *fi, *ae, *aw, *an, *as, *at, *ab, *ap, *su, *res;
+    double *bb, *bs, *bw, *bp, *bn, *be, *bt;
+};
+
+static int prepare(cap_ctx *ctx, const Lay &L, double *fi, const cap_coef *c, SolveIn &d,
+                   bool need_res_halo, bool need_b_halo)
+{
+    int rc;
+    if (!fi || !c || !c->ae || !c->aw || !c->an || !c->as || !c->at || !c->ab || !c->ap || !c->su)
+        return fail(ctx, CAP_ERR_ARG, "null array argument");
+    if ((rc = ensure_tiled(ctx, L.n))) return rc;
+    if (!ctx->progress || ctx->ntiles_cap < L.ntiles) {
+        if (ctx->progress) (void)hipFree(ctx->progress);
+        HIPCHK(hipMalloc(&ctx->progress, (size_t)L.ntiles * sizeof(uint64_t)));
+        HIPCHK(hipMemsetAsync(ctx->progress, 0, (size_t)L.ntiles * sizeof(uint64_t), ctx->stream));
+        ctx->ntiles_cap = L.ntiles;
+    }
+    if ((rc = dev_in(ctx, S_FI, fi, &d.fi))) return rc;
+    const double *src[9] = {c->ae, c->aw, c->an, c->as, c->at, c->ab, c->ap, c->su, nullptr};
+    double **dst[9] = {&d.ae, &d.aw, &d.an, &d.as, &d.at, &d.ab, &d.ap, &d.su, nullptr};
+    for (int v = 0; v < 8; ++v)
+        if ((rc = dev_in(ctx, S_AE + v, src[v], dst[v]))) return rc;
+    d.res = d.bb = d.bs = d.bw = d.bp = d.bn = d.be = d.bt = nullptr;
+    if (c->res && (rc = dev_in(ctx, S_RES, c->res, &d.res))) return rc;
+    if (c->bn) {
+        if ((rc = dev_in(ctx, S_BN, c->bn, &d.bn))) return rc;
+        if ((rc = dev_in(ctx, S_BE, c->be, &d.be))) return rc;
+        if ((rc = dev_in(ctx, S_BT, c->bt, &d.bt))) return rc;
+    }
+    d.bb = c->bb ? ((ctx->flags & CAP_DEVICE_POINTERS) ? c->bb : ctx->S[S_BB]) : nullptr;
+    d.bs = c->bs ? ((ctx->flags & CAP_DEVICE_POINTERS) ? c->bs : ctx->S[S_BS]) : nullptr;
+    d.bw = c->bw ? ((ctx->flags & CAP_DEVICE_POINTERS) ? c->bw : ctx->S[S_BW]) : nullptr;
+    d.bp = c->bp ? ((ctx->flags & CAP_DEVICE_POINTERS) ? c->bp : ctx->S[S_BP]) : nullptr;
+    if (!(ctx->flags & CAP_DEVICE_POINTERS) && (c->bb || c->bs || c->bw || c->bp)) {
+        if ((rc = ensure_std(ctx))) return rc;
+        // the reference leaves bb..bp untouched outside the range: keep them
+        if (c->bb) HIPCHK(hipMemcpyAsync(ctx->S[S_BB], c->bb, ctx->nstd * 8, hipMemcpyHostToDevice, ctx->stream));
+        if (c->bs) HIPCHK(hipMemcpyAsync(ctx->S[S_BS], c->bs, ctx->nstd * 8, hipMemcpyHostToDevice, ctx->stream));
+        if (c->bw) HIPCHK(hipMemcpyAsync(ctx->S[S_BW], c->bw, ctx->nstd * 8, hipMemcpyHostToDevice, ctx->stream));
+        if (c->bp) HIPCHK(hipMemcpyAsync(ctx->S[S_BP], c->bp, ctx->nstd * 8, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    double *const *T = ctx->T;
+    HIPCHK(hipEventRecord(ctx->ev0, s));
+    HIPCHK(hipMemsetAsync(ctx->ticket, 0, 64, s));
+    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.fi, T[T_FI]);
+    const double *cs[8] = {d.ae, d.aw, d.an, d.as, d.at, d.ab, d.ap, d.su};
+    const int ct[8] = {T_AE, T_AW, T_AN, T_AS, T_AT, T_AB, T_AP, T_SU};
+    for (int v = 0; v < 8; ++v)
+        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, cs[v], T[ct[v]]);
+    // halo values of res and bn/be/bt are data for the recursions
+    if (need_res_halo && d.res)
+        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.res, T[T_RES]);
+    else
+        hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_RES], L.n, 0.0);
+    if (need_b_halo && d.bn) {
+        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.bn, T[T_BN]);
+        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.be, T[T_BE]);
+        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.bt, T[T_BT]);
+    } else {
+        hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_BN], L.n, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_BE], L.n, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_BT], L.n, 0.0);
+    }
+    HIPCHK(hipGetLastError());
+    return CAP_OK;
+}
+
+static int init_state(cap_ctx *ctx, double sor, int nsw)
+{
+    DevState h;
+    memset(&h, 0, sizeof h);
+    h.sor = sor;
+    h.nsw = nsw;
+    h.small = (double)1e-30f;
+    h.s0 = (double)1e20f;
+    h.alf = 1.0; h.beto = 1.0; h.gam = 1.0;
+    if (nsw <= 0) h.done = 1;
+    memcpy(ctx->st_host, &h, sizeof h);
+    HIPCHK(hipMemcpyAsync(ctx->st, ctx->st_host, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+    return CAP_OK;
+}
+
+static int read_state(cap_ctx *ctx, DevState &h)
+{
+    HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    memcpy(&h, ctx->st_host, sizeof h);
+    return CAP_OK;
+}
+
+// SIP factorisation into T_BB..T_BT for the solve range (pcg-sip.f90:68-87)
+static void sip_factor(cap_ctx *ctx, const Lay &L, double alfa)
+{
+    double *const *T = ctx->T;
+    OpSipFactor op;
+    op.alfa = alfa;
+    op.small = (double)1e-30f;
+    WaveIO<OpSipFactor> io;
+    const int in[7] = {T_AB, T_AW, T_AS, T_AP, T_AN, T_AE, T_AT};
+    for (int q = 0; q < 7; ++q) io.in[q] = T[in[q]];
+    io.dep[0] = T[T_BN]; io.dep[1] = T[T_BE]; io.dep[2] = T[T_BT];
+    io.xo[0] = T[T_BB]; io.xo[1] = T[T_BW]; io.xo[2] = T[T_BS]; io.xo[3] = T[T_BP];
+    launch_wave<OpSipFactor, true, F_NONE>(ctx, L, op, io);
+}
+
+static void sip_fwd(cap_ctx *ctx, const Lay &L, const double *rhs, double *z)
+{
+    double *const *T = ctx->T;
+    WaveIO<OpSipFwd> io;
+    io.in[0] = rhs; io.in[1] = T[T_BB]; io.in[2] = T[T_BW]; io.in[3] = T[T_BS]; io.in[4] = T[T_BP];
+    io.dep[0] = z;
+    io.xo[0] = nullptr;
+    launch_wave<OpSipFwd, true, F_NONE>(ctx, L, OpSipFwd(), io);
+}
+
+static void sip_bwd(cap_ctx *ctx, const Lay &L, double *z)
+{
+    double *const *T = ctx->T;
+    WaveIO<OpSipBwd> io;
+    io.in[0] = z; io.in[1] = T[T_BN]; io.in[2] = T[T_BE]; io.in[3] = T[T_BT];
+    io.dep[0] = z;
+    io.xo[0] = nullptr;
+    launch_wave<OpSipBwd, false, F_NONE>(ctx, L, OpSipBwd(), io);
+}
+
+// write back what the reference leaves in its module arrays
+static int writeback(cap_ctx *ctx, const Lay &L, double *fi_host, const cap_coef *c,
+                     SolveIn &d, bool res_zero_first, bool factors)
+{
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    double *const *T = ctx->T;
+    double *dsts[7] = {d.bb, d.bs, d.bw, d.bp, d.bn, d.be, d.bt};
+    double *hs[7] = {c->bb, c->bs, c->bw, c->bp, c->bn, c->be, c->bt};
+    const int ts[7] = {T_BB, T_BS, T_BW, T_BP, T_BN, T_BE, T_BT};
+    hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_FI], d.fi);
+    if (d.res) {
+        if (res_zero_first) {
+            int gs = (int)std::min<size_t>((ctx->nstd + BLOCK - 1) / BLOCK, 4096);
+            hipLaunchKernelGGL(k_fill, dim3(gs), dim3(BLOCK), 0, s, d.res, ctx->nstd, 0.0);
+        }
+        hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_RES], d.res);
+    }
+    if (factors)
+        for (int v = 0; v < 7; ++v)
+            if (dsts[v])
+                hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[ts[v]], dsts[v]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev1, s));
+    int rc = dev_out(ctx, fi_host, d.fi);
+    if (rc) return rc;
+    if (d.res && (rc = dev_out(ctx, c->res, d.res))) return rc;
+    if (factors)
+        for (int v = 0; v < 7; ++v)
+            if (dsts[v] && (rc = dev_out(ctx, hs[v], dsts[v]))) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    return CAP_OK;
+}
+
+static int finish_report(cap_ctx *ctx, const DevState &h, cap_report *rep, float ms)
+{
+    if (h.err == CAP_ERR_TIMEOUT) return fail(ctx, CAP_ERR_TIMEOUT, "wavefront dependency wait expired");
+    if (h.err == CAP_ERR_ARG) return fail(ctx, CAP_ERR_ARG, "wavefront tile claim out of range");
+    if (rep) {
+        rep->resor = h.l >= 1 ? h.resor : h.res0;
+        rep->resl = h.resl;
+        rep->iters = h.conv ? h.l : h.nsw + 1;
+        rep->status = h.err == 2 ? CAP_NAN : (h.conv ? CAP_OK : CAP_MAXITER);
+        rep->ms = ms;
+    }
+    return h.err == 2 ? CAP_NAN : (h.conv ? CAP_OK : CAP_MAXITER);
+}
+
+// Run `one_iter` in batches until the device says stop.
+template <class F>
+static int iterate(cap_ctx *ctx, int nsw, F one_iter, DevState &h)
+{
+    int rc;
+    int batch = 4;
+    for (int l = 0; l < nsw;) {
+        int nb = std::min(batch, nsw - l);
+        for (int b = 0; b < nb; ++b) one_iter();
+        HIPCHK(hipGetLastError());
+        l += nb;
+        if ((rc = read_state(ctx, h))) return rc;
+        if (h.done || h.err) break;
+        batch = std::min(batch * 2, 32);
+    }
+    return CAP_OK;
+}
+
+static Lay lay_for(cap_ctx *ctx, int klo, int khi)
+{
+    return make_lay(ctx->ni, ctx->nj, ctx->nk, 3, ctx->ni - 2, 3, ctx->nj - 2, klo, khi);
+}
+
+// ======================================================== the four solvers
+extern "C" int cap_pcgsip(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa, double sor,
+                          int nsw, cap_report *rep)
+{
+    if (!ctx) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 3, ctx->nk - 2);
+    SolveIn d;
+    int rc = prepare(ctx, L, fi, c, d, false, true);
+    if (rc) return rc;
+    if ((rc = init_state(ctx, sor, nsw))) return rc;
+    double *const *T = ctx->T;
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_ZK], L.n, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_PK], L.n, 0.0);
+    hipLaunchKernelGGL((k_resid<F_RES0>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                       (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
+                       ctx->partials, ctx->ticket);
+    sip_factor(ctx, L, alfa);
+    auto one = [&]() {
+        sip_fwd(ctx, L, T[T_RES], T[T_ZK]);
+        WaveIO<OpSipBwdDot> io;
+        io.in[0] = T[T_ZK]; io.in[1] = T[T_BN]; io.in[2] = T[T_BE]; io.in[3] = T[T_BT];
+        io.in[4] = T[T_RES];
+        io.dep[0] = T[T_ZK];
+        io.xo[0] = nullptr;
+        launch_wave<OpSipBwdDot, false, F_PCG_SK>(ctx, L, OpSipBwdDot(), io);
+        hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
+                           T[T_PK], ctx->st);
+        hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
+                           ctx->partials, ctx->ticket);
+        hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                           T[T_RES], (const double *)T[T_PK], (const double *)T[T_W], ctx->st,
+                           ctx->partials, ctx->ticket);
+    };
+    DevState h;
+    if ((rc = iterate(ctx, nsw, one, h))) return rc;
+    if ((rc = writeback(ctx, L, fi, c, d, true, true))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    return finish_report(ctx, h, rep, ms);
+}
+
+extern "C" int cap_sipsol(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa, double sor,
+                          int nsw, cap_report *rep)
+{
+    if (!ctx) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 3, ctx->nk - 2);
+    SolveIn d;
+    int rc = prepare(ctx, L, fi, c, d, true, true);
+    if (rc) return rc;
+    if ((rc = init_state(ctx, sor, nsw))) return rc;
+    double *const *T = ctx->T;
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    sip_factor(ctx, L, alfa);
+    auto one = [&]() {
+        // residual of the current fi (sipsol.f90:66-69) into T_W, then the
+        // forward sweep over res (70-71), backward sweep and update (86-88)
+        hipLaunchKernelGGL((k_resid<F_SIP_RES1>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_SU], (const double *)T[T_FI], T[T_W], ctx->st,
+                           ctx->partials, ctx->ticket);
+        sip_fwd(ctx, L, T[T_W], T[T_RES]);
+        WaveIO<OpSipBwdUpdate> io;
+        io.in[0] = T[T_RES]; io.in[1] = T[T_BN]; io.in[2] = T[T_BE]; io.in[3] = T[T_BT];
+        io.in[4] = T[T_FI];
+        io.dep[0] = T[T_RES];
+        io.xo[0] = T[T_FI];
+        launch_wave<OpSipBwdUpdate, false, F_SIP_STEP>(ctx, L, OpSipBwdUpdate(), io);
+    };
+    DevState h;
+    if ((rc = iterate(ctx, nsw, one, h))) return rc;
+    if ((rc = writeback(ctx, L, fi, c, d, false, true))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    return finish_report(ctx, h, rep, ms);
+}
+
+extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor, int nsw,
+                        cap_report *rep)
+{
+    if (!ctx) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 2, ctx->nk - 1);
+    SolveIn d;
+    int rc = prepare(ctx, L, fi, c, d, false, false);
+    if (rc) return rc;
+    if ((rc = init_state(ctx, sor, nsw))) return rc;
+    double *const *T = ctx->T;
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_ZK], L.n, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_PK], L.n, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_D], L.n, 0.0);
+    hipLaunchKernelGGL((k_resid<F_RES0>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                       (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
+                       ctx->partials, ctx->ticket);
+    {   // d = 1/(ap - aw^2 d_w - as^2 d_s - ab^2 d_b)  (iccg.f90:73-81)
+        WaveIO<OpIcDiag> io;
+        io.in[0] = T[T_AP]; io.in[1] = T[T_AW]; io.in[2] = T[T_AS]; io.in[3] = T[T_AB];
+        io.dep[0] = T[T_D];
+        io.xo[0] = nullptr;
+        launch_wave<OpIcDiag, true, F_NONE>(ctx, L, OpIcDiag(), io);
+    }
+    auto one = [&]() {
+        WaveIO<OpIcFwd> f;
+        f.in[0] = T[T_RES]; f.in[1] = T[T_AW]; f.in[2] = T[T_AS]; f.in[3] = T[T_AB];
+        f.in[4] = T[T_D];
+        f.dep[0] = T[T_ZK];
+        f.xo[0] = nullptr;
+        launch_wave<OpIcFwd, true, F_NONE>(ctx, L, OpIcFwd(), f);
+        OpIcBwdDot ob;
+        ob.small = (double)1e-30f;
+        WaveIO<OpIcBwdDot> b;
+        b.in[0] = T[T_ZK]; b.in[1] = T[T_AE]; b.in[2] = T[T_AN]; b.in[3] = T[T_AT];
+        b.in[4] = T[T_D]; b.in[5] = T[T_RES];
+        b.dep[0] = T[T_ZK];
+        b.xo[0] = nullptr;
+        launch_wave<OpIcBwdDot, false, F_PCG_SK>(ctx, L, ob, b);
+        hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
+                           T[T_PK], ctx->st);
+        hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
+                           ctx->partials, ctx->ticket);
+        hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                           T[T_RES], (const double *)T[T_PK], (const double *)T[T_W], ctx->st,
+                           ctx->partials, ctx->ticket);
+    };
+    DevState h;
+    if ((rc = iterate(ctx, nsw, one, h))) return rc;
+    if ((rc = writeback(ctx, L, fi, c, d, true, false))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    return finish_report(ctx, h, rep, ms);
+}
+
+extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
+                              double sor, int nsw, cap_report *rep)
+{
+    if (!ctx) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 2, ctx->nk - 1);
+    SolveIn d;
+    int rc = prepare(ctx, L, fi, c, d, false, true);
+    if (rc) return rc;
+    if ((rc = init_state(ctx, sor, nsw))) return rc;
+    double *const *T = ctx->T;
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    // the solver's own work arrays are zero on and around the range
+    const int zero[5] = {T_ZK, T_PK, T_UK, T_VK, T_RESO};
+    for (int v = 0; v < 5; ++v)
+        hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[zero[v]], L.n, 0.0);
+    hipLaunchKernelGGL((k_resid<F_RES0>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                       (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
+                       ctx->partials, ctx->ticket);
+    sip_factor(ctx, L, alfa);
+    auto one = [&]() {
+        // bet = res.reso; om; beto (cgstab_sip.f90:106-116).  On the first
+        // iteration reso := res (83-94) is folded into the same pass.
+        hipLaunchKernelGGL((k_dot<F_BIC_BET, false>), dim3(gb), dim3(BLOCK), 0, s, L,
+                           (const double *)T[T_RES], T[T_RESO], ctx->st, ctx->partials, ctx->ticket);
+        hipLaunchKernelGGL(k_bic_p, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_RES],
+                           (const double *)T[T_UK], T[T_PK], ctx->st);
+        sip_fwd(ctx, L, T[T_PK], T[T_ZK]);
+        sip_bwd(ctx, L, T[T_ZK]);
+        hipLaunchKernelGGL((k_spmv<double, 1, F_BIC_GAM>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_ZK], T[T_UK], (const double *)T[T_RESO], ctx->st,
+                           ctx->partials, ctx->ticket);
+        hipLaunchKernelGGL((k_update<double, 1, F_NONE>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                           T[T_RES], (const double *)T[T_ZK], (const double *)T[T_UK], ctx->st,
+                           ctx->partials, ctx->ticket);
+        sip_fwd(ctx, L, T[T_RES], T[T_ZK]);
+        sip_bwd(ctx, L, T[T_ZK]);
+        hipLaunchKernelGGL((k_spmv<double, 2, F_BIC_ALF>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_ZK], T[T_VK], (const double *)T[T_RES], ctx->st,
+                           ctx->partials, ctx->ticket);
+        hipLaunchKernelGGL((k_update<double, 0, F_BIC_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                           T[T_RES], (const double *)T[T_ZK], (const double *)T[T_VK], ctx->st,
+                           ctx->partials, ctx->ticket);
+    };
+    // reso = res before the first iteration (cgstab_sip.f90:87)
+    hipLaunchKernelGGL((k_dot<F_RAW, true>), dim3(gb), dim3(BLOCK), 0, s, L,
+                       (const double *)T[T_RES], T[T_RESO], ctx->st, ctx->partials, ctx->ticket);
+    DevState h;
+    if ((rc = iterate(ctx, nsw, one, h))) return rc;
+    if ((rc = writeback(ctx, L, fi, c, d, false, true))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    return finish_report(ctx, h, rep, ms);
+}
+
+
+// ============================================ mixed precision (FP32 inner)
+__global__ __launch_bounds__(BLOCK) void k_to_f32(const double *__restrict__ a,
+                                                  float *__restrict__ b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK)
+        b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fillf(float *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK)
+        p[i] = 0.0f;
+}
+
+// x += e on the solve range (FP64 accumulation of the FP32 correction)
+__global__ __launch_bounds__(BLOCK) void k_add_corr(Lay L, double *__restrict__ x,
+                                                    const float *__restrict__ e)
+{
+    const size_t nts = (size_t)L.ntiles * L.S;
+    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
+        Cell c = cell_of(L, ts, threadIdx.x);
+        if (!c.in) continue;
+        x[c.e] = x[c.e] + (double)e[c.e];
+    }
+}
+
+static int ensure_f32(cap_ctx *ctx, size_t n)
+{
+    if (ctx->fcap >= n) return CAP_OK;
+    for (int v = 0; v < F_COUNT; ++v)
+        if (ctx->F[v]) { (void)hipFree(ctx->F[v]); ctx->F[v] = nullptr; }
+    for (int v = 0; v < F_COUNT; ++v) HIPCHK(hipMalloc(&ctx->F[v], n * sizeof(float)));
+    ctx->fcap = n;
+    return CAP_OK;
+}
+
+// FP64 iterative refinement: r = b - A x in FP64 on the tile layout, then
+// an FP32 PCG-SIP solve A e = r (FP32 coefficients, FP32 SIP factors taken
+// from the FP64 factorisation, FP32 SpMV, FP64 dot products and norms),
+// x += e in FP64, until sum|r| / sum|r0| < sor with r the FP64 residual.
+extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
+                                double sor, int nsw, double inner_rtol, cap_report *rep)
+{
+    if (!ctx || !(inner_rtol > 0.0) || inner_rtol >= 1.0) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 3, ctx->nk - 2);
+    SolveIn d;
+    int rc = prepare(ctx, L, fi, c, d, false, true);
+    if (rc) return rc;
+    if ((rc = ensure_f32(ctx, L.n))) return rc;
+    if ((rc = init_state(ctx, sor, nsw))) return rc;
+    double *const *T = ctx->T;
+    float *const *F = ctx->F;
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    hipLaunchKernelGGL((k_resid<F_RES0>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                       (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
+                       ctx->partials, ctx->ticket);
+    sip_factor(ctx, L, alfa);
+    const int c64[14] = {T_AE, T_AW, T_AN, T_AS, T_AT, T_AB, T_AP,
+                         T_BB, T_BW, T_BS, T_BP, T_BN, T_BE, T_BT};
+    const int c32[14] = {F_AE, F_AW, F_AN, F_AS, F_AT, F_AB, F_AP,
+                         F_BB, F_BW, F_BS, F_BP, F_BN, F_BE, F_BT};
+    const int gl = (int)std::min<size_t>((L.n + BLOCK - 1) / BLOCK, 8192);
+    for (int v = 0; v < 14; ++v)
+        hipLaunchKernelGGL(k_to_f32, dim3(gl), dim3(BLOCK), 0, s, (const double *)T[c64[v]], F[c32[v]], L.n);
+    Coef7T<float> A32;
+    A32.ae = F[F_AE]; A32.aw = F[F_AW]; A32.an = F[F_AN]; A32.as = F[F_AS];
+    A32.at = F[F_AT]; A32.ab = F[F_AB]; A32.ap = F[F_AP];
+    DevState h, hi;
+    if ((rc = read_state(ctx, h))) return rc;
+    const double res0 = h.res0;
+    double resl = res0;
+    int inner_total = 0, conv = 0, nan = 0;
+    DevState *st2 = ctx->st2;
+    while (true) {
+        if (!(resl == resl)) { nan = 1; break; }
+        if (resl / (res0 + (double)1e-30f) < sor) { conv = 1; break; }
+        if (inner_total >= nsw) break;
+        // inner solve from e = 0 with rhs r (FP32)
+        hipLaunchKernelGGL(k_to_f32, dim3(gl), dim3(BLOCK), 0, s, (const double *)T[T_RES], F[F_R], L.n);
+        hipLaunchKernelGGL(k_fillf, dim3(gl), dim3(BLOCK), 0, s, F[F_X], L.n);
+        hipLaunchKernelGGL(k_fillf, dim3(gl), dim3(BLOCK), 0, s, F[F_Z], L.n);
+        hipLaunchKernelGGL(k_fillf, dim3(gl), dim3(BLOCK), 0, s, F[F_P], L.n);
+        memset(&hi, 0, sizeof hi);
+        hi.sor = inner_rtol;
+        hi.nsw = nsw - inner_total;
+        hi.small = (double)1e-30f;
+        hi.s0 = (double)1e20f;
+        hi.res0 = resl;
+        memcpy(ctx->st_host, &hi, sizeof hi);
+        HIPCHK(hipMemcpyAsync(st2, ctx->st_host, sizeof hi, hipMemcpyHostToDevice, s));
+        auto one = [&]() {
+            WaveIO<OpSipFwdT<float>> f;
+            f.in[0] = F[F_R]; f.in[1] = F[F_BB]; f.in[2] = F[F_BW]; f.in[3] = F[F_BS];
+            f.in[4] = F[F_BP];
+            f.dep[0] = F[F_Z];
+            f.xo[0] = nullptr;
+            launch_wave<OpSipFwdT<float>, true, F_NONE>(ctx, L, OpSipFwdT<float>(), f, st2);
+            WaveIO<OpSipBwdDotT<float>> b;
+            b.in[0] = F[F_Z]; b.in[1] = F[F_BN]; b.in[2] = F[F_BE]; b.in[3] = F[F_BT];
+            b.in[4] = F[F_R];
+            b.dep[0] = F[F_Z];
+            b.xo[0] = nullptr;
+            launch_wave<OpSipBwdDotT<float>, false, F_PCG_SK>(ctx, L, OpSipBwdDotT<float>(), b, st2);
+            hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
+                               (const float *)F[F_Z], F[F_P], st2);
+            hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L,
+                               A32, (const float *)F[F_P], F[F_W], (const float *)nullptr, st2,
+                               ctx->partials, ctx->ticket);
+            hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
+                               F[F_X], F[F_R], (const float *)F[F_P], (const float *)F[F_W], st2,
+                               ctx->partials, ctx->ticket);
+        };
+        int batch = 4;
+        for (int l = 0; l < hi.nsw;) {
+            int nb = std::min(batch, hi.nsw - l);
+            for (int q = 0; q < nb; ++q) one();
+            HIPCHK(hipGetLastError());
+            l += nb;
+            HIPCHK(hipMemcpyAsync(ctx->st_host, st2, sizeof hi, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            memcpy(&hi, ctx->st_host, sizeof hi);
+            if (hi.done || hi.err) break;
+            batch = std::min(batch * 2, 32);
+        }
+        if (hi.err == CAP_ERR_TIMEOUT) return fail(ctx, CAP_ERR_TIMEOUT, "wavefront dependency wait expired");
+        inner_total += hi.l;
+        hipLaunchKernelGGL(k_add_corr, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI], (const float *)F[F_X]);
+        hipLaunchKernelGGL((k_resid<F_RAW>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
+                           ctx->partials, ctx->ticket);
+        if ((rc = read_state(ctx, h))) return rc;
+        resl = h.red[0];
+        if (hi.l == 0) break;
+    }
+    if ((rc = writeback(ctx, L, fi, c, d, true, true))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const int st_code = nan ? CAP_NAN : (conv ? CAP_OK : CAP_MAXITER);
+    if (rep) {
+        rep->resor = res0;
+        rep->resl = resl;
+        rep->iters = inner_total;
+        rep->status = st_code;
+        rep->ms = ms;
+    }
+    return st_code;
+}
+
+
+// ================================================================ timing
+// Measurement hook (no counterpart in the reference).  Re-launches each
+// kernel of one PCG-SIP iteration `reps` times on the context's stream, on
+// the tile vectors left by the last pcgsip (fp32 == 0) or pcgsip_mixed
+// (fp32 != 0) solve, bracketed by HIP events; ms[q] receives the average
+// duration of: 0 SIP forward sweep, 1 SIP backward sweep + dot, 2 pk update,
+// 3 SpMV + dot, 4 fi/res update + L1.
+extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
+{
+    if (!ctx || !ms || reps < 1 || ctx->tcap == 0 || (fp32 && ctx->fcap == 0)) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 3, ctx->nk - 2);
+    int rc;
+    if ((rc = init_state(ctx, 0.0, 1 << 30))) return rc;
+    HIPCHK(hipMemsetAsync(ctx->ticket, 0, 64, ctx->stream));
+    double *const *T = ctx->T;
+    float *const *F = ctx->F;
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    Coef7T<float> A32;
+    A32.ae = F[F_AE]; A32.aw = F[F_AW]; A32.an = F[F_AN]; A32.as = F[F_AS];
+    A32.at = F[F_AT]; A32.ab = F[F_AB]; A32.ap = F[F_AP];
+    for (int q = 0; q < 5; ++q) {
+        HIPCHK(hipEventRecord(ctx->ev0, s));
+        for (int r = 0; r < reps; ++r) {
+            if (!fp32) {
+                if (q == 0) sip_fwd(ctx, L, T[T_RES], T[T_ZK]);
+                if (q == 1) {
+                    WaveIO<OpSipBwdDot> io;
+                    io.in[0] = T[T_ZK]; io.in[1] = T[T_BN]; io.in[2] = T[T_BE]; io.in[3] = T[T_BT];
+                    io.in[4] = T[T_RES];
+                    io.dep[0] = T[T_ZK];
+                    io.xo[0] = nullptr;
+                    launch_wave<OpSipBwdDot, false, F_PCG_SK>(ctx, L, OpSipBwdDot(), io);
+                }
+                if (q == 2)
+                    hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L,
+                                       (const double *)T[T_ZK], T[T_PK], ctx->st);
+                if (q == 3)
+                    hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
+                                       L, A, (const double *)T[T_PK], T[T_W],
+                                       (const double *)nullptr, ctx->st, ctx->partials, ctx->ticket);
+                if (q == 4)
+                    hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0,
+                                       s, L, T[T_FI], T[T_RES], (const double *)T[T_PK],
+                                       (const double *)T[T_W], ctx->st, ctx->partials, ctx->ticket);
+            } else {
+                if (q == 0) {
+                    WaveIO<OpSipFwdT<float>> f;
+                    f.in[0] = F[F_R]; f.in[1] = F[F_BB]; f.in[2] = F[F_BW]; f.in[3] = F[F_BS];
+                    f.in[4] = F[F_BP];
+                    f.dep[0] = F[F_Z];
+                    f.xo[0] = nullptr;
+                    launch_wave<OpSipFwdT<float>, true, F_NONE>(ctx, L, OpSipFwdT<float>(), f);
+                }
+                if (q == 1) {
+                    WaveIO<OpSipBwdDotT<float>> b;
+                    b.in[0] = F[F_Z]; b.in[1] = F[F_BN]; b.in[2] = F[F_BE]; b.in[3] = F[F_BT];
+                    b.in[4] = F[F_R];
+                    b.dep[0] = F[F_Z];
+                    b.xo[0] = nullptr;
+                    launch_wave<OpSipBwdDotT<float>, false, F_PCG_SK>(ctx, L, OpSipBwdDotT<float>(), b);
+                }
+                if (q == 2)
+                    hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
+                                       (const float *)F[F_Z], F[F_P], ctx->st);
+                if (q == 3)
+                    hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
+                                       L, A32, (const float *)F[F_P], F[F_W],
+                                       (const float *)nullptr, ctx->st, ctx->partials, ctx->ticket);
+                if (q == 4)
+                    hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0,
+                                       s, L, F[F_X], F[F_R], (const float *)F[F_P],
+                                       (const float *)F[F_W], ctx->st, ctx->partials, ctx->ticket);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ctx->ev1, s));
+        HIPCHK(hipEventSynchronize(ctx->ev1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+        ms[q] = (double)t / reps;
+    }
+    DevState h;
+    if ((rc = read_state(ctx, h))) return rc;
+    if (h.done || h.err) return fail(ctx, CAP_ERR_ARG, "kernel timing hit a stop condition");
+    return CAP_OK;
+}
+
+// ================================================================ context
+extern "C" int cap_init(cap_ctx **out, int ni, int nj, int nk, int flags)
+{
+    if (!out || ni < 5 || nj < 5 || nk < 5) return CAP_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CAP_ERR_NODEV;
+    cap_ctx *ctx = (cap_ctx *)calloc(1, sizeof(cap_ctx));
+    if (!ctx) return CAP_ERR_NOMEM;
+    ctx->ni = ni; ctx->nj = nj; ctx->nk = nk;
+    ctx->nstd = (size_t)ni * nj * nk;
+    ctx->flags = flags;
+    HIPCHK(hipStreamCreate(&ctx->stream));
+    HIPCHK(hipEventCreate(&ctx->ev0));
+    HIPCHK(hipEventCreate(&ctx->ev1));
+    HIPCHK(hipMalloc(&ctx->st, sizeof(DevState)));
+    HIPCHK(hipMalloc(&ctx->st2, sizeof(DevState)));
+    HIPCHK(hipHostMalloc(&ctx->st_host, sizeof(DevState), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&ctx->partials, (size_t)MAX_RED_BLOCKS * 64 * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->ticket, 64));
+    HIPCHK(hipMemset(ctx->ticket, 0, 64));
+    HIPCHK(hipMalloc(&ctx->claim, 64));
+    HIPCHK(hipMemset(ctx->claim, 0, 64));
+    ctx->claim_base = 0;
+    ctx->epoch = 0;
+    *out = ctx;
+    return CAP_OK;
+}
+
+extern "C" int cap_finalize(cap_ctx *ctx)
+{
+    if (!ctx) return CAP_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int v = 0; v < T_COUNT; ++v) if (ctx->T[v]) (void)hipFree(ctx->T[v]);
+    for (int v = 0; v < S_COUNT; ++v) if (ctx->S[v]) (void)hipFree(ctx->S[v]);
+    cap_free_extra(ctx);
+    if (ctx->progress) (void)hipFree(ctx->progress);
+    (void)hipFree(ctx->st);
+    (void)hipFree(ctx->st2);
+    (void)hipHostFree(ctx->st_host);
+    (void)hipFree(ctx->partials);
+    (void)hipFree(ctx->ticket);
+    (void)hipFree(ctx->claim);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    free(ctx);
+    return CAP_OK;
+}
+
+extern "C" const char *cap_last_error(const cap_ctx *ctx)
+{
+    return ctx ? ctx->err : "no context";
+}

@@ -1,0 +1,243 @@
+// cap_wave.h -- hyperplane-wavefront schedule for the SIP / IC(0) recursions
+// over the tile-sheared layout (cap_layout.h), plus the per-cell recursions.
+//
+// One workgroup of 256 lanes owns one tile (16x16 columns, all k).  Lane
+// (a, b) has chain coordinates (pa, pb) = (a, b) for a forward recursion and
+// (15-a, 15-b) for a backward one.  At step t it handles chain index
+// rel = t - pa - pb, i.e. k-plane kk = rel (forward) or S-1-rel (backward).
+// Its three predecessors in the recursion are
+//   k-side: the lane itself at step t-1,
+//   j-side: lane (pa-1, pb) at step t-1   (tile tj-/+1 when pa == 0),
+//   i-side: lane (pa, pb-1) at step t-1   (tile ti-/+1 when pb == 0),
+// which is the (k-1, i-1, j-1) dependency of e.g. pcg-sip.f90:98-106 and the
+// (k+1, i+1, j+1) dependency of pcg-sip.f90:112-121.  A value produced by a
+// neighbouring tile at its step t+15 is consumed here at step t.
+//
+// Cells are classified per step: COMPUTE (interior column, 1<=kk<=nks),
+// LOAD (halo ring column or halo plane: the value is data, read from the
+// dependency array), IDLE (outside the ring).
+//
+// The recursion formulas (Op*::compute) keep the reference's operand order so
+// that, with contraction disabled, every value is bit-identical to the
+// sequential loop.
+#pragma once
+#include "cap_layout.h"
+
+namespace cap {
+
+enum { K_IDLE = 0, K_LOAD = 1, K_COMPUTE = 2 };
+
+struct LaneInfo {
+    int a, b, pa, pb;       // tile-local column and chain coordinates
+    int ci, cj;             // column in the column space
+    bool valid, interior;
+    int nb_a, nb_b;         // lanes providing the j-side / i-side value
+};
+
+template <bool FWD>
+CAP_HD void lane_setup(const Lay &L, int ti, int tj, int lane, LaneInfo &li)
+{
+    li.a = lane % TW;
+    li.b = lane / TW;
+    li.pa = FWD ? li.a : TW - 1 - li.a;
+    li.pb = FWD ? li.b : TW - 1 - li.b;
+    li.ci = ti * TW + li.b;
+    li.cj = tj * TW + li.a;
+    li.valid = col_valid(L, li.ci, li.cj);
+    li.interior = col_interior(L, li.ci, li.cj);
+    // lane index of (pa-1, pb) and (pa, pb-1); lanes at the chain start read
+    // a neighbouring tile instead and point at themselves here
+    li.nb_a = li.pa == 0 ? lane : (FWD ? lane - 1 : lane + 1);
+    li.nb_b = li.pb == 0 ? lane : (FWD ? lane - TW : lane + TW);
+}
+
+CAP_HD int nsteps(const Lay &L) { return L.S + 2 * (TW - 1); }
+
+// slot every lane of the tile touches at step t
+template <bool FWD>
+CAP_HD int step_slot(const Lay &L, int t)
+{
+    return FWD ? posmod(t, L.S) : posmod(L.S - 1 + 2 * (TW - 1) - t, L.S);
+}
+
+template <bool FWD>
+CAP_HD int step_kind(const Lay &L, const LaneInfo &li, int t, int &kk)
+{
+    int rel = t - li.pa - li.pb;
+    if (!li.valid || rel < 0 || rel >= L.S) { kk = -1; return K_IDLE; }
+    kk = FWD ? rel : L.S - 1 - rel;
+    if (li.interior && kk >= 1 && kk <= L.nks) return K_COMPUTE;
+    return K_LOAD;
+}
+
+// element of the j-side predecessor in the neighbouring tile (pa == 0 only)
+template <bool FWD>
+CAP_HD size_t xa_elem(const Lay &L, int ti, int tj, const LaneInfo &li, int kk)
+{
+    int ntj = FWD ? tj - 1 : tj + 1;
+    int na = FWD ? TW - 1 : 0;
+    return elem(L, ti, ntj, posmod(kk + na + li.b, L.S), li.b, na);
+}
+
+// element of the i-side predecessor in the neighbouring tile (pb == 0 only)
+template <bool FWD>
+CAP_HD size_t xb_elem(const Lay &L, int ti, int tj, const LaneInfo &li, int kk)
+{
+    int nti = FWD ? ti - 1 : ti + 1;
+    int nb = FWD ? TW - 1 : 0;
+    return elem(L, nti, tj, posmod(kk + li.a + nb, L.S), nb, li.a);
+}
+
+// a tile's value consumed by a neighbour at the neighbour's step t was
+// produced at step t + TW - 1, so step t needs this many completed steps
+CAP_HD int need_steps(const Lay &L, int t_last)
+{
+    int n = t_last + TW;
+    int m = nsteps(L);
+    return n < m ? n : m;
+}
+
+// lanes whose values neighbouring tiles read during the same sweep
+template <bool FWD>
+CAP_HD bool is_boundary(const LaneInfo &li)
+{
+    return li.pa == TW - 1 || li.pb == TW - 1;
+}
+
+// tile processed by the c-th claim (a topological order of the tile DAG)
+template <bool FWD>
+CAP_HD void claim_tile(const Lay &L, int c, int &ti, int &tj)
+{
+    int t = FWD ? c : L.ntiles - 1 - c;
+    ti = t / L.nTJ;
+    tj = t % L.nTJ;
+}
+
+// ---------------------------------------------------------------- recursions
+// Each op: NIN per-cell inputs x[], ND chained values (dk/di/dj are the k-,
+// i-, j-side predecessors in the sweep's direction), NX extra outputs, NR
+// reduction terms.
+
+// SIP forward substitution, pcg-sip.f90:102-103 / cgstab_sip.f90:135-136:
+//   z = (rhs - bb*z(k-1) - bw*z(i-1) - bs*z(j-1)) * bp
+// x = {rhs, bb, bw, bs, bp}
+template <class V>
+struct OpSipFwdT {
+    using T = V;
+    static constexpr int NIN = 5, ND = 1, NX = 0, NR = 0;
+    CAP_HD static void compute(const V *x, const V *dk, const V *di, const V *dj, V *d, V *,
+                               double *)
+    {
+        d[0] = (x[0] - x[1] * dk[0] - x[2] * di[0] - x[3] * dj[0]) * x[4];
+    }
+};
+using OpSipFwd = OpSipFwdT<double>;
+
+// SIP backward substitution + scalar product, pcg-sip.f90:116-118:
+//   z = z - bn*z(j+1) - be*z(i+1) - bt*z(k+1);  sk += res*z
+// x = {z, bn, be, bt, res}
+// (the FP32 form accumulates the product in FP64)
+template <class V>
+struct OpSipBwdDotT {
+    using T = V;
+    static constexpr int NIN = 5, ND = 1, NX = 0, NR = 1;
+    CAP_HD static void compute(const V *x, const V *dk, const V *di, const V *dj, V *d, V *,
+                               double *r)
+    {
+        d[0] = x[0] - x[1] * dj[0] - x[2] * di[0] - x[3] * dk[0];
+        r[0] = (double)x[4] * (double)d[0];
+    }
+};
+using OpSipBwdDot = OpSipBwdDotT<double>;
+
+// SIP backward substitution without a product, cgstab_sip.f90:148-149
+// x = {z, bn, be, bt}
+struct OpSipBwd {
+    using T = double;
+    static constexpr int NIN = 4, ND = 1, NX = 0, NR = 0;
+    CAP_HD static void compute(const double *x, const double *dk, const double *di,
+                               const double *dj, double *d, double *, double *)
+    {
+        d[0] = x[0] - x[1] * dj[0] - x[2] * di[0] - x[3] * dk[0];
+    }
+};
+
+// sipsol backward sweep with the update, sipsol.f90:86-88:
+//   res = res - bn*res(j+1) - be*res(i+1) - bt*res(k+1);  fi = fi + res
+// x = {res, bn, be, bt, fi}; extra output: fi
+struct OpSipBwdUpdate {
+    using T = double;
+    static constexpr int NIN = 5, ND = 1, NX = 1, NR = 0;
+    CAP_HD static void compute(const double *x, const double *dk, const double *di,
+                               const double *dj, double *d, double *xo, double *)
+    {
+        d[0] = x[0] - x[1] * dj[0] - x[2] * di[0] - x[3] * dk[0];
+        xo[0] = x[4] + d[0];
+    }
+};
+
+// SIP (Stone) factorisation, pcg-sip.f90:72-84 (== sipsol.f90:34-46):
+// x = {ab, aw, as, ap, an, ae, at}; chained {bn, be, bt}; extra {bb, bw, bs, bp}
+struct OpSipFactor {
+    using T = double;
+    static constexpr int NIN = 7, ND = 3, NX = 4, NR = 0;
+    double alfa, small;
+    CAP_HD void compute(const double *x, const double *dk, const double *di,
+                        const double *dj, double *d, double *xo, double *) const
+    {
+        // dk = {bn,be,bt}(k-1), di = (i-1), dj = (j-1)
+        double bb = -x[0] / (1. + alfa * (dk[0] + dk[1]));
+        double bw = -x[1] / (1. + alfa * (di[0] + di[2]));
+        double bs = -x[2] / (1. + alfa * (dj[1] + dj[2]));
+        double p1 = alfa * (bb * dk[0] + bw * di[0]);
+        double p2 = alfa * (bb * dk[1] + bs * dj[1]);
+        double p3 = alfa * (bw * di[2] + bs * dj[2]);
+        double bp = 1. / (x[3] + p1 + p2 + p3 - bb * dk[2] - bw * di[1] - bs * dj[0] + small);
+        d[0] = (-x[4] - p1) * bp;
+        d[1] = (-x[5] - p2) * bp;
+        d[2] = (-x[6] - p3) * bp;
+        xo[0] = bb; xo[1] = bw; xo[2] = bs; xo[3] = bp;
+    }
+};
+
+// IC(0) diagonal, iccg.f90:77-78:
+//   d = 1/(ap - aw^2 d(i-1) - as^2 d(j-1) - ab^2 d(k-1));  x = {ap, aw, as, ab}
+struct OpIcDiag {
+    using T = double;
+    static constexpr int NIN = 4, ND = 1, NX = 0, NR = 0;
+    CAP_HD static void compute(const double *x, const double *dk, const double *di,
+                               const double *dj, double *d, double *, double *)
+    {
+        d[0] = 1. / (x[0] - x[1] * x[1] * di[0] - x[2] * x[2] * dj[0] - x[3] * x[3] * dk[0]);
+    }
+};
+
+// IC(0) forward, iccg.f90:96-97:
+//   z = (res + aw*z(i-1) + as*z(j-1) + ab*z(k-1)) * d;  x = {res, aw, as, ab, d}
+struct OpIcFwd {
+    using T = double;
+    static constexpr int NIN = 5, ND = 1, NX = 0, NR = 0;
+    CAP_HD static void compute(const double *x, const double *dk, const double *di,
+                               const double *dj, double *d, double *, double *)
+    {
+        d[0] = (x[0] + x[1] * di[0] + x[2] * dj[0] + x[3] * dk[0]) * x[4];
+    }
+};
+
+// IC(0) scaling + backward + product, iccg.f90:106 then 118-120:
+//   z = z/(d+small);  z = (z + ae*z(i+1) + an*z(j+1) + at*z(k+1)) * d;
+//   sk += res*z.   x = {z, ae, an, at, d, res}
+struct OpIcBwdDot {
+    using T = double;
+    static constexpr int NIN = 6, ND = 1, NX = 0, NR = 1;
+    double small;
+    CAP_HD void compute(const double *x, const double *dk, const double *di,
+                        const double *dj, double *d, double *, double *r) const
+    {
+        double z = x[0] / (x[4] + small);
+        d[0] = (z + x[1] * di[0] + x[2] * dj[0] + x[3] * dk[0]) * x[4];
+        r[0] = x[5] * d[0];
+    }
+};
+
+}  // namespace cap

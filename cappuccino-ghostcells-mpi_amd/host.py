@@ -1,0 +1,186 @@
+"""Host-side binding of the MI355X p' library (``libcapamd.so``).
+
+Mirrors the reference's solver interface: the reference calls
+``call pcgsip(fi, ifi)`` with the matrix in module arrays ``ae..ap, su, res``
+(module ``coef``) and ``bb..bt`` (module ``coefb``) -- see
+modules_allocatable.f90:161-180 and pcg-sip.f90:3.  Here the module arrays
+are an explicit :class:`Coef` object and the field id is replaced by the
+explicit ``alfa / sor / nsw`` the reference reads from module ``parameters``.
+
+Arrays are float64 in the reference's padded j-fastest layout (length
+``ni*nj*nk``): numpy arrays for a host-pointer context, or torch tensors on
+``cuda`` for a device-pointer context.  The library must be built
+(``__graft_entry__.build()``); a missing library raises at import time --
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcapamd.so")
+
+CAP_OK, CAP_MAXITER, CAP_NAN, CAP_GUESS_REJECTED = 0, 1, 2, 3
+CAP_ERR_ARG, CAP_ERR_HIP, CAP_ERR_TIMEOUT, CAP_ERR_NOMEM, CAP_ERR_NODEV = -1, -2, -3, -4, -5
+CAP_DEVICE_POINTERS = 1
+
+COEF_FIELDS = ("ae", "aw", "an", "as_", "at", "ab", "ap", "su", "res",
+               "bb", "bs", "bw", "bp", "bn", "be", "bt")
+GEOM_FIELDS = ("fx", "fy", "fz", "ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z",
+               "ar3x", "ar3y", "ar3z", "vol")
+
+# every entry point declared in include/cap_amd.h
+EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgsip",
+           "cap_iccg", "cap_cgstab_sip", "cap_pcgsip_mixed", "cap_grad_gauss", "cap_bpres",
+           "cap_build_lognormal", "cap_time_kernels")
+
+
+class CapError(RuntimeError):
+    pass
+
+
+class _Coef(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in COEF_FIELDS]
+
+
+class _Report(C.Structure):
+    _fields_ = [("resor", C.c_double), ("resl", C.c_double), ("iters", C.c_int),
+                ("status", C.c_int), ("ms", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load libcapamd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CapError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    L.cap_init.argtypes = [C.POINTER(vp), i, i, i, i]
+    L.cap_finalize.argtypes = [vp]
+    L.cap_last_error.argtypes = [vp]
+    L.cap_last_error.restype = C.c_char_p
+    for fn in ("cap_sipsol", "cap_pcgsip", "cap_cgstab_sip"):
+        getattr(L, fn).argtypes = [vp, vp, C.POINTER(_Coef), d, d, i, C.POINTER(_Report)]
+    L.cap_iccg.argtypes = [vp, vp, C.POINTER(_Coef), d, i, C.POINTER(_Report)]
+    L.cap_pcgsip_mixed.argtypes = [vp, vp, C.POINTER(_Coef), d, d, i, d, C.POINTER(_Report)]
+    L.cap_grad_gauss.argtypes = [vp, vp, C.POINTER(vp), vp, vp, vp]
+    L.cap_bpres.argtypes = [vp, vp, vp, vp, vp]
+    L.cap_build_lognormal.argtypes = [vp, C.c_ulonglong, d, C.POINTER(_Coef)]
+    L.cap_time_kernels.argtypes = [vp, i, i, C.POINTER(d)]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):            # torch tensor
+        import torch
+        assert a.dtype == torch.float64 and a.is_contiguous()
+        return a.data_ptr()
+    import numpy as np
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data
+
+
+class Coef:
+    """The module arrays of `coef` and `coefb` that the solvers read/write."""
+
+    def __init__(self, **arrays):
+        for f in COEF_FIELDS:
+            setattr(self, f, arrays.get(f))
+
+    def _struct(self):
+        return _Coef(*[_ptr(getattr(self, f)) for f in COEF_FIELDS])
+
+
+class Context:
+    """One solver context (one HIP stream) for arrays of extents ni x nj x nk."""
+
+    def __init__(self, ni, nj, nk, device_pointers=False):
+        self._lib = load()
+        self.dims = (int(ni), int(nj), int(nk))
+        self.device_pointers = bool(device_pointers)
+        h = C.c_void_p()
+        rc = self._lib.cap_init(C.byref(h), *self.dims,
+                                CAP_DEVICE_POINTERS if device_pointers else 0)
+        if rc != CAP_OK:
+            raise CapError(f"cap_init failed with status {rc}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.cap_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def error(self):
+        return self._lib.cap_last_error(self._h).decode()
+
+    def _check(self, rc, what, allow=(CAP_OK, CAP_MAXITER)):
+        if rc not in allow:
+            raise CapError(f"{what} failed with status {rc}: {self.error()}")
+        return rc
+
+    def _solve(self, fn, fi, coef, *args):
+        rep = _Report()
+        st = coef._struct()
+        rc = fn(self._h, _ptr(fi), C.byref(st), *args, C.byref(rep))
+        self._check(rc, fn.__name__, allow=(CAP_OK, CAP_MAXITER, CAP_NAN))
+        return dict(resor=rep.resor, resl=rep.resl, iters=rep.iters, status=rep.status,
+                    converged=rep.status == CAP_OK, ms=rep.ms)
+
+    # --- drop-ins for the reference solvers -------------------------------
+    def sipsol(self, fi, coef, alfa=0.92, sor=1e-8, nsw=1000):
+        return self._solve(self._lib.cap_sipsol, fi, coef, alfa, sor, nsw)
+
+    def pcgsip(self, fi, coef, alfa=0.92, sor=1e-8, nsw=1000):
+        return self._solve(self._lib.cap_pcgsip, fi, coef, alfa, sor, nsw)
+
+    def iccg(self, fi, coef, sor=1e-8, nsw=1000):
+        return self._solve(self._lib.cap_iccg, fi, coef, sor, nsw)
+
+    def cgstab_sip(self, fi, coef, alfa=0.92, sor=1e-8, nsw=1000):
+        return self._solve(self._lib.cap_cgstab_sip, fi, coef, alfa, sor, nsw)
+
+    def pcgsip_mixed(self, fi, coef, alfa=0.92, sor=1e-8, nsw=5000, inner_rtol=1e-3):
+        return self._solve(self._lib.cap_pcgsip_mixed, fi, coef, alfa, sor, nsw, inner_rtol)
+
+    # --- stencils -----------------------------------------------------------
+    def grad_gauss(self, u, geom, dudx, dudy, dudz):
+        arr = (C.c_void_p * 13)(*[_ptr(geom[f]) for f in GEOM_FIELDS])
+        rc = self._lib.cap_grad_gauss(self._h, _ptr(u), arr, _ptr(dudx), _ptr(dudy), _ptr(dudz))
+        self._check(rc, "cap_grad_gauss", allow=(CAP_OK,))
+
+    def bpres(self, p, fx, fy, fz):
+        rc = self._lib.cap_bpres(self._h, _ptr(p), _ptr(fx), _ptr(fy), _ptr(fz))
+        self._check(rc, "cap_bpres", allow=(CAP_OK,))
+
+    def time_kernels(self, fp32=False, reps=20):
+        """Average ms of the 5 kernels of one PCG-SIP iteration (see cap_amd.h)."""
+        ms = (C.c_double * 5)()
+        rc = self._lib.cap_time_kernels(self._h, 1 if fp32 else 0, reps, ms)
+        self._check(rc, "cap_time_kernels", allow=(CAP_OK,))
+        return list(ms)
+
+    def build_lognormal(self, coef, seed, sigma=1.0):
+        st = coef._struct()
+        rc = self._lib.cap_build_lognormal(self._h, seed, sigma, C.byref(st))
+        self._check(rc, "cap_build_lognormal", allow=(CAP_OK,))

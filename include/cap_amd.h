@@ -1,0 +1,126 @@
+/*
+ * cap_amd.h -- C ABI of the MI355X pressure-correction (p') library.
+ *
+ * Drop-in boundary for the reference's linear solvers and p' stencils.  The
+ * reference calls Fortran external procedures with implicit interfaces
+ * (`call pcgsip(fi, ifi)` etc.) that read and write module arrays.  Here the
+ * module arrays are passed explicitly as plain pointers in the reference's
+ * padded j-fastest layout (ijk = (k-1)*ni*nj + (i-1)*nj + j, 1-based;
+ * C offset = ijk-1; ni,nj,nk = interior cells + 4).  The Fortran side binds
+ * these with ISO_C_BINDING (cappuccino-ghostcells-mpi_amd/fortran/cap_hip.f90).
+ *
+ * Pointers are host pointers unless the context was created with
+ * CAP_DEVICE_POINTERS, in which case they are device (HBM) pointers and no
+ * host<->device copy is made.  Every entry point returns a status code; no
+ * exception crosses the ABI.  One context drives one HIP stream; calls are
+ * synchronous on return.
+ */
+#ifndef CAP_AMD_H
+#define CAP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CAP_OK              0
+#define CAP_MAXITER         1   /* nsw iterations done without reaching sor */
+#define CAP_NAN             2   /* a residual norm became NaN or Inf       */
+#define CAP_GUESS_REJECTED  3   /* learned guess raised the residual, unused */
+#define CAP_ERR_ARG        -1
+#define CAP_ERR_HIP        -2
+#define CAP_ERR_TIMEOUT    -3   /* a wavefront dependency wait expired      */
+#define CAP_ERR_NOMEM      -4
+#define CAP_ERR_NODEV      -5
+
+/* context flags */
+#define CAP_DEVICE_POINTERS 1
+
+typedef struct cap_ctx cap_ctx;
+
+/*
+ * Module `coef` and `coefb` arrays (modules_allocatable.f90:161-180).
+ * ae..ap and su are read.  res and bb..bt are the reference's module work
+ * arrays: may be NULL (their halo values are then taken as zero and nothing
+ * is written back); when given, their values outside the solve range enter
+ * the recursions exactly as in the reference (e.g. sipsol.f90:70-71 reads
+ * res on the boundary planes, pcg-sip.f90:72-81 reads bn/be/bt there) and on
+ * return they hold what the reference leaves in them.
+ */
+typedef struct {
+    double *ae, *aw, *an, *as, *at, *ab, *ap, *su, *res;
+    double *bb, *bs, *bw, *bp, *bn, *be, *bt;
+} cap_coef;
+
+typedef struct {
+    double resor;   /* initial L1 residual, the reference's resor(ifi)      */
+    double resl;    /* final L1 residual the reference reports              */
+    int iters;      /* the reference's reported iteration number: the index
+                       of the converging iteration, or nsw+1 when it ran out */
+    int status;     /* CAP_OK / CAP_MAXITER / CAP_NAN                        */
+    double ms;      /* device time of the solve, milliseconds               */
+} cap_report;
+
+/* Context for arrays of padded extents ni x nj x nk. */
+int cap_init(cap_ctx **ctx, int ni, int nj, int nk, int flags);
+int cap_finalize(cap_ctx *ctx);
+const char *cap_last_error(const cap_ctx *ctx);
+
+/* Replaces `call sipsol(fi, ifi)` (sipsol.f90:3-106): Stone's SIP,
+ * k,i,j = 3..n?mm, stop when RES1/RESOR < sor. */
+int cap_sipsol(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
+               double sor, int nsw, cap_report *rep);
+
+/* Replaces `call pcgsip(fi, ifi)` (pcg-sip.f90:3-200): CG with the SIP
+ * preconditioner, k,i,j = 3..n?mm, stop when resl/res0 < sor. */
+int cap_pcgsip(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
+               double sor, int nsw, cap_report *rep);
+
+/* Replaces `call iccg(fi, ifi)` (iccg.f90:3-197): CG with IC(0),
+ * k = 2..nkm, i,j = 3..n?mm. */
+int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor, int nsw,
+             cap_report *rep);
+
+/* Replaces `call cgstab_sip(fi, ifi)` (cgstab_sip.f90:1-278): BiCGStab with
+ * the SIP preconditioner, k = 2..nkm, i,j = 3..n?mm. */
+int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
+                   double sor, int nsw, cap_report *rep);
+
+/* No counterpart in the reference (its solvers are FP64 only).  Solves the
+ * pcgsip system (same arrays, same range, same L1 stop test against the FP64
+ * residual of the starting fi) by FP64 iterative refinement around an FP32
+ * PCG with an FP32 SIP preconditioner and FP32 SpMV; the true residual, the
+ * dot products and the norms are FP64.  inner_rtol is the L1 reduction the
+ * FP32 inner solve is asked for per refinement step.  rep->iters counts FP32
+ * inner iterations. */
+int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
+                     double sor, int nsw, double inner_rtol, cap_report *rep);
+
+/* Replaces `call grad_gauss(u, dudx, dudy, dudz)` (GRAD_GAUSS.f90:1-105).
+ * geom holds fx, fy, fz, ar1x, ar1y, ar1z, ar2x, ar2y, ar2z, ar3x, ar3y,
+ * ar3z, vol in this order (module `geometry`). */
+int cap_grad_gauss(cap_ctx *ctx, const double *u, const double *const *geom,
+                   double *dudx, double *dudy, double *dudz);
+
+/* Replaces `call bpres(p)` (bpres.f90:3-81). */
+int cap_bpres(cap_ctx *ctx, double *p, const double *fx, const double *fy,
+              const double *fz);
+
+/* Problem generator of the measured workload (not a reference routine):
+ * the log-normal Neumann p' operator on a uniform grid with the reference-
+ * pressure cell (3,3,3) pinned as PISO_multiple_correction.f90:117-129 does,
+ * written into the given arrays (host or device per the context). */
+int cap_build_lognormal(cap_ctx *ctx, unsigned long long seed, double sigma,
+                        cap_coef *c);
+
+/* Measurement hook, no counterpart in the reference: average device time
+ * (ms, HIP events on the context's stream) of each kernel of one PCG-SIP
+ * iteration re-launched `reps` times on the vectors of the last pcgsip
+ * (fp32 = 0) or pcgsip_mixed (fp32 = 1) solve.  ms[0..4]: SIP forward sweep,
+ * SIP backward sweep + dot, pk update, SpMV + dot, fi/res update + L1. */
+int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

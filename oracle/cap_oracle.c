@@ -1,0 +1,662 @@
+/*
+ * cap_oracle.c -- CPU oracle (test infrastructure) for the p' hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  The product library never
+ * links or calls it.
+ *
+ * Each routine restates one reference routine with the reference's loop
+ * ranges, loop order (k outer, i middle, j inner = unit stride), operand
+ * order and single-precision literals promoted to double:
+ *   small = 1e-30   (modules_allocatable.f90:39)  -> (double)1e-30f
+ *   s0    = 1.e20   (pcg-sip.f90:89, iccg.f90:83)  -> (double)1e20f
+ * Compiled with -O2 -ffp-contract=off so no multiply-add is fused, which is
+ * what the reference's -O2 x86-64 build does (no FMA in the base ISA).
+ *
+ * Parity pinning (tests/test_oracle.py):
+ *  - orc_sipsol, orc_cgstab_sip, orc_grad_gauss, orc_bpres: against golden
+ *    vectors written by the reference Fortran compiled with amdflang
+ *    (oracle/Makefile target `ref`, fixtures in tests/golden/).
+ *  - orc_pcgsip, orc_iccg: the reference files call global_sum / exchange,
+ *    whose only definitions need mpif.h (absent), so they are not built.
+ *    Pinned instead by: (1) the SIP factors they share with sipsol/cgstab_sip
+ *    (golden); (2) forward/backward sweeps == dense solve with the assembled
+ *    L and U (numpy); (3) the PCG scalar recurrence against an independent
+ *    numpy PCG using dense L U as the preconditioner (iteration history to
+ *    1e-12); (4) the IC(0) diagonal/sweeps against a dense IC(0) of the same
+ *    7-diagonal matrix.
+ */
+#include "cap_oracle.h"
+#include <math.h>
+#include <string.h>
+#include <stdlib.h>
+#include <time.h>
+
+#define SMALL ((double)1e-30f)
+#define S0INIT ((double)1e20f)
+
+/* 1-based (i,j,k) -> C offset; li/lk of setind.f90:34-40 */
+#define IX(i, j, k) ((size_t)((k)-1) * (size_t)nij + (size_t)((i)-1) * (size_t)nj + (size_t)((j)-1))
+
+/* ------------------------------------------------------------------ RNG -- */
+static inline uint64_t smix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+uint64_t orc_hash(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return smix(smix(seed ^ (stream * 0xD1B54A32D192ED03ull)) + idx);
+}
+
+double orc_u01(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return (double)(orc_hash(seed, stream, idx) >> 11) * 0x1.0p-53;
+}
+
+double orc_normal(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    /* Box-Muller on two counter-keyed uniforms, u1 in (0,1] */
+    double u1 = (double)((orc_hash(seed, stream, 2 * idx) >> 11) + 1) * 0x1.0p-53;
+    double u2 = (double)(orc_hash(seed, stream, 2 * idx + 1) >> 11) * 0x1.0p-53;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+/* ------------------------------------------------------------ builders -- */
+static void zero_sys(size_t n, orc_sys *s)
+{
+    double *arrs[] = {s->ae, s->aw, s->an, s->as, s->at, s->ab, s->su};
+    for (int a = 0; a < 7; ++a) memset(arrs[a], 0, n * sizeof(double));
+    for (size_t c = 0; c < n; ++c) s->ap[c] = 1.0;
+}
+
+/*
+ * Neumann p' operator with log-normal face coefficients on a uniform grid:
+ * interior face (east/north/top of an interior cell whose neighbour is also
+ * interior) a_f = h*exp(sigma*g_f), g_f ~ N(0,1) keyed by (seed, dir, ijk);
+ * aw(E)=ae(P), as(N)=an(P), ab(T)=at(P); boundary faces 0; ap = sum a_nb;
+ * su ~ N(0,1) projected to zero mean; ghost rows a*=0, ap=1, su=0; the
+ * reference-pressure cell (3,3,3) pinned.
+ */
+void orc_build_lognormal(int ni, int nj, int nk, uint64_t seed, double sigma,
+                         orc_sys *s)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const size_t n = (size_t)nij * nk;
+    const double h = 1.0 / (double)(ni - 4);
+    zero_sys(n, s);
+    for (int k = 3; k <= nkmm; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                if (i < nimm) {
+                    double a = h * exp(sigma * orc_normal(seed, 1, c));
+                    s->ae[c] = a;
+                    s->aw[c + nj] = a;
+                }
+                if (j < njmm) {
+                    double a = h * exp(sigma * orc_normal(seed, 2, c));
+                    s->an[c] = a;
+                    s->as[c + 1] = a;
+                }
+                if (k < nkmm) {
+                    double a = h * exp(sigma * orc_normal(seed, 3, c));
+                    s->at[c] = a;
+                    s->ab[c + nij] = a;
+                }
+            }
+    double sum = 0.0;
+    long cnt = 0;
+    for (int k = 3; k <= nkmm; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                s->ap[c] = s->ae[c] + s->aw[c] + s->an[c] + s->as[c] + s->at[c] + s->ab[c];
+                s->su[c] = orc_normal(seed, 4, c);
+                sum += s->su[c];
+                ++cnt;
+            }
+    double mean = sum / (double)cnt;
+    for (int k = 3; k <= nkmm; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) s->su[IX(i, j, k)] -= mean;
+    /* reference-pressure cell (3,3,3) pinned as PISO_multiple_correction.f90:
+     * 117-129 does: ap = 1, a_nb = 0, su = p_ref (= 0, the starting value) */
+    size_t r = IX(3, 3, 3);
+    s->ae[r] = s->aw[r] = s->an[r] = s->as[r] = s->at[r] = s->ab[r] = 0.0;
+    s->ap[r] = 1.0;
+    s->su[r] = 0.0;
+}
+
+/*
+ * General non-symmetric, diagonally dominant 7-diagonal system whose rows
+ * couple to the ghost cells (Dirichlet data) and whose ghost planes k=2 and
+ * k=nkm carry rows of their own (the k-range of iccg / cgstab_sip).  fi gets
+ * random values everywhere (initial guess and ghost data).
+ */
+void orc_build_general(int ni, int nj, int nk, uint64_t seed, orc_sys *s,
+                       double *fi)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    const size_t n = (size_t)nij * nk;
+    zero_sys(n, s);
+    for (size_t c = 0; c < n; ++c) fi[c] = 2.0 * orc_u01(seed, 20, c) - 1.0;
+    for (int k = 2; k <= nkm; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                s->ae[c] = 0.5 + orc_u01(seed, 11, c);
+                s->aw[c] = 0.5 + orc_u01(seed, 12, c);
+                s->an[c] = 0.5 + orc_u01(seed, 13, c);
+                s->as[c] = 0.5 + orc_u01(seed, 14, c);
+                s->at[c] = 0.5 + orc_u01(seed, 15, c);
+                s->ab[c] = 0.5 + orc_u01(seed, 16, c);
+                s->ap[c] = (s->ae[c] + s->aw[c] + s->an[c] + s->as[c] + s->at[c] + s->ab[c]) *
+                           (1.0 + 0.05 * orc_u01(seed, 17, c));
+                s->su[c] = 2.0 * orc_u01(seed, 18, c) - 1.0;
+            }
+}
+
+/* ------------------------------------------------------- SIP building -- */
+/* SIP (Stone) factorisation: sipsol.f90:28-50 == pcg-sip.f90:68-87 ==
+ * cgstab_sip.f90:60-79; k range is the caller's. */
+void orc_sip_factor(int ni, int nj, int nk, int k_lo, int k_hi, orc_sys *s,
+                    double alfa)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2;
+    (void)nk;
+    double *bb = s->bb, *bw = s->bw, *bs = s->bs, *bp = s->bp, *bn = s->bn,
+           *be = s->be, *bt = s->bt;
+    for (int k = k_lo; k <= k_hi; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                bb[c] = -s->ab[c] / (1. + alfa * (bn[c - nij] + be[c - nij]));
+                bw[c] = -s->aw[c] / (1. + alfa * (bn[c - nj] + bt[c - nj]));
+                bs[c] = -s->as[c] / (1. + alfa * (be[c - 1] + bt[c - 1]));
+                double p1 = alfa * (bb[c] * bn[c - nij] + bw[c] * bn[c - nj]);
+                double p2 = alfa * (bb[c] * be[c - nij] + bs[c] * be[c - 1]);
+                double p3 = alfa * (bw[c] * bt[c - nj] + bs[c] * bt[c - 1]);
+                bp[c] = 1. / (s->ap[c] + p1 + p2 + p3 - bb[c] * bt[c - nij] -
+                              bw[c] * be[c - nj] - bs[c] * bn[c - 1] + SMALL);
+                bn[c] = (-s->an[c] - p1) * bp[c];
+                be[c] = (-s->ae[c] - p2) * bp[c];
+                bt[c] = (-s->at[c] - p3) * bp[c];
+            }
+}
+
+/* z = (LU)^-1 rhs: forward pcg-sip.f90:98-106, backward pcg-sip.f90:112-121.
+ * z must be zero outside the range (as the reference's zeroed zk). */
+void orc_sip_apply(int ni, int nj, int nk, int k_lo, int k_hi,
+                   const orc_sys *s, const double *rhs, double *z)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2;
+    (void)nk;
+    for (int k = k_lo; k <= k_hi; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                z[c] = (rhs[c] - s->bb[c] * z[c - nij] - s->bw[c] * z[c - nj] -
+                        s->bs[c] * z[c - 1]) * s->bp[c];
+            }
+    for (int k = k_hi; k >= k_lo; --k)
+        for (int i = nimm; i >= 3; --i)
+            for (int j = njmm; j >= 3; --j) {
+                size_t c = IX(i, j, k);
+                z[c] = z[c] - s->bn[c] * z[c + 1] - s->be[c] * z[c + nj] -
+                       s->bt[c] * z[c + nij];
+            }
+}
+
+/* z = A p with the operand order of pcg-sip.f90:147-149 */
+void orc_spmv(int ni, int nj, int nk, int k_lo, int k_hi, const orc_sys *s,
+              const double *p, double *z)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2;
+    (void)nk;
+    for (int k = k_lo; k <= k_hi; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                z[c] = s->ap[c] * p[c] - s->ae[c] * p[c + nj] - s->aw[c] * p[c - nj] -
+                       s->an[c] * p[c + 1] - s->as[c] * p[c - 1] - s->at[c] * p[c + nij] -
+                       s->ab[c] * p[c - nij];
+            }
+}
+
+/* residual r = sum a_nb fi_nb + su - ap fi: pcg-sip.f90:53-55 (same operand
+ * order in sipsol.f90:66-68, iccg.f90:57-59, cgstab_sip.f90:49-51) */
+static inline double resid(const orc_sys *s, const double *fi, size_t c, int nj, int nij)
+{
+    return s->ae[c] * fi[c + nj] + s->aw[c] * fi[c - nj] + s->an[c] * fi[c + 1] +
+           s->as[c] * fi[c - 1] + s->at[c] * fi[c + nij] + s->ab[c] * fi[c - nij] +
+           s->su[c] - s->ap[c] * fi[c];
+}
+
+/* --------------------------------------------------------------- SIPSOL -- */
+/* sipsol.f90:3-106 */
+void orc_sipsol(int ni, int nj, int nk, double *fi, orc_sys *s, double alfa,
+                double sor, int nsw, orc_report *rep)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    double *res = s->res;
+    orc_sip_factor(ni, nj, nk, 3, nkmm, s, alfa);            /* 28-50 */
+    double resor = 0.0, res1 = 0.0;
+    int n, conv = 0;
+    for (n = 1; n <= nsw; ++n) {
+        res1 = 0.0;
+        for (int k = 3; k <= nkmm; ++k)                       /* 60-75 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    res[c] = resid(s, fi, c, nj, nij);
+                    res1 = res1 + fabs(res[c]);
+                    res[c] = (res[c] - s->bb[c] * res[c - nij] - s->bw[c] * res[c - nj] -
+                              s->bs[c] * res[c - 1]) * s->bp[c];
+                }
+        if (n == 1) resor = res1;                             /* 77 */
+        double rsm = res1 / (resor + SMALL);                  /* 78 */
+        for (int k = nkmm; k >= 3; --k)                       /* 81-92 */
+            for (int i = nimm; i >= 3; --i)
+                for (int j = njmm; j >= 3; --j) {
+                    size_t c = IX(i, j, k);
+                    res[c] = res[c] - s->bn[c] * res[c + 1] - s->be[c] * res[c + nj] -
+                             s->bt[c] * res[c + nij];
+                    fi[c] = fi[c] + res[c];
+                }
+        if (rsm < sor) { conv = 1; break; }                   /* 96 */
+    }
+    rep->resor = resor;
+    rep->resl = res1;
+    rep->iters = n;   /* N after EXIT, or NS+1 when the DO loop ran out */
+    rep->converged = conv;
+}
+
+/* --------------------------------------------------------------- PCGSIP -- */
+/* pcg-sip.f90:3-200 (global_sum is the identity on one domain) */
+static void pcgsip_core(int ni, int nj, int nk, double *fi, orc_sys *s,
+                        double alfa, double sor, int nsw, int fixed_iters,
+                        orc_report *rep)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const size_t n = (size_t)nij * nk;
+    double *pk = calloc(n, sizeof(double)), *zk = calloc(n, sizeof(double));
+    double *res = s->res;
+    memset(res, 0, n * sizeof(double));                        /* 42-44 */
+    double res0 = 0.0;
+    for (int k = 3; k <= nkmm; ++k)                            /* 49-59 */
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                res[c] = resid(s, fi, c, nj, nij);
+                res0 = res0 + fabs(res[c]);
+            }
+    orc_sip_factor(ni, nj, nk, 3, nkmm, s, alfa);              /* 68-87 */
+    double s0 = S0INIT, resl = 0.0, resor = 0.0;
+    int l, conv = 0;
+    for (l = 1; l <= nsw; ++l) {
+        for (int k = 3; k <= nkmm; ++k)                        /* 98-106 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    zk[c] = (res[c] - s->bb[c] * zk[c - nij] - s->bw[c] * zk[c - nj] -
+                             s->bs[c] * zk[c - 1]) * s->bp[c];
+                }
+        double sk = 0.0;
+        for (int k = nkmm; k >= 3; --k)                        /* 111-121 */
+            for (int i = nimm; i >= 3; --i)
+                for (int j = njmm; j >= 3; --j) {
+                    size_t c = IX(i, j, k);
+                    zk[c] = zk[c] - s->bn[c] * zk[c + 1] - s->be[c] * zk[c + nj] -
+                            s->bt[c] * zk[c + nij];
+                    sk = sk + res[c] * zk[c];
+                }
+        double bet = sk / s0;                                  /* 127 */
+        for (int k = 3; k <= nkmm; ++k)                        /* 131-138 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    pk[c] = zk[c] + bet * pk[c];
+                }
+        double pkapk = 0.0;
+        for (int k = 3; k <= nkmm; ++k)                        /* 142-153 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    zk[c] = s->ap[c] * pk[c] - s->ae[c] * pk[c + nj] - s->aw[c] * pk[c - nj] -
+                            s->an[c] * pk[c + 1] - s->as[c] * pk[c - 1] -
+                            s->at[c] * pk[c + nij] - s->ab[c] * pk[c - nij];
+                    pkapk = pkapk + pk[c] * zk[c];
+                }
+        double alf = sk / pkapk;                               /* 157 */
+        resl = 0.0;
+        for (int k = 3; k <= nkmm; ++k)                        /* 161-175 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    fi[c] = fi[c] + alf * pk[c];
+                    res[c] = res[c] - alf * zk[c];
+                    resl = resl + fabs(res[c]);
+                }
+        s0 = sk;                                               /* 181 */
+        if (l == 1) resor = res0;                              /* 185 */
+        double rsm = resl / (resor + SMALL);                   /* 186 */
+        if (!fixed_iters && rsm < sor) { conv = 1; break; }    /* 188 */
+    }
+    rep->resor = res0;
+    rep->resl = resl;
+    rep->iters = l;
+    rep->converged = conv;
+    free(pk);
+    free(zk);
+}
+
+void orc_pcgsip(int ni, int nj, int nk, double *fi, orc_sys *s, double alfa,
+                double sor, int nsw, orc_report *rep)
+{
+    pcgsip_core(ni, nj, nk, fi, s, alfa, sor, nsw, 0, rep);
+}
+
+double orc_pcgsip_time_iters(int ni, int nj, int nk, double *fi, orc_sys *s,
+                             double alfa, int n_iters)
+{
+    struct timespec t0, t1;
+    orc_report rep;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    pcgsip_core(ni, nj, nk, fi, s, alfa, 0.0, n_iters, 1, &rep);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ----------------------------------------------------------------- ICCG -- */
+/* iccg.f90:3-197; k range 2..nkm */
+void orc_iccg(int ni, int nj, int nk, double *fi, orc_sys *s, double sor,
+              int nsw, orc_report *rep)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    const size_t n = (size_t)nij * nk;
+    double *pk = calloc(n, sizeof(double)), *zk = calloc(n, sizeof(double)),
+           *d = calloc(n, sizeof(double));
+    double *res = s->res;
+    memset(res, 0, n * sizeof(double));                        /* 45-48 */
+    double res0 = 0.0;
+    for (int k = 2; k <= nkm; ++k)                             /* 52-63 */
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                res[c] = resid(s, fi, c, nj, nij);
+                res0 = res0 + fabs(res[c]);
+            }
+    for (int k = 2; k <= nkm; ++k)                             /* 73-81 */
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                d[c] = 1. / (s->ap[c] - s->aw[c] * s->aw[c] * d[c - nj] -
+                             s->as[c] * s->as[c] * d[c - 1] -
+                             s->ab[c] * s->ab[c] * d[c - nij]);
+            }
+    double s0 = S0INIT, resl = 0.0, resor = 0.0;
+    int l, conv = 0;
+    for (l = 1; l <= nsw; ++l) {
+        for (int k = 2; k <= nkm; ++k)                         /* 92-100 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    zk[c] = (res[c] + s->aw[c] * zk[c - nj] + s->as[c] * zk[c - 1] +
+                             s->ab[c] * zk[c - nij]) * d[c];
+                }
+        for (int k = 2; k <= nkm; ++k)                         /* 102-109 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    zk[c] = zk[c] / (d[c] + SMALL);
+                }
+        double sk = 0.0;
+        for (int k = nkm; k >= 2; --k)                         /* 113-123 */
+            for (int i = nimm; i >= 3; --i)
+                for (int j = njmm; j >= 3; --j) {
+                    size_t c = IX(i, j, k);
+                    zk[c] = (zk[c] + s->ae[c] * zk[c + nj] + s->an[c] * zk[c + 1] +
+                             s->at[c] * zk[c + nij]) * d[c];
+                    sk = sk + res[c] * zk[c];
+                }
+        double bet = sk / s0;                                  /* 129 */
+        for (int k = 2; k <= nkm; ++k)                         /* 133-140 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    pk[c] = zk[c] + bet * pk[c];
+                }
+        double pkapk = 0.0;
+        for (int k = 2; k <= nkm; ++k)                         /* 144-155 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    zk[c] = s->ap[c] * pk[c] - s->ae[c] * pk[c + nj] - s->aw[c] * pk[c - nj] -
+                            s->an[c] * pk[c + 1] - s->as[c] * pk[c - 1] -
+                            s->at[c] * pk[c + nij] - s->ab[c] * pk[c - nij];
+                    pkapk = pkapk + pk[c] * zk[c];
+                }
+        double alf = sk / pkapk;                               /* 160 */
+        resl = 0.0;
+        for (int k = 2; k <= nkm; ++k)                         /* 164-174 */
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k);
+                    fi[c] = fi[c] + alf * pk[c];
+                    res[c] = res[c] - alf * zk[c];
+                    resl = resl + fabs(res[c]);
+                }
+        s0 = sk;                                               /* 178 */
+        if (l == 1) resor = res0;                              /* 182 */
+        double rsm = resl / (resor + SMALL);                   /* 183 */
+        if (rsm < sor) { conv = 1; break; }                    /* 185 */
+    }
+    rep->resor = res0;
+    rep->resl = resl;
+    rep->iters = l;
+    rep->converged = conv;
+    free(pk);
+    free(zk);
+    free(d);
+}
+
+/* ----------------------------------------------------------- CGSTAB_SIP -- */
+/* cgstab_sip.f90:1-278; k range 2..nkm; its local work arrays are zero
+ * outside the solve range (they are uninitialised in the reference). */
+void orc_cgstab_sip(int ni, int nj, int nk, double *fi, orc_sys *s,
+                    double alfa, double sor, int nsw, orc_report *rep)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    const size_t n = (size_t)nij * nk;
+    double *reso = calloc(n, sizeof(double)), *pk = calloc(n, sizeof(double)),
+           *uk = calloc(n, sizeof(double)), *zk = calloc(n, sizeof(double)),
+           *vk = calloc(n, sizeof(double));
+    double *res = s->res;
+#define LOOP_F for (int k = 2; k <= nkm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j)
+#define LOOP_B for (int k = nkm; k >= 2; --k) for (int i = nimm; i >= 3; --i) for (int j = njmm; j >= 3; --j)
+    double res0 = 0.0;
+    LOOP_F {                                                   /* 44-55 */
+        size_t c = IX(i, j, k);
+        res[c] = resid(s, fi, c, nj, nij);
+        res0 = res0 + fabs(res[c]);
+    }
+    orc_sip_factor(ni, nj, nk, 2, nkm, s, alfa);               /* 60-79 */
+    LOOP_F {                                                   /* 83-94 */
+        size_t c = IX(i, j, k);
+        reso[c] = res[c];
+        pk[c] = 0.0; uk[c] = 0.0; zk[c] = 0.0; vk[c] = 0.0;
+    }
+    double alf = 1.0, beto = 1.0, gam = 1.0, resl = 0.0, resor = 0.0;
+    int l, conv = 0;
+    for (l = 1; l <= nsw; ++l) {
+        double bet = 0.0;                                      /* 106-116 */
+        LOOP_F { size_t c = IX(i, j, k); bet = bet + res[c] * reso[c]; }
+        double om = bet * gam / (alf * beto + SMALL);
+        beto = bet;
+        LOOP_F {                                               /* 120-127 */
+            size_t c = IX(i, j, k);
+            pk[c] = res[c] + om * (pk[c] - alf * uk[c]);
+        }
+        LOOP_F {                                               /* 131-139 */
+            size_t c = IX(i, j, k);
+            zk[c] = (pk[c] - s->bb[c] * zk[c - nij] - s->bw[c] * zk[c - nj] -
+                     s->bs[c] * zk[c - 1]) * s->bp[c];
+        }
+        LOOP_B {                                               /* 144-152 */
+            size_t c = IX(i, j, k);
+            zk[c] = zk[c] - s->bn[c] * zk[c + 1] - s->be[c] * zk[c + nj] - s->bt[c] * zk[c + nij];
+        }
+        LOOP_F {                                               /* 156-166 */
+            size_t c = IX(i, j, k);
+            uk[c] = s->ap[c] * zk[c] - s->ae[c] * zk[c + nj] - s->aw[c] * zk[c - nj] -
+                    s->an[c] * zk[c + 1] - s->as[c] * zk[c - 1] - s->at[c] * zk[c + nij] -
+                    s->ab[c] * zk[c - nij];
+        }
+        double ukreso = 0.0;                                   /* 170-179 */
+        LOOP_F { size_t c = IX(i, j, k); ukreso = ukreso + uk[c] * reso[c]; }
+        gam = bet / ukreso;
+        LOOP_F {                                               /* 183-191 */
+            size_t c = IX(i, j, k);
+            fi[c] = fi[c] + gam * zk[c];
+            res[c] = res[c] - gam * uk[c];
+        }
+        LOOP_F {                                               /* 195-203 */
+            size_t c = IX(i, j, k);
+            zk[c] = (res[c] - s->bb[c] * zk[c - nij] - s->bw[c] * zk[c - nj] -
+                     s->bs[c] * zk[c - 1]) * s->bp[c];
+        }
+        LOOP_B {                                               /* 207-215 */
+            size_t c = IX(i, j, k);
+            zk[c] = zk[c] - s->bn[c] * zk[c + 1] - s->be[c] * zk[c + nj] - s->bt[c] * zk[c + nij];
+        }
+        LOOP_F {                                               /* 219-229 */
+            size_t c = IX(i, j, k);
+            vk[c] = s->ap[c] * zk[c] - s->ae[c] * zk[c + nj] - s->aw[c] * zk[c - nj] -
+                    s->an[c] * zk[c + 1] - s->as[c] * zk[c - 1] - s->at[c] * zk[c + nij] -
+                    s->ab[c] * zk[c - nij];
+        }
+        double vres = 0.0, vv = 0.0;                           /* 233-245 */
+        LOOP_F {
+            size_t c = IX(i, j, k);
+            vres = vres + vk[c] * res[c];
+            vv = vv + vk[c] * vk[c];
+        }
+        alf = vres / (vv + SMALL);
+        resl = 0.0;                                            /* 249-259 */
+        LOOP_F {
+            size_t c = IX(i, j, k);
+            fi[c] = fi[c] + alf * zk[c];
+            res[c] = res[c] - alf * vk[c];
+            resl = resl + fabs(res[c]);
+        }
+        if (l == 1) resor = res0;                              /* 263 */
+        double rsm = resl / (resor + SMALL);
+        if (rsm < sor) { conv = 1; break; }                    /* 266 */
+    }
+#undef LOOP_F
+#undef LOOP_B
+    rep->resor = res0;
+    rep->resl = resl;
+    rep->iters = l;
+    rep->converged = conv;
+    free(reso); free(pk); free(uk); free(zk); free(vk);
+}
+
+/* ----------------------------------------------------------- stencils -- */
+/* GRAD_GAUSS.f90:1-105: zero, face scatter over k=2..nkm, i=2..nimm,
+ * j=2..njmm, then scale by volr=1/vol on k=2..nkm, i,j=3..n?mm. */
+void orc_grad_gauss(int ni, int nj, int nk, const double *u, const double *fx,
+                    const double *fy, const double *fz, const double *ar1x,
+                    const double *ar1y, const double *ar1z, const double *ar2x,
+                    const double *ar2y, const double *ar2z, const double *ar3x,
+                    const double *ar3y, const double *ar3z, const double *vol,
+                    double *dudx, double *dudy, double *dudz)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    const size_t n = (size_t)nij * nk;
+    memset(dudx, 0, n * sizeof(double));
+    memset(dudy, 0, n * sizeof(double));
+    memset(dudz, 0, n * sizeof(double));
+    for (int k = 2; k <= nkm; ++k)
+        for (int i = 2; i <= nimm; ++i)
+            for (int j = 2; j <= njmm; ++j) {
+                size_t p = IX(i, j, k), e = p + nj, nn = p + 1, t = p + nij;
+                double ue = u[e] * fx[p] + u[p] * (1.0 - fx[p]);
+                double un = u[nn] * fy[p] + u[p] * (1.0 - fy[p]);
+                double ut = u[t] * fz[p] + u[p] * (1.0 - fz[p]);
+                dudx[p] = dudx[p] + (ue * ar1x[p] + un * ar2x[p] + ut * ar3x[p]);
+                dudx[e] = dudx[e] - (ue * ar1x[p]);
+                dudx[nn] = dudx[nn] - (un * ar2x[p]);
+                dudx[t] = dudx[t] - (ut * ar3x[p]);
+                dudy[p] = dudy[p] + (ue * ar1y[p] + un * ar2y[p] + ut * ar3y[p]);
+                dudy[e] = dudy[e] - (ue * ar1y[p]);
+                dudy[nn] = dudy[nn] - (un * ar2y[p]);
+                dudy[t] = dudy[t] - (ut * ar3y[p]);
+                dudz[p] = dudz[p] + (ue * ar1z[p] + un * ar2z[p] + ut * ar3z[p]);
+                dudz[e] = dudz[e] - (ue * ar1z[p]);
+                dudz[nn] = dudz[nn] - (un * ar2z[p]);
+                dudz[t] = dudz[t] - (ut * ar3z[p]);
+            }
+    for (int k = 2; k <= nkm; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t p = IX(i, j, k);
+                double volr = 1.0 / vol[p];
+                dudx[p] = dudx[p] * volr;
+                dudy[p] = dudy[p] * volr;
+                dudz[p] = dudz[p] * volr;
+            }
+}
+
+/* bpres.f90:29-79: linear extrapolation of p into the six ghost planes */
+void orc_bpres(int ni, int nj, int nk, double *p, const double *fx,
+               const double *fy, const double *fz)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    (void)nk;
+    for (int i = 3; i <= nimm; ++i)
+        for (int j = 3; j <= njmm; ++j) {
+            size_t c = IX(i, j, 3);
+            p[c - nij] = p[c] - (p[c + nij] - p[c]) * fz[c];
+        }
+    for (int i = 3; i <= nimm; ++i)
+        for (int j = 3; j <= njmm; ++j) {
+            size_t c = IX(i, j, nkmm);
+            p[c + nij] = p[c] + (p[c] - p[c - nij]) * (1.0 - fz[c - nij]);
+        }
+    for (int j = 3; j <= njmm; ++j)
+        for (int k = 3; k <= nkmm; ++k) {
+            size_t c = IX(3, j, k);
+            p[c - nj] = p[c] - (p[c + nj] - p[c]) * fx[c];
+        }
+    for (int j = 3; j <= njmm; ++j)
+        for (int k = 3; k <= nkmm; ++k) {
+            size_t c = IX(nimm, j, k);
+            p[c + nj] = p[c] + (p[c] - p[c - nj]) * (1.0 - fx[c - nj]);
+        }
+    for (int i = 3; i <= nimm; ++i)
+        for (int k = 3; k <= nkmm; ++k) {
+            size_t c = IX(i, 3, k);
+            p[c - 1] = p[c] - (p[c + 1] - p[c]) * fy[c];
+        }
+    for (int i = 3; i <= nimm; ++i)
+        for (int k = 3; k <= nkmm; ++k) {
+            size_t c = IX(i, njmm, k);
+            p[c + 1] = p[c] + (p[c] - p[c - 1]) * (1.0 - fy[c - 1]);
+        }
+}

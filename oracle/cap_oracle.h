@@ -1,0 +1,84 @@
+/*
+ * cap_oracle.h -- CPU oracle for the pressure-correction (p') hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links this; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, and
+ * only as the checker (or the timed CPU baseline), never as the thing shipped.
+ *
+ * A single-threaded C restatement of the reference Fortran routines, with the
+ * reference's loop ranges, loop order, operand order and single-precision
+ * literals, on the reference's padded j-fastest layout:
+ *   ijk = lk(k) + li(i) + j, li(i) = (i-1)*nj, lk(k) = (k-1)*ni*nj   (1-based)
+ *   C offset = ijk - 1.
+ * Each function names the reference file:line it follows.
+ */
+#ifndef CAP_ORACLE_H
+#define CAP_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Module `coef` + `coefb` arrays the reference solvers read and write
+ * (modules_allocatable.f90:161-180).  Every pointer is a full padded array of
+ * ni*nj*nk doubles. */
+typedef struct {
+    double *ae, *aw, *an, *as, *at, *ab, *ap, *su, *res;
+    double *bb, *bs, *bw, *bp, *bn, *be, *bt;
+} orc_sys;
+
+typedef struct {
+    double resor;   /* initial L1 residual (reference `resor(ifi)`)          */
+    double resl;    /* last L1 residual the reference prints as "Final"      */
+    int iters;      /* the loop index the reference prints ("No Iterations") */
+    int converged;  /* 1 if the loop exited on the relative-residual test    */
+} orc_report;
+
+/* counter-based generator shared bit-for-bit with the product's generator */
+uint64_t orc_hash(uint64_t seed, uint64_t stream, uint64_t idx);
+double orc_u01(uint64_t seed, uint64_t stream, uint64_t idx);
+double orc_normal(uint64_t seed, uint64_t stream, uint64_t idx);
+
+/* problem builders (synthetic inputs of the measured workloads) */
+void orc_build_lognormal(int ni, int nj, int nk, uint64_t seed, double sigma,
+                         orc_sys *s);
+void orc_build_general(int ni, int nj, int nk, uint64_t seed, orc_sys *s,
+                       double *fi);
+
+/* solvers */
+void orc_sipsol(int ni, int nj, int nk, double *fi, orc_sys *s, double alfa,
+                double sor, int nsw, orc_report *rep);
+void orc_pcgsip(int ni, int nj, int nk, double *fi, orc_sys *s, double alfa,
+                double sor, int nsw, orc_report *rep);
+void orc_iccg(int ni, int nj, int nk, double *fi, orc_sys *s, double sor,
+              int nsw, orc_report *rep);
+void orc_cgstab_sip(int ni, int nj, int nk, double *fi, orc_sys *s,
+                    double alfa, double sor, int nsw, orc_report *rep);
+
+/* pieces used by the parity tests to pin single steps */
+void orc_sip_factor(int ni, int nj, int nk, int k_lo, int k_hi, orc_sys *s,
+                    double alfa);
+void orc_sip_apply(int ni, int nj, int nk, int k_lo, int k_hi,
+                   const orc_sys *s, const double *rhs, double *z);
+void orc_spmv(int ni, int nj, int nk, int k_lo, int k_hi, const orc_sys *s,
+              const double *p, double *z);
+
+/* stencils */
+void orc_grad_gauss(int ni, int nj, int nk, const double *u, const double *fx,
+                    const double *fy, const double *fz, const double *ar1x,
+                    const double *ar1y, const double *ar1z, const double *ar2x,
+                    const double *ar2y, const double *ar2z, const double *ar3x,
+                    const double *ar3y, const double *ar3z, const double *vol,
+                    double *dudx, double *dudy, double *dudz);
+void orc_bpres(int ni, int nj, int nk, double *p, const double *fx,
+               const double *fy, const double *fz);
+
+/* timing helper for the CPU baseline: n PCG-SIP iterations from fi */
+double orc_pcgsip_time_iters(int ni, int nj, int nk, double *fi, orc_sys *s,
+                             double alfa, int n_iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,228 @@
+"""CPU oracle for the p' hot path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front-end of ``liboracle.so`` (the C restatement in ``cap_oracle.c``)
+plus a runner for the reference-compiled driver in ``oracle/_ref/`` (build
+container only).  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg import this module, and only as the checker.
+
+Arrays are numpy float64 vectors of length ni*nj*nk in the reference's padded
+j-fastest layout (``ijk = (k-1)*ni*nj + (i-1)*nj + j``, C offset ``ijk-1``);
+``as_grid(a, dims)`` views one as ``[k, i, j]`` (0-based).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+
+SYS_FIELDS = ("ae", "aw", "an", "as_", "at", "ab", "ap", "su", "res",
+              "bb", "bs", "bw", "bp", "bn", "be", "bt")
+
+SMALL = float(np.float32(1e-30))
+ALFA = 0.92
+
+
+class _Sys(C.Structure):
+    _fields_ = [(f, C.POINTER(C.c_double)) for f in SYS_FIELDS]
+
+
+class _Report(C.Structure):
+    _fields_ = [("resor", C.c_double), ("resl", C.c_double),
+                ("iters", C.c_int), ("converged", C.c_int)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        i, u64, d = C.c_int, C.c_uint64, C.c_double
+        L.orc_u01.argtypes = [u64, u64, u64]
+        L.orc_u01.restype = d
+        L.orc_normal.argtypes = [u64, u64, u64]
+        L.orc_normal.restype = d
+        L.orc_hash.argtypes = [u64, u64, u64]
+        L.orc_hash.restype = u64
+        L.orc_build_lognormal.argtypes = [i, i, i, u64, d, C.POINTER(_Sys)]
+        L.orc_build_general.argtypes = [i, i, i, u64, C.POINTER(_Sys), dp]
+        for fn in ("orc_sipsol", "orc_pcgsip", "orc_cgstab_sip"):
+            getattr(L, fn).argtypes = [i, i, i, dp, C.POINTER(_Sys), d, d, i,
+                                       C.POINTER(_Report)]
+        L.orc_iccg.argtypes = [i, i, i, dp, C.POINTER(_Sys), d, i, C.POINTER(_Report)]
+        L.orc_sip_factor.argtypes = [i, i, i, i, i, C.POINTER(_Sys), d]
+        L.orc_sip_apply.argtypes = [i, i, i, i, i, C.POINTER(_Sys), dp, dp]
+        L.orc_spmv.argtypes = [i, i, i, i, i, C.POINTER(_Sys), dp, dp]
+        L.orc_grad_gauss.argtypes = [i, i, i] + [dp] * 17
+        L.orc_bpres.argtypes = [i, i, i, dp, dp, dp, dp]
+        L.orc_pcgsip_time_iters.argtypes = [i, i, i, dp, C.POINTER(_Sys), d, i]
+        L.orc_pcgsip_time_iters.restype = d
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class System:
+    """The module arrays `coef`/`coefb` that the reference solvers use."""
+
+    def __init__(self, dims):
+        self.dims = tuple(int(x) for x in dims)
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        for f in SYS_FIELDS:
+            setattr(self, f, np.zeros(n))
+        self.ap[:] = 1.0
+
+    def copy(self):
+        s = System(self.dims)
+        for f in SYS_FIELDS:
+            getattr(s, f)[:] = getattr(self, f)
+        return s
+
+    def _struct(self):
+        return _Sys(*[_p(getattr(self, f)) for f in SYS_FIELDS])
+
+    @property
+    def n(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+
+def as_grid(a, dims):
+    ni, nj, nk = dims
+    return a.reshape(nk, ni, nj)
+
+
+def build_lognormal(dims, seed, sigma=1.0):
+    s = System(dims)
+    lib().orc_build_lognormal(*s.dims, seed, sigma, C.byref(s._struct()))
+    return s
+
+
+def build_general(dims, seed):
+    s = System(dims)
+    fi = np.zeros(s.n)
+    lib().orc_build_general(*s.dims, seed, C.byref(s._struct()), _p(fi))
+    return s, fi
+
+
+def _solve(fn, s, fi, *args):
+    rep = _Report()
+    st = s._struct()
+    fn(*s.dims, _p(fi), C.byref(st), *args, C.byref(rep))
+    return dict(resor=rep.resor, resl=rep.resl, iters=rep.iters,
+                converged=bool(rep.converged))
+
+
+def sipsol(s, fi, alfa=ALFA, sor=1e-8, nsw=1000):
+    return _solve(lib().orc_sipsol, s, fi, alfa, sor, nsw)
+
+
+def pcgsip(s, fi, alfa=ALFA, sor=1e-8, nsw=1000):
+    return _solve(lib().orc_pcgsip, s, fi, alfa, sor, nsw)
+
+
+def cgstab_sip(s, fi, alfa=ALFA, sor=1e-8, nsw=1000):
+    return _solve(lib().orc_cgstab_sip, s, fi, alfa, sor, nsw)
+
+
+def iccg(s, fi, sor=1e-8, nsw=1000):
+    return _solve(lib().orc_iccg, s, fi, sor, nsw)
+
+
+def sip_factor(s, k_lo, k_hi, alfa=ALFA):
+    lib().orc_sip_factor(*s.dims, k_lo, k_hi, C.byref(s._struct()), alfa)
+
+
+def sip_apply(s, k_lo, k_hi, rhs):
+    z = np.zeros(s.n)
+    lib().orc_sip_apply(*s.dims, k_lo, k_hi, C.byref(s._struct()), _p(rhs), _p(z))
+    return z
+
+
+def spmv(s, k_lo, k_hi, p):
+    z = np.zeros(s.n)
+    lib().orc_spmv(*s.dims, k_lo, k_hi, C.byref(s._struct()), _p(p), _p(z))
+    return z
+
+
+def pcgsip_time_iters(s, fi, n_iters, alfa=ALFA):
+    return lib().orc_pcgsip_time_iters(*s.dims, _p(fi), C.byref(s._struct()), alfa, n_iters)
+
+
+GEOM_FIELDS = ("fx", "fy", "fz", "ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z",
+               "ar3x", "ar3y", "ar3z", "vol")
+
+
+def grad_gauss(dims, u, geom):
+    n = dims[0] * dims[1] * dims[2]
+    out = [np.zeros(n) for _ in range(3)]
+    lib().orc_grad_gauss(*dims, _p(u), *[_p(geom[f]) for f in GEOM_FIELDS],
+                         *[_p(o) for o in out])
+    return out
+
+
+def bpres(dims, p, geom):
+    lib().orc_bpres(*dims, _p(p), _p(geom["fx"]), _p(geom["fy"]), _p(geom["fz"]))
+
+
+def rng_u01(seed, stream, idx):
+    return lib().orc_u01(seed, stream, idx)
+
+
+def rng_normal(seed, stream, idx):
+    return lib().orc_normal(seed, stream, idx)
+
+
+# ----------------------------------------------------- reference driver --
+def ref_available():
+    return os.path.exists(REF_DRIVER)
+
+
+_LOG_RE = re.compile(r"Final residual =\s*([-+0-9.Ee]+), No Iterations (\d+)")
+
+
+def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8):
+    """Run the reference-compiled driver; returns (outputs, resor, log)."""
+    ni, nj, nk = dims
+    n = ni * nj * nk
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout, flog = (os.path.join(td, x) for x in ("in.bin", "out.bin", "log.txt"))
+        with open(fin, "wb") as f:
+            np.array([ni, nj, nk, nsw], dtype=np.int32).tofile(f)
+            np.array([alfa, sor], dtype=np.float64).tofile(f)
+            for a in arrays:
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        subprocess.run([REF_DRIVER, mode, fin, fout, flog], check=True,
+                       capture_output=True)
+        raw = np.fromfile(fout, dtype=np.float64)
+        log = open(flog).read()
+    nout = {"sipsol": 9, "cgstab_sip": 9, "grad_gauss": 3, "bpres": 1}[mode]
+    outs = [raw[i * n:(i + 1) * n].copy() for i in range(nout)]
+    resor = float(raw[nout * n]) if raw.size > nout * n else None
+    return outs, resor, log
+
+
+def parse_log(log):
+    m = _LOG_RE.search(log)
+    if not m:
+        return None
+    return dict(resl=float(m.group(1)), iters=int(m.group(2)))

@@ -1,0 +1,129 @@
+// emu_wave.cpp -- host emulation of the GPU wavefront schedule (test helper).
+//
+// Runs the tile-sheared layout and the per-step lane schedule of
+// cap_wave.h sequentially: tiles in claim order, steps in order, the 256
+// lanes of a step one after another, with the previous step's lane values
+// standing in for the workgroup's LDS exchange.  Used by tests to check the
+// schedule and the layout against the oracle bit for bit without a GPU.
+#include "../cappuccino-ghostcells-mpi_amd/csrc/cap_wave.h"
+#include <vector>
+#include <cstring>
+
+using namespace cap;
+
+template <class Op, bool FWD>
+static double run(const Lay &L, const Op &op, const double *const *in, double *const *dep,
+                  double *const *xo)
+{
+    constexpr int ND = Op::ND, NIN = Op::NIN;
+    constexpr int NX = Op::NX > 0 ? Op::NX : 1, NR = Op::NR > 0 ? Op::NR : 1;
+    double red = 0.0;
+    const int NST = nsteps(L);
+    for (int c = 0; c < L.ntiles; ++c) {
+        int ti, tj;
+        claim_tile<FWD>(L, c, ti, tj);
+        std::vector<LaneInfo> li(TCOL);
+        for (int lane = 0; lane < TCOL; ++lane) lane_setup<FWD>(L, ti, tj, lane, li[lane]);
+        std::vector<double> prev(TCOL * ND, 0.0), cur(TCOL * ND, 0.0);
+        const size_t tb = tile_base(L, ti, tj);
+        for (int t = 0; t < NST; ++t) {
+            const int slot = step_slot<FWD>(L, t);
+            for (int lane = 0; lane < TCOL; ++lane) {
+                const LaneInfo &l = li[lane];
+                int kk;
+                int kind = step_kind<FWD>(L, l, t, kk);
+                size_t e = tb + (size_t)slot * TCOL + lane;
+                double out[ND];
+                if (kind == K_COMPUTE) {
+                    double x[NIN], dk[ND], di[ND], dj[ND], xv[NX], r[NR];
+                    for (int q = 0; q < NIN; ++q) x[q] = in[q][e];
+                    for (int q = 0; q < ND; ++q) {
+                        dk[q] = prev[lane * ND + q];
+                        dj[q] = l.pa == 0 ? dep[q][xa_elem<FWD>(L, ti, tj, l, kk)]
+                                          : prev[l.nb_a * ND + q];
+                        di[q] = l.pb == 0 ? dep[q][xb_elem<FWD>(L, ti, tj, l, kk)]
+                                          : prev[l.nb_b * ND + q];
+                    }
+                    op.compute(x, dk, di, dj, out, xv, r);
+                    for (int q = 0; q < ND; ++q) dep[q][e] = out[q];
+                    for (int q = 0; q < Op::NX; ++q) xo[q][e] = xv[q];
+                    if (Op::NR) red += r[0];
+                } else if (kind == K_LOAD) {
+                    for (int q = 0; q < ND; ++q) out[q] = dep[q][e];
+                } else {
+                    for (int q = 0; q < ND; ++q) out[q] = 0.0;
+                }
+                for (int q = 0; q < ND; ++q) cur[lane * ND + q] = out[q];
+            }
+            prev.swap(cur);
+        }
+    }
+    return red;
+}
+
+extern "C" {
+
+void emu_lay(int ni, int nj, int nk, int klo, int khi, int *out)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    out[0] = L.nTI; out[1] = L.nTJ; out[2] = L.S; out[3] = L.ntiles;
+    out[4] = (int)L.n;
+}
+
+void emu_gather(int ni, int nj, int nk, int klo, int khi, const double *std_, double *t)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    for (size_t e = 0; e < L.n; ++e) {
+        int ci, cj, kk;
+        t[e] = decode(L, e, ci, cj, kk) ? std_[std_index(L, ci, cj, kk)] : 0.0;
+    }
+}
+
+void emu_scatter(int ni, int nj, int nk, int klo, int khi, const double *t, double *std_)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    for (size_t e = 0; e < L.n; ++e) {
+        int ci, cj, kk;
+        if (decode(L, e, ci, cj, kk) && col_interior(L, ci, cj) && kk >= 1 && kk <= L.nks)
+            std_[std_index(L, ci, cj, kk)] = t[e];
+    }
+}
+
+// op: 0 sip factor, 1 sip fwd, 2 sip bwd+dot, 3 ic diag, 4 ic fwd, 5 ic bwd+dot,
+//     6 sip bwd (no dot), 7 sipsol bwd+update
+double emu_wave(int op, int ni, int nj, int nk, int klo, int khi, double alfa,
+                const double *const *in, double *const *dep, double *const *xo)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    const double small = (double)1e-30f;
+    switch (op) {
+    case 0: { OpSipFactor o; o.alfa = alfa; o.small = small; return run<OpSipFactor, true>(L, o, in, dep, xo); }
+    case 1: return run<OpSipFwd, true>(L, OpSipFwd(), in, dep, xo);
+    case 2: return run<OpSipBwdDot, false>(L, OpSipBwdDot(), in, dep, xo);
+    case 3: return run<OpIcDiag, true>(L, OpIcDiag(), in, dep, xo);
+    case 4: return run<OpIcFwd, true>(L, OpIcFwd(), in, dep, xo);
+    case 5: { OpIcBwdDot o; o.small = small; return run<OpIcBwdDot, false>(L, o, in, dep, xo); }
+    case 6: return run<OpSipBwd, false>(L, OpSipBwd(), in, dep, xo);
+    case 7: return run<OpSipBwdUpdate, false>(L, OpSipBwdUpdate(), in, dep, xo);
+    }
+    return 0.0;
+}
+
+// z = A p on interior cells with the operand order of pcg-sip.f90:147-149
+void emu_spmv(int ni, int nj, int nk, int klo, int khi, const double *const *A,
+              const double *p, double *z)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    // A = {ae, aw, an, as, at, ab, ap}
+    for (size_t e = 0; e < L.n; ++e) {
+        int ci, cj, kk;
+        if (!decode(L, e, ci, cj, kk) || !col_interior(L, ci, cj) || kk < 1 || kk > L.nks) continue;
+        int ti = ci / TW, b = ci % TW, tj = cj / TW, a = cj % TW;
+        int slot = posmod(kk + a + b, L.S);
+        Nb nb = neighbours(L, ti, tj, slot, b, a, kk);
+        z[e] = A[6][e] * p[e] - A[0][e] * p[nb.e] - A[1][e] * p[nb.w] - A[2][e] * p[nb.n] -
+               A[3][e] * p[nb.s] - A[4][e] * p[nb.t] - A[5][e] * p[nb.bt];
+    }
+}
+
+}  // extern "C"
