@@ -1,0 +1,105 @@
+#!/usr/bin/env python3
+"""Write the golden vectors in tests/golden/ from the reference routines.
+
+Runs oracle/_ref/ref_driver -- the reference's own sipsol.f90, cgstab_sip.f90,
+GRAD_GAUSS.f90 and bpres.f90 compiled with amdflang by `make -C oracle ref`
+(build container only: the reference sources are not on the GPU machines) --
+on seeded inputs, and stores inputs' seeds and the reference outputs as
+small .npz fixtures.  Inputs are regenerated by the tests from the same seeds
+(oracle builders + numpy PCG64), so only outputs are stored.
+
+usage: python tests/golden/make_golden.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as O  # noqa: E402
+
+B_FIELDS = ("bb", "bs", "bw", "bp", "bn", "be", "bt")
+SOLVER_OUT = ("fi", "res") + B_FIELDS
+
+
+def solver_inputs(kind, dims, seed):
+    """Seeded inputs: the system, fi, and random halo data in res / coefb."""
+    if kind == "general":
+        s, fi = O.build_general(dims, seed)
+    else:
+        s = O.build_lognormal(dims, seed, 1.0)
+        fi = np.zeros(s.n)
+    rng = np.random.default_rng(seed)
+    s.res[:] = rng.uniform(-1, 1, s.n)
+    for f in B_FIELDS:
+        getattr(s, f)[:] = rng.uniform(-0.2, 0.2, s.n)
+    return s, fi
+
+
+def geometry(dims, seed):
+    rng = np.random.default_rng(seed)
+    n = dims[0] * dims[1] * dims[2]
+    g = {f: rng.uniform(0.2, 0.8, n) for f in ("fx", "fy", "fz")}
+    for f in ("ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z"):
+        g[f] = rng.uniform(-1, 1, n)
+    g["vol"] = rng.uniform(0.5, 2.0, n)
+    u = rng.normal(size=n)
+    return g, u
+
+
+# (name, routine, kind, dims, seed, nsw, alfa, full arrays?)
+SOLVER_CASES = [
+    ("sipsol_general_12x10x9", "sipsol", "general", (12, 10, 9), 101, 25, 0.92, True),
+    ("sipsol_general_20x37x23", "sipsol", "general", (20, 37, 23), 102, 12, 0.92, False),
+    ("sipsol_lognormal_alfa0_16", "sipsol", "lognormal", (16, 16, 16), 103, 30, 0.0, True),
+    ("cgstab_general_12x10x9", "cgstab_sip", "general", (12, 10, 9), 104, 400, 0.92, True),
+    ("cgstab_lognormal_20", "cgstab_sip", "lognormal", (20, 20, 20), 105, 400, 0.92, True),
+    ("cgstab_lognormal_36", "cgstab_sip", "lognormal", (36, 36, 36), 106, 400, 0.92, False),
+]
+STENCIL_CASES = [
+    ("grad_gauss_12x10x9", "grad_gauss", (12, 10, 9), 201),
+    ("grad_gauss_20x37x23", "grad_gauss", (20, 37, 23), 202),
+    ("bpres_12x10x9", "bpres", (12, 10, 9), 203),
+]
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def main():
+    if not O.ref_available():
+        sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
+    for name, routine, kind, dims, seed, nsw, alfa, full in SOLVER_CASES:
+        s, fi = solver_inputs(kind, dims, seed)
+        arrays = [fi, s.ae, s.aw, s.an, s.as_, s.at, s.ab, s.ap, s.su, s.res] + \
+                 [getattr(s, f) for f in B_FIELDS]
+        outs, resor, log = O.run_ref(routine, dims, arrays, nsw=nsw, alfa=alfa, sor=1e-8)
+        info = O.parse_log(log)
+        rec = dict(routine=routine, kind=kind, dims=np.array(dims), seed=seed, nsw=nsw,
+                   alfa=alfa, resor=resor, iters=info["iters"], resl_printed=info["resl"])
+        for f, a in zip(SOLVER_OUT, outs):
+            if full:
+                rec[f] = a
+            rec[f + "_sha256"] = digest(a)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print(f"{name}: iters {info['iters']} resor {resor:.6e}")
+    for name, routine, dims, seed in STENCIL_CASES:
+        g, u = geometry(dims, seed)
+        if routine == "grad_gauss":
+            arrays = [u] + [g[f] for f in O.GEOM_FIELDS]
+            outs, _, _ = O.run_ref("grad_gauss", dims, arrays)
+            rec = dict(dims=np.array(dims), seed=seed, dudx=outs[0], dudy=outs[1], dudz=outs[2])
+        else:
+            arrays = [u, g["fx"], g["fy"], g["fz"]]
+            outs, _, _ = O.run_ref("bpres", dims, arrays)
+            rec = dict(dims=np.array(dims), seed=seed, p=outs[0])
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print(f"{name}: written")
+
+
+if __name__ == "__main__":
+    main()

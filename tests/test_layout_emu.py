@@ -1,0 +1,139 @@
+"""The GPU wavefront schedule and tile-sheared layout, emulated on the host.
+
+tests/emu_wave.cpp runs the same index functions and per-step lane schedule
+as the HIP kernels (csrc/cap_layout.h, csrc/cap_wave.h) sequentially; the
+results must be bit-identical to the oracle's sequential loops, across tile
+boundaries, both k-ranges and halo data in res / bn / be / bt.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "emu_wave.cpp")
+LIB = os.path.join(HERE, "libemu_wave.so")
+dp = C.POINTER(C.c_double)
+
+
+@pytest.fixture(scope="module")
+def E():
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c++17",
+                    "-o", LIB, SRC], check=True)
+    L = C.CDLL(LIB)
+    L.emu_wave.restype = C.c_double
+    L.emu_wave.argtypes = [C.c_int] * 6 + [C.c_double, C.POINTER(dp), C.POINTER(dp),
+                                           C.POINTER(dp)]
+    return L
+
+
+def P(a):
+    return a.ctypes.data_as(dp)
+
+
+def arr(lst):
+    return (dp * len(lst))(*[P(a) for a in lst])
+
+
+class Emu:
+    def __init__(self, E, dims, klo, khi):
+        self.E, self.dims, self.k = E, dims, (klo, khi)
+        o = (C.c_int * 5)()
+        E.emu_lay(*dims, klo, khi, o)
+        self.n = o[4]
+
+    def gather(self, a):
+        t = np.zeros(self.n)
+        self.E.emu_gather(*self.dims, *self.k, P(a), P(t))
+        return t
+
+    def scatter(self, t, base):
+        out = base.copy()
+        self.E.emu_scatter(*self.dims, *self.k, P(t), P(out))
+        return out
+
+    def wave(self, op, alfa, ins, deps, xos):
+        return self.E.emu_wave(op, *self.dims, *self.k, alfa, arr(ins), arr(deps),
+                               arr(xos if xos else [deps[0]]))
+
+
+CASES = [((12, 10, 9), 3, 7), ((12, 10, 9), 2, 8), ((21, 38, 7), 3, 5), ((36, 36, 36), 2, 35),
+         ((40, 23, 19), 3, 17)]
+
+
+def halo_system(O, dims, seed):
+    s, fi = O.build_general(dims, seed)
+    rng = np.random.default_rng(seed)
+    for f in ("bn", "be", "bt", "bb", "bw", "bs", "bp", "res"):
+        getattr(s, f)[:] = rng.uniform(-0.2, 0.2, s.n)
+    return s, fi
+
+
+@pytest.mark.parametrize("dims,klo,khi", CASES)
+def test_sip_factor_and_sweeps(E, O, dims, klo, khi):
+    s, _ = halo_system(O, dims, 3)
+    ref = s.copy()
+    O.sip_factor(ref, klo, khi, alfa=0.92)
+    em = Emu(E, dims, klo, khi)
+    T = {f: em.gather(getattr(s, f)) for f in ("ab", "aw", "as_", "ap", "an", "ae", "at",
+                                                "bn", "be", "bt")}
+    xo = {f: np.zeros(em.n) for f in ("bb", "bw", "bs", "bp")}
+    em.wave(0, 0.92, [T[f] for f in ("ab", "aw", "as_", "ap", "an", "ae", "at")],
+            [T["bn"], T["be"], T["bt"]], [xo["bb"], xo["bw"], xo["bs"], xo["bp"]])
+    for f, t in (("bn", T["bn"]), ("be", T["be"]), ("bt", T["bt"]), ("bb", xo["bb"]),
+                 ("bw", xo["bw"]), ("bs", xo["bs"]), ("bp", xo["bp"])):
+        np.testing.assert_array_equal(em.scatter(t, getattr(s, f)), getattr(ref, f), err_msg=f)
+    rhs = np.random.default_rng(1).normal(size=s.n)
+    zref = O.sip_apply(ref, klo, khi, rhs)
+    F = {f: em.gather(getattr(ref, f)) for f in ("bb", "bw", "bs", "bp", "bn", "be", "bt")}
+    tr, z = em.gather(rhs), np.zeros(em.n)
+    em.wave(1, 0.0, [tr, F["bb"], F["bw"], F["bs"], F["bp"]], [z], None)
+    dot = em.wave(2, 0.0, [z, F["bn"], F["be"], F["bt"], tr], [z], None)
+    np.testing.assert_array_equal(em.scatter(z, np.zeros(s.n)), zref)
+    mask = zref != 0
+    assert dot == pytest.approx(np.dot(rhs[mask], zref[mask]), rel=1e-12)
+
+
+@pytest.mark.parametrize("dims,klo,khi", CASES)
+def test_spmv(E, O, dims, klo, khi):
+    s, _ = halo_system(O, dims, 4)
+    ni, nj, nk = dims
+    p = np.random.default_rng(2).normal(size=s.n)
+    inside = np.zeros((nk, ni, nj), bool)
+    inside[klo - 1:khi, 2:ni - 2, 2:nj - 2] = True
+    p[~inside.ravel()] = 0.0
+    zref = O.spmv(s, klo, khi, p)
+    em = Emu(E, dims, klo, khi)
+    A = [em.gather(getattr(s, f)) for f in ("ae", "aw", "an", "as_", "at", "ab", "ap")]
+    tz = np.zeros(em.n)
+    E.emu_spmv(*dims, klo, khi, arr(A), P(em.gather(p)), P(tz))
+    np.testing.assert_array_equal(em.scatter(tz, np.zeros(s.n)), zref)
+
+
+def test_sipsol_sweep_with_res_halo(E, O):
+    """sipsol's sweeps read res on the boundary planes (sipsol.f90:70-71,86-87)."""
+    dims, klo, khi = (20, 37, 23), 3, 21
+    s, fi = halo_system(O, dims, 6)
+    ref, fref = s.copy(), fi.copy()
+    O.sipsol(ref, fref, sor=0.0, nsw=1)
+    em = Emu(E, dims, klo, khi)
+    s2 = s.copy()
+    O.sip_factor(s2, klo, khi, alfa=0.92)
+    # residual of the starting fi in the operand order of sipsol.f90:66-68
+    ni, nj, nk = dims
+    nij = ni * nj
+    inside = np.zeros((nk, ni, nj), bool)
+    inside[klo - 1:khi, 2:ni - 2, 2:nj - 2] = True
+    c = np.flatnonzero(inside.ravel())
+    r = np.zeros(s.n)
+    r[c] = (s2.ae[c] * fi[c + nj] + s2.aw[c] * fi[c - nj] + s2.an[c] * fi[c + 1] +
+            s2.as_[c] * fi[c - 1] + s2.at[c] * fi[c + nij] + s2.ab[c] * fi[c - nij] + s2.su[c] -
+            s2.ap[c] * fi[c])
+    F = {f: em.gather(getattr(s2, f)) for f in ("bb", "bw", "bs", "bp", "bn", "be", "bt")}
+    tres, tfi = em.gather(s.res), em.gather(fi)
+    em.wave(1, 0.0, [em.gather(r), F["bb"], F["bw"], F["bs"], F["bp"]], [tres], None)
+    em.wave(7, 0.0, [tres, F["bn"], F["be"], F["bt"], tfi], [tres], [tfi])
+    np.testing.assert_array_equal(em.scatter(tfi, fi), fref)
+    np.testing.assert_array_equal(em.scatter(tres, s.res), ref.res)

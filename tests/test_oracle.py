@@ -1,0 +1,321 @@
+"""Pins the CPU oracle (the checker of every GPU parity test).
+
+1. Golden vectors from the reference's own routines (sipsol.f90,
+   cgstab_sip.f90, GRAD_GAUSS.f90, bpres.f90 compiled with amdflang):
+   bit-identical outputs, same resor and iteration count.
+2. pcgsip / iccg (whose reference files need MPI and are not built):
+   independent dense-matrix restatements -- M = L U from the SIP factors,
+   M = (D^-1 + A_L) D (D^-1 + A_U) for IC(0) -- drive a textbook PCG whose
+   iterates must match the oracle's; ILU(0)/IC(0) exactness on a chain
+   (one-iteration convergence); SpMV against the dense product.
+3. Closed forms: Green-Gauss is exact for linear fields on uniform grids;
+   bpres extrapolates linear fields exactly.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+B_FIELDS = ("bb", "bs", "bw", "bp", "bn", "be", "bt")
+
+import sys  # noqa: E402
+sys.path.insert(0, GOLD)
+import make_golden as MG  # noqa: E402
+
+
+# --------------------------------------------------------------- golden
+SOLVER_FILES = sorted(glob.glob(os.path.join(GOLD, "sipsol_*.npz")) +
+                      glob.glob(os.path.join(GOLD, "cgstab_*.npz")))
+
+
+@pytest.mark.parametrize("path", SOLVER_FILES, ids=os.path.basename)
+def test_solver_golden(O, path):
+    g = np.load(path)
+    dims = tuple(int(x) for x in g["dims"])
+    s, fi = MG.solver_inputs(str(g["kind"]), dims, int(g["seed"]))
+    fn = O.sipsol if str(g["routine"]) == "sipsol" else O.cgstab_sip
+    rep = fn(s, fi, alfa=float(g["alfa"]), sor=1e-8, nsw=int(g["nsw"]))
+    assert rep["resor"] == float(g["resor"])
+    assert rep["iters"] == int(g["iters"])
+    got = dict(fi=fi, res=s.res, **{f: getattr(s, f) for f in B_FIELDS})
+    for f in MG.SOLVER_OUT:
+        if f in g.files:
+            np.testing.assert_array_equal(got[f], g[f], err_msg=f)
+        assert MG.digest(got[f]) == str(g[f + "_sha256"]), f
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "grad_gauss_*.npz"))),
+                         ids=os.path.basename)
+def test_grad_gauss_golden(O, path):
+    g = np.load(path)
+    dims = tuple(int(x) for x in g["dims"])
+    geom, u = MG.geometry(dims, int(g["seed"]))
+    dx, dy, dz = O.grad_gauss(dims, u, geom)
+    np.testing.assert_array_equal(dx, g["dudx"])
+    np.testing.assert_array_equal(dy, g["dudy"])
+    np.testing.assert_array_equal(dz, g["dudz"])
+
+
+def test_bpres_golden(O):
+    g = np.load(os.path.join(GOLD, "bpres_12x10x9.npz"))
+    dims = tuple(int(x) for x in g["dims"])
+    geom, u = MG.geometry(dims, int(g["seed"]))
+    O.bpres(dims, u, geom)
+    np.testing.assert_array_equal(u, g["p"])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(HERE), "oracle", "_ref",
+                                                    "ref_driver")),
+                    reason="reference build only exists where the reference sources are")
+@pytest.mark.parametrize("routine,kind,seed", [("sipsol", "general", 7), ("cgstab_sip", "general", 8),
+                                               ("cgstab_sip", "lognormal", 9)])
+def test_against_reference_build_fresh(O, routine, kind, seed):
+    """Fresh seeds straight through the reference build (not just the fixtures)."""
+    dims = (14, 21, 11)
+    s, fi = MG.solver_inputs(kind, dims, seed)
+    arrays = [fi, s.ae, s.aw, s.an, s.as_, s.at, s.ab, s.ap, s.su, s.res] + \
+             [getattr(s, f) for f in B_FIELDS]
+    outs, resor, log = O.run_ref(routine, dims, [a.copy() for a in arrays], nsw=40)
+    fn = O.sipsol if routine == "sipsol" else O.cgstab_sip
+    rep = fn(s, fi, sor=1e-8, nsw=40)
+    assert rep["resor"] == resor and rep["iters"] == O.parse_log(log)["iters"]
+    for a, b in zip(outs, [fi, s.res] + [getattr(s, f) for f in B_FIELDS]):
+        np.testing.assert_array_equal(a, b)
+
+
+# ------------------------------------------------------ dense restatements
+def interior_index(dims, klo, khi):
+    ni, nj, nk = dims
+    idx = []
+    for k in range(klo, khi + 1):
+        for i in range(3, ni - 1):
+            for j in range(3, nj - 1):
+                idx.append((k - 1) * ni * nj + (i - 1) * nj + (j - 1))
+    return np.array(idx)
+
+
+def dense_A(s, dims, idx):
+    """The 7-diagonal operator restricted to the solve range (zero halo)."""
+    ni, nj, nk = dims
+    pos = {c: q for q, c in enumerate(idx)}
+    m = len(idx)
+    A = np.zeros((m, m))
+    offs = {"ae": nj, "aw": -nj, "an": 1, "as_": -1, "at": ni * nj, "ab": -ni * nj}
+    for q, c in enumerate(idx):
+        A[q, q] = s.ap[c]
+        for f, o in offs.items():
+            if c + o in pos:
+                A[q, pos[c + o]] = -getattr(s, f)[c]
+    return A
+
+
+def dense_LU(s, dims, idx):
+    ni, nj, nk = dims
+    pos = {c: q for q, c in enumerate(idx)}
+    m = len(idx)
+    Lm, Um = np.zeros((m, m)), np.eye(m)
+    for q, c in enumerate(idx):
+        Lm[q, q] = 1.0 / s.bp[c]
+        for f, o in (("bb", -ni * nj), ("bw", -nj), ("bs", -1)):
+            if c + o in pos:
+                Lm[q, pos[c + o]] = getattr(s, f)[c]
+        for f, o in (("bn", 1), ("be", nj), ("bt", ni * nj)):
+            if c + o in pos:
+                Um[q, pos[c + o]] = getattr(s, f)[c]
+    return Lm, Um
+
+
+def textbook_pcg(A, b, x, Minv, nit):
+    """Preconditioned CG in the reference's scalar form (pcg-sip.f90:89-181)."""
+    r = b - A @ x
+    p = np.zeros_like(b)
+    s0 = float(np.float32(1e20))
+    hist = []
+    for _ in range(nit):
+        z = Minv(r)
+        sk = r @ z
+        p = z + (sk / s0) * p
+        w = A @ p
+        alf = sk / (p @ w)
+        x = x + alf * p
+        r = r - alf * w
+        s0 = sk
+        hist.append(np.sum(np.abs(r)))
+    return x, hist
+
+
+@pytest.mark.parametrize("dims,alfa", [((10, 9, 8), 0.0), ((10, 9, 8), 0.92), ((9, 11, 10), 0.5)])
+def test_pcgsip_against_dense_pcg(O, dims, alfa):
+    s, fi = O.build_general(dims, 41)
+    ref = s.copy()
+    idx = interior_index(dims, 3, dims[2] - 2)
+    nit = 8
+    rep = O.pcgsip(s, fi, alfa=alfa, sor=0.0, nsw=nit)
+    # factors from the oracle's factorisation (pinned by the sipsol golden vectors)
+    O.sip_factor(ref, 3, dims[2] - 2, alfa=alfa)
+    A = dense_A(ref, dims, idx)
+    Lm, Um = dense_LU(ref, dims, idx)
+    _, fi0 = O.build_general(dims, 41)
+    # ghost values of fi enter b through the boundary couplings
+    b_full = O.spmv(ref, 3, dims[2] - 2, np.where(np.isin(np.arange(ref.n), idx), 0.0, fi0))
+    b = ref.su[idx] - b_full[idx]
+    x, hist = textbook_pcg(A, b, fi0[idx], lambda r: np.linalg.solve(Um, np.linalg.solve(Lm, r)), nit)
+    assert rep["resl"] == pytest.approx(hist[-1], rel=1e-9)
+    np.testing.assert_allclose(fi[idx], x, rtol=0, atol=1e-10 * np.max(np.abs(x)))
+
+
+def test_ilu0_reproduces_A_on_pattern(O):
+    """SIP with alfa = 0 is ILU(0): (L U)_ij == A_ij on the 7-point pattern."""
+    dims = (9, 10, 8)
+    s = O.build_lognormal(dims, 5, 1.0)
+    O.sip_factor(s, 3, dims[2] - 2, alfa=0.0)
+    idx = interior_index(dims, 3, dims[2] - 2)
+    A = dense_A(s, dims, idx)
+    Lm, Um = dense_LU(s, dims, idx)
+    M = Lm @ Um
+    pattern = A != 0
+    np.fill_diagonal(pattern, True)
+    np.testing.assert_allclose(M[pattern], A[pattern], rtol=1e-12, atol=1e-15)
+
+
+def chain_system(O, dims, direction):
+    """Only the couplings along one direction: the matrix is tridiagonal in the
+    cell ordering, so ILU(0) / IC(0) are exact and PCG converges in one step."""
+    s = O.build_lognormal(dims, 17, 0.7)
+    keep = {"i": ("ae", "aw"), "j": ("an", "as_"), "k": ("at", "ab")}[direction]
+    for f in ("ae", "aw", "an", "as_", "at", "ab"):
+        if f not in keep:
+            getattr(s, f)[:] = 0.0
+    # the pinned reference cell (3,3,3) keeps its neighbours' couplings to it
+    # (non-symmetric row); decouple it fully so the chain matrix is symmetric
+    ni, nj, nk = dims
+    r = 2 * ni * nj + 2 * nj + 2
+    for fp, fm, off in (("ae", "aw", nj), ("an", "as_", 1), ("at", "ab", ni * nj)):
+        getattr(s, fm)[r + off] = 0.0
+        getattr(s, fp)[r - off] = 0.0
+    # diagonally dominant, non-singular
+    idx = interior_index(dims, 2, nk - 1)
+    s.ap[idx] = s.ae[idx] + s.aw[idx] + s.an[idx] + s.as_[idx] + s.at[idx] + s.ab[idx] + 1e-3
+    return s
+
+
+@pytest.mark.parametrize("direction", ["i", "j", "k"])
+def test_exact_factorisations_on_chains(O, direction):
+    dims = (9, 8, 10)
+    s = chain_system(O, dims, direction)
+    rep = O.pcgsip(s.copy(), np.zeros(s.n), alfa=0.0, sor=1e-8, nsw=5)
+    assert rep["iters"] == 1 and rep["converged"]
+    rep = O.iccg(s.copy(), np.zeros(s.n), sor=1e-8, nsw=5)
+    assert rep["iters"] == 1 and rep["converged"]
+
+
+@pytest.mark.parametrize("dims", [(9, 8, 9), (10, 9, 8)])
+def test_iccg_against_dense_ic(O, dims):
+    s = O.build_lognormal(dims, 23, 1.0)
+    ref = s.copy()
+    nit = 6
+    fi = np.zeros(s.n)
+    rep = O.iccg(s, fi, sor=0.0, nsw=nit)
+    idx = interior_index(dims, 2, dims[2] - 1)
+    A = dense_A(ref, dims, idx)
+    m = len(idx)
+    # IC(0) diagonal by its defining recursion on the dense lower part
+    d = np.zeros(m)
+    AL = np.tril(A, -1)
+    for q in range(m):
+        d[q] = 1.0 / (A[q, q] - np.sum(AL[q, :q] ** 2 * d[:q]))
+    Dinv = np.diag(1.0 / d)
+    AU = np.triu(A, 1)   # the row's own upper couplings (A is not symmetric at the pinned cell)
+    M = (Dinv + AL) @ np.diag(d) @ (Dinv + AU)
+    _, hist = textbook_pcg(A, ref.su[idx], np.zeros(m), lambda r: np.linalg.solve(M, r), nit)
+    assert rep["resl"] == pytest.approx(hist[-1], rel=1e-9)
+
+
+def test_spmv_dense(O):
+    dims = (10, 11, 9)
+    s, _ = O.build_general(dims, 3)
+    idx = interior_index(dims, 3, dims[2] - 2)
+    p = np.zeros(s.n)
+    p[idx] = np.random.default_rng(0).normal(size=len(idx))
+    z = O.spmv(s, 3, dims[2] - 2, p)
+    np.testing.assert_allclose(z[idx], dense_A(s, dims, idx) @ p[idx], rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------ closed forms
+def uniform_geometry(dims, h):
+    ni, nj, nk = dims
+    n = ni * nj * nk
+    g = {f: np.full(n, 0.5) for f in ("fx", "fy", "fz")}
+    for f in ("ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z"):
+        g[f] = np.zeros(n)
+    g["ar1x"][:] = h * h
+    g["ar2y"][:] = h * h
+    g["ar3z"][:] = h * h
+    g["vol"] = np.full(n, h ** 3)
+    return g
+
+
+def test_grad_gauss_linear_field_exact(O):
+    dims = (10, 9, 11)
+    ni, nj, nk = dims
+    h = 0.1
+    g = uniform_geometry(dims, h)
+    k, i, j = np.meshgrid(np.arange(nk), np.arange(ni), np.arange(nj), indexing="ij")
+    # reference axes: i -> x (ar1), j -> y (ar2), k -> z (ar3)
+    u = (2.0 * i * h - 3.0 * j * h + 0.5 * k * h).ravel()
+    dx, dy, dz = O.grad_gauss(dims, u, g)
+    inner = np.zeros(dims[::-1], bool).transpose(0, 1, 2)
+    inner = np.zeros((nk, ni, nj), bool)
+    inner[2:nk - 2, 2:ni - 2, 2:nj - 2] = True
+    inner = inner.ravel()
+    np.testing.assert_allclose(dx[inner], 2.0, rtol=1e-12)
+    np.testing.assert_allclose(dy[inner], -3.0, rtol=1e-12)
+    np.testing.assert_allclose(dz[inner], 0.5, rtol=1e-12)
+
+
+def test_bpres_linear_extrapolation(O):
+    """bpres puts the boundary-face value of a linear field into the ghost
+    cell: with interpolation factor f = 1/2 that is the field half a cell
+    beyond the first/last interior cell (bpres.f90:33-79)."""
+    dims = (9, 10, 8)
+    ni, nj, nk = dims
+    g = uniform_geometry(dims, 1.0)
+    cx, cy, cz = 1.5, -0.25, 2.0
+    k, i, j = np.meshgrid(np.arange(nk), np.arange(ni), np.arange(nj), indexing="ij")
+    p = (cx * i + cy * j + cz * k).ravel().astype(float)
+    exact = p.copy().reshape(nk, ni, nj)
+    exact[1, 2:ni - 2, 2:nj - 2] += 0.5 * cz
+    exact[nk - 2, 2:ni - 2, 2:nj - 2] -= 0.5 * cz
+    exact[2:nk - 2, 1, 2:nj - 2] += 0.5 * cx
+    exact[2:nk - 2, ni - 2, 2:nj - 2] -= 0.5 * cx
+    exact[2:nk - 2, 2:ni - 2, 1] += 0.5 * cy
+    exact[2:nk - 2, 2:ni - 2, nj - 2] -= 0.5 * cy
+    exact = exact.ravel()
+    ghost = np.zeros((nk, ni, nj), bool)
+    ghost[1, 2:ni - 2, 2:nj - 2] = ghost[nk - 2, 2:ni - 2, 2:nj - 2] = True
+    ghost[2:nk - 2, 1, 2:nj - 2] = ghost[2:nk - 2, ni - 2, 2:nj - 2] = True
+    ghost[2:nk - 2, 2:ni - 2, 1] = ghost[2:nk - 2, 2:ni - 2, nj - 2] = True
+    ghost = ghost.ravel()
+    p[ghost] = 1e9
+    O.bpres(dims, p, g)
+    np.testing.assert_allclose(p[ghost], exact[ghost], rtol=1e-13)
+    np.testing.assert_array_equal(p[~ghost], exact[~ghost])
+
+
+def test_generator_matches_published_transform(O):
+    """The counter-based generator: splitmix64 finaliser on (seed, stream, idx)."""
+    M = (1 << 64) - 1
+
+    def smix(x):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M
+        return x ^ (x >> 31)
+
+    for seed, stream, idx in ((1, 2, 3), (2024, 4, 123456), (0, 0, 0)):
+        h = smix((smix(seed ^ ((stream * 0xD1B54A32D192ED03) & M)) + idx) & M)
+        assert O.lib().orc_hash(seed, stream, idx) == h
+        assert O.rng_u01(seed, stream, idx) == (h >> 11) * 2.0 ** -53
