@@ -1,0 +1,145 @@
+!-----------------------------------------------------------------------
+! cap_hip.f90 -- ISO_C_BINDING interface to the MI355X p' library
+! (include/cap_amd.h).  Self-contained: needs only iso_c_binding.
+!
+! The C structs cap_coef (16 pointers) and cap_report map to the bind(c)
+! derived types below; every entry point returns an integer status.
+!-----------------------------------------------------------------------
+module cap_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: CAP_OK = 0, CAP_MAXITER = 1, CAP_NAN = 2, &
+                               CAP_GUESS_REJECTED = 3
+  integer(c_int), parameter :: CAP_ERR_ARG = -1, CAP_ERR_HIP = -2, CAP_ERR_TIMEOUT = -3, &
+                               CAP_ERR_NOMEM = -4, CAP_ERR_NODEV = -5
+  integer(c_int), parameter :: CAP_DEVICE_POINTERS = 1
+
+  ! module `coef` + `coefb` arrays (modules_allocatable.f90:161-180)
+  type, bind(c) :: cap_coef
+    type(c_ptr) :: ae = c_null_ptr, aw = c_null_ptr, an = c_null_ptr, as = c_null_ptr
+    type(c_ptr) :: at = c_null_ptr, ab = c_null_ptr, ap = c_null_ptr, su = c_null_ptr
+    type(c_ptr) :: res = c_null_ptr
+    type(c_ptr) :: bb = c_null_ptr, bs = c_null_ptr, bw = c_null_ptr, bp = c_null_ptr
+    type(c_ptr) :: bn = c_null_ptr, be = c_null_ptr, bt = c_null_ptr
+  end type cap_coef
+
+  type, bind(c) :: cap_report
+    real(c_double) :: resor, resl
+    integer(c_int) :: iters, status
+    real(c_double) :: ms
+  end type cap_report
+
+  interface
+    integer(c_int) function cap_init(ctx, ni, nj, nk, flags) bind(c, name='cap_init')
+      import :: c_ptr, c_int
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int), value :: ni, nj, nk, flags
+    end function
+    integer(c_int) function cap_finalize(ctx) bind(c, name='cap_finalize')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function cap_sipsol(ctx, fi, c, alfa, sor, nsw, rep) bind(c, name='cap_sipsol')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, fi
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: alfa, sor
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+    end function
+    integer(c_int) function cap_pcgsip(ctx, fi, c, alfa, sor, nsw, rep) bind(c, name='cap_pcgsip')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, fi
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: alfa, sor
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+    end function
+    integer(c_int) function cap_iccg(ctx, fi, c, sor, nsw, rep) bind(c, name='cap_iccg')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, fi
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: sor
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+    end function
+    integer(c_int) function cap_cgstab_sip(ctx, fi, c, alfa, sor, nsw, rep) &
+        bind(c, name='cap_cgstab_sip')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, fi
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: alfa, sor
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+    end function
+    integer(c_int) function cap_pcgsip_mixed(ctx, fi, c, alfa, sor, nsw, inner_rtol, rep) &
+        bind(c, name='cap_pcgsip_mixed')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, fi
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: alfa, sor, inner_rtol
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+    end function
+    integer(c_int) function cap_grad_gauss(ctx, u, geom, dudx, dudy, dudz) &
+        bind(c, name='cap_grad_gauss')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, u, dudx, dudy, dudz
+      type(c_ptr), intent(in) :: geom(13)
+    end function
+    integer(c_int) function cap_bpres(ctx, p, fx, fy, fz) bind(c, name='cap_bpres')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, p, fx, fy, fz
+    end function
+  end interface
+
+  ! one context per process, created on first use for the current extents
+  type(c_ptr), save :: cap_ctx_handle = c_null_ptr
+  integer, save :: cap_ctx_dims(3) = 0
+
+contains
+
+  ! the library context for arrays of extents ni x nj x nk (host pointers)
+  function cap_context(ni, nj, nk) result(ctx)
+    integer, intent(in) :: ni, nj, nk
+    type(c_ptr) :: ctx
+    integer(c_int) :: rc
+    if (c_associated(cap_ctx_handle)) then
+      if (all(cap_ctx_dims == [ni, nj, nk])) then
+        ctx = cap_ctx_handle
+        return
+      end if
+      rc = cap_finalize(cap_ctx_handle)
+      cap_ctx_handle = c_null_ptr
+    end if
+    rc = cap_init(cap_ctx_handle, int(ni, c_int), int(nj, c_int), int(nk, c_int), 0_c_int)
+    if (rc /= CAP_OK) then
+      write(*, '(a,i0)') 'cap_hip: cap_init failed with status ', rc
+      stop 3
+    end if
+    cap_ctx_dims = [ni, nj, nk]
+    ctx = cap_ctx_handle
+  end function cap_context
+
+  ! Pack the module arrays of the caller (explicit-shape dummies, so module
+  ! allocatables without the TARGET attribute can be passed) into cap_coef.
+  subroutine cap_pack(c, ae, aw, an, as, at, ab, ap, su, res, bb, bs, bw, bp, bn, be, bt)
+    type(cap_coef), intent(out) :: c
+    real(c_double), target, intent(inout) :: ae(*), aw(*), an(*), as(*), at(*), ab(*), ap(*), &
+                                             su(*), res(*), bb(*), bs(*), bw(*), bp(*), &
+                                             bn(*), be(*), bt(*)
+    c%ae = c_loc(ae); c%aw = c_loc(aw); c%an = c_loc(an); c%as = c_loc(as)
+    c%at = c_loc(at); c%ab = c_loc(ab); c%ap = c_loc(ap); c%su = c_loc(su)
+    c%res = c_loc(res)
+    c%bb = c_loc(bb); c%bs = c_loc(bs); c%bw = c_loc(bw); c%bp = c_loc(bp)
+    c%bn = c_loc(bn); c%be = c_loc(be); c%bt = c_loc(bt)
+  end subroutine cap_pack
+
+  function cap_addr(x) result(p)
+    real(c_double), target, intent(inout) :: x(*)
+    type(c_ptr) :: p
+    p = c_loc(x)
+  end function cap_addr
+
+end module cap_hip

@@ -22,6 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+DROPIN_DRIVER = os.path.join(HERE, "_ref", "dropin_driver")
 
 SYS_FIELDS = ("ae", "aw", "an", "as_", "at", "ab", "ap", "su", "res",
               "bb", "bs", "bw", "bp", "bn", "be", "bt")
@@ -200,8 +201,10 @@ def ref_available():
 _LOG_RE = re.compile(r"Final residual =\s*([-+0-9.Ee]+), No Iterations (\d+)")
 
 
-def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8):
-    """Run the reference-compiled driver; returns (outputs, resor, log)."""
+def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8, driver=REF_DRIVER):
+    """Run the reference-compiled driver (or, with driver=DROPIN_DRIVER, the
+    host program linked with the Fortran drop-ins); returns
+    (outputs, resor, log)."""
     ni, nj, nk = dims
     n = ni * nj * nk
     with tempfile.TemporaryDirectory() as td:
@@ -211,11 +214,12 @@ def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8):
             np.array([alfa, sor], dtype=np.float64).tofile(f)
             for a in arrays:
                 np.ascontiguousarray(a, dtype=np.float64).tofile(f)
-        subprocess.run([REF_DRIVER, mode, fin, fout, flog], check=True,
+        subprocess.run([driver, mode, fin, fout, flog], check=True,
                        capture_output=True)
         raw = np.fromfile(fout, dtype=np.float64)
         log = open(flog).read()
-    nout = {"sipsol": 9, "cgstab_sip": 9, "grad_gauss": 3, "bpres": 1}[mode]
+    nout = {"sipsol": 9, "cgstab_sip": 9, "pcgsip": 9, "iccg": 9, "grad_gauss": 3,
+            "bpres": 1}[mode]
     outs = [raw[i * n:(i + 1) * n].copy() for i in range(nout)]
     resor = float(raw[nout * n]) if raw.size > nout * n else None
     return outs, resor, log
