@@ -30,6 +30,8 @@ enum {
     F_COUNT
 };
 
+#define CAP_NSTAGE 48
+
 struct cap_ctx {
     int ni, nj, nk;
     size_t nstd;
@@ -41,7 +43,8 @@ struct cap_ctx {
     double *S[S_COUNT];
     float *F[F_COUNT];
     size_t fcap;
-    double *G[20];             // stencil staging (reference layout)
+    double *G[CAP_NSTAGE];      // stencil staging (reference layout)
+    size_t Gsz[CAP_NSTAGE];
     uint64_t *progress;
     int ntiles_cap;
     unsigned long long *claim;

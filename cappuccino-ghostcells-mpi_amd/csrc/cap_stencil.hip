@@ -155,12 +155,301 @@ __global__ __launch_bounds__(256) void k_bpres(int ni, int nj, int nk, double *_
     }
 }
 
-static int stage(cap_ctx *ctx, int slot, const double *p, const double **out)
+
+// ------------------------------------------------------ p' assembly
+// One thread per face of the reference's face loop, j (unit stride) fastest.
+// Each formula keeps the reference's operand order (contraction is off).
+struct Fld {
+    const double *x, *y, *z, *xc, *yc, *zc, *vol, *fx, *fy, *fz;
+    const double *ar1x, *ar1y, *ar1z, *ar2x, *ar2y, *ar2z, *ar3x, *ar3y, *ar3z;
+    const double *den, *apu, *apv, *apw, *u, *v, *w, *p;
+    const double *gu, *gv, *gw, *gp;   // 3 x n, component fastest
+};
+struct FaceDir {
+    int idew, idns, idtb;
+    const double *arvx, *arvy, *arvz, *fif;
+};
+
+// face-loop decode: k in [k0, k1], i in [2, nimm], j in [2, njmm]
+__device__ __forceinline__ bool face_of(int ni, int nj, int k0, int k1, size_t q, size_t &p)
+{
+    const int nimm = ni - 2, njmm = nj - 2;
+    const int NJ = njmm - 1, NI = nimm - 1;
+    const size_t per = (size_t)NI * NJ;
+    const int k = k0 + (int)(q / per);
+    if (k > k1) return false;
+    const int r = (int)(q % per);
+    const int i = 2 + r / NJ, j = 2 + r % NJ;
+    p = (size_t)(k - 1) * ni * nj + (size_t)(i - 1) * nj + (size_t)(j - 1);
+    return true;
+}
+
+#define G3(g, c, q) ((g)[3 * (c) + (q)])
+
+__global__ __launch_bounds__(256) void k_zero_st(double *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        p[i] = 0.0;
+}
+
+// fluxmass2.f90:41-192
+__global__ __launch_bounds__(256) void k_fluxmass(int ni, int nj, int nk, Fld f, FaceDir D,
+                                                  double *__restrict__ acfe,
+                                                  double *__restrict__ acfw,
+                                                  double *__restrict__ fcf)
+{
+    const size_t nf = (size_t)(nk - 2) * (ni - 3) * (nj - 3);
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nf; q += (size_t)gridDim.x * 256) {
+        size_t p;
+        if (!face_of(ni, nj, 2, nk - 1, q, p)) continue;
+        const size_t e = p + D.idew, s = p - D.idns, b = p - D.idtb, bs = b - D.idns;
+        const double fxe = D.fif[p], fxp = 1.0 - fxe;
+        const double xf = 0.25 * (f.x[p] + f.x[s] + f.x[b] + f.x[bs]);
+        const double yf = 0.25 * (f.y[p] + f.y[s] + f.y[b] + f.y[bs]);
+        const double zf = 0.25 * (f.z[p] + f.z[s] + f.z[b] + f.z[bs]);
+        const double xi = f.xc[p] * fxp + f.xc[e] * fxe;
+        const double yi = f.yc[p] * fxp + f.yc[e] * fxe;
+        const double zi = f.zc[p] * fxp + f.zc[e] * fxe;
+        const double xpn = f.xc[e] - f.xc[p], ypn = f.yc[e] - f.yc[p], zpn = f.zc[e] - f.zc[p];
+        const double arx = D.arvx[p], ary = D.arvy[p], arz = D.arvz[p];
+        const double are = sqrt(arx * arx + ary * ary + arz * arz);
+        const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+        const double sfdpnr = 1.0 / (arx * xpn * nxx + ary * ypn * nyy + arz * zpn * nzz);
+        const double dene = f.den[p] * fxp + f.den[e] * fxe;
+        const double smdpn = (arx * arx + ary * ary + arz * arz) * sfdpnr;
+        const double a = dene * (fxp * f.vol[p] * f.apu[p] + fxe * f.vol[e] * f.apu[e]) * smdpn;
+        acfe[p] = a;
+        acfw[e] = a;
+        double duxi = G3(f.gu, p, 0) * fxp + G3(f.gu, e, 0) * fxe;
+        double duyi = G3(f.gu, p, 1) * fxp + G3(f.gu, e, 1) * fxe;
+        double duzi = G3(f.gu, p, 2) * fxp + G3(f.gu, e, 2) * fxe;
+        const double ui = f.u[p] * fxp + f.u[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+        duxi = G3(f.gv, p, 0) * fxp + G3(f.gv, e, 0) * fxe;
+        duyi = G3(f.gv, p, 1) * fxp + G3(f.gv, e, 1) * fxe;
+        duzi = G3(f.gv, p, 2) * fxp + G3(f.gv, e, 2) * fxe;
+        const double vi = f.v[p] * fxp + f.v[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+        duxi = G3(f.gw, p, 0) * fxp + G3(f.gw, e, 0) * fxe;
+        duyi = G3(f.gw, p, 1) * fxp + G3(f.gw, e, 1) * fxe;
+        duzi = G3(f.gw, p, 2) * fxp + G3(f.gw, e, 2) * fxe;
+        const double wi = f.w[p] * fxp + f.w[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+        const double cu = fxe * f.vol[e] * f.apu[e] + fxp * f.vol[p] * f.apu[p];
+        const double cv = fxe * f.vol[e] * f.apv[e] + fxp * f.vol[p] * f.apv[p];
+        const double cw = fxe * f.vol[e] * f.apw[e] + fxp * f.vol[p] * f.apw[p];
+        const double dpxi = cu * (fxe * G3(f.gp, e, 0) + fxp * G3(f.gp, p, 0)) * xpn * nxx;
+        const double dpyi = cv * (fxe * G3(f.gp, e, 1) + fxp * G3(f.gp, p, 1)) * ypn * nyy;
+        const double dpzi = cw * (fxe * G3(f.gp, e, 2) + fxp * G3(f.gp, p, 2)) * zpn * nzz;
+        double xpp = xf - (xf - f.xc[p]) * nxx, ypp = yf - (yf - f.yc[p]) * nyy, zpp = zf - (zf - f.zc[p]) * nzz;
+        double xep = xf - (xf - f.xc[e]) * nxx, yep = yf - (yf - f.yc[e]) * nyy, zep = zf - (zf - f.zc[e]) * nzz;
+        xpp = xpp - f.xc[p]; ypp = ypp - f.yc[p]; zpp = zpp - f.zc[p];
+        xep = xep - f.xc[e]; yep = yep - f.yc[e]; zep = zep - f.zc[e];
+        // the reference's correction term, signs as written (fluxmass2.f90:171-173)
+        const double dpe = (f.p[e] - f.p[p]) +
+                           (G3(f.gp, e, 0) * xep + G3(f.gp, e, 1) * yep + G3(f.gp, e, 2) * zep -
+                            G3(f.gp, p, 0) * xpp + G3(f.gp, p, 1) * ypp + G3(f.gp, p, 2) * zpp);
+        const double dpex = cu * (arx * sfdpnr) * dpe;
+        const double dpey = cv * (ary * sfdpnr) * dpe;
+        const double dpez = cw * (arz * sfdpnr) * dpe;
+        const double ue = ui - dpex + dpxi, ve = vi - dpey + dpyi, we = wi - dpez + dpzi;
+        fcf[p] = dene * (ue * arx + ve * ary + we * arz);
+    }
+}
+
+// PISO_assemble_pressure_eq.f90:48-239 (face loop k=2..nkmm); bp zeroed before
+__global__ __launch_bounds__(256) void k_piso_assemble(int ni, int nj, int nk, Fld f, FaceDir D,
+                                                       int ipc, double *__restrict__ acfe,
+                                                       double *__restrict__ acfw,
+                                                       double *__restrict__ fcf,
+                                                       double *__restrict__ bp,
+                                                       double *__restrict__ su)
+{
+    const size_t nf = (size_t)(nk - 3) * (ni - 3) * (nj - 3);
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nf; q += (size_t)gridDim.x * 256) {
+        size_t p;
+        if (!face_of(ni, nj, 2, nk - 2, q, p)) continue;
+        const size_t e = p + D.idew, s = p - D.idns, b = p - D.idtb, bs = b - D.idns;
+        const double fxe = D.fif[p], fxp = 1.0 - fxe;
+        const double xf = 0.25 * (f.x[p] + f.x[s] + f.x[b] + f.x[bs]);
+        const double yf = 0.25 * (f.y[p] + f.y[s] + f.y[b] + f.y[bs]);
+        const double zf = 0.25 * (f.z[p] + f.z[s] + f.z[b] + f.z[bs]);
+        const double xi = f.xc[p] * fxp + f.xc[e] * fxe;
+        const double yi = f.yc[p] * fxp + f.yc[e] * fxe;
+        const double zi = f.zc[p] * fxp + f.zc[e] * fxe;
+        const double xpn = f.xc[e] - f.xc[p], ypn = f.yc[e] - f.yc[p], zpn = f.zc[e] - f.zc[p];
+        const double dpn = sqrt(xpn * xpn + ypn * ypn + zpn * zpn);
+        const double ixi1 = xpn / dpn, ixi2 = ypn / dpn, ixi3 = zpn / dpn;
+        const double arx = D.arvx[p], ary = D.arvy[p], arz = D.arvz[p];
+        const double are = sqrt(arx * arx + ary * ary + arz * arz);
+        const double vole = xpn * arx + ypn * ary + zpn * arz;
+        const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+        const double costn = nxx * ixi1 + nyy * ixi2 + nzz * ixi3;
+        const double d2x = xpn * costn, d2y = ypn * costn, d2z = zpn * costn;
+        const double dene = f.den[p] * fxp + f.den[e] * fxe;
+        const double game = dene * (fxp * f.vol[p] * f.apu[p] + fxe * f.vol[e] * f.apu[e]);
+        const double a = game * are / dpn;
+        acfe[p] = a;
+        acfw[e] = a;
+        if (ipc > 0) {
+            const double dfixi = G3(f.gp, p, 0) * fxp + G3(f.gp, e, 0) * fxe;
+            const double dfiyi = G3(f.gp, p, 1) * fxp + G3(f.gp, e, 1) * fxe;
+            const double dfizi = G3(f.gp, p, 2) * fxp + G3(f.gp, e, 2) * fxe;
+            const double dfixii = dfixi * costn + arx / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+            const double dfiyii = dfiyi * costn + ary / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+            const double dfizii = dfizi * costn + arz / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+            const double fdfie = game * (dfixii * arx + dfiyii * ary + dfizii * arz);
+            const double fdfii = game * are / dpn * (dfixi * xpn + dfiyi * ypn + dfizi * zpn);
+            const double suadd = fdfie - fdfii;
+            su[p] = su[p] + suadd;
+            bp[p] = suadd;
+            double duxi = G3(f.gu, p, 0) * fxp + G3(f.gu, e, 0) * fxe;
+            double duyi = G3(f.gu, p, 1) * fxp + G3(f.gu, e, 1) * fxe;
+            double duzi = G3(f.gu, p, 2) * fxp + G3(f.gu, e, 2) * fxe;
+            const double ui = f.u[p] * fxp + f.u[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
+            duxi = G3(f.gv, p, 0) * fxp + G3(f.gv, e, 0) * fxe;
+            duyi = G3(f.gv, p, 1) * fxp + G3(f.gv, e, 1) * fxe;
+            duzi = G3(f.gv, p, 2) * fxp + G3(f.gv, e, 2) * fxe;
+            const double vi = f.v[p] * fxp + f.v[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
+            duxi = G3(f.gw, p, 0) * fxp + G3(f.gw, e, 0) * fxe;
+            duyi = G3(f.gw, p, 1) * fxp + G3(f.gw, e, 1) * fxe;
+            duzi = G3(f.gw, p, 2) * fxp + G3(f.gw, e, 2) * fxe;
+            const double wi = f.w[p] * fxp + f.w[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
+            fcf[p] = dene * (ui * arx + vi * ary + wi * arz);
+        }
+    }
+}
+
+// PISO_assemble_pressure_eq.f90:241-253: su(c) = su(c) - bp(c), c = cell + idew
+__global__ __launch_bounds__(256) void k_piso_su(int ni, int nj, int nk, int idew,
+                                                 const double *__restrict__ bp,
+                                                 double *__restrict__ su)
+{
+    const int NI = ni - 4, NJ = nj - 4, NK = nk - 4;
+    const size_t n = (size_t)NI * NJ * NK;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const int k = 3 + (int)(q / ((size_t)NI * NJ));
+        const int r = (int)(q % ((size_t)NI * NJ));
+        const int i = 3 + r / NJ, j = 3 + r % NJ;
+        const size_t c = (size_t)(k - 1) * ni * nj + (size_t)(i - 1) * nj + (size_t)(j - 1) + idew;
+        su[c] = su[c] - bp[c];
+    }
+}
+
+// assemble_pressure_eq.f90:38-118
+__global__ __launch_bounds__(256) void k_assemble_p(int ni, int nj, int nk, Fld f, FaceDir D,
+                                                    const double *__restrict__ su,
+                                                    const double *__restrict__ sv,
+                                                    const double *__restrict__ sw,
+                                                    double *__restrict__ acfe,
+                                                    double *__restrict__ acfw,
+                                                    double *__restrict__ fcf)
+{
+    const size_t nf = (size_t)(nk - 3) * (ni - 3) * (nj - 3);
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nf; q += (size_t)gridDim.x * 256) {
+        size_t p;
+        if (!face_of(ni, nj, 2, nk - 2, q, p)) continue;
+        const size_t e = p + D.idew;
+        const double fxe = D.fif[p], fxp = 1.0 - fxe;
+        const double xpn = f.xc[e] - f.xc[p], ypn = f.yc[e] - f.yc[p], zpn = f.zc[e] - f.zc[p];
+        const double arx = D.arvx[p], ary = D.arvy[p], arz = D.arvz[p];
+        const double are = sqrt(arx * arx + ary * ary + arz * arz);
+        const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+        const double smdpn = (arx * arx + ary * ary + arz * arz) / (arx * xpn * nxx + ary * ypn * nyy + arz * zpn * nzz);
+        const double a = (fxp * f.vol[p] + fxe * f.vol[e]) * smdpn;
+        acfe[p] = a;
+        acfw[e] = a;
+        const double ui = su[p] * fxp + su[e] * fxe;
+        const double vi = sv[p] * fxp + sv[e] * fxe;
+        const double wi = sw[p] * fxp + sw[e] * fxe;
+        fcf[p] = ui * arx + vi * ary + wi * arz;
+    }
+}
+
+// fluxmc.f90:41-106 for every cell of k,i,j = 3..n?mm (others 0)
+__global__ __launch_bounds__(256) void k_fluxmc(int ni, int nj, int nk, Fld f, FaceDir D,
+                                                double *__restrict__ fmcor)
+{
+    const size_t n = (size_t)ni * nj * nk;
+    const int nij = ni * nj;
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
+        const int k = (int)(c / nij) + 1, rem = (int)(c % nij);
+        const int i = rem / nj + 1, j = rem % nj + 1;
+        if (k < 3 || k > nk - 2 || i < 3 || i > ni - 2 || j < 3 || j > nj - 2) {
+            fmcor[c] = 0.0;
+            continue;
+        }
+        const size_t p = c, e = p + D.idew, s = p - D.idns, b = p - D.idtb, bs = b - D.idns;
+        const double fxe = D.fif[p], fxp = 1. - fxe;
+        double arx, ary, arz;
+        if (D.idew == nj) { arx = f.ar1x[p]; ary = f.ar1y[p]; arz = f.ar1z[p]; }
+        else if (D.idew == 1) { arx = f.ar2x[p]; ary = f.ar2y[p]; arz = f.ar2z[p]; }
+        else { arx = f.ar3x[p]; ary = f.ar3y[p]; arz = f.ar3z[p]; }
+        const double xpn = f.xc[e] - f.xc[p], ypn = f.yc[e] - f.yc[p], zpn = f.zc[e] - f.zc[p];
+        const double are = sqrt(arx * arx + ary * ary + arz * arz);
+        const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+        const double dppnnr = 1. / ((xpn * nxx) + (ypn * nyy) + (zpn * nzz));
+        const double xf = 0.25 * (f.x[p] + f.x[s] + f.x[b] + f.x[bs]);
+        const double yf = 0.25 * (f.y[p] + f.y[s] + f.y[b] + f.y[bs]);
+        const double zf = 0.25 * (f.z[p] + f.z[s] + f.z[b] + f.z[bs]);
+        double xpp = xf - (xf - f.xc[p]) * nxx, ypp = yf - (yf - f.yc[p]) * nyy, zpp = zf - (zf - f.zc[p]) * nzz;
+        double xep = xf - (xf - f.xc[e]) * nxx, yep = yf - (yf - f.yc[e]) * nyy, zep = zf - (zf - f.zc[e]) * nzz;
+        xpp = xpp - f.xc[p]; ypp = ypp - f.yc[p]; zpp = zpp - f.zc[p];
+        xep = xep - f.xc[e]; yep = yep - f.yc[e]; zep = zep - f.zc[e];
+        const double rapr = (f.apu[p] * f.den[p] * f.vol[p] * fxp + f.apu[e] * f.den[e] * f.vol[e] * fxe);
+        fmcor[c] = rapr * are * ((G3(f.gp, e, 0) * xep - G3(f.gp, p, 0) * xpp) +
+                                 (G3(f.gp, e, 1) * yep - G3(f.gp, p, 1) * ypp) +
+                                 (G3(f.gp, e, 2) * zep - G3(f.gp, p, 2) * zpp)) * dppnnr;
+    }
+}
+
+// continuityErrors.h:63-77: global and sum|local| over k=2..nkm, i,j=3..n?mm
+__global__ __launch_bounds__(256) void k_continuity(int ni, int nj, int nk,
+                                                    const double *__restrict__ f1,
+                                                    const double *__restrict__ f2,
+                                                    const double *__restrict__ f3, DevState *st,
+                                                    double *partials, unsigned *ticket)
+{
+    const int NI = ni - 4, NJ = nj - 4, NK = nk - 2;
+    const size_t n = (size_t)NI * NJ * NK;
+    const int nij = ni * nj;
+    double acc[2] = {0.0, 0.0};
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const int k = 2 + (int)(q / ((size_t)NI * NJ));
+        const int r = (int)(q % ((size_t)NI * NJ));
+        const int i = 3 + r / NJ, j = 3 + r % NJ;
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        const double loc = -f1[c] + f1[c - nj] - f2[c] + f2[c - 1] - f3[c] + f3[c - nij];
+        acc[0] += loc;
+        acc[1] += fabs(loc);
+    }
+    grid_reduce<2, F_RAW>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
+}
+
+// device copy of a caller array in host-pointer mode (slot-cached buffers of
+// `mult` x nxyz doubles); the pointer itself in device-pointer mode
+static int stage_n(cap_ctx *ctx, int slot, const double *p, int mult, const double **out)
 {
     if (ctx->flags & CAP_DEVICE_POINTERS) { *out = p; return CAP_OK; }
-    if (!ctx->G[slot]) HIPCHK(hipMalloc(&ctx->G[slot], ctx->nstd * sizeof(double)));
-    if (p) HIPCHK(hipMemcpyAsync(ctx->G[slot], p, ctx->nstd * 8, hipMemcpyHostToDevice, ctx->stream));
+    if (slot < 0 || slot >= CAP_NSTAGE) return CAP_ERR_ARG;
+    const size_t bytes = ctx->nstd * sizeof(double) * (size_t)mult;
+    if (ctx->Gsz[slot] < bytes) {
+        if (ctx->G[slot]) (void)hipFree(ctx->G[slot]);
+        ctx->G[slot] = nullptr;
+        HIPCHK(hipMalloc(&ctx->G[slot], bytes));
+        ctx->Gsz[slot] = bytes;
+    }
+    if (p) HIPCHK(hipMemcpyAsync(ctx->G[slot], p, bytes, hipMemcpyHostToDevice, ctx->stream));
     *out = ctx->G[slot];
+    return CAP_OK;
+}
+
+static int stage(cap_ctx *ctx, int slot, const double *p, const double **out)
+{
+    return stage_n(ctx, slot, p, 1, out);
+}
+
+static int unstage(cap_ctx *ctx, double *host, const double *dev, int mult = 1)
+{
+    if (!host || (ctx->flags & CAP_DEVICE_POINTERS)) return CAP_OK;
+    HIPCHK(hipMemcpyAsync(host, dev, ctx->nstd * 8 * (size_t)mult, hipMemcpyDeviceToHost, ctx->stream));
     return CAP_OK;
 }
 
@@ -216,6 +505,188 @@ extern "C" int cap_bpres(cap_ctx *ctx, double *p, const double *fx, const double
     if (!(ctx->flags & CAP_DEVICE_POINTERS))
         HIPCHK(hipMemcpyAsync(p, dp, ctx->nstd * 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+
+// ------------------------------------------------ assembly entry points
+static int stage_fields(cap_ctx *ctx, const cap_fields *F, Fld &f)
+{
+    const double *src[31] = {F->x, F->y, F->z, F->xc, F->yc, F->zc, F->vol, F->fx, F->fy, F->fz,
+                             F->ar1x, F->ar1y, F->ar1z, F->ar2x, F->ar2y, F->ar2z, F->ar3x,
+                             F->ar3y, F->ar3z, F->den, F->apu, F->apv, F->apw, F->u, F->v,
+                             F->w, F->p, F->gradu, F->gradv, F->gradw, F->gradp};
+    const double *dst[31];
+    for (int q = 0; q < 31; ++q) {
+        if (!src[q]) { dst[q] = nullptr; continue; }
+        int rc = stage_n(ctx, q, src[q], q >= 27 ? 3 : 1, &dst[q]);
+        if (rc) return rc;
+    }
+    f = Fld{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8], dst[9],
+            dst[10], dst[11], dst[12], dst[13], dst[14], dst[15], dst[16], dst[17], dst[18],
+            dst[19], dst[20], dst[21], dst[22], dst[23], dst[24], dst[25], dst[26], dst[27],
+            dst[28], dst[29], dst[30]};
+    return CAP_OK;
+}
+
+static bool dir_ok(const cap_ctx *ctx, int idew, int idns, int idtb)
+{
+    const int nj = ctx->nj, nij = ctx->ni * ctx->nj;
+    return (idew == nj && idns == 1 && idtb == nij) || (idew == 1 && idns == nij && idtb == nj) ||
+           (idew == nij && idns == nj && idtb == 1);
+}
+
+static int stage_dir(cap_ctx *ctx, int idew, int idns, int idtb, const double *arvx,
+                     const double *arvy, const double *arvz, const double *fif, FaceDir &D)
+{
+    int rc;
+    D.idew = idew; D.idns = idns; D.idtb = idtb;
+    if ((rc = stage(ctx, 31, arvx, &D.arvx))) return rc;
+    if ((rc = stage(ctx, 32, arvy, &D.arvy))) return rc;
+    if ((rc = stage(ctx, 33, arvz, &D.arvz))) return rc;
+    return stage(ctx, 34, fif, &D.fif);
+}
+
+static int out3(cap_ctx *ctx, double *a, double *b, double *c, double **da, double **db, double **dc)
+{
+    const double *t;
+    int rc;
+    if ((rc = stage(ctx, 35, a, &t))) return rc; *da = (double *)t;
+    if ((rc = stage(ctx, 36, b, &t))) return rc; *db = (double *)t;
+    if ((rc = stage(ctx, 37, c, &t))) return rc; *dc = (double *)t;
+    return CAP_OK;
+}
+
+static int blocks_for(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
+
+extern "C" int cap_fluxmass(cap_ctx *ctx, const cap_fields *F, int idew, int idns, int idtb,
+                            const double *arvx, const double *arvy, const double *arvz,
+                            const double *fif, double *acfe, double *acfw, double *fcf)
+{
+    if (!ctx || !F || !arvx || !arvy || !arvz || !fif || !acfe || !acfw || !fcf ||
+        !dir_ok(ctx, idew, idns, idtb))
+        return CAP_ERR_ARG;
+    Fld f;
+    FaceDir D;
+    double *a, *b, *c;
+    int rc;
+    if ((rc = stage_fields(ctx, F, f)) || (rc = stage_dir(ctx, idew, idns, idtb, arvx, arvy, arvz, fif, D)) ||
+        (rc = out3(ctx, acfe, acfw, fcf, &a, &b, &c)))
+        return rc;
+    const size_t nf = (size_t)(ctx->nk - 2) * (ctx->ni - 3) * (ctx->nj - 3);
+    hipLaunchKernelGGL(k_fluxmass, dim3(blocks_for(nf)), dim3(256), 0, ctx->stream, ctx->ni,
+                       ctx->nj, ctx->nk, f, D, a, b, c);
+    HIPCHK(hipGetLastError());
+    if ((rc = unstage(ctx, acfe, a)) || (rc = unstage(ctx, acfw, b)) || (rc = unstage(ctx, fcf, c)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_piso_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *F, int idew,
+                                             int idns, int idtb, const double *arvx,
+                                             const double *arvy, const double *arvz,
+                                             const double *fif, double *acfe, double *acfw,
+                                             double *fcf, int ipc, double *bp, double *su)
+{
+    if (!ctx || !F || !arvx || !arvy || !arvz || !fif || !acfe || !acfw || !fcf || !bp || !su ||
+        !dir_ok(ctx, idew, idns, idtb))
+        return CAP_ERR_ARG;
+    Fld f;
+    FaceDir D;
+    double *a, *b, *c;
+    const double *tb, *ts;
+    int rc;
+    if ((rc = stage_fields(ctx, F, f)) || (rc = stage_dir(ctx, idew, idns, idtb, arvx, arvy, arvz, fif, D)) ||
+        (rc = out3(ctx, acfe, acfw, fcf, &a, &b, &c)) || (rc = stage(ctx, 38, nullptr, &tb)) ||
+        (rc = stage(ctx, 39, su, &ts)))
+        return rc;
+    double *dbp = (ctx->flags & CAP_DEVICE_POINTERS) ? bp : (double *)tb;
+    double *dsu = (double *)ts;
+    hipLaunchKernelGGL(k_zero_st, dim3(blocks_for(ctx->nstd)), dim3(256), 0, ctx->stream, dbp, ctx->nstd);
+    const size_t nf = (size_t)(ctx->nk - 3) * (ctx->ni - 3) * (ctx->nj - 3);
+    hipLaunchKernelGGL(k_piso_assemble, dim3(blocks_for(nf)), dim3(256), 0, ctx->stream, ctx->ni,
+                       ctx->nj, ctx->nk, f, D, ipc, a, b, c, dbp, dsu);
+    if (ipc > 0) {
+        const size_t nc = (size_t)(ctx->ni - 4) * (ctx->nj - 4) * (ctx->nk - 4);
+        hipLaunchKernelGGL(k_piso_su, dim3(blocks_for(nc)), dim3(256), 0, ctx->stream, ctx->ni,
+                           ctx->nj, ctx->nk, idew, (const double *)dbp, dsu);
+    }
+    HIPCHK(hipGetLastError());
+    if ((rc = unstage(ctx, acfe, a)) || (rc = unstage(ctx, acfw, b)) || (rc = unstage(ctx, fcf, c)) ||
+        (rc = unstage(ctx, bp, dbp)) || (rc = unstage(ctx, su, dsu)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *F, int idew, int idns,
+                                        int idtb, const double *arvx, const double *arvy,
+                                        const double *arvz, const double *fif, double *acfe,
+                                        double *acfw, double *fcf, const double *su,
+                                        const double *sv, const double *sw)
+{
+    if (!ctx || !F || !arvx || !arvy || !arvz || !fif || !acfe || !acfw || !fcf || !su || !sv ||
+        !sw || !dir_ok(ctx, idew, idns, idtb))
+        return CAP_ERR_ARG;
+    Fld f;
+    FaceDir D;
+    double *a, *b, *c;
+    const double *dsu, *dsv, *dsw;
+    int rc;
+    if ((rc = stage_fields(ctx, F, f)) || (rc = stage_dir(ctx, idew, idns, idtb, arvx, arvy, arvz, fif, D)) ||
+        (rc = out3(ctx, acfe, acfw, fcf, &a, &b, &c)) || (rc = stage(ctx, 39, su, &dsu)) ||
+        (rc = stage(ctx, 40, sv, &dsv)) || (rc = stage(ctx, 41, sw, &dsw)))
+        return rc;
+    const size_t nf = (size_t)(ctx->nk - 3) * (ctx->ni - 3) * (ctx->nj - 3);
+    hipLaunchKernelGGL(k_assemble_p, dim3(blocks_for(nf)), dim3(256), 0, ctx->stream, ctx->ni,
+                       ctx->nj, ctx->nk, f, D, dsu, dsv, dsw, a, b, c);
+    HIPCHK(hipGetLastError());
+    if ((rc = unstage(ctx, acfe, a)) || (rc = unstage(ctx, acfw, b)) || (rc = unstage(ctx, fcf, c)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_fluxmc(cap_ctx *ctx, const cap_fields *F, int idew, int idns, int idtb,
+                          const double *fif, double *fmcor)
+{
+    if (!ctx || !F || !fif || !fmcor || !dir_ok(ctx, idew, idns, idtb)) return CAP_ERR_ARG;
+    Fld f;
+    FaceDir D = {idew, idns, idtb, nullptr, nullptr, nullptr, nullptr};
+    const double *t;
+    int rc;
+    if ((rc = stage_fields(ctx, F, f)) || (rc = stage(ctx, 34, fif, &D.fif)) ||
+        (rc = stage(ctx, 35, fmcor, &t)))
+        return rc;
+    hipLaunchKernelGGL(k_fluxmc, dim3(blocks_for(ctx->nstd)), dim3(256), 0, ctx->stream, ctx->ni,
+                       ctx->nj, ctx->nk, f, D, (double *)t);
+    HIPCHK(hipGetLastError());
+    if ((rc = unstage(ctx, fmcor, t))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_continuity(cap_ctx *ctx, const double *f1, const double *f2, const double *f3,
+                              double *sums)
+{
+    if (!ctx || !f1 || !f2 || !f3 || !sums) return CAP_ERR_ARG;
+    const double *d1, *d2, *d3;
+    int rc;
+    if ((rc = stage(ctx, 35, f1, &d1)) || (rc = stage(ctx, 36, f2, &d2)) || (rc = stage(ctx, 37, f3, &d3)))
+        return rc;
+    HIPCHK(hipMemsetAsync(ctx->ticket, 0, 64, ctx->stream));
+    const size_t n = (size_t)(ctx->ni - 4) * (ctx->nj - 4) * (ctx->nk - 2);
+    hipLaunchKernelGGL(k_continuity, dim3(std::min(blocks_for(n), 1024)), dim3(256), 0,
+                       ctx->stream, ctx->ni, ctx->nj, ctx->nk, d1, d2, d3, ctx->st,
+                       ctx->partials, ctx->ticket);
+    HIPCHK(hipGetLastError());
+    DevState h;
+    HIPCHK(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    memcpy(&h, ctx->st_host, sizeof h);
+    sums[0] = h.red[0];
+    sums[1] = h.red[1];
     return CAP_OK;
 }
 
@@ -304,6 +775,6 @@ extern "C" int cap_build_lognormal(cap_ctx *ctx, unsigned long long seed, double
 
 void cap_free_extra(cap_ctx *ctx)
 {
-    for (int q = 0; q < 20; ++q) if (ctx->G[q]) (void)hipFree(ctx->G[q]);
+    for (int q = 0; q < CAP_NSTAGE; ++q) if (ctx->G[q]) (void)hipFree(ctx->G[q]);
     for (int q = 0; q < F_COUNT; ++q) if (ctx->F[q]) (void)hipFree(ctx->F[q]);
 }

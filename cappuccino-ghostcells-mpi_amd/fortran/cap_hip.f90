@@ -24,6 +24,19 @@ module cap_hip
     type(c_ptr) :: bn = c_null_ptr, be = c_null_ptr, bt = c_null_ptr
   end type cap_coef
 
+  ! fields read by the p' assembly (modules geometry, variables, coef, gradients)
+  type, bind(c) :: cap_fields
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr
+    type(c_ptr) :: xc = c_null_ptr, yc = c_null_ptr, zc = c_null_ptr, vol = c_null_ptr
+    type(c_ptr) :: fx = c_null_ptr, fy = c_null_ptr, fz = c_null_ptr
+    type(c_ptr) :: ar1x = c_null_ptr, ar1y = c_null_ptr, ar1z = c_null_ptr
+    type(c_ptr) :: ar2x = c_null_ptr, ar2y = c_null_ptr, ar2z = c_null_ptr
+    type(c_ptr) :: ar3x = c_null_ptr, ar3y = c_null_ptr, ar3z = c_null_ptr
+    type(c_ptr) :: den = c_null_ptr, apu = c_null_ptr, apv = c_null_ptr, apw = c_null_ptr
+    type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, w = c_null_ptr, p = c_null_ptr
+    type(c_ptr) :: gradu = c_null_ptr, gradv = c_null_ptr, gradw = c_null_ptr, gradp = c_null_ptr
+  end type cap_fields
+
   type, bind(c) :: cap_report
     real(c_double) :: resor, resl
     integer(c_int) :: iters, status
@@ -92,6 +105,38 @@ module cap_hip
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx, p, fx, fy, fz
     end function
+    integer(c_int) function cap_fluxmass(ctx, f, idew, idns, idtb, arvx, arvy, arvz, fif, &
+        acfe, acfw, fcf) bind(c, name='cap_fluxmass')
+      import :: c_ptr, c_int, cap_fields
+      type(c_ptr), value :: ctx, arvx, arvy, arvz, fif, acfe, acfw, fcf
+      type(cap_fields), intent(in) :: f
+      integer(c_int), value :: idew, idns, idtb
+    end function
+    integer(c_int) function cap_piso_assemble_pressure_eq(ctx, f, idew, idns, idtb, arvx, arvy, &
+        arvz, fif, acfe, acfw, fcf, ipc, bp, su) bind(c, name='cap_piso_assemble_pressure_eq')
+      import :: c_ptr, c_int, cap_fields
+      type(c_ptr), value :: ctx, arvx, arvy, arvz, fif, acfe, acfw, fcf, bp, su
+      type(cap_fields), intent(in) :: f
+      integer(c_int), value :: idew, idns, idtb, ipc
+    end function
+    integer(c_int) function cap_assemble_pressure_eq(ctx, f, idew, idns, idtb, arvx, arvy, arvz, &
+        fif, acfe, acfw, fcf, su, sv, sw) bind(c, name='cap_assemble_pressure_eq')
+      import :: c_ptr, c_int, cap_fields
+      type(c_ptr), value :: ctx, arvx, arvy, arvz, fif, acfe, acfw, fcf, su, sv, sw
+      type(cap_fields), intent(in) :: f
+      integer(c_int), value :: idew, idns, idtb
+    end function
+    integer(c_int) function cap_fluxmc(ctx, f, idew, idns, idtb, fif, fmcor) bind(c, name='cap_fluxmc')
+      import :: c_ptr, c_int, cap_fields
+      type(c_ptr), value :: ctx, fif, fmcor
+      type(cap_fields), intent(in) :: f
+      integer(c_int), value :: idew, idns, idtb
+    end function
+    integer(c_int) function cap_continuity(ctx, f1, f2, f3, sums) bind(c, name='cap_continuity')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx, f1, f2, f3
+      real(c_double), intent(out) :: sums(2)
+    end function
   end interface
 
   ! one context per process, created on first use for the current extents
@@ -141,5 +186,11 @@ contains
     type(c_ptr) :: p
     p = c_loc(x)
   end function cap_addr
+
+  function cap_caddr(x) result(p)
+    real(c_double), target, intent(in) :: x(*)
+    type(c_ptr) :: p
+    p = c_loc(x)
+  end function cap_caddr
 
 end module cap_hip

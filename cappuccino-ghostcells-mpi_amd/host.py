@@ -33,7 +33,12 @@ GEOM_FIELDS = ("fx", "fy", "fz", "ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z",
 # every entry point declared in include/cap_amd.h
 EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgsip",
            "cap_iccg", "cap_cgstab_sip", "cap_pcgsip_mixed", "cap_grad_gauss", "cap_bpres",
-           "cap_build_lognormal", "cap_time_kernels")
+           "cap_build_lognormal", "cap_time_kernels", "cap_fluxmass",
+           "cap_piso_assemble_pressure_eq", "cap_assemble_pressure_eq", "cap_fluxmc",
+           "cap_continuity")
+FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
+               "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
+               "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
 
 
 class CapError(RuntimeError):
@@ -42,6 +47,10 @@ class CapError(RuntimeError):
 
 class _Coef(C.Structure):
     _fields_ = [(f, C.c_void_p) for f in COEF_FIELDS]
+
+
+class _Fields(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in FIELD_NAMES]
 
 
 class _Report(C.Structure):
@@ -73,6 +82,13 @@ def load():
     L.cap_bpres.argtypes = [vp, vp, vp, vp, vp]
     L.cap_build_lognormal.argtypes = [vp, C.c_ulonglong, d, C.POINTER(_Coef)]
     L.cap_time_kernels.argtypes = [vp, i, i, C.POINTER(d)]
+    F = C.POINTER(_Fields)
+    L.cap_fluxmass.argtypes = [vp, F, i, i, i, vp, vp, vp, vp, vp, vp, vp]
+    L.cap_piso_assemble_pressure_eq.argtypes = [vp, F, i, i, i, vp, vp, vp, vp, vp, vp, vp, i,
+                                                vp, vp]
+    L.cap_assemble_pressure_eq.argtypes = [vp, F, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.cap_fluxmc.argtypes = [vp, F, i, i, i, vp, vp]
+    L.cap_continuity.argtypes = [vp, vp, vp, vp, C.POINTER(d)]
     _lib = L
     return L
 
@@ -179,6 +195,51 @@ class Context:
         rc = self._lib.cap_time_kernels(self._h, 1 if fp32 else 0, reps, ms)
         self._check(rc, "cap_time_kernels", allow=(CAP_OK,))
         return list(ms)
+
+    # --- p' assembly (fields: dict of the FIELD_NAMES arrays) --------------
+    def _dir(self, d):
+        ni, nj, nk = self.dims
+        return [(nj, 1, ni * nj), (1, ni * nj, nj), (ni * nj, nj, 1)][d]
+
+    def _fields(self, f):
+        return _Fields(*[_ptr(f.get(q)) for q in FIELD_NAMES])
+
+    def fluxmass(self, f, d, acfe, acfw, fcf):
+        """Direction d: 0 east (ar1, fx), 1 north (ar2, fy), 2 top (ar3, fz)."""
+        arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
+        st = self._fields(f)
+        rc = self._lib.cap_fluxmass(self._h, C.byref(st), *self._dir(d), *map(_ptr, arv),
+                                    _ptr(f[("fx", "fy", "fz")[d]]),
+                                    _ptr(acfe), _ptr(acfw), _ptr(fcf))
+        self._check(rc, "cap_fluxmass", allow=(CAP_OK,))
+
+    def piso_assemble_pressure_eq(self, f, d, acfe, acfw, fcf, ipc, bp, su):
+        arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
+        st = self._fields(f)
+        rc = self._lib.cap_piso_assemble_pressure_eq(
+            self._h, C.byref(st), *self._dir(d), *map(_ptr, arv), _ptr(f[("fx", "fy", "fz")[d]]),
+            _ptr(acfe), _ptr(acfw), _ptr(fcf), ipc, _ptr(bp), _ptr(su))
+        self._check(rc, "cap_piso_assemble_pressure_eq", allow=(CAP_OK,))
+
+    def assemble_pressure_eq(self, f, d, acfe, acfw, fcf, su, sv, sw):
+        arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
+        st = self._fields(f)
+        rc = self._lib.cap_assemble_pressure_eq(
+            self._h, C.byref(st), *self._dir(d), *map(_ptr, arv), _ptr(f[("fx", "fy", "fz")[d]]),
+            _ptr(acfe), _ptr(acfw), _ptr(fcf), _ptr(su), _ptr(sv), _ptr(sw))
+        self._check(rc, "cap_assemble_pressure_eq", allow=(CAP_OK,))
+
+    def fluxmc(self, f, d, fmcor):
+        st = self._fields(f)
+        rc = self._lib.cap_fluxmc(self._h, C.byref(st), *self._dir(d),
+                                  _ptr(f[("fx", "fy", "fz")[d]]), _ptr(fmcor))
+        self._check(rc, "cap_fluxmc", allow=(CAP_OK,))
+
+    def continuity(self, f1, f2, f3):
+        out = (C.c_double * 2)()
+        rc = self._lib.cap_continuity(self._h, _ptr(f1), _ptr(f2), _ptr(f3), out)
+        self._check(rc, "cap_continuity", allow=(CAP_OK,))
+        return list(out)
 
     def build_lognormal(self, coef, seed, sigma=1.0):
         st = coef._struct()

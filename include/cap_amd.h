@@ -102,6 +102,52 @@ int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
 int cap_grad_gauss(cap_ctx *ctx, const double *u, const double *const *geom,
                    double *dudx, double *dudy, double *dudz);
 
+/* Fields and geometry read by the p' assembly: modules geometry (x, y, z
+ * nodes; xc, yc, zc centres; vol; fx, fy, fz; ar1*-ar3* face areas),
+ * variables (den, u, v, w, p), coef (apu, apv, apw) and gradients (gradu,
+ * gradv, gradw, gradp: the reference's gradu(3, nxyz), component fastest,
+ * 3*nxyz doubles).  A pointer a routine does not read may be NULL. */
+typedef struct {
+    const double *x, *y, *z, *xc, *yc, *zc, *vol, *fx, *fy, *fz;
+    const double *ar1x, *ar1y, *ar1z, *ar2x, *ar2y, *ar2z, *ar3x, *ar3y, *ar3z;
+    const double *den, *apu, *apv, *apw, *u, *v, *w, *p;
+    const double *gradu, *gradv, *gradw, *gradp;
+} cap_fields;
+
+/* Replaces `call fluxmass(idew,idns,idtb, arvx,arvy,arvz, fif, acfe,acfw,fcf)`
+ * (fluxmass2.f90:3-195): p' face coefficients and Rhie-Chow mass flux on the
+ * face loop k=2..nkm, i=2..nimm, j=2..njmm.  (idew,idns,idtb) is one of
+ * (nj,1,nij), (1,nij,nj), (nij,nj,1) as at calcp-multiple_correction_SIMPLE.f90:50-60. */
+int cap_fluxmass(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
+                 const double *arvx, const double *arvy, const double *arvz,
+                 const double *fif, double *acfe, double *acfw, double *fcf);
+
+/* Replaces `call piso_assemble_pressure_eq(idew,idns,idtb, arvx,arvy,arvz,
+ * fif, acfe,acfw,fcf, ipc, bp)` (PISO_assemble_pressure_eq.f90:3-260); su is
+ * module coef's su, updated in place like the reference. */
+int cap_piso_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *f, int idew, int idns,
+                                  int idtb, const double *arvx, const double *arvy,
+                                  const double *arvz, const double *fif, double *acfe,
+                                  double *acfw, double *fcf, int ipc, double *bp, double *su);
+
+/* Replaces `call assemble_pressure_eq(...)` (assemble_pressure_eq.f90:3-124);
+ * su, sv, sw are the face-velocity sources (HbyA) it reads from module coef. */
+int cap_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
+                             const double *arvx, const double *arvy, const double *arvz,
+                             const double *fif, double *acfe, double *acfw, double *fcf,
+                             const double *su, const double *sv, const double *sw);
+
+/* `call fluxmc(inp,idew,idns,idtb,fif,fmcor)` (fluxmc.f90:3-109) for every
+ * cell k,i,j = 3..n?mm at once: fmcor(inp) for those cells, 0 elsewhere. */
+int cap_fluxmc(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
+               const double *fif, double *fmcor);
+
+/* The continuity-error sums of continuityErrors.h:55-81 over k=2..nkm,
+ * i,j=3..n?mm: sums[0] = global (signed), sums[1] = sum of |local|.  FP64,
+ * deterministic reduction order. */
+int cap_continuity(cap_ctx *ctx, const double *f1, const double *f2, const double *f3,
+                   double *sums);
+
 /* Replaces `call bpres(p)` (bpres.f90:3-81). */
 int cap_bpres(cap_ctx *ctx, double *p, const double *fx, const double *fy,
               const double *fz);

@@ -660,3 +660,225 @@ void orc_bpres(int ni, int nj, int nk, double *p, const double *fx,
             p[c + 1] = p[c] + (p[c] - p[c - 1]) * (1.0 - fy[c - 1]);
         }
 }
+
+/* ------------------------------------------------------ p' assembly -- */
+#define G3(g, c, q) ((g)[3 * (c) + (q)])
+
+/* fluxmass2.f90:41-192 -- face loop k=2..nkm, i=2..nimm, j=2..njmm */
+void orc_fluxmass(int ni, int nj, int nk, const orc_fields *f, int idew, int idns, int idtb,
+                  const double *arvx, const double *arvy, const double *arvz,
+                  const double *fif, double *acfe, double *acfw, double *fcf)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    for (int k = 2; k <= nkm; ++k)
+        for (int i = 2; i <= nimm; ++i)
+            for (int j = 2; j <= njmm; ++j) {
+                size_t p = IX(i, j, k), e = p + idew, s = p - idns, b = p - idtb, bs = b - idns;
+                double fxe = fif[p], fxp = 1.0 - fxe;
+                double xf = 0.25 * (f->x[p] + f->x[s] + f->x[b] + f->x[bs]);
+                double yf = 0.25 * (f->y[p] + f->y[s] + f->y[b] + f->y[bs]);
+                double zf = 0.25 * (f->z[p] + f->z[s] + f->z[b] + f->z[bs]);
+                double xi = f->xc[p] * fxp + f->xc[e] * fxe;
+                double yi = f->yc[p] * fxp + f->yc[e] * fxe;
+                double zi = f->zc[p] * fxp + f->zc[e] * fxe;
+                double xpn = f->xc[e] - f->xc[p], ypn = f->yc[e] - f->yc[p], zpn = f->zc[e] - f->zc[p];
+                double arx = arvx[p], ary = arvy[p], arz = arvz[p];
+                double are = sqrt(arx * arx + ary * ary + arz * arz);
+                double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+                double sfdpnr = 1.0 / (arx * xpn * nxx + ary * ypn * nyy + arz * zpn * nzz);
+                double dene = f->den[p] * fxp + f->den[e] * fxe;
+                double smdpn = (arx * arx + ary * ary + arz * arz) * sfdpnr;
+                acfe[p] = dene * (fxp * f->vol[p] * f->apu[p] + fxe * f->vol[e] * f->apu[e]) * smdpn;
+                acfw[e] = acfe[p];
+                double duxi = G3(f->gradu, p, 0) * fxp + G3(f->gradu, e, 0) * fxe;
+                double duyi = G3(f->gradu, p, 1) * fxp + G3(f->gradu, e, 1) * fxe;
+                double duzi = G3(f->gradu, p, 2) * fxp + G3(f->gradu, e, 2) * fxe;
+                double ui = f->u[p] * fxp + f->u[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+                duxi = G3(f->gradv, p, 0) * fxp + G3(f->gradv, e, 0) * fxe;
+                duyi = G3(f->gradv, p, 1) * fxp + G3(f->gradv, e, 1) * fxe;
+                duzi = G3(f->gradv, p, 2) * fxp + G3(f->gradv, e, 2) * fxe;
+                double vi = f->v[p] * fxp + f->v[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+                duxi = G3(f->gradw, p, 0) * fxp + G3(f->gradw, e, 0) * fxe;
+                duyi = G3(f->gradw, p, 1) * fxp + G3(f->gradw, e, 1) * fxe;
+                duzi = G3(f->gradw, p, 2) * fxp + G3(f->gradw, e, 2) * fxe;
+                double wi = f->w[p] * fxp + f->w[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+                double dpxi = (fxe * f->vol[e] * f->apu[e] + fxp * f->vol[p] * f->apu[p]) *
+                              (fxe * G3(f->gradp, e, 0) + fxp * G3(f->gradp, p, 0)) * xpn * nxx;
+                double dpyi = (fxe * f->vol[e] * f->apv[e] + fxp * f->vol[p] * f->apv[p]) *
+                              (fxe * G3(f->gradp, e, 1) + fxp * G3(f->gradp, p, 1)) * ypn * nyy;
+                double dpzi = (fxe * f->vol[e] * f->apw[e] + fxp * f->vol[p] * f->apw[p]) *
+                              (fxe * G3(f->gradp, e, 2) + fxp * G3(f->gradp, p, 2)) * zpn * nzz;
+                double xpp = xf - (xf - f->xc[p]) * nxx, ypp = yf - (yf - f->yc[p]) * nyy,
+                       zpp = zf - (zf - f->zc[p]) * nzz;
+                double xep = xf - (xf - f->xc[e]) * nxx, yep = yf - (yf - f->yc[e]) * nyy,
+                       zep = zf - (zf - f->zc[e]) * nzz;
+                xpp = xpp - f->xc[p]; ypp = ypp - f->yc[p]; zpp = zpp - f->zc[p];
+                xep = xep - f->xc[e]; yep = yep - f->yc[e]; zep = zep - f->zc[e];
+                double dpe = (f->p[e] - f->p[p]) +
+                             (G3(f->gradp, e, 0) * xep + G3(f->gradp, e, 1) * yep + G3(f->gradp, e, 2) * zep -
+                              G3(f->gradp, p, 0) * xpp + G3(f->gradp, p, 1) * ypp + G3(f->gradp, p, 2) * zpp);
+                double dpex = (fxe * f->vol[e] * f->apu[e] + fxp * f->vol[p] * f->apu[p]) * (arx * sfdpnr) * dpe;
+                double dpey = (fxe * f->vol[e] * f->apv[e] + fxp * f->vol[p] * f->apv[p]) * (ary * sfdpnr) * dpe;
+                double dpez = (fxe * f->vol[e] * f->apw[e] + fxp * f->vol[p] * f->apw[p]) * (arz * sfdpnr) * dpe;
+                double ue = ui - dpex + dpxi, ve = vi - dpey + dpyi, we = wi - dpez + dpzi;
+                fcf[p] = dene * (ue * arx + ve * ary + we * arz);
+            }
+}
+
+/* PISO_assemble_pressure_eq.f90:44-257 (bp is the scratch array the caller
+ * passes, zeroed first; su is module `coef` su) */
+void orc_piso_assemble(int ni, int nj, int nk, const orc_fields *f, int idew, int idns,
+                       int idtb, const double *arvx, const double *arvy, const double *arvz,
+                       const double *fif, double *acfe, double *acfw, double *fcf, int ipc,
+                       double *bp, double *su)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    memset(bp, 0, (size_t)nij * nk * sizeof(double));
+    for (int k = 2; k <= nkmm; ++k)
+        for (int i = 2; i <= nimm; ++i)
+            for (int j = 2; j <= njmm; ++j) {
+                size_t p = IX(i, j, k), e = p + idew, s = p - idns, b = p - idtb, bs = b - idns;
+                double fxe = fif[p], fxp = 1.0 - fxe;
+                double xf = 0.25 * (f->x[p] + f->x[s] + f->x[b] + f->x[bs]);
+                double yf = 0.25 * (f->y[p] + f->y[s] + f->y[b] + f->y[bs]);
+                double zf = 0.25 * (f->z[p] + f->z[s] + f->z[b] + f->z[bs]);
+                double xi = f->xc[p] * fxp + f->xc[e] * fxe;
+                double yi = f->yc[p] * fxp + f->yc[e] * fxe;
+                double zi = f->zc[p] * fxp + f->zc[e] * fxe;
+                double xpn = f->xc[e] - f->xc[p], ypn = f->yc[e] - f->yc[p], zpn = f->zc[e] - f->zc[p];
+                double dpn = sqrt(xpn * xpn + ypn * ypn + zpn * zpn);
+                double ixi1 = xpn / dpn, ixi2 = ypn / dpn, ixi3 = zpn / dpn;
+                double arx = arvx[p], ary = arvy[p], arz = arvz[p];
+                double are = sqrt(arx * arx + ary * ary + arz * arz);
+                double vole = xpn * arx + ypn * ary + zpn * arz;
+                double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+                double costn = nxx * ixi1 + nyy * ixi2 + nzz * ixi3;
+                double d1x = costn, d1y = costn, d1z = costn;
+                double d2x = xpn * costn, d2y = ypn * costn, d2z = zpn * costn;
+                double dene = f->den[p] * fxp + f->den[e] * fxe;
+                double game = dene * (fxp * f->vol[p] * f->apu[p] + fxe * f->vol[e] * f->apu[e]);
+                acfe[p] = game * are / dpn;
+                acfw[e] = acfe[p];
+                if (ipc > 0) {
+                    double dfixi = G3(f->gradp, p, 0) * fxp + G3(f->gradp, e, 0) * fxe;
+                    double dfiyi = G3(f->gradp, p, 1) * fxp + G3(f->gradp, e, 1) * fxe;
+                    double dfizi = G3(f->gradp, p, 2) * fxp + G3(f->gradp, e, 2) * fxe;
+                    double dfixii = dfixi * d1x + arx / vole * (f->p[e] - f->p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+                    double dfiyii = dfiyi * d1y + ary / vole * (f->p[e] - f->p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+                    double dfizii = dfizi * d1z + arz / vole * (f->p[e] - f->p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+                    double fdfie = game * (dfixii * arx + dfiyii * ary + dfizii * arz);
+                    double fdfii = game * are / dpn * (dfixi * xpn + dfiyi * ypn + dfizi * zpn);
+                    double suadd = fdfie - fdfii;
+                    su[p] = su[p] + suadd;
+                    bp[p] = suadd;
+                    double duxi = G3(f->gradu, p, 0) * fxp + G3(f->gradu, e, 0) * fxe;
+                    double duyi = G3(f->gradu, p, 1) * fxp + G3(f->gradu, e, 1) * fxe;
+                    double duzi = G3(f->gradu, p, 2) * fxp + G3(f->gradu, e, 2) * fxe;
+                    double ui = f->u[p] * fxp + f->u[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
+                    duxi = G3(f->gradv, p, 0) * fxp + G3(f->gradv, e, 0) * fxe;
+                    duyi = G3(f->gradv, p, 1) * fxp + G3(f->gradv, e, 1) * fxe;
+                    duzi = G3(f->gradv, p, 2) * fxp + G3(f->gradv, e, 2) * fxe;
+                    double vi = f->v[p] * fxp + f->v[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
+                    duxi = G3(f->gradw, p, 0) * fxp + G3(f->gradw, e, 0) * fxe;
+                    duyi = G3(f->gradw, p, 1) * fxp + G3(f->gradw, e, 1) * fxe;
+                    duzi = G3(f->gradw, p, 2) * fxp + G3(f->gradw, e, 2) * fxe;
+                    double wi = f->w[p] * fxp + f->w[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
+                    fcf[p] = dene * (ui * arx + vi * ary + wi * arz);
+                }
+            }
+    if (ipc > 0) {
+        /* PISO_assemble_pressure_eq.f90:241-253: su(c) -= bp(c) at c = cell + idew */
+        for (int k = 3; k <= nkmm; ++k)
+            for (int i = 3; i <= nimm; ++i)
+                for (int j = 3; j <= njmm; ++j) {
+                    size_t c = IX(i, j, k) + idew;
+                    su[c] = su[c] - bp[c];
+                }
+    }
+}
+
+/* assemble_pressure_eq.f90:38-118 (projection form: no density or apu in the
+ * coefficient; face velocity from su/sv/sw = HbyA, gradients unused) */
+void orc_assemble_p(int ni, int nj, int nk, const orc_fields *f, int idew, int idns, int idtb,
+                    const double *arvx, const double *arvy, const double *arvz,
+                    const double *fif, double *acfe, double *acfw, double *fcf,
+                    const double *su, const double *sv, const double *sw)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    for (int k = 2; k <= nkmm; ++k)
+        for (int i = 2; i <= nimm; ++i)
+            for (int j = 2; j <= njmm; ++j) {
+                size_t p = IX(i, j, k), e = p + idew;
+                double fxe = fif[p], fxp = 1.0 - fxe;
+                double xpn = f->xc[e] - f->xc[p], ypn = f->yc[e] - f->yc[p], zpn = f->zc[e] - f->zc[p];
+                double arx = arvx[p], ary = arvy[p], arz = arvz[p];
+                double are = sqrt(arx * arx + ary * ary + arz * arz);
+                double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+                double smdpn = (arx * arx + ary * ary + arz * arz) / (arx * xpn * nxx + ary * ypn * nyy + arz * zpn * nzz);
+                acfe[p] = (fxp * f->vol[p] + fxe * f->vol[e]) * smdpn;
+                acfw[e] = acfe[p];
+                double ui = su[p] * fxp + su[e] * fxe;
+                double vi = sv[p] * fxp + sv[e] * fxe;
+                double wi = sw[p] * fxp + sw[e] * fxe;
+                fcf[p] = ui * arx + vi * ary + wi * arz;
+            }
+    (void)idns; (void)idtb;
+}
+
+/* fluxmc.f90:41-106 for every cell of the corrector loop range of
+ * calcp-multiple_correction_SIMPLE.f90:120-162 (k,i,j = 3..n?mm) */
+void orc_fluxmc_all(int ni, int nj, int nk, const orc_fields *f, int idew, int idns, int idtb,
+                    const double *fif, double *fmcor)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    memset(fmcor, 0, (size_t)nij * nk * sizeof(double));
+    for (int k = 3; k <= nkmm; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t p = IX(i, j, k), e = p + idew, s = p - idns, b = p - idtb, bs = b - idns;
+                double fxe = fif[p], fxp = 1. - fxe;
+                double arx, ary, arz;
+                if (idew == nj) { arx = f->ar1x[p]; ary = f->ar1y[p]; arz = f->ar1z[p]; }
+                else if (idew == 1) { arx = f->ar2x[p]; ary = f->ar2y[p]; arz = f->ar2z[p]; }
+                else { arx = f->ar3x[p]; ary = f->ar3y[p]; arz = f->ar3z[p]; }
+                double xpn = f->xc[e] - f->xc[p], ypn = f->yc[e] - f->yc[p], zpn = f->zc[e] - f->zc[p];
+                double are = sqrt(arx * arx + ary * ary + arz * arz);
+                double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+                double dppnnr = 1. / ((xpn * nxx) + (ypn * nyy) + (zpn * nzz));
+                double xf = 0.25 * (f->x[p] + f->x[s] + f->x[b] + f->x[bs]);
+                double yf = 0.25 * (f->y[p] + f->y[s] + f->y[b] + f->y[bs]);
+                double zf = 0.25 * (f->z[p] + f->z[s] + f->z[b] + f->z[bs]);
+                double xpp = xf - (xf - f->xc[p]) * nxx, ypp = yf - (yf - f->yc[p]) * nyy, zpp = zf - (zf - f->zc[p]) * nzz;
+                double xep = xf - (xf - f->xc[e]) * nxx, yep = yf - (yf - f->yc[e]) * nyy, zep = zf - (zf - f->zc[e]) * nzz;
+                xpp = xpp - f->xc[p]; ypp = ypp - f->yc[p]; zpp = zpp - f->zc[p];
+                xep = xep - f->xc[e]; yep = yep - f->yc[e]; zep = zep - f->zc[e];
+                double rapr = (f->apu[p] * f->den[p] * f->vol[p] * fxp + f->apu[e] * f->den[e] * f->vol[e] * fxe);
+                fmcor[p] = rapr * are * ((G3(f->gradp, e, 0) * xep - G3(f->gradp, p, 0) * xpp) +
+                                         (G3(f->gradp, e, 1) * yep - G3(f->gradp, p, 1) * ypp) +
+                                         (G3(f->gradp, e, 2) * zep - G3(f->gradp, p, 2) * zpp)) * dppnnr;
+            }
+}
+
+/* continuityErrors.h:63-81: sums over k=2..nkm, i,j=3..n?mm of
+ * LocalContErr = -f1+f1(w) - f2+f2(s) - f3+f3(b); sums = {global, sum|local|} */
+void orc_continuity(int ni, int nj, int nk, const double *f1, const double *f2,
+                    const double *f3, double *sums)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    double g = 0.0, sl = 0.0;
+    for (int k = 2; k <= nkm; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                size_t c = IX(i, j, k);
+                double loc = -f1[c] + f1[c - nj] - f2[c] + f2[c - 1] - f3[c] + f3[c - nij];
+                g = g + loc;
+                sl = sl + fabs(loc);
+            }
+    sums[0] = g;
+    sums[1] = sl;
+}

@@ -82,3 +82,36 @@ double orc_pcgsip_time_iters(int ni, int nj, int nk, double *fi, orc_sys *s,
 }
 #endif
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Fields and geometry the p' assembly reads (modules geometry, variables,
+ * coef, gradients of modules_allocatable.f90).  Gradients are the reference's
+ * array-of-structures gradu(3, nxyz): component fastest. */
+typedef struct {
+    const double *x, *y, *z, *xc, *yc, *zc, *vol;
+    const double *fx, *fy, *fz;
+    const double *ar1x, *ar1y, *ar1z, *ar2x, *ar2y, *ar2z, *ar3x, *ar3y, *ar3z;
+    const double *den, *apu, *apv, *apw, *u, *v, *w, *p;
+    const double *gradu, *gradv, *gradw, *gradp;
+} orc_fields;
+
+void orc_fluxmass(int ni, int nj, int nk, const orc_fields *f, int idew, int idns, int idtb,
+                  const double *arvx, const double *arvy, const double *arvz,
+                  const double *fif, double *acfe, double *acfw, double *fcf);
+void orc_piso_assemble(int ni, int nj, int nk, const orc_fields *f, int idew, int idns,
+                       int idtb, const double *arvx, const double *arvy, const double *arvz,
+                       const double *fif, double *acfe, double *acfw, double *fcf, int ipc,
+                       double *bp, double *su);
+void orc_assemble_p(int ni, int nj, int nk, const orc_fields *f, int idew, int idns, int idtb,
+                    const double *arvx, const double *arvy, const double *arvz,
+                    const double *fif, double *acfe, double *acfw, double *fcf,
+                    const double *su, const double *sv, const double *sw);
+void orc_fluxmc_all(int ni, int nj, int nk, const orc_fields *f, int idew, int idns, int idtb,
+                    const double *fif, double *fmcor);
+void orc_continuity(int ni, int nj, int nk, const double *f1, const double *f2,
+                    const double *f3, double *sums);
+#ifdef __cplusplus
+}
+#endif

@@ -12,8 +12,10 @@
 ! global_sum/exchange) are not built.
 !
 ! usage: ref_driver <mode> <in.bin> <out.bin> <log.txt>
-!   mode = sipsol | cgstab_sip | grad_gauss | bpres
+!   mode = sipsol | cgstab_sip | grad_gauss | bpres | fluxmass | piso_assemble
+!          | assemble_p | fluxmc | continuity
 ! in.bin  : int32 ni,nj,nk,nsw ; real64 alfa,sor ; then mode-specific arrays
+!           (stencil modes: nsw = direction 0 east, 1 north, 2 top; alfa = ipc)
 ! out.bin : mode-specific arrays, then real64 resor(ip) for the solvers
 !-----------------------------------------------------------------------
 program ref_driver
@@ -23,11 +25,15 @@ program ref_driver
   use coef
   use coefb
   use geometry
+  use variables
+  use gradients
   implicit none
   character(len=256) :: mode, fin, fout, flog
   integer :: ni_, nj_, nk_, nsw_, n
   real(8) :: alfa_, sor_
-  real(8), allocatable :: fi(:), d1(:), d2(:), d3(:)
+  real(8), allocatable :: fi(:), d1(:), d2(:), d3(:), bsc(:)
+  integer :: idew, idns, idtb, inp, i, j, k
+  real(8) :: fmcor
 
   call get_command_argument(1, mode)
   call get_command_argument(2, fin)
@@ -57,7 +63,10 @@ program ref_driver
   allocate(bb(n),bs(n),bw(n),bp(n),bn(n),be(n),bt(n))
   allocate(fx(n),fy(n),fz(n),vol(n))
   allocate(ar1x(n),ar1y(n),ar1z(n),ar2x(n),ar2y(n),ar2z(n),ar3x(n),ar3y(n),ar3z(n))
-  allocate(fi(n), d1(n), d2(n), d3(n))
+  allocate(fi(n), d1(n), d2(n), d3(n), bsc(n))
+  allocate(x(n),y(n),z(n),xc(n),yc(n),zc(n),den(n),apu(n),apv(n),apw(n))
+  allocate(u(n),v(n),w(n),p(n),sv(n),sw(n),f1(n),f2(n),f3(n))
+  allocate(gradu(3,n),gradv(3,n),gradw(3,n),gradp(3,n))
 
   open(unit=66, file=trim(flog), status='replace')
   open(unit=11, file=trim(fout), access='stream', form='unformatted', status='replace')
@@ -81,6 +90,53 @@ program ref_driver
     read(10) fi, fx, fy, fz
     call bpres(fi)
     write(11) fi
+  case ('fluxmass', 'piso_assemble', 'assemble_p', 'fluxmc')
+    read(10) x, y, z, xc, yc, zc, vol, fx, fy, fz, ar1x, ar1y, ar1z, ar2x, ar2y, ar2z, &
+             ar3x, ar3y, ar3z, den, apu, apv, apw, u, v, w, p, gradu, gradv, gradw, gradp, &
+             su, sv, sw, d1, d2, d3, bsc
+    if (nsw_ == 0) then
+      idew = nj; idns = 1; idtb = nij
+    else if (nsw_ == 1) then
+      idew = 1; idns = nij; idtb = nj
+    else
+      idew = nij; idns = nj; idtb = 1
+    end if
+    select case (trim(mode))
+    case ('fluxmass')
+      if (nsw_ == 0) call fluxmass(idew, idns, idtb, ar1x, ar1y, ar1z, fx, d1, d2, d3)
+      if (nsw_ == 1) call fluxmass(idew, idns, idtb, ar2x, ar2y, ar2z, fy, d1, d2, d3)
+      if (nsw_ == 2) call fluxmass(idew, idns, idtb, ar3x, ar3y, ar3z, fz, d1, d2, d3)
+    case ('piso_assemble')
+      if (nsw_ == 0) call piso_assemble_pressure_eq(idew, idns, idtb, ar1x, ar1y, ar1z, fx, &
+                                                    d1, d2, d3, int(alfa_), bsc)
+      if (nsw_ == 1) call piso_assemble_pressure_eq(idew, idns, idtb, ar2x, ar2y, ar2z, fy, &
+                                                    d1, d2, d3, int(alfa_), bsc)
+      if (nsw_ == 2) call piso_assemble_pressure_eq(idew, idns, idtb, ar3x, ar3y, ar3z, fz, &
+                                                    d1, d2, d3, int(alfa_), bsc)
+    case ('assemble_p')
+      if (nsw_ == 0) call assemble_pressure_eq(idew, idns, idtb, ar1x, ar1y, ar1z, fx, d1, d2, d3)
+      if (nsw_ == 1) call assemble_pressure_eq(idew, idns, idtb, ar2x, ar2y, ar2z, fy, d1, d2, d3)
+      if (nsw_ == 2) call assemble_pressure_eq(idew, idns, idtb, ar3x, ar3y, ar3z, fz, d1, d2, d3)
+    case ('fluxmc')
+      d1 = 0.0d0
+      do k = 3, nk-2
+      do i = 3, ni-2
+      do j = 3, nj-2
+        inp = lk(k) + li(i) + j
+        fmcor = 0.0d0
+        if (nsw_ == 0) call fluxmc(inp, idew, idns, idtb, fx, fmcor)
+        if (nsw_ == 1) call fluxmc(inp, idew, idns, idtb, fy, fmcor)
+        if (nsw_ == 2) call fluxmc(inp, idew, idns, idtb, fz, fmcor)
+        d1(inp) = fmcor
+      end do
+      end do
+      end do
+    end select
+    write(11) d1, d2, d3, bsc, su
+  case ('continuity')
+    read(10) f1, f2, f3
+    call continuity_sums()
+    write(11) globalContErr, sumLocalContErr
   case default
     print *, 'unknown mode ', trim(mode)
     stop 2
@@ -88,4 +144,14 @@ program ref_driver
   close(10)
   close(11)
   close(66)
+
+contains
+
+  ! the reference's continuity-error fragment, included verbatim
+  subroutine continuity_sums()
+    integer :: idew, idns, idtb, i, j, k, inp
+    cumulativeContErr = 0.0d0
+    include 'continuityErrors.h'
+  end subroutine continuity_sums
+
 end program ref_driver

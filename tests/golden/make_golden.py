@@ -2,7 +2,9 @@
 """Write the golden vectors in tests/golden/ from the reference routines.
 
 Runs oracle/_ref/ref_driver -- the reference's own sipsol.f90, cgstab_sip.f90,
-GRAD_GAUSS.f90 and bpres.f90 compiled with amdflang by `make -C oracle ref`
+GRAD_GAUSS.f90, bpres.f90, fluxmass2.f90, PISO_assemble_pressure_eq.f90,
+assemble_pressure_eq.f90, fluxmc.f90 and continuityErrors.h compiled with
+amdflang by `make -C oracle ref`
 (build container only: the reference sources are not on the GPU machines) --
 on seeded inputs, and stores inputs' seeds and the reference outputs as
 small .npz fixtures.  Inputs are regenerated by the tests from the same seeds
@@ -50,6 +52,45 @@ def geometry(dims, seed):
     return g, u
 
 
+FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
+               "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
+               "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
+
+
+def fields(dims, seed):
+    """Seeded fields for the p' assembly stencils: a jittered, stretched
+    structured mesh (nodes x,y,z; centres xc,yc,zc) and random flow data."""
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = dims
+    n = ni * nj * nk
+    k, i, j = np.meshgrid(np.arange(nk), np.arange(ni), np.arange(nj), indexing="ij")
+    jit = lambda: rng.uniform(-0.05, 0.05, (nk, ni, nj))
+    f = {}
+    f["x"] = (1.02 ** i + jit()).ravel()
+    f["y"] = (0.9 * j + jit()).ravel()
+    f["z"] = (1.1 * k + 0.1 * i + jit()).ravel()
+    f["xc"] = (1.02 ** i - 0.5 + jit()).ravel()
+    f["yc"] = (0.9 * j - 0.45 + jit()).ravel()
+    f["zc"] = (1.1 * k - 0.55 + 0.1 * i + jit()).ravel()
+    f["vol"] = rng.uniform(0.5, 1.5, n)
+    for q in ("fx", "fy", "fz"):
+        f[q] = rng.uniform(0.3, 0.7, n)
+    for d, comp in (("1", "x"), ("2", "y"), ("3", "z")):
+        for c in "xyz":
+            f["ar" + d + c] = rng.uniform(0.8, 1.2, n) if c == comp else rng.uniform(-0.2, 0.2, n)
+    f["den"] = rng.uniform(0.9, 1.1, n)
+    for q in ("apu", "apv", "apw"):
+        f[q] = rng.uniform(0.1, 1.0, n)
+    for q in ("u", "v", "w", "p"):
+        f[q] = rng.normal(size=n)
+    for q in ("gradu", "gradv", "gradw", "gradp"):
+        f[q] = rng.normal(size=3 * n)
+    f["su"], f["sv"], f["sw"] = rng.normal(size=n), rng.normal(size=n), rng.normal(size=n)
+    f["out0"] = [rng.normal(size=n) for _ in range(4)]   # acfe, acfw, fcf, bp before the call
+    f["f123"] = [rng.normal(size=n) for _ in range(3)]
+    return f
+
+
 # (name, routine, kind, dims, seed, nsw, alfa, full arrays?)
 SOLVER_CASES = [
     ("sipsol_general_12x10x9", "sipsol", "general", (12, 10, 9), 101, 25, 0.92, True),
@@ -58,6 +99,11 @@ SOLVER_CASES = [
     ("cgstab_general_12x10x9", "cgstab_sip", "general", (12, 10, 9), 104, 400, 0.92, True),
     ("cgstab_lognormal_20", "cgstab_sip", "lognormal", (20, 20, 20), 105, 400, 0.92, True),
     ("cgstab_lognormal_36", "cgstab_sip", "lognormal", (36, 36, 36), 106, 400, 0.92, False),
+]
+ASSEMBLY_CASES = [
+    ("fluxmass", (12, 10, 9), 301), ("fluxmass", (20, 17, 11), 302),
+    ("piso_assemble", (12, 10, 9), 303), ("piso_assemble_ipc0", (12, 10, 9), 304),
+    ("assemble_p", (12, 10, 9), 305), ("fluxmc", (12, 10, 9), 306),
 ]
 STENCIL_CASES = [
     ("grad_gauss_12x10x9", "grad_gauss", (12, 10, 9), 201),
@@ -87,6 +133,25 @@ def main():
             rec[f + "_sha256"] = digest(a)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
         print(f"{name}: iters {info['iters']} resor {resor:.6e}")
+    for routine, dims, seed in ASSEMBLY_CASES:
+        f = fields(dims, seed)
+        ipc = 0 if routine.endswith("_ipc0") else 1
+        mode = routine.replace("_ipc0", "")
+        rec = dict(dims=np.array(dims), seed=seed, ipc=ipc)
+        for d in range(3):
+            arrays = [f[q] for q in FIELD_NAMES] + [f["su"], f["sv"], f["sw"]] + f["out0"]
+            outs, _, _ = O.run_ref(mode, dims, arrays, nsw=d, alfa=float(ipc))
+            for q, a in zip(("acfe", "acfw", "fcf", "bp", "su"), outs):
+                rec[f"{q}_{d}"] = a
+        name = f"{routine}_{dims[0]}x{dims[1]}x{dims[2]}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print(f"{name}: written")
+    for dims, seed in (((12, 10, 9), 401), ((20, 17, 11), 402)):
+        f = fields(dims, seed)
+        outs, _, _ = O.run_ref("continuity", dims, f["f123"])
+        np.savez_compressed(os.path.join(HERE, f"continuity_{dims[0]}x{dims[1]}x{dims[2]}.npz"),
+                            dims=np.array(dims), seed=seed, sums=outs[0])
+        print(f"continuity {dims}: {outs[0]}")
     for name, routine, dims, seed in STENCIL_CASES:
         g, u = geometry(dims, seed)
         if routine == "grad_gauss":

@@ -1,0 +1,57 @@
+"""GPU p' assembly stencils against the oracle (bit-identical) and the
+continuity sums (deterministic FP64 reduction, reassociation only)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_golden as MG  # noqa: E402
+from test_oracle import oracle_assembly  # noqa: E402
+
+DIMS = [(12, 10, 9), (20, 17, 11), (40, 23, 19)]
+
+
+def fields_for(gpu, dims, seed):
+    f = MG.fields(dims, seed)
+    return f, {q: f[q] for q in MG.FIELD_NAMES}
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("routine", ["fluxmass", "piso_ipc1", "piso_ipc0", "assemble_p", "fluxmc"])
+def test_assembly_bitwise(gpu, O, dims, routine):
+    f, flds = fields_for(gpu, dims, 77)
+    with gpu.Context(*dims) as ctx:
+        for d in range(3):
+            ipc = 0 if routine == "piso_ipc0" else 1
+            name = {"piso_ipc1": "piso", "piso_ipc0": "piso"}.get(routine, routine)
+            ref = oracle_assembly(O, name, dims, f, d, ipc)
+            acfe, acfw, fcf, bp = [a.copy() for a in f["out0"]]
+            su = f["su"].copy()
+            if routine == "fluxmass":
+                ctx.fluxmass(flds, d, acfe, acfw, fcf)
+            elif routine.startswith("piso"):
+                ctx.piso_assemble_pressure_eq(flds, d, acfe, acfw, fcf, ipc, bp, su)
+            elif routine == "assemble_p":
+                ctx.assemble_pressure_eq(flds, d, acfe, acfw, fcf, f["su"], f["sv"], f["sw"])
+            else:
+                ctx.fluxmc(flds, d, acfe)
+            got = dict(acfe=acfe, acfw=acfw, fcf=fcf, bp=bp, su=su)
+            keys = ("acfe",) if routine == "fluxmc" else ("acfe", "acfw", "fcf", "bp", "su")
+            for q in keys:
+                np.testing.assert_array_equal(got[q], ref[q], err_msg=f"{routine} {q} dir {d}")
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_continuity(gpu, O, dims):
+    import ctypes as C
+    f = MG.fields(dims, 91)
+    out = (C.c_double * 2)()
+    O.lib().orc_continuity(*dims, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in f["f123"]], out)
+    with gpu.Context(*dims) as ctx:
+        got = ctx.continuity(*f["f123"])
+    assert got[1] == pytest.approx(out[1], rel=1e-13)
+    assert got[0] == pytest.approx(out[0], rel=1e-10, abs=1e-12 * out[1])

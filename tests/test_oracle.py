@@ -319,3 +319,68 @@ def test_generator_matches_published_transform(O):
         h = smix((smix(seed ^ ((stream * 0xD1B54A32D192ED03) & M)) + idx) & M)
         assert O.lib().orc_hash(seed, stream, idx) == h
         assert O.rng_u01(seed, stream, idx) == (h >> 11) * 2.0 ** -53
+
+
+# -------------------------------------------------- p' assembly (golden)
+import ctypes as _C  # noqa: E402
+
+_dp = _C.POINTER(_C.c_double)
+
+
+class _Fields(_C.Structure):
+    _fields_ = [(n, _dp) for n in MG.FIELD_NAMES]
+
+
+def oracle_assembly(O, name, dims, f, d, ipc):
+    """Run the oracle's restatement of one assembly routine for direction d."""
+    L = O.lib()
+    P = lambda a: a.ctypes.data_as(_dp)  # noqa: E731
+    fs = _Fields(*[P(f[q]) for q in MG.FIELD_NAMES])
+    ni, nj, nk = dims
+    nij = ni * nj
+    idew, idns, idtb = [(nj, 1, nij), (1, nij, nj), (nij, nj, 1)][d]
+    arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
+    fif = f[("fx", "fy", "fz")[d]]
+    acfe, acfw, fcf, bp = [a.copy() for a in f["out0"]]
+    su = f["su"].copy()
+    if name.startswith("fluxmass"):
+        L.orc_fluxmass(*dims, _C.byref(fs), idew, idns, idtb, *map(P, arv), P(fif), P(acfe),
+                       P(acfw), P(fcf))
+    elif name.startswith("piso"):
+        L.orc_piso_assemble(*dims, _C.byref(fs), idew, idns, idtb, *map(P, arv), P(fif), P(acfe),
+                            P(acfw), P(fcf), ipc, P(bp), P(su))
+    elif name.startswith("assemble_p"):
+        L.orc_assemble_p(*dims, _C.byref(fs), idew, idns, idtb, *map(P, arv), P(fif), P(acfe),
+                         P(acfw), P(fcf), P(f["su"]), P(f["sv"]), P(f["sw"]))
+    elif name.startswith("fluxmc"):
+        L.orc_fluxmc_all(*dims, _C.byref(fs), idew, idns, idtb, P(fif), P(acfe))
+    return dict(acfe=acfe, acfw=acfw, fcf=fcf, bp=bp, su=su)
+
+
+ASSEMBLY_FILES = sorted(p for p in glob.glob(os.path.join(GOLD, "*.npz"))
+                        if os.path.basename(p).startswith(("fluxmass", "piso", "assemble_p",
+                                                           "fluxmc")))
+
+
+@pytest.mark.parametrize("path", ASSEMBLY_FILES, ids=os.path.basename)
+def test_assembly_golden(O, path):
+    g = np.load(path)
+    name = os.path.basename(path)
+    dims = tuple(int(x) for x in g["dims"])
+    f = MG.fields(dims, int(g["seed"]))
+    for d in range(3):
+        out = oracle_assembly(O, name, dims, f, d, int(g["ipc"]))
+        keys = ("acfe",) if name.startswith("fluxmc") else ("acfe", "acfw", "fcf", "bp", "su")
+        for q in keys:
+            np.testing.assert_array_equal(out[q], g[f"{q}_{d}"], err_msg=f"{q} dir {d}")
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "continuity_*.npz"))),
+                         ids=os.path.basename)
+def test_continuity_golden(O, path):
+    g = np.load(path)
+    dims = tuple(int(x) for x in g["dims"])
+    f = MG.fields(dims, int(g["seed"]))
+    out = (_C.c_double * 2)()
+    O.lib().orc_continuity(*dims, *[a.ctypes.data_as(_dp) for a in f["f123"]], out)
+    assert list(out) == list(g["sums"])
