@@ -33,6 +33,30 @@
 #include <time.h>
 
 #define SMALL ((double)1e-30f)
+
+/* Summation order of the solvers' scalar products and L1 norms.  0 (default):
+ * the reference's single running sum in loop order.  1: one running sum per
+ * k-plane, planes added in order -- a different but equally valid order, used
+ * by the tests to measure how far a solve moves when only the summation order
+ * changes (the GPU sums in yet another order). */
+static int g_plane_sums = 0;
+void orc_set_plane_sums(int on) { g_plane_sums = on; }
+
+typedef struct { double tot, part; int k; } psum;
+#define PS0 {0.0, 0.0, -1}
+/* adds v at plane k; returns the sum so far (bit-identical to `s = s + v`
+ * from 0.0 when plane sums are off) */
+static inline double ps_add(psum *ps, double v, int k)
+{
+    if (!g_plane_sums) { ps->part = ps->part + v; return ps->part; }
+    if (k != ps->k) {
+        if (ps->k != -1) ps->tot = ps->tot + ps->part;
+        ps->part = 0.0;
+        ps->k = k;
+    }
+    ps->part = ps->part + v;
+    return ps->tot + ps->part;
+}
 #define S0INIT ((double)1e20f)
 
 /* 1-based (i,j,k) -> C offset; li/lk of setind.f90:34-40 */
@@ -254,12 +278,13 @@ void orc_sipsol(int ni, int nj, int nk, double *fi, orc_sys *s, double alfa,
     int n, conv = 0;
     for (n = 1; n <= nsw; ++n) {
         res1 = 0.0;
+        psum res1_ps = PS0;
         for (int k = 3; k <= nkmm; ++k)                       /* 60-75 */
             for (int i = 3; i <= nimm; ++i)
                 for (int j = 3; j <= njmm; ++j) {
                     size_t c = IX(i, j, k);
                     res[c] = resid(s, fi, c, nj, nij);
-                    res1 = res1 + fabs(res[c]);
+                    res1 = ps_add(&res1_ps, fabs(res[c]), k);
                     res[c] = (res[c] - s->bb[c] * res[c - nij] - s->bw[c] * res[c - nj] -
                               s->bs[c] * res[c - 1]) * s->bp[c];
                 }
@@ -294,12 +319,13 @@ static void pcgsip_core(int ni, int nj, int nk, double *fi, orc_sys *s,
     double *res = s->res;
     memset(res, 0, n * sizeof(double));                        /* 42-44 */
     double res0 = 0.0;
+    psum res0_ps = PS0;
     for (int k = 3; k <= nkmm; ++k)                            /* 49-59 */
         for (int i = 3; i <= nimm; ++i)
             for (int j = 3; j <= njmm; ++j) {
                 size_t c = IX(i, j, k);
                 res[c] = resid(s, fi, c, nj, nij);
-                res0 = res0 + fabs(res[c]);
+                res0 = ps_add(&res0_ps, fabs(res[c]), k);
             }
     orc_sip_factor(ni, nj, nk, 3, nkmm, s, alfa);              /* 68-87 */
     double s0 = S0INIT, resl = 0.0, resor = 0.0;
@@ -313,13 +339,14 @@ static void pcgsip_core(int ni, int nj, int nk, double *fi, orc_sys *s,
                              s->bs[c] * zk[c - 1]) * s->bp[c];
                 }
         double sk = 0.0;
+        psum sk_ps = PS0;
         for (int k = nkmm; k >= 3; --k)                        /* 111-121 */
             for (int i = nimm; i >= 3; --i)
                 for (int j = njmm; j >= 3; --j) {
                     size_t c = IX(i, j, k);
                     zk[c] = zk[c] - s->bn[c] * zk[c + 1] - s->be[c] * zk[c + nj] -
                             s->bt[c] * zk[c + nij];
-                    sk = sk + res[c] * zk[c];
+                    sk = ps_add(&sk_ps, res[c] * zk[c], k);
                 }
         double bet = sk / s0;                                  /* 127 */
         for (int k = 3; k <= nkmm; ++k)                        /* 131-138 */
@@ -329,6 +356,7 @@ static void pcgsip_core(int ni, int nj, int nk, double *fi, orc_sys *s,
                     pk[c] = zk[c] + bet * pk[c];
                 }
         double pkapk = 0.0;
+        psum pkapk_ps = PS0;
         for (int k = 3; k <= nkmm; ++k)                        /* 142-153 */
             for (int i = 3; i <= nimm; ++i)
                 for (int j = 3; j <= njmm; ++j) {
@@ -336,17 +364,18 @@ static void pcgsip_core(int ni, int nj, int nk, double *fi, orc_sys *s,
                     zk[c] = s->ap[c] * pk[c] - s->ae[c] * pk[c + nj] - s->aw[c] * pk[c - nj] -
                             s->an[c] * pk[c + 1] - s->as[c] * pk[c - 1] -
                             s->at[c] * pk[c + nij] - s->ab[c] * pk[c - nij];
-                    pkapk = pkapk + pk[c] * zk[c];
+                    pkapk = ps_add(&pkapk_ps, pk[c] * zk[c], k);
                 }
         double alf = sk / pkapk;                               /* 157 */
         resl = 0.0;
+        psum resl_ps = PS0;
         for (int k = 3; k <= nkmm; ++k)                        /* 161-175 */
             for (int i = 3; i <= nimm; ++i)
                 for (int j = 3; j <= njmm; ++j) {
                     size_t c = IX(i, j, k);
                     fi[c] = fi[c] + alf * pk[c];
                     res[c] = res[c] - alf * zk[c];
-                    resl = resl + fabs(res[c]);
+                    resl = ps_add(&resl_ps, fabs(res[c]), k);
                 }
         s0 = sk;                                               /* 181 */
         if (l == 1) resor = res0;                              /* 185 */
@@ -391,12 +420,13 @@ void orc_iccg(int ni, int nj, int nk, double *fi, orc_sys *s, double sor,
     double *res = s->res;
     memset(res, 0, n * sizeof(double));                        /* 45-48 */
     double res0 = 0.0;
+    psum res0_ps = PS0;
     for (int k = 2; k <= nkm; ++k)                             /* 52-63 */
         for (int i = 3; i <= nimm; ++i)
             for (int j = 3; j <= njmm; ++j) {
                 size_t c = IX(i, j, k);
                 res[c] = resid(s, fi, c, nj, nij);
-                res0 = res0 + fabs(res[c]);
+                res0 = ps_add(&res0_ps, fabs(res[c]), k);
             }
     for (int k = 2; k <= nkm; ++k)                             /* 73-81 */
         for (int i = 3; i <= nimm; ++i)
@@ -423,13 +453,14 @@ void orc_iccg(int ni, int nj, int nk, double *fi, orc_sys *s, double sor,
                     zk[c] = zk[c] / (d[c] + SMALL);
                 }
         double sk = 0.0;
+        psum sk_ps = PS0;
         for (int k = nkm; k >= 2; --k)                         /* 113-123 */
             for (int i = nimm; i >= 3; --i)
                 for (int j = njmm; j >= 3; --j) {
                     size_t c = IX(i, j, k);
                     zk[c] = (zk[c] + s->ae[c] * zk[c + nj] + s->an[c] * zk[c + 1] +
                              s->at[c] * zk[c + nij]) * d[c];
-                    sk = sk + res[c] * zk[c];
+                    sk = ps_add(&sk_ps, res[c] * zk[c], k);
                 }
         double bet = sk / s0;                                  /* 129 */
         for (int k = 2; k <= nkm; ++k)                         /* 133-140 */
@@ -439,6 +470,7 @@ void orc_iccg(int ni, int nj, int nk, double *fi, orc_sys *s, double sor,
                     pk[c] = zk[c] + bet * pk[c];
                 }
         double pkapk = 0.0;
+        psum pkapk_ps = PS0;
         for (int k = 2; k <= nkm; ++k)                         /* 144-155 */
             for (int i = 3; i <= nimm; ++i)
                 for (int j = 3; j <= njmm; ++j) {
@@ -446,17 +478,18 @@ void orc_iccg(int ni, int nj, int nk, double *fi, orc_sys *s, double sor,
                     zk[c] = s->ap[c] * pk[c] - s->ae[c] * pk[c + nj] - s->aw[c] * pk[c - nj] -
                             s->an[c] * pk[c + 1] - s->as[c] * pk[c - 1] -
                             s->at[c] * pk[c + nij] - s->ab[c] * pk[c - nij];
-                    pkapk = pkapk + pk[c] * zk[c];
+                    pkapk = ps_add(&pkapk_ps, pk[c] * zk[c], k);
                 }
         double alf = sk / pkapk;                               /* 160 */
         resl = 0.0;
+        psum resl_ps = PS0;
         for (int k = 2; k <= nkm; ++k)                         /* 164-174 */
             for (int i = 3; i <= nimm; ++i)
                 for (int j = 3; j <= njmm; ++j) {
                     size_t c = IX(i, j, k);
                     fi[c] = fi[c] + alf * pk[c];
                     res[c] = res[c] - alf * zk[c];
-                    resl = resl + fabs(res[c]);
+                    resl = ps_add(&resl_ps, fabs(res[c]), k);
                 }
         s0 = sk;                                               /* 178 */
         if (l == 1) resor = res0;                              /* 182 */
@@ -488,10 +521,11 @@ void orc_cgstab_sip(int ni, int nj, int nk, double *fi, orc_sys *s,
 #define LOOP_F for (int k = 2; k <= nkm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j)
 #define LOOP_B for (int k = nkm; k >= 2; --k) for (int i = nimm; i >= 3; --i) for (int j = njmm; j >= 3; --j)
     double res0 = 0.0;
+    psum res0_ps = PS0;
     LOOP_F {                                                   /* 44-55 */
         size_t c = IX(i, j, k);
         res[c] = resid(s, fi, c, nj, nij);
-        res0 = res0 + fabs(res[c]);
+        res0 = ps_add(&res0_ps, fabs(res[c]), k);
     }
     orc_sip_factor(ni, nj, nk, 2, nkm, s, alfa);               /* 60-79 */
     LOOP_F {                                                   /* 83-94 */
@@ -502,8 +536,9 @@ void orc_cgstab_sip(int ni, int nj, int nk, double *fi, orc_sys *s,
     double alf = 1.0, beto = 1.0, gam = 1.0, resl = 0.0, resor = 0.0;
     int l, conv = 0;
     for (l = 1; l <= nsw; ++l) {
-        double bet = 0.0;                                      /* 106-116 */
-        LOOP_F { size_t c = IX(i, j, k); bet = bet + res[c] * reso[c]; }
+        double bet = 0.0;
+        psum bet_ps = PS0;                                      /* 106-116 */
+        LOOP_F { size_t c = IX(i, j, k); bet = ps_add(&bet_ps, res[c] * reso[c], k); }
         double om = bet * gam / (alf * beto + SMALL);
         beto = bet;
         LOOP_F {                                               /* 120-127 */
@@ -525,8 +560,9 @@ void orc_cgstab_sip(int ni, int nj, int nk, double *fi, orc_sys *s,
                     s->an[c] * zk[c + 1] - s->as[c] * zk[c - 1] - s->at[c] * zk[c + nij] -
                     s->ab[c] * zk[c - nij];
         }
-        double ukreso = 0.0;                                   /* 170-179 */
-        LOOP_F { size_t c = IX(i, j, k); ukreso = ukreso + uk[c] * reso[c]; }
+        double ukreso = 0.0;
+        psum ukreso_ps = PS0;                                   /* 170-179 */
+        LOOP_F { size_t c = IX(i, j, k); ukreso = ps_add(&ukreso_ps, uk[c] * reso[c], k); }
         gam = bet / ukreso;
         LOOP_F {                                               /* 183-191 */
             size_t c = IX(i, j, k);
@@ -548,19 +584,21 @@ void orc_cgstab_sip(int ni, int nj, int nk, double *fi, orc_sys *s,
                     s->an[c] * zk[c + 1] - s->as[c] * zk[c - 1] - s->at[c] * zk[c + nij] -
                     s->ab[c] * zk[c - nij];
         }
-        double vres = 0.0, vv = 0.0;                           /* 233-245 */
+        double vres = 0.0, vv = 0.0;
+        psum vres_ps = PS0, vv_ps = PS0;                           /* 233-245 */
         LOOP_F {
             size_t c = IX(i, j, k);
-            vres = vres + vk[c] * res[c];
-            vv = vv + vk[c] * vk[c];
+            vres = ps_add(&vres_ps, vk[c] * res[c], k);
+            vv = ps_add(&vv_ps, vk[c] * vk[c], k);
         }
         alf = vres / (vv + SMALL);
-        resl = 0.0;                                            /* 249-259 */
+        resl = 0.0;
+        psum resl_ps = PS0;                                            /* 249-259 */
         LOOP_F {
             size_t c = IX(i, j, k);
             fi[c] = fi[c] + alf * zk[c];
             res[c] = res[c] - alf * vk[c];
-            resl = resl + fabs(res[c]);
+            resl = ps_add(&resl_ps, fabs(res[c]), k);
         }
         if (l == 1) resor = res0;                              /* 263 */
         double rsm = resl / (resor + SMALL);

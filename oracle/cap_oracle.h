@@ -74,6 +74,10 @@ void orc_grad_gauss(int ni, int nj, int nk, const double *u, const double *fx,
 void orc_bpres(int ni, int nj, int nk, double *p, const double *fx,
                const double *fy, const double *fz);
 
+/* 1: scalar products / norms summed per k-plane (tests measure the solvers'
+ * sensitivity to summation order with it); 0: the reference's order */
+void orc_set_plane_sums(int on);
+
 /* timing helper for the CPU baseline: n PCG-SIP iterations from fi */
 double orc_pcgsip_time_iters(int ni, int nj, int nk, double *fi, orc_sys *s,
                              double alfa, int n_iters);

@@ -74,6 +74,7 @@ def lib():
         L.orc_bpres.argtypes = [i, i, i, dp, dp, dp, dp]
         L.orc_pcgsip_time_iters.argtypes = [i, i, i, dp, C.POINTER(_Sys), d, i]
         L.orc_pcgsip_time_iters.restype = d
+        L.orc_set_plane_sums.argtypes = [i]
         _lib = L
     return _lib
 
@@ -147,6 +148,28 @@ def cgstab_sip(s, fi, alfa=ALFA, sor=1e-8, nsw=1000):
 
 def iccg(s, fi, sor=1e-8, nsw=1000):
     return _solve(lib().orc_iccg, s, fi, sor, nsw)
+
+
+class plane_sums:
+    """Context manager: the oracle's scalar products and norms are summed per
+    k-plane instead of in one running sum (another valid summation order)."""
+
+    def __enter__(self):
+        lib().orc_set_plane_sums(1)
+
+    def __exit__(self, *exc):
+        lib().orc_set_plane_sums(0)
+
+
+def order_sensitivity(solver, s, fi, **kw):
+    """max|fi_a - fi_b| / max|fi_a| between the reference order and per-plane
+    sums: how far the solve moves when only the summation order changes."""
+    s1, f1 = s.copy(), fi.copy()
+    solver(s1, f1, **kw)
+    s2, f2 = s.copy(), fi.copy()
+    with plane_sums():
+        solver(s2, f2, **kw)
+    return float(np.max(np.abs(f1 - f2)) / max(np.max(np.abs(f1)), 1e-300))
 
 
 def sip_factor(s, k_lo, k_hi, alfa=ALFA):

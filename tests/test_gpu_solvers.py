@@ -7,7 +7,11 @@ restatement of the reference routine on the same inputs.
 Bar: the SIP/IC factorisations, sweeps and stencils are bit-identical (same
 operations in the same order, contraction off); the Krylov solvers differ
 only through the order of their FP64 reductions, so iteration counts agree to
-+/-1 and, when they agree exactly, fi agrees to 1e-10 relative.
++/-1 and, when they agree exactly, fi agrees to 1e-10 relative -- or, for
+long solves, to 4x the solver's own sensitivity to summation order, measured
+on the CPU by re-running the oracle with per-plane sums (CG amplifies a
+1e-16 change in a scalar product to 1e-9..1e-8 over a few hundred
+iterations of these ill-conditioned systems, on the CPU alone).
 """
 import numpy as np
 import pytest
@@ -42,16 +46,18 @@ def rel(a, b):
     return np.max(np.abs(a - b)) / (den if den > 0 else 1.0)
 
 
-def check_solve(name, rep, rep_ref, fi, fi_ref, coef, s_ref, bitwise_fi=False):
+def check_solve(name, rep, rep_ref, fi, fi_ref, coef, s_ref, bitwise_fi=False, sens=0.0):
+    tol = max(1e-10, 4.0 * sens)
     print(f"{name}: gpu iters {rep['iters']} cpu iters {rep_ref['iters']} "
-          f"fi rel {rel(fi, fi_ref):.2e} resor {rep['resor']:.6e}/{rep_ref['resor']:.6e}")
+          f"fi rel {rel(fi, fi_ref):.2e} (tol {tol:.1e}, cpu order sensitivity {sens:.1e}) "
+          f"resor {rep['resor']:.6e}/{rep_ref['resor']:.6e}")
     assert abs(rep["iters"] - rep_ref["iters"]) <= 1
     assert rep["resor"] == pytest.approx(rep_ref["resor"], rel=1e-12)
     if rep["iters"] == rep_ref["iters"]:
         if bitwise_fi:
             np.testing.assert_array_equal(fi, fi_ref)
         else:
-            assert rel(fi, fi_ref) <= 1e-10
+            assert rel(fi, fi_ref) <= tol
     else:
         assert rel(fi, fi_ref) <= 1e-6
     for f in B_FIELDS:
@@ -70,7 +76,8 @@ def test_pcgsip_lognormal(gpu, O, dims):
     with gpu.Context(*dims) as ctx:
         rep = ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=2000)
     assert rep["converged"] and rep_ref["converged"]
-    check_solve("pcgsip", rep, rep_ref, fi, fi_ref, coef, s_ref)
+    sens = O.order_sensitivity(O.pcgsip, s, fi0, alfa=0.0, sor=1e-8, nsw=2000)
+    check_solve("pcgsip", rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens)
     r0 = np.max(np.abs(s.su))
     assert np.max(np.abs(coef.res - s_ref.res)) <= 1e-9 * r0
 
@@ -123,7 +130,8 @@ def test_iccg_lognormal(gpu, O, dims):
     with gpu.Context(*dims) as ctx:
         rep = ctx.iccg(fi, coef, sor=1e-8, nsw=2000)
     assert rep["converged"]
-    check_solve("iccg", rep, rep_ref, fi, fi_ref, coef, s_ref)
+    sens = O.order_sensitivity(O.iccg, s, fi0, sor=1e-8, nsw=2000)
+    check_solve("iccg", rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens)
 
 
 @pytest.mark.parametrize("dims", SMALL_DIMS)
@@ -138,7 +146,8 @@ def test_cgstab_sip(gpu, O, dims, kind):
     coef, fi = coef_of(gpu, s), fi0.copy()
     with gpu.Context(*dims) as ctx:
         rep = ctx.cgstab_sip(fi, coef, sor=1e-8, nsw=400)
-    check_solve("cgstab_sip/" + kind, rep, rep_ref, fi, fi_ref, coef, s_ref)
+    sens = O.order_sensitivity(O.cgstab_sip, s, fi0, sor=1e-8, nsw=400)
+    check_solve("cgstab_sip/" + kind, rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens)
 
 
 def test_device_pointers(gpu, O):
@@ -153,7 +162,8 @@ def test_device_pointers(gpu, O):
     with gpu.Context(*dims, device_pointers=True) as ctx:
         rep = ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=2000)
     assert abs(rep["iters"] - rep_ref["iters"]) <= 1
-    assert rel(fi.cpu().numpy(), fi_ref) <= 1e-10 or rep["iters"] != rep_ref["iters"]
+    sens = O.order_sensitivity(O.pcgsip, s, fi0, alfa=0.0, sor=1e-8, nsw=2000)
+    assert rel(fi.cpu().numpy(), fi_ref) <= max(1e-10, 4 * sens) or rep["iters"] != rep_ref["iters"]
 
 
 @pytest.mark.parametrize("dims", [(36, 36, 36), (52, 70, 30)])
