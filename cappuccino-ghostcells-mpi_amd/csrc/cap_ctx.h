@@ -45,11 +45,11 @@ struct cap_ctx {
     size_t fcap;
     double *G[CAP_NSTAGE];      // stencil staging (reference layout)
     size_t Gsz[CAP_NSTAGE];
-    uint64_t *progress;
-    int ntiles_cap;
+    void *msg;                  // wavefront message slots
+    size_t msg_bytes;
     unsigned long long *claim;
     unsigned long long claim_base;
-    uint64_t epoch;
+    int wave_lag;                 // steps a wavefront tile keeps behind its producer
     cap::DevState *st;
     cap::DevState *st2;            // inner solve of the mixed-precision solver
     cap::DevState *st_host;

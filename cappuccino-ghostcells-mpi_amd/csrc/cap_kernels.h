@@ -110,10 +110,13 @@ __device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double 
 {
     __shared__ double sh[NV][BLOCK / 64];
     __shared__ int last;
+    // only the first BLOCK threads take part (a larger block may call this
+    // with every thread; the others only join the barriers)
     const int w = threadIdx.x >> 6;
+    const bool part = threadIdx.x < BLOCK;
     for (int q = 0; q < NV; ++q) {
         double x = wave_sum(v[q]);
-        if ((threadIdx.x & 63) == 0) sh[q][w] = x;
+        if (part && (threadIdx.x & 63) == 0) sh[q][w] = x;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -128,14 +131,14 @@ __device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double 
     __threadfence();
     double acc[NV];
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (int s = threadIdx.x; s < nslots; s += BLOCK)
+    for (int s = threadIdx.x; part && s < nslots; s += BLOCK)
         for (int q = 0; q < NV; ++q)
             acc[q] += __hip_atomic_load(&partials[(size_t)s * NV + q], __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     for (int q = 0; q < NV; ++q) {
         double x = wave_sum(acc[q]);
-        if ((threadIdx.x & 63) == 0) sh[q][w] = x;
+        if (part && (threadIdx.x & 63) == 0) sh[q][w] = x;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -166,7 +169,7 @@ __device__ __forceinline__ Cell cell_of(const Lay &L, size_t ts, int lane)
     c.kk = posmod(c.slot - c.a - c.b, L.S);
     c.e = ts * TCOL + lane;
     const int ci = c.ti * TW + c.b, cj = c.tj * TW + c.a;
-    c.in = col_interior(L, ci, cj) && c.kk >= 1 && c.kk <= L.nks;
+    c.in = col_valid(L, ci, cj) && c.kk >= 1 && c.kk <= L.nks;
     return c;
 }
 

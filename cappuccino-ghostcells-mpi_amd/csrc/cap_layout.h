@@ -9,20 +9,19 @@
 // zk, pk, ...) and its copies of the coefficients in a layout built for the
 // wavefront:
 //
-//  * columns (i,j) of the solve box plus a one-column halo ring are grouped
-//    into tiles of 16 (j) x 16 (i) columns;
+//  * the columns (i,j) of the solve box are grouped into tiles of 16 (j) x
+//    16 (i) columns;
 //  * inside a tile, element (slot s, b, a) holds column (i0+b, j0+a) at the
-//    k-plane kk = (s - a - b) mod S, where S = nks + 2 k-planes (solve range
-//    plus one halo plane at each end);
-//  * element address = ((ti*nTJ + tj)*S + s)*256 + b*16 + a.
+//    k-plane kk = (s - a - b) mod S, where S = nks + 2 k-planes (the solve
+//    range plus one halo plane at each end);
+//  * element address = ((ti*nTJ + tj)*S + s)*256 + b*16 + a;
+//  * the four halo columns-rows around the solve box (i = ia-1, i = ib+1,
+//    j = ja-1, j = jb+1; all S planes) live in four "ring" arrays after the
+//    tiles, so a 256^3 box is exactly 16 x 16 tiles.
 //
 // A wavefront lane (a,b) handles its column; at step t it works on
 // kk = t - a - b, so every lane of the tile touches slot t mod S at step t:
-// 4 waves x 64 lanes read 256 contiguous doubles.  The recursion's j-1 value
-// comes from lane a-1 one step earlier, i-1 from lane b-1 one step earlier,
-// k-1 from the lane itself one step earlier -- the reference's dependency
-// order, so each value is computed from the same operands as the sequential
-// loop.
+// 4 waves x 64 lanes read 256 contiguous doubles.
 //
 // Everything here is plain index arithmetic shared by the GPU kernels and the
 // host-side schedule check.
@@ -45,10 +44,11 @@ struct Lay {
     int ni, nj, nk, nij;      // padded extents of the reference arrays
     int ia, ja, ka;           // first solve cell (1-based)
     int nis, njs, nks;        // solve-box extents
-    int nci, ncj;             // column space incl. halo ring: nis+2, njs+2
     int nTI, nTJ;             // tiles along i and j
     int S;                    // slots per tile = nks + 2
     int ntiles;
+    size_t nmain;             // tile elements
+    size_t ring[4];           // ring arrays: i=ia-1, i=ib+1 (len njs), j=ja-1, j=jb+1 (len nis)
     size_t n;                 // elements per tiled vector
 };
 
@@ -58,12 +58,16 @@ CAP_HD Lay make_lay(int ni, int nj, int nk, int ia, int ib, int ja, int jb, int 
     L.ni = ni; L.nj = nj; L.nk = nk; L.nij = ni * nj;
     L.ia = ia; L.ja = ja; L.ka = ka;
     L.nis = ib - ia + 1; L.njs = jb - ja + 1; L.nks = kb - ka + 1;
-    L.nci = L.nis + 2; L.ncj = L.njs + 2;
-    L.nTI = (L.nci + TW - 1) / TW;
-    L.nTJ = (L.ncj + TW - 1) / TW;
+    L.nTI = (L.nis + TW - 1) / TW;
+    L.nTJ = (L.njs + TW - 1) / TW;
     L.S = L.nks + 2;
     L.ntiles = L.nTI * L.nTJ;
-    L.n = (size_t)L.ntiles * (size_t)L.S * TCOL;
+    L.nmain = (size_t)L.ntiles * (size_t)L.S * TCOL;
+    L.ring[0] = L.nmain;
+    L.ring[1] = L.ring[0] + (size_t)L.S * L.njs;
+    L.ring[2] = L.ring[1] + (size_t)L.S * L.njs;
+    L.ring[3] = L.ring[2] + (size_t)L.S * L.nis;
+    L.n = L.ring[3] + (size_t)L.S * L.nis;
     return L;
 }
 
@@ -79,9 +83,22 @@ CAP_HD size_t elem(const Lay &L, int ti, int tj, int slot, int b, int a)
     return tile_base(L, ti, tj) + (size_t)slot * TCOL + b * TW + a;
 }
 
-// element of column (ci, cj) at k-plane kk (all 0-based in the column space)
+// ring element: side 0 (i=ia-1) / 1 (i=ib+1) indexed by cj, side 2 (j=ja-1) /
+// 3 (j=jb+1) indexed by ci
+CAP_HD size_t ring_elem(const Lay &L, int side, int idx, int kk)
+{
+    const int len = side < 2 ? L.njs : L.nis;
+    return L.ring[side] + (size_t)kk * len + idx;
+}
+
+// element of solve column (ci, cj) (0-based) at plane kk; ring columns
+// ci = -1 / nis or cj = -1 / njs map to the ring arrays
 CAP_HD size_t elem_of(const Lay &L, int ci, int cj, int kk)
 {
+    if (ci < 0) return ring_elem(L, 0, cj, kk);
+    if (ci >= L.nis) return ring_elem(L, 1, cj, kk);
+    if (cj < 0) return ring_elem(L, 2, ci, kk);
+    if (cj >= L.njs) return ring_elem(L, 3, ci, kk);
     int ti = ci / TW, b = ci % TW, tj = cj / TW, a = cj % TW;
     return elem(L, ti, tj, posmod(kk + a + b, L.S), b, a);
 }
@@ -89,17 +106,13 @@ CAP_HD size_t elem_of(const Lay &L, int ci, int cj, int kk)
 // offset into the reference array of column (ci, cj), plane kk
 CAP_HD size_t std_index(const Lay &L, int ci, int cj, int kk)
 {
-    int i = L.ia - 1 + ci, j = L.ja - 1 + cj, k = L.ka - 1 + kk;
+    int i = L.ia + ci, j = L.ja + cj, k = L.ka - 1 + kk;
     return (size_t)(k - 1) * L.nij + (size_t)(i - 1) * L.nj + (size_t)(j - 1);
 }
 
-CAP_HD bool col_valid(const Lay &L, int ci, int cj) { return ci < L.nci && cj < L.ncj; }
-CAP_HD bool col_interior(const Lay &L, int ci, int cj)
-{
-    return ci >= 1 && ci <= L.nis && cj >= 1 && cj <= L.njs;
-}
+CAP_HD bool col_valid(const Lay &L, int ci, int cj) { return ci < L.nis && cj < L.njs; }
 
-// Neighbour elements of interior cell (ti,tj,slot,b,a) at plane kk.  The
+// Neighbour elements of solve cell (ti,tj,slot,b,a) at plane kk.  The
 // reference's names: east = i+1 (b+1), west = i-1, north = j+1 (a+1),
 // south = j-1, top = k+1, bottom = k-1.
 struct Nb { size_t e, w, n, s, t, bt; };
@@ -109,20 +122,41 @@ CAP_HD Nb neighbours(const Lay &L, int ti, int tj, int slot, int b, int a, int k
     const int S = L.S;
     const int sp = slot + 1 == S ? 0 : slot + 1;
     const int sm = slot == 0 ? S - 1 : slot - 1;
+    const int ci = ti * TW + b, cj = tj * TW + a;
     Nb r;
     r.t = elem(L, ti, tj, sp, b, a);
     r.bt = elem(L, ti, tj, sm, b, a);
-    r.n = a < TW - 1 ? elem(L, ti, tj, sp, b, a + 1) : elem(L, ti, tj + 1, posmod(kk + b, S), b, 0);
-    r.s = a > 0 ? elem(L, ti, tj, sm, b, a - 1) : elem(L, ti, tj - 1, posmod(kk + TW - 1 + b, S), b, TW - 1);
-    r.e = b < TW - 1 ? elem(L, ti, tj, sp, b + 1, a) : elem(L, ti + 1, tj, posmod(kk + a, S), 0, a);
-    r.w = b > 0 ? elem(L, ti, tj, sm, b - 1, a) : elem(L, ti - 1, tj, posmod(kk + a + TW - 1, S), TW - 1, a);
+    r.n = cj + 1 == L.njs ? ring_elem(L, 3, ci, kk)
+          : a < TW - 1    ? elem(L, ti, tj, sp, b, a + 1)
+                          : elem(L, ti, tj + 1, posmod(kk + b, S), b, 0);
+    r.s = cj == 0      ? ring_elem(L, 2, ci, kk)
+          : a > 0      ? elem(L, ti, tj, sm, b, a - 1)
+                       : elem(L, ti, tj - 1, posmod(kk + TW - 1 + b, S), b, TW - 1);
+    r.e = ci + 1 == L.nis ? ring_elem(L, 1, cj, kk)
+          : b < TW - 1    ? elem(L, ti, tj, sp, b + 1, a)
+                          : elem(L, ti + 1, tj, posmod(kk + a, S), 0, a);
+    r.w = ci == 0      ? ring_elem(L, 0, cj, kk)
+          : b > 0      ? elem(L, ti, tj, sm, b - 1, a)
+                       : elem(L, ti - 1, tj, posmod(kk + a + TW - 1, S), TW - 1, a);
     return r;
 }
 
-// Decompose an element index into (ci, cj, kk); returns false for slots of
-// columns beyond the ring.
+// Decompose an element index into (ci, cj, kk) (ring columns included);
+// returns false for slots of tile columns beyond the solve box.
 CAP_HD bool decode(const Lay &L, size_t e, int &ci, int &cj, int &kk)
 {
+    if (e >= L.nmain) {
+        int side = e >= L.ring[3] ? 3 : e >= L.ring[2] ? 2 : e >= L.ring[1] ? 1 : 0;
+        const int len = side < 2 ? L.njs : L.nis;
+        const size_t off = e - L.ring[side];
+        kk = (int)(off / len);
+        const int idx = (int)(off % len);
+        if (side == 0) { ci = -1; cj = idx; }
+        else if (side == 1) { ci = L.nis; cj = idx; }
+        else if (side == 2) { ci = idx; cj = -1; }
+        else { ci = idx; cj = L.njs; }
+        return true;
+    }
     size_t tslot = e / TCOL;
     int lane = (int)(e % TCOL);
     int b = lane / TW, a = lane % TW;

@@ -24,16 +24,13 @@
 
 using namespace cap;
 
-#define WAVE_G 8
-#define SPIN_LIMIT (1 << 22)
+#define SPIN_LIMIT (1 << 20)
 
 // ============================================================ layout moves
 __global__ __launch_bounds__(BLOCK) void k_gather(Lay L, const double *__restrict__ src,
                                                   double *__restrict__ dst)
 {
-    const size_t nts = (size_t)L.ntiles * L.S;
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        size_t e = ts * TCOL + threadIdx.x;
+    for (size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x; e < L.n; e += (size_t)gridDim.x * BLOCK) {
         int ci, cj, kk;
         double v = 0.0;
         if (decode(L, e, ci, cj, kk)) v = src[std_index(L, ci, cj, kk)];
@@ -194,16 +191,16 @@ struct WaveIO {
     const V *in[Op::NIN];
     V *dep[Op::ND];
     V *xo[Op::NX > 0 ? Op::NX : 1];
+    V *msg;                         // message slots (filled with the sentinel)
 };
 
 struct WaveSync {
-    uint64_t *progress;             // per tile: (epoch << 20) | steps completed
     unsigned long long *claim;      // tile claim counter
     unsigned long long claim_base;
-    uint64_t epoch;
     DevState *st;
     double *partials;
     unsigned *ticket;
+    int lag;                        // steps a consumer tile keeps behind its producer
 };
 
 template <class V>
@@ -217,32 +214,69 @@ __device__ __forceinline__ void coh_store(V *p, V v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ bool wait_ge(uint64_t *p, uint64_t target)
+__device__ __forceinline__ bool msg_empty(double v) { return (uint64_t)__double_as_longlong(v) == MSG_EMPTY; }
+__device__ __forceinline__ bool msg_empty(float v) { return (uint32_t)__float_as_uint(v) == MSG_EMPTY_F32; }
+
+template <class V> struct MsgEmpty;
+template <> struct MsgEmpty<double> {
+    static __device__ __forceinline__ double get() { return __longlong_as_double((long long)MSG_EMPTY); }
+};
+template <> struct MsgEmpty<float> {
+    static __device__ __forceinline__ float get() { return __uint_as_float(MSG_EMPTY_F32); }
+};
+
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_msg_fill(V *p, size_t n)
 {
-    for (int it = 0; it < SPIN_LIMIT; ++it) {
-        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return false;
+    const V s = MsgEmpty<V>::get();
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK)
+        p[i] = s;
 }
 
-// One workgroup = one tile (256 lanes = 16x16 columns), marching through the
-// tile's steps in groups of WAVE_G.  Per group: wait until the (one or two)
-// upstream tiles have published enough steps, fetch their boundary values
-// (device-coherent loads), prefetch the next group's inputs, run the steps
-// (one LDS exchange + barrier per step), then publish progress.  The tile a
-// workgroup works on is claimed from a counter in a topological order, so a
-// workgroup only ever waits for tiles already held by running workgroups.
+// spin until a message slot has been written (bounded: on expiry the solve is
+// flagged and ends with an error, every wave still runs to completion; once
+// a solve is flagged, waits give up at once)
+template <class V>
+__device__ __forceinline__ V msg_wait(const V *p, V v, DevState *st)
+{
+    for (int it = 0; it < SPIN_LIMIT && msg_empty(v); ++it) {
+        if (__hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return v;
+        __builtin_amdgcn_s_sleep(1);
+        v = coh_load(p);
+    }
+    if (msg_empty(v)) {
+        st->err = CAP_ERR_TIMEOUT;
+        st->done = 1;
+    }
+    return v;
+}
+
+// One workgroup = one tile (16x16 columns, all planes) marching through the
+// tile's steps, in five waves:
+//  * waves 0-3 (256 lanes, one per column) compute.  Their inputs for step
+//    t+U are loaded at step t (a register ring of depth U), so loads stay in
+//    flight across steps; values cross lanes through LDS, one barrier a step.
+//  * wave 4 carries the values that come from outside the tile: it reads the
+//    neighbouring tiles' messages (or the ring columns at the box edge) DM
+//    steps ahead and hands them to the compute waves through LDS, so a late
+//    message holds up only this wave's memory counter.  When a message is
+//    late it also waits until the producer is sy.lag steps further on, so
+//    that later messages are found written.
+// Tiles are claimed from a counter in a topological order, so a workgroup only
+// ever waits for tiles held by running workgroups.
+constexpr int WAVE_BLOCK = BLOCK + 64;
+
 template <class Op, bool FWD, int FIN>
-__global__ __launch_bounds__(BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io, WaveSync sy)
+__global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io, WaveSync sy)
 {
     using V = typename Op::T;
-    constexpr int ND = Op::ND, NIN = Op::NIN, G = WAVE_G;
+    constexpr int ND = Op::ND, NIN = Op::NIN, U = ND == 1 && NIN <= 5 ? 16 : 8;
+    constexpr int DM = 4;   // messages are read DM steps ahead of use
     constexpr int NXS = Op::NX > 0 ? Op::NX : 1, NRS = Op::NR > 0 ? Op::NR : 1;
     __shared__ V lds[2][ND][TCOL];
+    __shared__ V mbuf[2][ND][2][TW];   // [step parity][q][j-side by b / i-side by a]
     __shared__ int sh_tile, sh_flag;
-    const int lane = threadIdx.x;
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         const unsigned long long c = atomicAdd(sy.claim, 1ull) - sy.claim_base;
         sh_flag = sy.st->done;
         if (c >= (unsigned long long)L.ntiles) {   // launch bookkeeping broken
@@ -254,147 +288,196 @@ __global__ __launch_bounds__(BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io, Wav
     }
     __syncthreads();
     if (sh_flag) return;
-    const int claim = sh_tile;
     int ti, tj;
-    claim_tile<FWD>(L, claim, ti, tj);
-    LaneInfo li;
-    lane_setup<FWD>(L, ti, tj, lane, li);
-    const size_t tb = tile_base(L, ti, tj);
+    claim_tile<FWD>(L, sh_tile, ti, tj);
+    ti = __builtin_amdgcn_readfirstlane(ti);
+    tj = __builtin_amdgcn_readfirstlane(tj);
+    const int tile = ti * L.nTJ + tj;
     const int NST = nsteps(L);
-    const int ngroups = (NST + G - 1) / G;
-    const bool has_pi = FWD ? ti > 0 : ti < L.nTI - 1;
-    const bool has_pj = FWD ? tj > 0 : tj < L.nTJ - 1;
-    const int pid_i = (FWD ? ti - 1 : ti + 1) * L.nTJ + tj;
-    const int pid_j = ti * L.nTJ + (FWD ? tj - 1 : tj + 1);
-    const int my_id = ti * L.nTJ + tj;
-    const uint64_t ep = sy.epoch << 20;
-    const bool bnd = is_boundary<FWD>(li);
-
-    V bufA[G][NIN], bufB[G][NIN];
-    V prev[ND];
+    const int last_t = NST - 1;
+    // steps padded to a multiple of both loop strides (padded steps are idle)
+    constexpr int UM = U > DM ? U : DM;
+    static_assert(UM % U == 0 && UM % DM == 0, "strides must divide each other");
+    const int NSTP = (NST + UM - 1) / UM * UM;
+    V *const msgp = io.msg;
     double red[NRS];
-    for (int q = 0; q < ND; ++q) prev[q] = V(0);
     for (int q = 0; q < NRS; ++q) red[q] = 0.0;
 
-    // Branch-free prefetch: every lane issues the same loads (all addresses
-    // lie inside the tile); a LOAD cell takes its first ND slots from the
-    // dependency arrays instead of the inputs.  Keeping the loads
-    // unconditional lets them stay in flight across the steps.
-    auto prefetch = [&](int g, V (&buf)[G][NIN]) {
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == BLOCK / 64) {
+        // ------------------------------------------ wave 4: the tile's inputs
+        // lanes  0-15: j-side input of row b = idx (chain-start lane of the row)
+        //       16-31: i-side input of column a = idx
+        const int c = threadIdx.x - BLOCK;
+        const int grp = c >> 4, idx = c & (TW - 1), side = grp & 1;
+        const bool in = grp < 2;
+        const int a_last = min(L.njs - 1 - tj * TW, TW - 1);
+        const int b_last = min(L.nis - 1 - ti * TW, TW - 1);
+        const int sa = FWD ? 0 : (tj == L.nTJ - 1 ? a_last : TW - 1);
+        const int sb = FWD ? 0 : (ti == L.nTI - 1 ? b_last : TW - 1);
+        const int cl = side == 0 ? idx * TW + sa : sb * TW + idx;
+        LaneInfo li;
+        lane_setup<FWD>(L, ti, tj, cl, li);
+        const bool ring = side == 0 ? (FWD ? tj == 0 : tj == L.nTJ - 1)
+                                    : (FWD ? ti == 0 : ti == L.nTI - 1);
+        const int ptile = side == 0 ? li.ptile_a : li.ptile_b;
+        const int msl = side == 0 ? li.b : li.a;
+        V *depp[ND];
 #pragma unroll
-        for (int d = 0; d < G; ++d) {
-            const int t = g * G + d;
+        for (int q = 0; q < ND; ++q) depp[q] = io.dep[q];
+        // where the value for step t comes from (any valid address for lanes
+        // 32-63, whose loads are not used)
+        auto src = [&](int q, int t) -> const V * {
+            if (ring) {   // ring column at the plane of the chain-start lane
+                int kk = t - li.pa - li.pb;
+                if (!FWD) kk = L.S - 1 - kk;
+                kk = kk < 0 ? 0 : (kk > L.S - 1 ? L.S - 1 : kk);
+                return depp[q] + (side == 0 ? ring_elem(L, FWD ? 2 : 3, min(li.ci, L.nis - 1), kk)
+                                            : ring_elem(L, FWD ? 0 : 1, min(li.cj, L.njs - 1), kk));
+            }
+            const int tm = min(t + TW - 1, last_t);
+            return msgp + msg_index(L, ND, side, q, ptile < 0 ? 0 : ptile, tm, msl);
+        };
+        auto computes = [&](int t) {
+            int kk;
+            return step_kind<FWD>(L, li, t, kk) == K_COMPUTE;
+        };
+        auto needed = [&](int t) { return in && !ring && computes(t); };
+        auto fetch = [&](int t, V (&y)[ND]) {
+#pragma unroll
+            for (int q = 0; q < ND; ++q) y[q] = coh_load(src(q, t < last_t ? t : last_t));
+        };
+        V m[DM][ND];   // m[j] holds step t with t % DM == j
+        // Hand the values of step t (in m[t % DM]) to the cell waves.  When a
+        // message is late, the lanes wait for their values of step t and of
+        // step t + sy.lag (the producer is then sy.lag steps ahead) and the
+        // whole wave reads its messages in flight again.  Returns whether the
+        // wave was late.
+        auto deliver = [&](int t, V (&y)[ND]) -> bool {
+            bool late = false;
+            if (needed(t)) {
+#pragma unroll
+                for (int q = 0; q < ND; ++q) late |= msg_empty(y[q]);
+            }
+            const bool any_late = __any(late);
+            if (any_late) {
+                if (late) {
+#pragma unroll
+                    for (int q = 0; q < ND; ++q) y[q] = msg_wait(src(q, t), y[q], sy.st);
+                }
+                if (needed(t + sy.lag))
+                    (void)msg_wait(src(0, t + sy.lag), MsgEmpty<V>::get(), sy.st);
+            }
+            if (in) {
+#pragma unroll
+                for (int q = 0; q < ND; ++q) mbuf[t & 1][q][side][idx] = y[q];
+            }
+            return any_late;
+        };
+#pragma unroll
+        for (int d = 0; d < DM; ++d) fetch(d, m[d]);
+        (void)deliver(0, m[0]);
+        fetch(DM, m[0]);
+        __syncthreads();
+        for (int t0 = 0; t0 < NSTP; t0 += DM) {
+#pragma unroll
+            for (int d = 0; d < DM; ++d) {
+                // during step t: hand over step t+1, read step t+1+DM
+                const int t = t0 + d;
+                if (deliver(t + 1, m[(d + 1) % DM])) {
+#pragma unroll
+                    for (int j = 2; j <= DM; ++j) fetch(t + j, m[(d + j) % DM]);
+                }
+                fetch(t + 1 + DM, m[(d + 1) % DM]);
+                __syncthreads();
+            }
+        }
+    } else {
+        // --------------------------------------------------- waves 0-3: cells
+        const int lane = threadIdx.x;
+        LaneInfo li;
+        lane_setup<FWD>(L, ti, tj, lane, li);
+        const size_t tb = tile_base(L, ti, tj);
+        V inb[U][NIN];
+        V prev[ND];
+        for (int q = 0; q < ND; ++q) prev[q] = V(0);
+        // array pointers in registers (a per-lane choice between two of them
+        // must not turn into a load of the pointer itself)
+        const V *inp[NIN];
+        V *depp[ND];
+        V *xop[NXS];
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) inp[q] = io.in[q];
+#pragma unroll
+        for (int q = 0; q < ND; ++q) depp[q] = io.dep[q];
+#pragma unroll
+        for (int q = 0; q < NXS; ++q) xop[q] = io.xo[q];
+        // stores of lanes with no cell at a step go to private dummy slots, so
+        // every step issues the same memory operations
+        V *const dummy = msgp + msg_dummy(L, ND, tile, lane);
+        const bool sa_ext = li.src_a != SRC_LDS, sb_ext = li.src_b != SRC_LDS;
+        auto fetch_in = [&](int t, V (&x)[NIN]) {
             int kk;
             const int kind = step_kind<FWD>(L, li, t, kk);
             const size_t e = tb + (size_t)step_slot<FWD>(L, t) * TCOL + lane;
             const bool ld = kind != K_COMPUTE;
 #pragma unroll
             for (int q = 0; q < NIN; ++q) {
-                const V *src = (q < ND && ld) ? io.dep[q < ND ? q : 0] : io.in[q];
-                buf[d][q] = src[e];
+                const V *sp = q < ND ? (ld ? depp[q < ND ? q : 0] : inp[q]) : inp[q];
+                x[q] = sp[e];
             }
-        }
-    };
-
-    auto group = [&](int g, V (&cur)[G][NIN], V (&nxt)[G][NIN]) -> bool {
-        const int s0 = g * G;
-        if (lane == 0) {
-            const uint64_t target = ep | (uint64_t)need_steps(L, s0 + G - 1);
-            bool ok = true;
-            if (has_pi) ok = wait_ge(&sy.progress[pid_i], target);
-            if (ok && has_pj) ok = wait_ge(&sy.progress[pid_j], target);
-            if (!ok) {
-                sh_flag = 1;
-                sy.st->err = CAP_ERR_TIMEOUT;
-                sy.st->done = 1;
-            }
-        }
-        __syncthreads();
-        if (sh_flag) return false;
-        V xa[G][ND], xb[G][ND];
+        };
 #pragma unroll
-        for (int d = 0; d < G; ++d) {
-            int kk;
-            const int kind = step_kind<FWD>(L, li, s0 + d, kk);
+        for (int d = 0; d < U; ++d) fetch_in(d < NST ? d : last_t, inb[d]);
+        __syncthreads();   // step-0 inputs from outside the tile are in mbuf
+        for (int t0 = 0; t0 < NSTP; t0 += U) {
 #pragma unroll
-            for (int q = 0; q < ND; ++q) { xa[d][q] = V(0); xb[d][q] = V(0); }
-            if (kind == K_COMPUTE && li.pa == 0) {
-                const size_t x = xa_elem<FWD>(L, ti, tj, li, kk);
-#pragma unroll
-                for (int q = 0; q < ND; ++q) xa[d][q] = coh_load(io.dep[q] + x);
-            }
-            if (kind == K_COMPUTE && li.pb == 0) {
-                const size_t x = xb_elem<FWD>(L, ti, tj, li, kk);
-#pragma unroll
-                for (int q = 0; q < ND; ++q) xb[d][q] = coh_load(io.dep[q] + x);
-            }
-        }
-        if (g + 1 < ngroups) prefetch(g + 1, nxt);
-#pragma unroll
-        for (int d = 0; d < G; ++d) {
-            const int t = s0 + d;
-            if (t >= NST) break;
-            int kk;
-            const int kind = step_kind<FWD>(L, li, t, kk);
-            const size_t e = tb + (size_t)step_slot<FWD>(L, t) * TCOL + lane;
-            const int rp = (t + 1) & 1, wp = t & 1;
-            V dk[ND], di[ND], dj[ND], out[ND];
-#pragma unroll
-            for (int q = 0; q < ND; ++q) {
-                dk[q] = prev[q];
-                const V la = lds[rp][q][li.nb_a];
-                const V lb = lds[rp][q][li.nb_b];
-                dj[q] = li.pa == 0 ? xa[d][q] : la;
-                di[q] = li.pb == 0 ? xb[d][q] : lb;
-            }
-            if (kind == K_COMPUTE) {
-                V xv[NXS];
-                double r[NRS];
-                op.compute(cur[d], dk, di, dj, out, xv, r);
+            for (int d = 0; d < U; ++d) {
+                const int t = t0 + d;
+                int kk;
+                const int kind = step_kind<FWD>(L, li, t, kk);
+                const size_t e = tb + (size_t)step_slot<FWD>(L, t) * TCOL + lane;
+                const int rp = (t + 1) & 1, wp = t & 1;
+                const bool comp = kind == K_COMPUTE, idle = kind == K_IDLE;
+                V dk[ND], di[ND], dj[ND], out[ND];
 #pragma unroll
                 for (int q = 0; q < ND; ++q) {
-                    if (bnd) coh_store(io.dep[q] + e, out[q]);
-                    else io.dep[q][e] = out[q];
+                    dk[q] = prev[q];
+                    dj[q] = sa_ext ? mbuf[wp][q][0][li.b] : lds[rp][q][li.nb_a];
+                    di[q] = sb_ext ? mbuf[wp][q][1][li.a] : lds[rp][q][li.nb_b];
+                }
+                V xv[NXS];
+                double r[NRS];
+                op.compute(inb[d], dk, di, dj, out, xv, r);
+#pragma unroll
+                for (int q = 0; q < ND; ++q) {
+                    out[q] = comp ? out[q] : (kind == K_LOAD ? inb[d][q] : V(0));
+                    *(idle ? dummy + q * TCOL : depp[q] + e) = out[q];
                 }
 #pragma unroll
-                for (int q = 0; q < Op::NX; ++q) io.xo[q][e] = xv[q];
+                for (int q = 0; q < Op::NX; ++q) *(comp ? xop[q] + e : dummy + (ND + q) * TCOL) = xv[q];
+                if (comp) {
 #pragma unroll
-                for (int q = 0; q < Op::NR; ++q) red[q] += r[q];
-            } else if (kind == K_LOAD) {
+                    for (int q = 0; q < ND; ++q) {
+                        if (li.prod_a) coh_store(msgp + msg_index(L, ND, 0, q, tile, t, li.b), out[q]);
+                        if (li.prod_b) coh_store(msgp + msg_index(L, ND, 1, q, tile, t, li.a), out[q]);
+                    }
+                }
 #pragma unroll
-                for (int q = 0; q < ND; ++q) out[q] = cur[d][q];
-            } else {
+                for (int q = 0; q < Op::NR; ++q) red[q] += comp ? r[q] : 0.0;
 #pragma unroll
-                for (int q = 0; q < ND; ++q) out[q] = V(0);
+                for (int q = 0; q < ND; ++q) {
+                    prev[q] = out[q];
+                    lds[wp][q][lane] = out[q];
+                }
+                fetch_in(t + U < NST ? t + U : last_t, inb[d]);
+                __syncthreads();
             }
-#pragma unroll
-            for (int q = 0; q < ND; ++q) {
-                prev[q] = out[q];
-                lds[wp][q][lane] = out[q];
-            }
-            __syncthreads();
         }
-        // every wave's boundary stores must be complete before the flag
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-        __syncthreads();
-        if (lane == 0) {
-            const int done_steps = s0 + G < NST ? s0 + G : NST;
-            __hip_atomic_store(&sy.progress[my_id], ep | (uint64_t)done_steps, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return true;
-    };
-
-    prefetch(0, bufA);
-    for (int g = 0; g < ngroups; g += 2) {
-        if (!group(g, bufA, bufB)) return;
-        if (g + 1 < ngroups)
-            if (!group(g + 1, bufB, bufA)) return;
     }
     if constexpr (FIN != F_NONE) {
         double v[NRS];
         for (int q = 0; q < NRS; ++q) v[q] = Op::NR ? red[q] : 0.0;
-        grid_reduce<NRS, FIN>(v, my_id, L.ntiles, sy.partials, sy.ticket, sy.st);
+        grid_reduce<NRS, FIN>(v, tile, L.ntiles, sy.partials, sy.ticket, sy.st);
     }
 }
 
@@ -460,19 +543,23 @@ static int dev_out(cap_ctx *ctx, double *host, const double *dev)
 }
 
 template <class Op, bool FWD, int FIN>
-static void launch_wave(cap_ctx *ctx, const Lay &L, const Op &op, const WaveIO<Op> &io,
+static void launch_wave(cap_ctx *ctx, const Lay &L, const Op &op, WaveIO<Op> io,
                         DevState *st = nullptr)
 {
+    using V = typename Op::T;
     WaveSync sy;
-    sy.progress = ctx->progress;
     sy.claim = ctx->claim;
     sy.claim_base = ctx->claim_base;
-    sy.epoch = ++ctx->epoch;
     sy.st = st ? st : ctx->st;
     sy.partials = ctx->partials;
     sy.ticket = ctx->ticket;
+    sy.lag = ctx->wave_lag;
     ctx->claim_base += (unsigned long long)L.ntiles;
-    hipLaunchKernelGGL((k_wave<Op, FWD, FIN>), dim3(L.ntiles), dim3(BLOCK), 0, ctx->stream, L, op,
+    const size_t nmsg = msg_count(L, Op::ND);
+    io.msg = (V *)ctx->msg;
+    hipLaunchKernelGGL(k_msg_fill<V>, dim3((int)std::min<size_t>((nmsg + BLOCK - 1) / BLOCK, 2048)),
+                       dim3(BLOCK), 0, ctx->stream, io.msg, nmsg);
+    hipLaunchKernelGGL((k_wave<Op, FWD, FIN>), dim3(L.ntiles), dim3(WAVE_BLOCK), 0, ctx->stream, L, op,
                        io, sy);
 }
 
@@ -497,11 +584,14 @@ static int prepare(cap_ctx *ctx, const Lay &L, double *fi, const cap_coef *c, So
     if (!fi || !c || !c->ae || !c->aw || !c->an || !c->as || !c->at || !c->ab || !c->ap || !c->su)
         return fail(ctx, CAP_ERR_ARG, "null array argument");
     if ((rc = ensure_tiled(ctx, L.n))) return rc;
-    if (!ctx->progress || ctx->ntiles_cap < L.ntiles) {
-        if (ctx->progress) (void)hipFree(ctx->progress);
-        HIPCHK(hipMalloc(&ctx->progress, (size_t)L.ntiles * sizeof(uint64_t)));
-        HIPCHK(hipMemsetAsync(ctx->progress, 0, (size_t)L.ntiles * sizeof(uint64_t), ctx->stream));
-        ctx->ntiles_cap = L.ntiles;
+    {   // message slots for the largest sweep (three chained FP64 values)
+        const size_t bytes = msg_alloc(L, 3) * sizeof(double);
+        if (ctx->msg_bytes < bytes) {
+            if (ctx->msg) (void)hipFree(ctx->msg);
+            ctx->msg = nullptr;
+            HIPCHK(hipMalloc(&ctx->msg, bytes));
+            ctx->msg_bytes = bytes;
+        }
     }
     if ((rc = dev_in(ctx, S_FI, fi, &d.fi))) return rc;
     const double *src[9] = {c->ae, c->aw, c->an, c->as, c->at, c->ab, c->ap, c->su, nullptr};
@@ -1144,7 +1234,13 @@ extern "C" int cap_init(cap_ctx **out, int ni, int nj, int nk, int flags)
     HIPCHK(hipMalloc(&ctx->claim, 64));
     HIPCHK(hipMemset(ctx->claim, 0, 64));
     ctx->claim_base = 0;
-    ctx->epoch = 0;
+    // steps a consumer tile keeps behind its producer in the wavefront
+    // sweeps (CAP_WAVE_LAG in the environment overrides it, 0..64)
+    ctx->wave_lag = 4;
+    if (const char *lag = getenv("CAP_WAVE_LAG")) {
+        const int v = atoi(lag);
+        if (v >= 0 && v <= 64) ctx->wave_lag = v;
+    }
     *out = ctx;
     return CAP_OK;
 }
@@ -1156,7 +1252,7 @@ extern "C" int cap_finalize(cap_ctx *ctx)
     for (int v = 0; v < T_COUNT; ++v) if (ctx->T[v]) (void)hipFree(ctx->T[v]);
     for (int v = 0; v < S_COUNT; ++v) if (ctx->S[v]) (void)hipFree(ctx->S[v]);
     cap_free_extra(ctx);
-    if (ctx->progress) (void)hipFree(ctx->progress);
+    if (ctx->msg) (void)hipFree(ctx->msg);
     (void)hipFree(ctx->st);
     (void)hipFree(ctx->st2);
     (void)hipHostFree(ctx->st_host);

@@ -7,15 +7,19 @@
 // rel = t - pa - pb, i.e. k-plane kk = rel (forward) or S-1-rel (backward).
 // Its three predecessors in the recursion are
 //   k-side: the lane itself at step t-1,
-//   j-side: lane (pa-1, pb) at step t-1   (tile tj-/+1 when pa == 0),
-//   i-side: lane (pa, pb-1) at step t-1   (tile ti-/+1 when pb == 0),
+//   j-side: lane (pa-1, pb) at step t-1, or -- at the tile edge -- the
+//           neighbouring tile's edge lane, or the ring column at the box edge,
+//   i-side: lane (pa, pb-1) at step t-1, or the neighbouring tile / ring,
 // which is the (k-1, i-1, j-1) dependency of e.g. pcg-sip.f90:98-106 and the
-// (k+1, i+1, j+1) dependency of pcg-sip.f90:112-121.  A value produced by a
-// neighbouring tile at its step t+15 is consumed here at step t.
+// (k+1, i+1, j+1) dependency of pcg-sip.f90:112-121.
 //
-// Cells are classified per step: COMPUTE (interior column, 1<=kk<=nks),
-// LOAD (halo ring column or halo plane: the value is data, read from the
-// dependency array), IDLE (outside the ring).
+// Values cross tile edges as messages: the edge lane of the producing tile
+// writes its value for step t into a message slot; the consuming tile reads
+// it at its own step t-15 (same cell).  Every slot holds a NaN sentinel until
+// it is written, so a message is its own ready flag.
+//
+// Cells are classified per step: COMPUTE (solve cell, 1<=kk<=nks), LOAD
+// (k-halo plane: the value is data, read from the dependency array), IDLE.
 //
 // The recursion formulas (Op*::compute) keep the reference's operand order so
 // that, with contraction disabled, every value is bit-identical to the
@@ -26,12 +30,20 @@
 namespace cap {
 
 enum { K_IDLE = 0, K_LOAD = 1, K_COMPUTE = 2 };
+enum { SRC_LDS = 0, SRC_RING = 1, SRC_MSG = 2 };
+
+// bit pattern of an unwritten message slot (a NaN no arithmetic produces)
+constexpr uint64_t MSG_EMPTY = 0x7FF4DEADBEEFCAFEull;
+constexpr uint32_t MSG_EMPTY_F32 = 0x7FA5CAFEu;
 
 struct LaneInfo {
     int a, b, pa, pb;       // tile-local column and chain coordinates
-    int ci, cj;             // column in the column space
-    bool valid, interior;
-    int nb_a, nb_b;         // lanes providing the j-side / i-side value
+    int ci, cj;             // solve column
+    bool valid;
+    int src_a, src_b;       // where the j-side / i-side predecessor comes from
+    int nb_a, nb_b;         // its lane (SRC_LDS)
+    int ptile_a, ptile_b;   // its tile (SRC_MSG)
+    bool prod_a, prod_b;    // this lane's value goes to the next tile
 };
 
 template <bool FWD>
@@ -44,11 +56,19 @@ CAP_HD void lane_setup(const Lay &L, int ti, int tj, int lane, LaneInfo &li)
     li.ci = ti * TW + li.b;
     li.cj = tj * TW + li.a;
     li.valid = col_valid(L, li.ci, li.cj);
-    li.interior = col_interior(L, li.ci, li.cj);
-    // lane index of (pa-1, pb) and (pa, pb-1); lanes at the chain start read
-    // a neighbouring tile instead and point at themselves here
-    li.nb_a = li.pa == 0 ? lane : (FWD ? lane - 1 : lane + 1);
-    li.nb_b = li.pb == 0 ? lane : (FWD ? lane - TW : lane + TW);
+    if (FWD) {
+        li.src_a = li.cj == 0 ? SRC_RING : (li.a > 0 ? SRC_LDS : SRC_MSG);
+        li.src_b = li.ci == 0 ? SRC_RING : (li.b > 0 ? SRC_LDS : SRC_MSG);
+    } else {
+        li.src_a = li.cj + 1 == L.njs ? SRC_RING : (li.a < TW - 1 ? SRC_LDS : SRC_MSG);
+        li.src_b = li.ci + 1 == L.nis ? SRC_RING : (li.b < TW - 1 ? SRC_LDS : SRC_MSG);
+    }
+    li.nb_a = li.src_a == SRC_LDS ? (FWD ? lane - 1 : lane + 1) : lane;
+    li.nb_b = li.src_b == SRC_LDS ? (FWD ? lane - TW : lane + TW) : lane;
+    li.ptile_a = ti * L.nTJ + (FWD ? tj - 1 : tj + 1);
+    li.ptile_b = (FWD ? ti - 1 : ti + 1) * L.nTJ + tj;
+    li.prod_a = li.valid && (FWD ? (li.a == TW - 1 && tj + 1 < L.nTJ) : (li.a == 0 && tj > 0));
+    li.prod_b = li.valid && (FWD ? (li.b == TW - 1 && ti + 1 < L.nTI) : (li.b == 0 && ti > 0));
 }
 
 CAP_HD int nsteps(const Lay &L) { return L.S + 2 * (TW - 1); }
@@ -66,42 +86,56 @@ CAP_HD int step_kind(const Lay &L, const LaneInfo &li, int t, int &kk)
     int rel = t - li.pa - li.pb;
     if (!li.valid || rel < 0 || rel >= L.S) { kk = -1; return K_IDLE; }
     kk = FWD ? rel : L.S - 1 - rel;
-    if (li.interior && kk >= 1 && kk <= L.nks) return K_COMPUTE;
+    if (kk >= 1 && kk <= L.nks) return K_COMPUTE;
     return K_LOAD;
 }
 
-// element of the j-side predecessor in the neighbouring tile (pa == 0 only)
-template <bool FWD>
-CAP_HD size_t xa_elem(const Lay &L, int ti, int tj, const LaneInfo &li, int kk)
+// Message slots: [side a/b][component q < nd][tile][step][16], one double each.
+CAP_HD size_t msg_index(const Lay &L, int nd, int side, int q, int tile, int t, int s)
 {
-    int ntj = FWD ? tj - 1 : tj + 1;
-    int na = FWD ? TW - 1 : 0;
-    return elem(L, ti, ntj, posmod(kk + na + li.b, L.S), li.b, na);
+    return ((((size_t)side * nd + q) * L.ntiles + tile) * (size_t)nsteps(L) + t) * TW + s;
+}
+CAP_HD size_t msg_count(const Lay &L, int nd) { return msg_index(L, nd, 2, 0, 0, 0, 0); }
+
+// After the messages: per tile, MSG_DUMMY slots of 256 lanes that take the
+// stores of lanes with no cell at a step (one slot per chained value and per
+// extra output, at most 3 + 4).
+constexpr int MSG_DUMMY = 8;
+CAP_HD size_t msg_dummy(const Lay &L, int nd, int tile, int lane)
+{
+    return msg_count(L, nd) + (size_t)tile * MSG_DUMMY * TCOL + lane;
+}
+// message buffer length (elements) for sweeps with up to nd chained values
+CAP_HD size_t msg_alloc(const Lay &L, int nd) { return msg_dummy(L, nd, L.ntiles, 0); }
+
+// the producer of the value consumed at step t wrote it at its step t + 15
+CAP_HD size_t msg_in_a(const Lay &L, int nd, int q, const LaneInfo &li, int t)
+{
+    return msg_index(L, nd, 0, q, li.ptile_a, t + TW - 1, li.b);
+}
+CAP_HD size_t msg_in_b(const Lay &L, int nd, int q, const LaneInfo &li, int t)
+{
+    return msg_index(L, nd, 1, q, li.ptile_b, t + TW - 1, li.a);
+}
+CAP_HD size_t msg_out_a(const Lay &L, int nd, int q, const LaneInfo &li, int tile, int t)
+{
+    return msg_index(L, nd, 0, q, tile, t, li.b);
+}
+CAP_HD size_t msg_out_b(const Lay &L, int nd, int q, const LaneInfo &li, int tile, int t)
+{
+    return msg_index(L, nd, 1, q, tile, t, li.a);
 }
 
-// element of the i-side predecessor in the neighbouring tile (pb == 0 only)
+// ring elements holding the j-side / i-side predecessor at plane kk
 template <bool FWD>
-CAP_HD size_t xb_elem(const Lay &L, int ti, int tj, const LaneInfo &li, int kk)
+CAP_HD size_t ring_in_a(const Lay &L, const LaneInfo &li, int kk)
 {
-    int nti = FWD ? ti - 1 : ti + 1;
-    int nb = FWD ? TW - 1 : 0;
-    return elem(L, nti, tj, posmod(kk + li.a + nb, L.S), nb, li.a);
+    return ring_elem(L, FWD ? 2 : 3, li.ci, kk);
 }
-
-// a tile's value consumed by a neighbour at the neighbour's step t was
-// produced at step t + TW - 1, so step t needs this many completed steps
-CAP_HD int need_steps(const Lay &L, int t_last)
-{
-    int n = t_last + TW;
-    int m = nsteps(L);
-    return n < m ? n : m;
-}
-
-// lanes whose values neighbouring tiles read during the same sweep
 template <bool FWD>
-CAP_HD bool is_boundary(const LaneInfo &li)
+CAP_HD size_t ring_in_b(const Lay &L, const LaneInfo &li, int kk)
 {
-    return li.pa == TW - 1 || li.pb == TW - 1;
+    return ring_elem(L, FWD ? 0 : 1, li.cj, kk);
 }
 
 // tile processed by the c-th claim (a topological order of the tile DAG)

@@ -8,6 +8,8 @@
 #include "../cappuccino-ghostcells-mpi_amd/csrc/cap_wave.h"
 #include <vector>
 #include <cstring>
+#include <stdexcept>
+#include <cmath>
 
 using namespace cap;
 
@@ -19,9 +21,23 @@ static double run(const Lay &L, const Op &op, const double *const *in, double *c
     constexpr int NX = Op::NX > 0 ? Op::NX : 1, NR = Op::NR > 0 ? Op::NR : 1;
     double red = 0.0;
     const int NST = nsteps(L);
+    // message slots start empty; a consumer must only ever read written ones
+    std::vector<uint64_t> msg(msg_count(L, ND), MSG_EMPTY);
+    auto rd = [&](size_t i) {
+        if (msg[i] == MSG_EMPTY) throw std::runtime_error("message read before it was written");
+        double v;
+        std::memcpy(&v, &msg[i], 8);
+        msg[i] = MSG_EMPTY;   // each slot is consumed once
+        return v;
+    };
+    auto wr = [&](size_t i, double v) {
+        if (msg[i] != MSG_EMPTY) throw std::runtime_error("message slot written twice");
+        std::memcpy(&msg[i], &v, 8);
+    };
     for (int c = 0; c < L.ntiles; ++c) {
         int ti, tj;
         claim_tile<FWD>(L, c, ti, tj);
+        const int tile = ti * L.nTJ + tj;
         std::vector<LaneInfo> li(TCOL);
         for (int lane = 0; lane < TCOL; ++lane) lane_setup<FWD>(L, ti, tj, lane, li[lane]);
         std::vector<double> prev(TCOL * ND, 0.0), cur(TCOL * ND, 0.0);
@@ -39,18 +55,30 @@ static double run(const Lay &L, const Op &op, const double *const *in, double *c
                     for (int q = 0; q < NIN; ++q) x[q] = in[q][e];
                     for (int q = 0; q < ND; ++q) {
                         dk[q] = prev[lane * ND + q];
-                        dj[q] = l.pa == 0 ? dep[q][xa_elem<FWD>(L, ti, tj, l, kk)]
-                                          : prev[l.nb_a * ND + q];
-                        di[q] = l.pb == 0 ? dep[q][xb_elem<FWD>(L, ti, tj, l, kk)]
-                                          : prev[l.nb_b * ND + q];
+                        dj[q] = l.src_a == SRC_LDS ? prev[l.nb_a * ND + q]
+                              : l.src_a == SRC_RING ? dep[q][ring_in_a<FWD>(L, l, kk)]
+                                                    : rd(msg_in_a(L, ND, q, l, t));
+                        di[q] = l.src_b == SRC_LDS ? prev[l.nb_b * ND + q]
+                              : l.src_b == SRC_RING ? dep[q][ring_in_b<FWD>(L, l, kk)]
+                                                    : rd(msg_in_b(L, ND, q, l, t));
                     }
                     op.compute(x, dk, di, dj, out, xv, r);
                     for (int q = 0; q < ND; ++q) dep[q][e] = out[q];
                     for (int q = 0; q < Op::NX; ++q) xo[q][e] = xv[q];
                     if (Op::NR) red += r[0];
+                    for (int q = 0; q < ND; ++q) {
+                        if (l.prod_a) wr(msg_out_a(L, ND, q, l, tile, t), out[q]);
+                        if (l.prod_b) wr(msg_out_b(L, ND, q, l, tile, t), out[q]);
+                    }
                 } else if (kind == K_LOAD) {
                     for (int q = 0; q < ND; ++q) out[q] = dep[q][e];
                 } else {
+                    // the GPU kernel sends this lane's stores to its dummy slots
+                    for (int q = 0; q < ND + Op::NX; ++q)
+                        if (q >= MSG_DUMMY ||
+                            msg_dummy(L, ND, tile, lane) + (size_t)q * TCOL >= msg_alloc(L, ND) ||
+                            msg_dummy(L, ND, tile, lane) < msg_count(L, ND))
+                            throw std::runtime_error("dummy slot out of range");
                     for (int q = 0; q < ND; ++q) out[q] = 0.0;
                 }
                 for (int q = 0; q < ND; ++q) cur[lane * ND + q] = out[q];
@@ -58,6 +86,8 @@ static double run(const Lay &L, const Op &op, const double *const *in, double *c
             prev.swap(cur);
         }
     }
+    for (size_t i = 0; i < msg.size(); ++i)
+        if (msg[i] != MSG_EMPTY) throw std::runtime_error("message left unconsumed");
     return red;
 }
 
@@ -84,7 +114,7 @@ void emu_scatter(int ni, int nj, int nk, int klo, int khi, const double *t, doub
     Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
     for (size_t e = 0; e < L.n; ++e) {
         int ci, cj, kk;
-        if (decode(L, e, ci, cj, kk) && col_interior(L, ci, cj) && kk >= 1 && kk <= L.nks)
+        if (e < L.nmain && decode(L, e, ci, cj, kk) && kk >= 1 && kk <= L.nks)
             std_[std_index(L, ci, cj, kk)] = t[e];
     }
 }
@@ -96,6 +126,7 @@ double emu_wave(int op, int ni, int nj, int nk, int klo, int khi, double alfa,
 {
     Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
     const double small = (double)1e-30f;
+    try {
     switch (op) {
     case 0: { OpSipFactor o; o.alfa = alfa; o.small = small; return run<OpSipFactor, true>(L, o, in, dep, xo); }
     case 1: return run<OpSipFwd, true>(L, OpSipFwd(), in, dep, xo);
@@ -105,6 +136,9 @@ double emu_wave(int op, int ni, int nj, int nk, int klo, int khi, double alfa,
     case 5: { OpIcBwdDot o; o.small = small; return run<OpIcBwdDot, false>(L, o, in, dep, xo); }
     case 6: return run<OpSipBwd, false>(L, OpSipBwd(), in, dep, xo);
     case 7: return run<OpSipBwdUpdate, false>(L, OpSipBwdUpdate(), in, dep, xo);
+    }
+    } catch (const std::exception &) {
+        return std::nan("");
     }
     return 0.0;
 }
@@ -117,7 +151,7 @@ void emu_spmv(int ni, int nj, int nk, int klo, int khi, const double *const *A,
     // A = {ae, aw, an, as, at, ab, ap}
     for (size_t e = 0; e < L.n; ++e) {
         int ci, cj, kk;
-        if (!decode(L, e, ci, cj, kk) || !col_interior(L, ci, cj) || kk < 1 || kk > L.nks) continue;
+        if (e >= L.nmain || !decode(L, e, ci, cj, kk) || kk < 1 || kk > L.nks) continue;
         int ti = ci / TW, b = ci % TW, tj = cj / TW, a = cj % TW;
         int slot = posmod(kk + a + b, L.S);
         Nb nb = neighbours(L, ti, tj, slot, b, a, kk);
