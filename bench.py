@@ -53,6 +53,9 @@ def main():
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
     ap.add_argument("--others", default="cgstab_sip,iccg,mixed",
                     help="comma list of extra solves to time once each")
+    ap.add_argument("--pmc-sample", action="store_true",
+                    help="short run for hardware-counter passes: 3 PCG-SIP iterations, then "
+                         "a 1 GiB device copy as the byte-counter calibration; prints one line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -82,6 +85,17 @@ def main():
     def solve_pcg():
         fi.zero_()
         return ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=20000)
+
+    if args.pmc_sample:
+        fi.zero_()
+        r = ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=3)
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_sample": True, "iters": r["iters"], "copy_bytes": 1 << 30}))
+        ctx.close()
+        return
 
     for w in range(args.warmup):
         rep = solve_pcg()
@@ -166,7 +180,7 @@ def main():
                    "cells": ncell, "padded": list(dims), "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _pmc_traffic(dom),
                      "bytes_per_launch": KERNEL_BYTES[dom] * ncell,
                      "avg_launch_ms": kern[dom]["ms"]},
         "pct_hbm_iteration": round(100 * ITER_BYTES * ncell / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
@@ -177,6 +191,19 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def _pmc_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/pmc_traffic.txt, lines `traffic <kernel> <bytes>`), or None."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", "pmc_traffic.txt")):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
+                return int(f[2])
+    except OSError:
+        pass
+    return None
 
 
 def _cpu_model():
