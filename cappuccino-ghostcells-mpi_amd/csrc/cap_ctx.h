@@ -39,6 +39,7 @@ struct cap_ctx {
     hipStream_t stream;
     hipEvent_t ev0, ev1;
     double *T[T_COUNT];
+    void *tpool;                  // one allocation holding every T[] vector
     size_t tcap;
     double *S[S_COUNT];
     float *F[F_COUNT];

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "cap_wave.h"
+#include "cap_amd.h"
 
 namespace cap {
 
@@ -102,8 +103,15 @@ __device__ __forceinline__ double wave_sum(double x)
 }
 
 // Deterministic two-level reduction of NV per-thread sums over the grid.
-// Partial sums go to partials[slot*NV + q] (slot fixed per block), the last
-// block to arrive adds the partials in slot order and runs the finaliser.
+// Each block stores its sums in partials[slot*NV + q] (slot fixed per block)
+// with device-coherent stores; the last block to take a ticket adds the
+// partials in slot order and runs the finaliser.  No fence is used (on this
+// GPU a device-scope release writes back the whole L2): every partial slot
+// holds an empty marker (PARTIAL_EMPTY) until its block's store arrives, the
+// last block waits on each slot until it is written and puts the marker back.
+constexpr uint64_t PARTIAL_EMPTY = 0x7FF4C0FFEEC0FFEEull;
+#define CAP_RED_SPIN (1 << 20)
+
 template <int NV, int FIN>
 __device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double *partials,
                             unsigned *ticket, DevState *st)
@@ -114,6 +122,7 @@ __device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double 
     // with every thread; the others only join the barriers)
     const int w = threadIdx.x >> 6;
     const bool part = threadIdx.x < BLOCK;
+    const double empty = __longlong_as_double((long long)PARTIAL_EMPTY);
     for (int q = 0; q < NV; ++q) {
         double x = wave_sum(v[q]);
         if (part && (threadIdx.x & 63) == 0) sh[q][w] = x;
@@ -121,20 +130,34 @@ __device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double 
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int q = 0; q < NV; ++q)
-            partials[(size_t)slot * NV + q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
+            __hip_atomic_store(&partials[(size_t)slot * NV + q],
+                               ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     double acc[NV];
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    bool lost = false;
     for (int s = threadIdx.x; part && s < nslots; s += BLOCK)
-        for (int q = 0; q < NV; ++q)
-            acc[q] += __hip_atomic_load(&partials[(size_t)s * NV + q], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < NV; ++q) {
+            double *pp = &partials[(size_t)s * NV + q];
+            double x = __hip_atomic_load(pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int it = 0; it < CAP_RED_SPIN &&
+                             (uint64_t)__double_as_longlong(x) == PARTIAL_EMPTY; ++it) {
+                __builtin_amdgcn_s_sleep(1);
+                x = __hip_atomic_load(pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            lost |= (uint64_t)__double_as_longlong(x) == PARTIAL_EMPTY;
+            acc[q] += x;
+            __hip_atomic_store(pp, empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    if (lost) {
+        st->err = CAP_ERR_TIMEOUT;
+        st->done = 1;
+    }
     __syncthreads();
     for (int q = 0; q < NV; ++q) {
         double x = wave_sum(acc[q]);
@@ -145,8 +168,7 @@ __device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double 
         double tot[NV > 4 ? NV : 4] = {0.0, 0.0, 0.0, 0.0};
         for (int q = 0; q < NV; ++q) tot[q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
         finalize<FIN>(st, tot);
-        *ticket = 0;
-        __threadfence();
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -171,6 +193,42 @@ __device__ __forceinline__ Cell cell_of(const Lay &L, size_t ts, int lane)
     const int ci = c.ti * TW + c.b, cj = c.tj * TW + c.a;
     c.in = col_valid(L, ci, cj) && c.kk >= 1 && c.kk <= L.nks;
     return c;
+}
+
+// Tile-slot range [lo, hi) of this block in the flat kernels: contiguous, so a
+// block walks up the planes of one tile and the plane above / below a cell
+// (its k-, and through the shear its i- and j-neighbours) stays in the L2 of
+// the same XCD.
+__device__ __forceinline__ void block_range(const Lay &L, size_t &lo, size_t &hi)
+{
+    const size_t nts = (size_t)L.ntiles * L.S;
+    const size_t per = (nts + gridDim.x - 1) / gridDim.x;
+    lo = (size_t)blockIdx.x * per;
+    hi = lo + per < nts ? lo + per : nts;
+    if (lo > hi) lo = hi;
+}
+
+// Calls f(c) for every solve cell of this thread's lane in the block's slot
+// range (SlotWalk: no division per element).
+template <class F>
+__device__ __forceinline__ void for_cells(const Lay &L, F &&f)
+{
+    size_t lo, hi;
+    block_range(L, lo, hi);
+    if (lo >= hi) return;
+    const int lane = threadIdx.x;
+    SlotWalk w;
+    w.start(L, lo, lane);
+    for (size_t ts = lo; ts < hi; ++ts) {
+        if (w.cell(L)) {
+            Cell c;
+            c.e = ts * TCOL + lane;
+            c.ti = w.ti; c.tj = w.tj; c.slot = w.slot; c.b = w.b; c.a = w.a; c.kk = w.kk;
+            c.in = true;
+            f(c);
+        }
+        w.next(L);
+    }
 }
 
 }  // namespace cap

@@ -141,6 +141,71 @@ CAP_HD Nb neighbours(const Lay &L, int ti, int tj, int slot, int b, int a, int k
     return r;
 }
 
+// x mod S for 0 <= x (a few multiples of S at most)
+CAP_HD int wrap_slot(int x, int S)
+{
+    while (x >= S) x -= S;
+    return x;
+}
+
+// neighbours() for a solve cell whose element e, slot and plane kk are known:
+// the same lane one slot up / down is e +- 256 (wrapping at S), so only the
+// neighbours in other tiles need slot arithmetic.
+CAP_HD Nb neighbours_at(const Lay &L, int ti, int tj, size_t e, int slot, int b, int a, int kk)
+{
+    const int S = L.S;
+    const size_t up = slot + 1 == S ? e - (size_t)(S - 1) * TCOL : e + TCOL;
+    const size_t dn = slot == 0 ? e + (size_t)(S - 1) * TCOL : e - TCOL;
+    const int ci = ti * TW + b, cj = tj * TW + a;
+    Nb r;
+    r.t = up;
+    r.bt = dn;
+    r.n = cj + 1 == L.njs ? ring_elem(L, 3, ci, kk)
+          : a < TW - 1    ? up + 1
+                          : elem(L, ti, tj + 1, wrap_slot(kk + b, S), b, 0);
+    r.s = cj == 0      ? ring_elem(L, 2, ci, kk)
+          : a > 0      ? dn - 1
+                       : elem(L, ti, tj - 1, wrap_slot(kk + TW - 1 + b, S), b, TW - 1);
+    r.e = ci + 1 == L.nis ? ring_elem(L, 1, cj, kk)
+          : b < TW - 1    ? up + TW
+                          : elem(L, ti + 1, tj, wrap_slot(kk + a, S), 0, a);
+    r.w = ci == 0      ? ring_elem(L, 0, cj, kk)
+          : b > 0      ? dn - TW
+                       : elem(L, ti - 1, tj, wrap_slot(kk + a + TW - 1, S), TW - 1, a);
+    return r;
+}
+
+// One lane's walk over consecutive tile slots ts = lo, lo+1, ... (ts = tile*S
+// + slot): tile, slot and plane advance by wrapping, with no division per
+// step.  cell() tells whether the lane's element at the current slot is a
+// solve cell.
+struct SlotWalk {
+    int a, b, slot, ti, tj, kk;
+    bool colv;
+    CAP_HD void start(const Lay &L, size_t lo, int lane)
+    {
+        a = lane % TW;
+        b = lane / TW;
+        slot = (int)(lo % (size_t)L.S);
+        const int tile = (int)(lo / (size_t)L.S);
+        ti = tile / L.nTJ;
+        tj = tile % L.nTJ;
+        colv = col_valid(L, ti * TW + b, tj * TW + a);
+        kk = slot - a - b;
+        while (kk < 0) kk += L.S;
+    }
+    CAP_HD bool cell(const Lay &L) const { return colv && kk >= 1 && kk <= L.nks; }
+    CAP_HD void next(const Lay &L)
+    {
+        if (++kk == L.S) kk = 0;
+        if (++slot == L.S) {
+            slot = 0;
+            if (++tj == L.nTJ) { tj = 0; ++ti; }
+            colv = col_valid(L, ti * TW + b, tj * TW + a);
+        }
+    }
+};
+
 // Decompose an element index into (ci, cj, kk) (ring columns included);
 // returns false for slots of tile columns beyond the solve box.
 CAP_HD bool decode(const Lay &L, size_t e, int &ci, int &cj, int &kk)

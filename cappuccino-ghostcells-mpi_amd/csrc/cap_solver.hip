@@ -42,12 +42,9 @@ __global__ __launch_bounds__(BLOCK) void k_gather(Lay L, const double *__restric
 __global__ __launch_bounds__(BLOCK) void k_scatter(Lay L, const double *__restrict__ src,
                                                    double *__restrict__ dst)
 {
-    const size_t nts = (size_t)L.ntiles * L.S;
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        Cell c = cell_of(L, ts, threadIdx.x);
-        if (!c.in) continue;
+    for_cells(L, [&](const Cell &c) {
         dst[std_index(L, c.ti * TW + c.b, c.tj * TW + c.a, c.kk)] = src[c.e];
-    }
+    });
 }
 
 __global__ __launch_bounds__(BLOCK) void k_fill(double *p, size_t n, double v)
@@ -59,6 +56,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill(double *p, size_t n, double v)
 // ========================================================== vector kernels
 template <class V>
 struct Coef7T { const V *ae, *aw, *an, *as, *at, *ab, *ap; };
+
 using Coef7 = Coef7T<double>;
 
 // r = ae fi_e + aw fi_w + an fi_n + as fi_s + at fi_t + ab fi_b + su - ap fi
@@ -70,19 +68,16 @@ __global__ __launch_bounds__(BLOCK) void k_resid(Lay L, Coef7 A, const double *_
                                                  double *partials, unsigned *ticket)
 {
     if (st->done) return;
-    const size_t nts = (size_t)L.ntiles * L.S;
     double acc[1] = {0.0};
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        Cell c = cell_of(L, ts, threadIdx.x);
-        if (!c.in) continue;
-        Nb nb = neighbours(L, c.ti, c.tj, c.slot, c.b, c.a, c.kk);
+    for_cells(L, [&](const Cell &c) {
+        Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
         const size_t e = c.e;
         double v = A.ae[e] * fi[nb.e] + A.aw[e] * fi[nb.w] + A.an[e] * fi[nb.n] +
                    A.as[e] * fi[nb.s] + A.at[e] * fi[nb.t] + A.ab[e] * fi[nb.bt] + su[e] -
                    A.ap[e] * fi[e];
         r[e] = v;
         acc[0] += fabs(v);
-    }
+    });
     grid_reduce<1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
 }
 
@@ -95,12 +90,9 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(Lay L, Coef7T<V> A, const V *__r
                                                 unsigned *ticket)
 {
     if (st->done) return;
-    const size_t nts = (size_t)L.ntiles * L.S;
     double acc[MODE == 2 ? 2 : 1] = {};
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        Cell c = cell_of(L, ts, threadIdx.x);
-        if (!c.in) continue;
-        Nb nb = neighbours(L, c.ti, c.tj, c.slot, c.b, c.a, c.kk);
+    for_cells(L, [&](const Cell &c) {
+        Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
         const size_t e = c.e;
         V v = A.ap[e] * p[e] - A.ae[e] * p[nb.e] - A.aw[e] * p[nb.w] - A.an[e] * p[nb.n] -
               A.as[e] * p[nb.s] - A.at[e] * p[nb.t] - A.ab[e] * p[nb.bt];
@@ -108,7 +100,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(Lay L, Coef7T<V> A, const V *__r
         if constexpr (MODE == 0) acc[0] += (double)p[e] * (double)v;
         if constexpr (MODE == 1) acc[0] += (double)v * (double)y[e];
         if constexpr (MODE == 2) { acc[0] += (double)v * (double)y[e]; acc[1] += (double)v * (double)v; }
-    }
+    });
     grid_reduce<MODE == 2 ? 2 : 1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
 }
 
@@ -119,12 +111,7 @@ __global__ __launch_bounds__(BLOCK) void k_pupdate(Lay L, const V *__restrict__ 
 {
     if (st->done) return;
     const V bet = (V)st->bet;
-    const size_t nts = (size_t)L.ntiles * L.S;
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        Cell c = cell_of(L, ts, threadIdx.x);
-        if (!c.in) continue;
-        p[c.e] = z[c.e] + bet * p[c.e];
-    }
+    for_cells(L, [&](const Cell &c) { p[c.e] = z[c.e] + bet * p[c.e]; });
 }
 
 // pk = res + om (pk - alf uk)   (cgstab_sip.f90:124)
@@ -134,12 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_bic_p(Lay L, const double *__restrict
 {
     if (st->done) return;
     const double om = st->om, alf = st->alf;
-    const size_t nts = (size_t)L.ntiles * L.S;
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        Cell c = cell_of(L, ts, threadIdx.x);
-        if (!c.in) continue;
-        p[c.e] = res[c.e] + om * (p[c.e] - alf * uk[c.e]);
-    }
+    for_cells(L, [&](const Cell &c) { p[c.e] = res[c.e] + om * (p[c.e] - alf * uk[c.e]); });
 }
 
 // fi += a p; res -= a z; with a = alf (which 0) or gam (which 1);
@@ -152,16 +134,13 @@ __global__ __launch_bounds__(BLOCK) void k_update(Lay L, V *__restrict__ fi, V *
 {
     if (st->done) return;
     const V a = (V)(WHICH == 0 ? st->alf : st->gam);
-    const size_t nts = (size_t)L.ntiles * L.S;
     double acc[1] = {0.0};
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        Cell c = cell_of(L, ts, threadIdx.x);
-        if (!c.in) continue;
+    for_cells(L, [&](const Cell &c) {
         fi[c.e] = fi[c.e] + a * p[c.e];
         V r = res[c.e] - a * z[c.e];
         res[c.e] = r;
         acc[0] += fabs((double)r);
-    }
+    });
     if constexpr (FIN != F_NONE)
         grid_reduce<1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
 }
@@ -173,14 +152,11 @@ __global__ __launch_bounds__(BLOCK) void k_dot(Lay L, const double *__restrict__
                                                double *partials, unsigned *ticket)
 {
     if (st->done) return;
-    const size_t nts = (size_t)L.ntiles * L.S;
     double acc[1] = {0.0};
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        Cell c = cell_of(L, ts, threadIdx.x);
-        if (!c.in) continue;
+    for_cells(L, [&](const Cell &c) {
         if constexpr (COPY) y[c.e] = x[c.e];
         acc[0] += x[c.e] * y[c.e];
-    }
+    });
     grid_reduce<1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
 }
 
@@ -507,9 +483,14 @@ static int fail(cap_ctx *c, int code, const char *what)
 static int ensure_tiled(cap_ctx *ctx, size_t n)
 {
     if (ctx->tcap >= n) return CAP_OK;
-    for (int v = 0; v < T_COUNT; ++v)
-        if (ctx->T[v]) { (void)hipFree(ctx->T[v]); ctx->T[v] = nullptr; }
-    for (int v = 0; v < T_COUNT; ++v) HIPCHK(hipMalloc(&ctx->T[v], n * sizeof(double)));
+    if (ctx->tpool) { (void)hipFree(ctx->tpool); ctx->tpool = nullptr; }
+    for (int v = 0; v < T_COUNT; ++v) ctx->T[v] = nullptr;
+    // One pool; consecutive vectors start 4352 bytes (17 x 256) apart modulo
+    // 64 KiB, so the same element of different vectors does not fall on the
+    // same HBM channel when a kernel streams several vectors at once.
+    const size_t stride = ((n * sizeof(double) + 65535) / 65536) * 65536 + 4352;
+    HIPCHK(hipMalloc(&ctx->tpool, stride * T_COUNT));
+    for (int v = 0; v < T_COUNT; ++v) ctx->T[v] = (double *)((char *)ctx->tpool + stride * v);
     ctx->tcap = n;
     return CAP_OK;
 }
@@ -988,12 +969,7 @@ __global__ __launch_bounds__(BLOCK) void k_fillf(float *p, size_t n)
 __global__ __launch_bounds__(BLOCK) void k_add_corr(Lay L, double *__restrict__ x,
                                                     const float *__restrict__ e)
 {
-    const size_t nts = (size_t)L.ntiles * L.S;
-    for (size_t ts = blockIdx.x; ts < nts; ts += gridDim.x) {
-        Cell c = cell_of(L, ts, threadIdx.x);
-        if (!c.in) continue;
-        x[c.e] = x[c.e] + (double)e[c.e];
-    }
+    for_cells(L, [&](const Cell &c) { x[c.e] = x[c.e] + (double)e[c.e]; });
 }
 
 static int ensure_f32(cap_ctx *ctx, size_t n)
@@ -1229,6 +1205,15 @@ extern "C" int cap_init(cap_ctx **out, int ni, int nj, int nk, int flags)
     HIPCHK(hipMalloc(&ctx->st2, sizeof(DevState)));
     HIPCHK(hipHostMalloc(&ctx->st_host, sizeof(DevState), hipHostMallocDefault));
     HIPCHK(hipMalloc(&ctx->partials, (size_t)MAX_RED_BLOCKS * 64 * sizeof(double)));
+    {   // every reduction slot starts empty (grid_reduce puts the marker back)
+        double empty;
+        const uint64_t bits = PARTIAL_EMPTY;
+        memcpy(&empty, &bits, sizeof empty);
+        hipLaunchKernelGGL(k_fill, dim3(64), dim3(BLOCK), 0, ctx->stream, ctx->partials,
+                           (size_t)MAX_RED_BLOCKS * 64, empty);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     HIPCHK(hipMalloc(&ctx->ticket, 64));
     HIPCHK(hipMemset(ctx->ticket, 0, 64));
     HIPCHK(hipMalloc(&ctx->claim, 64));
@@ -1249,7 +1234,7 @@ extern "C" int cap_finalize(cap_ctx *ctx)
 {
     if (!ctx) return CAP_OK;
     (void)hipStreamSynchronize(ctx->stream);
-    for (int v = 0; v < T_COUNT; ++v) if (ctx->T[v]) (void)hipFree(ctx->T[v]);
+    if (ctx->tpool) (void)hipFree(ctx->tpool);
     for (int v = 0; v < S_COUNT; ++v) if (ctx->S[v]) (void)hipFree(ctx->S[v]);
     cap_free_extra(ctx);
     if (ctx->msg) (void)hipFree(ctx->msg);

@@ -143,6 +143,50 @@ double emu_wave(int op, int ni, int nj, int nk, int klo, int khi, double alfa,
     return 0.0;
 }
 
+// The flat kernels' walk (SlotWalk over per-block slot ranges, neighbours_at)
+// checked against decode() / neighbours(): every solve cell visited once per
+// lane, with the same plane and neighbours.  Returns the number of mismatches.
+int emu_check_walk(int ni, int nj, int nk, int klo, int khi, int nblocks)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    const size_t nts = (size_t)L.ntiles * L.S;
+    std::vector<int> seen(L.nmain, 0);
+    int bad = 0;
+    const size_t per = (nts + nblocks - 1) / nblocks;
+    for (int blk = 0; blk < nblocks; ++blk) {
+        size_t lo = (size_t)blk * per, hi = lo + per < nts ? lo + per : nts;
+        if (lo > hi) lo = hi;
+        if (lo >= hi) continue;
+        for (int lane = 0; lane < TCOL; ++lane) {
+            SlotWalk w;
+            w.start(L, lo, lane);
+            for (size_t ts = lo; ts < hi; ++ts) {
+                const size_t e = ts * TCOL + lane;
+                int ci, cj, kk;
+                const bool valid = decode(L, e, ci, cj, kk);
+                const bool in = valid && kk >= 1 && kk <= L.nks;
+                if (w.cell(L) != in) ++bad;
+                if (w.cell(L)) {
+                    seen[e]++;
+                    if (w.kk != kk || w.ti * TW + w.b != ci || w.tj * TW + w.a != cj ||
+                        w.slot != (int)(ts % (size_t)L.S)) ++bad;
+                    Nb r = neighbours(L, w.ti, w.tj, w.slot, w.b, w.a, w.kk);
+                    Nb f = neighbours_at(L, w.ti, w.tj, e, w.slot, w.b, w.a, w.kk);
+                    if (r.e != f.e || r.w != f.w || r.n != f.n || r.s != f.s || r.t != f.t ||
+                        r.bt != f.bt) ++bad;
+                }
+                w.next(L);
+            }
+        }
+    }
+    for (size_t e = 0; e < L.nmain; ++e) {
+        int ci, cj, kk;
+        const bool in = decode(L, e, ci, cj, kk) && kk >= 1 && kk <= L.nks;
+        if (seen[e] != (in ? 1 : 0)) ++bad;
+    }
+    return bad;
+}
+
 // z = A p on interior cells with the operand order of pcg-sip.f90:147-149
 void emu_spmv(int ni, int nj, int nk, int klo, int khi, const double *const *A,
               const double *p, double *z)

@@ -137,3 +137,10 @@ def test_sipsol_sweep_with_res_halo(E, O):
     em.wave(7, 0.0, [tres, F["bn"], F["be"], F["bt"], tfi], [tres], [tfi])
     np.testing.assert_array_equal(em.scatter(tfi, fi), fref)
     np.testing.assert_array_equal(em.scatter(tres, s.res), ref.res)
+
+
+@pytest.mark.parametrize("dims,klo,khi", CASES)
+@pytest.mark.parametrize("nblocks", [1, 7, 64, 2048])
+def test_flat_kernel_walk(E, dims, klo, khi, nblocks):
+    # the flat kernels' per-block slot walk and wrap-based neighbour indices
+    assert E.emu_check_walk(*dims, klo, khi, nblocks) == 0
