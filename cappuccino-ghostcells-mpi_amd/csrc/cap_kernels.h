@@ -102,57 +102,114 @@ __device__ __forceinline__ double wave_sum(double x)
     return x;
 }
 
+// Sum as an unevaluated pair hi + lo: every addition is Knuth's TwoSum (its
+// rounding error is carried in lo), so a reduction of ~1e7 terms keeps about
+// 30 more bits than a plain running sum and rounds to (nearly) the exact sum
+// whatever the reduction order.  The solvers' scalar products therefore do not
+// depend on the order in which the GPU adds them.  (Code is compiled without
+// FMA contraction, which TwoSum needs.)
+struct Acc {
+    double hi = 0.0, lo = 0.0;
+    __device__ __forceinline__ Acc &operator+=(double x)
+    {
+        const double s = hi + x;
+        const double bp = s - hi;
+        lo += (hi - (s - bp)) + (x - bp);
+        hi = s;
+        return *this;
+    }
+    __device__ __forceinline__ Acc &operator+=(const Acc &o)
+    {
+        *this += o.hi;
+        lo += o.lo;
+        return *this;
+    }
+    __device__ __forceinline__ double value() const { return hi + lo; }
+};
+
+__device__ __forceinline__ Acc wave_sum(Acc x)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        Acc o;
+        o.hi = __shfl_xor(x.hi, off, 64);
+        o.lo = __shfl_xor(x.lo, off, 64);
+        x += o;
+    }
+    return x;
+}
+
 // Deterministic two-level reduction of NV per-thread sums over the grid.
-// Each block stores its sums in partials[slot*NV + q] (slot fixed per block)
-// with device-coherent stores; the last block to take a ticket adds the
-// partials in slot order and runs the finaliser.  No fence is used (on this
-// GPU a device-scope release writes back the whole L2): every partial slot
-// holds an empty marker (PARTIAL_EMPTY) until its block's store arrives, the
-// last block waits on each slot until it is written and puts the marker back.
+// Each block stores its sums (hi, lo) in partials[(slot*NV + q)*2 + {0,1}]
+// (slot fixed per block) with device-coherent stores; the last block to take
+// a ticket adds the partials in slot order and runs the finaliser on the
+// rounded totals.  No fence is used (on this GPU a device-scope release writes
+// back the whole L2): every partial slot holds an empty marker (PARTIAL_EMPTY)
+// until its block's store arrives, the last block waits on each slot until it
+// is written and puts the marker back.
 constexpr uint64_t PARTIAL_EMPTY = 0x7FF4C0FFEEC0FFEEull;
 #define CAP_RED_SPIN (1 << 20)
 
 template <int NV, int FIN>
-__device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double *partials,
+__device__ void grid_reduce(const Acc (&v)[NV], int slot, int nslots, double *partials,
                             unsigned *ticket, DevState *st)
 {
-    __shared__ double sh[NV][BLOCK / 64];
+    __shared__ double shh[NV][BLOCK / 64], shl[NV][BLOCK / 64];
     __shared__ int last;
+    auto sh_put = [&](int q, int k, const Acc &x) { shh[q][k] = x.hi; shl[q][k] = x.lo; };
+    auto sh_sum = [&](int q) {
+        Acc b;
+        b.hi = shh[q][0];
+        b.lo = shl[q][0];
+        for (int k = 1; k < BLOCK / 64; ++k) {
+            Acc o;
+            o.hi = shh[q][k];
+            o.lo = shl[q][k];
+            b += o;
+        }
+        return b;
+    };
     // only the first BLOCK threads take part (a larger block may call this
     // with every thread; the others only join the barriers)
     const int w = threadIdx.x >> 6;
     const bool part = threadIdx.x < BLOCK;
     const double empty = __longlong_as_double((long long)PARTIAL_EMPTY);
     for (int q = 0; q < NV; ++q) {
-        double x = wave_sum(v[q]);
-        if (part && (threadIdx.x & 63) == 0) sh[q][w] = x;
+        Acc x = wave_sum(v[q]);
+        if (part && (threadIdx.x & 63) == 0) sh_put(q, w, x);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int q = 0; q < NV; ++q)
-            __hip_atomic_store(&partials[(size_t)slot * NV + q],
-                               ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < NV; ++q) {
+            const Acc b = sh_sum(q);
+            double *pp = &partials[((size_t)slot * NV + q) * 2];
+            __hip_atomic_store(pp, b.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(pp + 1, b.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!last) return;
-    double acc[NV];
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    Acc acc[NV];
     bool lost = false;
     for (int s = threadIdx.x; part && s < nslots; s += BLOCK)
         for (int q = 0; q < NV; ++q) {
-            double *pp = &partials[(size_t)s * NV + q];
-            double x = __hip_atomic_load(pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            for (int it = 0; it < CAP_RED_SPIN &&
-                             (uint64_t)__double_as_longlong(x) == PARTIAL_EMPTY; ++it) {
-                __builtin_amdgcn_s_sleep(1);
-                x = __hip_atomic_load(pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            double *pp = &partials[((size_t)s * NV + q) * 2];
+            double x[2];
+            for (int h = 0; h < 2; ++h) {
+                x[h] = __hip_atomic_load(pp + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int it = 0; it < CAP_RED_SPIN &&
+                                 (uint64_t)__double_as_longlong(x[h]) == PARTIAL_EMPTY; ++it) {
+                    __builtin_amdgcn_s_sleep(1);
+                    x[h] = __hip_atomic_load(pp + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                lost |= (uint64_t)__double_as_longlong(x[h]) == PARTIAL_EMPTY;
+                __hip_atomic_store(pp + h, empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            lost |= (uint64_t)__double_as_longlong(x) == PARTIAL_EMPTY;
-            acc[q] += x;
-            __hip_atomic_store(pp, empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            Acc o;
+            o.hi = x[0];
+            o.lo = x[1];
+            acc[q] += o;
         }
     if (lost) {
         st->err = CAP_ERR_TIMEOUT;
@@ -160,13 +217,15 @@ __device__ void grid_reduce(const double (&v)[NV], int slot, int nslots, double 
     }
     __syncthreads();
     for (int q = 0; q < NV; ++q) {
-        double x = wave_sum(acc[q]);
-        if (part && (threadIdx.x & 63) == 0) sh[q][w] = x;
+        Acc x = wave_sum(acc[q]);
+        if (part && (threadIdx.x & 63) == 0) sh_put(q, w, x);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double tot[NV > 4 ? NV : 4] = {0.0, 0.0, 0.0, 0.0};
-        for (int q = 0; q < NV; ++q) tot[q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+        for (int q = 0; q < NV; ++q) {
+            tot[q] = sh_sum(q).value();
+        }
         finalize<FIN>(st, tot);
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -189,7 +248,7 @@ __device__ __forceinline__ Cell cell_of(const Lay &L, size_t ts, int lane)
     c.b = lane / TW;
     c.a = lane % TW;
     c.kk = posmod(c.slot - c.a - c.b, L.S);
-    c.e = ts * TCOL + lane;
+    c.e = elem_ts(ts, c.slot, lane);
     const int ci = c.ti * TW + c.b, cj = c.tj * TW + c.a;
     c.in = col_valid(L, ci, cj) && c.kk >= 1 && c.kk <= L.nks;
     return c;
@@ -222,7 +281,7 @@ __device__ __forceinline__ void for_cells(const Lay &L, F &&f)
     for (size_t ts = lo; ts < hi; ++ts) {
         if (w.cell(L)) {
             Cell c;
-            c.e = ts * TCOL + lane;
+            c.e = elem_ts(ts, w.slot, lane);
             c.ti = w.ti; c.tj = w.tj; c.slot = w.slot; c.b = w.b; c.a = w.a; c.kk = w.kk;
             c.in = true;
             f(c);

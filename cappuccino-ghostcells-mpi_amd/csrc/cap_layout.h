@@ -78,9 +78,21 @@ CAP_HD size_t tile_base(const Lay &L, int ti, int tj)
     return ((size_t)(ti * L.nTJ + tj) * (size_t)L.S) * TCOL;
 }
 
+// element of lane (b*16 + a) at slot s of the tile whose first element is tb
+CAP_HD size_t elem_at(size_t tb, int slot, int lane)
+{
+    return tb + (size_t)slot * TCOL + lane;
+}
+
 CAP_HD size_t elem(const Lay &L, int ti, int tj, int slot, int b, int a)
 {
-    return tile_base(L, ti, tj) + (size_t)slot * TCOL + b * TW + a;
+    return elem_at(tile_base(L, ti, tj), slot, b * TW + a);
+}
+
+// element of the lane at tile-slot ts = tile*S + slot
+CAP_HD size_t elem_ts(size_t ts, int slot, int lane)
+{
+    return elem_at((ts - (size_t)slot) * TCOL, slot, lane);
 }
 
 // ring element: side 0 (i=ia-1) / 1 (i=ib+1) indexed by cj, side 2 (j=ja-1) /
@@ -154,8 +166,10 @@ CAP_HD int wrap_slot(int x, int S)
 CAP_HD Nb neighbours_at(const Lay &L, int ti, int tj, size_t e, int slot, int b, int a, int kk)
 {
     const int S = L.S;
-    const size_t up = slot + 1 == S ? e - (size_t)(S - 1) * TCOL : e + TCOL;
-    const size_t dn = slot == 0 ? e + (size_t)(S - 1) * TCOL : e - TCOL;
+    const int lane = b * TW + a;
+    const size_t tb = e - elem_at(0, slot, lane);
+    const size_t up = elem_at(tb, slot + 1 == S ? 0 : slot + 1, lane);
+    const size_t dn = elem_at(tb, slot == 0 ? S - 1 : slot - 1, lane);
     const int ci = ti * TW + b, cj = tj * TW + a;
     Nb r;
     r.t = up;
@@ -220,10 +234,10 @@ CAP_HD bool decode(const Lay &L, size_t e, int &ci, int &cj, int &kk)
         else if (side == 1) { ci = L.nis; cj = idx; }
         else if (side == 2) { ci = idx; cj = -1; }
         else { ci = idx; cj = L.njs; }
-        return true;
+        return kk <= L.nks + 1;
     }
-    size_t tslot = e / TCOL;
-    int lane = (int)(e % TCOL);
+    const size_t tslot = e / TCOL;
+    const int lane = (int)(e % TCOL);
     int b = lane / TW, a = lane % TW;
     int slot = (int)(tslot % (size_t)L.S);
     int t = (int)(tslot / (size_t)L.S);
@@ -231,7 +245,7 @@ CAP_HD bool decode(const Lay &L, size_t e, int &ci, int &cj, int &kk)
     ci = ti * TW + b;
     cj = tj * TW + a;
     kk = posmod(slot - a - b, L.S);
-    return col_valid(L, ci, cj);
+    return col_valid(L, ci, cj) && kk <= L.nks + 1;
 }
 
 }  // namespace cap

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(BLOCK) void k_resid(Lay L, Coef7 A, const double *_
                                                  double *partials, unsigned *ticket)
 {
     if (st->done) return;
-    double acc[1] = {0.0};
+    Acc acc[1];
     for_cells(L, [&](const Cell &c) {
         Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
         const size_t e = c.e;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(Lay L, Coef7T<V> A, const V *__r
                                                 unsigned *ticket)
 {
     if (st->done) return;
-    double acc[MODE == 2 ? 2 : 1] = {};
+    Acc acc[MODE == 2 ? 2 : 1];
     for_cells(L, [&](const Cell &c) {
         Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
         const size_t e = c.e;
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Lay L, V *__restrict__ fi, V *
 {
     if (st->done) return;
     const V a = (V)(WHICH == 0 ? st->alf : st->gam);
-    double acc[1] = {0.0};
+    Acc acc[1];
     for_cells(L, [&](const Cell &c) {
         fi[c.e] = fi[c.e] + a * p[c.e];
         V r = res[c.e] - a * z[c.e];
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(BLOCK) void k_dot(Lay L, const double *__restrict__
                                                double *partials, unsigned *ticket)
 {
     if (st->done) return;
-    double acc[1] = {0.0};
+    Acc acc[1];
     for_cells(L, [&](const Cell &c) {
         if constexpr (COPY) y[c.e] = x[c.e];
         acc[0] += x[c.e] * y[c.e];
@@ -276,8 +276,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
     static_assert(UM % U == 0 && UM % DM == 0, "strides must divide each other");
     const int NSTP = (NST + UM - 1) / UM * UM;
     V *const msgp = io.msg;
-    double red[NRS];
-    for (int q = 0; q < NRS; ++q) red[q] = 0.0;
+    Acc red[NRS];
 
     if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == BLOCK / 64) {
         // ------------------------------------------ wave 4: the tile's inputs
@@ -391,19 +390,24 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
         // every step issues the same memory operations
         V *const dummy = msgp + msg_dummy(L, ND, tile, lane);
         const bool sa_ext = li.src_a != SRC_LDS, sb_ext = li.src_b != SRC_LDS;
-        auto fetch_in = [&](int t, V (&x)[NIN]) {
-            int kk;
-            const int kind = step_kind<FWD>(L, li, t, kk);
-            const size_t e = tb + (size_t)step_slot<FWD>(L, t) * TCOL + lane;
-            const bool ld = kind != K_COMPUTE;
+        // the lane's two halo planes of the chained values (kk = 0 and nks+1)
+        V hlo[ND], hhi[ND];
+        {
+            const size_t e0 = elem_at(tb, (li.a + li.b) % L.S, lane);
+            const size_t e1 = elem_at(tb, (L.nks + 1 + li.a + li.b) % L.S, lane);
 #pragma unroll
-            for (int q = 0; q < NIN; ++q) {
-                const V *sp = q < ND ? (ld ? depp[q < ND ? q : 0] : inp[q]) : inp[q];
-                x[q] = sp[e];
+            for (int q = 0; q < ND; ++q) {
+                hlo[q] = depp[q][e0];
+                hhi[q] = depp[q][e1];
             }
+        }
+        auto fetch_in = [&](int t, V (&x)[NIN]) {
+            const size_t e = elem_at(tb, step_slot<FWD>(L, t), lane);
+#pragma unroll
+            for (int q = 0; q < NIN; ++q) x[q] = inp[q][e];
         };
 #pragma unroll
-        for (int d = 0; d < U; ++d) fetch_in(d < NST ? d : last_t, inb[d]);
+        for (int d = 0; d < U; ++d) fetch_in(d, inb[d]);
         __syncthreads();   // step-0 inputs from outside the tile are in mbuf
         for (int t0 = 0; t0 < NSTP; t0 += U) {
 #pragma unroll
@@ -411,7 +415,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                 const int t = t0 + d;
                 int kk;
                 const int kind = step_kind<FWD>(L, li, t, kk);
-                const size_t e = tb + (size_t)step_slot<FWD>(L, t) * TCOL + lane;
+                const size_t e = elem_at(tb, step_slot<FWD>(L, t), lane);
                 const int rp = (t + 1) & 1, wp = t & 1;
                 const bool comp = kind == K_COMPUTE, idle = kind == K_IDLE;
                 V dk[ND], di[ND], dj[ND], out[ND];
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                 op.compute(inb[d], dk, di, dj, out, xv, r);
 #pragma unroll
                 for (int q = 0; q < ND; ++q) {
-                    out[q] = comp ? out[q] : (kind == K_LOAD ? inb[d][q] : V(0));
+                    out[q] = comp ? out[q] : (kind == K_LOAD ? (kk == 0 ? hlo[q] : hhi[q]) : V(0));
                     *(idle ? dummy + q * TCOL : depp[q] + e) = out[q];
                 }
 #pragma unroll
@@ -445,15 +449,13 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                     prev[q] = out[q];
                     lds[wp][q][lane] = out[q];
                 }
-                fetch_in(t + U < NST ? t + U : last_t, inb[d]);
+                fetch_in(t + U, inb[d]);
                 __syncthreads();
             }
         }
     }
     if constexpr (FIN != F_NONE) {
-        double v[NRS];
-        for (int q = 0; q < NRS; ++q) v[q] = Op::NR ? red[q] : 0.0;
-        grid_reduce<NRS, FIN>(v, tile, L.ntiles, sy.partials, sy.ticket, sy.st);
+        grid_reduce<NRS, FIN>(red, tile, L.ntiles, sy.partials, sy.ticket, sy.st);
     }
 }
 

@@ -410,7 +410,7 @@ __global__ __launch_bounds__(256) void k_continuity(int ni, int nj, int nk,
     const int NI = ni - 4, NJ = nj - 4, NK = nk - 2;
     const size_t n = (size_t)NI * NJ * NK;
     const int nij = ni * nj;
-    double acc[2] = {0.0, 0.0};
+    Acc acc[2];
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
         const int k = 2 + (int)(q / ((size_t)NI * NJ));
         const int r = (int)(q % ((size_t)NI * NJ));

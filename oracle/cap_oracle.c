@@ -34,21 +34,31 @@
 
 #define SMALL ((double)1e-30f)
 
-/* Summation order of the solvers' scalar products and L1 norms.  0 (default):
- * the reference's single running sum in loop order.  1: one running sum per
- * k-plane, planes added in order -- a different but equally valid order, used
- * by the tests to measure how far a solve moves when only the summation order
- * changes (the GPU sums in yet another order). */
+/* Summation of the solvers' scalar products and L1 norms.  0 (default): the
+ * reference's single running sum in loop order.  1: one running sum per
+ * k-plane, planes added in order.  2: a compensated running sum (every
+ * addition is Knuth's TwoSum, the rounding errors are carried in a second
+ * word), read as hi + lo: the nearly exact sum, whatever the order -- what
+ * the GPU's reductions compute.  Modes 1 and 2 are used by the tests: mode 2
+ * reproduces the GPU's arithmetic, and the distance between modes measures
+ * how far a solve moves when only the summation changes. */
 static int g_plane_sums = 0;
 void orc_set_plane_sums(int on) { g_plane_sums = on; }
 
-typedef struct { double tot, part; int k; } psum;
-#define PS0 {0.0, 0.0, -1}
+typedef struct { double tot, part; int k; double hi, lo; } psum;
+#define PS0 {0.0, 0.0, -1, 0.0, 0.0}
 /* adds v at plane k; returns the sum so far (bit-identical to `s = s + v`
- * from 0.0 when plane sums are off) */
+ * from 0.0 in mode 0) */
 static inline double ps_add(psum *ps, double v, int k)
 {
     if (!g_plane_sums) { ps->part = ps->part + v; return ps->part; }
+    if (g_plane_sums == 2) {
+        const double s = ps->hi + v;
+        const double bp = s - ps->hi;
+        ps->lo += (ps->hi - (s - bp)) + (v - bp);
+        ps->hi = s;
+        return ps->hi + ps->lo;
+    }
     if (k != ps->k) {
         if (ps->k != -1) ps->tot = ps->tot + ps->part;
         ps->part = 0.0;

@@ -152,24 +152,42 @@ def iccg(s, fi, sor=1e-8, nsw=1000):
 
 class plane_sums:
     """Context manager: the oracle's scalar products and norms are summed per
-    k-plane instead of in one running sum (another valid summation order)."""
+    k-plane (mode 1) or as compensated (TwoSum) running sums, i.e. nearly
+    exactly (mode 2, the GPU's arithmetic), instead of in one plain running
+    sum (mode 0, the reference's)."""
+
+    def __init__(self, mode=1):
+        self.mode = mode
 
     def __enter__(self):
-        lib().orc_set_plane_sums(1)
+        lib().orc_set_plane_sums(self.mode)
 
     def __exit__(self, *exc):
         lib().orc_set_plane_sums(0)
 
 
 def order_sensitivity(solver, s, fi, **kw):
-    """max|fi_a - fi_b| / max|fi_a| between the reference order and per-plane
-    sums: how far the solve moves when only the summation order changes."""
+    """max|fi_a - fi_b| / max|fi_a| between the reference's summation and the
+    per-plane and compensated summations (the larger of the two): how far the
+    solve moves when only the summation order / rounding changes."""
     s1, f1 = s.copy(), fi.copy()
     solver(s1, f1, **kw)
+    worst = 0.0
+    for mode in (1, 2):
+        s2, f2 = s.copy(), fi.copy()
+        with plane_sums(mode):
+            solver(s2, f2, **kw)
+        worst = max(worst, float(np.max(np.abs(f1 - f2)) / max(np.max(np.abs(f1)), 1e-300)))
+    return worst
+
+
+def accurate_sums(solver, s, fi, **kw):
+    """The solve with compensated (nearly exact) scalar products and norms --
+    the arithmetic of the GPU's reductions.  Returns (report, s, fi)."""
     s2, f2 = s.copy(), fi.copy()
-    with plane_sums():
-        solver(s2, f2, **kw)
-    return float(np.max(np.abs(f1 - f2)) / max(np.max(np.abs(f1)), 1e-300))
+    with plane_sums(2):
+        rep = solver(s2, f2, **kw)
+    return rep, s2, f2
 
 
 def sip_factor(s, k_lo, k_hi, alfa=ALFA):

@@ -48,7 +48,7 @@ static double run(const Lay &L, const Op &op, const double *const *in, double *c
                 const LaneInfo &l = li[lane];
                 int kk;
                 int kind = step_kind<FWD>(L, l, t, kk);
-                size_t e = tb + (size_t)slot * TCOL + lane;
+                size_t e = elem_at(tb, slot, lane);
                 double out[ND];
                 if (kind == K_COMPUTE) {
                     double x[NIN], dk[ND], di[ND], dj[ND], xv[NX], r[NR];
@@ -161,7 +161,7 @@ int emu_check_walk(int ni, int nj, int nk, int klo, int khi, int nblocks)
             SlotWalk w;
             w.start(L, lo, lane);
             for (size_t ts = lo; ts < hi; ++ts) {
-                const size_t e = ts * TCOL + lane;
+                const size_t e = elem_ts(ts, w.slot, lane);
                 int ci, cj, kk;
                 const bool valid = decode(L, e, ci, cj, kk);
                 const bool in = valid && kk >= 1 && kk <= L.nks;

@@ -5,13 +5,16 @@ device-pointer contexts) on seeded inputs and compares with the oracle's
 restatement of the reference routine on the same inputs.
 
 Bar: the SIP/IC factorisations, sweeps and stencils are bit-identical (same
-operations in the same order, contraction off); the Krylov solvers differ
-only through the order of their FP64 reductions, so iteration counts agree to
-+/-1 and, when they agree exactly, fi agrees to 1e-10 relative -- or, for
-long solves, to 4x the solver's own sensitivity to summation order, measured
-on the CPU by re-running the oracle with per-plane sums (CG amplifies a
-1e-16 change in a scalar product to 1e-9..1e-8 over a few hundred
-iterations of these ill-conditioned systems, on the CPU alone).
+operations in the same order, contraction off).  The Krylov solvers' scalar
+products and norms are reduced on the GPU as compensated (TwoSum) sums, i.e.
+nearly exactly in any order; the oracle run with the same compensated sums
+(oracle mode 2) must then agree with the GPU bit for bit -- fi, iteration
+count and residuals.  Against the reference's own plain running sums (oracle
+mode 0) iteration counts agree to +/-1 and fi to 1e-10 relative -- or, for
+long solves, to 4x the solver's own sensitivity to its summation, measured on
+the CPU by re-running the oracle with per-plane and with compensated sums (CG
+amplifies a 1e-16 change in a scalar product to 1e-9..1e-8 over a few
+hundred iterations of these ill-conditioned systems, on the CPU alone).
 """
 import numpy as np
 import pytest
@@ -46,11 +49,17 @@ def rel(a, b):
     return np.max(np.abs(a - b)) / (den if den > 0 else 1.0)
 
 
-def check_solve(name, rep, rep_ref, fi, fi_ref, coef, s_ref, bitwise_fi=False, sens=0.0):
+def check_solve(name, rep, rep_ref, fi, fi_ref, coef, s_ref, bitwise_fi=False, sens=0.0,
+                acc=None):
+    """GPU result vs the oracle in the reference's summation (rep_ref, fi_ref,
+    s_ref) and, for the Krylov solvers, vs the oracle with compensated sums
+    (acc = (report, system, fi)), which must be bit-identical."""
     tol = max(1e-10, 4.0 * sens)
     print(f"{name}: gpu iters {rep['iters']} cpu iters {rep_ref['iters']} "
           f"fi rel {rel(fi, fi_ref):.2e} (tol {tol:.1e}, cpu order sensitivity {sens:.1e}) "
-          f"resor {rep['resor']:.6e}/{rep_ref['resor']:.6e}")
+          f"resor {rep['resor']:.6e}/{rep_ref['resor']:.6e}"
+          + ("" if acc is None else
+             f"; vs compensated-sum oracle: iters {acc[0]['iters']} fi rel {rel(fi, acc[2]):.2e}"))
     assert abs(rep["iters"] - rep_ref["iters"]) <= 1
     assert rep["resor"] == pytest.approx(rep_ref["resor"], rel=1e-12)
     if rep["iters"] == rep_ref["iters"]:
@@ -63,6 +72,13 @@ def check_solve(name, rep, rep_ref, fi, fi_ref, coef, s_ref, bitwise_fi=False, s
     for f in B_FIELDS:
         if getattr(coef, f) is not None:
             np.testing.assert_array_equal(getattr(coef, f), getattr(s_ref, f), err_msg=f)
+    if acc is not None:
+        rep_acc, s_acc, fi_acc = acc
+        assert rep["iters"] == rep_acc["iters"]
+        assert rep["resor"] == rep_acc["resor"] and rep["resl"] == rep_acc["resl"]
+        np.testing.assert_array_equal(fi, fi_acc)
+        if coef.res is not None:
+            np.testing.assert_array_equal(coef.res, s_acc.res)
 
 
 @pytest.mark.parametrize("dims", SMALL_DIMS)
@@ -77,7 +93,8 @@ def test_pcgsip_lognormal(gpu, O, dims):
         rep = ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=2000)
     assert rep["converged"] and rep_ref["converged"]
     sens = O.order_sensitivity(O.pcgsip, s, fi0, alfa=0.0, sor=1e-8, nsw=2000)
-    check_solve("pcgsip", rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens)
+    acc = O.accurate_sums(O.pcgsip, s, fi0, alfa=0.0, sor=1e-8, nsw=2000)
+    check_solve("pcgsip", rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens, acc=acc)
     # the returned res is the residual of the returned fi (res = 0 off range)
     ni, nj, nk = dims
     rng_mask = np.zeros((nk, ni, nj), bool)
@@ -102,6 +119,9 @@ def test_pcgsip_alfa092_stalls_alike(gpu, O, dims):
     assert not rep_ref["converged"] and not rep["converged"]
     assert rep["iters"] == rep_ref["iters"] == 61
     assert rep["resl"] == pytest.approx(rep_ref["resl"], rel=1e-6)
+    rep_acc, _, fi_acc = O.accurate_sums(O.pcgsip, s, fi0, alfa=0.92, sor=1e-8, nsw=60)
+    assert rep["resl"] == rep_acc["resl"]
+    np.testing.assert_array_equal(fi, fi_acc)
 
 
 @pytest.mark.parametrize("dims", SMALL_DIMS)
@@ -112,7 +132,8 @@ def test_pcgsip_general_halos(gpu, O, dims):
     coef, fi = coef_of(gpu, s), fi0.copy()
     with gpu.Context(*dims) as ctx:
         rep = ctx.pcgsip(fi, coef, sor=1e-8, nsw=30)
-    check_solve("pcgsip/general", rep, rep_ref, fi, fi_ref, coef, s_ref)
+    acc = O.accurate_sums(O.pcgsip, s, fi0, sor=1e-8, nsw=30)
+    check_solve("pcgsip/general", rep, rep_ref, fi, fi_ref, coef, s_ref, acc=acc)
 
 
 @pytest.mark.parametrize("dims", SMALL_DIMS)
@@ -138,7 +159,8 @@ def test_iccg_lognormal(gpu, O, dims):
         rep = ctx.iccg(fi, coef, sor=1e-8, nsw=2000)
     assert rep["converged"]
     sens = O.order_sensitivity(O.iccg, s, fi0, sor=1e-8, nsw=2000)
-    check_solve("iccg", rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens)
+    acc = O.accurate_sums(O.iccg, s, fi0, sor=1e-8, nsw=2000)
+    check_solve("iccg", rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens, acc=acc)
 
 
 @pytest.mark.parametrize("dims", SMALL_DIMS)
@@ -154,7 +176,8 @@ def test_cgstab_sip(gpu, O, dims, kind):
     with gpu.Context(*dims) as ctx:
         rep = ctx.cgstab_sip(fi, coef, sor=1e-8, nsw=400)
     sens = O.order_sensitivity(O.cgstab_sip, s, fi0, sor=1e-8, nsw=400)
-    check_solve("cgstab_sip/" + kind, rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens)
+    acc = O.accurate_sums(O.cgstab_sip, s, fi0, sor=1e-8, nsw=400)
+    check_solve("cgstab_sip/" + kind, rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens, acc=acc)
 
 
 def test_device_pointers(gpu, O):
@@ -171,6 +194,9 @@ def test_device_pointers(gpu, O):
     assert abs(rep["iters"] - rep_ref["iters"]) <= 1
     sens = O.order_sensitivity(O.pcgsip, s, fi0, alfa=0.0, sor=1e-8, nsw=2000)
     assert rel(fi.cpu().numpy(), fi_ref) <= max(1e-10, 4 * sens) or rep["iters"] != rep_ref["iters"]
+    rep_acc, _, fi_acc = O.accurate_sums(O.pcgsip, s, fi0, alfa=0.0, sor=1e-8, nsw=2000)
+    assert rep["iters"] == rep_acc["iters"]
+    np.testing.assert_array_equal(fi.cpu().numpy(), fi_acc)
 
 
 @pytest.mark.parametrize("dims", [(36, 36, 36), (52, 70, 30)])
