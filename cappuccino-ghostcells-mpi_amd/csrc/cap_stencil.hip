@@ -400,6 +400,206 @@ __global__ __launch_bounds__(256) void k_fluxmc(int ni, int nj, int nk, Fld f, F
     }
 }
 
+// ------------------------------------------------------ momentum fluxes
+// fluxuvw-overrelaxed-correction.f90: one thread per face of the direction's
+// face range (99-118); each formula keeps the reference's operand order.
+struct UvwOut {
+    const double *vis, *fcf;
+    double *acfe, *acfw, *su, *sv, *sw, *bp, *bt, *bb;
+};
+
+// QUICK high-order fluxes of the face east of cell p (287-341; valid where
+// the reference's interior test holds)
+__device__ __forceinline__ void uvw_quick(const Fld &f, const FaceDir &D, const double *fcf,
+                                          size_t p, double (&h)[3])
+{
+    const size_t e = p + D.idew, w = p - D.idew, ee = p + 2 * (size_t)D.idew;
+    const double flcf = fcf[p];
+    const double ce = flcf < 0.0 ? flcf : 0.0, cp = flcf > 0.0 ? flcf : 0.0;
+    const double fw_ = D.fif[w], fp_ = D.fif[p], fe_ = D.fif[e];
+    const double g11 = ((2.0 - fw_) * (fp_ * fp_)) / (1.0 + fp_ - fw_);
+    const double g12 = ((1.0 - fp_) * ((1.0 - fw_) * (1.0 - fw_))) / (1.0 + fp_ - fw_);
+    const double g21 = ((1.0 + fe_) * ((1.0 - fp_) * (1.0 - fp_))) / (1.0 + fe_ - fp_);
+    const double g22 = ((fe_ * fe_) * fp_) / (1.0 + fe_ - fp_);
+    const double *vel[3] = {f.u, f.v, f.w};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double *q = vel[c];
+        h[c] = ce * (g21 * q[p] + (1.0 - g21 + g22) * q[e] - g22 * q[ee]) +
+               cp * (g11 * q[e] + (1.0 - g11 + g12) * q[p] - g12 * q[w]);
+    }
+}
+
+__device__ __forceinline__ double fmin_f(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double fmax_f(double a, double b) { return b > a ? b : a; }
+
+__global__ __launch_bounds__(256) void k_fluxuvw(int ni, int nj, int nk, Fld f, FaceDir D,
+                                                 UvwOut o, int flags, double gam, int dir,
+                                                 int kst, int ist, int jst)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const int NJ = njmm - jst + 1, NI = nimm - ist + 1, NK = nkmm - kst + 1;
+    const size_t nf = (size_t)NK * NI * NJ;
+    const bool lcds = flags & CAP_UVW_CDS, lcds4 = flags & CAP_UVW_CDS4;
+    const bool lluds = flags & CAP_UVW_LUDS, lquds = flags & CAP_UVW_QUICK;
+    const bool flim = flags & CAP_UVW_LIMITER;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nf; q += (size_t)gridDim.x * 256) {
+        const int k = kst + (int)(q / ((size_t)NI * NJ));
+        const int r = (int)(q % ((size_t)NI * NJ));
+        const int i = ist + r / NJ, j = jst + r % NJ;
+        const size_t inp = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        const size_t ine = inp + D.idew, inw = inp - D.idew, ins = inp - D.idns;
+        const size_t inb = inp - D.idtb, inee = inp + 2 * (size_t)D.idew, inbs = inb - D.idns;
+        const double fxe = D.fif[inp], fxp = 1.0 - fxe;
+        const double fxpw = D.fif[inw], fxw = 1.0 - fxpw;
+        const double fxee = D.fif[ine], fxew = 1.0 - fxee;
+        const double xpn = f.xc[ine] - f.xc[inp], ypn = f.yc[ine] - f.yc[inp], zpn = f.zc[ine] - f.zc[inp];
+        const double dpn = sqrt(xpn * xpn + ypn * ypn + zpn * zpn);
+        const double arx = D.arvx[inp], ary = D.arvy[inp], arz = D.arvz[inp];
+        const double are = sqrt(arx * arx + ary * ary + arz * arz);
+        const double costn = 1.0;   // orthogonal correction (178)
+        const double vole = xpn * arx + ypn * ary + zpn * arz;
+        const double d1x = costn, d1y = costn, d1z = costn;
+        const double d2x = xpn * costn, d2y = ypn * costn, d2z = zpn * costn;
+        const double game = o.vis[inp] * fxp + o.vis[ine] * fxe;
+        const double xf = 0.25 * (f.x[inp] + f.x[ins] + f.x[inb] + f.x[inbs]);
+        const double yf = 0.25 * (f.y[inp] + f.y[ins] + f.y[inb] + f.y[inbs]);
+        const double zf = 0.25 * (f.z[inp] + f.z[ins] + f.z[inb] + f.z[inbs]);
+        const double xi = f.xc[inp] * fxp + f.xc[ine] * fxe;
+        const double yi = f.yc[inp] * fxp + f.yc[ine] * fxe;
+        const double zi = f.zc[inp] * fxp + f.zc[ine] * fxe;
+        const double *vel[3] = {f.u, f.v, f.w};
+        const double *grd[3] = {f.gu, f.gv, f.gw};
+        double dxi[3][3], dxii[3][3];   // [velocity][x,y,z]
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double *g = grd[c], *vq = vel[c];
+            dxi[c][0] = G3(g, inp, 0) * fxp + G3(g, ine, 0) * fxe;
+            dxi[c][1] = G3(g, inp, 1) * fxp + G3(g, ine, 1) * fxe;
+            dxi[c][2] = G3(g, inp, 2) * fxp + G3(g, ine, 2) * fxe;
+            const double dd = vq[ine] - vq[inp] - dxi[c][0] * d2x - dxi[c][1] * d2y - dxi[c][2] * d2z;
+            dxii[c][0] = dxi[c][0] * d1x + arx / vole * dd;
+            dxii[c][1] = dxi[c][1] * d1y + ary / vole * dd;
+            dxii[c][2] = dxi[c][2] * d1z + arz / vole * dd;
+        }
+        // explicit (fd*e) and implicit (fd*i) diffusion (262-269)
+        const double duxii = dxii[0][0], duyii = dxii[0][1], duzii = dxii[0][2];
+        const double dvxii = dxii[1][0], dvyii = dxii[1][1], dvzii = dxii[1][2];
+        const double dwxii = dxii[2][0], dwyii = dxii[2][1], dwzii = dxii[2][2];
+        double fde[3], fdi[3];
+        fde[0] = game * ((duxii + duxii) * arx + (duyii + dvxii) * ary + (duzii + dwxii) * arz);
+        fde[1] = game * ((duyii + dvxii) * arx + (dvyii + dvyii) * ary + (dvzii + dwyii) * arz);
+        fde[2] = game * ((duzii + dwxii) * arx + (dwyii + dvzii) * ary + (dwzii + dwzii) * arz);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            fdi[c] = game * are / dpn * (dxi[c][0] * xpn + dxi[c][1] * ypn + dxi[c][2] * zpn);
+        const double de = game * are / dpn;
+        const double flcf = o.fcf[inp];
+        const double ce = flcf < 0.0 ? flcf : 0.0, cp = flcf > 0.0 ? flcf : 0.0;
+        o.acfe[inp] = de - ce;
+        o.acfw[ine] = de + cp;
+        double fuds[3], fh[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fuds[c] = cp * vel[c][inp] + ce * vel[c][ine];
+        bool set = false;
+        if (lcds) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double ue = vel[c][inp] * fxp + vel[c][ine] * fxe +
+                                  (dxi[c][0] * (xf - xi) + dxi[c][1] * (yf - yi) + dxi[c][2] * (zf - zi));
+                fh[c] = flcf * ue;
+            }
+            set = true;
+        }
+        if (lcds4 && dir != 3) {
+            const double c48 = 1.0 / 48.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double *vq = vel[c];
+                fh[c] = flcf * (c48 * (27.0 * vq[inp] + 27.0 * vq[ine] - 3.0 * vq[inw] - 3.0 * vq[inee]));
+            }
+            set = true;
+        }
+        if (lluds) {
+            const double dx[3] = {xf - f.xc[inp], yf - f.yc[inp], zf - f.zc[inp]};
+            const double dxe[3] = {f.xc[ine] - xf, f.yc[ine] - yf, f.zc[ine] - zf};
+            const int cc = dir - 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double *g = grd[c], *vq = vel[c];
+                const double dfu = G3(g, inw, cc) * fxw + G3(g, inp, cc) * fxpw;
+                const double dfu1 = G3(g, inee, cc) * fxee + G3(g, ine, cc) * fxew;
+                fh[c] = ce * (vq[ine] + dxe[c] * dfu1) + cp * (vq[inp] + dx[c] * dfu);
+            }
+            set = true;
+        }
+        if (lquds) {
+            if (k > 2 && k < nk - 1 && j > 2 && j < nj - 1 && i > 2 && i < ni - 1) {
+                uvw_quick(f, D, o.fcf, inp, fh);
+                set = true;
+            } else if (!set) {
+                // the reference keeps the previous face's values: the last face
+                // of the previous row (north faces) or plane that QUICK set
+                int pk = k, pi = i;
+                bool any = false;
+                if (dir == 2 && i - 1 >= ist) { pi = i - 1; any = true; }
+                else if (dir != 3 && k - 1 >= kst) { pk = k - 1; pi = nimm; any = true; }
+                if (any) {
+                    const size_t pp = (size_t)(pk - 1) * nij + (size_t)(pi - 1) * nj + (size_t)(njmm - 1);
+                    uvw_quick(f, D, o.fcf, pp, fh);
+                }
+            }
+        }
+        if (flim) {
+            double psi[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double *g = grd[c], *vq = vel[c];
+                const double rw = (2.0 * G3(g, inp, 0) * xpn + 2.0 * G3(g, inp, 1) * ypn + 2.0 * G3(g, inp, 2) * zpn) / (vq[ine] - vq[inp]) - 1.0;
+                const double re = (2.0 * G3(g, ine, 0) * xpn + 2.0 * G3(g, ine, 1) * ypn + 2.0 * G3(g, ine, 2) * zpn) / (vq[inp] - vq[ine]) - 1.0;
+                const double rr[2] = {rw, re};
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const double x = rr[s];
+                    double ps = 0.0;
+                    if (flags & CAP_UVW_SMART) ps = fmax_f(0.0, fmin_f(fmin_f(2.0 * x, 0.75 * x + 0.25), 4.0));
+                    if (flags & CAP_UVW_AVL) ps = fmax_f(0.0, fmin_f(fmin_f(1.5 * x, 0.75 * x + 0.25), 2.5));
+                    if (flags & CAP_UVW_MUSCL) ps = fmax_f(0.0, fmin_f(fmin_f(2.0 * x, 0.5 * x + 0.5), 2.0));
+                    if (flags & CAP_UVW_UMIST) ps = fmax_f(0.0, fmin_f(fmin_f(fmin_f(2.0 * x, 0.75 * x + 0.25), 0.25 * x + 0.75), 2.0));
+                    if (flags & CAP_UVW_GAMMA) ps = fmax_f(0.0, fmin_f(x, 2.0 * x / (x + 1.0)));
+                    psi[3 * s + c] = ps;
+                }
+                fh[c] = ce * (vq[ine] + fxe * psi[3 + c] * (vq[inp] - vq[ine])) +
+                        cp * (vq[inp] + fxp * psi[c] * (vq[ine] - vq[inp]));
+            }
+            set = true;
+        }
+        double *srcs[3] = {o.su, o.sv, o.sw}, *scr[3] = {o.bp, o.bt, o.bb};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            srcs[c][inp] = srcs[c][inp] + gam * (fuds[c] - fh[c]) + fde[c] - fdi[c];
+            scr[c][ine] = -gam * (fuds[c] - fh[c]) - fde[c] + fdi[c];
+        }
+    }
+}
+
+// second loop (632-644): the east cell's share, k=2..nkmm, i=2..nimm, j=2..njmm
+__global__ __launch_bounds__(256) void k_fluxuvw_east(int ni, int nj, int nk, int idew, UvwOut o)
+{
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const int NJ = njmm - 1, NI = nimm - 1;
+    const size_t nf = (size_t)(nkmm - 1) * NI * NJ;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nf; q += (size_t)gridDim.x * 256) {
+        const int k = 2 + (int)(q / ((size_t)NI * NJ));
+        const int r = (int)(q % ((size_t)NI * NJ));
+        const int i = 2 + r / NJ, j = 2 + r % NJ;
+        const size_t p = (size_t)(k - 1) * ni * nj + (size_t)(i - 1) * nj + (size_t)(j - 1) + idew;
+        o.su[p] = o.su[p] + o.bp[p];
+        o.sv[p] = o.sv[p] + o.bt[p];
+        o.sw[p] = o.sw[p] + o.bb[p];
+    }
+}
+
 // continuityErrors.h:63-77: global and sum|local| over k=2..nkm, i,j=3..n?mm
 __global__ __launch_bounds__(256) void k_continuity(int ni, int nj, int nk,
                                                     const double *__restrict__ f1,
@@ -663,6 +863,51 @@ extern "C" int cap_fluxmc(cap_ctx *ctx, const cap_fields *F, int idew, int idns,
                        ctx->nj, ctx->nk, f, D, (double *)t);
     HIPCHK(hipGetLastError());
     if ((rc = unstage(ctx, fmcor, t))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_fluxuvw(cap_ctx *ctx, const cap_fields *F, const double *vis, int idew,
+                           int idns, int idtb, const double *arvx, const double *arvy,
+                           const double *arvz, const double *fif, double *acfe, double *acfw,
+                           const double *fcf, double *su, double *sv, double *sw, double *bp,
+                           double *bt, double *bb, int schemes, double gds)
+{
+    if (!ctx || !F || !vis || !arvx || !arvy || !arvz || !fif || !acfe || !acfw || !fcf || !su ||
+        !sv || !sw || !bp || !bt || !bb || !dir_ok(ctx, idew, idns, idtb) ||
+        (schemes & ~CAP_UVW_ALL) || ctx->ni < 6 || ctx->nj < 6 || ctx->nk < 6)
+        return CAP_ERR_ARG;
+    Fld f;
+    FaceDir D;
+    UvwOut o;
+    int rc;
+    double *a, *b, *unused;
+    const double *t[8];
+    if ((rc = stage_fields(ctx, F, f)) || (rc = stage_dir(ctx, idew, idns, idtb, arvx, arvy, arvz, fif, D)) ||
+        (rc = out3(ctx, acfe, acfw, nullptr, &a, &b, &unused)))
+        return rc;
+    const double *src[8] = {vis, fcf, su, sv, sw, bp, bt, bb};
+    for (int q = 0; q < 8; ++q)
+        if ((rc = stage(ctx, 40 + q, src[q], &t[q]))) return rc;
+    o.vis = t[0]; o.fcf = t[1]; o.acfe = a; o.acfw = b;
+    o.su = (double *)t[2]; o.sv = (double *)t[3]; o.sw = (double *)t[4];
+    o.bp = (double *)t[5]; o.bt = (double *)t[6]; o.bb = (double *)t[7];
+    const int dir = idew == ctx->nj ? 1 : (idew == 1 ? 2 : 3);
+    const int kst = dir == 3 ? 2 : 3, ist = dir == 1 ? 2 : 3, jst = dir == 2 ? 2 : 3;
+    const size_t nf = (size_t)(ctx->nk - 2 - kst + 1) * (ctx->ni - 2 - ist + 1) * (ctx->nj - 2 - jst + 1);
+    hipLaunchKernelGGL(k_fluxuvw, dim3(blocks_for(nf)), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj,
+                       ctx->nk, f, D, o, schemes, gds, dir, kst, ist, jst);
+    const size_t ne = (size_t)(ctx->nk - 3) * (ctx->ni - 3) * (ctx->nj - 3);
+    hipLaunchKernelGGL(k_fluxuvw_east, dim3(blocks_for(ne)), dim3(256), 0, ctx->stream, ctx->ni,
+                       ctx->nj, ctx->nk, idew, o);
+    for (int q = 5; q < 8; ++q)
+        hipLaunchKernelGGL(k_zero_st, dim3(blocks_for(ctx->nstd)), dim3(256), 0, ctx->stream,
+                           (double *)t[q], ctx->nstd);
+    HIPCHK(hipGetLastError());
+    if ((rc = unstage(ctx, acfe, a)) || (rc = unstage(ctx, acfw, b)) ||
+        (rc = unstage(ctx, su, t[2])) || (rc = unstage(ctx, sv, t[3])) || (rc = unstage(ctx, sw, t[4])) ||
+        (rc = unstage(ctx, bp, t[5])) || (rc = unstage(ctx, bt, t[6])) || (rc = unstage(ctx, bb, t[7])))
+        return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CAP_OK;
 }

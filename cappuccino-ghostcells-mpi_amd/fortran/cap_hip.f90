@@ -14,6 +14,12 @@ module cap_hip
   integer(c_int), parameter :: CAP_ERR_ARG = -1, CAP_ERR_HIP = -2, CAP_ERR_TIMEOUT = -3, &
                                CAP_ERR_NOMEM = -4, CAP_ERR_NODEV = -5
   integer(c_int), parameter :: CAP_DEVICE_POINTERS = 1
+  ! convection schemes of cap_fluxuvw (the reference's lcds, lcds4, lluds,
+  ! lquds, flux_limiter, lsmart, lavl, lmuscl, lumist, lgamma)
+  integer(c_int), parameter :: CAP_UVW_CDS = 1, CAP_UVW_CDS4 = 2, CAP_UVW_LUDS = 4, &
+                               CAP_UVW_QUICK = 8, CAP_UVW_LIMITER = 16, CAP_UVW_SMART = 32, &
+                               CAP_UVW_AVL = 64, CAP_UVW_MUSCL = 128, CAP_UVW_UMIST = 256, &
+                               CAP_UVW_GAMMA = 512
 
   ! module `coef` + `coefb` arrays (modules_allocatable.f90:161-180)
   type, bind(c) :: cap_coef
@@ -131,6 +137,14 @@ module cap_hip
       type(c_ptr), value :: ctx, fif, fmcor
       type(cap_fields), intent(in) :: f
       integer(c_int), value :: idew, idns, idtb
+    end function
+    integer(c_int) function cap_fluxuvw(ctx, f, vis, idew, idns, idtb, arvx, arvy, arvz, fif, &
+        acfe, acfw, fcf, su, sv, sw, bp, bt, bb, schemes, gds) bind(c, name='cap_fluxuvw')
+      import :: c_ptr, c_int, c_double, cap_fields
+      type(c_ptr), value :: ctx, vis, arvx, arvy, arvz, fif, acfe, acfw, fcf, su, sv, sw, bp, bt, bb
+      type(cap_fields), intent(in) :: f
+      integer(c_int), value :: idew, idns, idtb, schemes
+      real(c_double), value :: gds
     end function
     integer(c_int) function cap_continuity(ctx, f1, f2, f3, sums) bind(c, name='cap_continuity')
       import :: c_ptr, c_int, c_double

@@ -35,7 +35,7 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_iccg", "cap_cgstab_sip", "cap_pcgsip_mixed", "cap_grad_gauss", "cap_bpres",
            "cap_build_lognormal", "cap_time_kernels", "cap_fluxmass",
            "cap_piso_assemble_pressure_eq", "cap_assemble_pressure_eq", "cap_fluxmc",
-           "cap_continuity")
+           "cap_continuity", "cap_fluxuvw")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -89,6 +89,7 @@ def load():
     L.cap_assemble_pressure_eq.argtypes = [vp, F, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.cap_fluxmc.argtypes = [vp, F, i, i, i, vp, vp]
     L.cap_continuity.argtypes = [vp, vp, vp, vp, C.POINTER(d)]
+    L.cap_fluxuvw.argtypes = [vp, F, vp, i, i, i] + [vp] * 13 + [i, d]
     _lib = L
     return L
 
@@ -228,6 +229,17 @@ class Context:
             self._h, C.byref(st), *self._dir(d), *map(_ptr, arv), _ptr(f[("fx", "fy", "fz")[d]]),
             _ptr(acfe), _ptr(acfw), _ptr(fcf), _ptr(su), _ptr(sv), _ptr(sw))
         self._check(rc, "cap_assemble_pressure_eq", allow=(CAP_OK,))
+
+    def fluxuvw(self, f, d, vis, acfe, acfw, fcf, su, sv, sw, bp, bt, bb, schemes, gds):
+        """Momentum face fluxes of direction d (0 east, 1 north, 2 top);
+        schemes: CAP_UVW_* bits (the reference's lcds, lcds4, ... flags)."""
+        arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
+        st = self._fields(f)
+        rc = self._lib.cap_fluxuvw(
+            self._h, C.byref(st), _ptr(vis), *self._dir(d), *map(_ptr, arv),
+            _ptr(f[("fx", "fy", "fz")[d]]), _ptr(acfe), _ptr(acfw), _ptr(fcf), _ptr(su),
+            _ptr(sv), _ptr(sw), _ptr(bp), _ptr(bt), _ptr(bb), int(schemes), float(gds))
+        self._check(rc, "cap_fluxuvw", allow=(CAP_OK,))
 
     def fluxmc(self, f, d, fmcor):
         st = self._fields(f)

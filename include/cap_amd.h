@@ -142,6 +142,35 @@ int cap_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *f, int idew, int id
 int cap_fluxmc(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
                const double *fif, double *fmcor);
 
+/* Convection schemes of cap_fluxuvw: the reference's module logicals
+ * lcds, lcds4, lluds, lquds, flux_limiter and the limiter choice lsmart,
+ * lavl, lmuscl, lumist, lgamma (modules_allocatable.f90:58, 90), applied in
+ * the reference's order (a later one overrides an earlier one). */
+#define CAP_UVW_CDS      1
+#define CAP_UVW_CDS4     2
+#define CAP_UVW_LUDS     4
+#define CAP_UVW_QUICK    8
+#define CAP_UVW_LIMITER  16
+#define CAP_UVW_SMART    32
+#define CAP_UVW_AVL      64
+#define CAP_UVW_MUSCL    128
+#define CAP_UVW_UMIST    256
+#define CAP_UVW_GAMMA    512
+#define CAP_UVW_ALL      1023
+
+/* Replaces `call fluxuvw(idew, idns, idtb, arvx, arvy, arvz, fif, acfe, acfw,
+ * fcf)` (fluxuvw-overrelaxed-correction.f90:3-659, called for the three
+ * directions at calcuvw.f90:143-149): UDS coefficients acfe(P) / acfw(E),
+ * explicit minus implicit diffusion and the deferred correction
+ * gds * (F_uds - F_high) added to su / sv / sw (module coef), the east cell's
+ * share through the scratch bp / bt / bb, which are zero on return.  vis is
+ * the module `variables` viscosity; gds the reference's gds(iu). */
+int cap_fluxuvw(cap_ctx *ctx, const cap_fields *f, const double *vis, int idew, int idns,
+                int idtb, const double *arvx, const double *arvy, const double *arvz,
+                const double *fif, double *acfe, double *acfw, const double *fcf, double *su,
+                double *sv, double *sw, double *bp, double *bt, double *bb, int schemes,
+                double gds);
+
 /* The continuity-error sums of continuityErrors.h:55-81 over k=2..nkm,
  * i,j=3..n?mm: sums[0] = global (signed), sums[1] = sum of |local|.  FP64,
  * deterministic reduction order. */

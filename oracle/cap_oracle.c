@@ -69,6 +69,10 @@ static inline double ps_add(psum *ps, double v, int k)
 }
 #define S0INIT ((double)1e20f)
 
+/* Fortran MIN/MAX of two ordered values (no NaN inputs) */
+static inline double orc_min2(double a, double b) { return b < a ? b : a; }
+static inline double orc_max2(double a, double b) { return b > a ? b : a; }
+
 /* 1-based (i,j,k) -> C offset; li/lk of setind.f90:34-40 */
 #define IX(i, j, k) ((size_t)((k)-1) * (size_t)nij + (size_t)((i)-1) * (size_t)nj + (size_t)((j)-1))
 
@@ -929,4 +933,184 @@ void orc_continuity(int ni, int nj, int nk, const double *f1, const double *f2,
             }
     sums[0] = g;
     sums[1] = sl;
+}
+
+/* fluxuvw-overrelaxed-correction.f90:3-659: momentum face fluxes of one
+ * direction.  Face loop over the direction's range (99-118), UDS implicit
+ * coefficients acfe(P)/acfw(E) (276-288), explicit minus implicit diffusion
+ * with the orthogonal correction costn = 1 (178-269), deferred correction
+ * gam*(F_uds - F_high) with the schemes selected by `flags` (297-619); the P
+ * side goes into su/sv/sw (625-627), the E side through the scratch
+ * bp/bt/bb (628-630) added in a second loop (632-644), after which the
+ * scratch is zeroed (652-656).  As in the reference, fuhigh/fvhigh/fwhigh
+ * keep their value from the previous face when no scheme sets them (QUICK
+ * skips faces next to the first cell of the range). */
+void orc_fluxuvw(int ni, int nj, int nk, const orc_fields *f, const double *vis, int idew,
+                 int idns, int idtb, const double *arvx, const double *arvy, const double *arvz,
+                 const double *fif, double *acfe, double *acfw, const double *fcf, double *su,
+                 double *sv, double *sw, double *bp, double *bt, double *bb, int flags,
+                 double gds)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const int lcds = flags & ORC_UVW_CDS, lcds4 = flags & ORC_UVW_CDS4, lluds = flags & ORC_UVW_LUDS;
+    const int lquds = flags & ORC_UVW_QUICK, flim = flags & ORC_UVW_LIMITER;
+    const double gam = gds;
+    double psie1 = 0.0, psie2 = 0.0, psie3 = 0.0, psiw1 = 0.0, psiw2 = 0.0, psiw3 = 0.0;
+    double fuhigh = 0.0, fvhigh = 0.0, fwhigh = 0.0;
+    int dir = 0;
+    if (idew == nj) dir = 1;
+    if (idew == 1) dir = 2;
+    if (idew == nij) dir = 3;
+    int kst = 2, kend = nkmm, ist = 2, iend = nimm, jst = 2, jend = njmm;
+    if (dir == 1) { jst = 3; kst = 3; }
+    else if (dir == 2) { ist = 3; kst = 3; }
+    else if (dir == 3) { ist = 3; jst = 3; }
+    const double *gu = f->gradu, *gv = f->gradv, *gw = f->gradw;
+    const double *u = f->u, *v = f->v, *w = f->w;
+    for (int k = kst; k <= kend; ++k)
+        for (int i = ist; i <= iend; ++i)
+            for (int j = jst; j <= jend; ++j) {
+                const size_t inp = IX(i, j, k);
+                const size_t ine = inp + idew, inw = inp - idew, ins = inp - idns;
+                const size_t inb = inp - idtb, inee = inp + 2 * (size_t)idew, inbs = inb - idns;
+                const double fxe = fif[inp], fxp = 1.0 - fxe;
+                const double fxpw = fif[inw], fxw = 1.0 - fxpw;
+                const double fxee = fif[ine], fxew = 1.0 - fxee;
+                const double xpn = f->xc[ine] - f->xc[inp], ypn = f->yc[ine] - f->yc[inp],
+                             zpn = f->zc[ine] - f->zc[inp];
+                const double dpn = sqrt(xpn * xpn + ypn * ypn + zpn * zpn);
+                const double arx = arvx[inp], ary = arvy[inp], arz = arvz[inp];
+                const double are = sqrt(arx * arx + ary * ary + arz * arz);
+                const double costn = 1.0;
+                const double vole = xpn * arx + ypn * ary + zpn * arz;
+                const double d1x = costn, d1y = costn, d1z = costn;
+                const double d2x = xpn * costn, d2y = ypn * costn, d2z = zpn * costn;
+                const double game = vis[inp] * fxp + vis[ine] * fxe;
+                const double xf = 0.25 * (f->x[inp] + f->x[ins] + f->x[inb] + f->x[inbs]);
+                const double yf = 0.25 * (f->y[inp] + f->y[ins] + f->y[inb] + f->y[inbs]);
+                const double zf = 0.25 * (f->z[inp] + f->z[ins] + f->z[inb] + f->z[inbs]);
+                const double xi = f->xc[inp] * fxp + f->xc[ine] * fxe;
+                const double yi = f->yc[inp] * fxp + f->yc[ine] * fxe;
+                const double zi = f->zc[inp] * fxp + f->zc[ine] * fxe;
+                const double duxi = G3(gu, inp, 0) * fxp + G3(gu, ine, 0) * fxe;
+                const double duyi = G3(gu, inp, 1) * fxp + G3(gu, ine, 1) * fxe;
+                const double duzi = G3(gu, inp, 2) * fxp + G3(gu, ine, 2) * fxe;
+                const double du = u[ine] - u[inp] - duxi * d2x - duyi * d2y - duzi * d2z;
+                const double duxii = duxi * d1x + arx / vole * du;
+                const double duyii = duyi * d1y + ary / vole * du;
+                const double duzii = duzi * d1z + arz / vole * du;
+                const double dvxi = G3(gv, inp, 0) * fxp + G3(gv, ine, 0) * fxe;
+                const double dvyi = G3(gv, inp, 1) * fxp + G3(gv, ine, 1) * fxe;
+                const double dvzi = G3(gv, inp, 2) * fxp + G3(gv, ine, 2) * fxe;
+                const double dv = v[ine] - v[inp] - dvxi * d2x - dvyi * d2y - dvzi * d2z;
+                const double dvxii = dvxi * d1x + arx / vole * dv;
+                const double dvyii = dvyi * d1y + ary / vole * dv;
+                const double dvzii = dvzi * d1z + arz / vole * dv;
+                const double dwxi = G3(gw, inp, 0) * fxp + G3(gw, ine, 0) * fxe;
+                const double dwyi = G3(gw, inp, 1) * fxp + G3(gw, ine, 1) * fxe;
+                const double dwzi = G3(gw, inp, 2) * fxp + G3(gw, ine, 2) * fxe;
+                const double dw = w[ine] - w[inp] - dwxi * d2x - dwyi * d2y - dwzi * d2z;
+                const double dwxii = dwxi * d1x + arx / vole * dw;
+                const double dwyii = dwyi * d1y + ary / vole * dw;
+                const double dwzii = dwzi * d1z + arz / vole * dw;
+                const double fdue = game * ((duxii + duxii) * arx + (duyii + dvxii) * ary + (duzii + dwxii) * arz);
+                const double fdve = game * ((duyii + dvxii) * arx + (dvyii + dvyii) * ary + (dvzii + dwyii) * arz);
+                const double fdwe = game * ((duzii + dwxii) * arx + (dwyii + dvzii) * ary + (dwzii + dwzii) * arz);
+                const double fdui = game * are / dpn * (duxi * xpn + duyi * ypn + duzi * zpn);
+                const double fdvi = game * are / dpn * (dvxi * xpn + dvyi * ypn + dvzi * zpn);
+                const double fdwi = game * are / dpn * (dwxi * xpn + dwyi * ypn + dwzi * zpn);
+                const double de = game * are / dpn;
+                const double flcf = fcf[inp];
+                const double ce = flcf < 0.0 ? flcf : 0.0;
+                const double cp = flcf > 0.0 ? flcf : 0.0;
+                acfe[inp] = de - ce;
+                acfw[ine] = de + cp;
+                const double fuuds = cp * u[inp] + ce * u[ine];
+                const double fvuds = cp * v[inp] + ce * v[ine];
+                const double fwuds = cp * w[inp] + ce * w[ine];
+                if (lcds) {
+                    const double ue = u[inp] * fxp + u[ine] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+                    const double ve = v[inp] * fxp + v[ine] * fxe + (dvxi * (xf - xi) + dvyi * (yf - yi) + dvzi * (zf - zi));
+                    const double we = w[inp] * fxp + w[ine] * fxe + (dwxi * (xf - xi) + dwyi * (yf - yi) + dwzi * (zf - zi));
+                    fuhigh = flcf * ue;
+                    fvhigh = flcf * ve;
+                    fwhigh = flcf * we;
+                }
+                if (lcds4 && (idew == nj || idew == 1)) {
+                    const double c48 = 1.0 / 48.0;
+                    const double ue = c48 * (27.0 * u[inp] + 27.0 * u[ine] - 3.0 * u[inw] - 3.0 * u[inee]);
+                    const double ve = c48 * (27.0 * v[inp] + 27.0 * v[ine] - 3.0 * v[inw] - 3.0 * v[inee]);
+                    const double we = c48 * (27.0 * w[inp] + 27.0 * w[ine] - 3.0 * w[inw] - 3.0 * w[inee]);
+                    fuhigh = flcf * ue;
+                    fvhigh = flcf * ve;
+                    fwhigh = flcf * we;
+                }
+                if (lluds) {
+                    const double dxc = xf - f->xc[inp], dyc = yf - f->yc[inp], dzc = zf - f->zc[inp];
+                    const double dxe = f->xc[ine] - xf, dye = f->yc[ine] - yf, dze = f->zc[ine] - zf;
+                    const int c = dir - 1;
+                    const double dfidx_u = G3(gu, inw, c) * fxw + G3(gu, inp, c) * fxpw;
+                    const double dfidy_u = G3(gv, inw, c) * fxw + G3(gv, inp, c) * fxpw;
+                    const double dfidz_u = G3(gw, inw, c) * fxw + G3(gw, inp, c) * fxpw;
+                    const double dfidx_u1 = G3(gu, inee, c) * fxee + G3(gu, ine, c) * fxew;
+                    const double dfidy_u1 = G3(gv, inee, c) * fxee + G3(gv, ine, c) * fxew;
+                    const double dfidz_u1 = G3(gw, inee, c) * fxee + G3(gw, ine, c) * fxew;
+                    fuhigh = ce * (u[ine] + dxe * dfidx_u1) + cp * (u[inp] + dxc * dfidx_u);
+                    fvhigh = ce * (v[ine] + dye * dfidy_u1) + cp * (v[inp] + dyc * dfidy_u);
+                    fwhigh = ce * (w[ine] + dze * dfidz_u1) + cp * (w[inp] + dzc * dfidz_u);
+                }
+                if (lquds && k > 2 && k < nk - 1 && j > 2 && j < nj - 1 && i > 2 && i < ni - 1) {
+                    const double fw_ = fif[inp - idew], fp_ = fif[inp], fe_ = fif[inp + idew];
+                    const double g11 = ((2.0 - fw_) * (fp_ * fp_)) / (1.0 + fp_ - fw_);
+                    const double g12 = ((1.0 - fp_) * ((1.0 - fw_) * (1.0 - fw_))) / (1.0 + fp_ - fw_);
+                    const double g21 = ((1.0 + fe_) * ((1.0 - fp_) * (1.0 - fp_))) / (1.0 + fe_ - fp_);
+                    const double g22 = ((fe_ * fe_) * fp_) / (1.0 + fe_ - fp_);
+                    fuhigh = ce * (g21 * u[inp] + (1.0 - g21 + g22) * u[ine] - g22 * u[inee]) +
+                             cp * (g11 * u[ine] + (1.0 - g11 + g12) * u[inp] - g12 * u[inw]);
+                    fvhigh = ce * (g21 * v[inp] + (1.0 - g21 + g22) * v[ine] - g22 * v[inee]) +
+                             cp * (g11 * v[ine] + (1.0 - g11 + g12) * v[inp] - g12 * v[inw]);
+                    fwhigh = ce * (g21 * w[inp] + (1.0 - g21 + g22) * w[ine] - g22 * w[inee]) +
+                             cp * (g11 * w[ine] + (1.0 - g11 + g12) * w[inp] - g12 * w[inw]);
+                }
+                if (flim) {
+                    const double r1 = (2.0 * G3(gu, inp, 0) * xpn + 2.0 * G3(gu, inp, 1) * ypn + 2.0 * G3(gu, inp, 2) * zpn) / (u[ine] - u[inp]) - 1.0;
+                    const double r2 = (2.0 * G3(gv, inp, 0) * xpn + 2.0 * G3(gv, inp, 1) * ypn + 2.0 * G3(gv, inp, 2) * zpn) / (v[ine] - v[inp]) - 1.0;
+                    const double r3 = (2.0 * G3(gw, inp, 0) * xpn + 2.0 * G3(gw, inp, 1) * ypn + 2.0 * G3(gw, inp, 2) * zpn) / (w[ine] - w[inp]) - 1.0;
+                    const double r4 = (2.0 * G3(gu, ine, 0) * xpn + 2.0 * G3(gu, ine, 1) * ypn + 2.0 * G3(gu, ine, 2) * zpn) / (u[inp] - u[ine]) - 1.0;
+                    const double r5 = (2.0 * G3(gv, ine, 0) * xpn + 2.0 * G3(gv, ine, 1) * ypn + 2.0 * G3(gv, ine, 2) * zpn) / (v[inp] - v[ine]) - 1.0;
+                    const double r6 = (2.0 * G3(gw, ine, 0) * xpn + 2.0 * G3(gw, ine, 1) * ypn + 2.0 * G3(gw, ine, 2) * zpn) / (w[inp] - w[ine]) - 1.0;
+                    const double r[6] = {r1, r2, r3, r4, r5, r6};
+                    double psi[6] = {psiw1, psiw2, psiw3, psie1, psie2, psie3};
+                    for (int q = 0; q < 6; ++q) {
+                        const double x = r[q];
+                        if (flags & ORC_UVW_SMART) psi[q] = orc_max2(0.0, orc_min2(orc_min2(2.0 * x, 0.75 * x + 0.25), 4.0));
+                        if (flags & ORC_UVW_AVL) psi[q] = orc_max2(0.0, orc_min2(orc_min2(1.5 * x, 0.75 * x + 0.25), 2.5));
+                        if (flags & ORC_UVW_MUSCL) psi[q] = orc_max2(0.0, orc_min2(orc_min2(2.0 * x, 0.5 * x + 0.5), 2.0));
+                        if (flags & ORC_UVW_UMIST) psi[q] = orc_max2(0.0, orc_min2(orc_min2(orc_min2(2.0 * x, 0.75 * x + 0.25), 0.25 * x + 0.75), 2.0));
+                        if (flags & ORC_UVW_GAMMA) psi[q] = orc_max2(0.0, orc_min2(x, 2.0 * x / (x + 1.0)));
+                    }
+                    psiw1 = psi[0]; psiw2 = psi[1]; psiw3 = psi[2];
+                    psie1 = psi[3]; psie2 = psi[4]; psie3 = psi[5];
+                    fuhigh = ce * (u[ine] + fxe * psie1 * (u[inp] - u[ine])) + cp * (u[inp] + fxp * psiw1 * (u[ine] - u[inp]));
+                    fvhigh = ce * (v[ine] + fxe * psie2 * (v[inp] - v[ine])) + cp * (v[inp] + fxp * psiw2 * (v[ine] - v[inp]));
+                    fwhigh = ce * (w[ine] + fxe * psie3 * (w[inp] - w[ine])) + cp * (w[inp] + fxp * psiw3 * (w[ine] - w[inp]));
+                }
+                su[inp] = su[inp] + gam * (fuuds - fuhigh) + fdue - fdui;
+                sv[inp] = sv[inp] + gam * (fvuds - fvhigh) + fdve - fdvi;
+                sw[inp] = sw[inp] + gam * (fwuds - fwhigh) + fdwe - fdwi;
+                bp[ine] = -gam * (fuuds - fuhigh) - fdue + fdui;
+                bt[ine] = -gam * (fvuds - fvhigh) - fdve + fdvi;
+                bb[ine] = -gam * (fwuds - fwhigh) - fdwe + fdwi;
+            }
+    for (int k = 2; k <= nkmm; ++k)
+        for (int i = 2; i <= nimm; ++i)
+            for (int j = 2; j <= njmm; ++j) {
+                const size_t inp = IX(i, j, k) + idew;
+                su[inp] = su[inp] + bp[inp];
+                sv[inp] = sv[inp] + bt[inp];
+                sw[inp] = sw[inp] + bb[inp];
+            }
+    const size_t n = (size_t)nij * nk;
+    for (size_t c = 0; c < n; ++c) { bp[c] = 0.0; bt[c] = 0.0; bb[c] = 0.0; }
 }

@@ -116,6 +116,25 @@ void orc_fluxmc_all(int ni, int nj, int nk, const orc_fields *f, int idew, int i
                     const double *fif, double *fmcor);
 void orc_continuity(int ni, int nj, int nk, const double *f1, const double *f2,
                     const double *f3, double *sums);
+
+/* scheme flags of orc_fluxuvw: the reference's logicals lcds, lcds4, lluds,
+ * lquds, flux_limiter and the limiter choice lsmart, lavl, lmuscl, lumist,
+ * lgamma (module parameters of modules_allocatable.f90) */
+#define ORC_UVW_CDS     1
+#define ORC_UVW_CDS4    2
+#define ORC_UVW_LUDS    4
+#define ORC_UVW_QUICK   8
+#define ORC_UVW_LIMITER 16
+#define ORC_UVW_SMART   32
+#define ORC_UVW_AVL     64
+#define ORC_UVW_MUSCL   128
+#define ORC_UVW_UMIST   256
+#define ORC_UVW_GAMMA   512
+void orc_fluxuvw(int ni, int nj, int nk, const orc_fields *f, const double *vis, int idew,
+                 int idns, int idtb, const double *arvx, const double *arvy, const double *arvz,
+                 const double *fif, double *acfe, double *acfw, const double *fcf, double *su,
+                 double *sv, double *sw, double *bp, double *bt, double *bb, int flags,
+                 double gds);
 #ifdef __cplusplus
 }
 #endif

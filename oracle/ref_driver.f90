@@ -13,9 +13,10 @@
 !
 ! usage: ref_driver <mode> <in.bin> <out.bin> <log.txt>
 !   mode = sipsol | cgstab_sip | grad_gauss | bpres | fluxmass | piso_assemble
-!          | assemble_p | fluxmc | continuity
+!          | assemble_p | fluxmc | continuity | fluxuvw
 ! in.bin  : int32 ni,nj,nk,nsw ; real64 alfa,sor ; then mode-specific arrays
-!           (stencil modes: nsw = direction 0 east, 1 north, 2 top; alfa = ipc)
+!           (stencil modes: nsw = direction 0 east, 1 north, 2 top; alfa = ipc;
+!            fluxuvw: nsw = direction + 8 * scheme flags, alfa = gds(iu))
 ! out.bin : mode-specific arrays, then real64 resor(ip) for the solvers
 !-----------------------------------------------------------------------
 program ref_driver
@@ -32,7 +33,7 @@ program ref_driver
   integer :: ni_, nj_, nk_, nsw_, n
   real(8) :: alfa_, sor_
   real(8), allocatable :: fi(:), d1(:), d2(:), d3(:), bsc(:)
-  integer :: idew, idns, idtb, inp, i, j, k
+  integer :: idew, idns, idtb, inp, i, j, k, fl, dirn
   real(8) :: fmcor
 
   call get_command_argument(1, mode)
@@ -67,6 +68,7 @@ program ref_driver
   allocate(x(n),y(n),z(n),xc(n),yc(n),zc(n),den(n),apu(n),apv(n),apw(n))
   allocate(u(n),v(n),w(n),p(n),sv(n),sw(n),f1(n),f2(n),f3(n))
   allocate(gradu(3,n),gradv(3,n),gradw(3,n),gradp(3,n))
+  allocate(vis(n))
 
   open(unit=66, file=trim(flog), status='replace')
   open(unit=11, file=trim(fout), access='stream', form='unformatted', status='replace')
@@ -133,6 +135,30 @@ program ref_driver
       end do
     end select
     write(11) d1, d2, d3, bsc, su
+  case ('fluxuvw')
+    read(10) x, y, z, xc, yc, zc, vol, fx, fy, fz, ar1x, ar1y, ar1z, ar2x, ar2y, ar2z, &
+             ar3x, ar3y, ar3z, den, apu, apv, apw, u, v, w, p, gradu, gradv, gradw, gradp, &
+             su, sv, sw, d1, d2, d3, bsc
+    read(10) vis, bt, bb
+    bp = bsc
+    dirn = mod(nsw_, 8)
+    fl = nsw_ / 8
+    lcds = iand(fl, 1) /= 0
+    lcds4 = iand(fl, 2) /= 0
+    lluds = iand(fl, 4) /= 0
+    lquds = iand(fl, 8) /= 0
+    flux_limiter = iand(fl, 16) /= 0
+    lsmart = iand(fl, 32) /= 0
+    lavl = iand(fl, 64) /= 0
+    lmuscl = iand(fl, 128) /= 0
+    lumist = iand(fl, 256) /= 0
+    lgamma = iand(fl, 512) /= 0
+    gds(iu) = alfa_
+    zero = 0.0d0
+    if (dirn == 0) call fluxuvw(nj, 1, nij, ar1x, ar1y, ar1z, fx, d1, d2, d3)
+    if (dirn == 1) call fluxuvw(1, nij, nj, ar2x, ar2y, ar2z, fy, d1, d2, d3)
+    if (dirn == 2) call fluxuvw(nij, nj, 1, ar3x, ar3y, ar3z, fz, d1, d2, d3)
+    write(11) d1, d2, su, sv, sw, bp, bt, bb
   case ('continuity')
     read(10) f1, f2, f3
     call continuity_sums()

@@ -88,7 +88,24 @@ def fields(dims, seed):
     f["su"], f["sv"], f["sw"] = rng.normal(size=n), rng.normal(size=n), rng.normal(size=n)
     f["out0"] = [rng.normal(size=n) for _ in range(4)]   # acfe, acfw, fcf, bp before the call
     f["f123"] = [rng.normal(size=n) for _ in range(3)]
+    f["vis"] = rng.uniform(0.01, 0.1, n)
+    f["bt0"], f["bb0"] = rng.normal(size=n), rng.normal(size=n)   # bt, bb before the call
     return f
+
+
+# fluxuvw scheme flags (the reference's module logicals), as the oracle's
+# ORC_UVW_* bits: cds 1, cds4 2, luds 4, quick 8, flux_limiter 16, smart 32,
+# avl 64, muscl 128, umist 256, gamma 512
+UVW_SCHEMES = {"cds": 1, "cds4": 2, "luds": 4, "quick": 8, "cds_quick": 9,
+               "smart": 16 | 32, "avl": 16 | 64, "muscl": 16 | 128, "umist": 16 | 256,
+               "gamma": 16 | 512, "uds": 0}
+
+
+def fluxuvw_inputs(f):
+    """Array list of the reference driver's fluxuvw mode (fields, su, sv, sw,
+    acfe, acfw, fcf, bp, then vis, bt, bb)."""
+    return [f[q] for q in FIELD_NAMES] + [f["su"], f["sv"], f["sw"]] + f["out0"] + \
+        [f["vis"], f["bt0"], f["bb0"]]
 
 
 # (name, routine, kind, dims, seed, nsw, alfa, full arrays?)
@@ -105,6 +122,13 @@ ASSEMBLY_CASES = [
     ("piso_assemble", (12, 10, 9), 303), ("piso_assemble_ipc0", (12, 10, 9), 304),
     ("assemble_p", (12, 10, 9), 305), ("fluxmc", (12, 10, 9), 306),
 ]
+FLUXUVW_CASES = [((12, 10, 9), 501, "cds", 1.0), ((12, 10, 9), 502, "luds", 1.0),
+                 ((12, 10, 9), 503, "quick", 1.0), ((12, 10, 9), 504, "cds4", 0.7),
+                 ((12, 10, 9), 505, "smart", 1.0), ((12, 10, 9), 506, "gamma", 1.0),
+                 ((12, 10, 9), 507, "cds_quick", 0.5), ((12, 10, 9), 508, "umist", 1.0),
+                 ((12, 10, 9), 509, "avl", 1.0), ((12, 10, 9), 510, "muscl", 1.0),
+                 ((12, 10, 9), 511, "uds", 1.0), ((20, 17, 11), 512, "cds", 1.0)]
+UVW_OUT = ("acfe", "acfw", "su", "sv", "sw", "bp", "bt", "bb")
 STENCIL_CASES = [
     ("grad_gauss_12x10x9", "grad_gauss", (12, 10, 9), 201),
     ("grad_gauss_20x37x23", "grad_gauss", (20, 37, 23), 202),
@@ -144,6 +168,17 @@ def main():
             for q, a in zip(("acfe", "acfw", "fcf", "bp", "su"), outs):
                 rec[f"{q}_{d}"] = a
         name = f"{routine}_{dims[0]}x{dims[1]}x{dims[2]}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print(f"{name}: written")
+    for dims, seed, scheme, gds in FLUXUVW_CASES:
+        f = fields(dims, seed)
+        rec = dict(dims=np.array(dims), seed=seed, flags=UVW_SCHEMES[scheme], gds=gds)
+        for d in range(3):
+            outs, _, _ = O.run_ref("fluxuvw", dims, fluxuvw_inputs(f),
+                                   nsw=d + 8 * UVW_SCHEMES[scheme], alfa=gds)
+            for q, a in zip(UVW_OUT, outs):
+                rec[f"{q}_{d}"] = a
+        name = f"fluxuvw_{scheme}_{dims[0]}x{dims[1]}x{dims[2]}"
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
         print(f"{name}: written")
     for dims, seed in (((12, 10, 9), 401), ((20, 17, 11), 402)):
