@@ -600,6 +600,132 @@ __global__ __launch_bounds__(256) void k_fluxuvw_east(int ni, int nj, int nk, in
     }
 }
 
+// ------------------------------------------------------ ghost-cell values
+// update_values_at_ghostcells_nonperiodic.f90: one thread per boundary cell
+// of the six faces (west, east, south, north, bottom, top); the six loops of
+// the reference write disjoint ghost cells and read only inner cells and the
+// ghost cell itself, so they run as one launch.
+struct GhostIO {
+    double *var[7];          // u, v, w, te, ed, t, vis
+    const double *geo[12];   // fx, fy, fz, ar1x..ar3z
+    const int *lb[6];        // lbw, lbe, lbs, lbn, lbb, lbt
+    double inl[6];           // uin, vin, win, tein, edin, tin
+};
+
+__device__ __forceinline__ void ghost_cell(const GhostIO &g, size_t inp, size_t ingc, size_t nb,
+                                           double fxp, double fxe, const double *ext_f,
+                                           size_t ext_c, int ext_sign, const double *ax,
+                                           const double *ay, const double *az, size_t iface,
+                                           int lb, bool side_wall_vis)
+{
+    double *u = g.var[0], *v = g.var[1], *w = g.var[2], *te = g.var[3], *ed = g.var[4],
+           *t = g.var[5], *vis = g.var[6];
+    const double fxer = 1.0 / fxe;
+    if (lb == 1) {   // inlet
+        u[ingc] = (g.inl[0] - u[inp] * fxp) * fxer;
+        v[ingc] = (g.inl[1] - v[inp] * fxp) * fxer;
+        w[ingc] = (g.inl[2] - w[inp] * fxp) * fxer;
+        te[ingc] = (g.inl[3] - te[inp] * fxp) * fxer;
+        ed[ingc] = (g.inl[4] - ed[inp] * fxp) * fxer;
+        t[ingc] = (g.inl[5] - t[inp] * fxp) * fxer;
+        vis[ingc] = 0.09 * (te[ingc] * te[ingc]) / ed[ingc];
+    } else if (lb == 2) {   // outlet: linear extrapolation from two inner cells
+#pragma unroll
+        for (int c = 0; c < 7; ++c) {
+            double *q = g.var[c];
+            q[ingc] = ext_sign < 0 ? q[inp] - (q[inp + nb] - q[inp]) * ext_f[ext_c]
+                                   : q[inp] + (q[inp] - q[inp - nb]) * (1.0 - ext_f[ext_c]);
+        }
+    } else if (lb == 3) {   // symmetry
+        const double un = u[inp] * ax[iface] + v[inp] * ay[iface] + w[inp] * az[iface];
+        u[ingc] = u[inp] - un * ax[iface];
+        v[ingc] = v[inp] - un * ay[iface];
+        w[ingc] = w[inp] - un * az[iface];
+        te[ingc] = te[inp];
+        ed[ingc] = ed[inp];
+        t[ingc] = t[inp];
+        vis[ingc] = vis[inp];
+    } else if (lb == 4) {   // wall
+        u[ingc] = (-u[inp] * fxp) * fxer;
+        v[ingc] = (-v[inp] * fxp) * fxer;
+        w[ingc] = (-w[inp] * fxp) * fxer;
+        te[ingc] = (-te[inp] * fxp) * fxer;
+        ed[ingc] = ed[inp];
+        t[ingc] = (g.inl[5] - t[inp] * fxp) * fxer;
+        vis[ingc] = side_wall_vis ? (vis[ingc] - vis[inp] * fxp) * fxer : vis[inp];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ghostcells(int ni, int nj, int nk, GhostIO g)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const size_t nJK = (size_t)(nj - 4) * (nk - 4), nIK = (size_t)(ni - 4) * (nk - 4),
+                 nIJ = (size_t)(ni - 4) * (nj - 4);
+    const size_t total = 2 * (nJK + nIK + nIJ);
+    const double *fx = g.geo[0], *fy = g.geo[1], *fz = g.geo[2];
+    auto IXd = [&](int i, int j, int k) {
+        return (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+    };
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < total; q += (size_t)gridDim.x * 256) {
+        size_t r = q;
+        if (r < 2 * nJK) {   // west / east: (k, j)
+            const bool east = r >= nJK;
+            if (east) r -= nJK;
+            const int k = 3 + (int)(r / (nj - 4)), j = 3 + (int)(r % (nj - 4));
+            const int lb = (east ? g.lb[1] : g.lb[0])[(j - 1) * nk + k - 1];
+            if (!east) {
+                const size_t inp = IXd(3, j, k);
+                const double fxp = fx[inp - nj];
+                ghost_cell(g, inp, inp - nj, nj, fxp, 1.0 - fxp, fx, inp, -1, g.geo[3], g.geo[4],
+                           g.geo[5], inp - nj, lb, true);
+            } else {
+                const size_t inp = IXd(nimm, j, k);
+                const double fxp = 1.0 - fx[inp];
+                ghost_cell(g, inp, inp + nj, nj, fxp, 1.0 - fxp, fx, inp - nj, 1, g.geo[3],
+                           g.geo[4], g.geo[5], inp, lb, true);
+            }
+            continue;
+        }
+        r -= 2 * nJK;
+        if (r < 2 * nIK) {   // south / north: (k, i)
+            const bool north = r >= nIK;
+            if (north) r -= nIK;
+            const int k = 3 + (int)(r / (ni - 4)), i = 3 + (int)(r % (ni - 4));
+            const int lb = (north ? g.lb[3] : g.lb[2])[(i - 1) * nk + k - 1];
+            if (!north) {
+                const size_t inp = IXd(i, 3, k);
+                const double fxp = 1.0 - fy[inp];
+                ghost_cell(g, inp, inp - 1, 1, fxp, 1.0 - fxp, fy, inp, -1, g.geo[6], g.geo[7],
+                           g.geo[8], inp - 1, lb, true);
+            } else {
+                const size_t inp = IXd(i, njmm, k);
+                const double fxp = 1.0 - fy[inp];
+                ghost_cell(g, inp, inp + 1, 1, fxp, 1.0 - fxp, fy, inp - 1, 1, g.geo[6], g.geo[7],
+                           g.geo[8], inp, lb, true);
+            }
+            continue;
+        }
+        r -= 2 * nIK;
+        {   // bottom / top: (i, j)
+            const bool top = r >= nIJ;
+            if (top) r -= nIJ;
+            const int i = 3 + (int)(r / (nj - 4)), j = 3 + (int)(r % (nj - 4));
+            const int lb = (top ? g.lb[5] : g.lb[4])[(i - 1) * nj + j - 1];
+            if (!top) {
+                const size_t inp = IXd(i, j, 3), ingc = inp - nij;
+                const double fxp = fz[ingc];
+                ghost_cell(g, inp, ingc, nij, fxp, 1.0 - fxp, fz, inp, -1, g.geo[9], g.geo[10],
+                           g.geo[11], ingc, lb, false);
+            } else {
+                const size_t inp = IXd(i, j, nkmm);
+                const double fxp = 1.0 - fz[inp];
+                ghost_cell(g, inp, inp + nij, nij, fxp, 1.0 - fxp, fz, inp - nij, 1, g.geo[9],
+                           g.geo[10], g.geo[11], inp, lb, false);
+            }
+        }
+    }
+}
+
 // continuityErrors.h:63-77: global and sum|local| over k=2..nkm, i,j=3..n?mm
 __global__ __launch_bounds__(256) void k_continuity(int ni, int nj, int nk,
                                                     const double *__restrict__ f1,
@@ -908,6 +1034,63 @@ extern "C" int cap_fluxuvw(cap_ctx *ctx, const cap_fields *F, const double *vis,
         (rc = unstage(ctx, su, t[2])) || (rc = unstage(ctx, sv, t[3])) || (rc = unstage(ctx, sw, t[4])) ||
         (rc = unstage(ctx, bp, t[5])) || (rc = unstage(ctx, bt, t[6])) || (rc = unstage(ctx, bb, t[7])))
         return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+// device copy of `bytes` of a caller array (host-pointer mode), or the
+// pointer itself (device-pointer mode)
+static int stage_bytes(cap_ctx *ctx, int slot, const void *p, size_t bytes, const void **out)
+{
+    if (ctx->flags & CAP_DEVICE_POINTERS) { *out = p; return CAP_OK; }
+    if (slot < 0 || slot >= CAP_NSTAGE) return CAP_ERR_ARG;
+    const size_t cap = ctx->nstd * sizeof(double);
+    if (bytes > cap) return CAP_ERR_ARG;
+    if (ctx->Gsz[slot] < cap) {
+        if (ctx->G[slot]) (void)hipFree(ctx->G[slot]);
+        ctx->G[slot] = nullptr;
+        HIPCHK(hipMalloc(&ctx->G[slot], cap));
+        ctx->Gsz[slot] = cap;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->G[slot], p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    *out = ctx->G[slot];
+    return CAP_OK;
+}
+
+extern "C" int cap_update_ghostcells(cap_ctx *ctx, double *const *var, const double *const *geom,
+                                     const cap_bc *bc)
+{
+    if (!ctx || !var || !geom || !bc || ctx->ni < 6 || ctx->nj < 6 || ctx->nk < 6)
+        return CAP_ERR_ARG;
+    for (int q = 0; q < 7; ++q) if (!var[q]) return CAP_ERR_ARG;
+    for (int q = 0; q < 12; ++q) if (!geom[q]) return CAP_ERR_ARG;
+    const int *lbs[6] = {bc->lbw, bc->lbe, bc->lbs, bc->lbn, bc->lbb, bc->lbt};
+    for (int q = 0; q < 6; ++q) if (!lbs[q]) return CAP_ERR_ARG;
+    const int ni = ctx->ni, nj = ctx->nj, nk = ctx->nk;
+    const size_t lbn[6] = {(size_t)nj * nk, (size_t)nj * nk, (size_t)ni * nk, (size_t)ni * nk,
+                           (size_t)ni * nj, (size_t)ni * nj};
+    GhostIO g;
+    int rc;
+    for (int q = 0; q < 7; ++q) {
+        const double *t;
+        if ((rc = stage(ctx, q, var[q], &t))) return rc;
+        g.var[q] = (double *)t;
+    }
+    for (int q = 0; q < 12; ++q)
+        if ((rc = stage(ctx, 7 + q, geom[q], &g.geo[q]))) return rc;
+    for (int q = 0; q < 6; ++q) {
+        const void *t;
+        if ((rc = stage_bytes(ctx, 19 + q, lbs[q], lbn[q] * sizeof(int), &t))) return rc;
+        g.lb[q] = (const int *)t;
+    }
+    g.inl[0] = bc->uin; g.inl[1] = bc->vin; g.inl[2] = bc->win;
+    g.inl[3] = bc->tein; g.inl[4] = bc->edin; g.inl[5] = bc->tin;
+    const size_t total = 2 * ((size_t)(nj - 4) * (nk - 4) + (size_t)(ni - 4) * (nk - 4) +
+                              (size_t)(ni - 4) * (nj - 4));
+    hipLaunchKernelGGL(k_ghostcells, dim3(blocks_for(total)), dim3(256), 0, ctx->stream, ni, nj, nk, g);
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < 7; ++q)
+        if ((rc = unstage(ctx, var[q], g.var[q]))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return CAP_OK;
 }

@@ -43,6 +43,13 @@ module cap_hip
     type(c_ptr) :: gradu = c_null_ptr, gradv = c_null_ptr, gradw = c_null_ptr, gradp = c_null_ptr
   end type cap_fields
 
+  ! boundary types per boundary cell and inlet values (modules bc, inlet)
+  type, bind(c) :: cap_bc
+    type(c_ptr) :: lbw = c_null_ptr, lbe = c_null_ptr, lbs = c_null_ptr, lbn = c_null_ptr
+    type(c_ptr) :: lbb = c_null_ptr, lbt = c_null_ptr
+    real(c_double) :: uin = 0, vin = 0, win = 0, tein = 0, edin = 0, tin = 0
+  end type cap_bc
+
   type, bind(c) :: cap_report
     real(c_double) :: resor, resl
     integer(c_int) :: iters, status
@@ -145,6 +152,13 @@ module cap_hip
       type(cap_fields), intent(in) :: f
       integer(c_int), value :: idew, idns, idtb, schemes
       real(c_double), value :: gds
+    end function
+    integer(c_int) function cap_update_ghostcells(ctx, var, geom, bc) &
+        bind(c, name='cap_update_ghostcells')
+      import :: c_ptr, c_int, cap_bc
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(in) :: var(7), geom(12)
+      type(cap_bc), intent(in) :: bc
     end function
     integer(c_int) function cap_continuity(ctx, f1, f2, f3, sums) bind(c, name='cap_continuity')
       import :: c_ptr, c_int, c_double

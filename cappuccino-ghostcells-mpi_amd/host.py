@@ -35,7 +35,7 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_iccg", "cap_cgstab_sip", "cap_pcgsip_mixed", "cap_grad_gauss", "cap_bpres",
            "cap_build_lognormal", "cap_time_kernels", "cap_fluxmass",
            "cap_piso_assemble_pressure_eq", "cap_assemble_pressure_eq", "cap_fluxmc",
-           "cap_continuity", "cap_fluxuvw")
+           "cap_continuity", "cap_fluxuvw", "cap_update_ghostcells")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -51,6 +51,11 @@ class _Coef(C.Structure):
 
 class _Fields(C.Structure):
     _fields_ = [(f, C.c_void_p) for f in FIELD_NAMES]
+
+
+class _Bc(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("lbw", "lbe", "lbs", "lbn", "lbb", "lbt")] + \
+               [(n, C.c_double) for n in ("uin", "vin", "win", "tein", "edin", "tin")]
 
 
 class _Report(C.Structure):
@@ -90,6 +95,7 @@ def load():
     L.cap_fluxmc.argtypes = [vp, F, i, i, i, vp, vp]
     L.cap_continuity.argtypes = [vp, vp, vp, vp, C.POINTER(d)]
     L.cap_fluxuvw.argtypes = [vp, F, vp, i, i, i] + [vp] * 13 + [i, d]
+    L.cap_update_ghostcells.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(_Bc)]
     _lib = L
     return L
 
@@ -103,6 +109,17 @@ def _ptr(a):
         return a.data_ptr()
     import numpy as np
     assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def _iptr(a):
+    """int32 array (numpy, or torch on the device)"""
+    if hasattr(a, "data_ptr"):
+        import torch
+        assert a.dtype == torch.int32 and a.is_contiguous()
+        return a.data_ptr()
+    import numpy as np
+    assert a.dtype == np.int32 and a.flags.c_contiguous
     return a.ctypes.data
 
 
@@ -240,6 +257,16 @@ class Context:
             _ptr(f[("fx", "fy", "fz")[d]]), _ptr(acfe), _ptr(acfw), _ptr(fcf), _ptr(su),
             _ptr(sv), _ptr(sw), _ptr(bp), _ptr(bt), _ptr(bb), int(schemes), float(gds))
         self._check(rc, "cap_fluxuvw", allow=(CAP_OK,))
+
+    def update_ghostcells(self, var, geom, lb, inl):
+        """var: [u, v, w, te, ed, t, vis] (updated in place); geom: [fx, fy, fz,
+        ar1x .. ar3z]; lb: int32 arrays [lbw, lbe, lbs, lbn, lbb, lbt]; inl:
+        (uin, vin, win, tein, edin, tin)."""
+        V = (C.c_void_p * 7)(*[_ptr(a) for a in var])
+        G = (C.c_void_p * 12)(*[_ptr(a) for a in geom])
+        bc = _Bc(*[_iptr(a) for a in lb], *[float(x) for x in inl])
+        rc = self._lib.cap_update_ghostcells(self._h, V, G, C.byref(bc))
+        self._check(rc, "cap_update_ghostcells", allow=(CAP_OK,))
 
     def fluxmc(self, f, d, fmcor):
         st = self._fields(f)

@@ -142,6 +142,25 @@ int cap_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *f, int idew, int id
 int cap_fluxmc(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
                const double *fif, double *fmcor);
 
+/* Boundary types of the ghost-cell update, per boundary cell: lbw / lbe
+ * indexed (j-1)*nk + k-1, lbs / lbn (i-1)*nk + k-1, lbb / lbt (i-1)*nj + j-1
+ * (module bc of modules_allocatable.f90:217-219); 1 inlet, 2 outlet,
+ * 3 symmetry, 4 wall.  Inlet values: module inlet (uin, vin, win, tein,
+ * edin, tin). */
+typedef struct {
+    const int *lbw, *lbe, *lbs, *lbn, *lbb, *lbt;
+    double uin, vin, win, tein, edin, tin;
+} cap_bc;
+
+/* Replaces `call update_values_at_ghostcells`
+ * (update_values_at_ghostcells_nonperiodic.f90:3-642): ghost-cell values of
+ * var = {u, v, w, te, ed, t, vis} next to the six boundary faces (edges and
+ * corners untouched), geom = {fx, fy, fz, ar1x, ar1y, ar1z, ar2x, ar2y, ar2z,
+ * ar3x, ar3y, ar3z}.  The on-device ghost fill of the path (one GPU: no
+ * exchange between ranks). */
+int cap_update_ghostcells(cap_ctx *ctx, double *const *var, const double *const *geom,
+                          const cap_bc *bc);
+
 /* Convection schemes of cap_fluxuvw: the reference's module logicals
  * lcds, lcds4, lluds, lquds, flux_limiter and the limiter choice lsmart,
  * lavl, lmuscl, lumist, lgamma (modules_allocatable.f90:58, 90), applied in

@@ -1114,3 +1114,108 @@ void orc_fluxuvw(int ni, int nj, int nk, const orc_fields *f, const double *vis,
     const size_t n = (size_t)nij * nk;
     for (size_t c = 0; c < n; ++c) { bp[c] = 0.0; bt[c] = 0.0; bb[c] = 0.0; }
 }
+
+/* update_values_at_ghostcells_nonperiodic.f90:3-642: values of u, v, w, te,
+ * ed, t, vis in the ghost cells next to the six boundary faces, by the
+ * boundary type of each boundary cell (lbw/lbe indexed (j-1)*nk+k, lbs/lbn
+ * (i-1)*nk+k, lbb/lbt (i-1)*nj+j): 1 inlet (Dirichlet uin..tin), 2 outlet
+ * (linear extrapolation from two inner cells), 3 symmetry, 4 wall.  Edges
+ * and corners are not filled.  var = {u, v, w, te, ed, t, vis}; geo = {fx,
+ * fy, fz, ar1x, ar1y, ar1z, ar2x, ar2y, ar2z, ar3x, ar3y, ar3z}; inl = {uin,
+ * vin, win, tein, edin, tin}. */
+static void orc_ghost_cell(double *const *var, size_t inp, size_t ingc, long nb, double fxp,
+                           double fxe, const double *ext_f, size_t ext_c, int ext_sign,
+                           const double *ax, const double *ay, const double *az, size_t iface,
+                           int lb, const double *inl, int side_wall_vis)
+{
+    double *u = var[0], *v = var[1], *w = var[2], *te = var[3], *ed = var[4], *t = var[5],
+           *vis = var[6];
+    const double fxer = 1.0 / fxe;
+    if (lb == 1) {
+        u[ingc] = (inl[0] - u[inp] * fxp) * fxer;
+        v[ingc] = (inl[1] - v[inp] * fxp) * fxer;
+        w[ingc] = (inl[2] - w[inp] * fxp) * fxer;
+        te[ingc] = (inl[3] - te[inp] * fxp) * fxer;
+        ed[ingc] = (inl[4] - ed[inp] * fxp) * fxer;
+        t[ingc] = (inl[5] - t[inp] * fxp) * fxer;
+        vis[ingc] = 0.09 * (te[ingc] * te[ingc]) / ed[ingc];
+    }
+    if (lb == 2) {
+        /* ext_sign < 0: q(inp) - (q(inp+nb) - q(inp)) * f(ext_c)   (west, south, bottom)
+         * ext_sign > 0: q(inp) + (q(inp) - q(inp-nb)) * (1 - f(ext_c))  (east, north, top) */
+        for (int c = 0; c < 7; ++c) {
+            double *q = var[c];
+            if (ext_sign < 0) q[ingc] = q[inp] - (q[inp + nb] - q[inp]) * ext_f[ext_c];
+            else q[ingc] = q[inp] + (q[inp] - q[inp - nb]) * (1.0 - ext_f[ext_c]);
+        }
+    }
+    if (lb == 3) {
+        const double un = u[inp] * ax[iface] + v[inp] * ay[iface] + w[inp] * az[iface];
+        u[ingc] = u[inp] - un * ax[iface];
+        v[ingc] = v[inp] - un * ay[iface];
+        w[ingc] = w[inp] - un * az[iface];
+        te[ingc] = te[inp];
+        ed[ingc] = ed[inp];
+        t[ingc] = t[inp];
+        vis[ingc] = vis[inp];
+    }
+    if (lb == 4) {
+        u[ingc] = (-u[inp] * fxp) * fxer;
+        v[ingc] = (-v[inp] * fxp) * fxer;
+        w[ingc] = (-w[inp] * fxp) * fxer;
+        te[ingc] = (-te[inp] * fxp) * fxer;
+        ed[ingc] = ed[inp];
+        t[ingc] = (inl[5] - t[inp] * fxp) * fxer;
+        vis[ingc] = side_wall_vis ? (vis[ingc] - vis[inp] * fxp) * fxer : vis[inp];
+    }
+}
+
+void orc_ghostcells(int ni, int nj, int nk, double *const *var, const double *const *geo,
+                    const int *lbw, const int *lbe, const int *lbs, const int *lbn,
+                    const int *lbb, const int *lbt, const double *inl)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const double *fx = geo[0], *fy = geo[1], *fz = geo[2];
+    for (int k = 3; k <= nkmm; ++k)          /* west (45-139) */
+        for (int j = 3; j <= njmm; ++j) {
+            const size_t inp = IX(3, j, k), ingc = inp - nj, iface = inp - nj;
+            const double fxp = fx[inp - nj], fxe = 1.0 - fxp;
+            orc_ghost_cell(var, inp, ingc, nj, fxp, fxe, fx, inp, -1, geo[3], geo[4], geo[5],
+                           iface, lbw[(j - 1) * nk + k - 1], inl, 1);
+        }
+    for (int k = 3; k <= nkmm; ++k)          /* east (143-239) */
+        for (int j = 3; j <= njmm; ++j) {
+            const size_t inp = IX(nimm, j, k), ingc = inp + nj, iface = inp;
+            const double fxp = 1.0 - fx[inp], fxe = 1.0 - fxp;
+            orc_ghost_cell(var, inp, ingc, nj, fxp, fxe, fx, inp - nj, 1, geo[3], geo[4],
+                           geo[5], iface, lbe[(j - 1) * nk + k - 1], inl, 1);
+        }
+    for (int k = 3; k <= nkmm; ++k)          /* south (243-337) */
+        for (int i = 3; i <= nimm; ++i) {
+            const size_t inp = IX(i, 3, k), ingc = inp - 1, iface = inp - 1;
+            const double fxp = 1.0 - fy[inp], fxe = 1.0 - fxp;
+            orc_ghost_cell(var, inp, ingc, 1, fxp, fxe, fy, inp, -1, geo[6], geo[7], geo[8],
+                           iface, lbs[(i - 1) * nk + k - 1], inl, 1);
+        }
+    for (int k = 3; k <= nkmm; ++k)          /* north (341-437) */
+        for (int i = 3; i <= nimm; ++i) {
+            const size_t inp = IX(i, njmm, k), ingc = inp + 1, iface = inp;
+            const double fxp = 1.0 - fy[inp], fxe = 1.0 - fxp;
+            orc_ghost_cell(var, inp, ingc, 1, fxp, fxe, fy, inp - 1, 1, geo[6], geo[7], geo[8],
+                           iface, lbn[(i - 1) * nk + k - 1], inl, 1);
+        }
+    for (int i = 3; i <= nimm; ++i)          /* bottom (441-537) */
+        for (int j = 3; j <= njmm; ++j) {
+            const size_t inp = IX(i, j, 3), ingc = inp - nij, iface = inp - nij;
+            const double fxp = fz[ingc], fxe = 1.0 - fxp;
+            orc_ghost_cell(var, inp, ingc, nij, fxp, fxe, fz, inp, -1, geo[9], geo[10], geo[11],
+                           iface, lbb[(i - 1) * nj + j - 1], inl, 0);
+        }
+    for (int i = 3; i <= nimm; ++i)          /* top (541-638) */
+        for (int j = 3; j <= njmm; ++j) {
+            const size_t inp = IX(i, j, nkmm), ingc = inp + nij, iface = inp;
+            const double fxp = 1.0 - fz[inp], fxe = 1.0 - fxp;
+            orc_ghost_cell(var, inp, ingc, nij, fxp, fxe, fz, inp - nij, 1, geo[9], geo[10],
+                           geo[11], iface, lbt[(i - 1) * nj + j - 1], inl, 0);
+        }
+}

@@ -263,7 +263,7 @@ def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8, driver=REF_DRIVER):
         return [raw.copy()], None, log
     nout = {"sipsol": 9, "cgstab_sip": 9, "pcgsip": 9, "iccg": 9, "grad_gauss": 3,
             "bpres": 1, "fluxmass": 5, "piso_assemble": 5, "assemble_p": 5, "fluxmc": 5,
-            "fluxuvw": 8}[mode]
+            "fluxuvw": 8, "ghost": 7}[mode]
     outs = [raw[i * n:(i + 1) * n].copy() for i in range(nout)]
     resor = float(raw[nout * n]) if raw.size > nout * n else None
     return outs, resor, log

@@ -13,7 +13,7 @@
 !
 ! usage: ref_driver <mode> <in.bin> <out.bin> <log.txt>
 !   mode = sipsol | cgstab_sip | grad_gauss | bpres | fluxmass | piso_assemble
-!          | assemble_p | fluxmc | continuity | fluxuvw
+!          | assemble_p | fluxmc | continuity | fluxuvw | ghost
 ! in.bin  : int32 ni,nj,nk,nsw ; real64 alfa,sor ; then mode-specific arrays
 !           (stencil modes: nsw = direction 0 east, 1 north, 2 top; alfa = ipc;
 !            fluxuvw: nsw = direction + 8 * scheme flags, alfa = gds(iu))
@@ -28,11 +28,13 @@ program ref_driver
   use geometry
   use variables
   use gradients
+  use bc
+  use inlet
   implicit none
   character(len=256) :: mode, fin, fout, flog
   integer :: ni_, nj_, nk_, nsw_, n
   real(8) :: alfa_, sor_
-  real(8), allocatable :: fi(:), d1(:), d2(:), d3(:), bsc(:)
+  real(8), allocatable :: fi(:), d1(:), d2(:), d3(:), bsc(:), lbd(:, :), inl(:)
   integer :: idew, idns, idtb, inp, i, j, k, fl, dirn
   real(8) :: fmcor
 
@@ -68,7 +70,7 @@ program ref_driver
   allocate(x(n),y(n),z(n),xc(n),yc(n),zc(n),den(n),apu(n),apv(n),apw(n))
   allocate(u(n),v(n),w(n),p(n),sv(n),sw(n),f1(n),f2(n),f3(n))
   allocate(gradu(3,n),gradv(3,n),gradw(3,n),gradp(3,n))
-  allocate(vis(n))
+  allocate(vis(n), te(n), ed(n), t(n))
 
   open(unit=66, file=trim(flog), status='replace')
   open(unit=11, file=trim(fout), access='stream', form='unformatted', status='replace')
@@ -159,6 +161,19 @@ program ref_driver
     if (dirn == 1) call fluxuvw(1, nij, nj, ar2x, ar2y, ar2z, fy, d1, d2, d3)
     if (dirn == 2) call fluxuvw(nij, nj, 1, ar3x, ar3y, ar3z, fz, d1, d2, d3)
     write(11) d1, d2, su, sv, sw, bp, bt, bb
+  case ('ghost')
+    ! boundary-type arrays travel as real(8), padded to nxyz
+    allocate(lbd(n, 6), inl(n))
+    allocate(lbw(nj*nk), lbe(nj*nk), lbs(ni*nk), lbn(ni*nk), lbb(ni*nj), lbt(ni*nj))
+    read(10) u, v, w, te, ed, t, vis, fx, fy, fz, ar1x, ar1y, ar1z, ar2x, ar2y, ar2z, &
+             ar3x, ar3y, ar3z
+    read(10) lbd, inl
+    lbw = int(lbd(1:nj*nk, 1)); lbe = int(lbd(1:nj*nk, 2))
+    lbs = int(lbd(1:ni*nk, 3)); lbn = int(lbd(1:ni*nk, 4))
+    lbb = int(lbd(1:ni*nj, 5)); lbt = int(lbd(1:ni*nj, 6))
+    uin = inl(1); vin = inl(2); win = inl(3); tein = inl(4); edin = inl(5); tin = inl(6)
+    call update_values_at_ghostcells
+    write(11) u, v, w, te, ed, t, vis
   case ('continuity')
     read(10) f1, f2, f3
     call continuity_sums()

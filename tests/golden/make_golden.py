@@ -122,6 +122,44 @@ ASSEMBLY_CASES = [
     ("piso_assemble", (12, 10, 9), 303), ("piso_assemble_ipc0", (12, 10, 9), 304),
     ("assemble_p", (12, 10, 9), 305), ("fluxmc", (12, 10, 9), 306),
 ]
+GHOST_VARS = ("u", "v", "w", "te", "ed", "t", "vis")
+GEO12 = ("fx", "fy", "fz", "ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z")
+
+
+def ghost_inputs(dims, seed):
+    """Seeded inputs of the ghost-cell update: the seven variables, the
+    geometry, boundary types 1..4 per boundary cell (lbw, lbe, lbs, lbn, lbb,
+    lbt) and the inlet values uin, vin, win, tein, edin, tin."""
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = dims
+    n = ni * nj * nk
+    var = {q: rng.normal(size=n) for q in GHOST_VARS}
+    var["te"] = rng.uniform(0.1, 1.0, n)
+    var["ed"] = rng.uniform(0.5, 2.0, n)
+    var["vis"] = rng.uniform(0.01, 0.1, n)
+    geo = {q: rng.uniform(0.3, 0.7, n) for q in ("fx", "fy", "fz")}
+    for q in GEO12[3:]:
+        geo[q] = rng.uniform(-1, 1, n)
+    lens = (nj * nk, nj * nk, ni * nk, ni * nk, ni * nj, ni * nj)
+    lb = [rng.integers(1, 5, m).astype(np.int32) for m in lens]
+    inl = rng.normal(size=6)
+    inl[3:5] = rng.uniform(0.5, 1.0, 2)
+    return var, geo, lb, inl
+
+
+def ghost_ref_arrays(dims, var, geo, lb, inl):
+    n = dims[0] * dims[1] * dims[2]
+    pads = []
+    for a in lb:
+        x = np.zeros(n)
+        x[:a.size] = a
+        pads.append(x)
+    il = np.zeros(n)
+    il[:6] = inl
+    return [var[q] for q in GHOST_VARS] + [geo[q] for q in GEO12] + pads + [il]
+
+
+GHOST_CASES = [((12, 10, 9), 601), ((20, 17, 11), 602)]
 FLUXUVW_CASES = [((12, 10, 9), 501, "cds", 1.0), ((12, 10, 9), 502, "luds", 1.0),
                  ((12, 10, 9), 503, "quick", 1.0), ((12, 10, 9), 504, "cds4", 0.7),
                  ((12, 10, 9), 505, "smart", 1.0), ((12, 10, 9), 506, "gamma", 1.0),
@@ -168,6 +206,13 @@ def main():
             for q, a in zip(("acfe", "acfw", "fcf", "bp", "su"), outs):
                 rec[f"{q}_{d}"] = a
         name = f"{routine}_{dims[0]}x{dims[1]}x{dims[2]}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print(f"{name}: written")
+    for dims, seed in GHOST_CASES:
+        var, geo, lb, inl = ghost_inputs(dims, seed)
+        outs, _, _ = O.run_ref("ghost", dims, ghost_ref_arrays(dims, var, geo, lb, inl))
+        rec = dict(dims=np.array(dims), seed=seed, **{q: a for q, a in zip(GHOST_VARS, outs)})
+        name = f"ghost_{dims[0]}x{dims[1]}x{dims[2]}"
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
         print(f"{name}: written")
     for dims, seed, scheme, gds in FLUXUVW_CASES:

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
 import make_golden as MG  # noqa: E402
-from test_oracle import oracle_assembly, oracle_fluxuvw  # noqa: E402
+from test_oracle import oracle_assembly, oracle_fluxuvw, oracle_ghost  # noqa: E402
 
 DIMS = [(12, 10, 9), (20, 17, 11), (40, 23, 19)]
 
@@ -75,3 +75,21 @@ def test_fluxuvw_bitwise(gpu, O, dims, scheme):
             got = dict(acfe=acfe, acfw=acfw, su=su, sv=sv, sw=sw, bp=bp, bt=bt, bb=bb)
             for q in MG.UVW_OUT:
                 np.testing.assert_array_equal(got[q], ref[q], err_msg=f"{scheme} {q} dir {d}")
+
+
+@pytest.mark.parametrize("dims", DIMS + [(36, 36, 36)])
+@pytest.mark.parametrize("types", ["mixed", "inlet", "outlet", "symmetry", "wall"])
+def test_ghostcells_bitwise(gpu, O, dims, types):
+    """The on-device ghost-cell fill: every boundary type on all six faces,
+    bit-identical to the oracle (itself pinned to the reference build)."""
+    var, geo, lb, inl = MG.ghost_inputs(dims, 97)
+    if types != "mixed":
+        code = {"inlet": 1, "outlet": 2, "symmetry": 3, "wall": 4}[types]
+        for a in lb:
+            a[:] = code
+    ref = oracle_ghost(O, dims, var, geo, lb, inl)
+    got = [var[q].copy() for q in MG.GHOST_VARS]
+    with gpu.Context(*dims) as ctx:
+        ctx.update_ghostcells(got, [geo[q] for q in MG.GEO12], lb, inl)
+    for q, a in zip(MG.GHOST_VARS, got):
+        np.testing.assert_array_equal(a, ref[q], err_msg=q)
