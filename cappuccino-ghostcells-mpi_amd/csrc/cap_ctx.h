@@ -48,6 +48,8 @@ struct cap_ctx {
     size_t Gsz[CAP_NSTAGE];
     void *msg;                  // wavefront message slots
     size_t msg_bytes;
+    double *scr;                  // small reduction scratch of the stencil entry points
+    size_t scr_bytes;
     unsigned long long *claim;
     unsigned long long claim_base;
     int wave_lag;                 // steps a wavefront tile keeps behind its producer

@@ -726,6 +726,89 @@ __global__ __launch_bounds__(256) void k_ghostcells(int ni, int nj, int nk, Ghos
     }
 }
 
+// ------------------------------------------------------ Courant number
+// CourantNo.h:30-51: per-cell su and per-block partials {sum su (hi, lo),
+// sum vol (hi, lo), max su/vol}; k_courant_fin adds the partials in block
+// order (compensated sums: the result does not depend on the block count).
+__global__ __launch_bounds__(256) void k_courant(int ni, int nj, int nk, const double *__restrict__ f1,
+                                                 const double *__restrict__ f2,
+                                                 const double *__restrict__ f3,
+                                                 const double *__restrict__ vol,
+                                                 double *__restrict__ su, double *part)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2;
+    const size_t n = (size_t)nij * nk;
+    Acc ssu, svol;
+    double mx = 0.0;
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
+        const int k = (int)(c / nij) + 1, r = (int)(c % nij);
+        const int i = r / nj + 1, j = r % nj + 1;
+        double s = 0.0;
+        if (k >= 2 && k <= nk - 1 && i >= 3 && i <= nimm && j >= 3 && j <= njmm) {
+            s = fabs(f1[c]) + fabs(f1[c - nj]) + fabs(f2[c]) + fabs(f2[c - 1]) + fabs(f3[c]) +
+                fabs(f3[c - nij]);
+            const double q = s / vol[c];
+            mx = q > mx ? q : mx;
+            ssu += s;
+            svol += vol[c];
+        }
+        su[c] = s;
+    }
+    __shared__ double sh[5][4];
+    Acc a = wave_sum(ssu), b = wave_sum(svol);
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][w] = a.hi; sh[1][w] = a.lo; sh[2][w] = b.hi; sh[3][w] = b.lo; sh[4][w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Acc ta, tb;
+        double m = 0.0;
+        for (int q = 0; q < 4; ++q) {
+            Acc x, y;
+            x.hi = sh[0][q]; x.lo = sh[1][q]; y.hi = sh[2][q]; y.lo = sh[3][q];
+            ta += x;
+            tb += y;
+            m = sh[4][q] > m ? sh[4][q] : m;
+        }
+        double *pp = part + (size_t)blockIdx.x * 5;
+        pp[0] = ta.hi; pp[1] = ta.lo; pp[2] = tb.hi; pp[3] = tb.lo; pp[4] = m;
+    }
+}
+
+// CourantNo.h:52-64; out = {CoNum, meanCoNum, timestep, time}
+__global__ void k_courant_fin(const double *part, int nblocks, double timestep, double time,
+                              int conumfix, double conumfixvalue, double *out)
+{
+    if (threadIdx.x != 0) return;
+    Acc ssu, svol;
+    double conum = 0.0;
+    for (int b = 0; b < nblocks; ++b) {
+        Acc x, y;
+        x.hi = part[5 * b]; x.lo = part[5 * b + 1];
+        y.hi = part[5 * b + 2]; y.lo = part[5 * b + 3];
+        ssu += x;
+        svol += y;
+        conum = part[5 * b + 4] > conum ? part[5 * b + 4] : conum;
+    }
+    conum = 0.5 * conum * timestep;
+    double mean = 0.5 * ssu.value() / svol.value() * timestep;
+    if (conumfix) {
+        const double dt = timestep;
+        timestep = conumfixvalue / conum * timestep;
+        conum = conum * timestep / dt;
+        mean = mean * timestep / dt;
+    }
+    out[0] = conum;
+    out[1] = mean;
+    out[2] = timestep;
+    out[3] = time + timestep;
+}
+
 // continuityErrors.h:63-77: global and sum|local| over k=2..nkm, i,j=3..n?mm
 __global__ __launch_bounds__(256) void k_continuity(int ni, int nj, int nk,
                                                     const double *__restrict__ f1,
@@ -1095,6 +1178,43 @@ extern "C" int cap_update_ghostcells(cap_ctx *ctx, double *const *var, const dou
     return CAP_OK;
 }
 
+extern "C" int cap_courant(cap_ctx *ctx, const double *f1, const double *f2, const double *f3,
+                           const double *vol, double *su, double *timestep, double *time,
+                           int conumfix, double conumfixvalue, double *conum)
+{
+    if (!ctx || !f1 || !f2 || !f3 || !vol || !su || !timestep || !time || !conum)
+        return CAP_ERR_ARG;
+    const double *d1, *d2, *d3, *dv, *ds;
+    int rc;
+    if ((rc = stage(ctx, 0, f1, &d1)) || (rc = stage(ctx, 1, f2, &d2)) || (rc = stage(ctx, 2, f3, &d3)) ||
+        (rc = stage(ctx, 3, vol, &dv)) || (rc = stage(ctx, 4, nullptr, &ds)))
+        return rc;
+    double *dsu = (ctx->flags & CAP_DEVICE_POINTERS) ? su : (double *)ds;
+    const int nb = blocks_for(ctx->nstd);
+    const size_t need = ((size_t)nb * 5 + 8) * sizeof(double);
+    if (ctx->scr_bytes < need) {
+        if (ctx->scr) (void)hipFree(ctx->scr);
+        ctx->scr = nullptr;
+        HIPCHK(hipMalloc(&ctx->scr, need));
+        ctx->scr_bytes = need;
+    }
+    double *out_d = ctx->scr + (size_t)nb * 5;
+    hipLaunchKernelGGL(k_courant, dim3(nb), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj, ctx->nk,
+                       d1, d2, d3, dv, dsu, ctx->scr);
+    hipLaunchKernelGGL(k_courant_fin, dim3(1), dim3(64), 0, ctx->stream, (const double *)ctx->scr,
+                       nb, *timestep, *time, conumfix, conumfixvalue, out_d);
+    HIPCHK(hipGetLastError());
+    double out[4];
+    HIPCHK(hipMemcpyAsync(out, out_d, sizeof out, hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = unstage(ctx, su, dsu))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    conum[0] = out[0];
+    conum[1] = out[1];
+    *timestep = out[2];
+    *time = out[3];
+    return CAP_OK;
+}
+
 extern "C" int cap_continuity(cap_ctx *ctx, const double *f1, const double *f2, const double *f3,
                               double *sums)
 {
@@ -1203,6 +1323,7 @@ extern "C" int cap_build_lognormal(cap_ctx *ctx, unsigned long long seed, double
 
 void cap_free_extra(cap_ctx *ctx)
 {
+    if (ctx->scr) (void)hipFree(ctx->scr);
     for (int q = 0; q < CAP_NSTAGE; ++q) if (ctx->G[q]) (void)hipFree(ctx->G[q]);
     for (int q = 0; q < F_COUNT; ++q) if (ctx->F[q]) (void)hipFree(ctx->F[q]);
 }

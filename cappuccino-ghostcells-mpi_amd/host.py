@@ -35,7 +35,8 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_iccg", "cap_cgstab_sip", "cap_pcgsip_mixed", "cap_grad_gauss", "cap_bpres",
            "cap_build_lognormal", "cap_time_kernels", "cap_fluxmass",
            "cap_piso_assemble_pressure_eq", "cap_assemble_pressure_eq", "cap_fluxmc",
-           "cap_continuity", "cap_fluxuvw", "cap_update_ghostcells")
+           "cap_continuity", "cap_fluxuvw", "cap_update_ghostcells",
+           "cap_courant")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -96,6 +97,8 @@ def load():
     L.cap_continuity.argtypes = [vp, vp, vp, vp, C.POINTER(d)]
     L.cap_fluxuvw.argtypes = [vp, F, vp, i, i, i] + [vp] * 13 + [i, d]
     L.cap_update_ghostcells.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(_Bc)]
+    L.cap_courant.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(d), C.POINTER(d), i, d,
+                              C.POINTER(d)]
     _lib = L
     return L
 
@@ -267,6 +270,16 @@ class Context:
         bc = _Bc(*[_iptr(a) for a in lb], *[float(x) for x in inl])
         rc = self._lib.cap_update_ghostcells(self._h, V, G, C.byref(bc))
         self._check(rc, "cap_update_ghostcells", allow=(CAP_OK,))
+
+    def courant(self, f1, f2, f3, vol, su, timestep, time, conumfix=False, conumfixvalue=0.5):
+        """Returns (CoNum, meanCoNum, timestep, time) after CourantNo.h."""
+        ts, tm = C.c_double(timestep), C.c_double(time)
+        out = (C.c_double * 2)()
+        rc = self._lib.cap_courant(self._h, _ptr(f1), _ptr(f2), _ptr(f3), _ptr(vol), _ptr(su),
+                                   C.byref(ts), C.byref(tm), 1 if conumfix else 0,
+                                   float(conumfixvalue), out)
+        self._check(rc, "cap_courant", allow=(CAP_OK,))
+        return out[0], out[1], ts.value, tm.value
 
     def fluxmc(self, f, d, fmcor):
         st = self._fields(f)

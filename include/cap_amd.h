@@ -142,6 +142,15 @@ int cap_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *f, int idew, int id
 int cap_fluxmc(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
                const double *fif, double *fmcor);
 
+/* The Courant-number block included in the time loop (CourantNo.h:10-81,
+ * ltransient branch): su = sum of |face mass fluxes| f1..f3 of each cell on
+ * k = 2..nkm, i, j = 3..n?mm (0 elsewhere), conum = {CoNum, meanCoNum}; with
+ * conumfix the time step is rescaled so that CoNum = conumfixvalue.  The time
+ * advances by the (new) time step.  Sums are compensated on the device. */
+int cap_courant(cap_ctx *ctx, const double *f1, const double *f2, const double *f3,
+                const double *vol, double *su, double *timestep, double *time, int conumfix,
+                double conumfixvalue, double *conum);
+
 /* Boundary types of the ghost-cell update, per boundary cell: lbw / lbe
  * indexed (j-1)*nk + k-1, lbs / lbn (i-1)*nk + k-1, lbb / lbt (i-1)*nj + j-1
  * (module bc of modules_allocatable.f90:217-219); 1 inlet, 2 outlet,

@@ -1219,3 +1219,40 @@ void orc_ghostcells(int ni, int nj, int nk, double *const *var, const double *co
                            geo[11], iface, lbt[(i - 1) * nj + j - 1], inl, 0);
         }
 }
+
+/* CourantNo.h:10-81 (ltransient branch): su = |f1|+|f1(w)|+|f2|+|f2(s)|+
+ * |f3|+|f3(b)| per cell on k = 2..nkm, i, j = 3..n?mm (su = 0 elsewhere),
+ * CoNum = 0.5 * max(su/vol) * dt, meanCoNum = 0.5 * sum(su)/sum(vol) * dt;
+ * with CoNumFix the time step is rescaled to CoNumFixValue (58-64), and time
+ * advances by the time step.  out = {CoNum, meanCoNum}.  (global_max is the
+ * identity on one rank.) */
+void orc_courant(int ni, int nj, int nk, const double *f1, const double *f2, const double *f3,
+                 const double *vol, double *su, double *timestep, double *time, int conumfix,
+                 double conumfixvalue, double *out)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    double conum = 0.0, meanconum = 0.0, suma = 0.0;
+    psum mean_ps = PS0, suma_ps = PS0;
+    memset(su, 0, (size_t)nij * nk * sizeof(double));
+    for (int k = 2; k <= nkm; ++k)
+        for (int i = 3; i <= nimm; ++i)
+            for (int j = 3; j <= njmm; ++j) {
+                const size_t inp = IX(i, j, k);
+                su[inp] = fabs(f1[inp]) + fabs(f1[inp - nj]) + fabs(f2[inp]) + fabs(f2[inp - 1]) +
+                          fabs(f3[inp]) + fabs(f3[inp - nij]);
+                conum = orc_max2(conum, su[inp] / vol[inp]);
+                suma = ps_add(&suma_ps, vol[inp], k);
+                meanconum = ps_add(&mean_ps, su[inp], k);
+            }
+    conum = 0.5 * conum * *timestep;
+    meanconum = 0.5 * meanconum / suma * *timestep;
+    if (conumfix) {
+        const double dt = *timestep;
+        *timestep = conumfixvalue / conum * *timestep;
+        conum = conum * *timestep / dt;
+        meanconum = meanconum * *timestep / dt;
+    }
+    *time = *time + *timestep;
+    out[0] = conum;
+    out[1] = meanconum;
+}

@@ -135,6 +135,9 @@ void orc_fluxuvw(int ni, int nj, int nk, const orc_fields *f, const double *vis,
                  const double *fif, double *acfe, double *acfw, const double *fcf, double *su,
                  double *sv, double *sw, double *bp, double *bt, double *bb, int flags,
                  double gds);
+void orc_courant(int ni, int nj, int nk, const double *f1, const double *f2, const double *f3,
+                 const double *vol, double *su, double *timestep, double *time, int conumfix,
+                 double conumfixvalue, double *out);
 void orc_ghostcells(int ni, int nj, int nk, double *const *var, const double *const *geo,
                     const int *lbw, const int *lbe, const int *lbs, const int *lbn,
                     const int *lbb, const int *lbt, const double *inl);

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
 import make_golden as MG  # noqa: E402
-from test_oracle import oracle_assembly, oracle_fluxuvw, oracle_ghost  # noqa: E402
+from test_oracle import oracle_assembly, oracle_courant, oracle_fluxuvw, oracle_ghost  # noqa: E402
 
 DIMS = [(12, 10, 9), (20, 17, 11), (40, 23, 19)]
 
@@ -93,3 +93,22 @@ def test_ghostcells_bitwise(gpu, O, dims, types):
         ctx.update_ghostcells(got, [geo[q] for q in MG.GEO12], lb, inl)
     for q, a in zip(MG.GHOST_VARS, got):
         np.testing.assert_array_equal(a, ref[q], err_msg=q)
+
+
+@pytest.mark.parametrize("dims", DIMS + [(36, 36, 36)])
+@pytest.mark.parametrize("fix", [0, 1])
+def test_courant(gpu, O, dims, fix):
+    """su and CoNum bit-identical; meanCoNum bit-identical to the oracle with
+    compensated sums and within 1e-13 of the reference's running sums."""
+    f = MG.fields(dims, 93)
+    f1, f2, f3 = f["f123"]
+    vol = f["vol"]
+    su0, co0, mean0, ts0, tm0 = oracle_courant(O, dims, f1, f2, f3, vol, 0.01, 2.0, fix, 0.8)
+    with O.plane_sums(2):
+        _, coa, meana, tsa, tma = oracle_courant(O, dims, f1, f2, f3, vol, 0.01, 2.0, fix, 0.8)
+    su = np.full_like(f1, np.nan)
+    with gpu.Context(*dims) as ctx:
+        co, mean, ts, tm = ctx.courant(f1, f2, f3, vol, su, 0.01, 2.0, bool(fix), 0.8)
+    np.testing.assert_array_equal(su, su0)
+    assert co == coa and mean == meana and ts == tsa and tm == tma
+    assert co == co0 and mean == pytest.approx(mean0, rel=1e-13)

@@ -484,3 +484,49 @@ def test_ghost_closed_forms(O):
     assert np.allclose(u1[2:-2, 1, 2:-2], 1.5) and np.allclose(u1[2:-2, ni - 2, 2:-2], ni - 2.5)
     # edges (i and j both ghost) are never written
     assert np.array_equal(G(out["v"])[:, 1, 1], G(var2["v"])[:, 1, 1])
+
+
+# ------------------------------------------------------ CourantNo.h
+def oracle_courant(O, dims, f1, f2, f3, vol, timestep, time, fix=0, fixval=0.0):
+    L = O.lib()
+    n = dims[0] * dims[1] * dims[2]
+    su = np.full(n, np.nan)
+    ts, tm = _C.c_double(timestep), _C.c_double(time)
+    out = (_C.c_double * 2)()
+    L.orc_courant.argtypes = [_C.c_int] * 3 + [_dp] * 5 + [_C.POINTER(_C.c_double)] * 2 + \
+        [_C.c_int, _C.c_double, _dp]
+    P = lambda a: a.ctypes.data_as(_dp)  # noqa: E731
+    L.orc_courant(*dims, P(f1), P(f2), P(f3), P(vol), P(su), _C.byref(ts), _C.byref(tm), fix,
+                  fixval, _C.cast(out, _dp))
+    return su, out[0], out[1], ts.value, tm.value
+
+
+def test_courant_closed_forms(O):
+    """CourantNo.h is not built here (it calls global_max, whose reference
+    file needs MPI), so the restatement is pinned by what the formulas fix:
+    uniform fluxes and volumes give CoNum = meanCoNum = 0.5*6|f|/vol*dt; the
+    fixed-Courant rescale returns exactly the target; the cell range is
+    k = 2..nkm (ghost planes included), i, j = 3..n?mm."""
+    dims = (9, 8, 7)
+    ni, nj, nk = dims
+    n = ni * nj * nk
+    c, v, dt = 0.3, 2.0, 0.01
+    f = [np.full(n, -c), np.full(n, c), np.full(n, c)]
+    vol = np.full(n, v)
+    su, co, mean, ts, tm = oracle_courant(O, dims, *f, vol, dt, 1.0)
+    assert co == pytest.approx(0.5 * 6 * c / v * dt, rel=1e-15)
+    assert mean == pytest.approx(0.5 * 6 * c / v * dt, rel=1e-14)
+    assert ts == dt and tm == 1.0 + dt
+    G = su.reshape(nk, ni, nj)
+    assert np.allclose(G[1:nk - 1, 2:ni - 2, 2:nj - 2], 6 * c, rtol=1e-15, atol=0)
+    G2 = G.copy()
+    G2[1:nk - 1, 2:ni - 2, 2:nj - 2] = 0.0
+    assert np.all(G2 == 0.0)
+    su, co, mean, ts, tm = oracle_courant(O, dims, *f, vol, dt, 1.0, fix=1, fixval=0.7)
+    co_before = 0.5 * 6 * c / v * dt
+    assert co == pytest.approx(0.7, rel=1e-15) and ts == pytest.approx(0.7 / co_before * dt, rel=1e-12)
+    # one large flux on a ghost-plane cell (k = 2) sets the maximum
+    f1 = np.zeros(n)
+    f1.reshape(nk, ni, nj)[1, 4, 4] = 5.0
+    su, co, mean, ts, tm = oracle_courant(O, dims, f1, np.zeros(n), np.zeros(n), vol, dt, 0.0)
+    assert co == pytest.approx(0.5 * 5.0 / v * dt)   # cells (k=2, i=5, j=5) and its east neighbour
