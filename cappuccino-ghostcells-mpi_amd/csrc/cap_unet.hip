@@ -1,0 +1,1100 @@
+// cap_unet.hip -- learned initial guess for the p' solves: a 3-D U-Net.
+//
+// No counterpart in the reference.  The network maps three channels on the
+// interior cells -- the source, seen as the residual su - A p' of the
+// previous p' (which is where a new source enters), the diagonal ap and the
+// previous p' (each normalised) -- to the change of p'; the unchanged
+// PCG-SIP solve then iterates from previous p' + change, and a guess that
+// does not lower the L1 residual below the default start's is discarded.
+//
+// Architecture: 4 levels with 16 / 32 / 48 / 64 channels, two 3x3x3
+// convolutions + ReLU per level, 2x2x2 average pooling down, nearest
+// upsampling up with the encoder output concatenated, a 3x3x3 head to one
+// channel.  Activations and weights are BF16, channels-last (voxel-major,
+// channel stride padded to 8); every convolution accumulates in FP32.
+//
+// Kernels:
+//  * k_conv: implicit-GEMM convolution on MFMA (v_mfma_f32_16x16x32_bf16):
+//    a wave computes 16 voxels x (16 * NT) output channels; K = 27 taps x
+//    input channels in steps of 32.  Used forward, and with the weights
+//    mirrored in the taps and transposed in the channels for the gradient
+//    with respect to the input.
+//  * k_wgrad: weight and bias gradients (FP32 multiply-add per tap and
+//    channel pair, summed over voxel blocks with atomics).
+//  * pooling, upsampling, gradient combination with the ReLU mask, loss,
+//    Adam.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include "cap_kernels.h"
+#include "cap_amd.h"
+#include "cap_ctx.h"
+
+using namespace cap;
+
+typedef unsigned short bf16_t;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__host__ __device__ inline bf16_t f2bf(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+    return (bf16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+namespace {
+
+constexpr int NL = 15;   // convolution layers
+
+// convolution layer: cin real input channels (stride cinp, padded to 8),
+// cout outputs; kp = 27*cinp rounded up to 32 (MFMA K step); coutp = cout
+// rounded up to 16 (MFMA N tile)
+struct ConvL {
+    int cin, cout, cinp, kp, coutp, relu, level;
+    size_t woff, boff;      // offsets of weights [cout][27][cin] and bias in the parameters
+    // backward-data packing: input channels cout (stride coutp8), outputs cin
+    int coutp8, kpb, cinp16;
+};
+
+__device__ __forceinline__ void tap_off(int tap, int &dz, int &dy, int &dx)
+{
+    dz = tap / 9 - 1;
+    dy = (tap / 3) % 3 - 1;
+    dx = tap % 3 - 1;
+}
+
+// Y = conv(X, W) (+ bias, ReLU) on a D x H x W grid, 16 voxels per wave.
+//  X: bf16 [nvox][cx] (channels xoff.. of stride cx); k = tap*cinp + ci
+//  Wt: bf16 [ntile*16][kp]  (row n = output channel, zero-padded)
+//  Y: bf16 [nvox][cy] at channel offset yoff (yf: fp32 [nvox][cout] instead)
+//  mask (optional): Y *= (M[v][moff + co] > 0), M of stride cm
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv(int D, int H, int W, const bf16_t *__restrict__ X,
+                                              int cx, int xoff, int cinp, const bf16_t *__restrict__ Wt,
+                                              int kp, const float *__restrict__ bias, int relu,
+                                              bf16_t *__restrict__ Y, int cy, int yoff, int cout,
+                                              float *__restrict__ yf, const bf16_t *__restrict__ M,
+                                              int cm, int moff)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long nvox = (long)D * H * W;
+    const long m0 = ((long)blockIdx.x * 4 + wave) * 16;
+    if (m0 >= nvox) return;
+    const long va = m0 + (lane & 15);        // voxel of this lane's A row
+    const int kg = lane >> 4;                // k group (8 values)
+    int z = 0, y = 0, x = 0;
+    const bool vin = va < nvox;
+    if (vin) {
+        x = (int)(va % W);
+        y = (int)((va / W) % H);
+        z = (int)(va / ((long)W * H));
+    }
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nk = kp / 32;
+    for (int kc = 0; kc < nk; ++kc) {
+        const int k = kc * 32 + kg * 8;
+        const int tap = k / cinp, ci = k % cinp;
+        bf16x8 a;
+        {
+            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+            if (vin && tap < 27) {
+                int dz, dy, dx;
+                tap_off(tap, dz, dy, dx);
+                const int zz = z + dz, yy = y + dy, xx = x + dx;
+                if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    const long nb = ((long)zz * H + yy) * W + xx;
+                    raw = *reinterpret_cast<const uint4 *>(X + nb * cx + xoff + ci);
+                }
+            }
+            memcpy(&a, &raw, 16);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const uint4 rb = *reinterpret_cast<const uint4 *>(Wt + (size_t)(t * 16 + (lane & 15)) * kp + k);
+            bf16x8 b;
+            memcpy(&b, &rb, 16);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+    }
+    // D[m][n]: lane holds rows m = 4*(lane/16) + j, column n = lane%16
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + (lane & 15);
+        if (co >= cout) continue;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long v = m0 + 4 * (lane >> 4) + j;
+            if (v >= nvox) continue;
+            float r = acc[t][j] + bv;
+            if (relu) r = r > 0.f ? r : 0.f;
+            if (M) r = bf2f(M[v * cm + moff + co]) > 0.f ? r : 0.f;
+            if (yf) yf[v * cout + co] = r;
+            else Y[v * cy + yoff + co] = f2bf(r);
+        }
+    }
+}
+
+// dW[co][tap][ci] += sum_v X[v + off(tap)][ci] * G[v][co];  db[co] += sum_v G[v][co]
+// block: (voxel chunk of 128, tap); threads loop over (ci, co) pairs
+__global__ __launch_bounds__(256) void k_wgrad(int D, int H, int W, const bf16_t *__restrict__ X,
+                                               int cx, int xoff, int cin,
+                                               const bf16_t *__restrict__ G, int cg, int cout,
+                                               float *__restrict__ dW, float *__restrict__ db)
+{
+    const long nvox = (long)D * H * W;
+    const long v0 = (long)blockIdx.x * 128;
+    const int tap = blockIdx.y;
+    int dz, dy, dx;
+    tap_off(tap, dz, dy, dx);
+    const long v1 = v0 + 128 < nvox ? v0 + 128 : nvox;
+    for (int pr = threadIdx.x; pr < cin * cout; pr += 256) {
+        const int ci = pr % cin, co = pr / cin;
+        float acc = 0.f, accb = 0.f;
+        for (long v = v0; v < v1; ++v) {
+            const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
+            const int zz = z + dz, yy = y + dy, xx = x + dx;
+            const float g = bf2f(G[v * cg + co]);
+            if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const long nb = ((long)zz * H + yy) * W + xx;
+                acc += bf2f(X[nb * cx + xoff + ci]) * g;
+            }
+            accb += g;
+        }
+        atomicAdd(&dW[((size_t)co * 27 + tap) * cin + ci], acc);
+        if (tap == 13 && ci == 0 && db) atomicAdd(&db[co], accb);
+    }
+}
+
+// 2x2x2 average pooling: out[Dc][Hc][Wc][c] from in (stride ci, offset ioff)
+__global__ void k_pool(int Dc, int Hc, int Wc, int C, const bf16_t *in, int cin_, int ioff,
+                       bf16_t *out, int cout_)
+{
+    const long n = (long)Dc * Hc * Wc * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long v = q / C;
+        const int x = (int)(v % Wc), y = (int)((v / Wc) % Hc), z = (int)(v / ((long)Wc * Hc));
+        const int W = 2 * Wc, H = 2 * Hc;
+        float s = 0.f;
+        for (int a = 0; a < 8; ++a) {
+            const long f = ((long)(2 * z + (a >> 2)) * H + (2 * y + ((a >> 1) & 1))) * W + 2 * x + (a & 1);
+            s += bf2f(in[f * cin_ + ioff + c]);
+        }
+        out[v * cout_ + c] = f2bf(0.125f * s);
+    }
+}
+
+// nearest upsampling: out[fine v][ooff + c] = in[coarse v][c]
+__global__ void k_up(int D, int H, int W, int C, const bf16_t *in, int cin_, bf16_t *out, int cout_,
+                     int ooff)
+{
+    const long n = (long)D * H * W * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long v = q / C;
+        const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
+        const long cv = ((long)(z >> 1) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
+        out[v * cout_ + ooff + c] = in[cv * cin_ + c];
+    }
+}
+
+// gradient of a coarse tensor: g[v][c] = (A? sum of the 8 fine children of
+// A[..][aoff + c] (upsampling) : 0) + (B? B[v][boff + c] : 0) + (P? pooled
+// gradient P[parent][c] / 8 : 0), times (mask[v][moff + c] > 0)
+__global__ void k_gcombine(int D, int H, int W, int C, bf16_t *g, int cg_, const bf16_t *A, int ca,
+                           int aoff, const bf16_t *B, int cb, int boff, const bf16_t *P, int cp,
+                           const bf16_t *mask, int cm, int moff)
+{
+    const long n = (long)D * H * W * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long v = q / C;
+        const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
+        float s = 0.f;
+        if (A) {   // A lives on the grid twice as fine
+            for (int a = 0; a < 8; ++a) {
+                const long f = ((long)(2 * z + (a >> 2)) * (2 * H) + (2 * y + ((a >> 1) & 1))) * (2 * W) +
+                               2 * x + (a & 1);
+                s += bf2f(A[f * ca + aoff + c]);
+            }
+        }
+        if (B) s += bf2f(B[v * cb + boff + c]);
+        if (P) {   // P lives on the grid twice as coarse
+            const long pv = ((long)(z >> 1) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
+            s += 0.125f * bf2f(P[pv * cp + c]);
+        }
+        if (mask && !(bf2f(mask[v * cm + moff + c]) > 0.f)) s = 0.f;
+        g[v * cg_ + c] = f2bf(s);
+    }
+}
+
+// network input: 3 fp32 channel-major arrays -> bf16 [nvox][8]
+__global__ void k_pack_in(long nvox, const float *in, float s0, float s1, float s2, bf16_t *x)
+{
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+        x[v * 8 + 0] = f2bf(in[v] * s0);
+        x[v * 8 + 1] = f2bf(in[nvox + v] * s1);
+        x[v * 8 + 2] = f2bf(in[2 * nvox + v] * s2);
+        for (int c = 3; c < 8; ++c) x[v * 8 + c] = 0;
+    }
+}
+
+// loss = mean((out - target)^2) (per-block partials), gradient 2 (out - target) / n in bf16
+__global__ void k_loss(long n, const float *out, const float *tgt, bf16_t *g, int cg_, float *part)
+{
+    __shared__ float sh[256];
+    float s = 0.f;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+        const float d = out[v] - tgt[v];
+        s += d * d;
+        g[v * cg_] = f2bf(2.f * d / (float)n);
+        for (int c = 1; c < cg_; ++c) g[v * cg_ + c] = 0;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Adam on fp32 master parameters
+__global__ void k_adam(long n, float *p, const float *g, float *m, float *v, float lr, float b1,
+                       float b2, float eps, float c1, float c2)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// forward packing Wt[n][k] (k = tap*cinp + ci) and backward-data packing
+// Wb[n = ci][k' = tap'*coutp8 + co] with tap' = 26 - tap
+__global__ void k_pack_w(const float *w, int cin, int cout, int cinp, int kp, int coutp, bf16_t *Wt,
+                         int coutp8, int kpb, int cinp16, bf16_t *Wb)
+{
+    const long nf = (long)coutp * kp, nb = (long)cinp16 * kpb;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nf + nb; q += (long)gridDim.x * blockDim.x) {
+        if (q < nf) {
+            const int n = (int)(q / kp), k = (int)(q % kp);
+            const int tap = k / cinp, ci = k % cinp;
+            float val = 0.f;
+            if (n < cout && tap < 27 && ci < cin) val = w[((size_t)n * 27 + tap) * cin + ci];
+            Wt[q] = f2bf(val);
+        } else {
+            const long r = q - nf;
+            const int n = (int)(r / kpb), k = (int)(r % kpb);
+            const int tapp = k / coutp8, co = k % coutp8;
+            float val = 0.f;
+            if (n < cin && tapp < 27 && co < cout) val = w[((size_t)co * 27 + (26 - tapp)) * cin + n];
+            Wb[r] = f2bf(val);
+        }
+    }
+}
+
+__global__ void k_fill_f(float *p, long n, float v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// L1 norm of the 7-point residual su - A fi on the solve range (reference
+// layout, k, i, j = 3..n?mm, operand order of pcg-sip.f90:53-55), compensated
+__global__ __launch_bounds__(256) void k_res_l1(int ni, int nj, int nk, const double *ae,
+                                                const double *aw, const double *an, const double *as,
+                                                const double *at, const double *ab, const double *ap,
+                                                const double *su, const double *fi, double *part)
+{
+    const int nij = ni * nj;
+    const long n = (long)(ni - 4) * (nj - 4) * (nk - 4);
+    Acc s;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(q % (nj - 4)), i = 3 + (int)((q / (nj - 4)) % (ni - 4)),
+                  k = 3 + (int)(q / ((long)(nj - 4) * (ni - 4)));
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        const double r = ae[c] * fi[c + nj] + aw[c] * fi[c - nj] + an[c] * fi[c + 1] +
+                         as[c] * fi[c - 1] + at[c] * fi[c + nij] + ab[c] * fi[c - nij] + su[c] -
+                         ap[c] * fi[c];
+        s += fabs(r);
+    }
+    Acc t = wave_sum(s);
+    __shared__ double hh[4], ll[4];
+    if ((threadIdx.x & 63) == 0) { hh[threadIdx.x >> 6] = t.hi; ll[threadIdx.x >> 6] = t.lo; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Acc b;
+        for (int w = 0; w < 4; ++w) { Acc o; o.hi = hh[w]; o.lo = ll[w]; b += o; }
+        part[2 * blockIdx.x] = b.hi;
+        part[2 * blockIdx.x + 1] = b.lo;
+    }
+}
+
+// interior cells of the reference layout <-> network voxels (x = j fastest,
+// y = i, z = k).  mode 0: the three raw input channels -- the residual
+// su - M x (M x = ap x - sum a_nb x_nb) of x = the previous-step p' (of the
+// start fi when prev is null), ap and the previous-step p' (zero when prev
+// is null); mode 1: the network's output
+// into the padded direction array g (whose other cells stay zero); mode 2:
+// fi += beta (prev - fi) + scale * g
+__global__ void k_ref_io(int ni, int nj, int nk, int mode, const double *ae, const double *aw,
+                         const double *an, const double *as, const double *at, const double *ab,
+                         const double *ap, const double *su, double *fi, const double *prev,
+                         float *in3, const float *out, double *g, double scale, double beta)
+{
+    const int nx = nj - 4, ny = ni - 4, nz = nk - 4, nij = ni * nj;
+    const long n = (long)nx * ny * nz;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long)nx * ny));
+        const size_t c = (size_t)(z + 2) * nij + (size_t)(y + 2) * nj + (size_t)(x + 2);
+        if (mode == 0) {
+            const double *x = prev ? prev : fi;
+            const double r = ae[c] * x[c + nj] + aw[c] * x[c - nj] + an[c] * x[c + 1] +
+                             as[c] * x[c - 1] + at[c] * x[c + nij] + ab[c] * x[c - nij] + su[c] -
+                             ap[c] * x[c];
+            in3[v] = (float)r;
+            in3[n + v] = (float)ap[c];
+            in3[2 * n + v] = prev ? (float)prev[c] : 0.f;
+        } else if (mode == 1) {
+            g[c] = (double)out[v];
+        } else {
+            fi[c] = fi[c] + (prev ? beta * (prev[c] - fi[c]) : 0.0) + scale * g[c];
+        }
+    }
+}
+
+// block partials (compensated) of the normalisers: sum r^2, sum ap, sum prev^2
+__global__ __launch_bounds__(256) void k_in_stats(long n, const float *in3, double *part)
+{
+    Acc s[3];
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < n; v += (long)gridDim.x * 256) {
+        const double r = in3[v], a = in3[n + v], q = in3[2 * n + v];
+        s[0] += r * r;
+        s[1] += a;
+        s[2] += q * q;
+    }
+    __shared__ double hh[3][4], ll[3][4];
+    for (int m = 0; m < 3; ++m) {
+        const Acc t = wave_sum(s[m]);
+        if ((threadIdx.x & 63) == 0) { hh[m][threadIdx.x >> 6] = t.hi; ll[m][threadIdx.x >> 6] = t.lo; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        Acc b;
+        for (int w = 0; w < 4; ++w) { Acc o; o.hi = hh[threadIdx.x][w]; o.lo = ll[threadIdx.x][w]; b += o; }
+        part[6 * blockIdx.x + 2 * threadIdx.x] = b.hi;
+        part[6 * blockIdx.x + 2 * threadIdx.x + 1] = b.lo;
+    }
+}
+
+// block partials (compensated) of g.r and g.Mg over the solve range, r =
+// su - M x: the energy-norm scale of the direction g as a correction of x
+__global__ __launch_bounds__(256) void k_edots(int ni, int nj, int nk, const double *ae,
+                                               const double *aw, const double *an, const double *as,
+                                               const double *at, const double *ab, const double *ap,
+                                               const double *su, const double *x, const double *g,
+                                               double *part)
+{
+    const int nij = ni * nj;
+    const long n = (long)(ni - 4) * (nj - 4) * (nk - 4);
+    Acc s[2];
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(q % (nj - 4)), i = 3 + (int)((q / (nj - 4)) % (ni - 4)),
+                  k = 3 + (int)(q / ((long)(nj - 4) * (ni - 4)));
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        const double r = ae[c] * x[c + nj] + aw[c] * x[c - nj] + an[c] * x[c + 1] +
+                         as[c] * x[c - 1] + at[c] * x[c + nij] + ab[c] * x[c - nij] + su[c] -
+                         ap[c] * x[c];
+        const double mg = ap[c] * g[c] - (ae[c] * g[c + nj] + aw[c] * g[c - nj] + an[c] * g[c + 1] +
+                                          as[c] * g[c - 1] + at[c] * g[c + nij] + ab[c] * g[c - nij]);
+        s[0] += g[c] * r;
+        s[1] += g[c] * mg;
+    }
+    __shared__ double hh[2][4], ll[2][4];
+    for (int m = 0; m < 2; ++m) {
+        const Acc t = wave_sum(s[m]);
+        if ((threadIdx.x & 63) == 0) { hh[m][threadIdx.x >> 6] = t.hi; ll[m][threadIdx.x >> 6] = t.lo; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        Acc b;
+        for (int w = 0; w < 4; ++w) { Acc o; o.hi = hh[threadIdx.x][w]; o.lo = ll[threadIdx.x][w]; b += o; }
+        part[4 * blockIdx.x + 2 * threadIdx.x] = b.hi;
+        part[4 * blockIdx.x + 2 * threadIdx.x + 1] = b.lo;
+    }
+}
+
+// one damped-Jacobi sweep on M g = su - M x for the correction g of x, from
+// sc * gin: gout = sc gin + w (su - M x - sc M gin) / ap on the solve range
+// (the other cells of gout stay zero)
+__global__ void k_jacobi(int ni, int nj, int nk, const double *ae, const double *aw,
+                         const double *an, const double *as, const double *at, const double *ab,
+                         const double *ap, const double *su, const double *x, const double *gin,
+                         double *gout, double sc, double w)
+{
+    const int nij = ni * nj;
+    const long n = (long)(ni - 4) * (nj - 4) * (nk - 4);
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int j = 3 + (int)(q % (nj - 4)), i = 3 + (int)((q / (nj - 4)) % (ni - 4)),
+                  k = 3 + (int)(q / ((long)(nj - 4) * (ni - 4)));
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        const double r = ae[c] * x[c + nj] + aw[c] * x[c - nj] + an[c] * x[c + 1] +
+                         as[c] * x[c - 1] + at[c] * x[c + nij] + ab[c] * x[c - nij] + su[c] -
+                         ap[c] * x[c];
+        const double *g = gin;
+        const double mg = ap[c] * g[c] - (ae[c] * g[c + nj] + aw[c] * g[c - nj] + an[c] * g[c + 1] +
+                                          as[c] * g[c - 1] + at[c] * g[c + nij] + ab[c] * g[c - nij]);
+        gout[c] = sc * g[c] + w * (r - sc * mg) / ap[c];
+    }
+}
+
+// block partials (compensated) over the solve range of the five sums of the
+// least-squares fit of fi + beta (prev - fi) + alpha g to M x = su:
+// r.Mg, Mg.Mg, r.Md, Md.Md, Md.Mg with r = su - M fi, M g = ap g - sum a_nb g_nb
+// and M d = M (prev - fi) = r - r(prev) (d = 0 when prev is null)
+__global__ __launch_bounds__(256) void k_alpha(int ni, int nj, int nk, const double *ae,
+                                               const double *aw, const double *an, const double *as,
+                                               const double *at, const double *ab, const double *ap,
+                                               const double *su, const double *fi, const double *prev,
+                                               const double *g, double *part)
+{
+    const int nij = ni * nj;
+    const long n = (long)(ni - 4) * (nj - 4) * (nk - 4);
+    Acc s[5];
+    auto res = [&](const double *x, size_t c) {
+        return ae[c] * x[c + nj] + aw[c] * x[c - nj] + an[c] * x[c + 1] + as[c] * x[c - 1] +
+               at[c] * x[c + nij] + ab[c] * x[c - nij] + su[c] - ap[c] * x[c];
+    };
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(q % (nj - 4)), i = 3 + (int)((q / (nj - 4)) % (ni - 4)),
+                  k = 3 + (int)(q / ((long)(nj - 4) * (ni - 4)));
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        const double r = res(fi, c);
+        const double mg = ap[c] * g[c] - (ae[c] * g[c + nj] + aw[c] * g[c - nj] + an[c] * g[c + 1] +
+                                          as[c] * g[c - 1] + at[c] * g[c + nij] + ab[c] * g[c - nij]);
+        const double md = prev ? r - res(prev, c) : 0.0;
+        s[0] += r * mg;
+        s[1] += mg * mg;
+        s[2] += r * md;
+        s[3] += md * md;
+        s[4] += md * mg;
+    }
+    __shared__ double hh[5][4], ll[5][4];
+    for (int m = 0; m < 5; ++m) {
+        const Acc t = wave_sum(s[m]);
+        if ((threadIdx.x & 63) == 0) { hh[m][threadIdx.x >> 6] = t.hi; ll[m][threadIdx.x >> 6] = t.lo; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        Acc b;
+        for (int w = 0; w < 4; ++w) { Acc o; o.hi = hh[threadIdx.x][w]; o.lo = ll[threadIdx.x][w]; b += o; }
+        part[10 * blockIdx.x + 2 * threadIdx.x] = b.hi;
+        part[10 * blockIdx.x + 2 * threadIdx.x + 1] = b.lo;
+    }
+}
+
+// damped-Jacobi sweeps (an even number: the result lands in gpad) and weight
+constexpr int GUESS_SWEEPS = 4;
+constexpr double GUESS_OMEGA = 0.8;
+
+int grid_of(long n, int bs = 256) { return (int)std::max<long>(1, std::min<long>((n + bs - 1) / bs, 16384)); }
+
+}  // namespace
+
+struct cap_unet {
+    cap_ctx *ctx;
+    int nx, ny, nz;
+    long nvox[4];
+    ConvL L[NL];
+    size_t nparam;
+    float *p, *g, *m, *v;              // parameters, gradients, Adam moments (device)
+    bf16_t *Wt[NL], *Wb[NL];           // packed weights
+    int packed;                        // Wt/Wb match p
+    long step;                         // Adam step count
+    // activations (bf16, channels-last)
+    bf16_t *in0, *a1, *c1, *p1, *a3, *c2, *p2, *a5, *c3, *p3, *a7, *a8, *a9, *a10, *a11, *a12,
+        *a13, *a14;
+    float *out, *lossp;
+    // gradients
+    bf16_t *g15, *g14, *g13, *gc1, *g12, *g11, *gc2, *g10, *g9, *gc3, *g8, *g7, *gp3, *g6, *g5,
+        *gp2, *g4, *g3, *gp1, *g2, *g1;
+    float *in3, *tgt;                  // staging of caller inputs (fp32)
+    double *resp;                      // reduction partials
+    double *gpad, *gtmp;               // guess correction and a sweep buffer, reference layout
+                                       // (allocated by the first guess)
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+int ualloc(cap_unet *n, void **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) return CAP_ERR_NOMEM;
+    n->allocs.push_back(*p);
+    return CAP_OK;
+}
+
+#define UCHK(call)                                                 \
+    do {                                                           \
+        int _rc = (call);                                          \
+        if (_rc) return _rc;                                       \
+    } while (0)
+#define HCHK(call)                                                 \
+    do {                                                           \
+        if ((call) != hipSuccess) return CAP_ERR_HIP;              \
+    } while (0)
+
+void conv_launch(cap_unet *n, int l, const bf16_t *X, int cx, int xoff, bf16_t *Y, int cy, int yoff,
+                 float *yf)
+{
+    const ConvL &c = n->L[l];
+    const int lv = c.level;
+    const int D = n->nz >> lv, H = n->ny >> lv, W = n->nx >> lv;
+    const long nvox = (long)D * H * W;
+    const int grid = (int)((nvox + 63) / 64);
+    const float *bias = n->p + c.boff;
+    hipStream_t s = n->ctx->stream;
+    switch (c.coutp / 16) {
+    case 1: hipLaunchKernelGGL(k_conv<1>, dim3(grid), dim3(256), 0, s, D, H, W, X, cx, xoff, c.cinp, n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0); break;
+    case 2: hipLaunchKernelGGL(k_conv<2>, dim3(grid), dim3(256), 0, s, D, H, W, X, cx, xoff, c.cinp, n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0); break;
+    case 3: hipLaunchKernelGGL(k_conv<3>, dim3(grid), dim3(256), 0, s, D, H, W, X, cx, xoff, c.cinp, n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0); break;
+    case 4: hipLaunchKernelGGL(k_conv<4>, dim3(grid), dim3(256), 0, s, D, H, W, X, cx, xoff, c.cinp, n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0); break;
+    }
+}
+
+// gradient with respect to the layer input: G (layer output gradient, stride
+// coutp8) -> dX (cin channels, stride cdx), optionally masked
+void conv_bwd_launch(cap_unet *n, int l, const bf16_t *G, bf16_t *dX, int cdx, const bf16_t *M,
+                     int cm, int moff)
+{
+    const ConvL &c = n->L[l];
+    const int lv = c.level;
+    const int D = n->nz >> lv, H = n->ny >> lv, W = n->nx >> lv;
+    const long nvox = (long)D * H * W;
+    const int grid = (int)((nvox + 63) / 64);
+    hipStream_t s = n->ctx->stream;
+    switch (c.cinp16 / 16) {
+    case 1: hipLaunchKernelGGL(k_conv<1>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
+    case 2: hipLaunchKernelGGL(k_conv<2>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
+    case 3: hipLaunchKernelGGL(k_conv<3>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
+    case 4: hipLaunchKernelGGL(k_conv<4>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
+    case 5: hipLaunchKernelGGL(k_conv<5>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
+    case 7: hipLaunchKernelGGL(k_conv<7>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
+    }
+}
+
+void wgrad_launch(cap_unet *n, int l, const bf16_t *X, int cx, int xoff, const bf16_t *G)
+{
+    const ConvL &c = n->L[l];
+    const int lv = c.level;
+    const int D = n->nz >> lv, H = n->ny >> lv, W = n->nx >> lv;
+    const long nvox = (long)D * H * W;
+    hipLaunchKernelGGL(k_wgrad, dim3((int)((nvox + 127) / 128), 27), dim3(256), 0, n->ctx->stream,
+                       D, H, W, X, cx, xoff, c.cin, G, c.coutp8, c.cout, n->g + c.woff, n->g + c.boff);
+}
+
+int pack_weights(cap_unet *n)
+{
+    if (n->packed) return CAP_OK;
+    for (int l = 0; l < NL; ++l) {
+        const ConvL &c = n->L[l];
+        const long tot = (long)c.coutp * c.kp + (long)c.cinp16 * c.kpb;
+        hipLaunchKernelGGL(k_pack_w, dim3(grid_of(tot)), dim3(256), 0, n->ctx->stream, n->p + c.woff,
+                           c.cin, c.cout, c.cinp, c.kp, c.coutp, n->Wt[l], c.coutp8, c.kpb, c.cinp16,
+                           n->Wb[l]);
+    }
+    HCHK(hipGetLastError());
+    n->packed = 1;
+    return CAP_OK;
+}
+
+void pool_launch(cap_unet *n, int lvc, int C, const bf16_t *in, int cin_, int ioff, bf16_t *out)
+{
+    const int D = n->nz >> lvc, H = n->ny >> lvc, W = n->nx >> lvc;
+    hipLaunchKernelGGL(k_pool, dim3(grid_of((long)D * H * W * C)), dim3(256), 0, n->ctx->stream, D, H,
+                       W, C, in, cin_, ioff, out, C);
+}
+
+void up_launch(cap_unet *n, int lvf, int C, const bf16_t *in, int cin_, bf16_t *out, int cout_)
+{
+    const int D = n->nz >> lvf, H = n->ny >> lvf, W = n->nx >> lvf;
+    hipLaunchKernelGGL(k_up, dim3(grid_of((long)D * H * W * C)), dim3(256), 0, n->ctx->stream, D, H, W,
+                       C, in, cin_, out, cout_, 0);
+}
+
+void gcomb_launch(cap_unet *n, int lv, int C, bf16_t *g, int cg_, const bf16_t *A, int ca, int aoff,
+                  const bf16_t *B, int cb, int boff, const bf16_t *P, int cp, const bf16_t *mask,
+                  int cm, int moff)
+{
+    const int D = n->nz >> lv, H = n->ny >> lv, W = n->nx >> lv;
+    hipLaunchKernelGGL(k_gcombine, dim3(grid_of((long)D * H * W * C)), dim3(256), 0, n->ctx->stream, D,
+                       H, W, C, g, cg_, A, ca, aoff, B, cb, boff, P, cp, mask, cm, moff);
+}
+
+// input staging: device pointers as given, host pointers copied
+int stage_f32(cap_unet *n, const float *src, float *dst, long count, const float **out)
+{
+    if (n->ctx->flags & CAP_DEVICE_POINTERS) { *out = src; return CAP_OK; }
+    HCHK(hipMemcpyAsync(dst, src, count * sizeof(float), hipMemcpyHostToDevice, n->ctx->stream));
+    *out = dst;
+    return CAP_OK;
+}
+
+int forward(cap_unet *n, const float *in, float s0 = 1.f, float s1 = 1.f, float s2 = 1.f)
+{
+    UCHK(pack_weights(n));
+    const long n0 = n->nvox[0];
+    hipStream_t s = n->ctx->stream;
+    // the channels are multiplied by s0, s1, s2 on the way in
+    hipLaunchKernelGGL(k_pack_in, dim3(grid_of(n0)), dim3(256), 0, s, n0, in, s0, s1, s2, n->in0);
+    conv_launch(n, 0, n->in0, 8, 0, n->a1, 16, 0, nullptr);           // 3 -> 16
+    conv_launch(n, 1, n->a1, 16, 0, n->c1, 48, 32, nullptr);           // 16 -> 16 (skip 1)
+    pool_launch(n, 1, 16, n->c1, 48, 32, n->p1);
+    conv_launch(n, 2, n->p1, 16, 0, n->a3, 32, 0, nullptr);            // 16 -> 32
+    conv_launch(n, 3, n->a3, 32, 0, n->c2, 80, 48, nullptr);           // 32 -> 32 (skip 2)
+    pool_launch(n, 2, 32, n->c2, 80, 48, n->p2);
+    conv_launch(n, 4, n->p2, 32, 0, n->a5, 48, 0, nullptr);            // 32 -> 48
+    conv_launch(n, 5, n->a5, 48, 0, n->c3, 112, 64, nullptr);          // 48 -> 48 (skip 3)
+    pool_launch(n, 3, 48, n->c3, 112, 64, n->p3);
+    conv_launch(n, 6, n->p3, 48, 0, n->a7, 64, 0, nullptr);            // 48 -> 64
+    conv_launch(n, 7, n->a7, 64, 0, n->a8, 64, 0, nullptr);            // 64 -> 64
+    up_launch(n, 2, 64, n->a8, 64, n->c3, 112);
+    conv_launch(n, 8, n->c3, 112, 0, n->a9, 48, 0, nullptr);           // 112 -> 48
+    conv_launch(n, 9, n->a9, 48, 0, n->a10, 48, 0, nullptr);           // 48 -> 48
+    up_launch(n, 1, 48, n->a10, 48, n->c2, 80);
+    conv_launch(n, 10, n->c2, 80, 0, n->a11, 32, 0, nullptr);          // 80 -> 32
+    conv_launch(n, 11, n->a11, 32, 0, n->a12, 32, 0, nullptr);         // 32 -> 32
+    up_launch(n, 0, 32, n->a12, 32, n->c1, 48);
+    conv_launch(n, 12, n->c1, 48, 0, n->a13, 16, 0, nullptr);          // 48 -> 16
+    conv_launch(n, 13, n->a13, 16, 0, n->a14, 16, 0, nullptr);         // 16 -> 16
+    conv_launch(n, 14, n->a14, 16, 0, nullptr, 0, 0, n->out);          // 16 -> 1 (fp32)
+    HCHK(hipGetLastError());
+    return CAP_OK;
+}
+
+// gradients of mean((out - tgt)^2) into n->g; partial losses into n->lossp
+int backward(cap_unet *n, const float *tgt, int *nlossp)
+{
+    const long n0 = n->nvox[0];
+    hipStream_t s = n->ctx->stream;
+    HCHK(hipMemsetAsync(n->g, 0, n->nparam * sizeof(float), s));
+    const int lb = grid_of(n0);
+    hipLaunchKernelGGL(k_loss, dim3(lb), dim3(256), 0, s, n0, n->out, tgt, n->g15, 8, n->lossp);
+    *nlossp = lb;
+    // head
+    wgrad_launch(n, 14, n->a14, 16, 0, n->g15);
+    conv_bwd_launch(n, 14, n->g15, n->g14, 16, n->a14, 16, 0);
+    wgrad_launch(n, 13, n->a13, 16, 0, n->g14);
+    conv_bwd_launch(n, 13, n->g14, n->g13, 16, n->a13, 16, 0);
+    wgrad_launch(n, 12, n->c1, 48, 0, n->g13);
+    conv_bwd_launch(n, 12, n->g13, n->gc1, 48, nullptr, 0, 0);
+    gcomb_launch(n, 1, 32, n->g12, 32, n->gc1, 48, 0, nullptr, 0, 0, nullptr, 0, n->a12, 32, 0);
+    wgrad_launch(n, 11, n->a11, 32, 0, n->g12);
+    conv_bwd_launch(n, 11, n->g12, n->g11, 32, n->a11, 32, 0);
+    wgrad_launch(n, 10, n->c2, 80, 0, n->g11);
+    conv_bwd_launch(n, 10, n->g11, n->gc2, 80, nullptr, 0, 0);
+    gcomb_launch(n, 2, 48, n->g10, 48, n->gc2, 80, 0, nullptr, 0, 0, nullptr, 0, n->a10, 48, 0);
+    wgrad_launch(n, 9, n->a9, 48, 0, n->g10);
+    conv_bwd_launch(n, 9, n->g10, n->g9, 48, n->a9, 48, 0);
+    wgrad_launch(n, 8, n->c3, 112, 0, n->g9);
+    conv_bwd_launch(n, 8, n->g9, n->gc3, 112, nullptr, 0, 0);
+    gcomb_launch(n, 3, 64, n->g8, 64, n->gc3, 112, 0, nullptr, 0, 0, nullptr, 0, n->a8, 64, 0);
+    wgrad_launch(n, 7, n->a7, 64, 0, n->g8);
+    conv_bwd_launch(n, 7, n->g8, n->g7, 64, n->a7, 64, 0);
+    wgrad_launch(n, 6, n->p3, 48, 0, n->g7);
+    conv_bwd_launch(n, 6, n->g7, n->gp3, 48, nullptr, 0, 0);
+    gcomb_launch(n, 2, 48, n->g6, 48, nullptr, 0, 0, n->gc3, 112, 64, n->gp3, 48, n->c3, 112, 64);
+    wgrad_launch(n, 5, n->a5, 48, 0, n->g6);
+    conv_bwd_launch(n, 5, n->g6, n->g5, 48, n->a5, 48, 0);
+    wgrad_launch(n, 4, n->p2, 32, 0, n->g5);
+    conv_bwd_launch(n, 4, n->g5, n->gp2, 32, nullptr, 0, 0);
+    gcomb_launch(n, 1, 32, n->g4, 32, nullptr, 0, 0, n->gc2, 80, 48, n->gp2, 32, n->c2, 80, 48);
+    wgrad_launch(n, 3, n->a3, 32, 0, n->g4);
+    conv_bwd_launch(n, 3, n->g4, n->g3, 32, n->a3, 32, 0);
+    wgrad_launch(n, 2, n->p1, 16, 0, n->g3);
+    conv_bwd_launch(n, 2, n->g3, n->gp1, 16, nullptr, 0, 0);
+    gcomb_launch(n, 0, 16, n->g2, 16, nullptr, 0, 0, n->gc1, 48, 32, n->gp1, 16, n->c1, 48, 32);
+    wgrad_launch(n, 1, n->a1, 16, 0, n->g2);
+    conv_bwd_launch(n, 1, n->g2, n->g1, 16, n->a1, 16, 0);
+    wgrad_launch(n, 0, n->in0, 8, 0, n->g1);
+    HCHK(hipGetLastError());
+    return CAP_OK;
+}
+
+int read_loss(cap_unet *n, int np, float *loss)
+{
+    std::vector<float> h(np);
+    HCHK(hipMemcpyAsync(h.data(), n->lossp, np * sizeof(float), hipMemcpyDeviceToHost, n->ctx->stream));
+    HCHK(hipStreamSynchronize(n->ctx->stream));
+    double s = 0.0;
+    for (int i = 0; i < np; ++i) s += h[i];
+    *loss = (float)(s / (double)n->nvox[0]);
+    return CAP_OK;
+}
+
+uint64_t smx(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+}  // namespace
+
+extern "C" int cap_unet_create(cap_ctx *ctx, int nx, int ny, int nz, unsigned long long seed,
+                               cap_unet **out)
+{
+    if (!ctx || !out || nx < 8 || ny < 8 || nz < 8 || nx % 8 || ny % 8 || nz % 8) return CAP_ERR_ARG;
+    cap_unet *n = new (std::nothrow) cap_unet();
+    if (!n) return CAP_ERR_NOMEM;
+    n->ctx = ctx;
+    n->nx = nx; n->ny = ny; n->nz = nz;
+    for (int l = 0; l < 4; ++l) n->nvox[l] = (long)(nx >> l) * (ny >> l) * (nz >> l);
+    // layer table: cin, cout, level, relu
+    const int spec[NL][4] = {{3, 16, 0, 1},  {16, 16, 0, 1}, {16, 32, 1, 1}, {32, 32, 1, 1},
+                             {32, 48, 2, 1}, {48, 48, 2, 1}, {48, 64, 3, 1}, {64, 64, 3, 1},
+                             {112, 48, 2, 1}, {48, 48, 2, 1}, {80, 32, 1, 1}, {32, 32, 1, 1},
+                             {48, 16, 0, 1}, {16, 16, 0, 1}, {16, 1, 0, 0}};
+    size_t off = 0;
+    for (int l = 0; l < NL; ++l) {
+        ConvL &c = n->L[l];
+        c.cin = spec[l][0]; c.cout = spec[l][1]; c.level = spec[l][2]; c.relu = spec[l][3];
+        c.cinp = (c.cin + 7) / 8 * 8;
+        c.kp = (27 * c.cinp + 31) / 32 * 32;
+        c.coutp = (c.cout + 15) / 16 * 16;
+        c.coutp8 = (c.cout + 7) / 8 * 8;
+        c.kpb = (27 * c.coutp8 + 31) / 32 * 32;
+        c.cinp16 = (c.cin + 15) / 16 * 16;
+        c.woff = off;
+        off += (size_t)c.cout * 27 * c.cin;
+        c.boff = off;
+        off += (size_t)c.cout;
+    }
+    n->nparam = off;
+    int rc;
+    auto A = [&](void **p, size_t bytes) { return ualloc(n, p, bytes); };
+    const long n0 = n->nvox[0], n1 = n->nvox[1], n2 = n->nvox[2], n3 = n->nvox[3];
+    const size_t B = sizeof(bf16_t);
+    if ((rc = A((void **)&n->p, off * 4)) || (rc = A((void **)&n->g, off * 4)) ||
+        (rc = A((void **)&n->m, off * 4)) || (rc = A((void **)&n->v, off * 4)))
+        goto fail;
+    for (int l = 0; l < NL; ++l) {
+        const ConvL &c = n->L[l];
+        if ((rc = A((void **)&n->Wt[l], (size_t)c.coutp * c.kp * B)) ||
+            (rc = A((void **)&n->Wb[l], (size_t)c.cinp16 * c.kpb * B)))
+            goto fail;
+    }
+    if ((rc = A((void **)&n->in0, n0 * 8 * B)) || (rc = A((void **)&n->a1, n0 * 16 * B)) ||
+        (rc = A((void **)&n->c1, n0 * 48 * B)) || (rc = A((void **)&n->p1, n1 * 16 * B)) ||
+        (rc = A((void **)&n->a3, n1 * 32 * B)) || (rc = A((void **)&n->c2, n1 * 80 * B)) ||
+        (rc = A((void **)&n->p2, n2 * 32 * B)) || (rc = A((void **)&n->a5, n2 * 48 * B)) ||
+        (rc = A((void **)&n->c3, n2 * 112 * B)) || (rc = A((void **)&n->p3, n3 * 48 * B)) ||
+        (rc = A((void **)&n->a7, n3 * 64 * B)) || (rc = A((void **)&n->a8, n3 * 64 * B)) ||
+        (rc = A((void **)&n->a9, n2 * 48 * B)) || (rc = A((void **)&n->a10, n2 * 48 * B)) ||
+        (rc = A((void **)&n->a11, n1 * 32 * B)) || (rc = A((void **)&n->a12, n1 * 32 * B)) ||
+        (rc = A((void **)&n->a13, n0 * 16 * B)) || (rc = A((void **)&n->a14, n0 * 16 * B)) ||
+        (rc = A((void **)&n->out, n0 * 4)) || (rc = A((void **)&n->lossp, 16384 * 4)) ||
+        (rc = A((void **)&n->g15, n0 * 8 * B)) || (rc = A((void **)&n->g14, n0 * 16 * B)) ||
+        (rc = A((void **)&n->g13, n0 * 16 * B)) || (rc = A((void **)&n->gc1, n0 * 48 * B)) ||
+        (rc = A((void **)&n->g12, n1 * 32 * B)) || (rc = A((void **)&n->g11, n1 * 32 * B)) ||
+        (rc = A((void **)&n->gc2, n1 * 80 * B)) || (rc = A((void **)&n->g10, n2 * 48 * B)) ||
+        (rc = A((void **)&n->g9, n2 * 48 * B)) || (rc = A((void **)&n->gc3, n2 * 112 * B)) ||
+        (rc = A((void **)&n->g8, n3 * 64 * B)) || (rc = A((void **)&n->g7, n3 * 64 * B)) ||
+        (rc = A((void **)&n->gp3, n3 * 48 * B)) || (rc = A((void **)&n->g6, n2 * 48 * B)) ||
+        (rc = A((void **)&n->g5, n2 * 48 * B)) || (rc = A((void **)&n->gp2, n2 * 32 * B)) ||
+        (rc = A((void **)&n->g4, n1 * 32 * B)) || (rc = A((void **)&n->g3, n1 * 32 * B)) ||
+        (rc = A((void **)&n->gp1, n1 * 16 * B)) || (rc = A((void **)&n->g2, n0 * 16 * B)) ||
+        (rc = A((void **)&n->g1, n0 * 16 * B)) || (rc = A((void **)&n->in3, n0 * 3 * 4)) ||
+        (rc = A((void **)&n->tgt, n0 * 4)) || (rc = A((void **)&n->resp, 16384 * 2 * 8)))
+        goto fail;
+    {   // He initialisation (counter-based normal deviates), zero biases
+        std::vector<float> h(off, 0.f);
+        for (int l = 0; l < NL; ++l) {
+            const ConvL &c = n->L[l];
+            const double sd = (l == NL - 1 ? 0.1 : 1.0) * sqrt(2.0 / (27.0 * c.cin));
+            for (size_t i = 0; i < (size_t)c.cout * 27 * c.cin; ++i) {
+                const uint64_t k = smx(seed ^ smx((uint64_t)l * 0x100000001ull + i));
+                const double u1 = ((k >> 11) + 0.5) * 0x1.0p-53, u2 = ((smx(k) >> 11) + 0.5) * 0x1.0p-53;
+                h[c.woff + i] = (float)(sd * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+            }
+        }
+        if (hipMemcpy(n->p, h.data(), off * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(n->m, 0, off * 4) != hipSuccess || hipMemset(n->v, 0, off * 4) != hipSuccess) {
+            rc = CAP_ERR_HIP;
+            goto fail;
+        }
+    }
+    // concat buffers are written piecewise: clear them (padding channels too)
+    if (hipMemset(n->c1, 0, n0 * 48 * B) != hipSuccess || hipMemset(n->c2, 0, n1 * 80 * B) != hipSuccess ||
+        hipMemset(n->c3, 0, n2 * 112 * B) != hipSuccess || hipMemset(n->in0, 0, n0 * 8 * B) != hipSuccess) {
+        rc = CAP_ERR_HIP;
+        goto fail;
+    }
+    n->packed = 0;
+    n->step = 0;
+    *out = n;
+    return CAP_OK;
+fail:
+    for (void *p : n->allocs) (void)hipFree(p);
+    delete n;
+    return rc;
+}
+
+extern "C" int cap_unet_destroy(cap_unet *n)
+{
+    if (!n) return CAP_OK;
+    (void)hipStreamSynchronize(n->ctx->stream);
+    for (void *p : n->allocs) (void)hipFree(p);
+    delete n;
+    return CAP_OK;
+}
+
+extern "C" int cap_unet_num_params(const cap_unet *n, long *count)
+{
+    if (!n || !count) return CAP_ERR_ARG;
+    *count = (long)n->nparam;
+    return CAP_OK;
+}
+
+extern "C" int cap_unet_get_params(cap_unet *n, float *p)
+{
+    if (!n || !p) return CAP_ERR_ARG;
+    HCHK(hipMemcpyAsync(p, n->p, n->nparam * 4, hipMemcpyDeviceToHost, n->ctx->stream));
+    HCHK(hipStreamSynchronize(n->ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_unet_set_params(cap_unet *n, const float *p)
+{
+    if (!n || !p) return CAP_ERR_ARG;
+    HCHK(hipMemcpyAsync(n->p, p, n->nparam * 4, hipMemcpyHostToDevice, n->ctx->stream));
+    HCHK(hipStreamSynchronize(n->ctx->stream));
+    n->packed = 0;
+    return CAP_OK;
+}
+
+extern "C" int cap_unet_forward(cap_unet *n, const float *in, float *out)
+{
+    if (!n || !in || !out) return CAP_ERR_ARG;
+    const long n0 = n->nvox[0];
+    const float *din;
+    UCHK(stage_f32(n, in, n->in3, 3 * n0, &din));
+    UCHK(forward(n, din));
+    if (n->ctx->flags & CAP_DEVICE_POINTERS)
+        HCHK(hipMemcpyAsync(out, n->out, n0 * 4, hipMemcpyDeviceToDevice, n->ctx->stream));
+    else
+        HCHK(hipMemcpyAsync(out, n->out, n0 * 4, hipMemcpyDeviceToHost, n->ctx->stream));
+    HCHK(hipStreamSynchronize(n->ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_unet_gradients(cap_unet *n, const float *in, const float *target, float *loss,
+                                  float *grads)
+{
+    if (!n || !in || !target || !loss) return CAP_ERR_ARG;
+    const long n0 = n->nvox[0];
+    const float *din, *dt;
+    UCHK(stage_f32(n, in, n->in3, 3 * n0, &din));
+    UCHK(stage_f32(n, target, n->tgt, n0, &dt));
+    UCHK(forward(n, din));
+    int np;
+    UCHK(backward(n, dt, &np));
+    UCHK(read_loss(n, np, loss));
+    if (grads) {
+        HCHK(hipMemcpyAsync(grads, n->g, n->nparam * 4,
+                            (n->ctx->flags & CAP_DEVICE_POINTERS) ? hipMemcpyDeviceToDevice
+                                                                  : hipMemcpyDeviceToHost,
+                            n->ctx->stream));
+        HCHK(hipStreamSynchronize(n->ctx->stream));
+    }
+    return CAP_OK;
+}
+
+extern "C" int cap_unet_train_step(cap_unet *n, const float *in, const float *target, float lr,
+                                   float *loss)
+{
+    if (!n || !in || !target || !loss || !(lr > 0.f)) return CAP_ERR_ARG;
+    UCHK(cap_unet_gradients(n, in, target, loss, nullptr));
+    n->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)n->step), c2 = 1.f - powf(b2, (float)n->step);
+    hipLaunchKernelGGL(k_adam, dim3(grid_of((long)n->nparam)), dim3(256), 0, n->ctx->stream,
+                       (long)n->nparam, n->p, (const float *)n->g, n->m, n->v, lr, b1, b2, 1e-8f, c1, c2);
+    HCHK(hipGetLastError());
+    n->packed = 0;
+    HCHK(hipStreamSynchronize(n->ctx->stream));
+    return CAP_OK;
+}
+
+// ---------------------------------------------------------- guess + solve
+namespace {
+int res_l1(cap_ctx *ctx, cap_unet *n, const cap_coef *c, const double *fi, double *out)
+{
+    const long cells = (long)(ctx->ni - 4) * (ctx->nj - 4) * (ctx->nk - 4);
+    const int nb = grid_of(cells);
+    hipLaunchKernelGGL(k_res_l1, dim3(nb), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj, ctx->nk,
+                       (const double *)c->ae, (const double *)c->aw, (const double *)c->an,
+                       (const double *)c->as, (const double *)c->at, (const double *)c->ab,
+                       (const double *)c->ap, (const double *)c->su, fi, n->resp);
+    HCHK(hipGetLastError());
+    std::vector<double> h(2 * nb);
+    HCHK(hipMemcpyAsync(h.data(), n->resp, 2 * nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    long double s = 0.0L;
+    for (int i = 0; i < 2 * nb; ++i) s += h[i];
+    *out = (double)s;
+    return CAP_OK;
+}
+}  // namespace
+
+namespace {
+// sums of the per-block compensated partials (nb blocks x nv values)
+int sum_partials(cap_unet *n, int nb, int nv, double *out)
+{
+    std::vector<double> h((size_t)2 * nv * nb);
+    HCHK(hipMemcpyAsync(h.data(), n->resp, h.size() * 8, hipMemcpyDeviceToHost, n->ctx->stream));
+    HCHK(hipStreamSynchronize(n->ctx->stream));
+    for (int m = 0; m < nv; ++m) {
+        long double t = 0.0L;
+        for (int b = 0; b < nb; ++b) t += (long double)h[(size_t)2 * nv * b + 2 * m] + h[(size_t)2 * nv * b + 2 * m + 1];
+        out[m] = (double)t;
+    }
+    return CAP_OK;
+}
+}  // namespace
+
+extern "C" int cap_unet_guess(cap_ctx *ctx, cap_unet *n, double *fi, const double *prev,
+                              const cap_coef *c, double *scales)
+{
+    if (!ctx || !n || !fi || !c || n->ctx != ctx) return CAP_ERR_ARG;
+    if (!(ctx->flags & CAP_DEVICE_POINTERS)) return CAP_ERR_ARG;   // device-resident fields
+    if (ctx->nj - 4 != n->nx || ctx->ni - 4 != n->ny || ctx->nk - 4 != n->nz) return CAP_ERR_ARG;
+    const double *A[8] = {(const double *)c->ae, (const double *)c->aw, (const double *)c->an,
+                          (const double *)c->as, (const double *)c->at, (const double *)c->ab,
+                          (const double *)c->ap, (const double *)c->su};
+    for (int q = 0; q < 8; ++q) if (!A[q]) return CAP_ERR_ARG;
+    if (!n->gpad) {
+        UCHK(ualloc(n, (void **)&n->gpad, ctx->nstd * sizeof(double)));
+        UCHK(ualloc(n, (void **)&n->gtmp, ctx->nstd * sizeof(double)));
+        HCHK(hipMemsetAsync(n->gpad, 0, ctx->nstd * sizeof(double), ctx->stream));
+        HCHK(hipMemsetAsync(n->gtmp, 0, ctx->nstd * sizeof(double), ctx->stream));
+    }
+    const long n0 = n->nvox[0];
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_ref_io, dim3(grid_of(n0)), dim3(256), 0, st, ctx->ni, ctx->nj, ctx->nk, 0,
+                       A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], fi, prev, n->in3,
+                       (const float *)nullptr, (double *)nullptr, 0.0, 0.0);
+    // normalisers: the residual's RMS s, the mean a of ap, the RMS ps of the
+    // previous p' (1 when it is zero or absent)
+    const int nbs = std::min(grid_of(n0, 256 * 4), 4096);   // 6 doubles each fit resp
+    hipLaunchKernelGGL(k_in_stats, dim3(nbs), dim3(256), 0, st, n0, (const float *)n->in3, n->resp);
+    HCHK(hipGetLastError());
+    double m[3];
+    UCHK(sum_partials(n, nbs, 3, m));
+    const double s = m[0] > 0 ? sqrt(m[0] / n0) : 1.0, a = m[1] != 0 ? m[1] / n0 : 1.0,
+                 ps = m[2] > 0 ? sqrt(m[2] / n0) : 1.0;
+    UCHK(forward(n, n->in3, (float)(1.0 / s), (float)(1.0 / a), (float)(1.0 / ps)));
+    hipLaunchKernelGGL(k_ref_io, dim3(grid_of(n0)), dim3(256), 0, st, ctx->ni, ctx->nj, ctx->nk, 1,
+                       A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], fi, prev, (float *)nullptr,
+                       (const float *)n->out, n->gpad, 0.0, 0.0);
+    // the network gives the direction g of the correction of x (the previous
+    // p', or the start): scale it by its energy-norm step g.r / g.Mg, then
+    // damp its high-frequency error -- which the residual weights most --
+    // with GUESS_SWEEPS damped-Jacobi sweeps on M g = su - M x
+    const double *x = prev ? prev : fi;
+    const int nbe = std::min(grid_of(n0, 256 * 4), 4096);
+    hipLaunchKernelGGL(k_edots, dim3(nbe), dim3(256), 0, st, ctx->ni, ctx->nj, ctx->nk, A[0], A[1],
+                       A[2], A[3], A[4], A[5], A[6], A[7], x, (const double *)n->gpad, n->resp);
+    HCHK(hipGetLastError());
+    double e[2];
+    UCHK(sum_partials(n, nbe, 2, e));
+    double sc = e[1] > 0 ? e[0] / e[1] : 0.0;
+    for (int sw = 0; sw < GUESS_SWEEPS; ++sw) {
+        const double *gi = (sw & 1) ? n->gtmp : n->gpad;
+        double *go = (sw & 1) ? n->gpad : n->gtmp;
+        hipLaunchKernelGGL(k_jacobi, dim3(grid_of(n0)), dim3(256), 0, st, ctx->ni, ctx->nj, ctx->nk,
+                           A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], x, gi, go, sc, GUESS_OMEGA);
+        sc = 1.0;
+    }
+    HCHK(hipGetLastError());
+    // the guess is fi + beta (prev - fi) + alpha g with the least-squares
+    // (beta, alpha) of the residual's 2-norm (alpha alone when prev is null),
+    // so it never raises that norm above the start's or the previous p''s
+    const int nba = std::min(grid_of(n0, 256 * 4), 3072);   // 10 doubles each fit resp
+    hipLaunchKernelGGL(k_alpha, dim3(nba), dim3(256), 0, st, ctx->ni, ctx->nj, ctx->nk, A[0], A[1],
+                       A[2], A[3], A[4], A[5], A[6], A[7], (const double *)fi, prev,
+                       (const double *)n->gpad, n->resp);
+    HCHK(hipGetLastError());
+    double q[5];
+    UCHK(sum_partials(n, nba, 5, q));
+    double alpha = q[1] > 0 ? q[0] / q[1] : 0.0, beta = 0.0;
+    if (prev && q[3] > 0) {
+        const double det = q[1] * q[3] - q[4] * q[4];
+        if (det > 1e-12 * q[1] * q[3]) {
+            alpha = (q[0] * q[3] - q[2] * q[4]) / det;
+            beta = (q[2] * q[1] - q[0] * q[4]) / det;
+        } else {    // g and prev - fi are (nearly) parallel
+            alpha = 0.0;
+            beta = q[2] / q[3];
+        }
+    }
+    hipLaunchKernelGGL(k_ref_io, dim3(grid_of(n0)), dim3(256), 0, st, ctx->ni, ctx->nj, ctx->nk, 2,
+                       A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], fi, prev, (float *)nullptr,
+                       (const float *)nullptr, n->gpad, alpha, beta);
+    HCHK(hipGetLastError());
+    HCHK(hipStreamSynchronize(st));
+    if (scales) { scales[0] = s; scales[1] = a; scales[2] = ps; scales[3] = alpha; scales[4] = beta; }
+    return CAP_OK;
+}
+
+extern "C" int cap_pcgsip_guess(cap_ctx *ctx, cap_unet *n, double *fi, const double *prev,
+                                const cap_coef *c, double alfa, double sor, int nsw, cap_report *rep,
+                                int *accepted)
+{
+    if (!ctx || !n || !fi || !c || !rep || !accepted) return CAP_ERR_ARG;
+    if (!(ctx->flags & CAP_DEVICE_POINTERS)) return CAP_ERR_ARG;
+    const size_t bytes = ctx->nstd * sizeof(double);
+    double *keep;
+    HCHK(hipMalloc(&keep, bytes));
+    int rc = CAP_OK;
+    double r0 = 0.0, rg = 0.0;
+    if (hipMemcpyAsync(keep, fi, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) rc = CAP_ERR_HIP;
+    if (!rc) rc = res_l1(ctx, n, c, fi, &r0);
+    if (!rc) rc = cap_unet_guess(ctx, n, fi, prev, c, nullptr);
+    if (!rc) rc = res_l1(ctx, n, c, fi, &rg);
+    // the guess is used only if it lowers the L1 residual of the start
+    *accepted = !rc && rg < r0;
+    if (!rc && !*accepted &&
+        hipMemcpyAsync(fi, keep, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+        rc = CAP_ERR_HIP;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(keep);
+    if (rc) return rc;
+    // the unchanged solver stops on its L1 residual relative to its own
+    // start; from the guess, the tolerance is rescaled so that it stops at
+    // the absolute residual the solve from the caller's start stops at
+    const double sor_eff = *accepted && rg > 0 ? sor * (r0 / rg) : sor;
+    rc = cap_pcgsip(ctx, fi, c, alfa, sor_eff, nsw, rep);
+    if (rc >= 0 && !*accepted && rep->status == CAP_OK) return CAP_GUESS_REJECTED;
+    return rc;
+}

@@ -169,6 +169,64 @@ module cap_hip
       type(c_ptr), intent(in) :: var(7), geom(12)
       type(cap_bc), intent(in) :: bc
     end function
+    ! learned initial guess (no counterpart in the reference)
+    integer(c_int) function cap_unet_create(ctx, nx, ny, nz, seed, net) bind(c, name='cap_unet_create')
+      import :: c_ptr, c_int, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nx, ny, nz
+      integer(c_long_long), value :: seed
+      type(c_ptr), intent(out) :: net
+    end function
+    integer(c_int) function cap_unet_destroy(net) bind(c, name='cap_unet_destroy')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: net
+    end function
+    integer(c_int) function cap_unet_num_params(net, count) bind(c, name='cap_unet_num_params')
+      import :: c_ptr, c_int, c_long
+      type(c_ptr), value :: net
+      integer(c_long), intent(out) :: count
+    end function
+    integer(c_int) function cap_unet_get_params(net, params) bind(c, name='cap_unet_get_params')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: net, params
+    end function
+    integer(c_int) function cap_unet_set_params(net, params) bind(c, name='cap_unet_set_params')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: net, params
+    end function
+    integer(c_int) function cap_unet_forward(net, in, out) bind(c, name='cap_unet_forward')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: net, in, out
+    end function
+    integer(c_int) function cap_unet_gradients(net, in, target, loss, grads) &
+        bind(c, name='cap_unet_gradients')
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: net, in, target, grads
+      real(c_float), intent(out) :: loss
+    end function
+    integer(c_int) function cap_unet_train_step(net, in, target, lr, loss) &
+        bind(c, name='cap_unet_train_step')
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: net, in, target
+      real(c_float), value :: lr
+      real(c_float), intent(out) :: loss
+    end function
+    integer(c_int) function cap_unet_guess(ctx, net, fi, prev, c, scales) bind(c, name='cap_unet_guess')
+      import :: c_ptr, c_int, c_double, cap_coef
+      type(c_ptr), value :: ctx, net, fi, prev
+      type(cap_coef), intent(in) :: c
+      real(c_double), intent(out) :: scales(5)
+    end function
+    integer(c_int) function cap_pcgsip_guess(ctx, net, fi, prev, c, alfa, sor, nsw, rep, accepted) &
+        bind(c, name='cap_pcgsip_guess')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, net, fi, prev
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: alfa, sor
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+      integer(c_int), intent(out) :: accepted
+    end function
     integer(c_int) function cap_continuity(ctx, f1, f2, f3, sums) bind(c, name='cap_continuity')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx, f1, f2, f3

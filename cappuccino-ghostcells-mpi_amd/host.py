@@ -36,7 +36,9 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_build_lognormal", "cap_time_kernels", "cap_fluxmass",
            "cap_piso_assemble_pressure_eq", "cap_assemble_pressure_eq", "cap_fluxmc",
            "cap_continuity", "cap_fluxuvw", "cap_update_ghostcells",
-           "cap_courant")
+           "cap_courant", "cap_unet_create", "cap_unet_destroy", "cap_unet_num_params",
+           "cap_unet_get_params", "cap_unet_set_params", "cap_unet_forward",
+           "cap_unet_gradients", "cap_unet_train_step", "cap_unet_guess", "cap_pcgsip_guess")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -99,6 +101,18 @@ def load():
     L.cap_update_ghostcells.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(_Bc)]
     L.cap_courant.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(d), C.POINTER(d), i, d,
                               C.POINTER(d)]
+    f = C.c_float
+    L.cap_unet_create.argtypes = [vp, i, i, i, C.c_ulonglong, C.POINTER(vp)]
+    L.cap_unet_destroy.argtypes = [vp]
+    L.cap_unet_num_params.argtypes = [vp, C.POINTER(C.c_long)]
+    L.cap_unet_get_params.argtypes = [vp, vp]
+    L.cap_unet_set_params.argtypes = [vp, vp]
+    L.cap_unet_forward.argtypes = [vp, vp, vp]
+    L.cap_unet_gradients.argtypes = [vp, vp, vp, C.POINTER(f), vp]
+    L.cap_unet_train_step.argtypes = [vp, vp, vp, f, C.POINTER(f)]
+    L.cap_unet_guess.argtypes = [vp, vp, vp, vp, C.POINTER(_Coef), C.POINTER(d)]
+    L.cap_pcgsip_guess.argtypes = [vp, vp, vp, vp, C.POINTER(_Coef), d, d, i, C.POINTER(_Report),
+                                   C.POINTER(i)]
     _lib = L
     return L
 
@@ -297,3 +311,95 @@ class Context:
         st = coef._struct()
         rc = self._lib.cap_build_lognormal(self._h, seed, sigma, C.byref(st))
         self._check(rc, "cap_build_lognormal", allow=(CAP_OK,))
+
+
+def _fptr(a):
+    """float32 array (numpy, or torch on the device)"""
+    if hasattr(a, "data_ptr"):
+        import torch
+        assert a.dtype == torch.float32 and a.is_contiguous()
+        return a.data_ptr()
+    import numpy as np
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data
+
+
+class UNet:
+    """The learned initial guess (csrc/cap_unet.hip) on one context's grid:
+    nx x ny x nz interior cells (x = the reference's j index, fastest)."""
+
+    def __init__(self, ctx, nx, ny, nz, seed=1):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        self.shape = (nz, ny, nx)
+        h = C.c_void_p()
+        rc = self._lib.cap_unet_create(ctx._h, nx, ny, nz, seed, C.byref(h))
+        ctx._check(rc, "cap_unet_create", allow=(CAP_OK,))
+        self._h = h
+        n = C.c_long()
+        self._lib.cap_unet_num_params(self._h, C.byref(n))
+        self.num_params = n.value
+
+    def close(self):
+        if self._h:
+            self._lib.cap_unet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        self._ctx._check(rc, what, allow=(CAP_OK,))
+
+    def get_params(self):
+        import numpy as np
+        p = np.zeros(self.num_params, np.float32)
+        self._chk(self._lib.cap_unet_get_params(self._h, p.ctypes.data), "cap_unet_get_params")
+        return p
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._chk(self._lib.cap_unet_set_params(self._h, p.ctypes.data), "cap_unet_set_params")
+
+    def forward(self, x, out):
+        self._chk(self._lib.cap_unet_forward(self._h, _fptr(x), _fptr(out)), "cap_unet_forward")
+
+    def gradients(self, x, target, grads=None):
+        loss = C.c_float()
+        rc = self._lib.cap_unet_gradients(self._h, _fptr(x), _fptr(target), C.byref(loss),
+                                          _fptr(grads) if grads is not None else None)
+        self._chk(rc, "cap_unet_gradients")
+        return loss.value
+
+    def train_step(self, x, target, lr):
+        loss = C.c_float()
+        rc = self._lib.cap_unet_train_step(self._h, _fptr(x), _fptr(target), float(lr),
+                                           C.byref(loss))
+        self._chk(rc, "cap_unet_train_step")
+        return loss.value
+
+    def guess(self, fi, coef, prev=None):
+        """fi (device, reference layout) becomes the guess; returns
+        (s, a, ps, alpha, beta), the normalisers and the fitted weights."""
+        sc = (C.c_double * 5)()
+        st = coef._struct()
+        rc = self._lib.cap_unet_guess(self._ctx._h, self._h, _ptr(fi),
+                                      _ptr(prev) if prev is not None else None, C.byref(st), sc)
+        self._chk(rc, "cap_unet_guess")
+        return tuple(sc)
+
+    def pcgsip(self, fi, coef, prev=None, alfa=0.0, sor=1e-8, nsw=2000):
+        rep = _Report()
+        acc = C.c_int()
+        st = coef._struct()
+        rc = self._lib.cap_pcgsip_guess(self._ctx._h, self._h, _ptr(fi),
+                                        _ptr(prev) if prev is not None else None, C.byref(st),
+                                        alfa, sor, nsw, C.byref(rep), C.byref(acc))
+        self._ctx._check(rc, "cap_pcgsip_guess",
+                         allow=(CAP_OK, CAP_MAXITER, CAP_GUESS_REJECTED))
+        return dict(resor=rep.resor, resl=rep.resl, iters=rep.iters, status=rep.status,
+                    ms=rep.ms, converged=rep.status == CAP_OK, accepted=bool(acc.value))

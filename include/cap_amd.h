@@ -142,6 +142,50 @@ int cap_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *f, int idew, int id
 int cap_fluxmc(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
                const double *fif, double *fmcor);
 
+/* ---- Learned initial guess (no counterpart in the reference) ----------
+ * A 3-D U-Net (4 levels, 16/32/48/64 channels, 3x3x3 convolutions, BF16
+ * weights and activations, FP32 accumulation on the matrix cores) that maps
+ * (su, ap, previous p') on the nx x ny x nz interior cells (x = the j index,
+ * fastest) to the direction of the correction of p'.  Inputs of
+ * cap_unet_forward / _gradients / _train_step are three fp32 channels,
+ * channel-major, normalised by the caller; pointers follow the context's
+ * CAP_DEVICE_POINTERS flag. */
+typedef struct cap_unet cap_unet;
+int cap_unet_create(cap_ctx *ctx, int nx, int ny, int nz, unsigned long long seed,
+                    cap_unet **net);
+int cap_unet_destroy(cap_unet *net);
+int cap_unet_num_params(const cap_unet *net, long *count);
+int cap_unet_get_params(cap_unet *net, float *params);
+int cap_unet_set_params(cap_unet *net, const float *params);
+int cap_unet_forward(cap_unet *net, const float *in, float *out);
+/* loss = mean((out - target)^2); grads (may be NULL): d loss / d params */
+int cap_unet_gradients(cap_unet *net, const float *in, const float *target, float *loss,
+                       float *grads);
+/* one Adam step (beta 0.9 / 0.999) on the same loss */
+int cap_unet_train_step(cap_unet *net, const float *in, const float *target, float lr,
+                        float *loss);
+/* fi (device, reference layout): in the start (0 in calcp, p in PISO),
+ * out the guess fi + beta (prev - fi) + alpha g on the solve range.  g is
+ * the network's output for the channels (r / s, ap / a, prev / ps): prev the
+ * previous-step p' (NULL: zero, and beta = 0), r = su - M prev its residual
+ * (M x = ap x - sum a_nb x_nb; su - M fi when prev is NULL), s the RMS of r,
+ * a the mean of ap and ps the RMS of prev (1 when zero).  (beta, alpha) is the least-squares fit of
+ * the residual's 2-norm, so the guess never raises that norm above the
+ * start's or prev's.  scales = {s, a, ps, alpha, beta} (may be NULL).  Needs
+ * a CAP_DEVICE_POINTERS context whose interior is nx x ny x nz. */
+int cap_unet_guess(cap_ctx *ctx, cap_unet *net, double *fi, const double *prev,
+                   const cap_coef *c, double *scales);
+/* PCG-SIP (as cap_pcgsip) from the network's guess when that guess lowers the
+ * L1 residual of the start fi, otherwise from fi (then the status is
+ * CAP_GUESS_REJECTED when the solve converged).  *accepted tells which.  The
+ * solver stops on its L1 residual relative to its own start, so from an
+ * accepted guess sor is rescaled by r0 / r_guess: the solve stops at the
+ * absolute residual that the solve from fi stops at, in fewer iterations when
+ * the guess is good. */
+int cap_pcgsip_guess(cap_ctx *ctx, cap_unet *net, double *fi, const double *prev,
+                     const cap_coef *c, double alfa, double sor, int nsw, cap_report *rep,
+                     int *accepted);
+
 /* The Courant-number block included in the time loop (CourantNo.h:10-81,
  * ltransient branch): su = sum of |face mass fluxes| f1..f3 of each cell on
  * k = 2..nkm, i, j = 3..n?mm (0 elsewhere), conum = {CoNum, meanCoNum}; with

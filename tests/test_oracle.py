@@ -530,3 +530,37 @@ def test_courant_closed_forms(O):
     f1.reshape(nk, ni, nj)[1, 4, 4] = 5.0
     su, co, mean, ts, tm = oracle_courant(O, dims, f1, np.zeros(n), np.zeros(n), vol, dt, 0.0)
     assert co == pytest.approx(0.5 * 5.0 / v * dt)   # cells (k=2, i=5, j=5) and its east neighbour
+
+
+# ------------------------------------------ U-Net oracle (finite differences)
+def test_unet_oracle_gradient_finite_differences():
+    """The U-Net oracle's backward pass against central differences of its
+    forward pass (float64), along random directions and for single weights
+    of every layer."""
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
+    import unet_ref as U
+    rng = np.random.default_rng(5)
+    shape = (8, 8, 8)
+    p = U.init_params(3)
+    for l, (w, b) in enumerate(U.split(p)):
+        b[:] = rng.normal(0, 0.05, b.size)
+    x = rng.normal(size=shape + (3,))
+    t = rng.normal(size=shape)
+    loss, g = U.loss_and_grad(p, x, t)
+    assert g.size == p.size == U.num_params()
+    h = 1e-6
+    for _ in range(3):
+        u = rng.normal(size=p.size)
+        lp, _ = U.loss_and_grad(p + h * u, x, t)
+        lm, _ = U.loss_and_grad(p - h * u, x, t)
+        fd = (lp - lm) / (2 * h)
+        assert fd == pytest.approx(float(g @ u), rel=1e-6)
+    o = 0
+    for l, (ci, co, _, _) in enumerate(U.LAYERS):
+        i = o + int(rng.integers(co * 27 * ci))
+        e = np.zeros(p.size)
+        e[i] = 1.0
+        lp, _ = U.loss_and_grad(p + h * e, x, t)
+        lm, _ = U.loss_and_grad(p - h * e, x, t)
+        assert (lp - lm) / (2 * h) == pytest.approx(g[i], rel=1e-4, abs=1e-9), f"layer {l}"
+        o += co * 27 * ci + co
