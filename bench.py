@@ -51,8 +51,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-iters", type=int, default=24,
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
-    ap.add_argument("--others", default="cgstab_sip,iccg,mixed",
-                    help="comma list of extra solves to time once each")
+    ap.add_argument("--others", default="cgstab_sip,iccg,mixed,unet",
+                    help="comma list of extra solves to time once each (unet: the learned "
+                         "guess's forward pass and training step on 128x96x64)")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: 3 PCG-SIP iterations, then "
                          "a 1 GiB device copy as the byte-counter calibration; prints one line")
@@ -138,6 +139,9 @@ def main():
             r = ctx.iccg(fi, coef, sor=1e-8, nsw=20000)
         elif other == "mixed":
             r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=1e-3)
+        elif other == "unet":
+            solvers["unet"] = _bench_unet(cap, torch)
+            continue
         else:
             continue
         torch.cuda.synchronize()
@@ -191,6 +195,48 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+# (cin, cout, level) of the U-Net's 3x3x3 convolutions, encoder then decoder
+UNET_LAYERS = [(3, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 48, 2), (48, 48, 2),
+               (48, 64, 3), (64, 64, 3), (112, 48, 2), (48, 48, 2), (80, 32, 1), (32, 32, 1),
+               (48, 16, 0), (16, 16, 0), (16, 1, 0)]
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def _bench_unet(cap, torch, nx=128, ny=96, nz=64, reps=5):
+    """Forward pass and training step of the learned guess's U-Net on the
+    128 x 96 x 64 channel box: ms and algorithmic TFLOP/s (2 x 27 x cin x
+    cout multiply-adds per voxel and layer, padding not counted)."""
+    ctx = cap.Context(ny + 4, nx + 4, nz + 4, device_pointers=True)
+    net = cap.UNet(ctx, nx, ny, nz, seed=1)
+    nvox = nx * ny * nz
+    x = torch.randn(3 * nvox, dtype=torch.float32, device="cuda")
+    t = torch.randn(nvox, dtype=torch.float32, device="cuda")
+    out = torch.empty(nvox, dtype=torch.float32, device="cuda")
+    flops = sum(2 * 27 * ci * co * (nvox >> (3 * lv)) for ci, co, lv in UNET_LAYERS)
+    net.forward(x, out)
+    net.train_step(x, t, 1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(x, out)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.train_step(x, t, 1e-4)
+    torch.cuda.synchronize()
+    trn = (time.perf_counter() - t0) * 1e3 / reps
+    net.close()
+    ctx.close()
+    tf = flops / (fwd * 1e-3) / 1e12
+    log(f"[bench] unet {nx}x{ny}x{nz}: forward {fwd:.2f} ms ({tf:.1f} TFLOP/s), "
+        f"train step {trn:.2f} ms")
+    return {"grid": [nx, ny, nz], "forward_ms": round(fwd, 3), "forward_gflop": round(flops / 1e9, 2),
+            "forward_TFLOPs": round(tf, 2), "mfma_frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
+            "train_step_ms": round(trn, 3),
+            "note": "host-side wall time per call including launch and synchronisation"}
 
 
 def _pmc_traffic(kernel):
