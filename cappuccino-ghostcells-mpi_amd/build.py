@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcapamd.so")
-SOURCES = ("cap_solver.hip", "cap_stencil.hip", "cap_unet.hip")
+SOURCES = ("cap_solver.hip", "cap_stencil.hip", "cap_unet.hip", "cap_driver.hip")
 
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

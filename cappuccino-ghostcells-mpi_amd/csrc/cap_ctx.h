@@ -62,3 +62,5 @@ struct cap_ctx {
 };
 
 void cap_free_extra(cap_ctx *ctx);
+// grad_gauss into gradu(3, nxyz) layout (device pointers; queued, not synchronised)
+int cap_grad_gauss_aos(cap_ctx *ctx, const double *u, const double *const *geom, double *g3);

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_grad_gauss(int ni, int nj, int nk,
                                                     const double *__restrict__ u, Geo g,
                                                     double *__restrict__ dx,
                                                     double *__restrict__ dy,
-                                                    double *__restrict__ dz)
+                                                    double *__restrict__ dz, int stride)
 {
     const int nij = ni * nj;
     const size_t n = (size_t)nij * nk;
@@ -90,9 +90,9 @@ __global__ __launch_bounds__(256) void k_grad_gauss(int ni, int nj, int nk,
             sy = sy * volr;
             sz = sz * volr;
         }
-        dx[c] = sx;
-        dy[c] = sy;
-        dz[c] = sz;
+        dx[c * stride] = sx;
+        dy[c * stride] = sy;
+        dz[c * stride] = sz;
     }
 }
 
@@ -884,7 +884,7 @@ extern "C" int cap_grad_gauss(cap_ctx *ctx, const double *u, const double *const
     }
     int blocks = (int)std::min<size_t>((ctx->nstd + 255) / 256, 8192);
     hipLaunchKernelGGL(k_grad_gauss, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj,
-                       ctx->nk, du, g, ox, oy, zbuf);
+                       ctx->nk, du, g, ox, oy, zbuf, 1);
     HIPCHK(hipGetLastError());
     if (!(ctx->flags & CAP_DEVICE_POINTERS)) {
         HIPCHK(hipMemcpyAsync(dudx, ox, ctx->nstd * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -892,6 +892,19 @@ extern "C" int cap_grad_gauss(cap_ctx *ctx, const double *u, const double *const
         HIPCHK(hipMemcpyAsync(dudz, zbuf, ctx->nstd * 8, hipMemcpyDeviceToHost, ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+// grad_gauss into the reference's gradu(3, nxyz) (component fastest), device
+// pointers, no synchronisation
+int cap_grad_gauss_aos(cap_ctx *ctx, const double *u, const double *const *geom, double *g3)
+{
+    Geo g = {geom[0], geom[1], geom[2], geom[3], geom[4], geom[5], geom[6], geom[7], geom[8],
+             geom[9], geom[10], geom[11], geom[12]};
+    int blocks = (int)std::min<size_t>((ctx->nstd + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_grad_gauss, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj,
+                       ctx->nk, u, g, g3, g3 + 1, g3 + 2, 3);
+    HIPCHK(hipGetLastError());
     return CAP_OK;
 }
 

@@ -50,6 +50,47 @@ module cap_hip
     real(c_double) :: uin = 0, vin = 0, win = 0, tein = 0, edin = 0, tin = 0
   end type cap_bc
 
+  ! module state of the pressure-correction drivers (device pointers)
+  type, bind(c) :: cap_flow
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr
+    type(c_ptr) :: xc = c_null_ptr, yc = c_null_ptr, zc = c_null_ptr, vol = c_null_ptr
+    type(c_ptr) :: fx = c_null_ptr, fy = c_null_ptr, fz = c_null_ptr
+    type(c_ptr) :: ar1x = c_null_ptr, ar1y = c_null_ptr, ar1z = c_null_ptr
+    type(c_ptr) :: ar2x = c_null_ptr, ar2y = c_null_ptr, ar2z = c_null_ptr
+    type(c_ptr) :: ar3x = c_null_ptr, ar3y = c_null_ptr, ar3z = c_null_ptr
+    type(c_ptr) :: den = c_null_ptr
+    type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, w = c_null_ptr, p = c_null_ptr, pp = c_null_ptr
+    type(c_ptr) :: f1 = c_null_ptr, f2 = c_null_ptr, f3 = c_null_ptr
+    type(c_ptr) :: ae = c_null_ptr, aw = c_null_ptr, an = c_null_ptr, as = c_null_ptr
+    type(c_ptr) :: at = c_null_ptr, ab = c_null_ptr, ap = c_null_ptr
+    type(c_ptr) :: su = c_null_ptr, sv = c_null_ptr, sw = c_null_ptr, sp = c_null_ptr
+    type(c_ptr) :: apu = c_null_ptr, apv = c_null_ptr, apw = c_null_ptr
+    type(c_ptr) :: res = c_null_ptr, bb = c_null_ptr, bs = c_null_ptr, bw = c_null_ptr
+    type(c_ptr) :: bp = c_null_ptr, bn = c_null_ptr, be = c_null_ptr, bt = c_null_ptr
+    type(c_ptr) :: he = c_null_ptr, hw = c_null_ptr, hn = c_null_ptr, hs = c_null_ptr
+    type(c_ptr) :: ht = c_null_ptr, hb = c_null_ptr
+    type(c_ptr) :: gradu = c_null_ptr, gradv = c_null_ptr, gradw = c_null_ptr, gradp = c_null_ptr
+    type(c_ptr) :: uo = c_null_ptr, vo = c_null_ptr, wo = c_null_ptr
+    type(c_ptr) :: uoo = c_null_ptr, voo = c_null_ptr, woo = c_null_ptr
+    type(c_ptr) :: t = c_null_ptr, te = c_null_ptr, ed = c_null_ptr, vis = c_null_ptr
+    type(c_ptr) :: vart = c_null_ptr, con = c_null_ptr
+    type(c_ptr) :: utt = c_null_ptr, vtt = c_null_ptr, wtt = c_null_ptr
+  end type cap_flow
+
+  ! the driver settings (modules parameters, boundc, time_mod)
+  type, bind(c) :: cap_drv_opts
+    integer(c_int) :: ncorr = 1, npcor = 1, solver = 5, nsw = 1000
+    real(c_double) :: urf = 0.2d0, sor = 1d-8, alfa = 0.92d0
+    integer(c_int) :: ijkpr = 1, const_mflux = 1
+    integer(c_int) :: lout = 0, low = 0, loe = 0, los = 0, lon = 0, lob = 0, lot = 0
+    real(c_double) :: flowin = 0
+    type(c_ptr) :: lbw = c_null_ptr, lbe = c_null_ptr, lbs = c_null_ptr, lbn = c_null_ptr
+    type(c_ptr) :: lbb = c_null_ptr, lbt = c_null_ptr
+    integer(c_int) :: cn = 0, bdf = 0, buoy = 0, boussinesq = 0
+    real(c_double) :: timestep = 0, btime = 0, densit = 1, beta = 0, tref = 0
+    real(c_double) :: gravx = 0, gravy = 0, gravz = 0, cumulative = 0
+  end type cap_drv_opts
+
   type, bind(c) :: cap_report
     real(c_double) :: resor, resl
     integer(c_int) :: iters, status
@@ -231,6 +272,40 @@ module cap_hip
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx, f1, f2, f3
       real(c_double), intent(out) :: sums(2)
+    end function
+    integer(c_int) function cap_calcp(ctx, f, o, reps, cont) bind(c, name='cap_calcp')
+      import :: c_ptr, c_int, c_double, cap_flow, cap_drv_opts, cap_report
+      type(c_ptr), value :: ctx
+      type(cap_flow), intent(in) :: f
+      type(cap_drv_opts), intent(in) :: o
+      type(cap_report), intent(out) :: reps(*)
+      real(c_double), intent(out) :: cont(*)
+    end function
+    integer(c_int) function cap_piso(ctx, f, o, pimple, reps, cont) bind(c, name='cap_piso')
+      import :: c_ptr, c_int, c_double, cap_flow, cap_drv_opts, cap_report
+      type(c_ptr), value :: ctx
+      type(cap_flow), intent(in) :: f
+      type(cap_drv_opts), intent(in) :: o
+      integer(c_int), value :: pimple
+      type(cap_report), intent(out) :: reps(*)
+      real(c_double), intent(out) :: cont(*)
+    end function
+    integer(c_int) function cap_outbc(ctx, f, o) bind(c, name='cap_outbc')
+      import :: c_ptr, c_int, cap_flow, cap_drv_opts
+      type(c_ptr), value :: ctx
+      type(cap_flow), intent(in) :: f
+      type(cap_drv_opts), intent(in) :: o
+    end function
+    integer(c_int) function cap_correct_bc(ctx, f) bind(c, name='cap_correct_bc')
+      import :: c_ptr, c_int, cap_flow
+      type(c_ptr), value :: ctx
+      type(cap_flow), intent(in) :: f
+    end function
+    integer(c_int) function cap_gethbya(ctx, f, o) bind(c, name='cap_gethbya')
+      import :: c_ptr, c_int, cap_flow, cap_drv_opts
+      type(c_ptr), value :: ctx
+      type(cap_flow), intent(in) :: f
+      type(cap_drv_opts), intent(in) :: o
     end function
   end interface
 

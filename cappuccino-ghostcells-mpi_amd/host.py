@@ -38,7 +38,8 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_continuity", "cap_fluxuvw", "cap_update_ghostcells",
            "cap_courant", "cap_unet_create", "cap_unet_destroy", "cap_unet_num_params",
            "cap_unet_get_params", "cap_unet_set_params", "cap_unet_forward",
-           "cap_unet_gradients", "cap_unet_train_step", "cap_unet_guess", "cap_pcgsip_guess")
+           "cap_unet_gradients", "cap_unet_train_step", "cap_unet_guess", "cap_pcgsip_guess",
+           "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -59,6 +60,37 @@ class _Fields(C.Structure):
 class _Bc(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("lbw", "lbe", "lbs", "lbn", "lbb", "lbt")] + \
                [(n, C.c_double) for n in ("uin", "vin", "win", "tein", "edin", "tin")]
+
+
+# module state of the pressure-correction drivers, in the order of cap_flow
+FLOW_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz",
+              "ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z",
+              "den", "u", "v", "w", "p", "pp", "f1", "f2", "f3",
+              "ae", "aw", "an", "as", "at", "ab", "ap", "su", "sv", "sw", "sp", "apu", "apv", "apw",
+              "res", "bb", "bs", "bw", "bp", "bn", "be", "bt",
+              "he", "hw", "hn", "hs", "ht", "hb",
+              "gradu", "gradv", "gradw", "gradp",
+              "uo", "vo", "wo", "uoo", "voo", "woo",
+              "t", "te", "ed", "vis", "vart", "con", "utt", "vtt", "wtt")
+
+
+class _Flow(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in FLOW_NAMES]
+
+
+class DrvOpts(C.Structure):
+    """Settings of the drivers (cap_drv_opts): module parameters, boundc and
+    time_mod values the reference routines read."""
+    _fields_ = [("ncorr", C.c_int), ("npcor", C.c_int), ("solver", C.c_int), ("nsw", C.c_int),
+                ("urf", C.c_double), ("sor", C.c_double), ("alfa", C.c_double),
+                ("ijkpr", C.c_int), ("const_mflux", C.c_int),
+                ("lout", C.c_int), ("low", C.c_int), ("loe", C.c_int), ("los", C.c_int),
+                ("lon", C.c_int), ("lob", C.c_int), ("lot", C.c_int),
+                ("flowin", C.c_double)] + \
+               [(n, C.c_void_p) for n in ("lbw", "lbe", "lbs", "lbn", "lbb", "lbt")] + \
+               [("cn", C.c_int), ("bdf", C.c_int), ("buoy", C.c_int), ("boussinesq", C.c_int)] + \
+               [(n, C.c_double) for n in ("timestep", "btime", "densit", "beta", "tref",
+                                          "gravx", "gravy", "gravz", "cumulative")]
 
 
 class _Report(C.Structure):
@@ -110,6 +142,12 @@ def load():
     L.cap_unet_forward.argtypes = [vp, vp, vp]
     L.cap_unet_gradients.argtypes = [vp, vp, vp, C.POINTER(f), vp]
     L.cap_unet_train_step.argtypes = [vp, vp, vp, f, C.POINTER(f)]
+    Fl, O = C.POINTER(_Flow), C.POINTER(DrvOpts)
+    L.cap_calcp.argtypes = [vp, Fl, O, C.POINTER(_Report), C.POINTER(d)]
+    L.cap_piso.argtypes = [vp, Fl, O, i, C.POINTER(_Report), C.POINTER(d)]
+    L.cap_outbc.argtypes = [vp, Fl, O]
+    L.cap_correct_bc.argtypes = [vp, Fl]
+    L.cap_gethbya.argtypes = [vp, Fl, O]
     L.cap_unet_guess.argtypes = [vp, vp, vp, vp, C.POINTER(_Coef), C.POINTER(d)]
     L.cap_pcgsip_guess.argtypes = [vp, vp, vp, vp, C.POINTER(_Coef), d, d, i, C.POINTER(_Report),
                                    C.POINTER(i)]
@@ -306,6 +344,53 @@ class Context:
         rc = self._lib.cap_continuity(self._h, _ptr(f1), _ptr(f2), _ptr(f3), out)
         self._check(rc, "cap_continuity", allow=(CAP_OK,))
         return list(out)
+
+    # ---------------------------------------------- pressure-correction drivers
+    @staticmethod
+    def _flow(f):
+        """cap_flow from a dict of device tensors (missing names: NULL)."""
+        return _Flow(*[_ptr(f[n]) if f.get(n) is not None else None for n in FLOW_NAMES])
+
+    @staticmethod
+    def _reports(reps, n):
+        return [dict(resor=r.resor, resl=r.resl, iters=r.iters, status=r.status, ms=r.ms,
+                     converged=r.status == CAP_OK) for r in reps[:n]]
+
+    def calcp(self, f, opts):
+        """`call calcp` on device-resident module state f (dict of tensors);
+        returns (solver reports, continuity sums per corrector)."""
+        st = self._flow(f)
+        n = opts.ncorr * opts.npcor
+        reps = (_Report * n)()
+        cont = (C.c_double * (3 * opts.ncorr))()
+        rc = self._lib.cap_calcp(self._h, C.byref(st), C.byref(opts), reps, cont)
+        self._check(rc, "cap_calcp")
+        return self._reports(reps, n), [tuple(cont[3 * q:3 * q + 3]) for q in range(opts.ncorr)]
+
+    def piso(self, f, opts, pimple=False):
+        """`call PISO_multiple_correction` (or PIMPLE_multiple_correction)."""
+        st = self._flow(f)
+        n = opts.ncorr * opts.npcor
+        reps = (_Report * n)()
+        cont = (C.c_double * (3 * opts.ncorr))()
+        rc = self._lib.cap_piso(self._h, C.byref(st), C.byref(opts), 1 if pimple else 0, reps, cont)
+        self._check(rc, "cap_piso")
+        return self._reports(reps, n), [tuple(cont[3 * q:3 * q + 3]) for q in range(opts.ncorr)]
+
+    def outbc(self, f, opts):
+        st = self._flow(f)
+        self._check(self._lib.cap_outbc(self._h, C.byref(st), C.byref(opts)), "cap_outbc",
+                    allow=(CAP_OK,))
+
+    def correct_bc(self, f):
+        st = self._flow(f)
+        self._check(self._lib.cap_correct_bc(self._h, C.byref(st)), "cap_correct_bc",
+                    allow=(CAP_OK,))
+
+    def gethbya(self, f, opts):
+        st = self._flow(f)
+        self._check(self._lib.cap_gethbya(self._h, C.byref(st), C.byref(opts)), "cap_gethbya",
+                    allow=(CAP_OK,))
 
     def build_lognormal(self, coef, seed, sigma=1.0):
         st = coef._struct()

@@ -142,6 +142,63 @@ int cap_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *f, int idew, int id
 int cap_fluxmc(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
                const double *fif, double *fmcor);
 
+/* ---- Pressure-correction drivers ---------------------------------------
+ * Module state of calcp / PISO / PIMPLE (modules geometry, variables, coef,
+ * coefb, hcoef, gradients of modules_allocatable.f90), all device (HBM)
+ * pointers of a CAP_DEVICE_POINTERS context, reference layout.  Gradients are
+ * gradu(3, nxyz), component fastest.  uo..woo are read only by
+ * PISO_getHbyA's BDF / Crank-Nicolson terms and t by its buoyancy term; outbc
+ * also copies t, te, ed, vis, vart, con, utt, vtt, wtt into the outlet cells
+ * (NULL: not copied). */
+typedef struct {
+    const double *x, *y, *z, *xc, *yc, *zc, *vol, *fx, *fy, *fz;
+    const double *ar1x, *ar1y, *ar1z, *ar2x, *ar2y, *ar2z, *ar3x, *ar3y, *ar3z;
+    const double *den;
+    double *u, *v, *w, *p, *pp, *f1, *f2, *f3;
+    double *ae, *aw, *an, *as, *at, *ab, *ap, *su, *sv, *sw, *sp, *apu, *apv, *apw;
+    double *res, *bb, *bs, *bw, *bp, *bn, *be, *bt;
+    double *he, *hw, *hn, *hs, *ht, *hb;
+    double *gradu, *gradv, *gradw, *gradp;
+    const double *uo, *vo, *wo, *uoo, *voo, *woo;
+    double *t, *te, *ed, *vis, *vart, *con, *utt, *vtt, *wtt;
+} cap_flow;
+
+/* Settings the drivers read from modules parameters, boundc and time_mod.
+ * solver is calcp's isol: 1 sipsol, 3 iccg, 4 pcgsip, 5 cgstab_sip (the
+ * reference's setting); PISO and PIMPLE always use pcgsip.  lbw..lbt are
+ * device arrays indexed as in cap_bc (outbc reads them when lout is set). */
+typedef struct {
+    int ncorr, npcor, solver, nsw;
+    double urf, sor, alfa;
+    int ijkpr;                       /* 1-based reference-pressure cell */
+    int const_mflux;                 /* outbc is skipped when set */
+    int lout, low, loe, los, lon, lob, lot;
+    double flowin;
+    const int *lbw, *lbe, *lbs, *lbn, *lbb, *lbt;
+    int cn, bdf, buoy, boussinesq;
+    double timestep, btime, densit, beta, tref, gravx, gravy, gravz;
+    double cumulative;               /* cumulativeContErr on entry */
+} cap_drv_opts;
+
+/* `call calcp` (calcp-multiple_correction_SIMPLE.f90:3-238, Gauss
+ * gradients): every step on the device.  reps: ncorr * npcor solver reports;
+ * cont: per corrector {sumLocalContErr, globalContErr, cumulativeContErr}. */
+int cap_calcp(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, cap_report *reps, double *cont);
+/* `call PISO_multiple_correction` (PISO_multiple_correction.f90:3-297, with
+ * PISO_getHbyA.f90:3-233) for pimple = 0, `call PIMPLE_multiple_correction`
+ * (PIMPLE_multiple_correction.f90:3-302) for pimple = 1.  Reports as cap_calcp. */
+int cap_piso(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple, cap_report *reps,
+             double *cont);
+/* `call outbc` (outbc.f90:3-318): outlet face fluxes, then velocities and
+ * fluxes of the outlet cells scaled by flowin / flow.  Returns at once
+ * unless o->lout. */
+int cap_outbc(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o);
+/* `call correctBoundaryConditions` (correctBoundaryConditions.f90:1-86):
+ * periodic copies of u, v, w (and f1, f2 on the west / south planes). */
+int cap_correct_bc(cap_ctx *ctx, cap_flow *f);
+/* `call PISO_getHbyA` (PISO_getHbyA.f90:3-233). */
+int cap_gethbya(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o);
+
 /* ---- Learned initial guess (no counterpart in the reference) ----------
  * A 3-D U-Net (4 levels, 16/32/48/64 channels, 3x3x3 convolutions, BF16
  * weights and activations, FP32 accumulation on the matrix cores) that maps

@@ -1256,3 +1256,399 @@ void orc_courant(int ni, int nj, int nk, const double *f1, const double *f2, con
     out[0] = conum;
     out[1] = meanconum;
 }
+
+/* ================================================ pressure-correction drivers */
+
+static orc_fields drv_fields(const orc_flow *F)
+{
+    orc_fields f = {F->x, F->y, F->z, F->xc, F->yc, F->zc, F->vol, F->fx, F->fy, F->fz,
+                    F->ar1x, F->ar1y, F->ar1z, F->ar2x, F->ar2y, F->ar2z, F->ar3x, F->ar3y,
+                    F->ar3z, F->den, F->apu, F->apv, F->apw, F->u, F->v, F->w, F->p,
+                    F->gradu, F->gradv, F->gradw, F->gradp};
+    return f;
+}
+
+/* `call grad_gauss(u, gradu(1,:), gradu(2,:), gradu(3,:))`: the strided
+ * sections are copied in, zeroed and filled by the callee, copied out */
+static void drv_grad(int ni, int nj, int nk, const orc_flow *F, const double *u, double *g3)
+{
+    const size_t n = (size_t)ni * nj * nk;
+    double *d = (double *)malloc(3 * n * sizeof(double));
+    orc_grad_gauss(ni, nj, nk, u, F->fx, F->fy, F->fz, F->ar1x, F->ar1y, F->ar1z, F->ar2x,
+                   F->ar2y, F->ar2z, F->ar3x, F->ar3y, F->ar3z, F->vol, d, d + n, d + 2 * n);
+    for (size_t c = 0; c < n; ++c) {
+        g3[3 * c] = d[c];
+        g3[3 * c + 1] = d[n + c];
+        g3[3 * c + 2] = d[2 * n + c];
+    }
+    free(d);
+}
+
+static orc_sys drv_sys(orc_flow *F)
+{
+    orc_sys s = {F->ae, F->aw, F->an, F->as, F->at, F->ab, F->ap, F->su, F->res,
+                 F->bb, F->bs, F->bw, F->bp, F->bn, F->be, F->bt};
+    return s;
+}
+
+/* continuityErrors.h:55-87 */
+static void drv_continuity(int ni, int nj, int nk, const orc_flow *F, double *cum, double *out)
+{
+    double s[2];
+    orc_continuity(ni, nj, nk, F->f1, F->f2, F->f3, s);
+    *cum = *cum + s[0];
+    out[0] = s[1];
+    out[1] = s[0];
+    out[2] = *cum;
+}
+
+/* outbc.f90:3-318.  The outflow face fluxes use ar1x/ar1y/ar1z on every
+ * face, as the reference does; flow is the running sum in face order. */
+void orc_outbc(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o)
+{
+    if (!o->lout) return;
+    const int nij = ni * nj, nim = ni - 1, njm = nj - 1, nkm = nk - 1;
+    psum fl = PS0;
+    double flow = 0.0;
+#define OUTF(fa, cface, cin)                                                                    \
+    fa[cface] = F->den[inp] * (F->ar1x[inp] * F->u[cin] + F->ar1y[inp] * F->v[cin] +              \
+                               F->ar1z[inp] * F->w[cin])
+    /* flux pass: west, east, south, north, bottom, top (40-118) */
+    if (o->low)
+        for (int k = 2; k <= nkm; ++k) for (int j = 2; j <= njm; ++j) {
+            size_t inp = IX(2, j, k); int inbc = (j - 1) * nk + k;
+            if (o->lbw[inbc - 1] == 2) { size_t inh = inp + nj; OUTF(F->f1, inp, inh); flow = ps_add(&fl, -F->f1[inp], 0); }
+        }
+    if (o->loe)
+        for (int k = 2; k <= nkm; ++k) for (int j = 2; j <= njm; ++j) {
+            size_t inp = IX(nim, j, k); int inbc = (j - 1) * nk + k;
+            if (o->lbe[inbc - 1] == 2) { size_t inh = inp - nj; OUTF(F->f1, inh, inh); flow = ps_add(&fl, F->f1[inh], 0); }
+        }
+    if (o->los)
+        for (int k = 2; k <= nkm; ++k) for (int i = 2; i <= nim; ++i) {
+            size_t inp = IX(i, 2, k); int inbc = (i - 1) * nk + k;
+            if (o->lbs[inbc - 1] == 2) { size_t inh = inp + 1; OUTF(F->f2, inp, inh); flow = ps_add(&fl, -F->f2[inp], 0); }
+        }
+    if (o->lon)
+        for (int k = 2; k <= nkm; ++k) for (int i = 2; i <= nim; ++i) {
+            size_t inp = IX(i, njm, k); int inbc = (i - 1) * nk + k;
+            if (o->lbn[inbc - 1] == 2) { size_t inh = inp - 1; OUTF(F->f2, inh, inh); flow = ps_add(&fl, F->f2[inh], 0); }
+        }
+    if (o->lob)
+        for (int i = 2; i <= nim; ++i) for (int j = 2; j <= njm; ++j) {
+            size_t inp = IX(i, j, 2); int inbc = (i - 1) * nj + j;
+            if (o->lbb[inbc - 1] == 2) { size_t inh = inp + nij; OUTF(F->f3, inp, inh); flow = ps_add(&fl, -F->f3[inp], 0); }
+        }
+    if (o->lot)
+        for (int i = 2; i <= nim; ++i) for (int j = 2; j <= njm; ++j) {
+            size_t inp = IX(i, j, nkm); int inbc = (i - 1) * nj + j;
+            if (o->lbt[inbc - 1] == 2) { size_t inh = inp - nij; OUTF(F->f3, inh, inh); flow = ps_add(&fl, F->f3[inh], 0); }
+        }
+#undef OUTF
+    /* the running sum starts from flow = 0. (a REAL*4 zero, exact) */
+    const double fac = o->flowin / (flow + SMALL);
+#define COPYS(inp, inh)                                                                         \
+    do {                                                                                        \
+        if (F->te) F->te[inp] = F->te[inh];                                                     \
+        if (F->ed) F->ed[inp] = F->ed[inh];                                                     \
+        if (F->vis) F->vis[inp] = F->vis[inh];                                                  \
+        if (F->t) F->t[inp] = F->t[inh];                                                        \
+        if (F->vart) F->vart[inp] = F->vart[inh];                                               \
+        if (F->con) F->con[inp] = F->con[inh];                                                  \
+        if (F->utt) F->utt[inp] = F->utt[inh];                                                  \
+        if (F->vtt) F->vtt[inp] = F->vtt[inh];                                                  \
+        if (F->wtt) F->wtt[inp] = F->wtt[inh];                                                  \
+    } while (0)
+#define SCALE(inp, inh)                                                                         \
+    do {                                                                                        \
+        F->u[inp] = F->u[inh] * fac;                                                            \
+        F->v[inp] = F->v[inh] * fac;                                                            \
+        F->w[inp] = F->w[inh] * fac;                                                            \
+    } while (0)
+    /* scaling pass (127-318) */
+    if (o->low)
+        for (int k = 2; k <= nkm; ++k) for (int j = 2; j <= njm; ++j) {
+            size_t inp = IX(2, j, k); int inbc = (j - 1) * nk + k;
+            if (o->lbw[inbc - 1] == 2) { size_t inh = inp + nj; SCALE(inp, inh); F->f1[inp] = F->f1[inp] * fac; COPYS(inp, inh); }
+        }
+    if (o->loe)
+        for (int k = 2; k <= nkm; ++k) for (int j = 2; j <= njm; ++j) {
+            size_t inp = IX(nim, j, k); int inbc = (j - 1) * nk + k;
+            if (o->lbe[inbc - 1] == 2) { size_t inh = inp - nj; SCALE(inp, inh); F->f1[inh] = F->f1[inh] * fac; COPYS(inp, inh); }
+        }
+    if (o->los)
+        for (int k = 2; k <= nkm; ++k) for (int i = 2; i <= nim; ++i) {
+            size_t inp = IX(i, 2, k); int inbc = (i - 1) * nk + k;
+            if (o->lbs[inbc - 1] == 2) { size_t inh = inp + 1; SCALE(inp, inh); F->f2[inp] = F->f2[inp] * fac; COPYS(inp, inh); }
+        }
+    if (o->lon)
+        for (int k = 2; k <= nkm; ++k) for (int i = 2; i <= nim; ++i) {
+            size_t inp = IX(i, njm, k); int inbc = (i - 1) * nk + k;
+            if (o->lbn[inbc - 1] == 2) { size_t inh = inp - 1; SCALE(inp, inh); F->f2[inh] = F->f2[inh] * fac; COPYS(inp, inh); }
+        }
+    if (o->lob)
+        for (int i = 2; i <= nim; ++i) for (int j = 2; j <= njm; ++j) {
+            size_t inp = IX(i, j, 2); int inbc = (i - 1) * nj + j;
+            if (o->lbb[inbc - 1] == 2) { size_t inh = inp + nij; SCALE(inp, inh); F->f3[inp] = F->f3[inp] * fac; COPYS(inp, inh); }
+        }
+    if (o->lot)
+        for (int i = 2; i <= nim; ++i) for (int j = 2; j <= njm; ++j) {
+            size_t inp = IX(i, j, nkm); int inbc = (i - 1) * nj + j;
+            if (o->lbt[inbc - 1] == 2) { size_t inh = inp - nij; SCALE(inp, inh); F->f3[inh] = F->f3[inh] * fac; COPYS(inp, inh); }
+        }
+#undef COPYS
+#undef SCALE
+}
+
+/* correctBoundaryConditions.f90:1-86: periodic copies into the padding
+ * planes, in the reference's order (far east, far west -- whose source plane
+ * is li(njm), as written -- north, south) */
+void orc_correct_bc(int ni, int nj, int nk, orc_flow *F)
+{
+    const int nij = ni * nj, nim = ni - 1, njm = nj - 1, nkm = nk - 1;
+    (void)nij;
+#define CP3(inp, inh) do { F->u[inp] = F->u[inh]; F->v[inp] = F->v[inh]; F->w[inp] = F->w[inh]; } while (0)
+    for (int k = 2; k <= nkm; ++k) for (int j = 2; j <= njm; ++j) {
+        size_t inp = IX(ni, j, k), inh = IX(2, j, k);
+        CP3(inp, inh);
+    }
+    for (int k = 2; k <= nkm; ++k) for (int j = 2; j <= njm; ++j) {
+        size_t inp = IX(1, j, k), inh = IX(njm, j, k);
+        CP3(inp, inh);
+        F->f1[inp] = F->f1[inh];
+    }
+    for (int k = 2; k <= nkm; ++k) for (int i = 2; i <= nim; ++i) {
+        size_t inp = IX(i, nj, k), inh = IX(i, 2, k);
+        CP3(inp, inh);
+    }
+    for (int k = 2; k <= nkm; ++k) for (int i = 2; i <= nim; ++i) {
+        size_t inp = IX(i, 1, k), inh = IX(i, njm, k);
+        CP3(inp, inh);
+        F->f2[inp] = F->f2[inh];
+    }
+#undef CP3
+}
+
+/* PISO_getHbyA.f90:3-233: su, sv, sw = b + sum h_nb u_nb on k, i, j =
+ * 3..n?mm (buoyancy 54-73, BDF 76-91, Crank-Nicolson 109-123 / 144-158 /
+ * 178-192), then u = su * apu (and v, w) (203-213), then the velocity
+ * gradients (Gauss branch, 222-226) */
+void orc_gethbya(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        size_t inp = IX(i, j, k);
+        F->sp[inp] = 0.0;
+        F->su[inp] = 0.0;
+        F->sv[inp] = 0.0;
+        F->sw[inp] = 0.0;
+        if (o->buoy) {
+            double heat = 0.0;
+            if (o->boussinesq) heat = o->beta * o->densit * (F->t[inp] - o->tref) * F->vol[inp];
+            else heat = (o->densit - F->den[inp]) * F->vol[inp];
+            F->su[inp] = F->su[inp] - o->gravx * heat;
+            F->sv[inp] = F->sv[inp] - o->gravy * heat;
+            F->sw[inp] = F->sw[inp] - o->gravz * heat;
+        }
+        if (o->bdf) {
+            const double apotime = F->den[inp] * F->vol[inp] / o->timestep;
+            const double sut = apotime * ((1.0 + o->btime) * F->uo[inp] - 0.5 * o->btime * F->uoo[inp]);
+            const double svt = apotime * ((1.0 + o->btime) * F->vo[inp] - 0.5 * o->btime * F->voo[inp]);
+            const double swt = apotime * ((1.0 + o->btime) * F->wo[inp] - 0.5 * o->btime * F->woo[inp]);
+            F->su[inp] = F->su[inp] + sut;
+            F->sv[inp] = F->sv[inp] + svt;
+            F->sw[inp] = F->sw[inp] + swt;
+        }
+    }
+    double *S[3] = {F->su, F->sv, F->sw};
+    const double *X[3] = {F->u, F->v, F->w}, *XO[3] = {F->uo, F->vo, F->wo};
+    for (int q = 0; q < 3; ++q)
+        for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+            size_t inp = IX(i, j, k), ine = inp + nj, inw = inp - nj, inn = inp + 1, ins = inp - 1,
+                   intp = inp + nij, inb = inp - nij;
+            const double *x = X[q];
+            if (o->cn) {
+                const double *xo = XO[q];
+                const double apotime = F->den[inp] * F->vol[inp] / o->timestep;
+                S[q][inp] = S[q][inp] + (F->he[inp] * xo[ine] + F->hw[inp] * xo[inw] +
+                                         F->hn[inp] * xo[inn] + F->hs[inp] * xo[ins] +
+                                         F->ht[inp] * xo[intp] + F->hb[inp] * xo[inb]) +
+                            (apotime - F->he[inp] - F->hw[inp] - F->hn[inp] - F->hs[inp] -
+                             F->ht[inp] - F->hb[inp]) * xo[inp];
+            }
+            S[q][inp] = S[q][inp] + (F->he[inp] * x[ine] + F->hw[inp] * x[inw] +
+                                     F->hn[inp] * x[inn] + F->hs[inp] * x[ins] +
+                                     F->ht[inp] * x[intp] + F->hb[inp] * x[inb]);
+        }
+    for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        size_t inp = IX(i, j, k);
+        F->u[inp] = F->su[inp] * F->apu[inp];
+        F->v[inp] = F->sv[inp] * F->apv[inp];
+        F->w[inp] = F->sw[inp] * F->apw[inp];
+    }
+    drv_grad(ni, nj, nk, F, F->u, F->gradu);
+    drv_grad(ni, nj, nk, F, F->v, F->gradv);
+    drv_grad(ni, nj, nk, F, F->w, F->gradw);
+}
+
+static void drv_solve(int ni, int nj, int nk, orc_flow *F, double *x, int solver, double alfa,
+                      double sor, int nsw, orc_report *rep)
+{
+    orc_sys s = drv_sys(F);
+    switch (solver) {
+    case 1: orc_sipsol(ni, nj, nk, x, &s, alfa, sor, nsw, rep); break;
+    case 3: orc_iccg(ni, nj, nk, x, &s, sor, nsw, rep); break;
+    case 4: orc_pcgsip(ni, nj, nk, x, &s, alfa, sor, nsw, rep); break;
+    default: orc_cgstab_sip(ni, nj, nk, x, &s, alfa, sor, nsw, rep); break;
+    }
+}
+
+/* calcp-multiple_correction_SIMPLE.f90:3-238 with the Gauss gradients
+ * (gauss branch of 38-46 and 112-116) */
+void orc_calcp(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o, orc_report *reps,
+               double *cont)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const size_t n = (size_t)nij * nk;
+    double cum = o->cumulative;
+    double *fm = (double *)malloc(3 * n * sizeof(double));
+    for (int icorr = 1; icorr <= o->ncorr; ++icorr) {
+        drv_grad(ni, nj, nk, F, F->u, F->gradu);
+        drv_grad(ni, nj, nk, F, F->v, F->gradv);
+        drv_grad(ni, nj, nk, F, F->w, F->gradw);
+        orc_fields f = drv_fields(F);
+        orc_fluxmass(ni, nj, nk, &f, nj, 1, nij, F->ar1x, F->ar1y, F->ar1z, F->fx, F->ae, F->aw, F->f1);
+        orc_fluxmass(ni, nj, nk, &f, 1, nij, nj, F->ar2x, F->ar2y, F->ar2z, F->fy, F->an, F->as, F->f2);
+        orc_fluxmass(ni, nj, nk, &f, nij, nj, 1, F->ar3x, F->ar3y, F->ar3z, F->fz, F->at, F->ab, F->f3);
+        if (!o->const_mflux) orc_outbc(ni, nj, nk, F, o);
+        for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+            size_t inp = IX(i, j, k);
+            F->su[inp] = -F->f1[inp] + F->f1[inp - nj] - F->f2[inp] + F->f2[inp - 1] - F->f3[inp] +
+                         F->f3[inp - nij];
+            F->ap[inp] = F->ae[inp] + F->aw[inp] + F->an[inp] + F->as[inp] + F->at[inp] + F->ab[inp];
+        }
+        for (int ipcorr = 1; ipcorr <= o->npcor; ++ipcorr) {
+            memset(F->pp, 0, n * sizeof(double));
+            drv_solve(ni, nj, nk, F, F->pp, o->solver, o->alfa, o->sor, o->nsw,
+                      &reps[(icorr - 1) * o->npcor + ipcorr - 1]);
+            const double ppref = F->pp[o->ijkpr - 1];
+            drv_grad(ni, nj, nk, F, F->pp, F->gradp);
+            if (ipcorr != o->npcor) {
+                memset(F->su, 0, n * sizeof(double));
+                f = drv_fields(F);
+                orc_fluxmc_all(ni, nj, nk, &f, nj, 1, nij, F->fx, fm);
+                orc_fluxmc_all(ni, nj, nk, &f, 1, nij, nj, F->fy, fm + n);
+                orc_fluxmc_all(ni, nj, nk, &f, nij, nj, 1, F->fz, fm + 2 * n);
+                for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+                    size_t inp = IX(i, j, k);
+                    double fmcor = fm[inp];
+                    F->f1[inp] = F->f1[inp] - fmcor;
+                    F->su[inp] = F->su[inp] + fmcor;
+                    F->su[inp + nj] = F->su[inp + nj] - fmcor;
+                    fmcor = fm[n + inp];
+                    F->f2[inp] = F->f2[inp] - fmcor;
+                    F->su[inp] = F->su[inp] + fmcor;
+                    F->su[inp + 1] = F->su[inp + 1] - fmcor;
+                    fmcor = fm[2 * n + inp];
+                    F->f3[inp] = F->f3[inp] - fmcor;
+                    F->su[inp] = F->su[inp] + fmcor;
+                    F->su[inp + nij] = F->su[inp + nij] - fmcor;
+                }
+            }
+            for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+                size_t inp = IX(i, j, k);
+                F->f1[inp] = F->f1[inp] - F->ae[inp] * (F->pp[inp + nj] - F->pp[inp]);
+                F->f2[inp] = F->f2[inp] - F->an[inp] * (F->pp[inp + 1] - F->pp[inp]);
+                F->f3[inp] = F->f3[inp] - F->at[inp] * (F->pp[inp + nij] - F->pp[inp]);
+            }
+            for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+                size_t inp = IX(i, j, k);
+                F->u[inp] = F->u[inp] - F->vol[inp] * F->apu[inp] * F->gradp[3 * inp];
+                F->v[inp] = F->v[inp] - F->vol[inp] * F->apv[inp] * F->gradp[3 * inp + 1];
+                F->w[inp] = F->w[inp] - F->vol[inp] * F->apw[inp] * F->gradp[3 * inp + 2];
+                F->p[inp] = F->p[inp] + o->urf * (F->pp[inp] - ppref);
+            }
+        }
+        drv_continuity(ni, nj, nk, F, &cum, cont + 3 * (icorr - 1));
+        orc_correct_bc(ni, nj, nk, F);
+    }
+    orc_bpres(ni, nj, nk, F->p, F->fx, F->fy, F->fz);
+    memcpy(F->pp, F->p, n * sizeof(double));
+    free(fm);
+}
+
+/* PISO_multiple_correction.f90:3-297 (pimple = 0) and
+ * PIMPLE_multiple_correction.f90:3-302 (pimple = 1), Gauss gradients */
+void orc_piso(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o, int pimple,
+              orc_report *reps, double *cont)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const size_t n = (size_t)nij * nk;
+    double cum = o->cumulative;
+    memcpy(F->he, F->ae, n * sizeof(double));
+    memcpy(F->hw, F->aw, n * sizeof(double));
+    memcpy(F->hn, F->an, n * sizeof(double));
+    memcpy(F->hs, F->as, n * sizeof(double));
+    memcpy(F->ht, F->at, n * sizeof(double));
+    memcpy(F->hb, F->ab, n * sizeof(double));
+    memset(F->gradp, 0, 3 * n * sizeof(double));
+    memset(F->p, 0, n * sizeof(double));
+    double *x = pimple ? F->pp : F->p;     /* the variable solved for */
+    const size_t inpr = (size_t)o->ijkpr - 1;
+    for (int icorr = 1; icorr <= o->ncorr; ++icorr) {
+        if (!pimple) orc_gethbya(ni, nj, nk, F, o);
+        for (int ipcorr = 1; ipcorr <= o->npcor; ++ipcorr) {
+            orc_fields f = drv_fields(F);
+            orc_piso_assemble(ni, nj, nk, &f, nj, 1, nij, F->ar1x, F->ar1y, F->ar1z, F->fx, F->ae,
+                              F->aw, F->f1, ipcorr, F->be, F->su);
+            orc_piso_assemble(ni, nj, nk, &f, 1, nij, nj, F->ar2x, F->ar2y, F->ar2z, F->fy, F->an,
+                              F->as, F->f2, ipcorr, F->bn, F->su);
+            orc_piso_assemble(ni, nj, nk, &f, nij, nj, 1, F->ar3x, F->ar3y, F->ar3z, F->fz, F->at,
+                              F->ab, F->f3, ipcorr, F->bt, F->su);
+            if (ipcorr == 1) {
+                if (!o->const_mflux) orc_outbc(ni, nj, nk, F, o);
+                for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+                    size_t inp = IX(i, j, k);
+                    F->su[inp] = -F->f1[inp] + F->f1[inp - nj] - F->f2[inp] + F->f2[inp - 1] -
+                                 F->f3[inp] + F->f3[inp - nij];
+                }
+            }
+            for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+                size_t inp = IX(i, j, k);
+                F->ap[inp] = F->ae[inp] + F->aw[inp] + F->an[inp] + F->as[inp] + F->at[inp] + F->ab[inp];
+            }
+            F->ap[inpr] = 1.0;
+            F->ae[inpr] = 0.0;
+            F->aw[inpr] = 0.0;
+            F->an[inpr] = 0.0;
+            F->as[inpr] = 0.0;
+            F->at[inpr] = 0.0;
+            F->ab[inpr] = 0.0;
+            const double ppref = F->p[inpr];
+            F->su[inpr] = ppref;
+            /* sor(ip) = 0.1 is a default-real literal */
+            const double sor = icorr != o->ncorr ? (double)0.1f : o->sor;
+            drv_solve(ni, nj, nk, F, x, 4, o->alfa, sor, o->nsw,
+                      &reps[(icorr - 1) * o->npcor + ipcorr - 1]);
+            orc_bpres(ni, nj, nk, x, F->fx, F->fy, F->fz);
+            drv_grad(ni, nj, nk, F, x, F->gradp);
+            if (ipcorr == o->npcor)
+                for (int k = 2; k <= nkmm; ++k) for (int i = 2; i <= nimm; ++i) for (int j = 2; j <= njmm; ++j) {
+                    size_t inp = IX(i, j, k);
+                    F->f1[inp] = F->f1[inp] - F->ae[inp] * (x[inp + nj] - x[inp]);
+                    F->f2[inp] = F->f2[inp] - F->an[inp] * (x[inp + 1] - x[inp]);
+                    F->f3[inp] = F->f3[inp] - F->at[inp] * (x[inp + nij] - x[inp]);
+                }
+        }
+        drv_continuity(ni, nj, nk, F, &cum, cont + 3 * (icorr - 1));
+        for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+            size_t inp = IX(i, j, k);
+            F->u[inp] = F->u[inp] - F->apu[inp] * F->gradp[3 * inp] * F->vol[inp];
+            F->v[inp] = F->v[inp] - F->apv[inp] * F->gradp[3 * inp + 1] * F->vol[inp];
+            F->w[inp] = F->w[inp] - F->apw[inp] * F->gradp[3 * inp + 2] * F->vol[inp];
+            if (pimple) F->p[inp] = F->p[inp] + o->urf * (F->pp[inp] - F->p[inp]);
+        }
+        if (pimple) orc_bpres(ni, nj, nk, F->p, F->fx, F->fy, F->fz);
+    }
+    memcpy(F->pp, F->p, n * sizeof(double));
+}

@@ -144,3 +144,53 @@ void orc_ghostcells(int ni, int nj, int nk, double *const *var, const double *co
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Module state of the pressure-correction drivers (modules geometry,
+ * variables, coef, coefb, hcoef, gradients of modules_allocatable.f90).
+ * Gradients are gradu(3, nxyz), component fastest.  uo..woo and t are read
+ * only by PISO_getHbyA's optional terms; t..wtt are copied by outbc (NULL:
+ * not copied). */
+typedef struct {
+    const double *x, *y, *z, *xc, *yc, *zc, *vol, *fx, *fy, *fz;
+    const double *ar1x, *ar1y, *ar1z, *ar2x, *ar2y, *ar2z, *ar3x, *ar3y, *ar3z;
+    const double *den;
+    double *u, *v, *w, *p, *pp, *f1, *f2, *f3;
+    double *ae, *aw, *an, *as, *at, *ab, *ap, *su, *sv, *sw, *sp, *apu, *apv, *apw;
+    double *res, *bb, *bs, *bw, *bp, *bn, *be, *bt;
+    double *he, *hw, *hn, *hs, *ht, *hb;
+    double *gradu, *gradv, *gradw, *gradp;
+    const double *uo, *vo, *wo, *uoo, *voo, *woo;
+    double *t, *te, *ed, *vis, *vart, *con, *utt, *vtt, *wtt;
+} orc_flow;
+
+/* Settings the drivers read from module parameters / boundc / time_mod.
+ * solver: calcp's isol (1 sipsol, 3 iccg, 4 pcgsip, 5 cgstab_sip). */
+typedef struct {
+    int ncorr, npcor, solver, nsw;
+    double urf, sor, alfa;
+    int ijkpr;                       /* 1-based reference-pressure cell */
+    int const_mflux;                 /* outbc skipped when set */
+    int lout, low, loe, los, lon, lob, lot;
+    double flowin;
+    const int *lbw, *lbe, *lbs, *lbn, *lbb, *lbt;
+    int cn, bdf, buoy, boussinesq;
+    double timestep, btime, densit, beta, tref, gravx, gravy, gravz;
+    double cumulative;               /* cumulativeContErr on entry */
+} orc_drv_opts;
+
+void orc_outbc(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o);
+void orc_correct_bc(int ni, int nj, int nk, orc_flow *F);
+void orc_gethbya(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o);
+/* reps: ncorr * npcor solver reports; cont: per corrector {sumLocalContErr,
+ * globalContErr, cumulativeContErr} */
+void orc_calcp(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o, orc_report *reps,
+               double *cont);
+/* pimple = 0: PISO_multiple_correction, 1: PIMPLE_multiple_correction */
+void orc_piso(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o, int pimple,
+              orc_report *reps, double *cont);
+#ifdef __cplusplus
+}
+#endif

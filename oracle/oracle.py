@@ -40,6 +40,36 @@ class _Report(C.Structure):
                 ("iters", C.c_int), ("converged", C.c_int)]
 
 
+# module state of the drivers, in the order of orc_flow
+FLOW_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz",
+              "ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z",
+              "den", "u", "v", "w", "p", "pp", "f1", "f2", "f3",
+              "ae", "aw", "an", "as", "at", "ab", "ap", "su", "sv", "sw", "sp", "apu", "apv", "apw",
+              "res", "bb", "bs", "bw", "bp", "bn", "be", "bt",
+              "he", "hw", "hn", "hs", "ht", "hb",
+              "gradu", "gradv", "gradw", "gradp",
+              "uo", "vo", "wo", "uoo", "voo", "woo",
+              "t", "te", "ed", "vis", "vart", "con", "utt", "vtt", "wtt")
+
+
+class _Flow(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in FLOW_NAMES]
+
+
+class DrvOpts(C.Structure):
+    """orc_drv_opts: the module settings the drivers read."""
+    _fields_ = [("ncorr", C.c_int), ("npcor", C.c_int), ("solver", C.c_int), ("nsw", C.c_int),
+                ("urf", C.c_double), ("sor", C.c_double), ("alfa", C.c_double),
+                ("ijkpr", C.c_int), ("const_mflux", C.c_int),
+                ("lout", C.c_int), ("low", C.c_int), ("loe", C.c_int), ("los", C.c_int),
+                ("lon", C.c_int), ("lob", C.c_int), ("lot", C.c_int),
+                ("flowin", C.c_double)] + \
+               [(n, C.c_void_p) for n in ("lbw", "lbe", "lbs", "lbn", "lbb", "lbt")] + \
+               [("cn", C.c_int), ("bdf", C.c_int), ("buoy", C.c_int), ("boussinesq", C.c_int)] + \
+               [(n, C.c_double) for n in ("timestep", "btime", "densit", "beta", "tref",
+                                          "gravx", "gravy", "gravz", "cumulative")]
+
+
 _lib = None
 
 
@@ -75,6 +105,12 @@ def lib():
         L.orc_pcgsip_time_iters.argtypes = [i, i, i, dp, C.POINTER(_Sys), d, i]
         L.orc_pcgsip_time_iters.restype = d
         L.orc_set_plane_sums.argtypes = [i]
+        Fl, Op = C.POINTER(_Flow), C.POINTER(DrvOpts)
+        L.orc_calcp.argtypes = [i, i, i, Fl, Op, C.POINTER(_Report), dp]
+        L.orc_piso.argtypes = [i, i, i, Fl, Op, i, C.POINTER(_Report), dp]
+        L.orc_outbc.argtypes = [i, i, i, Fl, Op]
+        L.orc_correct_bc.argtypes = [i, i, i, Fl]
+        L.orc_gethbya.argtypes = [i, i, i, Fl, Op]
         _lib = L
     return _lib
 
@@ -232,6 +268,59 @@ def rng_u01(seed, stream, idx):
 
 def rng_normal(seed, stream, idx):
     return lib().orc_normal(seed, stream, idx)
+
+
+# ------------------------------------------------- pressure-correction drivers --
+def _flow_struct(f):
+    ptrs = []
+    for name in FLOW_NAMES:
+        a = f.get(name)
+        if a is None:
+            ptrs.append(None)
+        else:
+            assert a.dtype == np.float64 and a.flags.c_contiguous, name
+            ptrs.append(a.ctypes.data)
+    return _Flow(*ptrs)
+
+
+def _reports(reps, n):
+    return [dict(resor=r.resor, resl=r.resl, iters=r.iters, converged=bool(r.converged))
+            for r in reps[:n]]
+
+
+def calcp(dims, f, opts):
+    """calcp-multiple_correction_SIMPLE on the dict of arrays f (modified in
+    place); returns (solver reports, continuity sums per corrector)."""
+    st = _flow_struct(f)
+    n = opts.ncorr * opts.npcor
+    reps = (_Report * n)()
+    cont = np.zeros(3 * opts.ncorr)
+    lib().orc_calcp(*dims, C.byref(st), C.byref(opts), reps, _p(cont))
+    return _reports(reps, n), [tuple(cont[3 * q:3 * q + 3]) for q in range(opts.ncorr)]
+
+
+def piso(dims, f, opts, pimple=False):
+    st = _flow_struct(f)
+    n = opts.ncorr * opts.npcor
+    reps = (_Report * n)()
+    cont = np.zeros(3 * opts.ncorr)
+    lib().orc_piso(*dims, C.byref(st), C.byref(opts), 1 if pimple else 0, reps, _p(cont))
+    return _reports(reps, n), [tuple(cont[3 * q:3 * q + 3]) for q in range(opts.ncorr)]
+
+
+def outbc(dims, f, opts):
+    st = _flow_struct(f)
+    lib().orc_outbc(*dims, C.byref(st), C.byref(opts))
+
+
+def correct_bc(dims, f):
+    st = _flow_struct(f)
+    lib().orc_correct_bc(*dims, C.byref(st))
+
+
+def gethbya(dims, f, opts):
+    st = _flow_struct(f)
+    lib().orc_gethbya(*dims, C.byref(st), C.byref(opts))
 
 
 # ----------------------------------------------------- reference driver --

@@ -1,0 +1,195 @@
+"""Pinning of the oracle's pressure-correction drivers (oracle/cap_oracle.c:
+orc_calcp, orc_piso, orc_outbc, orc_correct_bc, orc_gethbya).
+
+The reference's driver files cannot be built here: every driver links
+grad_lsq_qr (GRAD_LSQ_QR.f90), which needs LAPACK's dgeqrf, and PISO / PIMPLE
+call pcgsip, which needs MPI.  Their building blocks (fluxmass, grad_gauss,
+fluxmc, the solvers, bpres, continuity sums) are pinned against the reference
+build elsewhere; the steps between them are pinned here by what the
+mathematics fixes:
+  * after the flux correction the divergence of the corrected fluxes equals
+    the p' equation's residual (cells off the boundary);
+  * the pressure at the reference cell does not move, and the velocity
+    correction takes the matching gradient component;
+  * outbc's rescaled outflow equals flowin;
+  * correctBoundaryConditions' copies are periodic;
+  * PISO_getHbyA's closed forms (uniform old velocity, BDF without
+    neighbours, buoyancy);
+  * PISO's reference-cell row."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import flowcase as FC  # noqa: E402
+
+DIMS = (16, 13, 11)
+
+
+def grid(a, dims):
+    ni, nj, nk = dims
+    return a.reshape(nk, ni, nj)
+
+
+def inner(a, dims, lo=3, shrink=0):
+    ni, nj, nk = dims
+    g = grid(a, dims)
+    s = slice(lo - 1 + shrink, None)
+    return g[lo - 1 + shrink:nk - 2 - shrink, lo - 1 + shrink:ni - 2 - shrink,
+             lo - 1 + shrink:nj - 2 - shrink]
+
+
+def div(f, dims):
+    """-div f per cell (reference sign: su = -f1 + f1_w - f2 + f2_s - f3 + f3_b)"""
+    g1, g2, g3 = (grid(f[q], dims) for q in ("f1", "f2", "f3"))
+    out = np.zeros_like(g1)
+    out[1:, 1:, 1:] = (-g1[1:, 1:, 1:] + g1[1:, :-1, 1:] - g2[1:, 1:, 1:] + g2[1:, 1:, :-1]
+                       - g3[1:, 1:, 1:] + g3[:-1, 1:, 1:])
+    return out
+
+
+def residual(f, x, dims):
+    """sum a_nb x_nb + su - ap x per cell"""
+    g = {q: grid(f[q], dims) for q in ("ae", "aw", "an", "as", "at", "ab", "ap", "su")}
+    X = grid(x, dims)
+    r = np.zeros_like(X)
+    c = (slice(1, -1),) * 3
+    r[c] = (g["ae"][c] * X[1:-1, 2:, 1:-1] + g["aw"][c] * X[1:-1, :-2, 1:-1] +
+            g["an"][c] * X[1:-1, 1:-1, 2:] + g["as"][c] * X[1:-1, 1:-1, :-2] +
+            g["at"][c] * X[2:, 1:-1, 1:-1] + g["ab"][c] * X[:-2, 1:-1, 1:-1] +
+            g["su"][c] - g["ap"][c] * X[c])
+    return r
+
+
+@pytest.mark.parametrize("solver", [4, 1, 5])
+def test_calcp_corrected_fluxes_balance_the_residual(O, solver):
+    """Stopped after one iteration, so that the residual is not small against the
+    source: the corrected fluxes' divergence must be that residual, cell by
+    cell (a sign or index slip in the assembly or the correction leaves an
+    O(su) difference)."""
+    f = FC.flow_case(DIMS, 5)
+    p0 = f["p"].copy()
+    o = FC.opts(O.DrvOpts, solver=solver, alfa=0.0 if solver == 4 else 0.92, urf=0.5, nsw=1,
+                ijkpr=FC.ref_cell(DIMS, 5, 6, 4))
+    reps, cont = O.calcp(DIMS, f, o)
+    # pp up to a constant (ap = sum a_nb, so a constant does not change r)
+    pp = (f["p"] - p0) / o.urf
+    r = residual(f, pp, DIMS)
+    d = div(f, DIMS)
+    scale = np.max(np.abs(f["su"]))
+    got, want = d[3:-3, 3:-3, 3:-3], r[3:-3, 3:-3, 3:-3]
+    assert np.max(np.abs(want)) > 1e-4 * scale
+    assert np.max(np.abs(got - want)) < 1e-9 * scale
+
+
+def test_calcp_reference_cell(O):
+    f = FC.flow_case(DIMS, 6)
+    p0 = f["p"].copy()
+    ipr = FC.ref_cell(DIMS, 6, 5, 7)
+    o = FC.opts(O.DrvOpts, solver=4, ijkpr=ipr, ncorr=2, npcor=2)
+    O.calcp(DIMS, f, o)
+    assert f["p"][ipr - 1] == p0[ipr - 1]
+    assert not np.array_equal(inner(f["p"], DIMS), inner(p0, DIMS))
+    # pp = p on return
+    np.testing.assert_array_equal(f["pp"], f["p"])
+
+
+def test_calcp_velocity_correction_components(O):
+    f = FC.flow_case(DIMS, 7)
+    u0, v0, w0 = (f[q].copy() for q in ("u", "v", "w"))
+    O.calcp(DIMS, f, FC.opts(O.DrvOpts, solver=4, ijkpr=FC.ref_cell(DIMS)))
+    for m, (q, a0) in enumerate((("u", u0), ("v", v0), ("w", w0))):
+        want = a0 - f["vol"] * f["ap" + q] * f["gradp"][m::3]
+        np.testing.assert_allclose(inner(f[q], DIMS), inner(want, DIMS), rtol=1e-13, atol=1e-15)
+
+
+def test_outbc_rescaled_outflow_equals_flowin(O):
+    dims = DIMS
+    f = FC.flow_case(dims, 8)
+    lb = FC.boundary_types(dims)
+    o = FC.opts(O.DrvOpts, lout=1, loe=1, low=1, flowin=3.7)
+    for q, a in lb.items():
+        setattr(o, q, a.ctypes.data)
+    u_in = grid(f["u"], dims)[1:-1, -3, 1:-1].copy()
+    O.outbc(dims, f, o)
+    ni, nj, nk = dims
+    g1 = grid(f["f1"], dims)
+    outflow = g1[1:nk - 1, ni - 3, 1:nj - 1].sum()     # f1(inh), inh at i = nim - 1
+    assert outflow == pytest.approx(3.7, rel=1e-12)
+    # the outlet cells carry the scaled inner velocity
+    fac = grid(f["u"], dims)[1:-1, -2, 1:-1] / u_in
+    assert np.allclose(fac, fac.flat[0], rtol=1e-14)
+    # walls elsewhere: the west flux is untouched
+    assert np.all(grid(f["f1"], dims)[:, 1, :] == 0.0)
+
+
+def test_correct_bc_is_periodic(O):
+    dims = (14, 14, 9)
+    f = FC.flow_case(dims, 9)
+    f["f1"][:] = np.random.default_rng(1).normal(size=f["f1"].size)
+    f["f2"][:] = np.random.default_rng(2).normal(size=f["f2"].size)
+    O.correct_bc(dims, f)
+    ni, nj, nk = dims
+    for q in ("u", "v", "w"):
+        g = grid(f[q], dims)
+        np.testing.assert_array_equal(g[1:nk - 1, ni - 1, 1:nj - 1], g[1:nk - 1, 1, 1:nj - 1])
+        np.testing.assert_array_equal(g[1:nk - 1, 0, 1:nj - 1], g[1:nk - 1, nj - 2, 1:nj - 1])
+        np.testing.assert_array_equal(g[1:nk - 1, 1:ni - 1, nj - 1], g[1:nk - 1, 1:ni - 1, 1])
+        np.testing.assert_array_equal(g[1:nk - 1, 1:ni - 1, 0], g[1:nk - 1, 1:ni - 1, nj - 2])
+    g1, g2 = grid(f["f1"], dims), grid(f["f2"], dims)
+    np.testing.assert_array_equal(g1[1:nk - 1, 0, 1:nj - 1], g1[1:nk - 1, nj - 2, 1:nj - 1])
+    np.testing.assert_array_equal(g2[1:nk - 1, 1:ni - 1, 0], g2[1:nk - 1, 1:ni - 1, nj - 2])
+
+
+def test_gethbya_closed_forms(O):
+    dims = DIMS
+    # Crank-Nicolson with a uniform old velocity: the old-time part is apotime * c
+    f = FC.flow_case(dims, 10)
+    c, U = 0.7, 1.3
+    for q in ("uo", "vo", "wo"):
+        f[q][:] = c
+    for q in ("u", "v", "w"):
+        f[q][:] = U
+    o = FC.opts(O.DrvOpts, cn=1, timestep=0.02)
+    H = sum(f[q] for q in ("he", "hw", "hn", "hs", "ht", "hb"))
+    apot = f["den"] * f["vol"] / 0.02
+    want = f["apu"] * (apot * c + U * H)
+    O.gethbya(dims, f, o)
+    np.testing.assert_allclose(inner(f["u"], dims), inner(want, dims), rtol=1e-12)
+    # BDF2 alone, no neighbours
+    f = FC.flow_case(dims, 11)
+    for q in ("he", "hw", "hn", "hs", "ht", "hb"):
+        f[q][:] = 0.0
+    o = FC.opts(O.DrvOpts, bdf=1, btime=1.0, timestep=0.05)
+    want = f["apv"] * (f["den"] * f["vol"] / 0.05) * (2.0 * f["vo"] - 0.5 * f["voo"])
+    O.gethbya(dims, f, o)
+    np.testing.assert_allclose(inner(f["v"], dims), inner(want, dims), rtol=1e-13)
+    np.testing.assert_array_equal(inner(f["sp"], dims), 0.0)
+    # Boussinesq buoyancy alone, gravity along z
+    f = FC.flow_case(dims, 12)
+    for q in ("he", "hw", "hn", "hs", "ht", "hb"):
+        f[q][:] = 0.0
+    o = FC.opts(O.DrvOpts, buoy=1, boussinesq=1, beta=2e-3, densit=1.2, tref=1.0, gravz=-9.81)
+    heat = 2e-3 * 1.2 * (f["t"] - 1.0) * f["vol"]
+    O.gethbya(dims, f, o)
+    np.testing.assert_allclose(inner(f["sw"], dims), inner(9.81 * heat, dims), rtol=1e-13)
+    np.testing.assert_array_equal(inner(f["su"], dims), 0.0)
+
+
+@pytest.mark.parametrize("pimple", [False, True])
+def test_piso_reference_cell_row(O, pimple):
+    f = FC.flow_case(DIMS, 13)
+    ipr = FC.ref_cell(DIMS, 4, 4, 4)
+    o = FC.opts(O.DrvOpts, ncorr=2, npcor=2, ijkpr=ipr, nsw=200)
+    reps, cont = O.piso(DIMS, f, o, pimple=pimple)
+    c = ipr - 1
+    assert f["ap"][c] == 1.0
+    assert all(f[q][c] == 0.0 for q in ("ae", "aw", "an", "as", "at", "ab"))
+    # non-final correctors solve to sor = 0.1 (a default-real literal)
+    assert reps[0]["converged"] and reps[0]["resl"] < float(np.float32(0.1)) * reps[0]["resor"]
+    # PISO starts from p = 0 and ends with pp = p
+    np.testing.assert_array_equal(f["pp"], f["p"])
+    assert len(cont) == 2 and cont[1][2] == pytest.approx(cont[0][1] + cont[1][1], rel=1e-15)
