@@ -726,3 +726,339 @@ extern "C" int cap_piso(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pi
     HIPCHK(hipStreamSynchronize(s));
     return worst;
 }
+
+// ================================================================ calcuvw
+namespace {
+
+struct UvwIO {
+    const double *den, *vol, *t, *uo, *vo, *wo, *uoo, *voo, *woo, *gradp, *xc, *yc, *zc, *fz;
+    const double *ar3x, *ar3y, *ar3z, *vis;
+    double *u, *v, *w, *su, *sv, *sw, *sp, *spv, *ap, *apu, *apv, *apw;
+    double *ae, *aw, *an, *as, *at, *ab;
+    int const_mflux, buoy, boussinesq, bdf, cn;
+    double gradpcmf, timestep, btime, densit, beta, tref, gravx, gravy, gravz;
+};
+
+// calcuvw.f90:50-98: momentum sources on k = 2..nkm, i, j = 3..n?mm
+__global__ __launch_bounds__(256) void k_uvw_src(Dim d, UvwIO q)
+{
+    const int nx = d.nj - 4, ny = d.ni - 4, nz = d.nk - 2;
+    const long n = (long)nx * ny * nz;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(t % nx), i = 3 + (int)((t / nx) % ny), k = 2 + (int)(t / ((long)nx * ny));
+        const size_t c = (size_t)(k - 1) * d.nij + (size_t)(i - 1) * d.nj + (size_t)(j - 1);
+        double ap = 0.0, spv = 0.0, sp = 0.0;
+        double su = -q.gradp[3 * c] * q.vol[c];
+        double sv = -q.gradp[3 * c + 1] * q.vol[c];
+        double sw = -q.gradp[3 * c + 2] * q.vol[c];
+        if (q.const_mflux) su = su + q.gradpcmf * q.vol[c];
+        if (q.buoy) {
+            double heat = 0.0;
+            if (q.boussinesq) heat = q.beta * q.densit * (q.t[c] - q.tref) * q.vol[c];
+            else heat = (q.densit - q.den[c]) * q.vol[c];
+            su = su - q.gravx * heat;
+            sv = sv - q.gravy * heat;
+            sw = sw - q.gravz * heat;
+        }
+        if (q.bdf) {
+            const double apotime = q.den[c] * q.vol[c] / q.timestep;
+            const double sut = apotime * ((1.0 + q.btime) * q.uo[c] - 0.5 * q.btime * q.uoo[c]);
+            const double svt = apotime * ((1.0 + q.btime) * q.vo[c] - 0.5 * q.btime * q.voo[c]);
+            const double swt = apotime * ((1.0 + q.btime) * q.wo[c] - 0.5 * q.btime * q.woo[c]);
+            su = su + sut;
+            sv = sv + svt;
+            sw = sw + swt;
+            ap = ap + apotime * (1.0 + 0.5 * q.btime);
+            spv = spv + apotime * (1.0 + 0.5 * q.btime);
+            sp = sp + apotime * (1.0 + 0.5 * q.btime);
+        }
+        q.ap[c] = ap;
+        q.spv[c] = spv;
+        q.sp[c] = sp;
+        q.su[c] = su;
+        q.sv[c] = sv;
+        q.sw[c] = sw;
+    }
+}
+
+// calcuvw.f90:160-310 (first-order branch): wall shear of the bottom wall
+// (top = 0: cells on k = 3, ghosts below) or the top wall (top = 1: cells on
+// the plane kp = nimm, as written, ghosts above), one thread per (i, j)
+__global__ __launch_bounds__(256) void k_uvw_wall(Dim d, UvwIO q, int top, int kp, double dnx,
+                                                  double dny, double dnz)
+{
+    const int nx = d.nj - 4, ny = d.ni - 4;
+    const long n = (long)nx * ny;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(t % nx), i = 3 + (int)(t / nx);
+        const size_t inbc = (size_t)(kp - 1) * d.nij + (size_t)(i - 1) * d.nj + (size_t)(j - 1);
+        const size_t ingc = top ? inbc + d.nij : inbc - d.nij, iface = top ? inbc : ingc;
+        const double fxp = top ? 1.0 - q.fz[inbc] : q.fz[ingc];
+        const double fxe = 1.0 - fxp;
+        const double gx = q.xc[inbc] * fxp + q.xc[ingc] * fxe;
+        const double gy = q.yc[inbc] * fxp + q.yc[ingc] * fxe;
+        const double gz = q.zc[inbc] * fxp + q.zc[ingc] * fxe;
+        const double are = sqrt(q.ar3x[iface] * q.ar3x[iface] + q.ar3y[iface] * q.ar3y[iface] +
+                                q.ar3z[iface] * q.ar3z[iface]);
+        const double nxf = q.ar3x[iface] / are, nyf = q.ar3y[iface] / are, nzf = q.ar3z[iface] / are;
+        const double dnw = fabs((q.xc[inbc] - gx) * dnx + (q.yc[inbc] - gy) * dny + (q.zc[inbc] - gz) * dnz);
+        const double viss = q.vis[ingc];
+        const double cf = viss * are / dnw;
+        const double dxx = q.xc[inbc] - q.xc[ingc], dyy = q.yc[inbc] - q.yc[ingc],
+                     dzz = q.zc[inbc] - q.zc[ingc];
+        const double dpb = 0.5 * sqrt(dxx * dxx + dyy * dyy + dzz * dzz);
+        const double vsol = viss * are / dpb;
+        const double upb = q.u[inbc], vpb = q.v[inbc], wpb = q.w[inbc];
+        const double vnp = upb * nxf + vpb * nyf + wpb * nzf;
+        const double utp = upb - vnp * nxf, vtp = vpb - vnp * nyf, wtp = wpb - vnp * nzf;
+        q.ap[inbc] = q.ap[inbc] + vsol;
+        q.spv[inbc] = q.spv[inbc] + vsol;
+        q.sp[inbc] = q.sp[inbc] + vsol;
+        q.su[inbc] = q.su[inbc] + vsol * q.u[inbc] - cf * utp;
+        q.sv[inbc] = q.sv[inbc] + vsol * q.v[inbc] - cf * vtp;
+        q.sw[inbc] = q.sw[inbc] + vsol * q.w[inbc] - cf * wtp;
+    }
+}
+
+// Crank-Nicolson: a_nb = 0.5 a_nb on the whole arrays (calcuvw.f90:313-320)
+__global__ void k_half6(size_t n, double *ae, double *aw, double *an, double *as, double *at, double *ab)
+{
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
+        ae[c] = 0.5 * ae[c];
+        an[c] = 0.5 * an[c];
+        at[c] = 0.5 * at[c];
+        aw[c] = 0.5 * aw[c];
+        as[c] = 0.5 * as[c];
+        ab[c] = 0.5 * ab[c];
+    }
+}
+
+// per component m (u, v, w): calcuvw.f90:324-345 / 350-370 / 376-396
+__global__ __launch_bounds__(256) void k_uvw_asm(Dim d, UvwIO q, int m, double urfr, double urfm)
+{
+    const int nx = d.nj - 4, ny = d.ni - 4, nz = d.nk - 2;
+    const long n = (long)nx * ny * nz;
+    double *S = m == 0 ? q.su : m == 1 ? q.sv : q.sw;
+    double *D = m == 0 ? q.ap : m == 1 ? q.spv : q.sp;
+    const double *X = m == 0 ? q.u : m == 1 ? q.v : q.w;
+    const double *XO = m == 0 ? q.uo : m == 1 ? q.vo : q.wo;
+    double *AP = m == 0 ? q.apu : m == 1 ? q.apv : q.apw;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(t % nx), i = 3 + (int)((t / nx) % ny), k = 2 + (int)(t / ((long)nx * ny));
+        const size_t c = (size_t)(k - 1) * d.nij + (size_t)(i - 1) * d.nj + (size_t)(j - 1);
+        double s = S[c];
+        if (q.cn) {
+            const double apotime = q.den[c] * q.vol[c] / q.timestep;
+            s = s + (q.ae[c] * XO[c + d.nj] + q.aw[c] * XO[c - d.nj] + q.an[c] * XO[c + 1] +
+                     q.as[c] * XO[c - 1] + q.at[c] * XO[c + d.nij] + q.ab[c] * XO[c - d.nij]) +
+                (apotime - q.ae[c] - q.aw[c] - q.an[c] - q.as[c] - q.at[c] - q.ab[c]) * XO[c];
+            S[c] = s;
+            D[c] = D[c] + apotime;
+        }
+        double ap = q.ae[c] + q.aw[c] + q.an[c] + q.as[c] + q.at[c] + q.ab[c] + D[c];
+        ap = ap * urfr;
+        q.ap[c] = ap;
+        q.su[c] = s + urfm * ap * X[c];
+        AP[c] = 1.0 / (ap + SMALL_F);
+    }
+}
+
+// find_strain_rate.f90:36-77
+__global__ __launch_bounds__(256) void k_strain(Dim d, const double *gu, const double *gv,
+                                                const double *gw, double *strain, double *vort)
+{
+    const int nx = d.nj - 4, ny = d.ni - 4, nz = d.nk - 2;
+    const long n = (long)nx * ny * nz;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(t % nx), i = 3 + (int)((t / nx) % ny), k = 2 + (int)(t / ((long)nx * ny));
+        const size_t c = (size_t)(k - 1) * d.nij + (size_t)(i - 1) * d.nj + (size_t)(j - 1);
+        const double dudx = gu[3 * c], dudy = gu[3 * c + 1], dudz = gu[3 * c + 2];
+        const double dvdx = gv[3 * c], dvdy = gv[3 * c + 1], dvdz = gv[3 * c + 2];
+        const double dwdx = gw[3 * c], dwdy = gw[3 * c + 1], dwdz = gw[3 * c + 2];
+        const double s11 = dudx, s12 = 0.5 * (dudy + dvdx), s13 = 0.5 * (dudz + dwdx);
+        const double s22 = dvdy, s23 = 0.5 * (dvdz + dwdy), s33 = dwdz;
+        const double w12 = 0.5 * (dudy - dvdx), w13 = 0.5 * (dudz - dwdx), w23 = 0.5 * (dvdz - dwdy);
+        strain[c] = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 +
+                                2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+        vort[c] = sqrt(w12 * w12 + w23 * w23 + w13 * w13);
+    }
+}
+
+struct LesIO {
+    const double *u, *v, *w, *vol, *den, *fz, *xc, *yc, *zc, *ar3x, *ar3y, *ar3z, *strain, *wd;
+    double *vis;
+    double viscos, densit, cappa, elog, urf;
+};
+
+// calc_vis_les.f90:45-77: Smagorinsky with van Driest damping; default-real
+// literals C_delta = 0.158, C_Les = 0.17 and the exponent 1./3.
+__global__ __launch_bounds__(256) void k_les(Dim d, LesIO q)
+{
+    const double c_delta = (double)0.158f, c_les = (double)0.17f, third = (double)(1.0f / 3.0f);
+    const int nx = d.nj - 4, ny = d.ni - 4, nz = d.nk - 2;
+    const long n = (long)nx * ny * nz;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(t % nx), i = 3 + (int)((t / nx) % ny), k = 2 + (int)(t / ((long)nx * ny));
+        const size_t c = (size_t)(k - 1) * d.nij + (size_t)(i - 1) * d.nj + (size_t)(j - 1);
+        const double visold = q.vis[c];
+        const double wl = q.wd[c];
+        const double uref = sqrt(q.u[c] * q.u[c] + q.v[c] * q.v[c] + q.w[c] * q.w[c]);
+        const double utau = sqrt(q.viscos * uref / wl);
+        const double zplus = wl * utau / q.viscos;
+        const double a = c_les * pow(q.vol[c], third), b = q.cappa / c_delta * wl;
+        const double l = (b < a ? b : a) * (1.0 - exp(-zplus / 26.0));
+        const double vs = l * l * q.strain[c] * q.den[c];
+        const double vn = vs + q.viscos;
+        q.vis[c] = q.urf * vn + (1.0 - q.urf) * visold;
+    }
+}
+
+// calc_vis_les.f90:80-151: wall-function viscosity of the wall ghost cells
+// (top = 0: bottom wall, cells on k = 3; top = 1: cells on the plane kp)
+__global__ __launch_bounds__(256) void k_les_wall(Dim d, LesIO q, int top, int kp, double dnx,
+                                                  double dny, double dnz)
+{
+    const int nx = d.nj - 4, ny = d.ni - 4;
+    const long n = (long)nx * ny;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int j = 3 + (int)(t % nx), i = 3 + (int)(t / nx);
+        const size_t inbc = (size_t)(kp - 1) * d.nij + (size_t)(i - 1) * d.nj + (size_t)(j - 1);
+        const size_t ingc = top ? inbc + d.nij : inbc - d.nij, iface = top ? inbc : ingc;
+        const double fxp = top ? 1.0 - q.fz[inbc] : q.fz[ingc];
+        const double fxe = 1.0 - fxp;
+        const double gx = q.xc[inbc] * fxp + q.xc[ingc] * fxe;
+        const double gy = q.yc[inbc] * fxp + q.yc[ingc] * fxe;
+        const double gz = q.zc[inbc] * fxp + q.zc[ingc] * fxe;
+        const double are = sqrt(q.ar3x[iface] * q.ar3x[iface] + q.ar3y[iface] * q.ar3y[iface] +
+                                q.ar3z[iface] * q.ar3z[iface]);
+        const double nxf = q.ar3x[iface] / are, nyf = q.ar3y[iface] / are, nzf = q.ar3z[iface] / are;
+        const double uu = q.u[inbc], vv = q.v[inbc], ww = q.w[inbc];
+        const double vnp = uu * nxf + vv * nyf + ww * nzf;
+        double xtp = uu - vnp * nxf, ytp = vv - vnp * nyf, ztp = ww - vnp * nzf;
+        const double vtp = sqrt(xtp * xtp + ytp * ytp + ztp * ztp);
+        xtp = xtp / vtp;
+        ytp = ytp / vtp;
+        ztp = ztp / vtp;
+        const double ut2 = fabs(uu * xtp + vv * ytp + ww * ztp);
+        const double dnw = fabs((q.xc[inbc] - gx) * dnx + (q.yc[inbc] - gy) * dny + (q.zc[inbc] - gz) * dnz);
+        const double tw = q.viscos * ut2 / dnw;
+        const double utau = sqrt(tw / q.densit);
+        const double ypl = q.densit * utau * dnw / q.viscos;
+        double viscw = 0.0;
+        if (ypl > (double)11.61f) viscw = ypl * q.viscos * q.cappa / log(q.elog * ypl);
+        q.vis[ingc] = viscw > q.viscos ? viscw : q.viscos;
+    }
+}
+
+UvwIO uvw_io(cap_flow *f, double *spv, const cap_drv_opts *o, const cap_uvw_opts *q)
+{
+    UvwIO u;
+    u.den = f->den; u.vol = f->vol; u.t = f->t; u.uo = f->uo; u.vo = f->vo; u.wo = f->wo;
+    u.uoo = f->uoo; u.voo = f->voo; u.woo = f->woo; u.gradp = f->gradp;
+    u.xc = f->xc; u.yc = f->yc; u.zc = f->zc; u.fz = f->fz;
+    u.ar3x = f->ar3x; u.ar3y = f->ar3y; u.ar3z = f->ar3z; u.vis = f->vis;
+    u.u = f->u; u.v = f->v; u.w = f->w; u.su = f->su; u.sv = f->sv; u.sw = f->sw; u.sp = f->sp;
+    u.spv = spv; u.ap = f->ap; u.apu = f->apu; u.apv = f->apv; u.apw = f->apw;
+    u.ae = f->ae; u.aw = f->aw; u.an = f->an; u.as = f->as; u.at = f->at; u.ab = f->ab;
+    u.const_mflux = o->const_mflux; u.buoy = o->buoy; u.boussinesq = o->boussinesq;
+    u.bdf = o->bdf; u.cn = o->cn;
+    u.gradpcmf = q->gradpcmf; u.timestep = o->timestep; u.btime = o->btime; u.densit = o->densit;
+    u.beta = o->beta; u.tref = o->tref; u.gravx = o->gravx; u.gravy = o->gravy; u.gravz = o->gravz;
+    return u;
+}
+
+long plane_cells(const Dim &d) { return (long)(d.nj - 4) * (d.ni - 4); }
+long range2_cells(const Dim &d) { return (long)(d.nj - 4) * (d.ni - 4) * (d.nk - 2); }
+
+}  // namespace
+
+extern "C" int cap_calcuvw(cap_ctx *ctx, cap_flow *f, double *spv, const cap_drv_opts *o,
+                           const cap_uvw_opts *q, cap_report *reps)
+{
+    if (!dev_ok(ctx) || !f || !spv || !o || !q || !reps || !flow_ok(f) || !f->sv || !f->sw ||
+        !f->sp || !f->apu || !f->apv || !f->apw || !f->gradu || !f->gradv || !f->gradw ||
+        !f->vis || !f->den || !f->xc || !f->yc || !f->zc || !f->bp || !f->bt || !f->bb)
+        return CAP_ERR_ARG;
+    if ((o->bdf || o->cn) && (!f->uo || !f->vo || !f->wo)) return CAP_ERR_ARG;
+    if (o->bdf && (!f->uoo || !f->voo || !f->woo)) return CAP_ERR_ARG;
+    if (o->buoy && o->boussinesq && !f->t) return CAP_ERR_ARG;
+    const Dim d = dim_of(ctx);
+    if (d.ni - 1 > d.nk) return CAP_ERR_ARG;     // the top-wall plane lk(nimm) and its ghosts
+    hipStream_t s = ctx->stream;
+    RCHK(grad3(ctx, f, f->u, f->gradu));
+    RCHK(grad3(ctx, f, f->v, f->gradv));
+    RCHK(grad3(ctx, f, f->w, f->gradw));
+    RCHK(grad3(ctx, f, f->pp, f->gradp));
+    const UvwIO io = uvw_io(f, spv, o, q);
+    hipLaunchKernelGGL(k_uvw_src, dim3(grid_for(range2_cells(d))), dim3(256), 0, s, d, io);
+    HIPCHK(hipGetLastError());
+    const cap_fields F = fields_of(f);
+    RCHK(cap_fluxuvw(ctx, &F, f->vis, d.nj, 1, d.nij, f->ar1x, f->ar1y, f->ar1z, f->fx, f->ae, f->aw,
+                     f->f1, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
+    RCHK(cap_fluxuvw(ctx, &F, f->vis, 1, d.nij, d.nj, f->ar2x, f->ar2y, f->ar2z, f->fy, f->an, f->as,
+                     f->f2, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
+    RCHK(cap_fluxuvw(ctx, &F, f->vis, d.nij, d.nj, 1, f->ar3x, f->ar3y, f->ar3z, f->fz, f->at, f->ab,
+                     f->f3, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
+    hipLaunchKernelGGL(k_uvw_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, io, 0, 3,
+                       (double)q->nx, (double)q->ny, (double)q->nz);
+    hipLaunchKernelGGL(k_uvw_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, io, 1, d.ni - 2,
+                       (double)q->nx, (double)q->ny, (double)q->nz);
+    if (o->cn)
+        hipLaunchKernelGGL(k_half6, dim3(grid_for((long)ctx->nstd)), dim3(256), 0, s, ctx->nstd, f->ae,
+                           f->aw, f->an, f->as, f->at, f->ab);
+    HIPCHK(hipGetLastError());
+    const cap_coef C = coef_of(f);
+    double *X[3] = {f->u, f->v, f->w};
+    int worst = CAP_OK;
+    for (int m = 0; m < 3; ++m) {
+        hipLaunchKernelGGL(k_uvw_asm, dim3(grid_for(range2_cells(d))), dim3(256), 0, s, d, io, m,
+                           q->urfr[m], q->urfm[m]);
+        HIPCHK(hipGetLastError());
+        const int rc = cap_sipsol(ctx, X[m], &C, o->alfa, q->sor[m], q->nsw[m], &reps[m]);
+        if (rc < 0) return rc;
+        worst = std::max(worst, rc);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return worst;
+}
+
+extern "C" int cap_find_strain_rate(cap_ctx *ctx, cap_flow *f, double *strain, double *vorticity)
+{
+    if (!dev_ok(ctx) || !f || !strain || !vorticity || !f->u || !f->v || !f->w || !f->gradu ||
+        !f->gradv || !f->gradw || !f->vol || !f->fx || !f->ar1x)
+        return CAP_ERR_ARG;
+    const Dim d = dim_of(ctx);
+    RCHK(grad3(ctx, f, f->u, f->gradu));
+    RCHK(grad3(ctx, f, f->v, f->gradv));
+    RCHK(grad3(ctx, f, f->w, f->gradw));
+    hipLaunchKernelGGL(k_strain, dim3(grid_for(range2_cells(d))), dim3(256), 0, ctx->stream, d,
+                       (const double *)f->gradu, (const double *)f->gradv, (const double *)f->gradw,
+                       strain, vorticity);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_calc_vis_les(cap_ctx *ctx, cap_flow *f, const double *strain,
+                                const double *walldist, const cap_les_opts *o)
+{
+    if (!dev_ok(ctx) || !f || !strain || !walldist || !o || !f->u || !f->v || !f->w || !f->vol ||
+        !f->den || !f->fz || !f->xc || !f->yc || !f->zc || !f->ar3x || !f->ar3y || !f->ar3z || !f->vis)
+        return CAP_ERR_ARG;
+    const Dim d = dim_of(ctx);
+    if (d.ni - 1 > d.nk) return CAP_ERR_ARG;     // the top-wall plane lk(nimm) and its ghosts
+    LesIO q;
+    q.u = f->u; q.v = f->v; q.w = f->w; q.vol = f->vol; q.den = f->den; q.fz = f->fz;
+    q.xc = f->xc; q.yc = f->yc; q.zc = f->zc; q.ar3x = f->ar3x; q.ar3y = f->ar3y; q.ar3z = f->ar3z;
+    q.strain = strain; q.wd = walldist; q.vis = f->vis;
+    q.viscos = o->viscos; q.densit = o->densit; q.cappa = o->cappa; q.elog = o->elog; q.urf = o->urfvis;
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_les, dim3(grid_for(range2_cells(d))), dim3(256), 0, s, d, q);
+    hipLaunchKernelGGL(k_les_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, q, 0, 3,
+                       (double)o->nx, (double)o->ny, (double)o->nz);
+    hipLaunchKernelGGL(k_les_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, q, 1, d.ni - 2,
+                       (double)o->nx, (double)o->ny, (double)o->nz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return CAP_OK;
+}

@@ -91,6 +91,22 @@ module cap_hip
     real(c_double) :: gravx = 0, gravy = 0, gravz = 0, cumulative = 0
   end type cap_drv_opts
 
+  ! calcuvw settings beyond cap_drv_opts
+  type, bind(c) :: cap_uvw_opts
+    integer(c_int) :: schemes = 0
+    real(c_double) :: gds = 1
+    real(c_double) :: urfr(3) = 1, urfm(3) = 0, sor(3) = 0.1d0
+    integer(c_int) :: nsw(3) = 5
+    real(c_double) :: gradpcmf = 0
+    integer(c_int) :: nx = 0, ny = 0, nz = 0
+  end type cap_uvw_opts
+
+  ! calc_vis_les settings
+  type, bind(c) :: cap_les_opts
+    real(c_double) :: viscos = 0, densit = 1, cappa = 0.41d0, elog = 8.432d0, urfvis = 1
+    integer(c_int) :: nx = 0, ny = 0, nz = 0
+  end type cap_les_opts
+
   type, bind(c) :: cap_report
     real(c_double) :: resor, resl
     integer(c_int) :: iters, status
@@ -300,6 +316,27 @@ module cap_hip
       import :: c_ptr, c_int, cap_flow
       type(c_ptr), value :: ctx
       type(cap_flow), intent(in) :: f
+    end function
+    integer(c_int) function cap_calcuvw(ctx, f, spv, o, q, reps) bind(c, name='cap_calcuvw')
+      import :: c_ptr, c_int, cap_flow, cap_drv_opts, cap_uvw_opts, cap_report
+      type(c_ptr), value :: ctx, spv
+      type(cap_flow), intent(in) :: f
+      type(cap_drv_opts), intent(in) :: o
+      type(cap_uvw_opts), intent(in) :: q
+      type(cap_report), intent(out) :: reps(3)
+    end function
+    integer(c_int) function cap_find_strain_rate(ctx, f, strain, vorticity) &
+        bind(c, name='cap_find_strain_rate')
+      import :: c_ptr, c_int, cap_flow
+      type(c_ptr), value :: ctx, strain, vorticity
+      type(cap_flow), intent(in) :: f
+    end function
+    integer(c_int) function cap_calc_vis_les(ctx, f, strain, walldist, o) &
+        bind(c, name='cap_calc_vis_les')
+      import :: c_ptr, c_int, cap_flow, cap_les_opts
+      type(c_ptr), value :: ctx, strain, walldist
+      type(cap_flow), intent(in) :: f
+      type(cap_les_opts), intent(in) :: o
     end function
     integer(c_int) function cap_gethbya(ctx, f, o) bind(c, name='cap_gethbya')
       import :: c_ptr, c_int, cap_flow, cap_drv_opts

@@ -39,7 +39,8 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_courant", "cap_unet_create", "cap_unet_destroy", "cap_unet_num_params",
            "cap_unet_get_params", "cap_unet_set_params", "cap_unet_forward",
            "cap_unet_gradients", "cap_unet_train_step", "cap_unet_guess", "cap_pcgsip_guess",
-           "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya")
+           "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya",
+           "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -91,6 +92,20 @@ class DrvOpts(C.Structure):
                [("cn", C.c_int), ("bdf", C.c_int), ("buoy", C.c_int), ("boussinesq", C.c_int)] + \
                [(n, C.c_double) for n in ("timestep", "btime", "densit", "beta", "tref",
                                           "gravx", "gravy", "gravz", "cumulative")]
+
+
+class UvwOpts(C.Structure):
+    """calcuvw's settings beyond DrvOpts (cap_uvw_opts)."""
+    _fields_ = [("schemes", C.c_int), ("gds", C.c_double), ("urfr", C.c_double * 3),
+                ("urfm", C.c_double * 3), ("sor", C.c_double * 3), ("nsw", C.c_int * 3),
+                ("gradpcmf", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
+
+
+class LesOpts(C.Structure):
+    """calc_vis_les settings (cap_les_opts)."""
+    _fields_ = [("viscos", C.c_double), ("densit", C.c_double), ("cappa", C.c_double),
+                ("elog", C.c_double), ("urfvis", C.c_double), ("nx", C.c_int), ("ny", C.c_int),
+                ("nz", C.c_int)]
 
 
 class _Report(C.Structure):
@@ -148,6 +163,9 @@ def load():
     L.cap_outbc.argtypes = [vp, Fl, O]
     L.cap_correct_bc.argtypes = [vp, Fl]
     L.cap_gethbya.argtypes = [vp, Fl, O]
+    L.cap_calcuvw.argtypes = [vp, Fl, vp, O, C.POINTER(UvwOpts), C.POINTER(_Report)]
+    L.cap_find_strain_rate.argtypes = [vp, Fl, vp, vp]
+    L.cap_calc_vis_les.argtypes = [vp, Fl, vp, vp, C.POINTER(LesOpts)]
     L.cap_unet_guess.argtypes = [vp, vp, vp, vp, C.POINTER(_Coef), C.POINTER(d)]
     L.cap_pcgsip_guess.argtypes = [vp, vp, vp, vp, C.POINTER(_Coef), d, d, i, C.POINTER(_Report),
                                    C.POINTER(i)]
@@ -391,6 +409,25 @@ class Context:
         st = self._flow(f)
         self._check(self._lib.cap_gethbya(self._h, C.byref(st), C.byref(opts)), "cap_gethbya",
                     allow=(CAP_OK,))
+
+    def calcuvw(self, f, spv, opts, uvw):
+        """`call calcuvw`; returns the three sipsol reports (u, v, w)."""
+        st = self._flow(f)
+        reps = (_Report * 3)()
+        rc = self._lib.cap_calcuvw(self._h, C.byref(st), _ptr(spv), C.byref(opts), C.byref(uvw), reps)
+        self._check(rc, "cap_calcuvw")
+        return self._reports(reps, 3)
+
+    def find_strain_rate(self, f, strain, vorticity):
+        st = self._flow(f)
+        rc = self._lib.cap_find_strain_rate(self._h, C.byref(st), _ptr(strain), _ptr(vorticity))
+        self._check(rc, "cap_find_strain_rate", allow=(CAP_OK,))
+
+    def calc_vis_les(self, f, strain, walldist, les):
+        st = self._flow(f)
+        rc = self._lib.cap_calc_vis_les(self._h, C.byref(st), _ptr(strain), _ptr(walldist),
+                                        C.byref(les))
+        self._check(rc, "cap_calc_vis_les", allow=(CAP_OK,))
 
     def build_lognormal(self, coef, seed, sigma=1.0):
         st = coef._struct()

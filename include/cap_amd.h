@@ -199,6 +199,49 @@ int cap_correct_bc(cap_ctx *ctx, cap_flow *f);
 /* `call PISO_getHbyA` (PISO_getHbyA.f90:3-233). */
 int cap_gethbya(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o);
 
+/* calcuvw settings beyond cap_drv_opts (of which calcuvw reads const_mflux,
+ * buoy, boussinesq, bdf, cn, timestep, btime, densit, beta, tref, grav*,
+ * alfa): fluxuvw's scheme flags (CAP_UVW_*) and gds(iu); urfr, urfm, sor, nsw
+ * of u, v, w; gradpcmf; the module integers nx, ny, nz of the wall loops. */
+typedef struct {
+    int schemes;
+    double gds;
+    double urfr[3], urfm[3], sor[3];
+    int nsw[3];
+    double gradpcmf;
+    int nx, ny, nz;
+} cap_uvw_opts;
+
+/* `call calcuvw` (calcuvw.f90:3-464, Gauss gradients, isol = 1): gradients of
+ * u, v, w, pp; momentum sources; fluxuvw for the three directions; bottom and
+ * top wall shear (the top wall's cells on the plane lk(nimm), as written:
+ * needs ni <= nk + 1); Crank-Nicolson halving; per component the
+ * under-relaxed assembly, apu / apv / apw and sipsol.  spv is the module
+ * array of the v equation's diagonal source (device).  reps[3]: the three
+ * sipsol reports.  calcstress (Reynolds stresses of the EVM / ASM models,
+ * read neither by fluxuvw nor by the assembly) is not part of it. */
+int cap_calcuvw(cap_ctx *ctx, cap_flow *f, double *spv, const cap_drv_opts *o,
+                const cap_uvw_opts *q, cap_report *reps);
+
+/* `call find_strain_rate` (find_strain_rate.f90:1-79, Gauss gradients):
+ * gradu, gradv, gradw of f, then strain and vorticity magnitudes on
+ * k = 2..nkm, i, j = 3..n?mm (device arrays). */
+int cap_find_strain_rate(cap_ctx *ctx, cap_flow *f, double *strain, double *vorticity);
+
+typedef struct {
+    double viscos, densit, cappa, elog, urfvis;   /* urfvis = urf(ivis) */
+    int nx, ny, nz;                               /* module integers of the wall loops */
+} cap_les_opts;
+
+/* `call calc_vis_les` (calc_vis_les.f90:3-151): Smagorinsky viscosity with
+ * van Driest damping (C_Les = 0.17, C_delta = 0.158) into f->vis on
+ * k = 2..nkm, i, j = 3..n?mm, then the wall-function viscosity of the bottom
+ * and top wall ghost cells (top wall on the plane lk(nimm), as written:
+ * needs ni <= nk + 1).  strain and walldist are device arrays.  pow, exp
+ * and log are the device's: within a few ulp of the host's. */
+int cap_calc_vis_les(cap_ctx *ctx, cap_flow *f, const double *strain, const double *walldist,
+                     const cap_les_opts *o);
+
 /* ---- Learned initial guess (no counterpart in the reference) ----------
  * A 3-D U-Net (4 levels, 16/32/48/64 channels, 3x3x3 convolutions, BF16
  * weights and activations, FP32 accumulation on the matrix cores) that maps

@@ -1652,3 +1652,213 @@ void orc_piso(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o, int pi
     }
     memcpy(F->pp, F->p, n * sizeof(double));
 }
+
+/* ======================================================= LES and momentum */
+
+void orc_strain_rate(int ni, int nj, int nk, const double *gradu, const double *gradv,
+                     const double *gradw, double *strain, double *vorticity)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    (void)nij;
+    for (int k = 2; k <= nkm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        const size_t c = IX(i, j, k);
+        const double dudx = gradu[3 * c], dudy = gradu[3 * c + 1], dudz = gradu[3 * c + 2];
+        const double dvdx = gradv[3 * c], dvdy = gradv[3 * c + 1], dvdz = gradv[3 * c + 2];
+        const double dwdx = gradw[3 * c], dwdy = gradw[3 * c + 1], dwdz = gradw[3 * c + 2];
+        const double s11 = dudx, s12 = 0.5 * (dudy + dvdx), s13 = 0.5 * (dudz + dwdx);
+        const double s22 = dvdy, s23 = 0.5 * (dvdz + dwdy), s33 = dwdz;
+        const double w12 = 0.5 * (dudy - dvdx), w13 = 0.5 * (dudz - dwdx), w23 = 0.5 * (dvdz - dwdy);
+        strain[c] = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 +
+                                2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+        vorticity[c] = sqrt(w12 * w12 + w23 * w23 + w13 * w13);
+    }
+}
+
+/* the wall-function viscosity of one wall cell pair (calc_vis_les.f90:84-115) */
+static void les_wall(size_t inbc, size_t ingc, size_t iface, double fxp, const double *u,
+                     const double *v, const double *w, const double *xc, const double *yc,
+                     const double *zc, const double *ar3x, const double *ar3y, const double *ar3z,
+                     double *vis, double viscos, double densit, double cappa, double elog,
+                     double dnx, double dny, double dnz)
+{
+    const double fxe = 1.0 - fxp;
+    const double gx = xc[inbc] * fxp + xc[ingc] * fxe;
+    const double gy = yc[inbc] * fxp + yc[ingc] * fxe;
+    const double gz = zc[inbc] * fxp + zc[ingc] * fxe;
+    const double are = sqrt(ar3x[iface] * ar3x[iface] + ar3y[iface] * ar3y[iface] +
+                            ar3z[iface] * ar3z[iface]);
+    const double nxf = ar3x[iface] / are, nyf = ar3y[iface] / are, nzf = ar3z[iface] / are;
+    const double vnp = u[inbc] * nxf + v[inbc] * nyf + w[inbc] * nzf;
+    double xtp = u[inbc] - vnp * nxf, ytp = v[inbc] - vnp * nyf, ztp = w[inbc] - vnp * nzf;
+    const double vtp = sqrt(xtp * xtp + ytp * ytp + ztp * ztp);
+    xtp = xtp / vtp;
+    ytp = ytp / vtp;
+    ztp = ztp / vtp;
+    const double ut2 = fabs(u[inbc] * xtp + v[inbc] * ytp + w[inbc] * ztp);
+    const double dnw = fabs((xc[inbc] - gx) * dnx + (yc[inbc] - gy) * dny + (zc[inbc] - gz) * dnz);
+    const double tw = viscos * ut2 / dnw;
+    const double utau = sqrt(tw / densit);
+    const double ypl = densit * utau * dnw / viscos;
+    double viscw = 0.0;
+    if (ypl > (double)11.61f) viscw = ypl * viscos * cappa / log(elog * ypl);
+    vis[ingc] = orc_max2(viscos, viscw);
+}
+
+void orc_calc_vis_les(int ni, int nj, int nk, const double *u, const double *v, const double *w,
+                      const double *vol, const double *den, const double *fz, const double *xc,
+                      const double *yc, const double *zc, const double *ar3x, const double *ar3y,
+                      const double *ar3z, const double *strain, const double *walldist,
+                      double *vis, double viscos, double densit, double cappa, double elog,
+                      double urfvis, int nx, int ny, int nz)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    /* default-real literals: C_delta = 0.158, C_Les = 0.17, 1./3. */
+    const double c_delta = (double)0.158f, c_les = (double)0.17f, third = (double)(1.0f / 3.0f);
+    for (int k = 2; k <= nkm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        const size_t c = IX(i, j, k);
+        const double visold = vis[c];
+        const double wl = walldist[c];
+        const double uref = sqrt(u[c] * u[c] + v[c] * v[c] + w[c] * w[c]);
+        const double utau = sqrt(viscos * uref / wl);
+        const double zplus = wl * utau / viscos;
+        const double l = orc_min2(c_les * pow(vol[c], third), cappa / c_delta * wl) *
+                         (1.0 - exp(-zplus / 26.0));
+        const double vs = l * l * strain[c] * den[c];
+        vis[c] = vs + viscos;
+        vis[c] = urfvis * vis[c] + (1.0 - urfvis) * visold;
+    }
+    const double dnx = nx, dny = ny, dnz = nz;
+    for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        const size_t inbc = IX(i, j, 3), ingc = inbc - nij;
+        les_wall(inbc, ingc, ingc, fz[ingc], u, v, w, xc, yc, zc, ar3x, ar3y, ar3z, vis, viscos,
+                 densit, cappa, elog, dnx, dny, dnz);
+    }
+    /* the top wall's cells are taken on the plane lk(nimm), as written */
+    for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        const size_t inbc = IX(i, j, nimm), ingc = inbc + nij;
+        les_wall(inbc, ingc, inbc, 1.0 - fz[inbc], u, v, w, xc, yc, zc, ar3x, ar3y, ar3z, vis,
+                 viscos, densit, cappa, elog, dnx, dny, dnz);
+    }
+}
+
+/* the wall shear of one wall cell pair in calcuvw (calcuvw.f90:160-240,
+ * first-order branch) */
+static void uvw_wall(orc_flow *F, double *spv, size_t inbc, size_t ingc, size_t iface, double fxp,
+                     double dnx, double dny, double dnz)
+{
+    const double fxe = 1.0 - fxp;
+    const double gx = F->xc[inbc] * fxp + F->xc[ingc] * fxe;
+    const double gy = F->yc[inbc] * fxp + F->yc[ingc] * fxe;
+    const double gz = F->zc[inbc] * fxp + F->zc[ingc] * fxe;
+    const double are = sqrt(F->ar3x[iface] * F->ar3x[iface] + F->ar3y[iface] * F->ar3y[iface] +
+                            F->ar3z[iface] * F->ar3z[iface]);
+    const double nxf = F->ar3x[iface] / are, nyf = F->ar3y[iface] / are, nzf = F->ar3z[iface] / are;
+    const double dnw = fabs((F->xc[inbc] - gx) * dnx + (F->yc[inbc] - gy) * dny +
+                            (F->zc[inbc] - gz) * dnz);
+    const double viss = F->vis[ingc];
+    const double cf = viss * are / dnw;
+    const double dxx = F->xc[inbc] - F->xc[ingc], dyy = F->yc[inbc] - F->yc[ingc],
+                 dzz = F->zc[inbc] - F->zc[ingc];
+    const double dpb = 0.5 * sqrt(dxx * dxx + dyy * dyy + dzz * dzz);
+    const double vsol = viss * are / dpb;
+    const double upb = F->u[inbc], vpb = F->v[inbc], wpb = F->w[inbc];
+    const double vnp = upb * nxf + vpb * nyf + wpb * nzf;
+    const double utp = upb - vnp * nxf, vtp = vpb - vnp * nyf, wtp = wpb - vnp * nzf;
+    F->ap[inbc] = F->ap[inbc] + vsol;
+    spv[inbc] = spv[inbc] + vsol;
+    F->sp[inbc] = F->sp[inbc] + vsol;
+    F->su[inbc] = F->su[inbc] + vsol * F->u[inbc] - cf * utp;
+    F->sv[inbc] = F->sv[inbc] + vsol * F->v[inbc] - cf * vtp;
+    F->sw[inbc] = F->sw[inbc] + vsol * F->w[inbc] - cf * wtp;
+}
+
+void orc_calcuvw(int ni, int nj, int nk, orc_flow *F, double *spv, const orc_drv_opts *o,
+                 const orc_uvw_opts *q, orc_report *reps)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
+    const size_t n = (size_t)nij * nk;
+    drv_grad(ni, nj, nk, F, F->u, F->gradu);
+    drv_grad(ni, nj, nk, F, F->v, F->gradv);
+    drv_grad(ni, nj, nk, F, F->w, F->gradw);
+    drv_grad(ni, nj, nk, F, F->pp, F->gradp);
+    for (int k = 2; k <= nkm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        const size_t c = IX(i, j, k);
+        F->ap[c] = 0.0;
+        spv[c] = 0.0;
+        F->sp[c] = 0.0;
+        F->su[c] = -F->gradp[3 * c] * F->vol[c];
+        F->sv[c] = -F->gradp[3 * c + 1] * F->vol[c];
+        F->sw[c] = -F->gradp[3 * c + 2] * F->vol[c];
+        if (o->const_mflux) F->su[c] = F->su[c] + q->gradpcmf * F->vol[c];
+        if (o->buoy) {
+            double heat = 0.0;
+            if (o->boussinesq) heat = o->beta * o->densit * (F->t[c] - o->tref) * F->vol[c];
+            else heat = (o->densit - F->den[c]) * F->vol[c];
+            F->su[c] = F->su[c] - o->gravx * heat;
+            F->sv[c] = F->sv[c] - o->gravy * heat;
+            F->sw[c] = F->sw[c] - o->gravz * heat;
+        }
+        if (o->bdf) {
+            const double apotime = F->den[c] * F->vol[c] / o->timestep;
+            const double sut = apotime * ((1.0 + o->btime) * F->uo[c] - 0.5 * o->btime * F->uoo[c]);
+            const double svt = apotime * ((1.0 + o->btime) * F->vo[c] - 0.5 * o->btime * F->voo[c]);
+            const double swt = apotime * ((1.0 + o->btime) * F->wo[c] - 0.5 * o->btime * F->woo[c]);
+            F->su[c] = F->su[c] + sut;
+            F->sv[c] = F->sv[c] + svt;
+            F->sw[c] = F->sw[c] + swt;
+            F->ap[c] = F->ap[c] + apotime * (1.0 + 0.5 * o->btime);
+            spv[c] = spv[c] + apotime * (1.0 + 0.5 * o->btime);
+            F->sp[c] = F->sp[c] + apotime * (1.0 + 0.5 * o->btime);
+        }
+    }
+    orc_fields f = drv_fields(F);
+    orc_fluxuvw(ni, nj, nk, &f, F->vis, nj, 1, nij, F->ar1x, F->ar1y, F->ar1z, F->fx, F->ae, F->aw,
+                F->f1, F->su, F->sv, F->sw, F->bp, F->bt, F->bb, q->schemes, q->gds);
+    orc_fluxuvw(ni, nj, nk, &f, F->vis, 1, nij, nj, F->ar2x, F->ar2y, F->ar2z, F->fy, F->an, F->as,
+                F->f2, F->su, F->sv, F->sw, F->bp, F->bt, F->bb, q->schemes, q->gds);
+    orc_fluxuvw(ni, nj, nk, &f, F->vis, nij, nj, 1, F->ar3x, F->ar3y, F->ar3z, F->fz, F->at, F->ab,
+                F->f3, F->su, F->sv, F->sw, F->bp, F->bt, F->bb, q->schemes, q->gds);
+    const double dnx = q->nx, dny = q->ny, dnz = q->nz;
+    for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        const size_t inbc = IX(i, j, 3), ingc = inbc - nij;
+        uvw_wall(F, spv, inbc, ingc, ingc, F->fz[ingc], dnx, dny, dnz);
+    }
+    /* the top wall's cells are taken on the plane lk(nimm), as written */
+    for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+        const size_t inbc = IX(i, j, nimm), ingc = inbc + nij;
+        uvw_wall(F, spv, inbc, ingc, inbc, 1.0 - F->fz[inbc], dnx, dny, dnz);
+    }
+    if (o->cn)
+        for (size_t c = 0; c < n; ++c) {
+            F->ae[c] = 0.5 * F->ae[c];
+            F->an[c] = 0.5 * F->an[c];
+            F->at[c] = 0.5 * F->at[c];
+            F->aw[c] = 0.5 * F->aw[c];
+            F->as[c] = 0.5 * F->as[c];
+            F->ab[c] = 0.5 * F->ab[c];
+        }
+    double *X[3] = {F->u, F->v, F->w}, *S[3] = {F->su, F->sv, F->sw}, *D[3] = {F->ap, spv, F->sp};
+    const double *XO[3] = {F->uo, F->vo, F->wo};
+    double *AP[3] = {F->apu, F->apv, F->apw};
+    orc_sys s = drv_sys(F);
+    for (int m = 0; m < 3; ++m) {
+        for (int k = 2; k <= nkm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
+            const size_t c = IX(i, j, k);
+            if (o->cn) {
+                const double *xo = XO[m];
+                const double apotime = F->den[c] * F->vol[c] / o->timestep;
+                S[m][c] = S[m][c] + (F->ae[c] * xo[c + nj] + F->aw[c] * xo[c - nj] +
+                                     F->an[c] * xo[c + 1] + F->as[c] * xo[c - 1] +
+                                     F->at[c] * xo[c + nij] + F->ab[c] * xo[c - nij]) +
+                          (apotime - F->ae[c] - F->aw[c] - F->an[c] - F->as[c] - F->at[c] -
+                           F->ab[c]) * xo[c];
+                D[m][c] = D[m][c] + apotime;
+            }
+            F->ap[c] = F->ae[c] + F->aw[c] + F->an[c] + F->as[c] + F->at[c] + F->ab[c] + D[m][c];
+            F->ap[c] = F->ap[c] * q->urfr[m];
+            F->su[c] = S[m][c] + q->urfm[m] * F->ap[c] * X[m][c];
+            AP[m][c] = 1.0 / (F->ap[c] + SMALL);
+        }
+        orc_sipsol(ni, nj, nk, X[m], &s, o->alfa, q->sor[m], q->nsw[m], &reps[m]);
+    }
+}

@@ -194,3 +194,43 @@ void orc_piso(int ni, int nj, int nk, orc_flow *F, const orc_drv_opts *o, int pi
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* find_strain_rate.f90:21-79 after its gradients: strain and vorticity
+ * magnitudes on k = 2..nkm, i, j = 3..n?mm from gradu/gradv/gradw(3, nxyz) */
+void orc_strain_rate(int ni, int nj, int nk, const double *gradu, const double *gradv,
+                     const double *gradw, double *strain, double *vorticity);
+/* calc_vis_les.f90:45-151 (Smagorinsky with van Driest damping, then the
+ * wall-function viscosity of the bottom and top wall ghost planes).  nx, ny,
+ * nz are the module integers the wall distance is (as written) projected on. */
+void orc_calc_vis_les(int ni, int nj, int nk, const double *u, const double *v, const double *w,
+                      const double *vol, const double *den, const double *fz, const double *xc,
+                      const double *yc, const double *zc, const double *ar3x, const double *ar3y,
+                      const double *ar3z, const double *strain, const double *walldist,
+                      double *vis, double viscos, double densit, double cappa, double elog,
+                      double urfvis, int nx, int ny, int nz);
+
+/* calcuvw settings beyond the driver settings */
+typedef struct {
+    int schemes;                 /* fluxuvw scheme flags (ORC_UVW_*) */
+    double gds;                  /* gds(iu) */
+    double urfr[3], urfm[3];     /* urfr(iu..iw), urfm(iu..iw) */
+    double sor[3];
+    int nsw[3];
+    double gradpcmf;
+    int nx, ny, nz;              /* module integers of the wall loops */
+} orc_uvw_opts;
+
+/* calcuvw.f90:3-464 (Gauss gradients, isol = 1): momentum sources, fluxuvw
+ * for the three directions, the bottom / top wall terms, Crank-Nicolson
+ * halving, then per component under-relaxed assembly, apu/apv/apw and
+ * sipsol.  spv is the module array of the v equation's diagonal source.
+ * calcstress (Reynolds stresses of the EVM / ASM models, read by neither
+ * fluxuvw nor the assembly) is not part of it. */
+void orc_calcuvw(int ni, int nj, int nk, orc_flow *F, double *spv, const orc_drv_opts *o,
+                 const orc_uvw_opts *q, orc_report *reps);
+#ifdef __cplusplus
+}
+#endif

@@ -70,6 +70,13 @@ class DrvOpts(C.Structure):
                                           "gravx", "gravy", "gravz", "cumulative")]
 
 
+class UvwOpts(C.Structure):
+    """orc_uvw_opts: calcuvw's settings beyond the driver settings."""
+    _fields_ = [("schemes", C.c_int), ("gds", C.c_double), ("urfr", C.c_double * 3),
+                ("urfm", C.c_double * 3), ("sor", C.c_double * 3), ("nsw", C.c_int * 3),
+                ("gradpcmf", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
+
+
 _lib = None
 
 
@@ -111,6 +118,9 @@ def lib():
         L.orc_outbc.argtypes = [i, i, i, Fl, Op]
         L.orc_correct_bc.argtypes = [i, i, i, Fl]
         L.orc_gethbya.argtypes = [i, i, i, Fl, Op]
+        L.orc_strain_rate.argtypes = [i, i, i, dp, dp, dp, dp, dp]
+        L.orc_calc_vis_les.argtypes = [i, i, i] + [dp] * 15 + [d, d, d, d, d, i, i, i]
+        L.orc_calcuvw.argtypes = [i, i, i, Fl, dp, Op, C.POINTER(UvwOpts), C.POINTER(_Report)]
         _lib = L
     return _lib
 
@@ -323,6 +333,31 @@ def gethbya(dims, f, opts):
     lib().orc_gethbya(*dims, C.byref(st), C.byref(opts))
 
 
+def strain_rate(dims, gradu, gradv, gradw):
+    n = dims[0] * dims[1] * dims[2]
+    strain, vort = np.zeros(n), np.zeros(n)
+    lib().orc_strain_rate(*dims, _p(gradu), _p(gradv), _p(gradw), _p(strain), _p(vort))
+    return strain, vort
+
+
+LES_IN = ("u", "v", "w", "vol", "den", "fz", "xc", "yc", "zc", "ar3x", "ar3y", "ar3z")
+
+
+def calc_vis_les(dims, f, vis, par):
+    """calc_vis_les on the dict f (LES_IN arrays plus strain, walldist); vis
+    is updated in place; par: viscos, densit, cappa, elog, urf, nx, ny, nz"""
+    lib().orc_calc_vis_les(*dims, *[_p(f[q]) for q in LES_IN], _p(f["strain"]),
+                           _p(f["walldist"]), _p(vis), par["viscos"], par["densit"],
+                           par["cappa"], par["elog"], par["urf"], par["nx"], par["ny"], par["nz"])
+
+
+def calcuvw(dims, f, spv, opts, uvw):
+    st = _flow_struct(f)
+    reps = (_Report * 3)()
+    lib().orc_calcuvw(*dims, C.byref(st), _p(spv), C.byref(opts), C.byref(uvw), reps)
+    return _reports(reps, 3)
+
+
 # ----------------------------------------------------- reference driver --
 def ref_available():
     return os.path.exists(REF_DRIVER)
@@ -343,7 +378,8 @@ def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8, driver=REF_DRIVER):
             np.array([ni, nj, nk, nsw], dtype=np.int32).tofile(f)
             np.array([alfa, sor], dtype=np.float64).tofile(f)
             for a in arrays:
-                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+                dt = np.int32 if getattr(a, "dtype", None) == np.int32 else np.float64
+                np.ascontiguousarray(a, dtype=dt).tofile(f)
         subprocess.run([driver, mode, fin, fout, flog], check=True,
                        capture_output=True)
         raw = np.fromfile(fout, dtype=np.float64)
@@ -352,7 +388,7 @@ def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8, driver=REF_DRIVER):
         return [raw.copy()], None, log
     nout = {"sipsol": 9, "cgstab_sip": 9, "pcgsip": 9, "iccg": 9, "grad_gauss": 3,
             "bpres": 1, "fluxmass": 5, "piso_assemble": 5, "assemble_p": 5, "fluxmc": 5,
-            "fluxuvw": 8, "ghost": 7}[mode]
+            "fluxuvw": 8, "ghost": 7, "outbc": 15, "cbc": 5, "les": 1}[mode]
     outs = [raw[i * n:(i + 1) * n].copy() for i in range(nout)]
     resor = float(raw[nout * n]) if raw.size > nout * n else None
     return outs, resor, log

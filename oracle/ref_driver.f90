@@ -13,10 +13,13 @@
 !
 ! usage: ref_driver <mode> <in.bin> <out.bin> <log.txt>
 !   mode = sipsol | cgstab_sip | grad_gauss | bpres | fluxmass | piso_assemble
-!          | assemble_p | fluxmc | continuity | fluxuvw | ghost
+!          | assemble_p | fluxmc | continuity | fluxuvw | ghost | outbc | cbc | les
 ! in.bin  : int32 ni,nj,nk,nsw ; real64 alfa,sor ; then mode-specific arrays
 !           (stencil modes: nsw = direction 0 east, 1 north, 2 top; alfa = ipc;
-!            fluxuvw: nsw = direction + 8 * scheme flags, alfa = gds(iu))
+!            fluxuvw: nsw = direction + 8 * scheme flags, alfa = gds(iu);
+!            outbc: nsw = lout + 2 low + 4 loe + 8 los + 16 lon + 32 lob + 64 lot,
+!            alfa = flowin; les: alfa = urf(ivis), then real64 viscos, densit,
+!            cappa, elog and int32 nx, ny, nz)
 ! out.bin : mode-specific arrays, then real64 resor(ip) for the solvers
 !-----------------------------------------------------------------------
 program ref_driver
@@ -30,6 +33,7 @@ program ref_driver
   use gradients
   use bc
   use inlet
+  use boundc
   implicit none
   character(len=256) :: mode, fin, fout, flog
   integer :: ni_, nj_, nk_, nsw_, n
@@ -37,6 +41,8 @@ program ref_driver
   real(8), allocatable :: fi(:), d1(:), d2(:), d3(:), bsc(:), lbd(:, :), inl(:)
   integer :: idew, idns, idtb, inp, i, j, k, fl, dirn
   real(8) :: fmcor
+  real(8) :: lesp(4)
+  integer :: nxyz3(3)
 
   call get_command_argument(1, mode)
   call get_command_argument(2, fin)
@@ -70,7 +76,8 @@ program ref_driver
   allocate(x(n),y(n),z(n),xc(n),yc(n),zc(n),den(n),apu(n),apv(n),apw(n))
   allocate(u(n),v(n),w(n),p(n),sv(n),sw(n),f1(n),f2(n),f3(n))
   allocate(gradu(3,n),gradv(3,n),gradw(3,n),gradp(3,n))
-  allocate(vis(n), te(n), ed(n), t(n))
+  allocate(vis(n), te(n), ed(n), t(n), vart(n), con(n), utt(n), vtt(n), wtt(n))
+  allocate(strain(n), wallDistance(n))
 
   open(unit=66, file=trim(flog), status='replace')
   open(unit=11, file=trim(fout), access='stream', form='unformatted', status='replace')
@@ -178,6 +185,39 @@ program ref_driver
     read(10) f1, f2, f3
     call continuity_sums()
     write(11) globalContErr, sumLocalContErr
+  case ('outbc')
+    ! boundary-type arrays travel as real(8), padded to nxyz
+    allocate(lbd(n, 6))
+    allocate(lbw(nj*nk), lbe(nj*nk), lbs(ni*nk), lbn(ni*nk), lbb(ni*nj), lbt(ni*nj))
+    read(10) u, v, w, f1, f2, f3, den, ar1x, ar1y, ar1z, te, ed, vis, t, vart, con, utt, vtt, wtt
+    read(10) lbd
+    lbw = int(lbd(1:nj*nk, 1)); lbe = int(lbd(1:nj*nk, 2))
+    lbs = int(lbd(1:ni*nk, 3)); lbn = int(lbd(1:ni*nk, 4))
+    lbb = int(lbd(1:ni*nj, 5)); lbt = int(lbd(1:ni*nj, 6))
+    ijs = 0; iks = 0; jks = 0
+    lout = iand(nsw_, 1) /= 0
+    low = iand(nsw_, 2) /= 0
+    loe = iand(nsw_, 4) /= 0
+    los = iand(nsw_, 8) /= 0
+    lon = iand(nsw_, 16) /= 0
+    lob = iand(nsw_, 32) /= 0
+    lot = iand(nsw_, 64) /= 0
+    flowin = alfa_
+    call outbc
+    write(11) u, v, w, f1, f2, f3, te, ed, vis, t, vart, con, utt, vtt, wtt
+  case ('cbc')
+    read(10) u, v, w, f1, f2
+    call correctBoundaryConditions
+    write(11) u, v, w, f1, f2
+  case ('les')
+    read(10) u, v, w, vol, den, fz, xc, yc, zc, ar3x, ar3y, ar3z, vis, strain, wallDistance
+    read(10) lesp, nxyz3
+    viscos = lesp(1); densit = lesp(2); cappa = lesp(3); elog = lesp(4)
+    nx = nxyz3(1); ny = nxyz3(2); nz = nxyz3(3)
+    urf(ivis) = alfa_
+    zero = 0.0d0
+    call calc_vis_les
+    write(11) vis
   case default
     print *, 'unknown mode ', trim(mode)
     stop 2

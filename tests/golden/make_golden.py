@@ -178,6 +178,81 @@ def digest(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
 
 
+# outbc: (dims, seed, face flags lout + 2 low + 4 loe + ... + 64 lot)
+OUTBC_VARS = ("u", "v", "w", "f1", "f2", "f3", "te", "ed", "vis", "t", "vart", "con", "utt",
+              "vtt", "wtt")
+OUTBC_CASES = [((12, 10, 9), 701, 1 + 4), ((12, 10, 9), 702, 127), ((20, 17, 11), 703, 1 + 2 + 8 + 64),
+               ((12, 10, 9), 704, 126)]
+
+
+def outbc_inputs(dims, seed):
+    """u..wtt, den, ar1x..ar1z, boundary types (outlets common) and flowin"""
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = dims
+    n = ni * nj * nk
+    var = {q: rng.normal(size=n) for q in OUTBC_VARS}
+    var["u"] = 1.0 + 0.2 * rng.normal(size=n)
+    geo = {"den": rng.uniform(0.9, 1.1, n)}
+    for q in ("ar1x", "ar1y", "ar1z"):
+        geo[q] = rng.uniform(0.5, 1.5, n) if q == "ar1x" else rng.uniform(-0.1, 0.1, n)
+    lens = (nj * nk, nj * nk, ni * nk, ni * nk, ni * nj, ni * nj)
+    lb = [np.where(rng.uniform(size=m) < 0.7, 2, rng.integers(1, 5, m)).astype(np.int32)
+          for m in lens]
+    return var, geo, lb, float(rng.uniform(2.0, 5.0))
+
+
+def outbc_ref_arrays(dims, var, geo, lb):
+    n = dims[0] * dims[1] * dims[2]
+    pads = []
+    for a in lb:
+        x = np.zeros(n)
+        x[:a.size] = a
+        pads.append(x)
+    return [var[q] for q in ("u", "v", "w", "f1", "f2", "f3")] + \
+        [geo[q] for q in ("den", "ar1x", "ar1y", "ar1z")] + \
+        [var[q] for q in OUTBC_VARS[6:]] + [np.stack(pads, axis=1).T.copy().ravel()]
+
+
+CBC_CASES = [((12, 12, 9), 801), ((17, 17, 11), 802)]
+CBC_VARS = ("u", "v", "w", "f1", "f2")
+
+# calc_vis_les: (dims, seed); ni <= nk + 1 keeps the top wall plane lk(nimm)
+# and its ghost plane inside the array
+LES_CASES = [((12, 10, 12), 901), ((10, 13, 11), 902), ((16, 14, 16), 903)]
+LES_IN = ("u", "v", "w", "vol", "den", "fz", "xc", "yc", "zc", "ar3x", "ar3y", "ar3z", "vis",
+          "strain", "walldist")
+
+
+def les_inputs(dims, seed):
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = dims
+    n = ni * nj * nk
+    k, i, j = np.meshgrid(np.arange(nk), np.arange(ni), np.arange(nj), indexing="ij")
+    f = {}
+    sc = rng.uniform(0.01, 2.0, n)
+    f["u"], f["v"], f["w"] = sc * rng.normal(size=n), sc * rng.normal(size=n), 0.3 * rng.normal(size=n)
+    f["vol"] = rng.uniform(0.5, 1.5, n)
+    f["den"] = rng.uniform(0.95, 1.05, n)
+    f["fz"] = rng.uniform(0.3, 0.7, n)
+    f["xc"] = (i + 0.1 * rng.normal(size=i.shape)).ravel()
+    f["yc"] = (j + 0.1 * rng.normal(size=j.shape)).ravel()
+    f["zc"] = (k + 0.1 * rng.normal(size=k.shape)).ravel()
+    f["ar3x"], f["ar3y"] = rng.uniform(-0.1, 0.1, n), rng.uniform(-0.1, 0.1, n)
+    f["ar3z"] = rng.uniform(0.8, 1.2, n)
+    f["vis"] = rng.uniform(0.02, 0.05, n)
+    f["strain"] = rng.uniform(0.0, 3.0, n)
+    f["walldist"] = rng.uniform(0.05, 2.0, n)
+    par = dict(viscos=0.02, densit=1.0, cappa=0.41, elog=8.432, urf=0.7,
+               nx=ni - 2, ny=nj - 2, nz=nk - 2)
+    return f, par
+
+
+def les_ref_arrays(f, par):
+    return [f[q] for q in LES_IN] + [np.array([par["viscos"], par["densit"], par["cappa"],
+                                               par["elog"]]),
+                                     np.array([par["nx"], par["ny"], par["nz"]], dtype=np.int32)]
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
@@ -208,6 +283,25 @@ def main():
         name = f"{routine}_{dims[0]}x{dims[1]}x{dims[2]}"
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
         print(f"{name}: written")
+    for dims, seed, flags in OUTBC_CASES:
+        var, geo, lb, flowin = outbc_inputs(dims, seed)
+        outs, _, _ = O.run_ref("outbc", dims, outbc_ref_arrays(dims, var, geo, lb), nsw=flags,
+                               alfa=flowin)
+        rec = dict(dims=np.array(dims), seed=seed, flags=flags,
+                   **{q: a for q, a in zip(OUTBC_VARS, outs)})
+        name = f"outbc_{dims[0]}x{dims[1]}x{dims[2]}_{seed}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+    for dims, seed in CBC_CASES:
+        rng = np.random.default_rng(seed)
+        arrs = [rng.normal(size=dims[0] * dims[1] * dims[2]) for _ in CBC_VARS]
+        outs, _, _ = O.run_ref("cbc", dims, arrs)
+        rec = dict(dims=np.array(dims), seed=seed, **{q: a for q, a in zip(CBC_VARS, outs)})
+        np.savez_compressed(os.path.join(HERE, f"cbc_{dims[0]}x{dims[1]}x{dims[2]}.npz"), **rec)
+    for dims, seed in LES_CASES:
+        f, par = les_inputs(dims, seed)
+        outs, _, _ = O.run_ref("les", dims, les_ref_arrays(f, par), alfa=par["urf"])
+        rec = dict(dims=np.array(dims), seed=seed, vis=outs[0])
+        np.savez_compressed(os.path.join(HERE, f"les_{dims[0]}x{dims[1]}x{dims[2]}.npz"), **rec)
     for dims, seed in GHOST_CASES:
         var, geo, lb, inl = ghost_inputs(dims, seed)
         outs, _, _ = O.run_ref("ghost", dims, ghost_ref_arrays(dims, var, geo, lb, inl))

@@ -1,10 +1,13 @@
 """The pressure-correction drivers on the GPU (calcp, PISO, PIMPLE, outbc,
-correctBoundaryConditions, PISO_getHbyA) against the oracle's restatement,
-run with the GPU's compensated scalar products (plane_sums(2)).
+correctBoundaryConditions, PISO_getHbyA, calcuvw, find_strain_rate,
+calc_vis_les) against the oracle's restatement, run with the GPU's
+compensated scalar products (plane_sums(2)).
 
 Every array of the module state must be bit-identical, and so must the
 solver reports (iterations, initial and final L1 residuals).  The continuity
-sums are reductions in a different order (relative 1e-12)."""
+sums are reductions in a different order (relative 1e-12); the LES viscosity
+goes through pow / exp / log, which are the device library's on the GPU
+(a few ulp from the host's)."""
 import os
 import sys
 
@@ -136,3 +139,82 @@ def test_boundary_routines_bitwise(gpu, O, dims):
     for opts in (dict(bdf=1), dict(cn=1, buoy=1, boussinesq=1), dict(buoy=1)):
         fo, fg, _, _ = run_pair(gpu, O, dims, 33, "gethbya", **opts)
         check_state(fo, fg, f"gethbya {opts} {dims}")
+
+
+# ------------------------------------------------ calcuvw, strain rate, LES
+def _uvw_opts(cls, dims, schemes):
+    q = cls()
+    q.schemes, q.gds = schemes, 1.0
+    for m in range(3):
+        q.urfr[m], q.urfm[m], q.sor[m], q.nsw[m] = 1 / 0.7, 0.3, 1e-7, 60
+    q.nx, q.ny, q.nz = dims[0] - 2, dims[1] - 2, dims[2] - 2
+    q.gradpcmf = 0.05
+    return q
+
+
+@pytest.mark.parametrize("schemes,opts", [(1, dict(bdf=1)), (8, dict(bdf=1, cn=1)),
+                                          (16 | 32, dict(bdf=1, buoy=1, boussinesq=1, const_mflux=1)),
+                                          (0, dict(bdf=1, buoy=1, const_mflux=0))])
+def test_calcuvw_bitwise(gpu, O, schemes, opts):
+    import torch
+    dims = (18, 14, 18)
+    what = f"calcuvw schemes {schemes} {opts}"
+    print(what, flush=True)
+    f = FC.flow_case(dims, 41)
+    rng = np.random.default_rng(42)
+    for q in ("f1", "f2", "f3"):
+        f[q][:] = 0.1 * rng.normal(size=f[q].size)
+    f["pp"][:] = rng.normal(size=f["pp"].size)
+    oo, go = FC.opts(O.DrvOpts, alfa=0.92, **opts), FC.opts(gpu.DrvOpts, alfa=0.92, **opts)
+    qo, qg = _uvw_opts(O.UvwOpts, dims, schemes), _uvw_opts(gpu.UvwOpts, dims, schemes)
+    fo = {k: a.copy() for k, a in f.items()}
+    spv_o = np.zeros_like(f["u"])
+    with O.plane_sums(2):
+        rep_o = O.calcuvw(dims, fo, spv_o, oo, qo)
+    fd = {k: torch.from_numpy(a.copy()).cuda() for k, a in f.items()}
+    spv_g = torch.zeros(f["u"].size, dtype=torch.float64, device="cuda")
+    with gpu.Context(*dims, device_pointers=True) as ctx:
+        rep_g = ctx.calcuvw(fd, spv_g, go, qg)
+    print("  iterations", [r["iters"] for r in rep_g], flush=True)
+    for a, b in zip(rep_o, rep_g):
+        assert (a["iters"], a["resor"], a["resl"]) == (b["iters"], b["resor"], b["resl"]), what
+    check_state(fo, {k: t.cpu().numpy() for k, t in fd.items()}, what)
+    np.testing.assert_array_equal(spv_g.cpu().numpy(), spv_o)
+
+
+@pytest.mark.parametrize("dims", [(12, 10, 12), (18, 14, 18), (10, 13, 11)])
+def test_strain_and_les_viscosity(gpu, O, dims):
+    import torch
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_golden as MG
+    f = FC.flow_case(dims, 51)
+    n = f["u"].size
+    with gpu.Context(*dims, device_pointers=True) as ctx:
+        fd = {k: torch.from_numpy(a.copy()).cuda() for k, a in f.items()}
+        st = torch.zeros(n, dtype=torch.float64, device="cuda")
+        vo = torch.zeros(n, dtype=torch.float64, device="cuda")
+        ctx.find_strain_rate(fd, st, vo)
+        gu, gv, gw = (fd[q].cpu().numpy() for q in ("gradu", "gradv", "gradw"))
+        so, wo = O.strain_rate(dims, gu, gv, gw)
+        for q in ("u", "v", "w"):
+            gx, gy, gz = O.grad_gauss(dims, f[q], {k: f[k] for k in O.GEOM_FIELDS})
+            np.testing.assert_array_equal(fd["grad" + q].cpu().numpy(),
+                                          np.stack([gx, gy, gz], axis=1).ravel())
+        np.testing.assert_array_equal(st.cpu().numpy(), so)
+        np.testing.assert_array_equal(vo.cpu().numpy(), wo)
+        # LES viscosity on the golden-style inputs (the oracle is pinned to
+        # the reference build on them)
+        lf, par = MG.les_inputs(dims, 61)
+        vis_o = lf["vis"].copy()
+        O.calc_vis_les(dims, lf, vis_o, par)
+        fl = {k: torch.from_numpy(lf[k].copy()).cuda() for k in O.LES_IN + ("vis",)}
+        les = gpu.LesOpts()
+        les.viscos, les.densit, les.cappa, les.elog, les.urfvis = (par[k] for k in ("viscos", "densit", "cappa", "elog", "urf"))
+        les.nx, les.ny, les.nz = par["nx"], par["ny"], par["nz"]
+        ctx.calc_vis_les(fl, torch.from_numpy(lf["strain"]).cuda(), torch.from_numpy(lf["walldist"]).cuda(), les)
+        vis_g = fl["vis"].cpu().numpy()
+    # pow / exp / log are the device library's: a few ulp from the host's
+    same = np.mean(vis_g == vis_o)
+    print(f"LES viscosity {dims}: bit-identical on {same:.1%} of the cells, max rel "
+          f"{np.max(np.abs(vis_g - vis_o) / np.abs(vis_o)):.1e}", flush=True)
+    np.testing.assert_allclose(vis_g, vis_o, rtol=8e-16 * 8, atol=0)

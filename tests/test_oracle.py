@@ -564,3 +564,80 @@ def test_unet_oracle_gradient_finite_differences():
         lm, _ = U.loss_and_grad(p - h * e, x, t)
         assert (lp - lm) / (2 * h) == pytest.approx(g[i], rel=1e-4, abs=1e-9), f"layer {l}"
         o += co * 27 * ci + co
+
+
+# ------------------------------------------- outbc, correctBoundaryConditions, LES
+def _outbc_flow(var, geo):
+    f = {q: var[q].copy() for q in MG.OUTBC_VARS}
+    f.update({q: geo[q] for q in ("den", "ar1x", "ar1y", "ar1z")})
+    return f
+
+
+def oracle_outbc(O, dims, var, geo, lb, flags, flowin, opts_cls=None):
+    f = _outbc_flow(var, geo)
+    o = (opts_cls or O.DrvOpts)()
+    o.lout, o.low, o.loe, o.los, o.lon, o.lob, o.lot = [(flags >> b) & 1 for b in range(7)]
+    o.flowin = flowin
+    for name, a in zip(("lbw", "lbe", "lbs", "lbn", "lbb", "lbt"), lb):
+        setattr(o, name, a.ctypes.data)
+    O.outbc(dims, f, o)
+    return f
+
+
+@pytest.mark.parametrize("case", MG.OUTBC_CASES)
+def test_outbc_golden(O, case):
+    dims, seed, flags = case
+    g = np.load(os.path.join(GOLD, f"outbc_{dims[0]}x{dims[1]}x{dims[2]}_{seed}.npz"))
+    var, geo, lb, flowin = MG.outbc_inputs(dims, seed)
+    f = oracle_outbc(O, dims, var, geo, lb, flags, flowin)
+    for q in MG.OUTBC_VARS:
+        np.testing.assert_array_equal(f[q], g[q], err_msg=q)
+    changed = sum(not np.array_equal(g[q], var[q]) for q in MG.OUTBC_VARS)
+    # without lout the reference returns at once (outbc.f90:31)
+    assert changed >= 3 if flags & 1 else changed == 0
+
+
+@pytest.mark.parametrize("case", MG.CBC_CASES)
+def test_correct_bc_golden(O, case):
+    dims, seed = case
+    g = np.load(os.path.join(GOLD, f"cbc_{dims[0]}x{dims[1]}x{dims[2]}.npz"))
+    rng = np.random.default_rng(seed)
+    f = {q: rng.normal(size=dims[0] * dims[1] * dims[2]) for q in MG.CBC_VARS}
+    O.correct_bc(dims, f)
+    for q in MG.CBC_VARS:
+        np.testing.assert_array_equal(f[q], g[q], err_msg=q)
+
+
+@pytest.mark.parametrize("case", MG.LES_CASES)
+def test_les_golden(O, case):
+    dims, seed = case
+    g = np.load(os.path.join(GOLD, f"les_{dims[0]}x{dims[1]}x{dims[2]}.npz"))
+    f, par = MG.les_inputs(dims, seed)
+    vis = f["vis"].copy()
+    O.calc_vis_les(dims, f, vis, par)
+    np.testing.assert_array_equal(vis, g["vis"])
+    # both branches of the wall function occur
+    assert np.any(vis != f["vis"])
+
+
+def test_strain_rate_of_a_shear_flow(O):
+    """u = g y, v = w = 0 on a uniform grid: Green-Gauss is exact for linear
+    fields, so strain = |g| and vorticity = |g| / 2 in the interior."""
+    dims = (12, 11, 10)
+    ni, nj, nk = dims
+    n = ni * nj * nk
+    k, i, j = np.meshgrid(np.arange(nk), np.arange(ni), np.arange(nj), indexing="ij")
+    gam = 0.37
+    geom = {q: np.full(n, 0.5) for q in ("fx", "fy", "fz")}
+    for a in ("1", "2", "3"):
+        for c in "xyz":
+            geom["ar" + a + c] = np.full(n, 1.0 if "xyz"[int(a) - 1] == c else 0.0)
+    geom["vol"] = np.ones(n)
+    grads = []
+    for fld in (gam * j.ravel().astype(float), np.zeros(n), np.zeros(n)):
+        gx, gy, gz = O.grad_gauss(dims, fld, geom)
+        grads.append(np.stack([gx, gy, gz], axis=1).ravel())
+    strain, vort = O.strain_rate(dims, *grads)
+    inner = (slice(3, nk - 3), slice(3, ni - 3), slice(3, nj - 3))
+    np.testing.assert_allclose(strain.reshape(nk, ni, nj)[inner], gam, rtol=1e-14)
+    np.testing.assert_allclose(vort.reshape(nk, ni, nj)[inner], gam / 2, rtol=1e-14)

@@ -193,3 +193,55 @@ def test_piso_reference_cell_row(O, pimple):
     # PISO starts from p = 0 and ends with pp = p
     np.testing.assert_array_equal(f["pp"], f["p"])
     assert len(cont) == 2 and cont[1][2] == pytest.approx(cont[0][1] + cont[1][1], rel=1e-15)
+
+
+def uvw_case(O, dims, seed, **kw):
+    f = FC.flow_case(dims, seed)
+    rng = np.random.default_rng(seed + 100)
+    for q in ("f1", "f2", "f3"):
+        f[q][:] = 0.1 * rng.normal(size=f[q].size)
+    f["pp"][:] = rng.normal(size=f["pp"].size)
+    spv = np.zeros_like(f["u"])
+    o = FC.opts(O.DrvOpts, alfa=0.92, **kw)
+    q = O.UvwOpts()
+    q.schemes, q.gds = 1, 1.0
+    for m in range(3):
+        q.urfr[m], q.urfm[m], q.sor[m], q.nsw[m] = 1 / 0.7, 0.3, 1e-6, 50
+    q.nx, q.ny, q.nz = dims[0] - 2, dims[1] - 2, dims[2] - 2
+    return f, spv, o, q
+
+
+UVW_DIMS = (14, 12, 14)
+
+
+def test_calcuvw_diagonals_and_inverse(O):
+    """spv and sp receive the same additions (time term, wall shear); the
+    last assembled diagonal is the w equation's, (sum a_nb + sp) urfr(iw),
+    and apw = 1 / (ap + small); the w solve converged on it."""
+    f, spv, o, q = uvw_case(O, UVW_DIMS, 3, bdf=1)
+    reps = O.calcuvw(UVW_DIMS, f, spv, o, q)
+    assert all(r["converged"] for r in reps)
+    ni, nj, nk = UVW_DIMS
+    g = {k: grid(f[k], UVW_DIMS)[1:nk - 1, 2:ni - 2, 2:nj - 2] for k in
+         ("ae", "aw", "an", "as", "at", "ab", "ap", "sp", "apw")}
+    np.testing.assert_array_equal(grid(spv, UVW_DIMS)[1:nk - 1, 2:ni - 2, 2:nj - 2], g["sp"])
+    s = g["ae"] + g["aw"] + g["an"] + g["as"] + g["at"] + g["ab"] + g["sp"]
+    np.testing.assert_array_equal(g["ap"], s * q.urfr[2])
+    np.testing.assert_array_equal(g["apw"], 1.0 / (g["ap"] + float(np.float32(1e-30))))
+    # away from the walls spv is the BDF time term alone; the wall shear adds
+    # to it on the first cells above the bottom wall
+    tt = grid(f["den"] * f["vol"] / o.timestep * (1.0 + 0.5 * o.btime), UVW_DIMS)
+    sg = grid(spv, UVW_DIMS)
+    np.testing.assert_array_equal(sg[4, 2:ni - 2, 2:nj - 2], tt[4, 2:ni - 2, 2:nj - 2])
+    assert np.all(sg[2, 2:ni - 2, 2:nj - 2] > tt[2, 2:ni - 2, 2:nj - 2])
+
+
+def test_calcuvw_crank_nicolson_halves_the_coefficients(O):
+    fa, spa, oa, qa = uvw_case(O, UVW_DIMS, 4, bdf=1)
+    fb, spb, ob, qb = uvw_case(O, UVW_DIMS, 4, bdf=1, cn=1)
+    O.calcuvw(UVW_DIMS, fa, spa, oa, qa)
+    O.calcuvw(UVW_DIMS, fb, spb, ob, qb)
+    for k in ("ae", "aw", "an", "as", "at", "ab"):
+        np.testing.assert_array_equal(fb[k], 0.5 * fa[k], err_msg=k)
+    # the Crank-Nicolson time term joins the diagonal source of each equation
+    assert np.all(grid(spb, UVW_DIMS)[2:-2, 2:-2, 2:-2] > grid(spa, UVW_DIMS)[2:-2, 2:-2, 2:-2])
