@@ -44,6 +44,11 @@ struct cap_ctx {
     double *S[S_COUNT];
     float *F[F_COUNT];
     size_t fcap;
+    // the forward sweep's SIP factors {bb, bw, bs, bp} packed four to an
+    // element (FP64 and the FP32 copy of the mixed-precision solver)
+    double *pkd;
+    float *pkf;
+    int bad_launch;               // a sweep was queued without its packed factors
     double *G[CAP_NSTAGE];      // stencil staging (reference layout)
     size_t Gsz[CAP_NSTAGE];
     void *msg;                  // wavefront message slots

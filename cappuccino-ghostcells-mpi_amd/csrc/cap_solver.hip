@@ -17,6 +17,7 @@
 #include <string.h>
 #include <math.h>
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 #include "cap_kernels.h"
 #include "cap_amd.h"
@@ -168,7 +169,43 @@ struct WaveIO {
     V *dep[Op::ND];
     V *xo[Op::NX > 0 ? Op::NX : 1];
     V *msg;                         // message slots (filled with the sentinel)
+    const V *pk = nullptr;          // inputs 1..PKN packed four to an element
 };
+
+// how many inputs (from x[1]) an op reads packed four to an element
+template <class Op, class = void>
+struct PkN {
+    static constexpr int v = 0;
+};
+template <class Op>
+struct PkN<Op, std::void_t<decltype(Op::PKN)>> {
+    static constexpr int v = Op::PKN;
+};
+
+// one element of four packed values: two 16-byte loads (FP64) or one (FP32)
+__device__ __forceinline__ void load4(const double *p, double (&v)[4])
+{
+    const double2 a = *reinterpret_cast<const double2 *>(p);
+    const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void load4(const float *p, float (&v)[4])
+{
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+
+// pack four tiled vectors (d may be null: zeros) into out[4 e + q]
+template <class V>
+__global__ void k_pack4(size_t n, const V *a, const V *b, const V *c, const V *d, V *out)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        out[4 * e] = a[e];
+        out[4 * e + 1] = b[e];
+        out[4 * e + 2] = c[e];
+        out[4 * e + 3] = d ? d[e] : V(0);
+    }
+}
 
 struct WaveSync {
     unsigned long long *claim;      // tile claim counter
@@ -246,6 +283,7 @@ template <class Op, bool FWD, int FIN>
 __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io, WaveSync sy)
 {
     using V = typename Op::T;
+    constexpr int PKN = PkN<Op>::v;
     constexpr int ND = Op::ND, NIN = Op::NIN, U = ND == 1 && NIN <= 5 ? 16 : 8;
     constexpr int DM = 4;   // messages are read DM steps ahead of use
     constexpr int NXS = Op::NX > 0 ? Op::NX : 1, NRS = Op::NR > 0 ? Op::NR : 1;
@@ -401,10 +439,21 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                 hhi[q] = depp[q][e1];
             }
         }
+        const V *const pkp = io.pk;
         auto fetch_in = [&](int t, V (&x)[NIN]) {
             const size_t e = elem_at(tb, step_slot<FWD>(L, t), lane);
+            if constexpr (PKN > 0) {
+                x[0] = inp[0][e];
+                V p4[4];
+                load4(pkp + 4 * e, p4);
 #pragma unroll
-            for (int q = 0; q < NIN; ++q) x[q] = inp[q][e];
+                for (int q = 1; q <= PKN; ++q) x[q] = p4[q - 1];
+#pragma unroll
+                for (int q = PKN + 1; q < NIN; ++q) x[q] = inp[q][e];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NIN; ++q) x[q] = inp[q][e];
+            }
         };
 #pragma unroll
         for (int d = 0; d < U; ++d) fetch_in(d, inb[d]);
@@ -493,6 +542,9 @@ static int ensure_tiled(cap_ctx *ctx, size_t n)
     const size_t stride = ((n * sizeof(double) + 65535) / 65536) * 65536 + 4352;
     HIPCHK(hipMalloc(&ctx->tpool, stride * T_COUNT));
     for (int v = 0; v < T_COUNT; ++v) ctx->T[v] = (double *)((char *)ctx->tpool + stride * v);
+    if (ctx->pkd) { (void)hipFree(ctx->pkd); ctx->pkd = nullptr; }
+    HIPCHK(hipMalloc(&ctx->pkd, 4 * n * sizeof(double)));
+
     ctx->tcap = n;
     return CAP_OK;
 }
@@ -530,6 +582,10 @@ static void launch_wave(cap_ctx *ctx, const Lay &L, const Op &op, WaveIO<Op> io,
                         DevState *st = nullptr)
 {
     using V = typename Op::T;
+    if (PkN<Op>::v > 0 && !io.pk) {     // never queue a sweep without its packed factors
+        ctx->bad_launch = 1;
+        return;
+    }
     WaveSync sy;
     sy.claim = ctx->claim;
     sy.claim_base = ctx->claim_base;
@@ -664,6 +720,11 @@ static void sip_factor(cap_ctx *ctx, const Lay &L, double alfa)
     io.dep[0] = T[T_BN]; io.dep[1] = T[T_BE]; io.dep[2] = T[T_BT];
     io.xo[0] = T[T_BB]; io.xo[1] = T[T_BW]; io.xo[2] = T[T_BS]; io.xo[3] = T[T_BP];
     launch_wave<OpSipFactor, true, F_NONE>(ctx, L, op, io);
+    // the forward sweep reads its four factors packed: {bb, bw, bs, bp}
+    const int gp = (int)std::min<size_t>((L.n + BLOCK - 1) / BLOCK, 8192);
+    hipLaunchKernelGGL(k_pack4<double>, dim3(gp), dim3(BLOCK), 0, ctx->stream, L.n,
+                       (const double *)T[T_BB], (const double *)T[T_BW], (const double *)T[T_BS],
+                       (const double *)T[T_BP], ctx->pkd);
 }
 
 static void sip_fwd(cap_ctx *ctx, const Lay &L, const double *rhs, double *z)
@@ -673,6 +734,7 @@ static void sip_fwd(cap_ctx *ctx, const Lay &L, const double *rhs, double *z)
     io.in[0] = rhs; io.in[1] = T[T_BB]; io.in[2] = T[T_BW]; io.in[3] = T[T_BS]; io.in[4] = T[T_BP];
     io.dep[0] = z;
     io.xo[0] = nullptr;
+    io.pk = ctx->pkd;
     launch_wave<OpSipFwd, true, F_NONE>(ctx, L, OpSipFwd(), io);
 }
 
@@ -722,6 +784,10 @@ static int writeback(cap_ctx *ctx, const Lay &L, double *fi_host, const cap_coef
 
 static int finish_report(cap_ctx *ctx, const DevState &h, cap_report *rep, float ms)
 {
+    if (ctx->bad_launch) {
+        ctx->bad_launch = 0;
+        return fail(ctx, CAP_ERR_ARG, "a SIP sweep was queued without its packed factors");
+    }
     if (h.err == CAP_ERR_TIMEOUT) return fail(ctx, CAP_ERR_TIMEOUT, "wavefront dependency wait expired");
     if (h.err == CAP_ERR_ARG) return fail(ctx, CAP_ERR_ARG, "wavefront tile claim out of range");
     if (rep) {
@@ -980,6 +1046,8 @@ static int ensure_f32(cap_ctx *ctx, size_t n)
     for (int v = 0; v < F_COUNT; ++v)
         if (ctx->F[v]) { (void)hipFree(ctx->F[v]); ctx->F[v] = nullptr; }
     for (int v = 0; v < F_COUNT; ++v) HIPCHK(hipMalloc(&ctx->F[v], n * sizeof(float)));
+    if (ctx->pkf) { (void)hipFree(ctx->pkf); ctx->pkf = nullptr; }
+    HIPCHK(hipMalloc(&ctx->pkf, 4 * n * sizeof(float)));
     ctx->fcap = n;
     return CAP_OK;
 }
@@ -1014,6 +1082,9 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
     const int gl = (int)std::min<size_t>((L.n + BLOCK - 1) / BLOCK, 8192);
     for (int v = 0; v < 14; ++v)
         hipLaunchKernelGGL(k_to_f32, dim3(gl), dim3(BLOCK), 0, s, (const double *)T[c64[v]], F[c32[v]], L.n);
+    hipLaunchKernelGGL(k_pack4<float>, dim3(gl), dim3(BLOCK), 0, s, L.n, (const float *)F[F_BB],
+                       (const float *)F[F_BW], (const float *)F[F_BS], (const float *)F[F_BP], ctx->pkf);
+
     Coef7T<float> A32;
     A32.ae = F[F_AE]; A32.aw = F[F_AW]; A32.an = F[F_AN]; A32.as = F[F_AS];
     A32.at = F[F_AT]; A32.ab = F[F_AB]; A32.ap = F[F_AP];
@@ -1046,6 +1117,7 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
             f.in[4] = F[F_BP];
             f.dep[0] = F[F_Z];
             f.xo[0] = nullptr;
+            f.pk = ctx->pkf;
             launch_wave<OpSipFwdT<float>, true, F_NONE>(ctx, L, OpSipFwdT<float>(), f, st2);
             WaveIO<OpSipBwdDotT<float>> b;
             b.in[0] = F[F_Z]; b.in[1] = F[F_BN]; b.in[2] = F[F_BE]; b.in[3] = F[F_BT];
@@ -1132,7 +1204,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                     io.in[4] = T[T_RES];
                     io.dep[0] = T[T_ZK];
                     io.xo[0] = nullptr;
-                    launch_wave<OpSipBwdDot, false, F_PCG_SK>(ctx, L, OpSipBwdDot(), io);
+                                launch_wave<OpSipBwdDot, false, F_PCG_SK>(ctx, L, OpSipBwdDot(), io);
                 }
                 if (q == 2)
                     hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L,
@@ -1152,6 +1224,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                     f.in[4] = F[F_BP];
                     f.dep[0] = F[F_Z];
                     f.xo[0] = nullptr;
+                    f.pk = ctx->pkf;
                     launch_wave<OpSipFwdT<float>, true, F_NONE>(ctx, L, OpSipFwdT<float>(), f);
                 }
                 if (q == 1) {
@@ -1160,7 +1233,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                     b.in[4] = F[F_R];
                     b.dep[0] = F[F_Z];
                     b.xo[0] = nullptr;
-                    launch_wave<OpSipBwdDotT<float>, false, F_PCG_SK>(ctx, L, OpSipBwdDotT<float>(), b);
+                            launch_wave<OpSipBwdDotT<float>, false, F_PCG_SK>(ctx, L, OpSipBwdDotT<float>(), b);
                 }
                 if (q == 2)
                     hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,

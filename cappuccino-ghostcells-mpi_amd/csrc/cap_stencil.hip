@@ -1339,4 +1339,6 @@ void cap_free_extra(cap_ctx *ctx)
     if (ctx->scr) (void)hipFree(ctx->scr);
     for (int q = 0; q < CAP_NSTAGE; ++q) if (ctx->G[q]) (void)hipFree(ctx->G[q]);
     for (int q = 0; q < F_COUNT; ++q) if (ctx->F[q]) (void)hipFree(ctx->F[q]);
+    if (ctx->pkd) (void)hipFree(ctx->pkd);
+    if (ctx->pkf) (void)hipFree(ctx->pkf);
 }

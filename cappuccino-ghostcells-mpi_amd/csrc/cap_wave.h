@@ -159,6 +159,7 @@ template <class V>
 struct OpSipFwdT {
     using T = V;
     static constexpr int NIN = 5, ND = 1, NX = 0, NR = 0;
+    static constexpr int PKN = 4;   // x[1..4] come packed: {bb, bw, bs, bp}
     CAP_HD static void compute(const V *x, const V *dk, const V *di, const V *dj, V *d, V *,
                                double *)
     {
