@@ -139,7 +139,7 @@ def main():
         elif other == "iccg":
             r = ctx.iccg(fi, coef, sor=1e-8, nsw=20000)
         elif other == "mixed":
-            r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=1e-3)
+            r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=1e-4)
         elif other == "unet":
             solvers["unet"] = _bench_unet(cap, torch)
             continue
