@@ -15,10 +15,11 @@
 //
 // Kernels:
 //  * k_conv: implicit-GEMM convolution on MFMA (v_mfma_f32_16x16x32_bf16):
-//    a wave computes 16 voxels x (16 * NT) output channels; K = 27 taps x
-//    input channels in steps of 32.  Used forward, and with the weights
-//    mirrored in the taps and transposed in the channels for the gradient
-//    with respect to the input.
+//    a workgroup computes a 16 x 4 x 4 voxel brick for all output channels
+//    from an LDS-staged halo brick of 32 input channels at a time; K = 27
+//    taps x input channels in steps of 32.  Used forward, and with the
+//    weights mirrored in the taps and transposed in the channels for the
+//    gradient with respect to the input.
 //  * k_wgrad: weight and bias gradients (FP32 multiply-add per tap and
 //    channel pair, summed over voxel blocks with atomics).
 //  * pooling, upsampling, gradient combination with the ReLU mask, loss,
@@ -76,79 +77,128 @@ __device__ __forceinline__ void tap_off(int tap, int &dz, int &dy, int &dx)
     dx = tap % 3 - 1;
 }
 
-// Y = conv(X, W) (+ bias, ReLU) on a D x H x W grid, 16 voxels per wave.
-//  X: bf16 [nvox][cx] (channels xoff.. of stride cx); k = tap*cinp + ci
-//  Wt: bf16 [ntile*16][kp]  (row n = output channel, zero-padded)
+// Y = conv(X, W) (+ bias, ReLU) on a D x H x W grid as an implicit GEMM on
+// the matrix cores, with the input staged in LDS.
+//  X: bf16 [nvox][cx], channels xoff .. xoff + cinp - 1 (cinp a multiple of 8)
+//  Wt: bf16 [ntile*16][kp], row n = output channel, k = tap*cinp + ci
 //  Y: bf16 [nvox][cy] at channel offset yoff (yf: fp32 [nvox][cout] instead)
 //  mask (optional): Y *= (M[v][moff + co] > 0), M of stride cm
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv(int D, int H, int W, const bf16_t *__restrict__ X,
+// A workgroup (4 waves) computes an output brick of BX x BY x BZ voxels; wave
+// w the plane z = w of it as BY M-tiles of 16 voxels (one row y each), for
+// all 16*NT output channels.  The input channels go in chunks of CC = 32:
+// the chunk's halo brick ((BX+2) x (BY+2) x (BZ+2) voxels, zero outside the
+// grid and beyond cinp) is staged in LDS once and read by all 27 taps, so
+// each input value is read from memory about 1.7 times instead of 27.
+constexpr int BX = 16, BY = 4;
+constexpr int HX = BX + 2, HY = BY + 2;
+constexpr int CC = 32;       // input channels per chunk (one MFMA K step)
+constexpr int VST = 40;      // LDS stride of a voxel in bf16 (80 bytes: spreads the banks)
+
+template <int NT, int BZ>
+__global__ __launch_bounds__(64 * BZ) void k_conv(int D, int H, int W, const bf16_t *__restrict__ X,
                                               int cx, int xoff, int cinp, const bf16_t *__restrict__ Wt,
                                               int kp, const float *__restrict__ bias, int relu,
                                               bf16_t *__restrict__ Y, int cy, int yoff, int cout,
                                               float *__restrict__ yf, const bf16_t *__restrict__ M,
                                               int cm, int moff)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long nvox = (long)D * H * W;
-    const long m0 = ((long)blockIdx.x * 4 + wave) * 16;
-    if (m0 >= nvox) return;
-    const long va = m0 + (lane & 15);        // voxel of this lane's A row
-    const int kg = lane >> 4;                // k group (8 values)
-    int z = 0, y = 0, x = 0;
-    const bool vin = va < nvox;
-    if (vin) {
-        x = (int)(va % W);
-        y = (int)((va / W) % H);
-        z = (int)(va / ((long)W * H));
-    }
-    f32x4 acc[NT];
+    constexpr int HZ = BZ + 2, HV = HX * HY * HZ, NTH = 64 * BZ;
+    __shared__ __attribute__((aligned(16))) bf16_t lds[HV * VST];
+    const int nbx = (W + BX - 1) / BX, nby = (H + BY - 1) / BY;
+    const int bid = blockIdx.x;
+    const int x0 = (bid % nbx) * BX, y0 = ((bid / nbx) % nby) * BY, z0 = (bid / (nbx * nby)) * BZ;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int kg = lane >> 4, mrow = lane & 15;
+    f32x4 acc[BY][NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int nk = kp / 32;
-    for (int kc = 0; kc < nk; ++kc) {
-        const int k = kc * 32 + kg * 8;
-        const int tap = k / cinp, ci = k % cinp;
-        bf16x8 a;
-        {
-            uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-            if (vin && tap < 27) {
-                int dz, dy, dx;
-                tap_off(tap, dz, dy, dx);
-                const int zz = z + dz, yy = y + dy, xx = x + dx;
-                if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                    const long nb = ((long)zz * H + yy) * W + xx;
-                    raw = *reinterpret_cast<const uint4 *>(X + nb * cx + xoff + ci);
+    for (int m = 0; m < BY; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nchunk = (cinp + CC - 1) / CC;
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const int c0 = ch * CC;
+        __syncthreads();   // the previous chunk's readers are done
+        // the halo brick, in batches of 4 loads per thread issued before their stores
+        constexpr int NQ = HV * (CC / 8), NB = 4;
+        for (int q0 = 0; q0 < NQ; q0 += NB * NTH) {
+            uint4 raw[NB];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const int q = q0 + u * NTH + (int)threadIdx.x;
+                raw[u] = make_uint4(0u, 0u, 0u, 0u);
+                const int hv = q / (CC / 8), g = q % (CC / 8);
+                const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+                const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+                const int ci = c0 + 8 * g;
+                if (q < NQ && ci < cinp && gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D) {
+                    const long v = ((long)gz * H + gy) * W + gx;
+                    raw[u] = *reinterpret_cast<const uint4 *>(X + v * cx + xoff + ci);
                 }
             }
-            memcpy(&a, &raw, 16);
-        }
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const uint4 rb = *reinterpret_cast<const uint4 *>(Wt + (size_t)(t * 16 + (lane & 15)) * kp + k);
-            bf16x8 b;
-            memcpy(&b, &rb, 16);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            for (int u = 0; u < NB; ++u) {
+                const int q = q0 + u * NTH + (int)threadIdx.x;
+                if (q < NQ) *reinterpret_cast<uint4 *>(lds + (q / (CC / 8)) * VST + 8 * (q % (CC / 8))) = raw[u];
+            }
+        }
+        __syncthreads();
+        const bool kin = c0 + 8 * kg < cinp;
+        for (int tap = 0; tap < 27; ++tap) {
+            int dz, dy, dx;
+            tap_off(tap, dz, dy, dx);
+            const int k = tap * cinp + c0 + 8 * kg;
+            bf16x8 bv[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                uint4 rb = make_uint4(0u, 0u, 0u, 0u);
+                if (kin) rb = *reinterpret_cast<const uint4 *>(Wt + (size_t)(t * 16 + mrow) * kp + k);
+                memcpy(&bv[t], &rb, 16);
+            }
+#pragma unroll
+            for (int m = 0; m < BY; ++m) {
+                const int hv = ((w + 1 + dz) * HY + (m + 1 + dy)) * HX + (mrow + 1 + dx);
+                const uint4 ra = *reinterpret_cast<const uint4 *>(lds + hv * VST + 8 * kg);
+                bf16x8 av;
+                memcpy(&av, &ra, 16);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[t], acc[m][t], 0, 0, 0);
+            }
         }
     }
-    // D[m][n]: lane holds rows m = 4*(lane/16) + j, column n = lane%16
+    // D[row][n]: lane holds rows 4*(lane/16) + j (voxel x = x0 + row), column n = lane%16
+    const int z = z0 + w;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + (lane & 15);
-        if (co >= cout) continue;
-        const float bv = bias ? bias[co] : 0.f;
+    for (int m = 0; m < BY; ++m) {
+        const int y = y0 + m;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long v = m0 + 4 * (lane >> 4) + j;
-            if (v >= nvox) continue;
-            float r = acc[t][j] + bv;
-            if (relu) r = r > 0.f ? r : 0.f;
-            if (M) r = bf2f(M[v * cm + moff + co]) > 0.f ? r : 0.f;
-            if (yf) yf[v * cout + co] = r;
-            else Y[v * cy + yoff + co] = f2bf(r);
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + mrow;
+            if (co >= cout || y >= H || z >= D) continue;
+            const float bvv = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int x = x0 + 4 * kg + j;
+                if (x >= W) continue;
+                const long v = ((long)z * H + y) * W + x;
+                float r = acc[m][t][j] + bvv;
+                if (relu) r = r > 0.f ? r : 0.f;
+                if (M) r = bf2f(M[v * cm + moff + co]) > 0.f ? r : 0.f;
+                if (yf) yf[v * cout + co] = r;
+                else Y[v * cy + yoff + co] = f2bf(r);
+            }
         }
     }
 }
+
+int conv_grid(int D, int H, int W, int bz)
+{
+    return ((W + BX - 1) / BX) * ((H + BY - 1) / BY) * ((D + bz - 1) / bz);
+}
+
+// bricks 4 planes deep (4 waves) where they give enough workgroups, else 1
+// plane (1 wave): small grids are latency-bound and want more workgroups
+int conv_bz(int D, int H, int W) { return conv_grid(D, H, W, 4) >= 512 ? 4 : 1; }
 
 // dW[co][tap][ci] += sum_v X[v + off(tap)][ci] * G[v][co];  db[co] += sum_v G[v][co]
 // block: (voxel chunk of 128, tap); threads loop over (ci, co) pairs
@@ -204,13 +254,16 @@ __global__ void k_pool(int Dc, int Hc, int Wc, int C, const bf16_t *in, int cin_
 __global__ void k_up(int D, int H, int W, int C, const bf16_t *in, int cin_, bf16_t *out, int cout_,
                      int ooff)
 {
-    const long n = (long)D * H * W * C;
+    // 8 channels (16 bytes) per thread: C, cin_, cout_ and ooff are multiples of 8
+    const int C8 = C / 8;
+    const long n = (long)D * H * W * C8;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C);
-        const long v = q / C;
+        const int c = 8 * (int)(q % C8);
+        const long v = q / C8;
         const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
         const long cv = ((long)(z >> 1) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
-        out[v * cout_ + ooff + c] = in[cv * cin_ + c];
+        *reinterpret_cast<uint4 *>(out + v * cout_ + ooff + c) =
+            *reinterpret_cast<const uint4 *>(in + cv * cin_ + c);
     }
 }
 
@@ -569,16 +622,22 @@ void conv_launch(cap_unet *n, int l, const bf16_t *X, int cx, int xoff, bf16_t *
     const ConvL &c = n->L[l];
     const int lv = c.level;
     const int D = n->nz >> lv, H = n->ny >> lv, W = n->nx >> lv;
-    const long nvox = (long)D * H * W;
-    const int grid = (int)((nvox + 63) / 64);
+    const int bz = conv_bz(D, H, W);
+    const int grid = conv_grid(D, H, W, bz);
     const float *bias = n->p + c.boff;
     hipStream_t s = n->ctx->stream;
-    switch (c.coutp / 16) {
-    case 1: hipLaunchKernelGGL(k_conv<1>, dim3(grid), dim3(256), 0, s, D, H, W, X, cx, xoff, c.cinp, n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0); break;
-    case 2: hipLaunchKernelGGL(k_conv<2>, dim3(grid), dim3(256), 0, s, D, H, W, X, cx, xoff, c.cinp, n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0); break;
-    case 3: hipLaunchKernelGGL(k_conv<3>, dim3(grid), dim3(256), 0, s, D, H, W, X, cx, xoff, c.cinp, n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0); break;
-    case 4: hipLaunchKernelGGL(k_conv<4>, dim3(grid), dim3(256), 0, s, D, H, W, X, cx, xoff, c.cinp, n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0); break;
+#define CONV_FWD(NT, BZ)                                                                            \
+    hipLaunchKernelGGL((k_conv<NT, BZ>), dim3(grid), dim3(64 * BZ), 0, s, D, H, W, X, cx, xoff, c.cinp, \
+                       n->Wt[l], c.kp, bias, c.relu, Y, cy, yoff, c.cout, yf, (const bf16_t *)nullptr, 0, 0)
+    const int nt = c.coutp / 16;
+    if (bz == 4) {
+        switch (nt) { case 1: CONV_FWD(1, 4); break; case 2: CONV_FWD(2, 4); break;
+                      case 3: CONV_FWD(3, 4); break; case 4: CONV_FWD(4, 4); break; }
+    } else {
+        switch (nt) { case 1: CONV_FWD(1, 1); break; case 2: CONV_FWD(2, 1); break;
+                      case 3: CONV_FWD(3, 1); break; case 4: CONV_FWD(4, 1); break; }
     }
+#undef CONV_FWD
 }
 
 // gradient with respect to the layer input: G (layer output gradient, stride
@@ -589,17 +648,24 @@ void conv_bwd_launch(cap_unet *n, int l, const bf16_t *G, bf16_t *dX, int cdx, c
     const ConvL &c = n->L[l];
     const int lv = c.level;
     const int D = n->nz >> lv, H = n->ny >> lv, W = n->nx >> lv;
-    const long nvox = (long)D * H * W;
-    const int grid = (int)((nvox + 63) / 64);
+    const int bz = conv_bz(D, H, W);
+    const int grid = conv_grid(D, H, W, bz);
     hipStream_t s = n->ctx->stream;
-    switch (c.cinp16 / 16) {
-    case 1: hipLaunchKernelGGL(k_conv<1>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
-    case 2: hipLaunchKernelGGL(k_conv<2>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
-    case 3: hipLaunchKernelGGL(k_conv<3>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
-    case 4: hipLaunchKernelGGL(k_conv<4>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
-    case 5: hipLaunchKernelGGL(k_conv<5>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
-    case 7: hipLaunchKernelGGL(k_conv<7>, dim3(grid), dim3(256), 0, s, D, H, W, G, c.coutp8, 0, c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin, (float *)nullptr, M, cm, moff); break;
+#define CONV_BWD(NT, BZ)                                                                            \
+    hipLaunchKernelGGL((k_conv<NT, BZ>), dim3(grid), dim3(64 * BZ), 0, s, D, H, W, G, c.coutp8, 0,      \
+                       c.coutp8, n->Wb[l], c.kpb, (const float *)nullptr, 0, dX, cdx, 0, c.cin,         \
+                       (float *)nullptr, M, cm, moff)
+    const int nt = c.cinp16 / 16;
+    if (bz == 4) {
+        switch (nt) { case 1: CONV_BWD(1, 4); break; case 2: CONV_BWD(2, 4); break;
+                      case 3: CONV_BWD(3, 4); break; case 4: CONV_BWD(4, 4); break;
+                      case 5: CONV_BWD(5, 4); break; case 7: CONV_BWD(7, 4); break; }
+    } else {
+        switch (nt) { case 1: CONV_BWD(1, 1); break; case 2: CONV_BWD(2, 1); break;
+                      case 3: CONV_BWD(3, 1); break; case 4: CONV_BWD(4, 1); break;
+                      case 5: CONV_BWD(5, 1); break; case 7: CONV_BWD(7, 1); break; }
     }
+#undef CONV_BWD
 }
 
 void wgrad_launch(cap_unet *n, int l, const bf16_t *X, int cx, int xoff, const bf16_t *G)
