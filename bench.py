@@ -51,10 +51,11 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-iters", type=int, default=24,
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
-    ap.add_argument("--others", default="cgstab_sip,iccg,mixed,unet,piso",
+    ap.add_argument("--others", default="cgstab_sip,iccg,mixed,unet,piso,host",
                     help="comma list of extra solves to time once each (unet: the learned "
                          "guess's forward pass and training step on 128x96x64; piso: time "
-                         "steps of LES + calcuvw + PISO on the 256^3 channel)")
+                         "steps of LES + calcuvw + PISO on the 256^3 channel; host: the same "
+                         "p' solve through host arrays, PCIe copies included)")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: 3 PCG-SIP iterations, then "
                          "a 1 GiB device copy as the byte-counter calibration; prints one line")
@@ -146,6 +147,9 @@ def main():
         elif other == "piso":
             solvers["piso_step"] = _bench_piso(cap, torch, n)
             continue
+        elif other == "host":
+            solvers["pcgsip_host_pointers"] = _bench_host(cap, torch, arrs, dims)
+            continue
         else:
             continue
         torch.cuda.synchronize()
@@ -199,6 +203,29 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def _bench_host(cap, torch, arrs, dims):
+    """The same p' solve through a host-pointer context (the drop-in's mode:
+    the coefficients go up and fi, res and the factors come back over PCIe
+    on every call): ms per call, copies included."""
+    host = {f: a.cpu().numpy() for f, a in arrs.items()}
+    npad = dims[0] * dims[1] * dims[2]
+    import numpy as np
+    extra = {f: np.zeros(npad) for f in ("res", "bb", "bs", "bw", "bp", "bn", "be", "bt")}
+    coef = cap.Coef(**host, **extra)
+    fi = np.zeros(npad)
+    with cap.Context(*dims) as hctx:
+        hctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=20000)     # first call allocates
+        fi[:] = 0.0
+        t = time.perf_counter()
+        r = hctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=20000)
+        ms = (time.perf_counter() - t) * 1e3
+    log(f"[bench] pcgsip through host arrays: {ms:.1f} ms (device time {r['ms']:.1f} ms), "
+        f"iters {r['iters']}")
+    return {"ms": round(ms, 2), "device_ms": round(r["ms"], 2), "iters": r["iters"],
+            "note": "host-pointer context: 8 coefficient arrays up, fi, res and 7 factor arrays "
+                    "down per call"}
 
 
 # (cin, cout, level) of the U-Net's 3x3x3 convolutions, encoder then decoder
