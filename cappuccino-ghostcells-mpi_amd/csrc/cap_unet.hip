@@ -20,8 +20,8 @@
 //    taps x input channels in steps of 32.  Used forward, and with the
 //    weights mirrored in the taps and transposed in the channels for the
 //    gradient with respect to the input.
-//  * k_wgrad: weight and bias gradients (FP32 multiply-add per tap and
-//    channel pair, summed over voxel blocks with atomics).
+//  * k_wgrad: weight and bias gradients on MFMA (K = voxels, staged in LDS
+//    transposed), one atomic add per weight and block of 4096 voxels.
 //  * pooling, upsampling, gradient combination with the ReLU mask, loss,
 //    Adam.
 #include <hip/hip_runtime.h>
@@ -200,35 +200,91 @@ int conv_grid(int D, int H, int W, int bz)
 // plane (1 wave): small grids are latency-bound and want more workgroups
 int conv_bz(int D, int H, int W) { return conv_grid(D, H, W, 4) >= 512 ? 4 : 1; }
 
-// dW[co][tap][ci] += sum_v X[v + off(tap)][ci] * G[v][co];  db[co] += sum_v G[v][co]
-// block: (voxel chunk of 128, tap); threads loop over (ci, co) pairs
+// dW[co][tap][ci] += sum_v G[v][co] X[v + off(tap)][ci] on the matrix cores
+// (C = A B with M = co, N = ci, K = voxels).  A block (4 waves) takes one tap
+// and WVB voxels in steps of 32 (one MFMA K step); each step stages G as
+// gt[co][32] and the shifted X (zero outside the grid) as xt[ci][32] in LDS,
+// so that a lane's 8 consecutive voxels of one channel are one 16-byte read.
+// Wave w accumulates the ci tiles w and w + 4 for all NCO co tiles; each
+// weight gets one atomic add per block.  Blocks of the centre tap also sum G
+// for the bias gradient db[co].
+constexpr int WVB = 4096;    // voxels per block
+constexpr int KST = 40;      // LDS row stride (bf16) of gt / xt
+constexpr int MAXCI = 112;   // largest input channel count of the network
+
+template <int NCO>
 __global__ __launch_bounds__(256) void k_wgrad(int D, int H, int W, const bf16_t *__restrict__ X,
                                                int cx, int xoff, int cin,
                                                const bf16_t *__restrict__ G, int cg, int cout,
                                                float *__restrict__ dW, float *__restrict__ db)
 {
-    const long nvox = (long)D * H * W;
-    const long v0 = (long)blockIdx.x * 128;
+    __shared__ __attribute__((aligned(16))) bf16_t gt[NCO * 16 * KST];
+    __shared__ __attribute__((aligned(16))) bf16_t xt[MAXCI * KST];
     const int tap = blockIdx.y;
     int dz, dy, dx;
     tap_off(tap, dz, dy, dx);
-    const long v1 = v0 + 128 < nvox ? v0 + 128 : nvox;
-    for (int pr = threadIdx.x; pr < cin * cout; pr += 256) {
-        const int ci = pr % cin, co = pr / cin;
-        float acc = 0.f, accb = 0.f;
-        for (long v = v0; v < v1; ++v) {
-            const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
-            const int zz = z + dz, yy = y + dy, xx = x + dx;
-            const float g = bf2f(G[v * cg + co]);
-            if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                const long nb = ((long)zz * H + yy) * W + xx;
-                acc += bf2f(X[nb * cx + xoff + ci]) * g;
-            }
-            accb += g;
+    const int nvox = D * H * W;
+    const int vb0 = blockIdx.x * WVB;
+    const int vb1 = vb0 + WVB < nvox ? vb0 + WVB : nvox;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, kg = lane >> 4, col = lane & 15;
+    const int ncit = (cin + 15) / 16, nco16 = NCO * 16, nci16 = ncit * 16;
+    f32x4 acc[NCO][2];
+#pragma unroll
+    for (int t = 0; t < NCO; ++t) acc[t][0] = acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    for (int v0 = vb0; v0 < vb1; v0 += 32) {
+        __syncthreads();   // the previous step's readers are done
+        for (int q = threadIdx.x; q < 32 * nco16; q += 256) {
+            const int co = q % nco16, k = q / nco16, v = v0 + k;
+            bf16_t val = 0;
+            if (co < cout && v < vb1) val = G[(long)v * cg + co];
+            gt[co * KST + k] = val;
         }
-        atomicAdd(&dW[((size_t)co * 27 + tap) * cin + ci], acc);
-        if (tap == 13 && ci == 0 && db) atomicAdd(&db[co], accb);
+        for (int q = threadIdx.x; q < 32 * nci16; q += 256) {
+            const int ci = q % nci16, k = q / nci16, v = v0 + k;
+            bf16_t val = 0;
+            if (ci < cin && v < vb1) {
+                const int x = v % W, y = (v / W) % H, z = v / (W * H);
+                const int xx = x + dx, yy = y + dy, zz = z + dz;
+                if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
+                    val = X[(long)((zz * H + yy) * W + xx) * cx + xoff + ci];
+            }
+            xt[ci * KST + k] = val;
+        }
+        __syncthreads();
+        if (db && tap == 13 && (int)threadIdx.x < cout)
+            for (int k = 0; k < 32; ++k) bsum += bf2f(gt[threadIdx.x * KST + k]);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int cit = w + 4 * c;
+            if (cit >= ncit) break;   // uniform across the wave
+            const uint4 rb = *reinterpret_cast<const uint4 *>(xt + (cit * 16 + col) * KST + 8 * kg);
+            bf16x8 bv;
+            memcpy(&bv, &rb, 16);
+#pragma unroll
+            for (int t = 0; t < NCO; ++t) {
+                const uint4 ra = *reinterpret_cast<const uint4 *>(gt + (t * 16 + col) * KST + 8 * kg);
+                bf16x8 av;
+                memcpy(&av, &ra, 16);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t][c], 0, 0, 0);
+            }
+        }
     }
+    // D[row][n]: lane holds rows 4*(lane/16) + j (co), column lane%16 (ci)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int cit = w + 4 * c;
+        if (cit >= ncit) break;
+        const int ci = cit * 16 + col;
+#pragma unroll
+        for (int t = 0; t < NCO; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int co = t * 16 + 4 * kg + j;
+                if (co < cout && ci < cin) atomicAdd(&dW[((size_t)co * 27 + tap) * cin + ci], acc[t][c][j]);
+            }
+    }
+    if (db && tap == 13 && (int)threadIdx.x < cout) atomicAdd(&db[threadIdx.x], bsum);
 }
 
 // 2x2x2 average pooling: out[Dc][Hc][Wc][c] from in (stride ci, offset ioff)
@@ -674,8 +730,18 @@ void wgrad_launch(cap_unet *n, int l, const bf16_t *X, int cx, int xoff, const b
     const int lv = c.level;
     const int D = n->nz >> lv, H = n->ny >> lv, W = n->nx >> lv;
     const long nvox = (long)D * H * W;
-    hipLaunchKernelGGL(k_wgrad, dim3((int)((nvox + 127) / 128), 27), dim3(256), 0, n->ctx->stream,
-                       D, H, W, X, cx, xoff, c.cin, G, c.coutp8, c.cout, n->g + c.woff, n->g + c.boff);
+    const dim3 grid((unsigned)((nvox + WVB - 1) / WVB), 27);
+    hipStream_t s = n->ctx->stream;
+#define WGRAD(NCO)                                                                                  \
+    hipLaunchKernelGGL(k_wgrad<NCO>, grid, dim3(256), 0, s, D, H, W, X, cx, xoff, c.cin, G, c.coutp8, \
+                       c.cout, n->g + c.woff, n->g + c.boff)
+    switch ((c.cout + 15) / 16) {
+    case 1: WGRAD(1); break;
+    case 2: WGRAD(2); break;
+    case 3: WGRAD(3); break;
+    case 4: WGRAD(4); break;
+    }
+#undef WGRAD
 }
 
 int pack_weights(cap_unet *n)
