@@ -115,6 +115,31 @@ __global__ __launch_bounds__(BLOCK) void k_pupdate(Lay L, const V *__restrict__ 
     for_cells(L, [&](const Cell &c) { p[c.e] = z[c.e] + bet * p[c.e]; });
 }
 
+// The previous iteration's fi = fi + alf pk (pcg-sip.f90:168), while its pk
+// and alf are still current, then pk = zk + bet pk (pcg-sip.f90:135)
+__global__ __launch_bounds__(BLOCK) void k_pupdate_fi(Lay L, const double *__restrict__ z,
+                                                      double *__restrict__ p,
+                                                      double *__restrict__ fi, DevState *st)
+{
+    if (st->done) return;
+    const double bet = st->bet, alf = st->alf;
+    for_cells(L, [&](const Cell &c) {
+        const double po = p[c.e];
+        fi[c.e] = fi[c.e] + alf * po;
+        p[c.e] = z[c.e] + bet * po;
+    });
+}
+
+// fi = fi + alf pk after a stop in the forward sweep, whose residual update
+// already ran; the state's done flag is set by then (pcg-sip.f90:168)
+__global__ __launch_bounds__(BLOCK) void k_fi_last(Lay L, double *__restrict__ fi,
+                                                   const double *__restrict__ p,
+                                                   const DevState *st)
+{
+    const double alf = st->alf;
+    for_cells(L, [&](const Cell &c) { fi[c.e] = fi[c.e] + alf * p[c.e]; });
+}
+
 // pk = res + om (pk - alf uk)   (cgstab_sip.f90:124)
 __global__ __launch_bounds__(BLOCK) void k_bic_p(Lay L, const double *__restrict__ res,
                                                  const double *__restrict__ uk,
@@ -180,6 +205,23 @@ struct PkN {
 template <class Op>
 struct PkN<Op, std::void_t<decltype(Op::PKN)>> {
     static constexpr int v = Op::PKN;
+};
+
+// ops that take the iteration's step length (alf) from the solver state
+template <class Op, class = void>
+struct TakesAlf : std::false_type {};
+template <class Op>
+struct TakesAlf<Op, std::void_t<decltype(Op::TAKES_ALF)>> : std::true_type {};
+
+// steps of inputs a compute lane keeps in flight: 16 for one chained value
+// and up to five inputs, else 8 (register budget), unless the op says
+template <class Op, class = void>
+struct LookAhead {
+    static constexpr int v = Op::ND == 1 && Op::NIN <= 5 ? 16 : 8;
+};
+template <class Op>
+struct LookAhead<Op, std::void_t<decltype(Op::LOOKAHEAD)>> {
+    static constexpr int v = Op::LOOKAHEAD;
 };
 
 // one element of four packed values: two 16-byte loads (FP64) or one (FP32)
@@ -284,7 +326,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
 {
     using V = typename Op::T;
     constexpr int PKN = PkN<Op>::v;
-    constexpr int ND = Op::ND, NIN = Op::NIN, U = ND == 1 && NIN <= 5 ? 16 : 8;
+    constexpr int ND = Op::ND, NIN = Op::NIN, U = LookAhead<Op>::v;
     constexpr int DM = 4;   // messages are read DM steps ahead of use
     constexpr int NXS = Op::NX > 0 ? Op::NX : 1, NRS = Op::NR > 0 ? Op::NR : 1;
     __shared__ V lds[2][ND][TCOL];
@@ -406,6 +448,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
         }
     } else {
         // --------------------------------------------------- waves 0-3: cells
+        if constexpr (TakesAlf<Op>::value) op.alf = (V)sy.st->alf;
         const int lane = threadIdx.x;
         LaneInfo li;
         lane_setup<FWD>(L, ti, tj, lane, li);
@@ -738,6 +781,20 @@ static void sip_fwd(cap_ctx *ctx, const Lay &L, const double *rhs, double *z)
     launch_wave<OpSipFwd, true, F_NONE>(ctx, L, OpSipFwd(), io);
 }
 
+// the previous PCG-SIP iteration's residual update and L1 stop test, fused
+// into this iteration's forward sweep (z := forward sweep of the new res)
+static void sip_fwd_res(cap_ctx *ctx, const Lay &L)
+{
+    double *const *T = ctx->T;
+    WaveIO<OpSipFwdRes> io;
+    io.in[0] = T[T_RES]; io.in[1] = T[T_BB]; io.in[2] = T[T_BW]; io.in[3] = T[T_BS];
+    io.in[4] = T[T_BP]; io.in[5] = T[T_W];
+    io.dep[0] = T[T_ZK];
+    io.xo[0] = T[T_RES];
+    io.pk = ctx->pkd;
+    launch_wave<OpSipFwdRes, true, F_PCG_RESL>(ctx, L, OpSipFwdRes(), io);
+}
+
 static void sip_bwd(cap_ctx *ctx, const Lay &L, double *z)
 {
     double *const *T = ctx->T;
@@ -843,25 +900,47 @@ extern "C" int cap_pcgsip(cap_ctx *ctx, double *fi, const cap_coef *c, double al
                        (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
                        ctx->partials, ctx->ticket);
     sip_factor(ctx, L, alfa);
+    // Iteration l's update (pcg-sip.f90:167-171) is split over iteration
+    // l+1: the residual update, its L1 sum and the stop test (172-188) run at
+    // the start of the forward sweep, the fi update in the pk update.  When
+    // the forward sweep stops the solve, the last fi update follows the loop;
+    // when the loop runs out, the last iteration's whole update does.
+    int queued = 0;
     auto one = [&]() {
-        sip_fwd(ctx, L, T[T_RES], T[T_ZK]);
+        const bool first = queued++ == 0;
+        if (first)
+            sip_fwd(ctx, L, T[T_RES], T[T_ZK]);
+        else
+            sip_fwd_res(ctx, L);
         WaveIO<OpSipBwdDot> io;
         io.in[0] = T[T_ZK]; io.in[1] = T[T_BN]; io.in[2] = T[T_BE]; io.in[3] = T[T_BT];
         io.in[4] = T[T_RES];
         io.dep[0] = T[T_ZK];
         io.xo[0] = nullptr;
         launch_wave<OpSipBwdDot, false, F_PCG_SK>(ctx, L, OpSipBwdDot(), io);
-        hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
-                           T[T_PK], ctx->st);
+        if (first)
+            hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L,
+                               (const double *)T[T_ZK], T[T_PK], ctx->st);
+        else
+            hipLaunchKernelGGL(k_pupdate_fi, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
+                               T[T_PK], T[T_FI], ctx->st);
         hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
-                           ctx->partials, ctx->ticket);
-        hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
-                           T[T_RES], (const double *)T[T_PK], (const double *)T[T_W], ctx->st,
                            ctx->partials, ctx->ticket);
     };
     DevState h;
     if ((rc = iterate(ctx, nsw, one, h))) return rc;
+    if (queued > 0 && !h.err) {
+        if (h.done)   // stopped by a forward sweep: its iteration's fi update
+            hipLaunchKernelGGL(k_fi_last, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                               (const double *)T[T_PK], (const DevState *)ctx->st);
+        else          // ran out of iterations: the last one's whole update
+            hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
+                               T[T_FI], T[T_RES], (const double *)T[T_PK], (const double *)T[T_W],
+                               ctx->st, ctx->partials, ctx->ticket);
+        HIPCHK(hipGetLastError());
+    }
+    if ((rc = read_state(ctx, h))) return rc;
     if ((rc = writeback(ctx, L, fi, c, d, true, true))) return rc;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1193,11 +1272,16 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
     Coef7T<float> A32;
     A32.ae = F[F_AE]; A32.aw = F[F_AW]; A32.an = F[F_AN]; A32.as = F[F_AS];
     A32.at = F[F_AT]; A32.ab = F[F_AB]; A32.ap = F[F_AP];
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < 6; ++q) {
+        if (fp32 && q == 5) {   // the fused sweep has no FP32 form
+            ms[q] = 0.0;
+            continue;
+        }
         HIPCHK(hipEventRecord(ctx->ev0, s));
         for (int r = 0; r < reps; ++r) {
             if (!fp32) {
                 if (q == 0) sip_fwd(ctx, L, T[T_RES], T[T_ZK]);
+                if (q == 5) sip_fwd_res(ctx, L);
                 if (q == 1) {
                     WaveIO<OpSipBwdDot> io;
                     io.in[0] = T[T_ZK]; io.in[1] = T[T_BN]; io.in[2] = T[T_BE]; io.in[3] = T[T_BT];
@@ -1207,8 +1291,8 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                                 launch_wave<OpSipBwdDot, false, F_PCG_SK>(ctx, L, OpSipBwdDot(), io);
                 }
                 if (q == 2)
-                    hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L,
-                                       (const double *)T[T_ZK], T[T_PK], ctx->st);
+                    hipLaunchKernelGGL(k_pupdate_fi, dim3(gb), dim3(BLOCK), 0, s, L,
+                                       (const double *)T[T_ZK], T[T_PK], T[T_FI], ctx->st);
                 if (q == 3)
                     hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
                                        L, A, (const double *)T[T_PK], T[T_W],

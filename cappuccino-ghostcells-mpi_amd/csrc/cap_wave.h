@@ -168,6 +168,32 @@ struct OpSipFwdT {
 };
 using OpSipFwd = OpSipFwdT<double>;
 
+// The previous PCG-SIP iteration's residual update (pcg-sip.f90:169-171)
+// done by the next iteration's forward sweep (pcg-sip.f90:102-103), which
+// reads the new residual right after producing it:
+//   r = res - alf*zk;  sum |r|;  z = (r - bb*z(k-1) - bw*z(i-1) - bs*z(j-1)) * bp
+// x = {res, bb, bw, bs, bp, A pk}; extra output r; alf is the previous
+// iteration's step length, read from the solver state when the sweep starts.
+// (That iteration's fi update, pcg-sip.f90:168, goes with the pk update.)
+template <class V>
+struct OpSipFwdResT {
+    using T = V;
+    static constexpr int NIN = 6, ND = 1, NX = 1, NR = 1;
+    static constexpr int PKN = 4;   // x[1..4] come packed: {bb, bw, bs, bp}
+    static constexpr int LOOKAHEAD = 12;
+    static constexpr bool TAKES_ALF = true;
+    V alf;
+    CAP_HD void compute(const V *x, const V *dk, const V *di, const V *dj, V *d, V *xo,
+                        double *r) const
+    {
+        const V res = x[0] - alf * x[5];
+        xo[0] = res;
+        r[0] = __builtin_fabs((double)res);
+        d[0] = (res - x[1] * dk[0] - x[2] * di[0] - x[3] * dj[0]) * x[4];
+    }
+};
+using OpSipFwdRes = OpSipFwdResT<double>;
+
 // SIP backward substitution + scalar product, pcg-sip.f90:116-118:
 //   z = z - bn*z(j+1) - be*z(i+1) - bt*z(k+1);  sk += res*z
 // x = {z, bn, be, bt, res}

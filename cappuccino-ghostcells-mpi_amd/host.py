@@ -281,8 +281,8 @@ class Context:
         self._check(rc, "cap_bpres", allow=(CAP_OK,))
 
     def time_kernels(self, fp32=False, reps=20):
-        """Average ms of the 5 kernels of one PCG-SIP iteration (see cap_amd.h)."""
-        ms = (C.c_double * 5)()
+        """Average ms of the 6 PCG-SIP kernels (see cap_amd.h)."""
+        ms = (C.c_double * 6)()
         rc = self._lib.cap_time_kernels(self._h, 1 if fp32 else 0, reps, ms)
         self._check(rc, "cap_time_kernels", allow=(CAP_OK,))
         return list(ms)

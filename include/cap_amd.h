@@ -363,8 +363,11 @@ int cap_build_lognormal(cap_ctx *ctx, unsigned long long seed, double sigma,
 /* Measurement hook, no counterpart in the reference: average device time
  * (ms, HIP events on the context's stream) of each kernel of one PCG-SIP
  * iteration re-launched `reps` times on the vectors of the last pcgsip
- * (fp32 = 0) or pcgsip_mixed (fp32 = 1) solve.  ms[0..4]: SIP forward sweep,
- * SIP backward sweep + dot, pk update, SpMV + dot, fi/res update + L1. */
+ * (fp32 = 0) or pcgsip_mixed (fp32 = 1) solve.  ms[0..5]: SIP forward sweep,
+ * SIP backward sweep + dot, pk update, SpMV + dot, fi/res update + L1, and
+ * the forward sweep that first updates the residual (0 for fp32 = 1).  An FP64
+ * pcgsip iteration after the first runs ms[5], ms[1], ms[2], ms[3]; its pk
+ * update (ms[2] for fp32 = 0) also does the previous iteration's fi update. */
 int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms);
 
 #ifdef __cplusplus
