@@ -498,6 +498,10 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                 for (int q = 0; q < NIN; ++q) x[q] = inp[q][e];
             }
         };
+        // Steps at which every lane of a full tile has a solve cell: there
+        // the step kinds, halo values and dummy stores drop out.
+        const bool full = L.nis - ti * TW >= TW && L.njs - tj * TW >= TW;
+        const int t_lo = 2 * (TW - 1) + 1, t_hi = L.nks;
 #pragma unroll
         for (int d = 0; d < U; ++d) fetch_in(d, inb[d]);
         __syncthreads();   // step-0 inputs from outside the tile are in mbuf
@@ -505,11 +509,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
 #pragma unroll
             for (int d = 0; d < U; ++d) {
                 const int t = t0 + d;
-                int kk;
-                const int kind = step_kind<FWD>(L, li, t, kk);
                 const size_t e = elem_at(tb, step_slot<FWD>(L, t), lane);
                 const int rp = (t + 1) & 1, wp = t & 1;
-                const bool comp = kind == K_COMPUTE, idle = kind == K_IDLE;
                 V dk[ND], di[ND], dj[ND], out[ND];
 #pragma unroll
                 for (int q = 0; q < ND; ++q) {
@@ -520,27 +521,43 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                 V xv[NXS];
                 double r[NRS];
                 op.compute(inb[d], dk, di, dj, out, xv, r);
+                if (full && t >= t_lo && t <= t_hi) {
 #pragma unroll
-                for (int q = 0; q < ND; ++q) {
-                    out[q] = comp ? out[q] : (kind == K_LOAD ? (kk == 0 ? hlo[q] : hhi[q]) : V(0));
-                    *(idle ? dummy + q * TCOL : depp[q] + e) = out[q];
-                }
-#pragma unroll
-                for (int q = 0; q < Op::NX; ++q) *(comp ? xop[q] + e : dummy + (ND + q) * TCOL) = xv[q];
-                if (comp) {
+                    for (int q = 0; q < ND; ++q) lds[wp][q][lane] = out[q];
 #pragma unroll
                     for (int q = 0; q < ND; ++q) {
+                        depp[q][e] = out[q];
                         if (li.prod_a) coh_store(msgp + msg_index(L, ND, 0, q, tile, t, li.b), out[q]);
                         if (li.prod_b) coh_store(msgp + msg_index(L, ND, 1, q, tile, t, li.a), out[q]);
                     }
+#pragma unroll
+                    for (int q = 0; q < Op::NX; ++q) xop[q][e] = xv[q];
+#pragma unroll
+                    for (int q = 0; q < Op::NR; ++q) red[q] += r[q];
+                } else {
+                    int kk;
+                    const int kind = step_kind<FWD>(L, li, t, kk);
+                    const bool comp = kind == K_COMPUTE, idle = kind == K_IDLE;
+#pragma unroll
+                    for (int q = 0; q < ND; ++q) {
+                        out[q] = comp ? out[q] : (kind == K_LOAD ? (kk == 0 ? hlo[q] : hhi[q]) : V(0));
+                        lds[wp][q][lane] = out[q];
+                        *(idle ? dummy + q * TCOL : depp[q] + e) = out[q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < Op::NX; ++q) *(comp ? xop[q] + e : dummy + (ND + q) * TCOL) = xv[q];
+                    if (comp) {
+#pragma unroll
+                        for (int q = 0; q < ND; ++q) {
+                            if (li.prod_a) coh_store(msgp + msg_index(L, ND, 0, q, tile, t, li.b), out[q]);
+                            if (li.prod_b) coh_store(msgp + msg_index(L, ND, 1, q, tile, t, li.a), out[q]);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < Op::NR; ++q) red[q] += comp ? r[q] : 0.0;
                 }
 #pragma unroll
-                for (int q = 0; q < Op::NR; ++q) red[q] += comp ? r[q] : 0.0;
-#pragma unroll
-                for (int q = 0; q < ND; ++q) {
-                    prev[q] = out[q];
-                    lds[wp][q][lane] = out[q];
-                }
+                for (int q = 0; q < ND; ++q) prev[q] = out[q];
                 fetch_in(t + U, inb[d]);
                 __syncthreads();
             }
