@@ -327,7 +327,9 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
     using V = typename Op::T;
     constexpr int PKN = PkN<Op>::v;
     constexpr int ND = Op::ND, NIN = Op::NIN, U = LookAhead<Op>::v;
-    constexpr int DM = 4;   // messages are read DM steps ahead of use
+    // messages are read DM steps ahead of use: further ahead, more of them are
+    // found unwritten and the consumer tile falls further behind its producer
+    constexpr int DM = 2;
     constexpr int NXS = Op::NX > 0 ? Op::NX : 1, NRS = Op::NR > 0 ? Op::NR : 1;
     __shared__ V lds[2][ND][TCOL];
     __shared__ V mbuf[2][ND][2][TW];   // [step parity][q][j-side by b / i-side by a]
@@ -1397,7 +1399,7 @@ extern "C" int cap_init(cap_ctx **out, int ni, int nj, int nk, int flags)
     ctx->claim_base = 0;
     // steps a consumer tile keeps behind its producer in the wavefront
     // sweeps (CAP_WAVE_LAG in the environment overrides it, 0..64)
-    ctx->wave_lag = 4;
+    ctx->wave_lag = 1;
     if (const char *lag = getenv("CAP_WAVE_LAG")) {
         const int v = atoi(lag);
         if (v >= 0 && v <= 64) ctx->wave_lag = v;
