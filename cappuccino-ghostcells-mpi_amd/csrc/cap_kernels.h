@@ -254,39 +254,34 @@ __device__ __forceinline__ Cell cell_of(const Lay &L, size_t ts, int lane)
     return c;
 }
 
-// Tile-slot range [lo, hi) of this block in the flat kernels: contiguous, so a
-// block walks up the planes of one tile and the plane above / below a cell
-// (its k-, and through the shear its i- and j-neighbours) stays in the L2 of
-// the same XCD.
-__device__ __forceinline__ void block_range(const Lay &L, size_t &lo, size_t &hi)
-{
-    const size_t nts = (size_t)L.ntiles * L.S;
-    const size_t per = (nts + gridDim.x - 1) / gridDim.x;
-    lo = (size_t)blockIdx.x * per;
-    hi = lo + per < nts ? lo + per : nts;
-    if (lo > hi) lo = hi;
-}
-
-// Calls f(c) for every solve cell of this thread's lane in the block's slot
-// range (SlotWalk: no division per element).
+// Blocks of the flat kernels come in groups of eight (the launch grid is a
+// multiple of 8): block b takes the (b % 8)-th eighth of the slots of tiles
+// b/8, b/8 + G, ... (G = gridDim / 8).  Blocks are spread over the eight XCDs
+// by b % 8, so every tile's same eighth of the planes is handled on the same
+// XCD: a stencil's reads of a neighbouring tile's edge columns (eight-byte
+// pieces of separate cache lines for the j-edges) find the lines that the
+// neighbour's block has pulled into that XCD's L2.
 template <class F>
 __device__ __forceinline__ void for_cells(const Lay &L, F &&f)
 {
-    size_t lo, hi;
-    block_range(L, lo, hi);
-    if (lo >= hi) return;
+    const int c = blockIdx.x & 7, G = (int)(gridDim.x >> 3);
+    const int s0 = c * L.S / 8, s1 = (c + 1) * L.S / 8;
+    if (G == 0 || s0 >= s1) return;
     const int lane = threadIdx.x;
-    SlotWalk w;
-    w.start(L, lo, lane);
-    for (size_t ts = lo; ts < hi; ++ts) {
-        if (w.cell(L)) {
-            Cell c;
-            c.e = elem_ts(ts, w.slot, lane);
-            c.ti = w.ti; c.tj = w.tj; c.slot = w.slot; c.b = w.b; c.a = w.a; c.kk = w.kk;
-            c.in = true;
-            f(c);
+    for (int t = (int)(blockIdx.x >> 3); t < L.ntiles; t += G) {
+        const size_t lo = (size_t)t * L.S + s0;
+        SlotWalk w;
+        w.start(L, lo, lane);
+        for (int s = s0; s < s1; ++s) {
+            if (w.cell(L)) {
+                Cell cl;
+                cl.e = elem_ts(lo + (size_t)(s - s0), w.slot, lane);
+                cl.ti = w.ti; cl.tj = w.tj; cl.slot = w.slot; cl.b = w.b; cl.a = w.a; cl.kk = w.kk;
+                cl.in = true;
+                f(cl);
+            }
+            w.next(L);
         }
-        w.next(L);
     }
 }
 

@@ -571,10 +571,11 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
 }
 
 // ============================================================ host helpers
+// grid of the flat kernels: eight blocks a tile (one per XCD, each an eighth of
+// the tile's slots), at most 2048 blocks (tiles beyond 256 are strided)
 static int grid_for(const Lay &L)
 {
-    size_t nts = (size_t)L.ntiles * L.S;
-    return (int)std::min<size_t>(nts, 2048);
+    return 8 * std::min(L.ntiles, 256);
 }
 
 static int fail(cap_ctx *c, int code, const char *what)
