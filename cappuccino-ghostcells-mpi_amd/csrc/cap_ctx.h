@@ -62,6 +62,7 @@ struct cap_ctx {
     cap::DevState *st2;            // inner solve of the mixed-precision solver
     cap::DevState *st_host;
     double *partials;
+    int *asym;                    // per flat-kernel block: 1 = SpMV reads aw, as, ab
     unsigned *ticket;
     char err[256];
 };

@@ -82,26 +82,59 @@ __global__ __launch_bounds__(BLOCK) void k_resid(Lay L, Coef7 A, const double *_
     grid_reduce<1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
 }
 
+// Flags the blocks of the flat-kernel grid whose solve cells are not coupled
+// symmetrically entry for entry: aw, as, ab of a cell differ (bit for bit)
+// from ae, an, at of its west, south and bottom neighbours.  The p' matrix
+// of the reference is assembled face by face (the west coefficient of a
+// cell's east neighbour is its east coefficient, fluxmass2.f90:95-96), so
+// only a few cells differ, e.g. the neighbours of a pinned reference cell;
+// the SpMV reads 4 coefficient arrays instead of 7 in every other block.
+__global__ __launch_bounds__(BLOCK) void k_symcheck(Lay L, Coef7 A, int *asym)
+{
+    auto bits = [](double x) { return __double_as_longlong(x); };
+    bool differ = false;
+    for_cells(L, [&](const Cell &c) {
+        Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
+        const size_t e = c.e;
+        differ |= bits(A.aw[e]) != bits(A.ae[nb.w]) || bits(A.as[e]) != bits(A.an[nb.s]) ||
+                  bits(A.ab[e]) != bits(A.at[nb.bt]);
+    });
+    const int any = __syncthreads_or(differ ? 1 : 0);
+    if (threadIdx.x == 0) asym[blockIdx.x] = any;
+}
+
 // z = ap p - ae p_e - aw p_w - an p_n - as p_s - at p_t - ab p_b
 // (pcg-sip.f90:147-149).  MODE 0: sum p.z   MODE 1: sum z.y   MODE 2: sum z.y, sum z.z
+// In blocks that k_symcheck found symmetric (asym[block] == 0), aw, as, ab
+// are read as ae, an, at of the neighbour: the same values, bit for bit.
 template <class V, int MODE, int FIN>
 __global__ __launch_bounds__(BLOCK) void k_spmv(Lay L, Coef7T<V> A, const V *__restrict__ p,
                                                 V *__restrict__ z, const V *__restrict__ y,
                                                 DevState *st, double *partials,
-                                                unsigned *ticket)
+                                                unsigned *ticket, const int *asym)
 {
     if (st->done) return;
     Acc acc[MODE == 2 ? 2 : 1];
-    for_cells(L, [&](const Cell &c) {
-        Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
-        const size_t e = c.e;
-        V v = A.ap[e] * p[e] - A.ae[e] * p[nb.e] - A.aw[e] * p[nb.w] - A.an[e] * p[nb.n] -
-              A.as[e] * p[nb.s] - A.at[e] * p[nb.t] - A.ab[e] * p[nb.bt];
-        z[e] = v;
-        if constexpr (MODE == 0) acc[0] += (double)p[e] * (double)v;
-        if constexpr (MODE == 1) acc[0] += (double)v * (double)y[e];
-        if constexpr (MODE == 2) { acc[0] += (double)v * (double)y[e]; acc[1] += (double)v * (double)v; }
-    });
+    auto run = [&](auto symc) {
+        constexpr bool SYM = decltype(symc)::value;
+        for_cells(L, [&](const Cell &c) {
+            Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
+            const size_t e = c.e;
+            const V aw = SYM ? A.ae[nb.w] : A.aw[e];
+            const V as = SYM ? A.an[nb.s] : A.as[e];
+            const V ab = SYM ? A.at[nb.bt] : A.ab[e];
+            V v = A.ap[e] * p[e] - A.ae[e] * p[nb.e] - aw * p[nb.w] - A.an[e] * p[nb.n] -
+                  as * p[nb.s] - A.at[e] * p[nb.t] - ab * p[nb.bt];
+            z[e] = v;
+            if constexpr (MODE == 0) acc[0] += (double)p[e] * (double)v;
+            if constexpr (MODE == 1) acc[0] += (double)v * (double)y[e];
+            if constexpr (MODE == 2) { acc[0] += (double)v * (double)y[e]; acc[1] += (double)v * (double)v; }
+        });
+    };
+    if (asym && asym[blockIdx.x] == 0)
+        run(std::true_type());
+    else
+        run(std::false_type());
     grid_reduce<MODE == 2 ? 2 : 1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
 }
 
@@ -673,6 +706,13 @@ static Coef7 coef7(cap_ctx *ctx)
     return A;
 }
 
+// flags the SpMV blocks of this solve's layout that must read aw, as, ab
+static void sym_check(cap_ctx *ctx, const Lay &L)
+{
+    hipLaunchKernelGGL(k_symcheck, dim3(grid_for(L)), dim3(BLOCK), 0, ctx->stream, L, coef7(ctx),
+                       ctx->asym);
+}
+
 // Everything a solve needs in device memory, in both layouts.
 struct SolveIn {
     double *fi, *ae, *aw, *an, *as, *at, *ab, *ap, *su, *res;
@@ -910,6 +950,7 @@ extern "C" int cap_pcgsip(cap_ctx *ctx, double *fi, const cap_coef *c, double al
     int rc = prepare(ctx, L, fi, c, d, false, true);
     if (rc) return rc;
     if ((rc = init_state(ctx, sor, nsw))) return rc;
+    sym_check(ctx, L);
     double *const *T = ctx->T;
     const int gb = grid_for(L);
     hipStream_t s = ctx->stream;
@@ -946,7 +987,7 @@ extern "C" int cap_pcgsip(cap_ctx *ctx, double *fi, const cap_coef *c, double al
                                T[T_PK], T[T_FI], ctx->st);
         hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
-                           ctx->partials, ctx->ticket);
+                           ctx->partials, ctx->ticket, ctx->asym);
     };
     DevState h;
     if ((rc = iterate(ctx, nsw, one, h))) return rc;
@@ -1012,6 +1053,7 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
     int rc = prepare(ctx, L, fi, c, d, false, false);
     if (rc) return rc;
     if ((rc = init_state(ctx, sor, nsw))) return rc;
+    sym_check(ctx, L);
     double *const *T = ctx->T;
     const int gb = grid_for(L);
     hipStream_t s = ctx->stream;
@@ -1048,7 +1090,7 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
                            T[T_PK], ctx->st);
         hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
-                           ctx->partials, ctx->ticket);
+                           ctx->partials, ctx->ticket, ctx->asym);
         hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
                            T[T_RES], (const double *)T[T_PK], (const double *)T[T_W], ctx->st,
                            ctx->partials, ctx->ticket);
@@ -1070,6 +1112,7 @@ extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, doubl
     int rc = prepare(ctx, L, fi, c, d, false, true);
     if (rc) return rc;
     if ((rc = init_state(ctx, sor, nsw))) return rc;
+    sym_check(ctx, L);
     double *const *T = ctx->T;
     const int gb = grid_for(L);
     hipStream_t s = ctx->stream;
@@ -1093,7 +1136,7 @@ extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, doubl
         sip_bwd(ctx, L, T[T_ZK]);
         hipLaunchKernelGGL((k_spmv<double, 1, F_BIC_GAM>), dim3(gb), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_ZK], T[T_UK], (const double *)T[T_RESO], ctx->st,
-                           ctx->partials, ctx->ticket);
+                           ctx->partials, ctx->ticket, ctx->asym);
         hipLaunchKernelGGL((k_update<double, 1, F_NONE>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
                            T[T_RES], (const double *)T[T_ZK], (const double *)T[T_UK], ctx->st,
                            ctx->partials, ctx->ticket);
@@ -1101,7 +1144,7 @@ extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, doubl
         sip_bwd(ctx, L, T[T_ZK]);
         hipLaunchKernelGGL((k_spmv<double, 2, F_BIC_ALF>), dim3(gb), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_ZK], T[T_VK], (const double *)T[T_RES], ctx->st,
-                           ctx->partials, ctx->ticket);
+                           ctx->partials, ctx->ticket, ctx->asym);
         hipLaunchKernelGGL((k_update<double, 0, F_BIC_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
                            T[T_RES], (const double *)T[T_ZK], (const double *)T[T_VK], ctx->st,
                            ctx->partials, ctx->ticket);
@@ -1165,6 +1208,7 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
     if (rc) return rc;
     if ((rc = ensure_f32(ctx, L.n))) return rc;
     if ((rc = init_state(ctx, sor, nsw))) return rc;
+    sym_check(ctx, L);
     double *const *T = ctx->T;
     float *const *F = ctx->F;
     const int gb = grid_for(L);
@@ -1228,7 +1272,7 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
                                (const float *)F[F_Z], F[F_P], st2);
             hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L,
                                A32, (const float *)F[F_P], F[F_W], (const float *)nullptr, st2,
-                               ctx->partials, ctx->ticket);
+                               ctx->partials, ctx->ticket, ctx->asym);
             hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
                                F[F_X], F[F_R], (const float *)F[F_P], (const float *)F[F_W], st2,
                                ctx->partials, ctx->ticket);
@@ -1283,6 +1327,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
     const Lay L = lay_for(ctx, 3, ctx->nk - 2);
     int rc;
     if ((rc = init_state(ctx, 0.0, 1 << 30))) return rc;
+    sym_check(ctx, L);
     HIPCHK(hipMemsetAsync(ctx->ticket, 0, 64, ctx->stream));
     double *const *T = ctx->T;
     float *const *F = ctx->F;
@@ -1316,7 +1361,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                 if (q == 3)
                     hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
                                        L, A, (const double *)T[T_PK], T[T_W],
-                                       (const double *)nullptr, ctx->st, ctx->partials, ctx->ticket);
+                                       (const double *)nullptr, ctx->st, ctx->partials, ctx->ticket, ctx->asym);
                 if (q == 4)
                     hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0,
                                        s, L, T[T_FI], T[T_RES], (const double *)T[T_PK],
@@ -1345,7 +1390,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                 if (q == 3)
                     hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
                                        L, A32, (const float *)F[F_P], F[F_W],
-                                       (const float *)nullptr, ctx->st, ctx->partials, ctx->ticket);
+                                       (const float *)nullptr, ctx->st, ctx->partials, ctx->ticket, ctx->asym);
                 if (q == 4)
                     hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0,
                                        s, L, F[F_X], F[F_R], (const float *)F[F_P],
@@ -1384,6 +1429,9 @@ extern "C" int cap_init(cap_ctx **out, int ni, int nj, int nk, int flags)
     HIPCHK(hipMalloc(&ctx->st2, sizeof(DevState)));
     HIPCHK(hipHostMalloc(&ctx->st_host, sizeof(DevState), hipHostMallocDefault));
     HIPCHK(hipMalloc(&ctx->partials, (size_t)MAX_RED_BLOCKS * 64 * sizeof(double)));
+    // no block is known to be symmetric until a solve's k_symcheck says so
+    HIPCHK(hipMalloc(&ctx->asym, (size_t)MAX_RED_BLOCKS * sizeof(int)));
+    HIPCHK(hipMemset(ctx->asym, 1, (size_t)MAX_RED_BLOCKS * sizeof(int)));
     {   // every reduction slot starts empty (grid_reduce puts the marker back)
         double empty;
         const uint64_t bits = PARTIAL_EMPTY;
@@ -1421,6 +1469,7 @@ extern "C" int cap_finalize(cap_ctx *ctx)
     (void)hipFree(ctx->st2);
     (void)hipHostFree(ctx->st_host);
     (void)hipFree(ctx->partials);
+    (void)hipFree(ctx->asym);
     (void)hipFree(ctx->ticket);
     (void)hipFree(ctx->claim);
     (void)hipEventDestroy(ctx->ev0);

@@ -641,3 +641,36 @@ def test_strain_rate_of_a_shear_flow(O):
     inner = (slice(3, nk - 3), slice(3, ni - 3), slice(3, nj - 3))
     np.testing.assert_allclose(strain.reshape(nk, ni, nj)[inner], gam, rtol=1e-14)
     np.testing.assert_allclose(vort.reshape(nk, ni, nj)[inner], gam / 2, rtol=1e-14)
+
+
+def _asymmetric_couplings(s, dims, k_lo, k_hi):
+    """solve cells (k_lo..k_hi, i, j = 3..n-2) whose aw, as, ab differ bit for
+    bit from ae, an, at of the west, south, bottom neighbour (the pairs the
+    GPU SpMV reads from one array where they agree)"""
+    ni, nj, nk = dims
+    k, i, j = np.meshgrid(np.arange(k_lo, k_hi + 1), np.arange(3, ni - 1), np.arange(3, nj - 1),
+                          indexing="ij")
+    c = ((k - 1) * ni * nj + (i - 1) * nj + (j - 1)).ravel()
+    bits = lambda a: a.view(np.int64)  # noqa: E731
+    bad = ((bits(s.aw)[c] != bits(s.ae)[c - nj]) | (bits(s.as_)[c] != bits(s.an)[c - 1]) |
+           (bits(s.ab)[c] != bits(s.at)[c - ni * nj]))
+    return c[bad]
+
+
+@pytest.mark.parametrize("dims", [(16, 14, 12), (20, 9, 11)])
+def test_symmetry_of_the_test_systems(O, dims):
+    """The log-normal systems couple every cell symmetrically entry for entry
+    except the three neighbours of the pinned reference cell, whose row alone
+    is zeroed; the general systems are asymmetric everywhere.  So the GPU
+    tests on log-normal systems run the SpMV's symmetric path in all blocks but
+    those holding the pinned cell's neighbours, and the general ones only its
+    general path."""
+    ni, nj, _ = dims
+    s = O.build_lognormal(dims, 3, 1.0)
+    for k_lo, k_hi in ((3, dims[2] - 2), (2, dims[2] - 1)):
+        bad = _asymmetric_couplings(s, dims, k_lo, k_hi)
+        p = 2 * ni * nj + 2 * nj + 2            # cell (3, 3, 3), 0-based index
+        assert sorted(bad.tolist()) == sorted([p + 1, p + nj, p + ni * nj])
+    g, _ = O.build_general(dims, 5)
+    bad = _asymmetric_couplings(g, dims, 3, dims[2] - 2)
+    assert bad.size > 0.9 * (ni - 4) * (nj - 4) * (dims[2] - 4)
