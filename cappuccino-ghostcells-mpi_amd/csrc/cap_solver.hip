@@ -611,6 +611,15 @@ static int grid_for(const Lay &L)
     return 8 * std::min(L.ntiles, 256);
 }
 
+// grid of the SpMV (and of its symmetry check, whose per-block flags it
+// reads): fewer, longer blocks, so that the p and coefficient lines of a
+// block's previous planes, which its stencil reads again, are still in the
+// XCD's L2 (measured at 256^3: 0.242 ms with 2048 blocks, 0.227 with 512)
+static int grid_spmv(const Lay &L)
+{
+    return 8 * std::min(L.ntiles, 64);
+}
+
 static int fail(cap_ctx *c, int code, const char *what)
 {
     snprintf(c->err, sizeof c->err, "%s", what);
@@ -709,7 +718,7 @@ static Coef7 coef7(cap_ctx *ctx)
 // flags the SpMV blocks of this solve's layout that must read aw, as, ab
 static void sym_check(cap_ctx *ctx, const Lay &L)
 {
-    hipLaunchKernelGGL(k_symcheck, dim3(grid_for(L)), dim3(BLOCK), 0, ctx->stream, L, coef7(ctx),
+    hipLaunchKernelGGL(k_symcheck, dim3(grid_spmv(L)), dim3(BLOCK), 0, ctx->stream, L, coef7(ctx),
                        ctx->asym);
 }
 
@@ -985,7 +994,7 @@ extern "C" int cap_pcgsip(cap_ctx *ctx, double *fi, const cap_coef *c, double al
         else
             hipLaunchKernelGGL(k_pupdate_fi, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
                                T[T_PK], T[T_FI], ctx->st);
-        hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+        hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
                            ctx->partials, ctx->ticket, ctx->asym);
     };
@@ -1088,7 +1097,7 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
         launch_wave<OpIcBwdDot, false, F_PCG_SK>(ctx, L, ob, b);
         hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
                            T[T_PK], ctx->st);
-        hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+        hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
                            ctx->partials, ctx->ticket, ctx->asym);
         hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
@@ -1134,7 +1143,7 @@ extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, doubl
                            (const double *)T[T_UK], T[T_PK], ctx->st);
         sip_fwd(ctx, L, T[T_PK], T[T_ZK]);
         sip_bwd(ctx, L, T[T_ZK]);
-        hipLaunchKernelGGL((k_spmv<double, 1, F_BIC_GAM>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+        hipLaunchKernelGGL((k_spmv<double, 1, F_BIC_GAM>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_ZK], T[T_UK], (const double *)T[T_RESO], ctx->st,
                            ctx->partials, ctx->ticket, ctx->asym);
         hipLaunchKernelGGL((k_update<double, 1, F_NONE>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
@@ -1142,7 +1151,7 @@ extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, doubl
                            ctx->partials, ctx->ticket);
         sip_fwd(ctx, L, T[T_RES], T[T_ZK]);
         sip_bwd(ctx, L, T[T_ZK]);
-        hipLaunchKernelGGL((k_spmv<double, 2, F_BIC_ALF>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+        hipLaunchKernelGGL((k_spmv<double, 2, F_BIC_ALF>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_ZK], T[T_VK], (const double *)T[T_RES], ctx->st,
                            ctx->partials, ctx->ticket, ctx->asym);
         hipLaunchKernelGGL((k_update<double, 0, F_BIC_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
@@ -1270,7 +1279,7 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
             launch_wave<OpSipBwdDotT<float>, false, F_PCG_SK>(ctx, L, OpSipBwdDotT<float>(), b, st2);
             hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
                                (const float *)F[F_Z], F[F_P], st2);
-            hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L,
+            hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L,
                                A32, (const float *)F[F_P], F[F_W], (const float *)nullptr, st2,
                                ctx->partials, ctx->ticket, ctx->asym);
             hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
@@ -1359,7 +1368,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                     hipLaunchKernelGGL(k_pupdate_fi, dim3(gb), dim3(BLOCK), 0, s, L,
                                        (const double *)T[T_ZK], T[T_PK], T[T_FI], ctx->st);
                 if (q == 3)
-                    hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
+                    hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s,
                                        L, A, (const double *)T[T_PK], T[T_W],
                                        (const double *)nullptr, ctx->st, ctx->partials, ctx->ticket, ctx->asym);
                 if (q == 4)
@@ -1388,7 +1397,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                     hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
                                        (const float *)F[F_Z], F[F_P], ctx->st);
                 if (q == 3)
-                    hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
+                    hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s,
                                        L, A32, (const float *)F[F_P], F[F_W],
                                        (const float *)nullptr, ctx->st, ctx->partials, ctx->ticket, ctx->asym);
                 if (q == 4)
