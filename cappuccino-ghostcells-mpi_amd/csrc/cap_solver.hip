@@ -321,21 +321,34 @@ __global__ __launch_bounds__(BLOCK) void k_msg_fill(V *p, size_t n)
         p[i] = s;
 }
 
-// spin until a message slot has been written (bounded: on expiry the solve is
-// flagged and ends with an error, every wave still runs to completion; once
-// a solve is flagged, waits give up at once)
+// Spin until a message slot has been written.  Four reads of the slot are
+// kept in flight, so a store is seen about a quarter of a round trip after it
+// lands rather than up to two round trips later; the solve's error flag is
+// read every 32 polls.  Bounded: on expiry the solve is flagged and ends with
+// an error, every wave still runs to completion; once a solve is flagged,
+// waits give up at the next check.
 template <class V>
 __device__ __forceinline__ V msg_wait(const V *p, V v, DevState *st)
 {
-    for (int it = 0; it < SPIN_LIMIT && msg_empty(v); ++it) {
-        if (__hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return v;
+    if (!msg_empty(v)) return v;
+    V q[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        q[j] = coh_load(p);
         __builtin_amdgcn_s_sleep(1);
-        v = coh_load(p);
     }
-    if (msg_empty(v)) {
-        st->err = CAP_ERR_TIMEOUT;
-        st->done = 1;
+    for (int it = 0; it < SPIN_LIMIT; it += 4) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!msg_empty(q[j])) return q[j];
+            q[j] = coh_load(p);
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if ((it & 31) == 28 && __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return v;
     }
+    st->err = CAP_ERR_TIMEOUT;
+    st->done = 1;
     return v;
 }
 
