@@ -62,7 +62,7 @@ struct cap_ctx {
     cap::DevState *st2;            // inner solve of the mixed-precision solver
     cap::DevState *st_host;
     double *partials;
-    int *asym;                    // per flat-kernel block: 1 = SpMV reads aw, as, ab
+    int *asym;                    // per SpMV block: 1 = reads aw, as, ab (FP64 grid, then FP32 grid)
     unsigned *ticket;
     char err[256];
 };

@@ -321,34 +321,21 @@ __global__ __launch_bounds__(BLOCK) void k_msg_fill(V *p, size_t n)
         p[i] = s;
 }
 
-// Spin until a message slot has been written.  Four reads of the slot are
-// kept in flight, so a store is seen about a quarter of a round trip after it
-// lands rather than up to two round trips later; the solve's error flag is
-// read every 32 polls.  Bounded: on expiry the solve is flagged and ends with
-// an error, every wave still runs to completion; once a solve is flagged,
-// waits give up at the next check.
+// spin until a message slot has been written (bounded: on expiry the solve is
+// flagged and ends with an error, every wave still runs to completion; once
+// a solve is flagged, waits give up at once)
 template <class V>
 __device__ __forceinline__ V msg_wait(const V *p, V v, DevState *st)
 {
-    if (!msg_empty(v)) return v;
-    V q[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        q[j] = coh_load(p);
+    for (int it = 0; it < SPIN_LIMIT && msg_empty(v); ++it) {
+        if (__hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return v;
         __builtin_amdgcn_s_sleep(1);
+        v = coh_load(p);
     }
-    for (int it = 0; it < SPIN_LIMIT; it += 4) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (!msg_empty(q[j])) return q[j];
-            q[j] = coh_load(p);
-            __builtin_amdgcn_s_sleep(1);
-        }
-        if ((it & 31) == 28 && __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            return v;
+    if (msg_empty(v)) {
+        st->err = CAP_ERR_TIMEOUT;
+        st->done = 1;
     }
-    st->err = CAP_ERR_TIMEOUT;
-    st->done = 1;
     return v;
 }
 
@@ -728,11 +715,16 @@ static Coef7 coef7(cap_ctx *ctx)
     return A;
 }
 
-// flags the SpMV blocks of this solve's layout that must read aw, as, ab
-static void sym_check(cap_ctx *ctx, const Lay &L)
+// flags the SpMV blocks of this solve's layout that must read aw, as, ab: for
+// the FP64 SpMV's grid in asym, and with f32 also for the FP32 SpMV's grid
+// (the flat kernels' one, which suits four-byte values better) in asym32
+static void sym_check(cap_ctx *ctx, const Lay &L, bool f32 = false)
 {
     hipLaunchKernelGGL(k_symcheck, dim3(grid_spmv(L)), dim3(BLOCK), 0, ctx->stream, L, coef7(ctx),
                        ctx->asym);
+    if (f32)
+        hipLaunchKernelGGL(k_symcheck, dim3(grid_for(L)), dim3(BLOCK), 0, ctx->stream, L, coef7(ctx),
+                           ctx->asym + MAX_RED_BLOCKS);
 }
 
 // Everything a solve needs in device memory, in both layouts.
@@ -1230,7 +1222,7 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
     if (rc) return rc;
     if ((rc = ensure_f32(ctx, L.n))) return rc;
     if ((rc = init_state(ctx, sor, nsw))) return rc;
-    sym_check(ctx, L);
+    sym_check(ctx, L, true);
     double *const *T = ctx->T;
     float *const *F = ctx->F;
     const int gb = grid_for(L);
@@ -1292,9 +1284,9 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
             launch_wave<OpSipBwdDotT<float>, false, F_PCG_SK>(ctx, L, OpSipBwdDotT<float>(), b, st2);
             hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
                                (const float *)F[F_Z], F[F_P], st2);
-            hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L,
+            hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L,
                                A32, (const float *)F[F_P], F[F_W], (const float *)nullptr, st2,
-                               ctx->partials, ctx->ticket, ctx->asym);
+                               ctx->partials, ctx->ticket, ctx->asym + MAX_RED_BLOCKS);
             hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
                                F[F_X], F[F_R], (const float *)F[F_P], (const float *)F[F_W], st2,
                                ctx->partials, ctx->ticket);
@@ -1349,7 +1341,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
     const Lay L = lay_for(ctx, 3, ctx->nk - 2);
     int rc;
     if ((rc = init_state(ctx, 0.0, 1 << 30))) return rc;
-    sym_check(ctx, L);
+    sym_check(ctx, L, true);
     HIPCHK(hipMemsetAsync(ctx->ticket, 0, 64, ctx->stream));
     double *const *T = ctx->T;
     float *const *F = ctx->F;
@@ -1410,9 +1402,9 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                     hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
                                        (const float *)F[F_Z], F[F_P], ctx->st);
                 if (q == 3)
-                    hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s,
+                    hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
                                        L, A32, (const float *)F[F_P], F[F_W],
-                                       (const float *)nullptr, ctx->st, ctx->partials, ctx->ticket, ctx->asym);
+                                       (const float *)nullptr, ctx->st, ctx->partials, ctx->ticket, ctx->asym + MAX_RED_BLOCKS);
                 if (q == 4)
                     hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0,
                                        s, L, F[F_X], F[F_R], (const float *)F[F_P],
@@ -1452,8 +1444,8 @@ extern "C" int cap_init(cap_ctx **out, int ni, int nj, int nk, int flags)
     HIPCHK(hipHostMalloc(&ctx->st_host, sizeof(DevState), hipHostMallocDefault));
     HIPCHK(hipMalloc(&ctx->partials, (size_t)MAX_RED_BLOCKS * 64 * sizeof(double)));
     // no block is known to be symmetric until a solve's k_symcheck says so
-    HIPCHK(hipMalloc(&ctx->asym, (size_t)MAX_RED_BLOCKS * sizeof(int)));
-    HIPCHK(hipMemset(ctx->asym, 1, (size_t)MAX_RED_BLOCKS * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->asym, 2 * (size_t)MAX_RED_BLOCKS * sizeof(int)));
+    HIPCHK(hipMemset(ctx->asym, 1, 2 * (size_t)MAX_RED_BLOCKS * sizeof(int)));
     {   // every reduction slot starts empty (grid_reduce puts the marker back)
         double empty;
         const uint64_t bits = PARTIAL_EMPTY;
