@@ -21,7 +21,9 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SMALL_DIMS = [(12, 10, 9), (20, 37, 23), (36, 36, 36), (52, 70, 30)]
+# (36, 36, 68): 2 x 2 full 16 x 16 tiles and 64 solve planes, so the sweeps'
+# steady steps (every lane of a full tile computing, steps 31..nks) are many
+SMALL_DIMS = [(12, 10, 9), (20, 37, 23), (36, 36, 36), (52, 70, 30), (36, 36, 68)]
 B_FIELDS = ("bb", "bs", "bw", "bp", "bn", "be", "bt")
 
 
