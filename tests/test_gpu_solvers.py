@@ -256,3 +256,49 @@ def test_bpres_bitwise(gpu, O, dims):
     with gpu.Context(*dims) as ctx:
         ctx.bpres(p, g["fx"], g["fy"], g["fz"])
     np.testing.assert_array_equal(p, ref)
+
+
+def _true_residual_l1(torch, dims, c, fi):
+    """sum |su - A fi| over the pcgsip solve range (i, j, k = 3..n-2), in FP64
+    on the device with the reference's 7-point form (pcg-sip.f90:53-55)"""
+    ni, nj, nk = dims
+    g = lambda a: a.view(nk, ni, nj)  # noqa: E731
+    f = g(fi)
+    s = (slice(2, nk - 2), slice(2, ni - 2), slice(2, nj - 2))
+    sh = lambda dk, di, dj: f[2 + dk:nk - 2 + dk, 2 + di:ni - 2 + di, 2 + dj:nj - 2 + dj]  # noqa: E731
+    r = (g(c.ae)[s] * sh(0, 1, 0) + g(c.aw)[s] * sh(0, -1, 0) + g(c.an)[s] * sh(0, 0, 1) +
+         g(c.as_)[s] * sh(0, 0, -1) + g(c.at)[s] * sh(1, 0, 0) + g(c.ab)[s] * sh(-1, 0, 0) +
+         g(c.su)[s] - g(c.ap)[s] * f[s])
+    return float(r.abs().sum())
+
+
+@pytest.mark.parametrize("perturb", [False, True])
+def test_pcgsip_large_properties(gpu, perturb):
+    """128^3 interior (8 x 8 full tiles, 128 planes: many tile hops, long
+    steady sweep phases, 512-block SpMV), beyond what the oracle solves in
+    seconds, checked through properties that do not depend on the size:
+    convergence, the returned residual array sums to the reported L1
+    residual, and the true residual of the returned fi is at the requested
+    level.  `perturb` makes one coupling asymmetric, so the blocks holding
+    it take the SpMV's general path while the rest take the symmetric one."""
+    import torch
+    dims = (132, 132, 132)
+    n = dims[0] * dims[1] * dims[2]
+    dev = {f: torch.zeros(n, dtype=torch.float64, device="cuda") for f in gpu.COEF_FIELDS}
+    coef = gpu.Coef(**dev)
+    fi = torch.zeros(n, dtype=torch.float64, device="cuda")
+    with gpu.Context(*dims, device_pointers=True) as ctx:
+        ctx.build_lognormal(coef, 2024, 1.0)
+        if perturb:   # one interior cell's west coupling, 1 % stronger
+            c = (60 * dims[1] + 70) * dims[2] + 50
+            coef.aw[c] *= 1.01
+        res0 = _true_residual_l1(torch, dims, coef, fi)
+        rep = ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=4000)
+        torch.cuda.synchronize()
+    print(f"128^3 perturb={perturb}: {rep['iters']} iterations, resl/res0 {rep['resl'] / res0:.2e}",
+          flush=True)
+    assert rep["converged"] and 100 < rep["iters"] < 4000
+    assert rep["resl"] / rep["resor"] < 1e-8
+    np.testing.assert_allclose(float(dev["res"].abs().sum()), rep["resl"], rtol=1e-12)
+    true = _true_residual_l1(torch, dims, coef, fi)
+    assert true / res0 < 2e-8, true / res0
