@@ -1084,14 +1084,34 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
         io.dep[0] = T[T_D];
         io.xo[0] = nullptr;
         launch_wave<OpIcDiag, true, F_NONE>(ctx, L, OpIcDiag(), io);
+        // the forward sweep reads {aw, as, ab, d} packed in one 32-byte element
+        const int gp = (int)std::min<size_t>((L.n + BLOCK - 1) / BLOCK, 8192);
+        hipLaunchKernelGGL(k_pack4<double>, dim3(gp), dim3(BLOCK), 0, s, L.n, (const double *)T[T_AW],
+                           (const double *)T[T_AS], (const double *)T[T_AB], (const double *)T[T_D],
+                           ctx->pkd);
     }
+    // As in cap_pcgsip: iteration l's residual update and stop test run in
+    // iteration l+1's forward sweep, its fi update in the p update.
+    int queued = 0;
     auto one = [&]() {
-        WaveIO<OpIcFwd> f;
-        f.in[0] = T[T_RES]; f.in[1] = T[T_AW]; f.in[2] = T[T_AS]; f.in[3] = T[T_AB];
-        f.in[4] = T[T_D];
-        f.dep[0] = T[T_ZK];
-        f.xo[0] = nullptr;
-        launch_wave<OpIcFwd, true, F_NONE>(ctx, L, OpIcFwd(), f);
+        const bool first = queued++ == 0;
+        if (first) {
+            WaveIO<OpIcFwd> f;
+            f.in[0] = T[T_RES]; f.in[1] = T[T_AW]; f.in[2] = T[T_AS]; f.in[3] = T[T_AB];
+            f.in[4] = T[T_D];
+            f.dep[0] = T[T_ZK];
+            f.xo[0] = nullptr;
+            f.pk = ctx->pkd;
+            launch_wave<OpIcFwd, true, F_NONE>(ctx, L, OpIcFwd(), f);
+        } else {
+            WaveIO<OpIcFwdRes> f;
+            f.in[0] = T[T_RES]; f.in[1] = T[T_AW]; f.in[2] = T[T_AS]; f.in[3] = T[T_AB];
+            f.in[4] = T[T_D]; f.in[5] = T[T_W];
+            f.dep[0] = T[T_ZK];
+            f.xo[0] = T[T_RES];
+            f.pk = ctx->pkd;
+            launch_wave<OpIcFwdRes, true, F_PCG_RESL>(ctx, L, OpIcFwdRes(), f);
+        }
         OpIcBwdDot ob;
         ob.small = (double)1e-30f;
         WaveIO<OpIcBwdDot> b;
@@ -1100,17 +1120,29 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
         b.dep[0] = T[T_ZK];
         b.xo[0] = nullptr;
         launch_wave<OpIcBwdDot, false, F_PCG_SK>(ctx, L, ob, b);
-        hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
-                           T[T_PK], ctx->st);
+        if (first)
+            hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L,
+                               (const double *)T[T_ZK], T[T_PK], ctx->st);
+        else
+            hipLaunchKernelGGL(k_pupdate_fi, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
+                               T[T_PK], T[T_FI], ctx->st);
         hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
                            ctx->partials, ctx->ticket, ctx->asym);
-        hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
-                           T[T_RES], (const double *)T[T_PK], (const double *)T[T_W], ctx->st,
-                           ctx->partials, ctx->ticket);
     };
     DevState h;
     if ((rc = iterate(ctx, nsw, one, h))) return rc;
+    if (queued > 0 && !h.err) {
+        if (h.done)   // stopped by a forward sweep: its iteration's fi update
+            hipLaunchKernelGGL(k_fi_last, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                               (const double *)T[T_PK], (const DevState *)ctx->st);
+        else          // ran out of iterations: the last one's whole update
+            hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
+                               T[T_FI], T[T_RES], (const double *)T[T_PK], (const double *)T[T_W],
+                               ctx->st, ctx->partials, ctx->ticket);
+        HIPCHK(hipGetLastError());
+    }
+    if ((rc = read_state(ctx, h))) return rc;
     if ((rc = writeback(ctx, L, fi, c, d, true, false))) return rc;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

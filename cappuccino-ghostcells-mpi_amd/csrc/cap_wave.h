@@ -278,10 +278,32 @@ struct OpIcDiag {
 struct OpIcFwd {
     using T = double;
     static constexpr int NIN = 5, ND = 1, NX = 0, NR = 0;
+    static constexpr int PKN = 4;   // x[1..4] come packed: {aw, as, ab, d}
     CAP_HD static void compute(const double *x, const double *dk, const double *di,
                                const double *dj, double *d, double *, double *)
     {
         d[0] = (x[0] + x[1] * di[0] + x[2] * dj[0] + x[3] * dk[0]) * x[4];
+    }
+};
+
+// The previous ICCG iteration's residual update (iccg.f90, as pcg-sip.f90:
+// 169-171) done by the next iteration's IC(0) forward sweep, as OpSipFwdResT
+// does for PCG-SIP:  r = res - alf*zk;  sum |r|;  z = (r + aw*z(i-1) + ...)*d
+// x = {res, aw, as, ab, d, A pk}; extra output r
+struct OpIcFwdRes {
+    using T = double;
+    static constexpr int NIN = 6, ND = 1, NX = 1, NR = 1;
+    static constexpr int PKN = 4;   // x[1..4] come packed: {aw, as, ab, d}
+    static constexpr int LOOKAHEAD = 12;
+    static constexpr bool TAKES_ALF = true;
+    double alf;
+    CAP_HD void compute(const double *x, const double *dk, const double *di, const double *dj,
+                        double *d, double *xo, double *r) const
+    {
+        const double res = x[0] - alf * x[5];
+        xo[0] = res;
+        r[0] = __builtin_fabs(res);
+        d[0] = (res + x[1] * di[0] + x[2] * dj[0] + x[3] * dk[0]) * x[4];
     }
 };
 
