@@ -150,14 +150,15 @@ __global__ __launch_bounds__(BLOCK) void k_pupdate(Lay L, const V *__restrict__ 
 
 // The previous iteration's fi = fi + alf pk (pcg-sip.f90:168), while its pk
 // and alf are still current, then pk = zk + bet pk (pcg-sip.f90:135)
-__global__ __launch_bounds__(BLOCK) void k_pupdate_fi(Lay L, const double *__restrict__ z,
-                                                      double *__restrict__ p,
-                                                      double *__restrict__ fi, DevState *st)
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_pupdate_fi(Lay L, const V *__restrict__ z,
+                                                      V *__restrict__ p, V *__restrict__ fi,
+                                                      DevState *st)
 {
     if (st->done) return;
-    const double bet = st->bet, alf = st->alf;
+    const V bet = (V)st->bet, alf = (V)st->alf;
     for_cells(L, [&](const Cell &c) {
-        const double po = p[c.e];
+        const V po = p[c.e];
         fi[c.e] = fi[c.e] + alf * po;
         p[c.e] = z[c.e] + bet * po;
     });
@@ -165,11 +166,11 @@ __global__ __launch_bounds__(BLOCK) void k_pupdate_fi(Lay L, const double *__res
 
 // fi = fi + alf pk after a stop in the forward sweep, whose residual update
 // already ran; the state's done flag is set by then (pcg-sip.f90:168)
-__global__ __launch_bounds__(BLOCK) void k_fi_last(Lay L, double *__restrict__ fi,
-                                                   const double *__restrict__ p,
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_fi_last(Lay L, V *__restrict__ fi, const V *__restrict__ p,
                                                    const DevState *st)
 {
-    const double alf = st->alf;
+    const V alf = (V)st->alf;
     for_cells(L, [&](const Cell &c) { fi[c.e] = fi[c.e] + alf * p[c.e]; });
 }
 
@@ -997,7 +998,7 @@ extern "C" int cap_pcgsip(cap_ctx *ctx, double *fi, const cap_coef *c, double al
             hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L,
                                (const double *)T[T_ZK], T[T_PK], ctx->st);
         else
-            hipLaunchKernelGGL(k_pupdate_fi, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
+            hipLaunchKernelGGL(k_pupdate_fi<double>, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
                                T[T_PK], T[T_FI], ctx->st);
         hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
@@ -1007,7 +1008,7 @@ extern "C" int cap_pcgsip(cap_ctx *ctx, double *fi, const cap_coef *c, double al
     if ((rc = iterate(ctx, nsw, one, h))) return rc;
     if (queued > 0 && !h.err) {
         if (h.done)   // stopped by a forward sweep: its iteration's fi update
-            hipLaunchKernelGGL(k_fi_last, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+            hipLaunchKernelGGL(k_fi_last<double>, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
                                (const double *)T[T_PK], (const DevState *)ctx->st);
         else          // ran out of iterations: the last one's whole update
             hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
@@ -1124,7 +1125,7 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
             hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L,
                                (const double *)T[T_ZK], T[T_PK], ctx->st);
         else
-            hipLaunchKernelGGL(k_pupdate_fi, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
+            hipLaunchKernelGGL(k_pupdate_fi<double>, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
                                T[T_PK], T[T_FI], ctx->st);
         hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
@@ -1134,7 +1135,7 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
     if ((rc = iterate(ctx, nsw, one, h))) return rc;
     if (queued > 0 && !h.err) {
         if (h.done)   // stopped by a forward sweep: its iteration's fi update
-            hipLaunchKernelGGL(k_fi_last, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+            hipLaunchKernelGGL(k_fi_last<double>, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
                                (const double *)T[T_PK], (const DevState *)ctx->st);
         else          // ran out of iterations: the last one's whole update
             hipLaunchKernelGGL((k_update<double, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
@@ -1300,28 +1301,42 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
         hi.res0 = resl;
         memcpy(ctx->st_host, &hi, sizeof hi);
         HIPCHK(hipMemcpyAsync(st2, ctx->st_host, sizeof hi, hipMemcpyHostToDevice, s));
+        // the inner PCG-SIP iteration, its update split as in cap_pcgsip
+        int queued = 0;
         auto one = [&]() {
-            WaveIO<OpSipFwdT<float>> f;
-            f.in[0] = F[F_R]; f.in[1] = F[F_BB]; f.in[2] = F[F_BW]; f.in[3] = F[F_BS];
-            f.in[4] = F[F_BP];
-            f.dep[0] = F[F_Z];
-            f.xo[0] = nullptr;
-            f.pk = ctx->pkf;
-            launch_wave<OpSipFwdT<float>, true, F_NONE>(ctx, L, OpSipFwdT<float>(), f, st2);
+            const bool first = queued++ == 0;
+            if (first) {
+                WaveIO<OpSipFwdT<float>> f;
+                f.in[0] = F[F_R]; f.in[1] = F[F_BB]; f.in[2] = F[F_BW]; f.in[3] = F[F_BS];
+                f.in[4] = F[F_BP];
+                f.dep[0] = F[F_Z];
+                f.xo[0] = nullptr;
+                f.pk = ctx->pkf;
+                launch_wave<OpSipFwdT<float>, true, F_NONE>(ctx, L, OpSipFwdT<float>(), f, st2);
+            } else {
+                WaveIO<OpSipFwdResT<float>> f;
+                f.in[0] = F[F_R]; f.in[1] = F[F_BB]; f.in[2] = F[F_BW]; f.in[3] = F[F_BS];
+                f.in[4] = F[F_BP]; f.in[5] = F[F_W];
+                f.dep[0] = F[F_Z];
+                f.xo[0] = F[F_R];
+                f.pk = ctx->pkf;
+                launch_wave<OpSipFwdResT<float>, true, F_PCG_RESL>(ctx, L, OpSipFwdResT<float>(), f, st2);
+            }
             WaveIO<OpSipBwdDotT<float>> b;
             b.in[0] = F[F_Z]; b.in[1] = F[F_BN]; b.in[2] = F[F_BE]; b.in[3] = F[F_BT];
             b.in[4] = F[F_R];
             b.dep[0] = F[F_Z];
             b.xo[0] = nullptr;
             launch_wave<OpSipBwdDotT<float>, false, F_PCG_SK>(ctx, L, OpSipBwdDotT<float>(), b, st2);
-            hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
-                               (const float *)F[F_Z], F[F_P], st2);
+            if (first)
+                hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
+                                   (const float *)F[F_Z], F[F_P], st2);
+            else
+                hipLaunchKernelGGL(k_pupdate_fi<float>, dim3(gb), dim3(BLOCK), 0, s, L,
+                                   (const float *)F[F_Z], F[F_P], F[F_X], st2);
             hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L,
                                A32, (const float *)F[F_P], F[F_W], (const float *)nullptr, st2,
                                ctx->partials, ctx->ticket, ctx->asym + MAX_RED_BLOCKS);
-            hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
-                               F[F_X], F[F_R], (const float *)F[F_P], (const float *)F[F_W], st2,
-                               ctx->partials, ctx->ticket);
         };
         int batch = 4;
         for (int l = 0; l < hi.nsw;) {
@@ -1334,6 +1349,19 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
             memcpy(&hi, ctx->st_host, sizeof hi);
             if (hi.done || hi.err) break;
             batch = std::min(batch * 2, 32);
+        }
+        if (queued > 0 && !hi.err) {
+            if (hi.done)
+                hipLaunchKernelGGL(k_fi_last<float>, dim3(gb), dim3(BLOCK), 0, s, L, F[F_X],
+                                   (const float *)F[F_P], (const DevState *)st2);
+            else
+                hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0, s, L,
+                                   F[F_X], F[F_R], (const float *)F[F_P], (const float *)F[F_W], st2,
+                                   ctx->partials, ctx->ticket);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(ctx->st_host, st2, sizeof hi, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            memcpy(&hi, ctx->st_host, sizeof hi);
         }
         if (hi.err == CAP_ERR_TIMEOUT) return fail(ctx, CAP_ERR_TIMEOUT, "wavefront dependency wait expired");
         inner_total += hi.l;
@@ -1402,7 +1430,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                                 launch_wave<OpSipBwdDot, false, F_PCG_SK>(ctx, L, OpSipBwdDot(), io);
                 }
                 if (q == 2)
-                    hipLaunchKernelGGL(k_pupdate_fi, dim3(gb), dim3(BLOCK), 0, s, L,
+                    hipLaunchKernelGGL(k_pupdate_fi<double>, dim3(gb), dim3(BLOCK), 0, s, L,
                                        (const double *)T[T_ZK], T[T_PK], T[T_FI], ctx->st);
                 if (q == 3)
                     hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s,
