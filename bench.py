@@ -61,8 +61,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-iters", type=int, default=24,
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
-    ap.add_argument("--others", default="cgstab_sip,iccg,mixed,unet,piso,host",
-                    help="comma list of extra solves to time once each (unet: the learned "
+    ap.add_argument("--others", default="cgstab_sip,iccg,mixed,learned,unet,piso,host",
+                    help="comma list of extra solves to time once each (learned: the U-Net "
+                         "guess, trained on solver-generated 64^3 pairs, then the mixed and the "
+                         "FP64 solve from it to the same absolute residual; unet: the learned "
                          "guess's forward pass and training step on 128x96x64; piso: time "
                          "steps of LES + calcuvw + PISO on the 256^3 channel; host: the same "
                          "p' solve through host arrays, PCIe copies included)")
@@ -150,7 +152,11 @@ def main():
         elif other == "iccg":
             r = ctx.iccg(fi, coef, sor=1e-8, nsw=20000)
         elif other == "mixed":
-            r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=1e-4)
+            r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=3e-4)
+        elif other == "learned":
+            solvers["learned"] = _bench_learned(cap, torch, ctx, coef, arrs, dims, fi, args.seed,
+                                                rep["resor"], solvers.get("mixed"))
+            continue
         elif other == "unet":
             solvers["unet"] = _bench_unet(cap, torch)
             continue
@@ -236,6 +242,113 @@ def _bench_host(cap, torch, arrs, dims):
     return {"ms": round(ms, 2), "device_ms": round(r["ms"], 2), "iters": r["iters"],
             "note": "host-pointer context: 8 coefficient arrays up, fi, res and 7 factor arrays "
                     "down per call"}
+
+
+def _l1_residual(arrs, x, dims):
+    """sum |su - M x| over the solve range k, i, j = 3..n-2 (1-based) of a
+    padded reference-layout array x, with M x = ap x - sum a_nb x_nb; the
+    index is (k-1) ni nj + (i-1) nj + j, so the view is (k, i, j)."""
+    ni, nj, nk = dims
+    v = lambda a: a.view(nk, ni, nj)  # noqa: E731
+    c = slice(2, -2)
+    X = v(x)
+    r = (v(arrs["su"])[c, c, c] - v(arrs["ap"])[c, c, c] * X[c, c, c]
+         + v(arrs["ae"])[c, c, c] * X[c, 3:-1, c] + v(arrs["aw"])[c, c, c] * X[c, 1:-3, c]
+         + v(arrs["an"])[c, c, c] * X[c, c, 3:-1] + v(arrs["as_"])[c, c, c] * X[c, c, 1:-3]
+         + v(arrs["at"])[c, c, c] * X[3:-1, c, c] + v(arrs["ab"])[c, c, c] * X[1:-3, c, c])
+    return float(r.abs().sum())
+
+
+def _bench_learned(cap, torch, ctx, coef, arrs, dims, fi, seed, r0, mixed, n_small=64,
+                   ntrain=16, epochs=30):
+    """The learned initial guess on the benchmark system.  The U-Net is
+    trained on ntrain solver-generated pairs of the same operator family at
+    n_small^3 (other seeds: other coefficients and right-hand sides; inputs
+    su / rms, ap / mean, no previous p', target p' / rms), its weights are
+    moved to the full grid, and from its guess (kept only when it lowers the
+    L1 residual of fi = 0) the mixed-precision solve and the FP64 PCG-SIP run
+    to the same absolute residual as the plain solve from zero: the stop test
+    is relative to the solve's own start, so sor is scaled by r0 / r_guess.
+    Times are host wall clock around each part, synchronised."""
+    n = dims[0] - 4
+    ds = (n_small + 4,) * 3
+    npad_s = ds[0] * ds[1] * ds[2]
+    t0 = time.perf_counter()
+    sarr = {f: torch.zeros(npad_s, dtype=torch.float64, device="cuda")
+            for f in ("ae", "aw", "an", "as_", "at", "ab", "ap", "su")}
+    scoef = cap.Coef(**sarr)
+    sctx = cap.Context(*ds, device_pointers=True)
+    interior = lambda a, d: a.view(d[2], d[0], d[1])[2:-2, 2:-2, 2:-2].reshape(-1)  # noqa: E731
+    rms = lambda a: a.pow(2).mean().sqrt()  # noqa: E731
+    X, T = [], []
+    for q in range(ntrain):
+        sctx.build_lognormal(scoef, seed + 1 + q, 1.0)
+        sfi = torch.zeros(npad_s, dtype=torch.float64, device="cuda")
+        rs = sctx.pcgsip(sfi, scoef, alfa=0.0, sor=1e-8, nsw=20000)
+        assert rs["converged"]
+        su, ap, sol = interior(sarr["su"], ds), interior(sarr["ap"], ds), interior(sfi, ds)
+        X.append(torch.stack([su / rms(su), ap / ap.mean(), torch.zeros_like(su)])
+                 .float().reshape(-1).contiguous())
+        T.append((sol / rms(sol)).float().contiguous())
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = cap.UNet(sctx, n_small, n_small, n_small, seed=1)
+    t0 = time.perf_counter()
+    losses = [net.train_step(X[it % ntrain], T[it % ntrain], 5e-4 if it < 20 * ntrain else 1e-4)
+              for it in range(epochs * ntrain)]
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    params = net.get_params()
+    net.close()
+    sctx.close()
+    loss0, loss1 = sum(losses[:ntrain]) / ntrain, sum(losses[-ntrain:]) / ntrain
+    log(f"[bench] learned: {ntrain} pairs at {n_small}^3 in {t_data:.1f}s, {epochs * ntrain} "
+        f"training steps in {t_train:.1f}s, loss {loss0:.3f} -> {loss1:.3f}")
+
+    big = cap.UNet(ctx, n, n, n, seed=1)
+    big.set_params(params)
+    res = {"train": {"grid": [n_small] * 3, "pairs": ntrain, "steps": epochs * ntrain,
+                     "loss_first_epoch": round(loss0, 4), "loss_last_epoch": round(loss1, 4),
+                     "data_s": round(t_data, 2), "train_s": round(t_train, 2)}}
+    r_zero = _l1_residual(arrs, torch.zeros_like(fi), dims)
+    res["r0_check"] = {"torch_l1": r_zero, "solver_resor": r0}
+    for solver in ("mixed", "fp64"):
+        fi.zero_()
+        big.guess(fi, coef)           # warm-up of the full-grid network
+        fi.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        big.guess(fi, coef)
+        torch.cuda.synchronize()
+        t_guess = time.perf_counter() - t0
+        r_guess = _l1_residual(arrs, fi, dims)
+        accepted = r_guess < r0
+        if not accepted:
+            fi.zero_()
+        start = r_guess if accepted else r0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if solver == "mixed":
+            r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8 * r0 / start, nsw=40000,
+                                 inner_rtol=3e-4)
+        else:
+            r = ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8 * r0 / start, nsw=20000)
+        torch.cuda.synchronize()
+        t_solve = time.perf_counter() - t0
+        r_end = _l1_residual(arrs, fi, dims)
+        ms = (t_guess + t_solve) * 1e3
+        res[solver] = {"ms": round(ms, 2), "guess_ms": round(t_guess * 1e3, 2),
+                       "solve_ms": round(t_solve * 1e3, 2), "iters": r["iters"],
+                       "converged": r["converged"], "accepted": accepted,
+                       "r_guess_over_r0": round(r_guess / r0, 4),
+                       "final_l1_over_r0": float(f"{r_end / r0:.3e}")}
+        log(f"[bench] learned+{solver}: {ms:.1f} ms (guess {t_guess * 1e3:.1f} ms, accepted "
+            f"{accepted}, L1 {r_guess / r0:.3f} of r0; solve {t_solve * 1e3:.1f} ms, "
+            f"{r['iters']} iterations), final L1 {r_end / r0:.2e} of r0")
+    if mixed:
+        res["mixed_from_zero"] = {"ms": mixed["ms"], "iters": mixed["iters"]}
+    big.close()
+    return res
 
 
 # (cin, cout, level) of the U-Net's 3x3x3 convolutions, encoder then decoder

@@ -1243,8 +1243,9 @@ static int ensure_f32(cap_ctx *ctx, size_t n)
 
 // FP64 iterative refinement: r = b - A x in FP64 on the tile layout, then
 // an FP32 PCG-SIP solve A e = r (FP32 coefficients, FP32 SIP factors taken
-// from the FP64 factorisation, FP32 SpMV, FP64 dot products and norms),
-// x += e in FP64, until sum|r| / sum|r0| < sor with r the FP64 residual.
+// from the FP64 factorisation, FP32 SpMV, FP64 dot products and norms) to
+// max(inner_rtol, half the remaining reduction) of its start, x += e in FP64,
+// until sum|r| / sum|r0| < sor with r the FP64 residual.
 extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
                                 double sor, int nsw, double inner_rtol, cap_report *rep)
 {
@@ -1294,7 +1295,10 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
         hipLaunchKernelGGL(k_fillf, dim3(gl), dim3(BLOCK), 0, s, F[F_Z], L.n);
         hipLaunchKernelGGL(k_fillf, dim3(gl), dim3(BLOCK), 0, s, F[F_P], L.n);
         memset(&hi, 0, sizeof hi);
-        hi.sor = inner_rtol;
+        // an inner solve reduces its residual by inner_rtol, or by half of
+        // what the outer test still needs when that is less (the last cycle
+        // then stops near the requested residual instead of far below it)
+        hi.sor = std::max(inner_rtol, 0.5 * sor * (res0 + (double)1e-30f) / resl);
         hi.nsw = nsw - inner_total;
         hi.small = (double)1e-30f;
         hi.s0 = (double)1e20f;

@@ -91,8 +91,9 @@ int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
  * residual of the starting fi) by FP64 iterative refinement around an FP32
  * PCG with an FP32 SIP preconditioner and FP32 SpMV; the true residual, the
  * dot products and the norms are FP64.  inner_rtol is the L1 reduction the
- * FP32 inner solve is asked for per refinement step.  rep->iters counts FP32
- * inner iterations. */
+ * FP32 inner solve is asked for per refinement step (half the reduction the
+ * stop test still needs, when that is less).  rep->iters counts FP32 inner
+ * iterations. */
 int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
                      double sor, int nsw, double inner_rtol, cap_report *rep);
 
