@@ -196,5 +196,25 @@ def test_learned_guess_saves_iterations(gpu, O):
     print(f"mean iterations: from zero {np.mean(base_it):.1f}, from the learned guess "
           f"{np.mean(guess_it):.1f}, from previous p' alone {np.mean(warm_it):.1f}")
     assert np.mean(guess_it) < np.mean(base_it)
+
+    # learned + mixed: the guess as the start of the mixed-precision solve,
+    # which stops at the absolute L1 residual of the FP64 solve from zero
+    # (its test is relative to its own start, so sor is scaled)
+    step = 40
+    dev["su"].copy_(torch.from_numpy(srcs[step]))
+    s.su[:] = srcs[step]
+    f3 = torch.zeros(npad, dtype=torch.float64, device="cuda")
+    net.guess(f3, coef, prev=torch.from_numpy(sols[step - 1]).cuda())
+    l1 = lambda x: np.abs(interior(s.su - O.spmv(s, 3, dims[2] - 2, x))).sum()  # noqa: E731
+    r0_l1, rg_l1 = np.abs(interior(s.su)).sum(), l1(f3.cpu().numpy())
+    assert rg_l1 < r0_l1
+    rm = ctx.pcgsip_mixed(f3, coef, alfa=0.0, sor=1e-8 * r0_l1 / rg_l1, nsw=4000, inner_rtol=3e-4)
+    fm = f3.cpu().numpy()
+    err = np.max(np.abs(interior(fm - sols[step]))) / np.max(np.abs(interior(sols[step])))
+    print(f"learned + mixed, step {step}: guess L1 {rg_l1 / r0_l1:.3f} of r0, {rm['iters']} inner "
+          f"iterations, true L1 {l1(fm) / r0_l1:.2e} of r0, max err vs fp64 {err:.2e}")
+    assert rm["converged"]
+    assert l1(fm) < 1e-8 * r0_l1
+    assert err < 1e-5
     net.close()
     ctx.close()
