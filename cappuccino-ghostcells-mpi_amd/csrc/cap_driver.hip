@@ -486,6 +486,10 @@ int solve(cap_ctx *ctx, int solver, double *x, const cap_coef *c, const cap_drv_
     case 3: return cap_iccg(ctx, x, c, sor, o->nsw, rep);
     case 4: return cap_pcgsip(ctx, x, c, o->alfa, sor, o->nsw, rep);
     case 5: return cap_cgstab_sip(ctx, x, c, o->alfa, sor, o->nsw, rep);
+    // 6 has no counterpart in the reference: pcgsip's system solved by FP64
+    // refinement around FP32 PCG-SIP.  Its inner solves restart at each
+    // refinement step, so they may take twice the FP64 iteration limit.
+    case 6: return cap_pcgsip_mixed(ctx, x, c, o->alfa, sor, 2 * o->nsw, 3e-4, rep);
     default: return CAP_ERR_ARG;
     }
 }
@@ -592,7 +596,8 @@ extern "C" int cap_calcp(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, cap_r
         !f->gradw || !f->apu || !f->apv || !f->apw || !f->den)
         return CAP_ERR_ARG;
     if (o->ncorr < 1 || o->npcor < 1 || o->ijkpr < 1 || (size_t)o->ijkpr > ctx->nstd) return CAP_ERR_ARG;
-    if (o->solver != 1 && o->solver != 3 && o->solver != 4 && o->solver != 5) return CAP_ERR_ARG;
+    if (o->solver != 1 && o->solver != 3 && o->solver != 4 && o->solver != 5 && o->solver != 6)
+        return CAP_ERR_ARG;
     const Dim d = dim_of(ctx);
     const size_t n = ctx->nstd;
     const cap_fields F = fields_of(f);
@@ -701,7 +706,7 @@ extern "C" int cap_piso(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pi
             // sor(ip) = 0.1 on all but the last corrector: a default-real literal
             const double sor = icorr != o->ncorr ? (double)0.1f : o->sor;
             cap_report *rep = &reps[(icorr - 1) * o->npcor + ipcorr - 1];
-            const int rc = solve(ctx, 4, x, &C, o, sor, rep);
+            const int rc = solve(ctx, o->solver == 6 ? 6 : 4, x, &C, o, sor, rep);
             if (rc < 0) return rc;
             worst = std::max(worst, rc);
             RCHK(cap_bpres(ctx, x, f->fx, f->fy, f->fz));

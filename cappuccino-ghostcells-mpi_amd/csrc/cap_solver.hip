@@ -1245,7 +1245,10 @@ static int ensure_f32(cap_ctx *ctx, size_t n)
 // an FP32 PCG-SIP solve A e = r (FP32 coefficients, FP32 SIP factors taken
 // from the FP64 factorisation, FP32 SpMV, FP64 dot products and norms) to
 // max(inner_rtol, half the remaining reduction) of its start, x += e in FP64,
-// until sum|r| / sum|r0| < sor with r the FP64 residual.
+// until sum|r| / sum|r0| < sor with r the FP64 residual.  When a step leaves
+// more than 3/4 of the residual it started from, or its inner solve runs
+// out of its 4 (nis + njs + nks) iterations, the solve ends in FP64
+// (cap_pcgsip from the current fi, to the same absolute residual).
 extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
                                 double sor, int nsw, double inner_rtol, cap_report *rep)
 {
@@ -1283,12 +1286,14 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
     if ((rc = read_state(ctx, h))) return rc;
     const double res0 = h.res0;
     double resl = res0;
-    int inner_total = 0, conv = 0, nan = 0;
+    int inner_total = 0, conv = 0, nan = 0, stalled = 0;
+    const int cycle_cap = 4 * (L.nis + L.njs + L.nks);
     DevState *st2 = ctx->st2;
     while (true) {
         if (!(resl == resl)) { nan = 1; break; }
         if (resl / (res0 + (double)1e-30f) < sor) { conv = 1; break; }
         if (inner_total >= nsw) break;
+        const double resl_start = resl;
         // inner solve from e = 0 with rhs r (FP32)
         hipLaunchKernelGGL(k_to_f32, dim3(gl), dim3(BLOCK), 0, s, (const double *)T[T_RES], F[F_R], L.n);
         hipLaunchKernelGGL(k_fillf, dim3(gl), dim3(BLOCK), 0, s, F[F_X], L.n);
@@ -1299,7 +1304,9 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
         // what the outer test still needs when that is less (the last cycle
         // then stops near the requested residual instead of far below it)
         hi.sor = std::max(inner_rtol, 0.5 * sor * (res0 + (double)1e-30f) / resl);
-        hi.nsw = nsw - inner_total;
+        // an inner solve gets at most 4 (nis + njs + nks) iterations: CG on
+        // these grids reduces a residual by inner_rtol in a fraction of that
+        hi.nsw = std::min(nsw - inner_total, cycle_cap);
         hi.small = (double)1e-30f;
         hi.s0 = (double)1e20f;
         hi.res0 = resl;
@@ -1376,14 +1383,36 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
         if ((rc = read_state(ctx, h))) return rc;
         resl = h.red[0];
         if (hi.l == 0) break;
+        // every step asks for at most half of the remaining reduction; one
+        // that leaves more than 3/4 of the FP64 residual, or an inner solve
+        // that used its iterations without reaching its target, means the
+        // FP32 solve cannot resolve this system (too ill-conditioned for FP32)
+        if (!(resl / (res0 + (double)1e-30f) < sor) &&
+            (resl > 0.75 * resl_start || (!hi.conv && hi.l >= hi.nsw))) {
+            stalled = 1;
+            break;
+        }
     }
     if ((rc = writeback(ctx, L, fi, c, d, true, true))) return rc;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    const int st_code = nan ? CAP_NAN : (conv ? CAP_OK : CAP_MAXITER);
+    int st_code = nan ? CAP_NAN : (conv ? CAP_OK : CAP_MAXITER);
+    double resl_out = resl;
+    if (stalled && inner_total < nsw && resl == resl) {
+        // the rest of the solve in FP64: PCG-SIP from the current fi to the
+        // same absolute L1 residual (its stop test is relative to its start)
+        cap_report r64;
+        const double sor64 = sor * (res0 + (double)1e-30f) / resl;
+        rc = cap_pcgsip(ctx, fi, c, alfa, sor64, nsw - inner_total, &r64);
+        if (rc < 0) return rc;
+        st_code = r64.status;
+        resl_out = r64.resl;
+        inner_total += r64.iters;
+        ms += (float)r64.ms;
+    }
     if (rep) {
         rep->resor = res0;
-        rep->resl = resl;
+        rep->resl = resl_out;
         rep->iters = inner_total;
         rep->status = st_code;
         rep->ms = ms;

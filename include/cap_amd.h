@@ -92,8 +92,12 @@ int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
  * PCG with an FP32 SIP preconditioner and FP32 SpMV; the true residual, the
  * dot products and the norms are FP64.  inner_rtol is the L1 reduction the
  * FP32 inner solve is asked for per refinement step (half the reduction the
- * stop test still needs, when that is less).  rep->iters counts FP32 inner
- * iterations. */
+ * stop test still needs, when that is less).  A system too ill-conditioned
+ * for FP32 (a step that leaves more than 3/4 of the residual it started
+ * from, or whose inner solve runs out of its 4 (ni + nj + nk) iterations) is
+ * finished in FP64 by cap_pcgsip from the current fi, to the same absolute
+ * residual.  rep->iters counts FP32 inner iterations plus those FP64
+ * ones; nsw bounds their sum. */
 int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
                      double sor, int nsw, double inner_rtol, cap_report *rep);
 
@@ -166,7 +170,9 @@ typedef struct {
 
 /* Settings the drivers read from modules parameters, boundc and time_mod.
  * solver is calcp's isol: 1 sipsol, 3 iccg, 4 pcgsip, 5 cgstab_sip (the
- * reference's setting); PISO and PIMPLE always use pcgsip.  lbw..lbt are
+ * reference's setting); PISO and PIMPLE use pcgsip.  solver 6 (no
+ * counterpart in the reference) selects cap_pcgsip_mixed (inner_rtol 3e-4,
+ * at most 2 nsw FP32 inner iterations) in all three drivers.  lbw..lbt are
  * device arrays indexed as in cap_bc (outbc reads them when lout is set). */
 typedef struct {
     int ncorr, npcor, solver, nsw;
