@@ -61,12 +61,13 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-iters", type=int, default=24,
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
-    ap.add_argument("--others", default="cgstab_sip,iccg,mixed,learned,unet,piso,host",
+    ap.add_argument("--others", default="cgstab_sip,iccg,mixed,learned,unet,piso,piso_mixed,host",
                     help="comma list of extra solves to time once each (learned: the U-Net "
                          "guess, trained on solver-generated 64^3 pairs, then the mixed and the "
                          "FP64 solve from it to the same absolute residual; unet: the learned "
                          "guess's forward pass and training step on 128x96x64; piso: time "
-                         "steps of LES + calcuvw + PISO on the 256^3 channel; host: the same "
+                         "steps of LES + calcuvw + PISO on the 256^3 channel; piso_mixed: the "
+                         "same with the mixed-precision p' solve; host: the same "
                          "p' solve through host arrays, PCIe copies included)")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: 3 PCG-SIP iterations, then "
@@ -162,6 +163,9 @@ def main():
             continue
         elif other == "piso":
             solvers["piso_step"] = _bench_piso(cap, torch, n)
+            continue
+        elif other == "piso_mixed":
+            solvers["piso_step_mixed"] = _bench_piso(cap, torch, n, solver=6)
             continue
         elif other == "host":
             solvers["pcgsip_host_pointers"] = _bench_host(cap, torch, arrs, dims)
@@ -448,11 +452,12 @@ def _channel_state(torch, n, stretch=2.0, seed=7):
     return f, wd.contiguous().view(-1).clone()
 
 
-def _bench_piso(cap, torch, n, steps=2):
+def _bench_piso(cap, torch, n, steps=2, solver=4):
     """Time steps of the channel: LES viscosity (strain rate + calc_vis_les),
     the momentum predictor (calcuvw: fluxuvw, wall shear, three sipsol) and
     the pressure correction (PISO, 2 correctors, PCG-SIP alfa = 0, the last
-    to 1e-8), all device-resident.  Returns ms per step and per phase."""
+    to 1e-8; solver 6: the mixed-precision form), all device-resident.
+    Returns ms per step and per phase."""
     import numpy as np
     N = n + 4
     f, wd = _channel_state(torch, n)
@@ -463,7 +468,7 @@ def _bench_piso(cap, torch, n, steps=2):
     h = 6.283185307179586 / n
     dt = 0.5 * h / 1.5                                   # Courant 0.5 at the centreline
     o = cap.DrvOpts()
-    o.ncorr, o.npcor, o.solver, o.nsw = 2, 1, 4, 3000
+    o.ncorr, o.npcor, o.solver, o.nsw = 2, 1, solver, 3000
     o.urf, o.sor, o.alfa = 1.0, 1e-8, 0.0
     o.ijkpr = 2 * N * N + 2 * N + 3                      # the cell (3, 3, 3)
     o.const_mflux, o.bdf, o.timestep, o.btime, o.densit = 1, 1, dt, 1.0, 1.0
@@ -504,11 +509,12 @@ def _bench_piso(cap, torch, n, steps=2):
     ctx.close()
     per = {k: round(v / steps, 2) for k, v in phases.items()}
     total = round(sum(per.values()), 2)
-    log(f"[bench] piso step {n}^3: {total} ms ({per}), PCG-SIP iterations per corrector {iters}")
+    log(f"[bench] piso step {n}^3 (solver {solver}): {total} ms ({per}), p' solve iterations per "
+        f"corrector {iters}")
     return {"ms_per_step": total, "phases_ms": per, "pcg_iters": iters, "steps": steps,
             "case": f"{n}^3 channel (x, z uniform; y tanh-stretched 2.0), parabolic u + 10% noise, "
-                    "BDF2, CDS momentum, 2 PISO correctors (PCG-SIP alfa 0, sor 0.1 then 1e-8), "
-                    "LES viscosity each step"}
+                    "BDF2, CDS momentum, 2 PISO correctors (PCG-SIP alfa 0, sor 0.1 then 1e-8"
+                    + (", mixed-precision form" if solver == 6 else "") + "), LES viscosity each step"}
 
 
 def _pmc_traffic(kernel):
