@@ -92,6 +92,7 @@ __device__ __forceinline__ void tap_off(int tap, int &dz, int &dy, int &dx)
 constexpr int BX = 16, BY = 4;
 constexpr int HX = BX + 2, HY = BY + 2;
 constexpr int CC = 32;       // input channels per chunk (one MFMA K step)
+constexpr int WP = 4;        // taps of weights in flight in k_conv
 constexpr int VST = 40;      // LDS stride of a voxel in bf16 (80 bytes: spreads the banks)
 
 template <int NT, int BZ>
@@ -143,17 +144,28 @@ __global__ __launch_bounds__(64 * BZ) void k_conv(int D, int H, int W, const bf1
         }
         __syncthreads();
         const bool kin = c0 + 8 * kg < cinp;
+        // the weights of tap + WP are loaded while tap's MFMAs run (a ring of
+        // WP taps in registers).  Every lane loads, from column 0 of its row
+        // when its channel group lies beyond cinp, so the loads need no branch.
+        uint4 wring[WP][NT];
+        auto wload = [&](int tap, uint4 (&dst)[NT]) {
+            const int k = kin ? tap * cinp + c0 + 8 * kg : 0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const uint4 rb = *reinterpret_cast<const uint4 *>(Wt + (size_t)(t * 16 + mrow) * kp + k);
+                dst[t] = kin ? rb : make_uint4(0u, 0u, 0u, 0u);
+            }
+        };
+#pragma unroll
+        for (int q = 0; q < WP; ++q) wload(q, wring[q]);
+#pragma unroll
         for (int tap = 0; tap < 27; ++tap) {
             int dz, dy, dx;
             tap_off(tap, dz, dy, dx);
-            const int k = tap * cinp + c0 + 8 * kg;
             bf16x8 bv[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                uint4 rb = make_uint4(0u, 0u, 0u, 0u);
-                if (kin) rb = *reinterpret_cast<const uint4 *>(Wt + (size_t)(t * 16 + mrow) * kp + k);
-                memcpy(&bv[t], &rb, 16);
-            }
+            for (int t = 0; t < NT; ++t) memcpy(&bv[t], &wring[tap % WP][t], 16);
+            if (tap + WP < 27) wload(tap + WP, wring[tap % WP]);
 #pragma unroll
             for (int m = 0; m < BY; ++m) {
                 const int hv = ((w + 1 + dz) * HY + (m + 1 + dy)) * HX + (mrow + 1 + dx);
