@@ -246,22 +246,43 @@ __global__ __launch_bounds__(256) void k_wgrad(int D, int H, int W, const bf16_t
     float bsum = 0.f;
     for (int v0 = vb0; v0 < vb1; v0 += 32) {
         __syncthreads();   // the previous step's readers are done
-        for (int q = threadIdx.x; q < 32 * nco16; q += 256) {
-            const int co = q % nco16, k = q / nco16, v = v0 + k;
-            bf16_t val = 0;
-            if (co < cout && v < vb1) val = G[(long)v * cg + co];
-            gt[co * KST + k] = val;
+        // eight channels (16 bytes) per load; channels from cout / cin on are
+        // zero (cg, cx and xoff are multiples of 8, so a group that starts
+        // below cout / cin lies within its voxel's row)
+        for (int q = threadIdx.x; q < 32 * (nco16 / 8); q += 256) {
+            const int c0 = 8 * (q % (nco16 / 8)), k = q / (nco16 / 8), v = v0 + k;
+            bf16_t val[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) val[j] = 0;
+            if (c0 < cout && v < vb1) {
+                const uint4 raw = *reinterpret_cast<const uint4 *>(G + (long)v * cg + c0);
+                memcpy(val, &raw, 16);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (c0 + j >= cout) val[j] = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) gt[(c0 + j) * KST + k] = val[j];
         }
-        for (int q = threadIdx.x; q < 32 * nci16; q += 256) {
-            const int ci = q % nci16, k = q / nci16, v = v0 + k;
-            bf16_t val = 0;
-            if (ci < cin && v < vb1) {
+        for (int q = threadIdx.x; q < 32 * (nci16 / 8); q += 256) {
+            const int c0 = 8 * (q % (nci16 / 8)), k = q / (nci16 / 8), v = v0 + k;
+            bf16_t val[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) val[j] = 0;
+            if (c0 < cin && v < vb1) {
                 const int x = v % W, y = (v / W) % H, z = v / (W * H);
                 const int xx = x + dx, yy = y + dy, zz = z + dz;
-                if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
-                    val = X[(long)((zz * H + yy) * W + xx) * cx + xoff + ci];
+                if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D) {
+                    const uint4 raw =
+                        *reinterpret_cast<const uint4 *>(X + (long)((zz * H + yy) * W + xx) * cx + xoff + c0);
+                    memcpy(val, &raw, 16);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if (c0 + j >= cin) val[j] = 0;
+                }
             }
-            xt[ci * KST + k] = val;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) xt[(c0 + j) * KST + k] = val[j];
         }
         __syncthreads();
         if (db && tap == 13 && (int)threadIdx.x < cout)
