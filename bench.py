@@ -218,6 +218,12 @@ def main():
         "pct_hbm_iteration": round(100 * ITER_BYTES * ncell / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
         "cell_updates_per_s": round(ncell * iters / (ms_step * 1e-3), 1),
         "kernels": kern,
+        # the metric's pair: the plain FP64 drop-in (value) against the learned
+        # guess + mixed-precision solve, both to the same absolute residual
+        "plain_vs_learned_mixed": {
+            "plain_fp64_ms": round(ms_step, 2),
+            "learned_mixed_ms": solvers.get("learned", {}).get("mixed", {}).get("ms"),
+            "mixed_ms": solvers.get("mixed", {}).get("ms")},
         "solvers": solvers,
         "cpu_baseline": cpu,
     }
