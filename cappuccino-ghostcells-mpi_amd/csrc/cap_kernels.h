@@ -41,6 +41,7 @@ enum Fin {
     F_BIC_ALF,     // vres, vv, alf                          cgstab_sip.f90:239-245
     F_BIC_RESL,    // resl, stop test                        cgstab_sip.f90:256-266
     F_RAW,         // keep the sums in red[]
+    F_BIC_RESL_BET,  // F_BIC_RESL on v[0]; unless that stops, F_BIC_BET on v[1]
 };
 
 __device__ __forceinline__ void fin_resl(DevState *st, double resl)
@@ -93,6 +94,14 @@ __device__ __forceinline__ void finalize(DevState *st, const double *v)
         fin_resl(st, v[0]);
     } else if constexpr (FIN == F_RAW) {
         for (int q = 0; q < 4; ++q) st->red[q] = v[q];
+    } else if constexpr (FIN == F_BIC_RESL_BET) {
+        // the stop test, then (when the solve goes on) the next iteration's
+        // bet, om, beto, which its k_dot would compute from the same residual
+        fin_resl(st, v[0]);
+        if (!st->done) {
+            st->om = v[1] * st->gam / (st->alf * st->beto + st->small);
+            st->beto = v[1];
+        }
     }
 }
 

@@ -205,6 +205,30 @@ __global__ __launch_bounds__(BLOCK) void k_update(Lay L, V *__restrict__ fi, V *
         grid_reduce<1, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
 }
 
+// k_update<double, 0, F_BIC_RESL> (fi += alf zk, res -= alf vk, L1 of res and
+// the stop test, cgstab_sip.f90:254-266) with the next iteration's
+// bet = res.reso (cgstab_sip.f90:106-116) summed from the same new residual
+__global__ __launch_bounds__(BLOCK) void k_bic_update_bet(Lay L, double *__restrict__ fi,
+                                                          double *__restrict__ res,
+                                                          const double *__restrict__ p,
+                                                          const double *__restrict__ z,
+                                                          const double *__restrict__ reso,
+                                                          DevState *st, double *partials,
+                                                          unsigned *ticket)
+{
+    if (st->done) return;
+    const double a = st->alf;
+    Acc acc[2];
+    for_cells(L, [&](const Cell &c) {
+        fi[c.e] = fi[c.e] + a * p[c.e];
+        const double r = res[c.e] - a * z[c.e];
+        res[c.e] = r;
+        acc[0] += fabs(r);
+        acc[1] += r * reso[c.e];
+    });
+    grid_reduce<2, F_BIC_RESL_BET>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
+}
+
 // sum x.y -> FIN; optionally y := x first (COPY: reso = res, cgstab_sip.f90:87)
 template <int FIN, bool COPY>
 __global__ __launch_bounds__(BLOCK) void k_dot(Lay L, const double *__restrict__ x,
@@ -1172,11 +1196,13 @@ extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, doubl
                        (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
                        ctx->partials, ctx->ticket);
     sip_factor(ctx, L, alfa);
+    int queued = 0;
     auto one = [&]() {
-        // bet = res.reso; om; beto (cgstab_sip.f90:106-116).  On the first
-        // iteration reso := res (83-94) is folded into the same pass.
-        hipLaunchKernelGGL((k_dot<F_BIC_BET, false>), dim3(gb), dim3(BLOCK), 0, s, L,
-                           (const double *)T[T_RES], T[T_RESO], ctx->st, ctx->partials, ctx->ticket);
+        // bet = res.reso; om; beto (cgstab_sip.f90:106-116).  After the first
+        // iteration the previous iteration's last update has summed it.
+        if (queued++ == 0)
+            hipLaunchKernelGGL((k_dot<F_BIC_BET, false>), dim3(gb), dim3(BLOCK), 0, s, L,
+                               (const double *)T[T_RES], T[T_RESO], ctx->st, ctx->partials, ctx->ticket);
         hipLaunchKernelGGL(k_bic_p, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_RES],
                            (const double *)T[T_UK], T[T_PK], ctx->st);
         sip_fwd(ctx, L, T[T_PK], T[T_ZK]);
@@ -1192,9 +1218,9 @@ extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, doubl
         hipLaunchKernelGGL((k_spmv<double, 2, F_BIC_ALF>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
                            (const double *)T[T_ZK], T[T_VK], (const double *)T[T_RES], ctx->st,
                            ctx->partials, ctx->ticket, ctx->asym);
-        hipLaunchKernelGGL((k_update<double, 0, F_BIC_RESL>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
-                           T[T_RES], (const double *)T[T_ZK], (const double *)T[T_VK], ctx->st,
-                           ctx->partials, ctx->ticket);
+        hipLaunchKernelGGL(k_bic_update_bet, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI], T[T_RES],
+                           (const double *)T[T_ZK], (const double *)T[T_VK], (const double *)T[T_RESO],
+                           ctx->st, ctx->partials, ctx->ticket);
     };
     // reso = res before the first iteration (cgstab_sip.f90:87)
     hipLaunchKernelGGL((k_dot<F_RAW, true>), dim3(gb), dim3(BLOCK), 0, s, L,
