@@ -48,6 +48,8 @@ struct cap_ctx {
     // element (FP64 and the FP32 copy of the mixed-precision solver)
     double *pkd;
     float *pkf;
+    // iccg's backward sweep: {ae, an, at, d} packed four to an element
+    double *pkb;
     int bad_launch;               // a sweep was queued without its packed factors
     double *G[CAP_NSTAGE];      // stencil staging (reference layout)
     size_t Gsz[CAP_NSTAGE];

@@ -674,6 +674,8 @@ static int ensure_tiled(cap_ctx *ctx, size_t n)
     for (int v = 0; v < T_COUNT; ++v) ctx->T[v] = (double *)((char *)ctx->tpool + stride * v);
     if (ctx->pkd) { (void)hipFree(ctx->pkd); ctx->pkd = nullptr; }
     HIPCHK(hipMalloc(&ctx->pkd, 4 * n * sizeof(double)));
+    if (ctx->pkb) { (void)hipFree(ctx->pkb); ctx->pkb = nullptr; }
+    HIPCHK(hipMalloc(&ctx->pkb, 4 * n * sizeof(double)));
 
     ctx->tcap = n;
     return CAP_OK;
@@ -1109,11 +1111,15 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
         io.dep[0] = T[T_D];
         io.xo[0] = nullptr;
         launch_wave<OpIcDiag, true, F_NONE>(ctx, L, OpIcDiag(), io);
-        // the forward sweep reads {aw, as, ab, d} packed in one 32-byte element
+        // the forward sweep reads {aw, as, ab, d} packed in one 32-byte
+        // element, the backward sweep {ae, an, at, d}
         const int gp = (int)std::min<size_t>((L.n + BLOCK - 1) / BLOCK, 8192);
         hipLaunchKernelGGL(k_pack4<double>, dim3(gp), dim3(BLOCK), 0, s, L.n, (const double *)T[T_AW],
                            (const double *)T[T_AS], (const double *)T[T_AB], (const double *)T[T_D],
                            ctx->pkd);
+        hipLaunchKernelGGL(k_pack4<double>, dim3(gp), dim3(BLOCK), 0, s, L.n, (const double *)T[T_AE],
+                           (const double *)T[T_AN], (const double *)T[T_AT], (const double *)T[T_D],
+                           ctx->pkb);
     }
     // As in cap_pcgsip: iteration l's residual update and stop test run in
     // iteration l+1's forward sweep, its fi update in the p update.
@@ -1144,6 +1150,7 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
         b.in[4] = T[T_D]; b.in[5] = T[T_RES];
         b.dep[0] = T[T_ZK];
         b.xo[0] = nullptr;
+        b.pk = ctx->pkb;
         launch_wave<OpIcBwdDot, false, F_PCG_SK>(ctx, L, ob, b);
         if (first)
             hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L,

@@ -1340,5 +1340,6 @@ void cap_free_extra(cap_ctx *ctx)
     for (int q = 0; q < CAP_NSTAGE; ++q) if (ctx->G[q]) (void)hipFree(ctx->G[q]);
     for (int q = 0; q < F_COUNT; ++q) if (ctx->F[q]) (void)hipFree(ctx->F[q]);
     if (ctx->pkd) (void)hipFree(ctx->pkd);
+    if (ctx->pkb) (void)hipFree(ctx->pkb);
     if (ctx->pkf) (void)hipFree(ctx->pkf);
 }

@@ -313,6 +313,8 @@ struct OpIcFwdRes {
 struct OpIcBwdDot {
     using T = double;
     static constexpr int NIN = 6, ND = 1, NX = 0, NR = 1;
+    // x[1..4] = {ae, an, at, d} from one packed element; four loads a step
+    static constexpr int PKN = 4, LOOKAHEAD = 12;
     double small;
     CAP_HD void compute(const double *x, const double *dk, const double *di,
                         const double *dj, double *d, double *, double *r) const
