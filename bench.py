@@ -212,7 +212,7 @@ def main():
                    "cells": ncell, "padded": list(dims), "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _pmc_traffic(dom),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _pmc_traffic(dom, n),
                      "bytes_per_launch": KERNEL_BYTES[dom] * ncell,
                      "avg_launch_ms": kern[dom]["ms"]},
         "pct_hbm_iteration": round(100 * ITER_BYTES * ncell / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1),
@@ -523,17 +523,22 @@ def _bench_piso(cap, torch, n, steps=2, solver=4):
                     + (", mixed-precision form" if solver == 6 else "") + "), LES viscosity each step"}
 
 
-def _pmc_traffic(kernel):
-    """HBM bytes per launch of a kernel from the committed counter summary
-    (profiles/pmc_traffic.txt, lines `traffic <kernel> <bytes>`), or None."""
+def _pmc_traffic(kernel, n):
+    """HBM bytes per launch of a kernel at an n^3 grid from the committed
+    counter summary (profiles/pmc_traffic.txt, lines `traffic <kernel> <bytes>`
+    measured at the grid named by its `grid <n>` line), or None: the counters
+    were read at one grid size, so any other size reports no traffic."""
     try:
+        grid, found = None, None
         for line in open(os.path.join(ROOT, "profiles", "pmc_traffic.txt")):
             f = line.split()
+            if len(f) == 2 and f[0] == "grid":
+                grid = int(f[1])
             if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
-                return int(f[2])
-    except OSError:
-        pass
-    return None
+                found = int(f[2])
+        return found if grid == n else None
+    except (OSError, ValueError):
+        return None
 
 
 def _cpu_model():

@@ -49,7 +49,9 @@ struct cap_ctx {
     double *pkd;
     float *pkf;
     // iccg's backward sweep: {ae, an, at, d} packed four to an element
+    // (allocated by the first ICCG solve; pbcap elements)
     double *pkb;
+    size_t pbcap;
     int bad_launch;               // a sweep was queued without its packed factors
     double *G[CAP_NSTAGE];      // stencil staging (reference layout)
     size_t Gsz[CAP_NSTAGE];

@@ -674,10 +674,20 @@ static int ensure_tiled(cap_ctx *ctx, size_t n)
     for (int v = 0; v < T_COUNT; ++v) ctx->T[v] = (double *)((char *)ctx->tpool + stride * v);
     if (ctx->pkd) { (void)hipFree(ctx->pkd); ctx->pkd = nullptr; }
     HIPCHK(hipMalloc(&ctx->pkd, 4 * n * sizeof(double)));
-    if (ctx->pkb) { (void)hipFree(ctx->pkb); ctx->pkb = nullptr; }
-    HIPCHK(hipMalloc(&ctx->pkb, 4 * n * sizeof(double)));
 
     ctx->tcap = n;
+    return CAP_OK;
+}
+
+// iccg's backward-sweep pack {ae, an, at, d}: only cap_iccg reads it, so it
+// is allocated on the first ICCG solve of a size, not with the other vectors
+static int ensure_pkb(cap_ctx *ctx, size_t n)
+{
+    if (ctx->pbcap >= n) return CAP_OK;
+    if (ctx->pkb) { (void)hipFree(ctx->pkb); ctx->pkb = nullptr; }
+    ctx->pbcap = 0;
+    HIPCHK(hipMalloc(&ctx->pkb, 4 * n * sizeof(double)));
+    ctx->pbcap = n;
     return CAP_OK;
 }
 
@@ -940,14 +950,38 @@ static int writeback(cap_ctx *ctx, const Lay &L, double *fi_host, const cap_coef
     return CAP_OK;
 }
 
+// After a failed solve (an expired wait, a tile claim out of range, a sweep
+// that was not queued) the context's shared reduction and claim state may be
+// out of step: a block that lost the wait may still store its partial later,
+// and the tile claim counter may not match claim_base.  Wait for every queued
+// kernel, then put the empty marker back into every partial slot and zero
+// the ticket and the claim counter, so that the next solve on the context
+// starts from a clean state.  Returns `code` (the failure being reported).
+static int recover(cap_ctx *ctx, int code, const char *what)
+{
+    (void)hipStreamSynchronize(ctx->stream);
+    double empty;
+    const uint64_t bits = PARTIAL_EMPTY;
+    memcpy(&empty, &bits, sizeof empty);
+    hipLaunchKernelGGL(k_fill, dim3(64), dim3(BLOCK), 0, ctx->stream, ctx->partials,
+                       (size_t)MAX_RED_BLOCKS * 64, empty);
+    (void)hipMemsetAsync(ctx->ticket, 0, 64, ctx->stream);
+    (void)hipMemsetAsync(ctx->claim, 0, 64, ctx->stream);
+    ctx->claim_base = 0;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        return fail(ctx, CAP_ERR_HIP, "context reset after a failed solve did not complete");
+    return fail(ctx, code, what);
+}
+
 static int finish_report(cap_ctx *ctx, const DevState &h, cap_report *rep, float ms)
 {
     if (ctx->bad_launch) {
         ctx->bad_launch = 0;
-        return fail(ctx, CAP_ERR_ARG, "a SIP sweep was queued without its packed factors");
+        return recover(ctx, CAP_ERR_ARG, "a SIP sweep was queued without its packed factors");
     }
-    if (h.err == CAP_ERR_TIMEOUT) return fail(ctx, CAP_ERR_TIMEOUT, "wavefront dependency wait expired");
-    if (h.err == CAP_ERR_ARG) return fail(ctx, CAP_ERR_ARG, "wavefront tile claim out of range");
+    if (h.err == CAP_ERR_TIMEOUT)
+        return recover(ctx, CAP_ERR_TIMEOUT, "wavefront dependency wait expired");
+    if (h.err == CAP_ERR_ARG) return recover(ctx, CAP_ERR_ARG, "wavefront tile claim out of range");
     if (rep) {
         rep->resor = h.l >= 1 ? h.resor : h.res0;
         rep->resl = h.resl;
@@ -1093,6 +1127,7 @@ extern "C" int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor,
     SolveIn d;
     int rc = prepare(ctx, L, fi, c, d, false, false);
     if (rc) return rc;
+    if ((rc = ensure_pkb(ctx, L.n))) return rc;
     if ((rc = init_state(ctx, sor, nsw))) return rc;
     sym_check(ctx, L);
     double *const *T = ctx->T;
@@ -1407,7 +1442,9 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
             HIPCHK(hipStreamSynchronize(s));
             memcpy(&hi, ctx->st_host, sizeof hi);
         }
-        if (hi.err == CAP_ERR_TIMEOUT) return fail(ctx, CAP_ERR_TIMEOUT, "wavefront dependency wait expired");
+        if (hi.err == CAP_ERR_TIMEOUT)
+            return recover(ctx, CAP_ERR_TIMEOUT, "wavefront dependency wait expired");
+        if (hi.err == CAP_ERR_ARG) return recover(ctx, CAP_ERR_ARG, "wavefront tile claim out of range");
         inner_total += hi.l;
         hipLaunchKernelGGL(k_add_corr, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI], (const float *)F[F_X]);
         hipLaunchKernelGGL((k_resid<F_RAW>), dim3(gb), dim3(BLOCK), 0, s, L, A,

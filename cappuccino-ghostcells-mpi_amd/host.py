@@ -173,27 +173,50 @@ def load():
     return L
 
 
-def _ptr(a):
-    if a is None:
-        return None
+def _check_array(a, kind, n=None, device=None, what="array"):
+    """Raise CapError unless `a` is a contiguous array of dtype `kind`
+    ("float64", "float32" or "int32") with exactly n elements (when n is
+    given) that lives on the device when `device` is True and on the host
+    when it is False.  The C side reads and writes exactly n elements at the
+    address, so every mismatch is refused here rather than left to overrun a
+    buffer or to hand a host address to a kernel."""
     if hasattr(a, "data_ptr"):            # torch tensor
         import torch
-        assert a.dtype == torch.float64 and a.is_contiguous()
+        want = {"float64": torch.float64, "float32": torch.float32, "int32": torch.int32}[kind]
+        if a.dtype != want:
+            raise CapError(f"{what}: dtype {a.dtype}, expected {kind}")
+        if not a.is_contiguous():
+            raise CapError(f"{what}: not contiguous")
+        if device is not None and bool(a.is_cuda) != bool(device):
+            raise CapError(f"{what}: a {'device' if a.is_cuda else 'host'} tensor passed to a "
+                           f"{'device' if device else 'host'}-pointer context")
+        if n is not None and a.numel() != n:
+            raise CapError(f"{what}: {a.numel()} elements, expected {n}")
         return a.data_ptr()
     import numpy as np
-    assert a.dtype == np.float64 and a.flags.c_contiguous
+    if not isinstance(a, np.ndarray):
+        raise CapError(f"{what}: {type(a).__name__} is neither a numpy array nor a torch tensor")
+    if a.dtype != np.dtype(kind):
+        raise CapError(f"{what}: dtype {a.dtype}, expected {kind}")
+    if not a.flags.c_contiguous:
+        raise CapError(f"{what}: not contiguous")
+    if device:
+        raise CapError(f"{what}: a host array passed to a device-pointer context")
+    if n is not None and a.size != n:
+        raise CapError(f"{what}: {a.size} elements, expected {n}")
     return a.ctypes.data
 
 
-def _iptr(a):
+def _ptr(a, n=None, device=None, what="array"):
+    """float64 array (numpy, or torch on the device); None stays NULL."""
+    if a is None:
+        return None
+    return _check_array(a, "float64", n, device, what)
+
+
+def _iptr(a, n=None, device=None, what="array"):
     """int32 array (numpy, or torch on the device)"""
-    if hasattr(a, "data_ptr"):
-        import torch
-        assert a.dtype == torch.int32 and a.is_contiguous()
-        return a.data_ptr()
-    import numpy as np
-    assert a.dtype == np.int32 and a.flags.c_contiguous
-    return a.ctypes.data
+    return _check_array(a, "int32", n, device, what)
 
 
 class Coef:
@@ -203,8 +226,8 @@ class Coef:
         for f in COEF_FIELDS:
             setattr(self, f, arrays.get(f))
 
-    def _struct(self):
-        return _Coef(*[_ptr(getattr(self, f)) for f in COEF_FIELDS])
+    def _struct(self, ctx):
+        return _Coef(*[ctx._f(getattr(self, f), f) for f in COEF_FIELDS])
 
 
 class Context:
@@ -238,6 +261,20 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
+    @property
+    def n(self):
+        ni, nj, nk = self.dims
+        return ni * nj * nk
+
+    def _f(self, a, what="array", mult=1):
+        """Pointer of a float64 field of mult * ni*nj*nk elements on the
+        context's side (device or host); None stays NULL."""
+        return _ptr(a, mult * self.n, self.device_pointers, what)
+
+    def _lb_sizes(self):
+        ni, nj, nk = self.dims
+        return (nj * nk, nj * nk, ni * nk, ni * nk, ni * nj, ni * nj)
+
     def error(self):
         return self._lib.cap_last_error(self._h).decode()
 
@@ -248,8 +285,8 @@ class Context:
 
     def _solve(self, fn, fi, coef, *args):
         rep = _Report()
-        st = coef._struct()
-        rc = fn(self._h, _ptr(fi), C.byref(st), *args, C.byref(rep))
+        st = coef._struct(self)
+        rc = fn(self._h, self._f(fi), C.byref(st), *args, C.byref(rep))
         self._check(rc, fn.__name__, allow=(CAP_OK, CAP_MAXITER, CAP_NAN))
         return dict(resor=rep.resor, resl=rep.resl, iters=rep.iters, status=rep.status,
                     converged=rep.status == CAP_OK, ms=rep.ms)
@@ -272,12 +309,12 @@ class Context:
 
     # --- stencils -----------------------------------------------------------
     def grad_gauss(self, u, geom, dudx, dudy, dudz):
-        arr = (C.c_void_p * 13)(*[_ptr(geom[f]) for f in GEOM_FIELDS])
-        rc = self._lib.cap_grad_gauss(self._h, _ptr(u), arr, _ptr(dudx), _ptr(dudy), _ptr(dudz))
+        arr = (C.c_void_p * 13)(*[self._f(geom[f]) for f in GEOM_FIELDS])
+        rc = self._lib.cap_grad_gauss(self._h, self._f(u), arr, self._f(dudx), self._f(dudy), self._f(dudz))
         self._check(rc, "cap_grad_gauss", allow=(CAP_OK,))
 
     def bpres(self, p, fx, fy, fz):
-        rc = self._lib.cap_bpres(self._h, _ptr(p), _ptr(fx), _ptr(fy), _ptr(fz))
+        rc = self._lib.cap_bpres(self._h, self._f(p), self._f(fx), self._f(fy), self._f(fz))
         self._check(rc, "cap_bpres", allow=(CAP_OK,))
 
     def time_kernels(self, fp32=False, reps=20):
@@ -293,31 +330,32 @@ class Context:
         return [(nj, 1, ni * nj), (1, ni * nj, nj), (ni * nj, nj, 1)][d]
 
     def _fields(self, f):
-        return _Fields(*[_ptr(f.get(q)) for q in FIELD_NAMES])
+        return _Fields(*[self._f(f.get(q), q, 3 if q.startswith("grad") else 1)
+                         for q in FIELD_NAMES])
 
     def fluxmass(self, f, d, acfe, acfw, fcf):
         """Direction d: 0 east (ar1, fx), 1 north (ar2, fy), 2 top (ar3, fz)."""
         arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
         st = self._fields(f)
-        rc = self._lib.cap_fluxmass(self._h, C.byref(st), *self._dir(d), *map(_ptr, arv),
-                                    _ptr(f[("fx", "fy", "fz")[d]]),
-                                    _ptr(acfe), _ptr(acfw), _ptr(fcf))
+        rc = self._lib.cap_fluxmass(self._h, C.byref(st), *self._dir(d), *map(self._f, arv),
+                                    self._f(f[("fx", "fy", "fz")[d]]),
+                                    self._f(acfe), self._f(acfw), self._f(fcf))
         self._check(rc, "cap_fluxmass", allow=(CAP_OK,))
 
     def piso_assemble_pressure_eq(self, f, d, acfe, acfw, fcf, ipc, bp, su):
         arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
         st = self._fields(f)
         rc = self._lib.cap_piso_assemble_pressure_eq(
-            self._h, C.byref(st), *self._dir(d), *map(_ptr, arv), _ptr(f[("fx", "fy", "fz")[d]]),
-            _ptr(acfe), _ptr(acfw), _ptr(fcf), ipc, _ptr(bp), _ptr(su))
+            self._h, C.byref(st), *self._dir(d), *map(self._f, arv), self._f(f[("fx", "fy", "fz")[d]]),
+            self._f(acfe), self._f(acfw), self._f(fcf), ipc, self._f(bp), self._f(su))
         self._check(rc, "cap_piso_assemble_pressure_eq", allow=(CAP_OK,))
 
     def assemble_pressure_eq(self, f, d, acfe, acfw, fcf, su, sv, sw):
         arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
         st = self._fields(f)
         rc = self._lib.cap_assemble_pressure_eq(
-            self._h, C.byref(st), *self._dir(d), *map(_ptr, arv), _ptr(f[("fx", "fy", "fz")[d]]),
-            _ptr(acfe), _ptr(acfw), _ptr(fcf), _ptr(su), _ptr(sv), _ptr(sw))
+            self._h, C.byref(st), *self._dir(d), *map(self._f, arv), self._f(f[("fx", "fy", "fz")[d]]),
+            self._f(acfe), self._f(acfw), self._f(fcf), self._f(su), self._f(sv), self._f(sw))
         self._check(rc, "cap_assemble_pressure_eq", allow=(CAP_OK,))
 
     def fluxuvw(self, f, d, vis, acfe, acfw, fcf, su, sv, sw, bp, bt, bb, schemes, gds):
@@ -326,18 +364,19 @@ class Context:
         arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
         st = self._fields(f)
         rc = self._lib.cap_fluxuvw(
-            self._h, C.byref(st), _ptr(vis), *self._dir(d), *map(_ptr, arv),
-            _ptr(f[("fx", "fy", "fz")[d]]), _ptr(acfe), _ptr(acfw), _ptr(fcf), _ptr(su),
-            _ptr(sv), _ptr(sw), _ptr(bp), _ptr(bt), _ptr(bb), int(schemes), float(gds))
+            self._h, C.byref(st), self._f(vis), *self._dir(d), *map(self._f, arv),
+            self._f(f[("fx", "fy", "fz")[d]]), self._f(acfe), self._f(acfw), self._f(fcf), self._f(su),
+            self._f(sv), self._f(sw), self._f(bp), self._f(bt), self._f(bb), int(schemes), float(gds))
         self._check(rc, "cap_fluxuvw", allow=(CAP_OK,))
 
     def update_ghostcells(self, var, geom, lb, inl):
         """var: [u, v, w, te, ed, t, vis] (updated in place); geom: [fx, fy, fz,
         ar1x .. ar3z]; lb: int32 arrays [lbw, lbe, lbs, lbn, lbb, lbt]; inl:
         (uin, vin, win, tein, edin, tin)."""
-        V = (C.c_void_p * 7)(*[_ptr(a) for a in var])
-        G = (C.c_void_p * 12)(*[_ptr(a) for a in geom])
-        bc = _Bc(*[_iptr(a) for a in lb], *[float(x) for x in inl])
+        V = (C.c_void_p * 7)(*[self._f(a) for a in var])
+        G = (C.c_void_p * 12)(*[self._f(a) for a in geom])
+        bc = _Bc(*[_iptr(a, n, self.device_pointers, "lb")
+                   for a, n in zip(lb, self._lb_sizes())], *[float(x) for x in inl])
         rc = self._lib.cap_update_ghostcells(self._h, V, G, C.byref(bc))
         self._check(rc, "cap_update_ghostcells", allow=(CAP_OK,))
 
@@ -345,7 +384,7 @@ class Context:
         """Returns (CoNum, meanCoNum, timestep, time) after CourantNo.h."""
         ts, tm = C.c_double(timestep), C.c_double(time)
         out = (C.c_double * 2)()
-        rc = self._lib.cap_courant(self._h, _ptr(f1), _ptr(f2), _ptr(f3), _ptr(vol), _ptr(su),
+        rc = self._lib.cap_courant(self._h, self._f(f1), self._f(f2), self._f(f3), self._f(vol), self._f(su),
                                    C.byref(ts), C.byref(tm), 1 if conumfix else 0,
                                    float(conumfixvalue), out)
         self._check(rc, "cap_courant", allow=(CAP_OK,))
@@ -354,20 +393,20 @@ class Context:
     def fluxmc(self, f, d, fmcor):
         st = self._fields(f)
         rc = self._lib.cap_fluxmc(self._h, C.byref(st), *self._dir(d),
-                                  _ptr(f[("fx", "fy", "fz")[d]]), _ptr(fmcor))
+                                  self._f(f[("fx", "fy", "fz")[d]]), self._f(fmcor))
         self._check(rc, "cap_fluxmc", allow=(CAP_OK,))
 
     def continuity(self, f1, f2, f3):
         out = (C.c_double * 2)()
-        rc = self._lib.cap_continuity(self._h, _ptr(f1), _ptr(f2), _ptr(f3), out)
+        rc = self._lib.cap_continuity(self._h, self._f(f1), self._f(f2), self._f(f3), out)
         self._check(rc, "cap_continuity", allow=(CAP_OK,))
         return list(out)
 
     # ---------------------------------------------- pressure-correction drivers
-    @staticmethod
-    def _flow(f):
+    def _flow(self, f):
         """cap_flow from a dict of device tensors (missing names: NULL)."""
-        return _Flow(*[_ptr(f[n]) if f.get(n) is not None else None for n in FLOW_NAMES])
+        return _Flow(*[self._f(f.get(n), n, 3 if n.startswith("grad") else 1)
+                       for n in FLOW_NAMES])
 
     @staticmethod
     def _reports(reps, n):
@@ -414,36 +453,30 @@ class Context:
         """`call calcuvw`; returns the three sipsol reports (u, v, w)."""
         st = self._flow(f)
         reps = (_Report * 3)()
-        rc = self._lib.cap_calcuvw(self._h, C.byref(st), _ptr(spv), C.byref(opts), C.byref(uvw), reps)
+        rc = self._lib.cap_calcuvw(self._h, C.byref(st), self._f(spv), C.byref(opts), C.byref(uvw), reps)
         self._check(rc, "cap_calcuvw")
         return self._reports(reps, 3)
 
     def find_strain_rate(self, f, strain, vorticity):
         st = self._flow(f)
-        rc = self._lib.cap_find_strain_rate(self._h, C.byref(st), _ptr(strain), _ptr(vorticity))
+        rc = self._lib.cap_find_strain_rate(self._h, C.byref(st), self._f(strain), self._f(vorticity))
         self._check(rc, "cap_find_strain_rate", allow=(CAP_OK,))
 
     def calc_vis_les(self, f, strain, walldist, les):
         st = self._flow(f)
-        rc = self._lib.cap_calc_vis_les(self._h, C.byref(st), _ptr(strain), _ptr(walldist),
+        rc = self._lib.cap_calc_vis_les(self._h, C.byref(st), self._f(strain), self._f(walldist),
                                         C.byref(les))
         self._check(rc, "cap_calc_vis_les", allow=(CAP_OK,))
 
     def build_lognormal(self, coef, seed, sigma=1.0):
-        st = coef._struct()
+        st = coef._struct(self)
         rc = self._lib.cap_build_lognormal(self._h, seed, sigma, C.byref(st))
         self._check(rc, "cap_build_lognormal", allow=(CAP_OK,))
 
 
-def _fptr(a):
+def _fptr(a, n=None, device=None, what="array"):
     """float32 array (numpy, or torch on the device)"""
-    if hasattr(a, "data_ptr"):
-        import torch
-        assert a.dtype == torch.float32 and a.is_contiguous()
-        return a.data_ptr()
-    import numpy as np
-    assert a.dtype == np.float32 and a.flags.c_contiguous
-    return a.ctypes.data
+    return _check_array(a, "float32", n, device, what)
 
 
 class UNet:
@@ -473,6 +506,19 @@ class UNet:
         except Exception:
             pass
 
+    def _in(self, x):
+        nz, ny, nx = self.shape
+        return _fptr(x, 3 * nx * ny * nz, self._ctx.device_pointers, "unet input")
+
+    def _out(self, y):
+        nz, ny, nx = self.shape
+        return _fptr(y, nx * ny * nz, self._ctx.device_pointers, "unet output")
+
+    def _grads(self, g):
+        if g is None:
+            return None
+        return _fptr(g, self.num_params, self._ctx.device_pointers, "unet gradients")
+
     def _chk(self, rc, what):
         self._ctx._check(rc, what, allow=(CAP_OK,))
 
@@ -488,18 +534,18 @@ class UNet:
         self._chk(self._lib.cap_unet_set_params(self._h, p.ctypes.data), "cap_unet_set_params")
 
     def forward(self, x, out):
-        self._chk(self._lib.cap_unet_forward(self._h, _fptr(x), _fptr(out)), "cap_unet_forward")
+        self._chk(self._lib.cap_unet_forward(self._h, self._in(x), self._out(out)), "cap_unet_forward")
 
     def gradients(self, x, target, grads=None):
         loss = C.c_float()
-        rc = self._lib.cap_unet_gradients(self._h, _fptr(x), _fptr(target), C.byref(loss),
-                                          _fptr(grads) if grads is not None else None)
+        rc = self._lib.cap_unet_gradients(self._h, self._in(x), self._out(target), C.byref(loss),
+                                          self._grads(grads))
         self._chk(rc, "cap_unet_gradients")
         return loss.value
 
     def train_step(self, x, target, lr):
         loss = C.c_float()
-        rc = self._lib.cap_unet_train_step(self._h, _fptr(x), _fptr(target), float(lr),
+        rc = self._lib.cap_unet_train_step(self._h, self._in(x), self._out(target), float(lr),
                                            C.byref(loss))
         self._chk(rc, "cap_unet_train_step")
         return loss.value
@@ -508,18 +554,18 @@ class UNet:
         """fi (device, reference layout) becomes the guess; returns
         (s, a, ps, alpha, beta), the normalisers and the fitted weights."""
         sc = (C.c_double * 5)()
-        st = coef._struct()
-        rc = self._lib.cap_unet_guess(self._ctx._h, self._h, _ptr(fi),
-                                      _ptr(prev) if prev is not None else None, C.byref(st), sc)
+        st = coef._struct(self._ctx)
+        rc = self._lib.cap_unet_guess(self._ctx._h, self._h, self._ctx._f(fi, 'fi'),
+                                      self._ctx._f(prev, 'prev'), C.byref(st), sc)
         self._chk(rc, "cap_unet_guess")
         return tuple(sc)
 
     def pcgsip(self, fi, coef, prev=None, alfa=0.0, sor=1e-8, nsw=2000):
         rep = _Report()
         acc = C.c_int()
-        st = coef._struct()
-        rc = self._lib.cap_pcgsip_guess(self._ctx._h, self._h, _ptr(fi),
-                                        _ptr(prev) if prev is not None else None, C.byref(st),
+        st = coef._struct(self._ctx)
+        rc = self._lib.cap_pcgsip_guess(self._ctx._h, self._h, self._ctx._f(fi, 'fi'),
+                                        self._ctx._f(prev, 'prev'), C.byref(st),
                                         alfa, sor, nsw, C.byref(rep), C.byref(acc))
         self._ctx._check(rc, "cap_pcgsip_guess",
                          allow=(CAP_OK, CAP_MAXITER, CAP_GUESS_REJECTED))

@@ -29,7 +29,10 @@ extern "C" {
 #define CAP_GUESS_REJECTED  3   /* learned guess raised the residual, unused */
 #define CAP_ERR_ARG        -1
 #define CAP_ERR_HIP        -2
-#define CAP_ERR_TIMEOUT    -3   /* a wavefront dependency wait expired      */
+#define CAP_ERR_TIMEOUT    -3   /* a wavefront dependency wait expired; the
+                                   context's reduction and claim state is
+                                   reset before the call returns, so the
+                                   context stays usable                     */
 #define CAP_ERR_NOMEM      -4
 #define CAP_ERR_NODEV      -5
 

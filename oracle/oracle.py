@@ -22,7 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
-DROPIN_DRIVER = os.path.join(HERE, "_ref", "dropin_driver")
+DROPIN_DRIVER = os.path.join(os.path.dirname(HERE), "tests", "fortran", "_build", "dropin_driver")
 
 SYS_FIELDS = ("ae", "aw", "an", "as_", "at", "ab", "ap", "su", "res",
               "bb", "bs", "bw", "bp", "bn", "be", "bt")

@@ -27,3 +27,11 @@ def test_l1_residual_matches_oracle(O):
     # at x = 0 it is sum |su|, the solver's initial residual from zero
     z = bench._l1_residual(arrs, torch.zeros(s.n, dtype=torch.float64), dims)
     np.testing.assert_allclose(z, np.abs(s.su[mask]).sum(), rtol=1e-12)
+
+
+def test_traffic_only_at_the_profiled_grid():
+    """roofline.traffic comes from counters read at one grid size: another
+    --n must report null rather than that size's bytes."""
+    assert bench._pmc_traffic("sip_fwd_res", 256) is not None
+    assert bench._pmc_traffic("sip_fwd_res", 384) is None
+    assert bench._pmc_traffic("no_such_kernel", 256) is None

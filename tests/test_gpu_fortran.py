@@ -1,5 +1,6 @@
-"""The Fortran drop-ins: a host program built on the reference's module file
-calls `sipsol/pcgsip/iccg/cgstab_sip(fi, ip)`, resolved at link time to
+"""The Fortran drop-ins: a host program built on this repository's module
+file (tests/fortran/host_modules.f90, which restates the module names and
+kinds a host program of the solvers provides) calls `sipsol/pcgsip/iccg/cgstab_sip(fi, ip)`, resolved at link time to
 cappuccino-ghostcells-mpi_amd/fortran/cap_dropin.f90 (ISO_C_BINDING ->
 libcapamd.so).  Outputs must match the oracle like the C-ABI path does, and
 the solver report line must carry the same iteration count."""
@@ -11,7 +12,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DRIVER = os.path.join(ROOT, "oracle", "_ref", "dropin_driver")
+DRIVER = os.path.join(ROOT, "tests", "fortran", "_build", "dropin_driver")
 B_FIELDS = ("bb", "bs", "bw", "bp", "bn", "be", "bt")
 
 
