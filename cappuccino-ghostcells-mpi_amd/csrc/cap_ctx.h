@@ -68,9 +68,13 @@ struct cap_ctx {
     double *partials;
     int *asym;                    // per SpMV block: 1 = reads aw, as, ab (FP64 grid, then FP32 grid)
     unsigned *ticket;
+    int device;                   // the HIP device current at cap_init
     char err[256];
 };
 
 void cap_free_extra(cap_ctx *ctx);
+// looks up this device's copy of the static scratch array of the
+// allocation-free entry points (once per device; the device must be current)
+int cap_scratch_setup(int device);
 // grad_gauss into gradu(3, nxyz) layout (device pointers; queued, not synchronised)
 int cap_grad_gauss_aos(cap_ctx *ctx, const double *u, const double *const *geom, double *g3);

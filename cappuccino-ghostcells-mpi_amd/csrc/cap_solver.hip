@@ -1599,6 +1599,9 @@ extern "C" int cap_init(cap_ctx **out, int ni, int nj, int nk, int flags)
     ctx->ni = ni; ctx->nj = nj; ctx->nk = nk;
     ctx->nstd = (size_t)ni * nj * nk;
     ctx->flags = flags;
+    HIPCHK(hipGetDevice(&ctx->device));
+    if (cap_scratch_setup(ctx->device) != CAP_OK)
+        return fail(ctx, CAP_ERR_HIP, "device address of the static scratch array not found");
     HIPCHK(hipStreamCreate(&ctx->stream));
     HIPCHK(hipEventCreate(&ctx->ev0));
     HIPCHK(hipEventCreate(&ctx->ev1));

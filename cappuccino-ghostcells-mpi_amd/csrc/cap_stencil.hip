@@ -11,6 +11,7 @@
 #include <math.h>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include "cap_kernels.h"
 #include "cap_amd.h"
 #include "cap_ctx.h"
@@ -1220,6 +1221,80 @@ extern "C" int cap_courant(cap_ctx *ctx, const double *f1, const double *f2, con
     double out[4];
     HIPCHK(hipMemcpyAsync(out, out_d, sizeof out, hipMemcpyDeviceToHost, ctx->stream));
     if ((rc = unstage(ctx, su, dsu))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    conum[0] = out[0];
+    conum[1] = out[1];
+    *timestep = out[2];
+    *time = out[3];
+    return CAP_OK;
+}
+
+// ------------------------------------------- allocation-free Courant number
+// Scratch of cap_courant_noalloc: k_courant's per-block partials (5 doubles
+// for each of at most 8192 blocks, the cap of blocks_for) followed by
+// k_courant_fin's 4 results.  Each GPU holds its own copy of this array for
+// as long as the process runs.  In host code the name `scratch` is only a
+// host-side stand-in: kernels get this device's address of the array, looked
+// up once per device by cap_scratch_setup (at cap_init) and kept in
+// scratch_ptr.
+constexpr size_t CAP_SCRATCH_DOUBLES = (size_t)8192 * 5 + 8;
+static __device__ __align__(256) double scratch[CAP_SCRATCH_DOUBLES];
+
+constexpr int CAP_MAX_DEVICES = 64;
+static double *scratch_ptr[CAP_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+int cap_scratch_setup(int device)
+{
+    if (device < 0 || device >= CAP_MAX_DEVICES) return CAP_ERR_NODEV;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_ptr[device]) return CAP_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return CAP_ERR_HIP;
+    }
+    scratch_ptr[device] = (double *)p;
+    return CAP_OK;
+}
+
+extern "C" int cap_courant_noalloc(cap_ctx *ctx, const double *f1, const double *f2,
+                                   const double *f3, const double *vol, double *su,
+                                   double *timestep, double *time, int conumfix,
+                                   double conumfixvalue, double *conum)
+{
+    if (!ctx || !f1 || !f2 || !f3 || !vol || !su || !timestep || !time || !conum)
+        return CAP_ERR_ARG;
+    if (!(ctx->flags & CAP_DEVICE_POINTERS)) {
+        snprintf(ctx->err, sizeof ctx->err,
+                 "cap_courant_noalloc needs a CAP_DEVICE_POINTERS context (host arrays would "
+                 "be staged through allocated device buffers)");
+        return CAP_ERR_ARG;
+    }
+    double *scr = (ctx->device >= 0 && ctx->device < CAP_MAX_DEVICES) ? scratch_ptr[ctx->device]
+                                                                      : nullptr;
+    if (!scr) {
+        snprintf(ctx->err, sizeof ctx->err, "no scratch address for device %d", ctx->device);
+        return CAP_ERR_HIP;
+    }
+    const int nb = blocks_for(ctx->nstd);
+    const size_t need = (size_t)nb * 5 + 4;
+    if (need > CAP_SCRATCH_DOUBLES) {
+        snprintf(ctx->err, sizeof ctx->err,
+                 "cap_courant_noalloc: %d blocks need %zu scratch doubles, more than its %zu",
+                 nb, need, CAP_SCRATCH_DOUBLES);
+        return CAP_ERR_ARG;
+    }
+    // every partial k_courant_fin reads is written by k_courant before it on
+    // the same stream, so nothing needs clearing between calls
+    double *out_d = scr + (size_t)nb * 5;
+    hipLaunchKernelGGL(k_courant, dim3(nb), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj, ctx->nk,
+                       f1, f2, f3, vol, su, scr);
+    hipLaunchKernelGGL(k_courant_fin, dim3(1), dim3(64), 0, ctx->stream, (const double *)scr, nb,
+                       *timestep, *time, conumfix, conumfixvalue, out_d);
+    HIPCHK(hipGetLastError());
+    double out[4];
+    HIPCHK(hipMemcpyAsync(out, out_d, sizeof out, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     conum[0] = out[0];
     conum[1] = out[1];

@@ -219,6 +219,15 @@ module cap_hip
       real(c_double), value :: conumfixvalue
       real(c_double), intent(out) :: conum(2)
     end function
+    integer(c_int) function cap_courant_noalloc(ctx, f1, f2, f3, vol, su, timestep, time, &
+        conumfix, conumfixvalue, conum) bind(c, name='cap_courant_noalloc')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx, f1, f2, f3, vol, su
+      real(c_double), intent(inout) :: timestep, time
+      integer(c_int), value :: conumfix
+      real(c_double), value :: conumfixvalue
+      real(c_double), intent(out) :: conum(2)
+    end function
     integer(c_int) function cap_update_ghostcells(ctx, var, geom, bc) &
         bind(c, name='cap_update_ghostcells')
       import :: c_ptr, c_int, cap_bc

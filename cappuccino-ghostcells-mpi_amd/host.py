@@ -40,7 +40,7 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_unet_get_params", "cap_unet_set_params", "cap_unet_forward",
            "cap_unet_gradients", "cap_unet_train_step", "cap_unet_guess", "cap_pcgsip_guess",
            "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya",
-           "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les")
+           "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les", "cap_courant_noalloc")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -148,6 +148,7 @@ def load():
     L.cap_update_ghostcells.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(_Bc)]
     L.cap_courant.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(d), C.POINTER(d), i, d,
                               C.POINTER(d)]
+    L.cap_courant_noalloc.argtypes = L.cap_courant.argtypes
     f = C.c_float
     L.cap_unet_create.argtypes = [vp, i, i, i, C.c_ulonglong, C.POINTER(vp)]
     L.cap_unet_destroy.argtypes = [vp]
@@ -380,14 +381,17 @@ class Context:
         rc = self._lib.cap_update_ghostcells(self._h, V, G, C.byref(bc))
         self._check(rc, "cap_update_ghostcells", allow=(CAP_OK,))
 
-    def courant(self, f1, f2, f3, vol, su, timestep, time, conumfix=False, conumfixvalue=0.5):
-        """Returns (CoNum, meanCoNum, timestep, time) after CourantNo.h."""
+    def courant(self, f1, f2, f3, vol, su, timestep, time, conumfix=False, conumfixvalue=0.5,
+                noalloc=False):
+        """Returns (CoNum, meanCoNum, timestep, time) after CourantNo.h.
+        noalloc: through cap_courant_noalloc (static scratch, no device
+        allocation; device-pointer contexts only)."""
         ts, tm = C.c_double(timestep), C.c_double(time)
         out = (C.c_double * 2)()
-        rc = self._lib.cap_courant(self._h, self._f(f1), self._f(f2), self._f(f3), self._f(vol), self._f(su),
-                                   C.byref(ts), C.byref(tm), 1 if conumfix else 0,
-                                   float(conumfixvalue), out)
-        self._check(rc, "cap_courant", allow=(CAP_OK,))
+        fn = self._lib.cap_courant_noalloc if noalloc else self._lib.cap_courant
+        rc = fn(self._h, self._f(f1), self._f(f2), self._f(f3), self._f(vol), self._f(su),
+                C.byref(ts), C.byref(tm), 1 if conumfix else 0, float(conumfixvalue), out)
+        self._check(rc, fn.__name__, allow=(CAP_OK,))
         return out[0], out[1], ts.value, tm.value
 
     def fluxmc(self, f, d, fmcor):

@@ -305,6 +305,20 @@ int cap_courant(cap_ctx *ctx, const double *f1, const double *f2, const double *
                 const double *vol, double *su, double *timestep, double *time, int conumfix,
                 double conumfixvalue, double *conum);
 
+/* No counterpart in the reference.  cap_courant without any device memory
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight time-step loops, harnesses that forbid hipMalloc): the same
+ * kernels and results as cap_courant, with the per-block partials in a
+ * static device array of the library (8192 x 5 + 8 doubles, 320 KiB per GPU,
+ * held until the process ends) instead of a buffer the context allocates.
+ * Needs a CAP_DEVICE_POINTERS context (CAP_ERR_ARG otherwise: host arrays
+ * would be staged through allocated buffers).  All contexts on one GPU share
+ * that GPU's array: calls that overlap on one GPU (from several host threads
+ * or streams) race and must be serialised by the caller. */
+int cap_courant_noalloc(cap_ctx *ctx, const double *f1, const double *f2, const double *f3,
+                        const double *vol, double *su, double *timestep, double *time,
+                        int conumfix, double conumfixvalue, double *conum);
+
 /* Boundary types of the ghost-cell update, per boundary cell: lbw / lbe
  * indexed (j-1)*nk + k-1, lbs / lbn (i-1)*nk + k-1, lbb / lbt (i-1)*nj + j-1
  * (module bc of modules_allocatable.f90:217-219); 1 inlet, 2 outlet,
