@@ -6,5 +6,5 @@ The directory name is not an importable identifier; load it with
 ``load_package()`` from ``__graft_entry__`` (registered as ``cappuccino_amd``).
 """
 from .host import (Context, Coef, UNet, DrvOpts, UvwOpts, LesOpts, CapError, load, COEF_FIELDS, GEOM_FIELDS, EXPORTS,  # noqa: F401
-                   FLOW_NAMES,
+                   FLOW_NAMES, read_grid,
                    CAP_OK, CAP_MAXITER, CAP_NAN, CAP_GUESS_REJECTED, CAP_ERR_TIMEOUT)

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcapamd.so")
-SOURCES = ("cap_solver.hip", "cap_stencil.hip", "cap_unet.hip", "cap_driver.hip")
+SOURCES = ("cap_solver.hip", "cap_stencil.hip", "cap_unet.hip", "cap_driver.hip",
+           "cap_geometry.hip")
 
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

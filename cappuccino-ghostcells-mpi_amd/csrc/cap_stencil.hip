@@ -1329,83 +1329,244 @@ extern "C" int cap_continuity(cap_ctx *ctx, const double *f1, const double *f2, 
 // ------------------------------------------------------ problem generator
 // Counter-based generator (splitmix64 finaliser keyed by seed, stream and
 // cell index): the same inputs on every machine, independent of how the
-// work is split.
-static inline uint64_t smix(uint64_t x)
+// work is split.  Its log, cos(2 pi u) and exp are its own: range reduction
+// and fixed-degree polynomials in +, -, *, / (contraction is off for the
+// whole library), so any IEEE binary64 machine gives the same bits.
+__device__ __forceinline__ uint64_t gen_smix(uint64_t x)
 {
     x += 0x9E3779B97F4A7C15ull;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
     x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
     return x ^ (x >> 31);
 }
-static inline uint64_t hkey(uint64_t seed, uint64_t stream, uint64_t idx)
+__device__ __forceinline__ uint64_t gen_key(uint64_t seed, uint64_t stream, uint64_t idx)
 {
-    return smix(smix(seed ^ (stream * 0xD1B54A32D192ED03ull)) + idx);
+    return gen_smix(gen_smix(seed ^ (stream * 0xD1B54A32D192ED03ull)) + idx);
 }
-static inline double gauss(uint64_t seed, uint64_t stream, uint64_t idx)
+
+constexpr double GEN_LN2_HI = 6.93147180369123816490e-01;   // low 32 bits zero: e * hi exact
+constexpr double GEN_LN2_LO = 1.9082149292705877e-10;
+
+// ln x, x > 0 normal: x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s)
+// with s = (m - 1) / (m + 1), |s| <= 0.1716, odd series through s^21
+__device__ double gen_log(double x)
 {
-    double u1 = (double)((hkey(seed, stream, 2 * idx) >> 11) + 1) * 0x1.0p-53;
-    double u2 = (double)(hkey(seed, stream, 2 * idx + 1) >> 11) * 0x1.0p-53;
-    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1022;
+    double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FE0000000000000ull));
+    if (m < 0.70710678118654757) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    // 1/21, 1/19, ..., 1/3 (Horner in s^2)
+    const double cf[10] = {0.047619047619047616, 0.05263157894736842, 0.058823529411764705,
+                           0.06666666666666667, 0.07692307692307693, 0.09090909090909091,
+                           0.1111111111111111, 0.14285714285714285, 0.2, 0.3333333333333333};
+    double t = cf[0];
+    for (int q = 1; q < 10; ++q) t = cf[q] + z * t;
+    const double lm = 2.0 * s + 2.0 * s * (z * t);
+    const double de = (double)e;
+    return de * GEN_LN2_HI + (de * GEN_LN2_LO + lm);
+}
+
+// cos(2 pi u), u in [0, 1): v = |u - rint(u)|, quadrant q = rint(4 v),
+// r = v - q / 4 in [-1/8, 1/8] (all exact), theta = 2 pi r; Taylor series
+// of cos through theta^18 and of sin through theta^19
+__device__ double gen_cos2pi(double u)
+{
+    double v = u - rint(u);
+    if (v < 0.0) v = -v;
+    const double q = rint(4.0 * v);
+    const double r = v - 0.25 * q;
+    const double th = 6.283185307179586 * r;
+    const double z = th * th;
+    // (-1)^n / (2n)!, n = 9..1, and (-1)^n / (2n+1)!, n = 9..1
+    const double cc[9] = {-1.5619206968586225e-16, 4.779477332387385e-14, -1.1470745597729725e-11,
+                          2.08767569878681e-09, -2.755731922398589e-07, 2.48015873015873e-05,
+                          -0.001388888888888889, 0.041666666666666664, -0.5};
+    const double cs[9] = {-8.22063524662433e-18, 2.8114572543455206e-15, -7.647163731819816e-13,
+                          1.6059043836821613e-10, -2.505210838544172e-08, 2.7557319223985893e-06,
+                          -0.0001984126984126984, 0.008333333333333333, -0.16666666666666666};
+    double c = cc[0], sn = cs[0];
+    for (int k = 1; k < 9; ++k) {
+        c = cc[k] + z * c;
+        sn = cs[k] + z * sn;
+    }
+    c = 1.0 + z * c;
+    sn = th + th * (z * sn);
+    if (q == 0.0) return c;
+    if (q == 1.0) return -sn;
+    return -c;
+}
+
+// e^x, |x| < 700: k = rint(x / ln 2), r = x - k ln 2 (two-part ln 2),
+// |r| <= 0.347, Taylor series through r^13, times 2^k built from its bits
+__device__ double gen_exp(double x)
+{
+    const double k = rint(x * 1.4426950408889634);
+    const double r = (x - k * GEN_LN2_HI) - k * GEN_LN2_LO;
+    // 1/13!, 1/12!, ..., 1/2!, 1/1!, 1/0!
+    const double cf[14] = {1.6059043836821613e-10, 2.08767569878681e-09, 2.505210838544172e-08,
+                           2.755731922398589e-07, 2.7557319223985893e-06, 2.48015873015873e-05,
+                           0.0001984126984126984, 0.001388888888888889, 0.008333333333333333,
+                           0.041666666666666664, 0.16666666666666666, 0.5, 1.0, 1.0};
+    double p = cf[0];
+    for (int q = 1; q < 14; ++q) p = cf[q] + r * p;
+    return p * __longlong_as_double(((long long)k + 1023) << 52);
+}
+
+// Box-Muller on two counter-keyed uniforms, u1 in (0, 1], u2 in [0, 1)
+__device__ double gen_normal(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    const double u1 = (double)((gen_key(seed, stream, 2 * idx) >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)(gen_key(seed, stream, 2 * idx + 1) >> 11) * 0x1.0p-53;
+    return sqrt(-2.0 * gen_log(u1)) * gen_cos2pi(u2);
+}
+
+// One thread per cell: ghost rows a* = 0, ap = 1, su = 0; an interior cell's
+// faces to interior neighbours a = h exp(sigma g), g keyed by (seed,
+// direction, index of the face's west / south / bottom cell), so that
+// aw(E) = ae(P) etc.; ap = ae + aw + an + as + at + ab; su = g (stream 4),
+// its mean removed by k_gen_center.
+__global__ __launch_bounds__(256) void k_gen_lognormal(int ni, int nj, int nk, uint64_t seed,
+                                                       double sigma, double h,
+                                                       double *__restrict__ ae, double *__restrict__ aw,
+                                                       double *__restrict__ an, double *__restrict__ as,
+                                                       double *__restrict__ at, double *__restrict__ ab,
+                                                       double *__restrict__ ap, double *__restrict__ su)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const size_t n = (size_t)nij * nk;
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
+        const int k = (int)(c / nij) + 1, rr = (int)(c % nij);
+        const int i = rr / nj + 1, j = rr % nj + 1;
+        double e = 0.0, w = 0.0, no = 0.0, so = 0.0, t = 0.0, bo = 0.0, p = 1.0, s = 0.0;
+        if (k >= 3 && k <= nkmm && i >= 3 && i <= nimm && j >= 3 && j <= njmm) {
+            if (i < nimm) e = h * gen_exp(sigma * gen_normal(seed, 1, c));
+            if (i > 3) w = h * gen_exp(sigma * gen_normal(seed, 1, c - nj));
+            if (j < njmm) no = h * gen_exp(sigma * gen_normal(seed, 2, c));
+            if (j > 3) so = h * gen_exp(sigma * gen_normal(seed, 2, c - 1));
+            if (k < nkmm) t = h * gen_exp(sigma * gen_normal(seed, 3, c));
+            if (k > 3) bo = h * gen_exp(sigma * gen_normal(seed, 3, c - nij));
+            p = e + w + no + so + t + bo;
+            s = gen_normal(seed, 4, c);
+        }
+        ae[c] = e; aw[c] = w; an[c] = no; as[c] = so; at[c] = t; ab[c] = bo; ap[c] = p; su[c] = s;
+    }
+}
+
+// Sum of su over each interior (k, i) row, j in order (one thread a row).
+__global__ __launch_bounds__(256) void k_gen_rows(int ni, int nj, int nk, const double *__restrict__ su,
+                                                  double *__restrict__ rows)
+{
+    const int nis = ni - 4, njs = nj - 4, nks = nk - 4, nij = ni * nj;
+    const int nrow = nis * nks;
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < nrow; r += gridDim.x * 256) {
+        const int k = r / nis + 3, i = r % nis + 3;
+        const size_t c0 = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + 2;   // j = 3
+        double row = 0.0;
+        for (int j = 0; j < njs; ++j) row += su[c0 + j];
+        rows[r] = row;
+    }
+}
+
+// The rows of each plane in i order (one thread a plane), then the planes in
+// k order: mean = sum / count, stored in rows[nrow + nks].
+__global__ __launch_bounds__(256) void k_gen_mean(int ni, int nj, int nk, double *rows)
+{
+    const int nis = ni - 4, njs = nj - 4, nks = nk - 4;
+    const int nrow = nis * nks;
+    double *planes = rows + nrow;
+    for (int k = threadIdx.x; k < nks; k += blockDim.x) {
+        double plane = 0.0;
+        for (int i = 0; i < nis; ++i) plane += rows[k * nis + i];
+        planes[k] = plane;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int k = 0; k < nks; ++k) sum += planes[k];
+        planes[nks] = sum / (double)((long)nis * njs * nks);
+    }
+}
+
+// su -= mean on the interior; then the reference-pressure cell (3,3,3)
+// pinned as PISO_multiple_correction.f90:117-129 does (a_nb = 0, ap = 1,
+// su = 0; its neighbours keep their faces to it).
+__global__ __launch_bounds__(256) void k_gen_center(int ni, int nj, int nk, const double *mean_p,
+                                                    double *__restrict__ ae, double *__restrict__ aw,
+                                                    double *__restrict__ an, double *__restrict__ as,
+                                                    double *__restrict__ at, double *__restrict__ ab,
+                                                    double *__restrict__ ap, double *__restrict__ su)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const size_t n = (size_t)nij * nk;
+    const size_t ref = (size_t)2 * nij + (size_t)2 * nj + 2;
+    const double mean = *mean_p;
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
+        const int k = (int)(c / nij) + 1, rr = (int)(c % nij);
+        const int i = rr / nj + 1, j = rr % nj + 1;
+        if (!(k >= 3 && k <= nkmm && i >= 3 && i <= nimm && j >= 3 && j <= njmm)) continue;
+        if (c == ref) {
+            ae[c] = aw[c] = an[c] = as[c] = at[c] = ab[c] = 0.0;
+            ap[c] = 1.0;
+            su[c] = 0.0;
+        } else {
+            su[c] = su[c] - mean;
+        }
+    }
 }
 
 // Log-normal Neumann p' operator on a uniform grid: interior faces
 // a = h exp(sigma g), g ~ N(0,1) keyed by (seed, direction, cell); aw(E)=ae(P)
 // etc.; boundary faces 0; ap = sum a_nb; su ~ N(0,1) with zero mean; ghost
 // rows a* = 0, ap = 1, su = 0; reference-pressure cell (3,3,3) pinned.
+// Everything runs on the device; the mean is summed in a fixed order (rows,
+// planes, total), so the result is the same bits on every run.
 extern "C" int cap_build_lognormal(cap_ctx *ctx, unsigned long long seed, double sigma,
                                    cap_coef *c)
 {
     if (!ctx || !c || !c->ae || !c->aw || !c->an || !c->as || !c->at || !c->ab || !c->ap || !c->su)
         return CAP_ERR_ARG;
-    const int ni = ctx->ni, nj = ctx->nj, nk = ctx->nk, nij = ni * nj;
-    const size_t n = ctx->nstd;
-    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const int ni = ctx->ni, nj = ctx->nj, nk = ctx->nk;
+    if (ni < 7 || nj < 7 || nk < 7) return CAP_ERR_ARG;   // at least 3 interior cells each way
     const double h = 1.0 / (double)(ni - 4);
-    std::vector<double> A[8];
-    const bool dev = ctx->flags & CAP_DEVICE_POINTERS;
     double *out[8] = {c->ae, c->aw, c->an, c->as, c->at, c->ab, c->ap, c->su};
-    double *P[8];
+    double *d[8];
+    int rc;
     for (int q = 0; q < 8; ++q) {
-        if (dev) { A[q].assign(n, 0.0); P[q] = A[q].data(); }
-        else { P[q] = out[q]; memset(P[q], 0, n * sizeof(double)); }
+        const double *p;
+        if ((rc = stage(ctx, q, nullptr, &p))) return rc;   // device pointers: the caller's array
+        d[q] = (ctx->flags & CAP_DEVICE_POINTERS) ? out[q] : (double *)p;
     }
-    double *ae = P[0], *aw = P[1], *an = P[2], *as = P[3], *at = P[4], *ab = P[5], *ap = P[6], *su = P[7];
-    for (size_t q = 0; q < n; ++q) ap[q] = 1.0;
-    auto IX = [&](int i, int j, int k) {
-        return (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
-    };
-    for (int k = 3; k <= nkmm; ++k)
-        for (int i = 3; i <= nimm; ++i)
-            for (int j = 3; j <= njmm; ++j) {
-                size_t q = IX(i, j, k);
-                if (i < nimm) { double a = h * exp(sigma * gauss(seed, 1, q)); ae[q] = a; aw[q + nj] = a; }
-                if (j < njmm) { double a = h * exp(sigma * gauss(seed, 2, q)); an[q] = a; as[q + 1] = a; }
-                if (k < nkmm) { double a = h * exp(sigma * gauss(seed, 3, q)); at[q] = a; ab[q + nij] = a; }
-            }
-    double sum = 0.0;
-    long cnt = 0;
-    for (int k = 3; k <= nkmm; ++k)
-        for (int i = 3; i <= nimm; ++i)
-            for (int j = 3; j <= njmm; ++j) {
-                size_t q = IX(i, j, k);
-                ap[q] = ae[q] + aw[q] + an[q] + as[q] + at[q] + ab[q];
-                su[q] = gauss(seed, 4, q);
-                sum += su[q];
-                ++cnt;
-            }
-    const double mean = sum / (double)cnt;
-    for (int k = 3; k <= nkmm; ++k)
-        for (int i = 3; i <= nimm; ++i)
-            for (int j = 3; j <= njmm; ++j) su[IX(i, j, k)] -= mean;
-    // reference-pressure cell (3,3,3), pinned as PISO_multiple_correction.f90:117-129
-    const size_t r = IX(3, 3, 3);
-    ae[r] = aw[r] = an[r] = as[r] = at[r] = ab[r] = 0.0;
-    ap[r] = 1.0;
-    su[r] = 0.0;
-    if (dev) {
-        for (int q = 0; q < 8; ++q)
-            HIPCHK(hipMemcpy(out[q], P[q], n * sizeof(double), hipMemcpyHostToDevice));
+    const size_t nrow = (size_t)(ni - 4) * (nk - 4);
+    double *rows;
+    {   // row sums, plane sums and the mean, in staging slot 47
+        const size_t bytes = (nrow + (nk - 4) + 1) * sizeof(double);
+        if (ctx->Gsz[47] < bytes) {
+            if (ctx->G[47]) (void)hipFree(ctx->G[47]);
+            ctx->G[47] = nullptr;
+            ctx->Gsz[47] = 0;
+            HIPCHK(hipMalloc(&ctx->G[47], bytes));
+            ctx->Gsz[47] = bytes;
+        }
+        rows = ctx->G[47];
     }
+    hipLaunchKernelGGL(k_gen_lognormal, dim3(blocks_for(ctx->nstd)), dim3(256), 0, ctx->stream, ni,
+                       nj, nk, (uint64_t)seed, sigma, h, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7]);
+    hipLaunchKernelGGL(k_gen_rows, dim3(blocks_for(nrow)), dim3(256), 0, ctx->stream, ni, nj, nk,
+                       (const double *)d[7], rows);
+    hipLaunchKernelGGL(k_gen_mean, dim3(1), dim3(256), 0, ctx->stream, ni, nj, nk, rows);
+    hipLaunchKernelGGL(k_gen_center, dim3(blocks_for(ctx->nstd)), dim3(256), 0, ctx->stream, ni, nj,
+                       nk, (const double *)(rows + nrow + (nk - 4)), d[0], d[1], d[2], d[3], d[4], d[5],
+                       d[6], d[7]);
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < 8; ++q)
+        if ((rc = unstage(ctx, out[q], d[q]))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return CAP_OK;
 }
 

@@ -228,6 +228,30 @@ module cap_hip
       real(c_double), value :: conumfixvalue
       real(c_double), intent(out) :: conum(2)
     end function
+    integer(c_int) function cap_face_areas(ctx, x, y, z, ar) bind(c, name='cap_face_areas')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, x, y, z
+      type(c_ptr), intent(in) :: ar(9)
+    end function
+    integer(c_int) function cap_intfac(ctx, nodes_centres, fx, fy, fz) bind(c, name='cap_intfac')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(in) :: nodes_centres(6)
+      type(c_ptr), value :: fx, fy, fz
+    end function
+    integer(c_int) function cap_grid_dims(path, ni, nj, nk) bind(c, name='cap_grid_dims')
+      import :: c_char, c_int
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: ni, nj, nk
+    end function
+    integer(c_int) function cap_read_grid(path, ni, nj, nk, arrays, lb, flags) &
+        bind(c, name='cap_read_grid')
+      import :: c_char, c_int, c_ptr
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: ni, nj, nk
+      type(c_ptr), intent(in) :: arrays(19), lb(6)
+      integer(c_int), intent(out) :: flags(9)
+    end function
     integer(c_int) function cap_update_ghostcells(ctx, var, geom, bc) &
         bind(c, name='cap_update_ghostcells')
       import :: c_ptr, c_int, cap_bc

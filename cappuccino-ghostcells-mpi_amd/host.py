@@ -40,7 +40,8 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_unet_get_params", "cap_unet_set_params", "cap_unet_forward",
            "cap_unet_gradients", "cap_unet_train_step", "cap_unet_guess", "cap_pcgsip_guess",
            "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya",
-           "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les", "cap_courant_noalloc")
+           "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les", "cap_courant_noalloc",
+           "cap_face_areas", "cap_intfac", "cap_grid_dims", "cap_read_grid")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -149,6 +150,10 @@ def load():
     L.cap_courant.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(d), C.POINTER(d), i, d,
                               C.POINTER(d)]
     L.cap_courant_noalloc.argtypes = L.cap_courant.argtypes
+    L.cap_face_areas.argtypes = [vp, vp, vp, vp, C.POINTER(vp)]
+    L.cap_intfac.argtypes = [vp, C.POINTER(vp), vp, vp, vp]
+    L.cap_grid_dims.argtypes = [C.c_char_p, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
+    L.cap_read_grid.argtypes = [C.c_char_p, i, i, i, C.POINTER(vp), C.POINTER(vp), C.POINTER(i)]
     f = C.c_float
     L.cap_unet_create.argtypes = [vp, i, i, i, C.c_ulonglong, C.POINTER(vp)]
     L.cap_unet_destroy.argtypes = [vp]
@@ -472,10 +477,57 @@ class Context:
                                         C.byref(les))
         self._check(rc, "cap_calc_vis_les", allow=(CAP_OK,))
 
+    # --- grid geometry -----------------------------------------------------
+    AR_FIELDS = ("ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z")
+
+    def face_areas(self, g):
+        """init.f90:49-144: g's ar1x .. ar3z from its nodes x, y, z (in place)."""
+        A = (C.c_void_p * 9)(*[self._f(g[q], q) for q in self.AR_FIELDS])
+        rc = self._lib.cap_face_areas(self._h, self._f(g["x"], "x"), self._f(g["y"], "y"),
+                                      self._f(g["z"], "z"), A)
+        self._check(rc, "cap_face_areas", allow=(CAP_OK,))
+
+    def intfac(self, g):
+        """init.f90:147-149: g's fx, fy, fz from its nodes and centres (in place)."""
+        G = (C.c_void_p * 6)(*[self._f(g[q], q) for q in ("x", "y", "z", "xc", "yc", "zc")])
+        rc = self._lib.cap_intfac(self._h, G, self._f(g["fx"], "fx"), self._f(g["fy"], "fy"),
+                                  self._f(g["fz"], "fz"))
+        self._check(rc, "cap_intfac", allow=(CAP_OK,))
+
     def build_lognormal(self, coef, seed, sigma=1.0):
         st = coef._struct(self)
         rc = self._lib.cap_build_lognormal(self._h, seed, sigma, C.byref(st))
         self._check(rc, "cap_build_lognormal", allow=(CAP_OK,))
+
+
+GRID_ARRAYS = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz",
+               "ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z")
+GRID_LB = ("lbw", "lbe", "lbs", "lbn", "lbb", "lbt")
+GRID_FLAGS = ("low", "loe", "los", "lon", "lob", "lot", "lout", "lthermbc", "lconbc")
+
+
+def read_grid(path):
+    """The reference's grid file (read_grid.f90:126-161) through
+    cap_grid_dims / cap_read_grid: returns (dims, arrays, lb, flags) with
+    numpy arrays named as the reference's module variables."""
+    import numpy as np
+    L = load()
+    ni, nj, nk = C.c_int(), C.c_int(), C.c_int()
+    rc = L.cap_grid_dims(os.fsencode(path), C.byref(ni), C.byref(nj), C.byref(nk))
+    if rc != CAP_OK:
+        raise CapError(f"cap_grid_dims: {path} is not a readable grid file (status {rc})")
+    dims = (ni.value, nj.value, nk.value)
+    n = dims[0] * dims[1] * dims[2]
+    arrays = {q: np.zeros(n) for q in GRID_ARRAYS}
+    lens = (dims[1] * dims[2],) * 2 + (dims[0] * dims[2],) * 2 + (dims[0] * dims[1],) * 2
+    lb = {q: np.zeros(m, np.int32) for q, m in zip(GRID_LB, lens)}
+    flags = (C.c_int * 9)()
+    A = (C.c_void_p * 19)(*[arrays[q].ctypes.data for q in GRID_ARRAYS])
+    B = (C.c_void_p * 6)(*[lb[q].ctypes.data for q in GRID_LB])
+    rc = L.cap_read_grid(os.fsencode(path), *dims, A, B, flags)
+    if rc != CAP_OK:
+        raise CapError(f"cap_read_grid: {path} is malformed (status {rc})")
+    return dims, arrays, lb, dict(zip(GRID_FLAGS, (bool(v) for v in flags)))
 
 
 def _fptr(a, n=None, device=None, what="array"):

@@ -377,10 +377,46 @@ int cap_continuity(cap_ctx *ctx, const double *f1, const double *f2, const doubl
 int cap_bpres(cap_ctx *ctx, double *p, const double *fx, const double *fy,
               const double *fz);
 
+/* ---- Grid geometry ------------------------------------------------------
+ * Replaces the face-area loops of init.f90:49-144: from the nodes x, y, z
+ * (node ijk closes cell ijk) the face area vectors ar[9] = {ar1x, ar1y, ar1z,
+ * ar2x, .., ar3z} on the reference's loop ranges (east k = 2..nkm,
+ * i = 1..nim, j = 2..njm; north k = 2..nkm, i = 2..nim, j = 1..njm; top
+ * k = 1..nkm, i = 2..nim, j = 2..njm); other cells keep their values.  As
+ * the reference's loops are written, on a right-handed grid the east vector
+ * points to +i and the north and top vectors to -j and -k. */
+int cap_face_areas(cap_ctx *ctx, const double *x, const double *y, const double *z,
+                   double *const *ar);
+
+/* Replaces `call intfac(nj,1,nij,fx); call intfac(1,nj,nij,fy);
+ * call intfac(nij,1,nj,fz)` (init.f90:147-149; intfac.f90:3-51 with
+ * find_intersection_point.f90:3-76): the interpolation factor of each face
+ * on k = 2..nkm, i = 2..nim, j = 2..njm from nodes and centres
+ * nodes_centres[6] = {x, y, z, xc, yc, zc}; other cells keep their values. */
+int cap_intfac(cap_ctx *ctx, const double *const *nodes_centres, double *fx, double *fy,
+               double *fz);
+
+/* The grid file of read_grid.f90:126-161 (Fortran unformatted sequential
+ * records: ni, nj, nk; nij, nik, njk, nijk, ijs, iks, jks, ijks, lbw, lbe, lbs,
+ * lbn, lbb, lbt, low, loe, los, lon, lob, lot, lout, lthermbc, lconbc; then
+ * x, y, z, xc, yc, zc, vol, fx, fy, fz, ar1x .. ar3z).  Host code: plain
+ * host pointers, no context.  cap_grid_dims reads the extents; cap_read_grid
+ * fills arrays[19] (in the file's order, ni*nj*nk doubles each; a NULL entry
+ * is skipped), lb[6] = {lbw, lbe, lbs, lbn, lbb, lbt} (int32, may be NULL)
+ * and flags[9] = {low, loe, los, lon, lob, lot, lout, lthermbc, lconbc}
+ * (0 / 1, may be NULL).  CAP_ERR_ARG for a missing or malformed file or
+ * extents that differ from the file's. */
+int cap_grid_dims(const char *path, int *ni, int *nj, int *nk);
+int cap_read_grid(const char *path, int ni, int nj, int nk, double *const *arrays, int *const *lb,
+                  int *flags);
+
 /* Problem generator of the measured workload (not a reference routine):
  * the log-normal Neumann p' operator on a uniform grid with the reference-
  * pressure cell (3,3,3) pinned as PISO_multiple_correction.f90:117-129 does,
- * written into the given arrays (host or device per the context). */
+ * written into the given arrays (host or device per the context).  Computed
+ * on the device from a counter-based generator with its own log, cos and
+ * exp (IEEE basic operations only), so the arrays are the same bits on every
+ * machine and equal to the oracle's orc_build_lognormal. */
 int cap_build_lognormal(cap_ctx *ctx, unsigned long long seed, double sigma,
                         cap_coef *c);
 

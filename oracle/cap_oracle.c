@@ -95,12 +95,124 @@ double orc_u01(uint64_t seed, uint64_t stream, uint64_t idx)
     return (double)(orc_hash(seed, stream, idx) >> 11) * 0x1.0p-53;
 }
 
+/*
+ * The generator's own log, cos(2 pi u) and exp: range reduction plus
+ * fixed-degree polynomials in +, -, *, / only (no libm), so that any machine
+ * with IEEE binary64 arithmetic and no contraction -- this oracle and the
+ * device kernel of cap_build_lognormal -- produces the same bits.  Accuracy
+ * is a few ulp, which is all a synthetic input needs.
+ */
+static inline double gen_bits(uint64_t b)
+{
+    double x;
+    memcpy(&x, &b, sizeof x);
+    return x;
+}
+
+static inline uint64_t gen_u64(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, sizeof b);
+    return b;
+}
+
+#define GEN_LN2_HI 6.93147180369123816490e-01   /* 0x3FE62E42FEE00000: e * hi exact */
+#define GEN_LN2_LO 1.9082149292705877e-10
+
+/* ln x for a normal x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+ * ln m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.1716, series to s^21 */
+double orc_gen_log(double x)
+{
+    const uint64_t b = gen_u64(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1022;
+    double m = gen_bits((b & 0x000FFFFFFFFFFFFFull) | 0x3FE0000000000000ull);   /* [0.5, 1) */
+    if (m < 0.70710678118654757) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double t = 0.047619047619047616;                 /* 1/21 */
+    t = 0.05263157894736842 + z * t;                 /* 1/19 */
+    t = 0.058823529411764705 + z * t;                /* 1/17 */
+    t = 0.06666666666666667 + z * t;                 /* 1/15 */
+    t = 0.07692307692307693 + z * t;                 /* 1/13 */
+    t = 0.09090909090909091 + z * t;                 /* 1/11 */
+    t = 0.1111111111111111 + z * t;                  /* 1/9 */
+    t = 0.14285714285714285 + z * t;                 /* 1/7 */
+    t = 0.2 + z * t;                                 /* 1/5 */
+    t = 0.3333333333333333 + z * t;                  /* 1/3 */
+    const double lm = 2.0 * s + 2.0 * s * (z * t);
+    const double de = (double)e;
+    return de * GEN_LN2_HI + (de * GEN_LN2_LO + lm);
+}
+
+/* cos(2 pi u) for u in [0, 1): u - rint(u) exactly, folded to |v| <= 1/2,
+ * quadrant q = rint(4 v), r = v - q / 4 in [-1/8, 1/8] exactly, theta =
+ * 2 pi r, Taylor series of cos to theta^18 and of sin to theta^19 */
+double orc_gen_cos2pi(double u)
+{
+    double v = u - rint(u);
+    if (v < 0.0) v = -v;
+    const double q = rint(4.0 * v);
+    const double r = v - 0.25 * q;
+    const double th = 6.283185307179586 * r;
+    const double z = th * th;
+    double c = -1.5619206968586225e-16;              /* -1/18! */
+    c = 4.779477332387385e-14 + z * c;               /* 1/16! */
+    c = -1.1470745597729725e-11 + z * c;             /* -1/14! */
+    c = 2.08767569878681e-09 + z * c;                /* 1/12! */
+    c = -2.755731922398589e-07 + z * c;              /* -1/10! */
+    c = 2.48015873015873e-05 + z * c;                /* 1/8! */
+    c = -0.001388888888888889 + z * c;               /* -1/6! */
+    c = 0.041666666666666664 + z * c;                /* 1/4! */
+    c = -0.5 + z * c;                                /* -1/2! */
+    c = 1.0 + z * c;
+    double sn = -8.22063524662433e-18;               /* -1/19! */
+    sn = 2.8114572543455206e-15 + z * sn;            /* 1/17! */
+    sn = -7.647163731819816e-13 + z * sn;            /* -1/15! */
+    sn = 1.6059043836821613e-10 + z * sn;            /* 1/13! */
+    sn = -2.505210838544172e-08 + z * sn;            /* -1/11! */
+    sn = 2.7557319223985893e-06 + z * sn;            /* 1/9! */
+    sn = -0.0001984126984126984 + z * sn;            /* -1/7! */
+    sn = 0.008333333333333333 + z * sn;              /* 1/5! */
+    sn = -0.16666666666666666 + z * sn;              /* -1/3! */
+    sn = th + th * (z * sn);
+    if (q == 0.0) return c;
+    if (q == 1.0) return -sn;
+    return -c;                                       /* q == 2 */
+}
+
+/* e^x for |x| < 700: k = rint(x / ln 2), r = x - k ln 2 (two-part ln 2),
+ * |r| <= 0.347, Taylor series to r^13, times 2^k built from its bits */
+double orc_gen_exp(double x)
+{
+    const double k = rint(x * 1.4426950408889634);
+    const double r = (x - k * GEN_LN2_HI) - k * GEN_LN2_LO;
+    double p = 1.6059043836821613e-10;               /* 1/13! */
+    p = 2.08767569878681e-09 + r * p;                /* 1/12! */
+    p = 2.505210838544172e-08 + r * p;               /* 1/11! */
+    p = 2.755731922398589e-07 + r * p;               /* 1/10! */
+    p = 2.7557319223985893e-06 + r * p;              /* 1/9! */
+    p = 2.48015873015873e-05 + r * p;                /* 1/8! */
+    p = 0.0001984126984126984 + r * p;               /* 1/7! */
+    p = 0.001388888888888889 + r * p;                /* 1/6! */
+    p = 0.008333333333333333 + r * p;                /* 1/5! */
+    p = 0.041666666666666664 + r * p;                /* 1/4! */
+    p = 0.16666666666666666 + r * p;                 /* 1/3! */
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    return p * gen_bits((uint64_t)((long long)k + 1023) << 52);
+}
+
 double orc_normal(uint64_t seed, uint64_t stream, uint64_t idx)
 {
-    /* Box-Muller on two counter-keyed uniforms, u1 in (0,1] */
-    double u1 = (double)((orc_hash(seed, stream, 2 * idx) >> 11) + 1) * 0x1.0p-53;
-    double u2 = (double)(orc_hash(seed, stream, 2 * idx + 1) >> 11) * 0x1.0p-53;
-    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    /* Box-Muller on two counter-keyed uniforms, u1 in (0,1], u2 in [0,1) */
+    const double u1 = (double)((orc_hash(seed, stream, 2 * idx) >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)(orc_hash(seed, stream, 2 * idx + 1) >> 11) * 0x1.0p-53;
+    return sqrt(-2.0 * orc_gen_log(u1)) * orc_gen_cos2pi(u2);
 }
 
 /* ------------------------------------------------------------ builders -- */
@@ -132,32 +244,41 @@ void orc_build_lognormal(int ni, int nj, int nk, uint64_t seed, double sigma,
             for (int j = 3; j <= njmm; ++j) {
                 size_t c = IX(i, j, k);
                 if (i < nimm) {
-                    double a = h * exp(sigma * orc_normal(seed, 1, c));
+                    double a = h * orc_gen_exp(sigma * orc_normal(seed, 1, c));
                     s->ae[c] = a;
                     s->aw[c + nj] = a;
                 }
                 if (j < njmm) {
-                    double a = h * exp(sigma * orc_normal(seed, 2, c));
+                    double a = h * orc_gen_exp(sigma * orc_normal(seed, 2, c));
                     s->an[c] = a;
                     s->as[c + 1] = a;
                 }
                 if (k < nkmm) {
-                    double a = h * exp(sigma * orc_normal(seed, 3, c));
+                    double a = h * orc_gen_exp(sigma * orc_normal(seed, 3, c));
                     s->at[c] = a;
                     s->ab[c + nij] = a;
                 }
             }
+    /* the mean of su: a sum per (k, i) row over j, per plane over the rows
+     * in i order, then over the planes in k order (an order a device kernel
+     * reproduces exactly) */
     double sum = 0.0;
     long cnt = 0;
-    for (int k = 3; k <= nkmm; ++k)
-        for (int i = 3; i <= nimm; ++i)
+    for (int k = 3; k <= nkmm; ++k) {
+        double plane = 0.0;
+        for (int i = 3; i <= nimm; ++i) {
+            double row = 0.0;
             for (int j = 3; j <= njmm; ++j) {
                 size_t c = IX(i, j, k);
                 s->ap[c] = s->ae[c] + s->aw[c] + s->an[c] + s->as[c] + s->at[c] + s->ab[c];
                 s->su[c] = orc_normal(seed, 4, c);
-                sum += s->su[c];
+                row += s->su[c];
                 ++cnt;
             }
+            plane += row;
+        }
+        sum += plane;
+    }
     double mean = sum / (double)cnt;
     for (int k = 3; k <= nkmm; ++k)
         for (int i = 3; i <= nimm; ++i)
@@ -1861,4 +1982,102 @@ void orc_calcuvw(int ni, int nj, int nk, orc_flow *F, double *spv, const orc_drv
         }
         orc_sipsol(ni, nj, nk, X[m], &s, o->alfa, q->sor[m], q->nsw[m], &reps[m]);
     }
+}
+
+/* ------------------------------------------------------------ geometry -- */
+/*
+ * Face area vectors from the node coordinates (init.f90:49-144).  Node
+ * ijk is the (i, j, k) corner that closes cell ijk.  Per direction the face
+ * of cell P is spanned by its corner P, the corner one step back along idns
+ * ("south") and one back along idtb ("bottom"):
+ *   east  (idns 1,  idtb nij): k = 2..nkm, i = 1..nim, j = 2..njm  -> ar1
+ *   north (idns nj, idtb nij): k = 2..nkm, i = 2..nim, j = 1..njm  -> ar2
+ *   top   (idns 1,  idtb nj ): k = 1..nkm, i = 2..nim, j = 2..njm  -> ar3
+ * The vector is the cross product of the face's two mean edge vectors (the
+ * "et" and "zd" directions).  Cells outside these ranges keep their values.
+ */
+static void orc_face_dir(int ni, int nj, int nk, const double *x, const double *y,
+                         const double *z, int idns, int idtb, int klo, int ilo, int jlo,
+                         double *arx, double *ary, double *arz)
+{
+    const int nij = ni * nj, nim = ni - 1, njm = nj - 1, nkm = nk - 1;
+    for (int k = klo; k <= nkm; ++k)
+        for (int i = ilo; i <= nim; ++i)
+            for (int j = jlo; j <= njm; ++j) {
+                const size_t p = IX(i, j, k), s = p - idns, b = p - idtb, bs = b - idns;
+                const double xe = 0.5 * (x[p] - x[s] + x[b] - x[bs]);
+                const double ye = 0.5 * (y[p] - y[s] + y[b] - y[bs]);
+                const double ze = 0.5 * (z[p] - z[s] + z[b] - z[bs]);
+                const double xd = 0.5 * (x[p] - x[b] + x[s] - x[bs]);
+                const double yd = 0.5 * (y[p] - y[b] + y[s] - y[bs]);
+                const double zd = 0.5 * (z[p] - z[b] + z[s] - z[bs]);
+                arx[p] = ye * zd - yd * ze;
+                ary[p] = xd * ze - xe * zd;
+                arz[p] = xe * yd - ye * xd;
+            }
+}
+
+void orc_face_areas(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                    double *const *ar)
+{
+    const int nij = ni * nj;
+    orc_face_dir(ni, nj, nk, x, y, z, 1, nij, 2, 1, 2, ar[0], ar[1], ar[2]);
+    orc_face_dir(ni, nj, nk, x, y, z, nj, nij, 2, 2, 1, ar[3], ar[4], ar[5]);
+    orc_face_dir(ni, nj, nk, x, y, z, 1, nj, 1, 2, 2, ar[6], ar[7], ar[8]);
+}
+
+/*
+ * Intersection of the plane through p1, p2, p3 with the line through p4,
+ * p5 (find_intersection_point.f90:3-76): p4 + (p5 - p4) t with
+ * t = -det[1 1 1 1; p1 p2 p3 p4] / det[1 1 1 0; p1 p2 p3 p5-p4], both
+ * determinants expanded along the first row into the reference's sum order.
+ */
+void orc_find_intersection_point(const double *p1, const double *p2, const double *p3,
+                                  const double *p4, const double *p5, double *pj)
+{
+    const double x1 = p1[0], y1 = p1[1], z1 = p1[2], x2 = p2[0], y2 = p2[1], z2 = p2[2];
+    const double x3 = p3[0], y3 = p3[1], z3 = p3[2], x4 = p4[0], y4 = p4[1], z4 = p4[2];
+    const double dx = p5[0] - x4, dy = p5[1] - y4, dz = p5[2] - z4;
+    /* 2x2 minors of the y, z rows */
+    const double m34 = y3 * z4 - y4 * z3, m24 = y2 * z4 - y4 * z2, m14 = y1 * z4 - y4 * z1;
+    const double m23 = y2 * z3 - y3 * z2, m13 = y1 * z3 - y3 * z1, m12 = y1 * z2 - y2 * z1;
+    const double num = x2 * m34 - x1 * m34 - x3 * m24 + x1 * m24 + x3 * m14 - x2 * m14 + x4 * m23 -
+                       x1 * m23 - x4 * m13 + x2 * m13 + x4 * m12 - x3 * m12;
+    const double d3 = y3 * dz - dy * z3, d2 = y2 * dz - dy * z2, d1 = y1 * dz - dy * z1;
+    const double den = x2 * d3 - x1 * d3 - x3 * d2 + x1 * d2 + x3 * d1 - x2 * d1 + dx * m23 -
+                       dx * m13 + dx * m12;
+    const double t = -num / den;
+    pj[0] = x4 + dx * t;
+    pj[1] = y4 + dy * t;
+    pj[2] = z4 + dz * t;
+}
+
+/*
+ * Interpolation factor of the face between P and E = P + idew (intfac.f90:
+ * 3-51) on k = 2..nkm, i = 2..nim, j = 2..njm: the distance from P's centre
+ * to where the line of centres P-E crosses the face plane (corner P, P - idns,
+ * P - idtb), over the P-E distance.  `small` enters the z term of the
+ * denominator's square and the denominator itself, as written.
+ */
+void orc_intfac(int ni, int nj, int nk, int idew, int idns, int idtb, const double *x,
+                const double *y, const double *z, const double *xc, const double *yc,
+                const double *zc, double *lambda)
+{
+    const int nij = ni * nj, nim = ni - 1, njm = nj - 1, nkm = nk - 1;
+    const double small = (double)1e-30f;
+    for (int k = 2; k <= nkm; ++k)
+        for (int i = 2; i <= nim; ++i)
+            for (int j = 2; j <= njm; ++j) {
+                const size_t p = IX(i, j, k), e = p + idew;
+                const double c1[3] = {x[p], y[p], z[p]};
+                const double c2[3] = {x[p - idns], y[p - idns], z[p - idns]};
+                const double c3[3] = {x[p - idtb], y[p - idtb], z[p - idtb]};
+                const double c4[3] = {xc[p], yc[p], zc[p]};
+                const double c5[3] = {xc[e], yc[e], zc[e]};
+                double f[3];
+                orc_find_intersection_point(c1, c2, c3, c4, c5, f);
+                const double ax = f[0] - xc[p], ay = f[1] - yc[p], az = f[2] - zc[p];
+                const double bx = xc[e] - xc[p], by = yc[e] - yc[p], bz = zc[e] - zc[p] + small;
+                lambda[p] = sqrt(ax * ax + ay * ay + az * az) / (sqrt(bx * bx + by * by + bz * bz) + small);
+            }
 }

@@ -39,6 +39,10 @@ typedef struct {
 uint64_t orc_hash(uint64_t seed, uint64_t stream, uint64_t idx);
 double orc_u01(uint64_t seed, uint64_t stream, uint64_t idx);
 double orc_normal(uint64_t seed, uint64_t stream, uint64_t idx);
+/* the generator's libm-free log, cos(2 pi u) and exp (bit-reproducible) */
+double orc_gen_log(double x);
+double orc_gen_cos2pi(double u);
+double orc_gen_exp(double x);
 
 /* problem builders (synthetic inputs of the measured workloads) */
 void orc_build_lognormal(int ni, int nj, int nk, uint64_t seed, double sigma,
@@ -231,6 +235,16 @@ typedef struct {
  * fluxuvw nor the assembly) is not part of it. */
 void orc_calcuvw(int ni, int nj, int nk, orc_flow *F, double *spv, const orc_drv_opts *o,
                  const orc_uvw_opts *q, orc_report *reps);
+/* geometry: face areas (init.f90:49-144) into ar[9] = {ar1x, ar1y, ar1z,
+ * ar2x, .., ar3z}; the plane/line intersection of find_intersection_point.f90
+ * and the interpolation factors of intfac.f90 */
+void orc_face_areas(int ni, int nj, int nk, const double *x, const double *y, const double *z,
+                    double *const *ar);
+void orc_find_intersection_point(const double *p1, const double *p2, const double *p3,
+                                  const double *p4, const double *p5, double *pj);
+void orc_intfac(int ni, int nj, int nk, int idew, int idns, int idtb, const double *x,
+                const double *y, const double *z, const double *xc, const double *yc,
+                const double *zc, double *lambda);
 #ifdef __cplusplus
 }
 #endif

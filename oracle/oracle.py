@@ -98,6 +98,9 @@ def lib():
         L.orc_normal.restype = d
         L.orc_hash.argtypes = [u64, u64, u64]
         L.orc_hash.restype = u64
+        for fn in ("orc_gen_log", "orc_gen_cos2pi", "orc_gen_exp"):
+            getattr(L, fn).argtypes = [d]
+            getattr(L, fn).restype = d
         L.orc_build_lognormal.argtypes = [i, i, i, u64, d, C.POINTER(_Sys)]
         L.orc_build_general.argtypes = [i, i, i, u64, C.POINTER(_Sys), dp]
         for fn in ("orc_sipsol", "orc_pcgsip", "orc_cgstab_sip"):
@@ -121,6 +124,11 @@ def lib():
         L.orc_strain_rate.argtypes = [i, i, i, dp, dp, dp, dp, dp]
         L.orc_calc_vis_les.argtypes = [i, i, i] + [dp] * 15 + [d, d, d, d, d, i, i, i]
         L.orc_calcuvw.argtypes = [i, i, i, Fl, dp, Op, C.POINTER(UvwOpts), C.POINTER(_Report)]
+        L.orc_face_areas.argtypes = [i, i, i, dp, dp, dp, C.POINTER(dp)]
+        ip = C.POINTER(C.c_int)
+        L.orc_ghostcells.argtypes = [i, i, i, C.POINTER(dp), C.POINTER(dp)] + [ip] * 6 + [dp]
+        L.orc_find_intersection_point.argtypes = [dp] * 6
+        L.orc_intfac.argtypes = [i, i, i, i, i, i] + [dp] * 7
         _lib = L
     return _lib
 
@@ -272,12 +280,102 @@ def bpres(dims, p, geom):
     lib().orc_bpres(*dims, _p(p), _p(geom["fx"]), _p(geom["fy"]), _p(geom["fz"]))
 
 
+# the oracle's orc_fields: the arrays the p' assembly and fluxuvw read
+FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
+               "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
+               "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
+
+
+class _Fields(C.Structure):
+    _fields_ = [(q, C.POINTER(C.c_double)) for q in FIELD_NAMES]
+
+
+def fluxuvw(dims, f, vis, d, out, flags, gds):
+    """fluxuvw-overrelaxed-correction.f90 for direction d (0 east, 1 north,
+    2 top) on the fields f (FIELD_NAMES arrays) with fcf = f1 / f2 / f3 of f;
+    out: dict of acfe, acfw, su, sv, sw, bp, bt, bb, updated in place."""
+    ni, nj, nk = dims
+    nij = ni * nj
+    idew, idns, idtb = [(nj, 1, nij), (1, nij, nj), (nij, nj, 1)][d]
+    dp = C.POINTER(C.c_double)
+    fs = _Fields(*[_p(f[q]) for q in FIELD_NAMES])
+    L = lib()
+    L.orc_fluxuvw.argtypes = [C.c_int] * 3 + [C.c_void_p, dp] + [C.c_int] * 3 + [dp] * 13 + \
+        [C.c_int, C.c_double]
+    arv = [f[f"ar{d + 1}{c}"] for c in "xyz"]
+    L.orc_fluxuvw(*dims, C.byref(fs), _p(vis), idew, idns, idtb, *map(_p, arv),
+                  _p(f[("fx", "fy", "fz")[d]]), _p(out["acfe"]), _p(out["acfw"]),
+                  _p(f[("f1", "f2", "f3")[d]]), _p(out["su"]), _p(out["sv"]), _p(out["sw"]),
+                  _p(out["bp"]), _p(out["bt"]), _p(out["bb"]), int(flags), float(gds))
+
+
+def ghostcells(dims, var, geo, lb, inl):
+    """update_values_at_ghostcells_nonperiodic.f90 in place on var = [u, v,
+    w, te, ed, t, vis]; geo = [fx, fy, fz, ar1x .. ar3z]; lb = int32 [lbw,
+    lbe, lbs, lbn, lbb, lbt]; inl = (uin, vin, win, tein, edin, tin)."""
+    dp = C.POINTER(C.c_double)
+    V = (dp * 7)(*[_p(a) for a in var])
+    G = (dp * 12)(*[_p(a) for a in geo])
+    ip = C.POINTER(C.c_int)
+    for a in lb:
+        assert a.dtype == np.int32 and a.flags.c_contiguous
+    inl_ = np.ascontiguousarray(inl, dtype=np.float64)
+    lib().orc_ghostcells(*dims, V, G, *[a.ctypes.data_as(ip) for a in lb], _p(inl_))
+
+
+AR_FIELDS = ("ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z")
+
+
+def face_areas(dims, x, y, z, ar):
+    """init.f90:49-144 into ar (dict of the nine ar* arrays, updated in place
+    on the reference's loop ranges)"""
+    arr = (C.POINTER(C.c_double) * 9)(*[_p(ar[f]) for f in AR_FIELDS])
+    lib().orc_face_areas(*dims, _p(x), _p(y), _p(z), arr)
+
+
+def find_intersection_point(p1, p2, p3, p4, p5):
+    pts = [np.ascontiguousarray(q, dtype=np.float64) for q in (p1, p2, p3, p4, p5)]
+    out = np.zeros(3)
+    lib().orc_find_intersection_point(*[_p(q) for q in pts], _p(out))
+    return out
+
+
+def intfac(dims, g, lam=None):
+    """intfac.f90 for the three directions as init.f90:147-149 calls it:
+    returns (fx, fy, fz), starting from g's fx, fy, fz (or lam) outside the
+    loop range"""
+    ni, nj, nk = dims
+    nij = ni * nj
+    out = []
+    for q, (de, dn, dt) in zip(("fx", "fy", "fz"), ((nj, 1, nij), (1, nj, nij), (nij, 1, nj))):
+        a = (lam[q] if lam is not None else g[q]).copy()
+        lib().orc_intfac(*dims, de, dn, dt, *[_p(g[c]) for c in ("x", "y", "z", "xc", "yc", "zc")],
+                         _p(a))
+        out.append(a)
+    return out
+
+
 def rng_u01(seed, stream, idx):
     return lib().orc_u01(seed, stream, idx)
 
 
 def rng_normal(seed, stream, idx):
     return lib().orc_normal(seed, stream, idx)
+
+
+def gen_log(x):
+    """the generator's libm-free natural log"""
+    return lib().orc_gen_log(x)
+
+
+def gen_cos2pi(u):
+    """the generator's libm-free cos(2 pi u), u in [0, 1)"""
+    return lib().orc_gen_cos2pi(u)
+
+
+def gen_exp(x):
+    """the generator's libm-free exp"""
+    return lib().orc_gen_exp(x)
 
 
 # ------------------------------------------------- pressure-correction drivers --
@@ -384,11 +482,11 @@ def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8, driver=REF_DRIVER):
                        capture_output=True)
         raw = np.fromfile(fout, dtype=np.float64)
         log = open(flog).read()
-    if mode == "continuity":
+    if mode in ("continuity", "fip"):
         return [raw.copy()], None, log
     nout = {"sipsol": 9, "cgstab_sip": 9, "pcgsip": 9, "iccg": 9, "grad_gauss": 3,
             "bpres": 1, "fluxmass": 5, "piso_assemble": 5, "assemble_p": 5, "fluxmc": 5,
-            "fluxuvw": 8, "ghost": 7, "outbc": 15, "cbc": 5, "les": 1}[mode]
+            "fluxuvw": 8, "ghost": 7, "outbc": 15, "cbc": 5, "les": 1, "intfac": 3}[mode]
     outs = [raw[i * n:(i + 1) * n].copy() for i in range(nout)]
     resor = float(raw[nout * n]) if raw.size > nout * n else None
     return outs, resor, log

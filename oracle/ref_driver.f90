@@ -14,12 +14,14 @@
 ! usage: ref_driver <mode> <in.bin> <out.bin> <log.txt>
 !   mode = sipsol | cgstab_sip | grad_gauss | bpres | fluxmass | piso_assemble
 !          | assemble_p | fluxmc | continuity | fluxuvw | ghost | outbc | cbc | les
+!          | intfac | fip
 ! in.bin  : int32 ni,nj,nk,nsw ; real64 alfa,sor ; then mode-specific arrays
 !           (stencil modes: nsw = direction 0 east, 1 north, 2 top; alfa = ipc;
 !            fluxuvw: nsw = direction + 8 * scheme flags, alfa = gds(iu);
 !            outbc: nsw = lout + 2 low + 4 loe + 8 los + 16 lon + 32 lob + 64 lot,
 !            alfa = flowin; les: alfa = urf(ivis), then real64 viscos, densit,
-!            cappa, elog and int32 nx, ny, nz)
+!            cappa, elog and int32 nx, ny, nz; intfac: x..zc, fx, fy, fz;
+!            fip: nsw = number of point sets, then 15 reals per set)
 ! out.bin : mode-specific arrays, then real64 resor(ip) for the solvers
 !-----------------------------------------------------------------------
 program ref_driver
@@ -218,6 +220,22 @@ program ref_driver
     zero = 0.0d0
     call calc_vis_les
     write(11) vis
+  case ('intfac')
+    read(10) x, y, z, xc, yc, zc, fx, fy, fz
+    call intfac(nj, 1, nij, fx)
+    call intfac(1, nj, nij, fy)
+    call intfac(nij, 1, nj, fz)
+    write(11) fx, fy, fz
+  case ('fip')
+    ! find_intersection_point on nsw_ point sets of 15 values each (in x)
+    read(10) x(1:15*nsw_)
+    do i = 1, nsw_
+      call find_intersection_point(x(15*i-14), x(15*i-13), x(15*i-12), x(15*i-11), &
+                                   x(15*i-10), x(15*i-9), x(15*i-8), x(15*i-7), x(15*i-6), &
+                                   x(15*i-5), x(15*i-4), x(15*i-3), x(15*i-2), x(15*i-1), &
+                                   x(15*i), y(3*i-2), y(3*i-1), y(3*i))
+    end do
+    write(11) y(1:3*nsw_)
   case default
     print *, 'unknown mode ', trim(mode)
     stop 2

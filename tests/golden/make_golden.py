@@ -253,10 +253,12 @@ def les_ref_arrays(f, par):
                                      np.array([par["nx"], par["ny"], par["nz"]], dtype=np.int32)]
 
 
-def main():
-    if not O.ref_available():
-        sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
+def solver_cases(only=None):
+    """Solver fixtures from the reference build; `only`: names to rewrite
+    (None: all)."""
     for name, routine, kind, dims, seed, nsw, alfa, full in SOLVER_CASES:
+        if only is not None and name not in only:
+            continue
         s, fi = solver_inputs(kind, dims, seed)
         arrays = [fi, s.ae, s.aw, s.an, s.as_, s.at, s.ab, s.ap, s.su, s.res] + \
                  [getattr(s, f) for f in B_FIELDS]
@@ -270,6 +272,37 @@ def main():
             rec[f + "_sha256"] = digest(a)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
         print(f"{name}: iters {info['iters']} resor {resor:.6e}")
+
+
+# SHA-256 of the oracle's log-normal systems at sizes too large for a GPU
+# test to rebuild on the CPU: the generator's device output must match them
+LOGNORMAL_DIGESTS = [((260, 260, 260), 2024, 1.0), ((132, 132, 132), 7, 1.0)]
+
+
+def lognormal_digests():
+    import json
+    out = []
+    for dims, seed, sigma in LOGNORMAL_DIGESTS:
+        sy = O.build_lognormal(dims, seed, sigma)
+        out.append(dict(dims=list(dims), seed=seed, sigma=sigma,
+                        sha256={f: digest(getattr(sy, f))
+                                for f in ("ae", "aw", "an", "as_", "at", "ab", "ap", "su")}))
+    with open(os.path.join(HERE, "lognormal_digests.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print("lognormal_digests.json: written")
+
+
+def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--lognormal":
+        lognormal_digests()         # oracle only: needs no reference build
+        return
+    if not O.ref_available():
+        sys.exit("oracle/_ref/ref_driver is not built (make -C oracle ref)")
+    if len(sys.argv) > 1 and sys.argv[1] == "--solvers":
+        # rewrite only the named solver fixtures (all solver fixtures when none is named)
+        solver_cases(set(sys.argv[2:]) or None)
+        return
+    solver_cases()
     for routine, dims, seed in ASSEMBLY_CASES:
         f = fields(dims, seed)
         ipc = 0 if routine.endswith("_ipc0") else 1

@@ -674,3 +674,84 @@ def test_symmetry_of_the_test_systems(O, dims):
     g, _ = O.build_general(dims, 5)
     bad = _asymmetric_couplings(g, dims, 3, dims[2] - 2)
     assert bad.size > 0.9 * (ni - 4) * (nj - 4) * (dims[2] - 4)
+
+
+# ------------------------------------------- the seeded generator's functions
+def test_generator_functions_against_high_precision(O):
+    """The generator's libm-free log, cos(2 pi u) and exp (the same
+    operations as the device generator, so both give the same bits) against
+    200-bit references: log within 2 ulp, exp within 1 ulp, cos within 2e-16
+    absolute, on random and boundary arguments (a wrong coefficient, sign or
+    reduction step fails these by orders of magnitude)."""
+    import math
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.prec = 200
+    rng = np.random.default_rng(1)
+
+    def ulps(got, exact):
+        e = float(exact)
+        return abs(got - e) / math.ulp(e)
+    xs = np.concatenate([rng.uniform(0, 1, 1500), 2.0 ** -rng.uniform(0, 53, 800),
+                         [1.0, 0.5, 2.0 ** -53, 0.7071067811865476, 0.7071067811865475]])
+    assert max(ulps(O.gen_log(float(x)), mpmath.log(mpmath.mpf(float(x)))) for x in xs) <= 2.0
+    us = np.concatenate([rng.uniform(0, 1, 2000), [0.0, 0.125, 0.25, 0.375, 0.5, 0.625, 0.75,
+                                                   0.875, 1.0 - 2.0 ** -53]])
+    err = max(abs(O.gen_cos2pi(float(u)) - float(mpmath.cos(2 * mpmath.pi * mpmath.mpf(float(u)))))
+              for u in us)
+    assert err <= 2e-16
+    xe = np.concatenate([rng.uniform(-8, 8, 1500), rng.uniform(-700, 700, 300), [0.0, 1.0, -1.0]])
+    assert max(ulps(O.gen_exp(float(x)), mpmath.exp(mpmath.mpf(float(x)))) for x in xe) <= 1.0
+
+
+def test_generator_normal_moments(O):
+    """Box-Muller on the counter-keyed uniforms: 200 000 draws have the
+    moments of N(0, 1) to within a few standard errors."""
+    g = np.array([O.rng_normal(5, 4, i) for i in range(200000)])
+    se = 1.0 / np.sqrt(g.size)
+    assert abs(g.mean()) < 4 * se
+    assert abs(g.var() - 1.0) < 4 * np.sqrt(2.0) * se
+    assert abs(((g - g.mean()) ** 3).mean()) < 4 * np.sqrt(6.0) * se
+    assert abs(((g - g.mean()) ** 4).mean() / g.var() ** 2 - 3.0) < 4 * np.sqrt(24.0) * se
+
+
+def test_lognormal_system_structure(O):
+    """The log-normal operator: symmetric interior faces (aw(E) = ae(P) etc.),
+    ap = sum of the six, su of zero mean over the interior except the pinned
+    reference cell, ghost rows a* = 0, ap = 1, su = 0; log(a / h) ~ N(0, sigma^2)."""
+    dims = (14, 13, 12)
+    ni, nj, nk = dims
+    s = O.build_lognormal(dims, 9, 0.8)
+    sh = (nk, ni, nj)
+    A = {f: getattr(s, f).reshape(sh) for f in ("ae", "aw", "an", "as_", "at", "ab", "ap", "su")}
+    I = (slice(2, nk - 2), slice(2, ni - 2), slice(2, nj - 2))
+    # aw(E) = ae(P), as(N) = an(P), ab(T) = at(P) for interior P, E, N, T;
+    # pinning zeroes the reference cell's own row only, so its neighbours
+    # keep the face value it had
+    unpinned = dict(ae=A["ae"].copy(), an=A["an"].copy(), at=A["at"].copy())
+    unpinned["ae"][2, 2, 2] = A["aw"][2, 3, 2]
+    unpinned["an"][2, 2, 2] = A["as_"][2, 2, 3]
+    unpinned["at"][2, 2, 2] = A["ab"][3, 2, 2]
+    np.testing.assert_array_equal(A["aw"][2:-2, 3:-2, 2:-2], unpinned["ae"][2:-2, 2:-3, 2:-2])
+    np.testing.assert_array_equal(A["as_"][2:-2, 2:-2, 3:-2], unpinned["an"][2:-2, 2:-2, 2:-3])
+    np.testing.assert_array_equal(A["ab"][3:-2, 2:-2, 2:-2], unpinned["at"][2:-3, 2:-2, 2:-2])
+    assert np.all(unpinned["ae"][2:-2, 2:-3, 2:-2] > 0)
+    assert np.all(A["ae"][2:-2, -3, 2:-2] == 0) and np.all(A["aw"][2:-2, 2, 2:-2] == 0)
+    tot = A["ae"] + A["aw"] + A["an"] + A["as_"] + A["at"] + A["ab"]
+    m = np.ones(sh, bool)
+    m[2, 2, 2] = False
+    np.testing.assert_array_equal(A["ap"][I][m[I]], tot[I][m[I]])
+    assert A["ap"][2, 2, 2] == 1.0 and A["su"][2, 2, 2] == 0.0
+    ghost = np.ones(sh, bool)
+    ghost[I] = False
+    assert np.all(A["ap"][ghost] == 1.0) and np.all(A["su"][ghost] == 0.0)
+    # su = g - mean(g) with g the stream-4 normals of the interior cells
+    # (reference cell included in the mean, then pinned to 0)
+    idx = np.arange(s.n).reshape(sh)[I].ravel()
+    raw = np.array([O.rng_normal(9, 4, int(c)) for c in idx])
+    off = raw - A["su"][I].ravel()
+    off = np.delete(off, 0)                   # the reference cell is first in k, i, j order
+    assert np.ptp(off) < 1e-14
+    assert abs(off[0] - raw.mean()) < 1e-14
+    h = 1.0 / (ni - 4)
+    g = np.log(unpinned["ae"][2:-2, 2:-3, 2:-2] / h).ravel()
+    assert abs(g.std() - 0.8) < 0.1 and abs(g.mean()) < 0.1
