@@ -483,6 +483,7 @@ int solve(cap_ctx *ctx, int solver, double *x, const cap_coef *c, const cap_drv_
 {
     switch (solver) {
     case 1: return cap_sipsol(ctx, x, c, o->alfa, sor, o->nsw, rep);
+    case 2: return cap_cgstab(ctx, x, c, sor, o->nsw, rep);
     case 3: return cap_iccg(ctx, x, c, sor, o->nsw, rep);
     case 4: return cap_pcgsip(ctx, x, c, o->alfa, sor, o->nsw, rep);
     case 5: return cap_cgstab_sip(ctx, x, c, o->alfa, sor, o->nsw, rep);
@@ -596,8 +597,7 @@ extern "C" int cap_calcp(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, cap_r
         !f->gradw || !f->apu || !f->apv || !f->apw || !f->den)
         return CAP_ERR_ARG;
     if (o->ncorr < 1 || o->npcor < 1 || o->ijkpr < 1 || (size_t)o->ijkpr > ctx->nstd) return CAP_ERR_ARG;
-    if (o->solver != 1 && o->solver != 3 && o->solver != 4 && o->solver != 5 && o->solver != 6)
-        return CAP_ERR_ARG;
+    if (o->solver < 1 || o->solver > 6) return CAP_ERR_ARG;
     const Dim d = dim_of(ctx);
     const size_t n = ctx->nstd;
     const cap_fields F = fields_of(f);

@@ -42,6 +42,7 @@ enum Fin {
     F_BIC_RESL,    // resl, stop test                        cgstab_sip.f90:256-266
     F_RAW,         // keep the sums in red[]
     F_BIC_RESL_BET,  // F_BIC_RESL on v[0]; unless that stops, F_BIC_BET on v[1]
+    F_JAC_RESL_SK,   // F_PCG_RESL on v[0]; unless that stops, F_PCG_SK on v[1]
 };
 
 __device__ __forceinline__ void fin_resl(DevState *st, double resl)
@@ -94,6 +95,15 @@ __device__ __forceinline__ void finalize(DevState *st, const double *v)
         fin_resl(st, v[0]);
     } else if constexpr (FIN == F_RAW) {
         for (int q = 0; q < 4; ++q) st->red[q] = v[q];
+    } else if constexpr (FIN == F_JAC_RESL_SK) {
+        // the stop test (pcg-jacobi.f90:136-143), then the next iteration's
+        // sk and bet (77-92), summed in the same pass from the new residual
+        st->s0 = st->sk;
+        fin_resl(st, v[0]);
+        if (!st->done) {
+            st->sk = v[1];
+            st->bet = v[1] / st->s0;
+        }
     } else if constexpr (FIN == F_BIC_RESL_BET) {
         // the stop test, then (when the solve goes on) the next iteration's
         // bet, om, beto, which its k_dot would compute from the same residual

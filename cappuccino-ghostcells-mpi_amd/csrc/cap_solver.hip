@@ -679,15 +679,17 @@ static int ensure_tiled(cap_ctx *ctx, size_t n)
     return CAP_OK;
 }
 
-// iccg's backward-sweep pack {ae, an, at, d}: only cap_iccg reads it, so it
-// is allocated on the first ICCG solve of a size, not with the other vectors
-static int ensure_pkb(cap_ctx *ctx, size_t n)
+// the backward-sweep packs {ae, an, at, d} of iccg (one) and cgstab (two,
+// the second at pkb + 4 n): only those solvers read them, so they are
+// allocated on the first such solve of a size, not with the other vectors
+static int ensure_pkb(cap_ctx *ctx, size_t n, int packs = 1)
 {
-    if (ctx->pbcap >= n) return CAP_OK;
+    const size_t need = n * (size_t)packs;
+    if (ctx->pbcap >= need) return CAP_OK;
     if (ctx->pkb) { (void)hipFree(ctx->pkb); ctx->pkb = nullptr; }
     ctx->pbcap = 0;
-    HIPCHK(hipMalloc(&ctx->pkb, 4 * n * sizeof(double)));
-    ctx->pbcap = n;
+    HIPCHK(hipMalloc(&ctx->pkb, 4 * need * sizeof(double)));
+    ctx->pbcap = need;
     return CAP_OK;
 }
 
@@ -777,8 +779,8 @@ static int prepare(cap_ctx *ctx, const Lay &L, double *fi, const cap_coef *c, So
     if (!fi || !c || !c->ae || !c->aw || !c->an || !c->as || !c->at || !c->ab || !c->ap || !c->su)
         return fail(ctx, CAP_ERR_ARG, "null array argument");
     if ((rc = ensure_tiled(ctx, L.n))) return rc;
-    {   // message slots for the largest sweep (three chained FP64 values)
-        const size_t bytes = msg_alloc(L, 3) * sizeof(double);
+    {   // message slots for the largest sweep (four chained FP64 values)
+        const size_t bytes = msg_alloc(L, 4) * sizeof(double);
         if (ctx->msg_bytes < bytes) {
             if (ctx->msg) (void)hipFree(ctx->msg);
             ctx->msg = nullptr;
@@ -1270,6 +1272,213 @@ extern "C" int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, doubl
     DevState h;
     if ((rc = iterate(ctx, nsw, one, h))) return rc;
     if ((rc = writeback(ctx, L, fi, c, d, false, true))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    return finish_report(ctx, h, rep, ms);
+}
+
+
+// ------------------------------------------------------- cgstab (ILU(0))
+// Marks the planes k = 2 and nkm (kk = 1 and nks) of cgstab's second backward
+// sweep, which does not reach them (cgstab.f90:212): a NaN in place of ae in
+// their packed element.
+__global__ __launch_bounds__(BLOCK) void k_mark_end_planes(Lay L, double *__restrict__ pk)
+{
+    const double nan = __longlong_as_double(0x7FF8000000000001ll);
+    for_cells(L, [&](const Cell &c) {
+        if (c.kk == 1 || c.kk == L.nks) pk[4 * c.e] = nan;
+    });
+}
+
+// Replaces `call cgstab(fi, ifi)` (cgstab.f90:1-284): BiCGStab with the
+// diagonal ILU(0) preconditioner on k = 2..nkm, i, j = 3..n?mm.  The
+// preconditioner is the IC(0) sweep pair of iccg with the diagonal of
+// cgstab.f90:58-66; the BiCGStab kernels are cgstab_sip's.
+extern "C" int cap_cgstab(cap_ctx *ctx, double *fi, const cap_coef *c, double sor, int nsw,
+                          cap_report *rep)
+{
+    if (!ctx) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 2, ctx->nk - 1);
+    SolveIn d;
+    int rc = prepare(ctx, L, fi, c, d, false, false);
+    if (rc) return rc;
+    if ((rc = ensure_pkb(ctx, L.n, 2))) return rc;
+    if ((rc = init_state(ctx, sor, nsw))) return rc;
+    sym_check(ctx, L);
+    double *const *T = ctx->T;
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    // the solver's automatic arrays are zero on and around the range; BB, BW,
+    // BS carry the diagonal's chained ae, an, at
+    const int zero[9] = {T_ZK, T_PK, T_UK, T_VK, T_RESO, T_D, T_BB, T_BW, T_BS};
+    for (int v = 0; v < 9; ++v)
+        hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[zero[v]], L.n, 0.0);
+    hipLaunchKernelGGL((k_resid<F_RES0>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                       (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
+                       ctx->partials, ctx->ticket);
+    {
+        WaveIO<OpIluDiag> io;
+        io.in[0] = T[T_AP]; io.in[1] = T[T_AW]; io.in[2] = T[T_AS]; io.in[3] = T[T_AB];
+        io.in[4] = T[T_AE]; io.in[5] = T[T_AN]; io.in[6] = T[T_AT];
+        io.dep[0] = T[T_D]; io.dep[1] = T[T_BB]; io.dep[2] = T[T_BW]; io.dep[3] = T[T_BS];
+        io.xo[0] = nullptr;
+        launch_wave<OpIluDiag, true, F_NONE>(ctx, L, OpIluDiag(), io);
+        const int gp = (int)std::min<size_t>((L.n + BLOCK - 1) / BLOCK, 8192);
+        double *pkb2 = ctx->pkb + 4 * L.n;
+        hipLaunchKernelGGL(k_pack4<double>, dim3(gp), dim3(BLOCK), 0, s, L.n, (const double *)T[T_AW],
+                           (const double *)T[T_AS], (const double *)T[T_AB], (const double *)T[T_D],
+                           ctx->pkd);
+        hipLaunchKernelGGL(k_pack4<double>, dim3(gp), dim3(BLOCK), 0, s, L.n, (const double *)T[T_AE],
+                           (const double *)T[T_AN], (const double *)T[T_AT], (const double *)T[T_D],
+                           ctx->pkb);
+        hipLaunchKernelGGL(k_pack4<double>, dim3(gp), dim3(BLOCK), 0, s, L.n, (const double *)T[T_AE],
+                           (const double *)T[T_AN], (const double *)T[T_AT], (const double *)T[T_D],
+                           pkb2);
+        hipLaunchKernelGGL(k_mark_end_planes, dim3(gb), dim3(BLOCK), 0, s, L, pkb2);
+    }
+    // z = M^-1 rhs: forward sweep, then the scaled backward sweep over all
+    // planes (mid = false) or over k = 3..nkmm only (mid = true)
+    auto apply = [&](const double *rhs, bool mid) {
+        WaveIO<OpIcFwd> f;
+        f.in[0] = rhs; f.in[1] = T[T_AW]; f.in[2] = T[T_AS]; f.in[3] = T[T_AB]; f.in[4] = T[T_D];
+        f.dep[0] = T[T_ZK];
+        f.xo[0] = nullptr;
+        f.pk = ctx->pkd;
+        launch_wave<OpIcFwd, true, F_NONE>(ctx, L, OpIcFwd(), f);
+        if (!mid) {
+            OpIcBwd ob;
+            ob.small = (double)1e-30f;
+            WaveIO<OpIcBwd> b;
+            b.in[0] = T[T_ZK]; b.in[1] = T[T_AE]; b.in[2] = T[T_AN]; b.in[3] = T[T_AT]; b.in[4] = T[T_D];
+            b.dep[0] = T[T_ZK];
+            b.xo[0] = nullptr;
+            b.pk = ctx->pkb;
+            launch_wave<OpIcBwd, false, F_NONE>(ctx, L, ob, b);
+        } else {
+            OpIcBwdMid ob;
+            ob.small = (double)1e-30f;
+            WaveIO<OpIcBwdMid> b;
+            b.in[0] = T[T_ZK]; b.in[1] = T[T_AE]; b.in[2] = T[T_AN]; b.in[3] = T[T_AT]; b.in[4] = T[T_D];
+            b.dep[0] = T[T_ZK];
+            b.xo[0] = nullptr;
+            b.pk = ctx->pkb + 4 * L.n;
+            launch_wave<OpIcBwdMid, false, F_NONE>(ctx, L, ob, b);
+        }
+    };
+    int queued = 0;
+    auto one = [&]() {
+        // bet = res.reso, om, beto (cgstab.f90:94-105); after the first
+        // iteration the previous iteration's last update has summed it
+        if (queued++ == 0)
+            hipLaunchKernelGGL((k_dot<F_BIC_BET, false>), dim3(gb), dim3(BLOCK), 0, s, L,
+                               (const double *)T[T_RES], T[T_RESO], ctx->st, ctx->partials, ctx->ticket);
+        hipLaunchKernelGGL(k_bic_p, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_RES],
+                           (const double *)T[T_UK], T[T_PK], ctx->st);
+        apply(T[T_PK], false);
+        hipLaunchKernelGGL((k_spmv<double, 1, F_BIC_GAM>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_ZK], T[T_UK], (const double *)T[T_RESO], ctx->st,
+                           ctx->partials, ctx->ticket, ctx->asym);
+        hipLaunchKernelGGL((k_update<double, 1, F_NONE>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                           T[T_RES], (const double *)T[T_ZK], (const double *)T[T_UK], ctx->st,
+                           ctx->partials, ctx->ticket);
+        apply(T[T_RES], true);
+        hipLaunchKernelGGL((k_spmv<double, 2, F_BIC_ALF>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_ZK], T[T_VK], (const double *)T[T_RES], ctx->st,
+                           ctx->partials, ctx->ticket, ctx->asym);
+        hipLaunchKernelGGL(k_bic_update_bet, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI], T[T_RES],
+                           (const double *)T[T_ZK], (const double *)T[T_VK], (const double *)T[T_RESO],
+                           ctx->st, ctx->partials, ctx->ticket);
+    };
+    // reso = res before the first iteration (cgstab.f90:75)
+    hipLaunchKernelGGL((k_dot<F_RAW, true>), dim3(gb), dim3(BLOCK), 0, s, L,
+                       (const double *)T[T_RES], T[T_RESO], ctx->st, ctx->partials, ctx->ticket);
+    DevState h;
+    if ((rc = iterate(ctx, nsw, one, h))) return rc;
+    if ((rc = writeback(ctx, L, fi, c, d, false, false))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    return finish_report(ctx, h, rep, ms);
+}
+
+// --------------------------------------------------------- Jacobi PCG
+// z = r/(ap + small) and sk += r z (pcg-jacobi.f90:77-92), alone (FIN =
+// F_PCG_SK, the first iteration) or after the previous iteration's update
+// fi += alf p, r -= alf z, sum |r| and the stop test (pcg-jacobi.f90:124-143):
+// then FIN = F_JAC_RESL_SK, whose finaliser runs the stop test before it
+// takes sk and bet for the next iteration.
+template <bool UPDATE, int FIN>
+__global__ __launch_bounds__(BLOCK) void k_jac_z(Lay L, double *__restrict__ fi,
+                                                 double *__restrict__ res,
+                                                 const double *__restrict__ p,
+                                                 double *__restrict__ z,
+                                                 const double *__restrict__ ap, DevState *st,
+                                                 double *partials, unsigned *ticket)
+{
+    if (st->done) return;
+    const double a = st->alf, small = st->small;
+    Acc acc[2];
+    for_cells(L, [&](const Cell &c) {
+        double r = res[c.e];
+        if constexpr (UPDATE) {
+            fi[c.e] = fi[c.e] + a * p[c.e];
+            r = r - a * z[c.e];
+            res[c.e] = r;
+            acc[0] += fabs(r);
+        }
+        const double zz = r / (ap[c.e] + small);
+        z[c.e] = zz;
+        acc[1] += r * zz;
+    });
+    if constexpr (UPDATE) {
+        grid_reduce<2, FIN>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
+    } else {
+        Acc one[1] = {acc[1]};
+        grid_reduce<1, FIN>(one, blockIdx.x, gridDim.x, partials, ticket, st);
+    }
+}
+
+// Replaces `call Diagonally_Preconditioned_CG(fi, ifi)` (pcg-jacobi.f90:
+// 3-155): CG with the Jacobi preconditioner on k, i, j = 3..n?mm.  Each
+// iteration is three passes: p = z + bet p; z = A p with (p, z); then the
+// update with the next iteration's z and (r, z).
+extern "C" int cap_pcg_jacobi(cap_ctx *ctx, double *fi, const cap_coef *c, double sor, int nsw,
+                              cap_report *rep)
+{
+    if (!ctx) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 3, ctx->nk - 2);
+    SolveIn d;
+    int rc = prepare(ctx, L, fi, c, d, false, false);
+    if (rc) return rc;
+    if ((rc = init_state(ctx, sor, nsw))) return rc;
+    sym_check(ctx, L);
+    double *const *T = ctx->T;
+    const int gb = grid_for(L);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_ZK], L.n, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_PK], L.n, 0.0);
+    hipLaunchKernelGGL((k_resid<F_RES0>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                       (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
+                       ctx->partials, ctx->ticket);
+    int queued = 0;
+    auto one = [&]() {
+        if (queued++ == 0)
+            hipLaunchKernelGGL((k_jac_z<false, F_PCG_SK>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                               T[T_RES], (const double *)T[T_PK], T[T_ZK], (const double *)T[T_AP],
+                               ctx->st, ctx->partials, ctx->ticket);
+        hipLaunchKernelGGL(k_pupdate<double>, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_ZK],
+                           T[T_PK], ctx->st);
+        hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L,
+                           A, (const double *)T[T_PK], T[T_ZK], (const double *)nullptr, ctx->st,
+                           ctx->partials, ctx->ticket, ctx->asym);
+        hipLaunchKernelGGL((k_jac_z<true, F_JAC_RESL_SK>), dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI],
+                           T[T_RES], (const double *)T[T_PK], T[T_ZK], (const double *)T[T_AP],
+                           ctx->st, ctx->partials, ctx->ticket);
+    };
+    DevState h;
+    if ((rc = iterate(ctx, nsw, one, h))) return rc;
+    if ((rc = writeback(ctx, L, fi, c, d, true, false))) return rc;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     return finish_report(ctx, h, rep, ms);

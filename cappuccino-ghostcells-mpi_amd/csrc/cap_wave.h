@@ -325,4 +325,56 @@ struct OpIcBwdDot {
     }
 };
 
+// Diagonal ILU(0) of cgstab.f90:58-66:
+//   d = 1/(ap - aw d(i-1) ae(i-1) - as d(j-1) an(j-1) - ab d(k-1) at(k-1))
+// The west, south and bottom neighbours' own ae, an, at enter the product
+// after their d, so they travel with d as chained values: chained
+// {d, ae, an, at}, x = {ap, aw, as, ab, ae, an, at}.
+struct OpIluDiag {
+    using T = double;
+    static constexpr int NIN = 7, ND = 4, NX = 0, NR = 0;
+    CAP_HD static void compute(const double *x, const double *dk, const double *di,
+                               const double *dj, double *d, double *, double *)
+    {
+        d[0] = 1. / (x[0] - x[1] * di[0] * di[1] - x[2] * dj[0] * dj[2] - x[3] * dk[0] * dk[3]);
+        d[1] = x[4];
+        d[2] = x[5];
+        d[3] = x[6];
+    }
+};
+
+// ILU(0) scaling + backward substitution, cgstab.f90:129-148:
+//   z = z/(d+small);  z = (z + ae*z(i+1) + an*z(j+1) + at*z(k+1)) * d
+// x = {z, ae, an, at, d}, x[1..4] packed
+struct OpIcBwd {
+    using T = double;
+    static constexpr int NIN = 5, ND = 1, NX = 0, NR = 0;
+    static constexpr int PKN = 4;
+    double small;
+    CAP_HD void compute(const double *x, const double *dk, const double *di, const double *dj,
+                        double *d, double *, double *) const
+    {
+        const double z = x[0] / (x[4] + small);
+        d[0] = (z + x[1] * di[0] + x[2] * dj[0] + x[3] * dk[0]) * x[4];
+    }
+};
+
+// cgstab.f90's second preconditioner application: the scaling covers
+// k = 2..nkm (201-208) but the backward sweep only k = 3..nkmm (212-220), so
+// the cells of planes k = 2 and nkm keep the scaled value.  Those cells carry
+// a NaN in place of ae in their packed element (no coefficient is NaN).
+struct OpIcBwdMid {
+    using T = double;
+    static constexpr int NIN = 5, ND = 1, NX = 0, NR = 0;
+    static constexpr int PKN = 4;
+    double small;
+    CAP_HD void compute(const double *x, const double *dk, const double *di, const double *dj,
+                        double *d, double *, double *) const
+    {
+        const double z = x[0] / (x[4] + small);
+        const bool end_plane = x[1] != x[1];
+        d[0] = end_plane ? z : (z + x[1] * di[0] + x[2] * dj[0] + x[3] * dk[0]) * x[4];
+    }
+};
+
 }  // namespace cap

@@ -4,13 +4,17 @@
 ! subroutines taking (fi, ifi)):
 !   sipsol     sipsol.f90:3       pcgsip      pcg-sip.f90:3
 !   iccg       iccg.f90:3         cgstab_sip  cgstab_sip.f90:1
+!   cgstab     cgstab.f90:1       Diagonally_Preconditioned_CG  pcg-jacobi.f90:3
 ! and for the p' stencils, with the reference's argument lists:
 !   grad_gauss  GRAD_GAUSS.f90:1   bpres  bpres.f90:3   fluxmass  fluxmass2.f90:3
 !   assemble_pressure_eq  assemble_pressure_eq.f90:3
 !   piso_assemble_pressure_eq  PISO_assemble_pressure_eq.f90:3
+!   fluxuvw  fluxuvw-overrelaxed-correction.f90:3   fluxmc  fluxmc.f90:3
+!   update_values_at_ghostcells  update_values_at_ghostcells_nonperiodic.f90:3
 ! Build: compile this file together with the host program's own module
 ! file (the one defining modules types, parameters, indexes, coef, coefb,
-! title_mod) in place of the four solver files, and link libcapamd.so.
+! title_mod, geometry, variables, gradients, bc, inlet) in place of the
+! replaced files, and link libcapamd.so.
 ! Like the originals they read ae..ap, su, res, bb..bt, alfa, sor(ifi) and
 ! nsw(ifi) from the modules; they write fi, res, bb..bt and resor(ifi) and
 ! print the same one-line solver report on unit 66.
@@ -188,3 +192,134 @@ subroutine piso_assemble_pressure_eq(idew, idns, idtb, arvx, arvy, arvz, fif, ac
                  cap_addr(arvz), cap_addr(fif), cap_addr(acfe), cap_addr(acfw), cap_addr(fcf), &
                  int(ipc, c_int), cap_addr(bp), cap_addr(su)), 'piso_assemble_pressure_eq')
 end subroutine piso_assemble_pressure_eq
+
+subroutine cgstab(fi, ifi)
+  use types; use parameters; use indexes; use coef; use title_mod
+  use iso_c_binding; use cap_hip
+  implicit none
+  integer, intent(in) :: ifi
+  real(prec), dimension(nxyza) :: fi
+  type(cap_coef) :: c
+  type(cap_report) :: rep
+  integer(c_int) :: rc
+  ! no SIP factors: only the coef arrays and res are handed over
+  c%ae = cap_addr(ae); c%aw = cap_addr(aw); c%an = cap_addr(an); c%as = cap_addr(as)
+  c%at = cap_addr(at); c%ab = cap_addr(ab); c%ap = cap_addr(ap); c%su = cap_addr(su)
+  c%res = cap_addr(res)
+  rc = cap_cgstab(cap_context(ni, nj, nk), cap_addr(fi), c, sor(ifi), int(nsw(ifi), c_int), rep)
+  call cap_check(rc, 'cgstab')
+  resor(ifi) = rep%resor
+  write(66,'(3a,1PE10.3,a,1PE10.3,a,I0)') &
+  'BiCGStab(ILU(0)):  Solving for ',trim(chvarSolver(IFI)), &
+  ', Initial residual = ',rep%resor,', Final residual = ',rep%resl,', No Iterations ',rep%iters
+end subroutine cgstab
+
+subroutine Diagonally_Preconditioned_CG(fi, ifi)
+  use types; use parameters; use indexes; use coef; use title_mod
+  use iso_c_binding; use cap_hip
+  implicit none
+  integer, intent(in) :: ifi
+  real(prec), dimension(nxyza) :: fi
+  type(cap_coef) :: c
+  type(cap_report) :: rep
+  integer(c_int) :: rc
+  c%ae = cap_addr(ae); c%aw = cap_addr(aw); c%an = cap_addr(an); c%as = cap_addr(as)
+  c%at = cap_addr(at); c%ab = cap_addr(ab); c%ap = cap_addr(ap); c%su = cap_addr(su)
+  c%res = cap_addr(res)
+  rc = cap_pcg_jacobi(cap_context(ni, nj, nk), cap_addr(fi), c, sor(ifi), int(nsw(ifi), c_int), &
+                      rep)
+  call cap_check(rc, 'Diagonally_Preconditioned_CG')
+  resor(ifi) = rep%resor
+  write(66,'(3a,1PE10.3,a,1PE10.3,a,I0)') &
+  'PCG(Jacobi):  Solving for ',trim(chvarSolver(IFI)), &
+  ', Initial residual = ',rep%resor,', Final residual = ',rep%resl,', No Iterations ',rep%iters
+end subroutine Diagonally_Preconditioned_CG
+
+! `call fluxuvw(idew,idns,idtb, arvx,arvy,arvz, fif, acfe,acfw,fcf)` for one
+! direction: the module scheme logicals and gds(iu) select the deferred
+! correction; su, sv, sw (coef) gain the face terms; bp, bt, bb (coefb) are
+! the east cell's scratch and zero on return.
+subroutine fluxuvw(idew, idns, idtb, arvx, arvy, arvz, fif, acfe, acfw, fcf)
+  use types; use parameters; use indexes; use coef; use coefb; use variables
+  use iso_c_binding; use cap_hip
+  implicit none
+  integer, intent(in) :: idew, idns, idtb
+  real(prec), dimension(nxyza) :: arvx, arvy, arvz, fif, acfe, acfw, fcf
+  type(cap_fields) :: f
+  integer(c_int) :: sch
+  call cap_fields_of_modules(f)
+  sch = 0
+  if (lcds) sch = ior(sch, CAP_UVW_CDS)
+  if (lcds4) sch = ior(sch, CAP_UVW_CDS4)
+  if (lluds) sch = ior(sch, CAP_UVW_LUDS)
+  if (lquds) sch = ior(sch, CAP_UVW_QUICK)
+  if (flux_limiter) sch = ior(sch, CAP_UVW_LIMITER)
+  if (lsmart) sch = ior(sch, CAP_UVW_SMART)
+  if (lavl) sch = ior(sch, CAP_UVW_AVL)
+  if (lmuscl) sch = ior(sch, CAP_UVW_MUSCL)
+  if (lumist) sch = ior(sch, CAP_UVW_UMIST)
+  if (lgamma) sch = ior(sch, CAP_UVW_GAMMA)
+  call cap_check(cap_fluxuvw(cap_context(ni, nj, nk), f, cap_addr(vis), int(idew, c_int), &
+                 int(idns, c_int), int(idtb, c_int), cap_addr(arvx), cap_addr(arvy), &
+                 cap_addr(arvz), cap_addr(fif), cap_addr(acfe), cap_addr(acfw), cap_addr(fcf), &
+                 cap_addr(su), cap_addr(sv), cap_addr(sw), cap_addr(bp), cap_addr(bt), &
+                 cap_addr(bb), sch, real(gds(iu), c_double)), 'fluxuvw')
+end subroutine fluxuvw
+
+! `call fluxmc(inp,idew,idns,idtb,fif,fmcor)`: the reference computes one
+! face per call, inside the cell loop of calcp-multiple_correction_SIMPLE.f90:
+! 126-158.  The device computes the face of every cell of k,i,j = 3..n?mm at
+! once: a call for the loop's first cell (lk(3)+li(3)+3) recomputes the
+! direction's values, every other call reads them.  The inputs (nodes,
+! gradients of p', fif) must therefore not change within one such loop, which
+! holds in calcp, where the loop changes only f1..f3 and su.
+subroutine fluxmc(inp, idew, idns, idtb, fif, fmcor)
+  use types; use parameters; use indexes
+  use iso_c_binding; use cap_hip
+  implicit none
+  integer, intent(in) :: inp, idew, idns, idtb
+  real(prec), dimension(nxyza), intent(in) :: fif
+  real(prec), intent(inout) :: fmcor
+  real(c_double), allocatable, target, save :: mc(:, :)
+  type(cap_fields) :: f
+  integer :: d
+  if (idew == nj) then
+    d = 1
+  else if (idew == 1) then
+    d = 2
+  else
+    d = 3
+  end if
+  if (.not. allocated(mc)) allocate(mc(nxyza, 3))
+  if (size(mc, 1) /= nxyza) then
+    deallocate(mc)
+    allocate(mc(nxyza, 3))
+  end if
+  if (inp == lk(3) + li(3) + 3) then
+    call cap_fields_of_modules(f)
+    call cap_check(cap_fluxmc(cap_context(ni, nj, nk), f, int(idew, c_int), int(idns, c_int), &
+                              int(idtb, c_int), cap_caddr(fif), c_loc(mc(1, d))), 'fluxmc')
+  end if
+  fmcor = mc(inp, d)
+end subroutine fluxmc
+
+! `call update_values_at_ghostcells` (update_values_at_ghostcells_nonperiodic.f90:
+! 3-642): ghost values of u, v, w, te, ed, t, vis by the boundary types of
+! module bc and the inlet values of module inlet.
+subroutine update_values_at_ghostcells
+  use types; use parameters; use indexes; use geometry; use variables; use bc; use inlet
+  use iso_c_binding; use cap_hip
+  implicit none
+  type(c_ptr) :: var(7), g(12)
+  type(cap_bc) :: b
+  var = [cap_addr(u), cap_addr(v), cap_addr(w), cap_addr(te), cap_addr(ed), cap_addr(t), &
+         cap_addr(vis)]
+  g = [cap_addr(fx), cap_addr(fy), cap_addr(fz), cap_addr(ar1x), cap_addr(ar1y), cap_addr(ar1z), &
+       cap_addr(ar2x), cap_addr(ar2y), cap_addr(ar2z), cap_addr(ar3x), cap_addr(ar3y), &
+       cap_addr(ar3z)]
+  b%lbw = cap_iaddr(lbw); b%lbe = cap_iaddr(lbe); b%lbs = cap_iaddr(lbs)
+  b%lbn = cap_iaddr(lbn); b%lbb = cap_iaddr(lbb); b%lbt = cap_iaddr(lbt)
+  b%uin = uin; b%vin = vin; b%win = win; b%tein = tein; b%edin = edin; b%tin = tin
+  call cap_check(cap_update_ghostcells(cap_context(ni, nj, nk), var, g, b), &
+                 'update_values_at_ghostcells')
+end subroutine update_values_at_ghostcells

@@ -139,6 +139,22 @@ module cap_hip
       integer(c_int), value :: nsw
       type(cap_report), intent(out) :: rep
     end function
+    integer(c_int) function cap_cgstab(ctx, fi, c, sor, nsw, rep) bind(c, name='cap_cgstab')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, fi
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: sor
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+    end function
+    integer(c_int) function cap_pcg_jacobi(ctx, fi, c, sor, nsw, rep) bind(c, name='cap_pcg_jacobi')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, fi
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: sor
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+    end function
     integer(c_int) function cap_iccg(ctx, fi, c, sor, nsw, rep) bind(c, name='cap_iccg')
       import :: c_ptr, c_int, c_double, cap_coef, cap_report
       type(c_ptr), value :: ctx, fi
@@ -432,5 +448,11 @@ contains
     type(c_ptr) :: p
     p = c_loc(x)
   end function cap_caddr
+
+  function cap_iaddr(x) result(p)
+    integer(c_int), target, intent(in) :: x(*)
+    type(c_ptr) :: p
+    p = c_loc(x)
+  end function cap_iaddr
 
 end module cap_hip

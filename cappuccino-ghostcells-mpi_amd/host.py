@@ -41,7 +41,8 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_unet_gradients", "cap_unet_train_step", "cap_unet_guess", "cap_pcgsip_guess",
            "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya",
            "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les", "cap_courant_noalloc",
-           "cap_face_areas", "cap_intfac", "cap_grid_dims", "cap_read_grid")
+           "cap_face_areas", "cap_intfac", "cap_grid_dims", "cap_read_grid", "cap_cgstab",
+           "cap_pcg_jacobi")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -133,6 +134,8 @@ def load():
     for fn in ("cap_sipsol", "cap_pcgsip", "cap_cgstab_sip"):
         getattr(L, fn).argtypes = [vp, vp, C.POINTER(_Coef), d, d, i, C.POINTER(_Report)]
     L.cap_iccg.argtypes = [vp, vp, C.POINTER(_Coef), d, i, C.POINTER(_Report)]
+    L.cap_cgstab.argtypes = [vp, vp, C.POINTER(_Coef), d, i, C.POINTER(_Report)]
+    L.cap_pcg_jacobi.argtypes = [vp, vp, C.POINTER(_Coef), d, i, C.POINTER(_Report)]
     L.cap_pcgsip_mixed.argtypes = [vp, vp, C.POINTER(_Coef), d, d, i, d, C.POINTER(_Report)]
     L.cap_grad_gauss.argtypes = [vp, vp, C.POINTER(vp), vp, vp, vp]
     L.cap_bpres.argtypes = [vp, vp, vp, vp, vp]
@@ -309,6 +312,14 @@ class Context:
 
     def cgstab_sip(self, fi, coef, alfa=0.92, sor=1e-8, nsw=1000):
         return self._solve(self._lib.cap_cgstab_sip, fi, coef, alfa, sor, nsw)
+
+    def cgstab(self, fi, coef, sor=1e-8, nsw=1000):
+        """cgstab.f90: BiCGStab with the diagonal ILU(0) preconditioner"""
+        return self._solve(self._lib.cap_cgstab, fi, coef, sor, nsw)
+
+    def pcg_jacobi(self, fi, coef, sor=1e-8, nsw=1000):
+        """pcg-jacobi.f90 (Diagonally_Preconditioned_CG)"""
+        return self._solve(self._lib.cap_pcg_jacobi, fi, coef, sor, nsw)
 
     def pcgsip_mixed(self, fi, coef, alfa=0.92, sor=1e-8, nsw=5000, inner_rtol=1e-3):
         return self._solve(self._lib.cap_pcgsip_mixed, fi, coef, alfa, sor, nsw, inner_rtol)

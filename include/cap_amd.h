@@ -89,6 +89,20 @@ int cap_iccg(cap_ctx *ctx, double *fi, const cap_coef *c, double sor, int nsw,
 int cap_cgstab_sip(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
                    double sor, int nsw, cap_report *rep);
 
+/* Replaces `call cgstab(fi, ifi)` (cgstab.f90:1-284): BiCGStab with the
+ * diagonal ILU(0) preconditioner (d of cgstab.f90:58-66), k = 2..nkm,
+ * i,j = 3..n?mm; its second backward sweep covers k = 3..nkmm only, as
+ * written (cgstab.f90:212).  Reached from calcp with isol = 2
+ * (calcp-multiple_correction_SIMPLE.f90:99).  bb..bt are not used. */
+int cap_cgstab(cap_ctx *ctx, double *fi, const cap_coef *c, double sor, int nsw,
+               cap_report *rep);
+
+/* Replaces `call Diagonally_Preconditioned_CG(fi, ifi)` (pcg-jacobi.f90:
+ * 3-155): CG with the Jacobi preconditioner z = r / (ap + small),
+ * k,i,j = 3..n?mm; res is zero outside the range on return. */
+int cap_pcg_jacobi(cap_ctx *ctx, double *fi, const cap_coef *c, double sor, int nsw,
+                   cap_report *rep);
+
 /* No counterpart in the reference (its solvers are FP64 only).  Solves the
  * pcgsip system (same arrays, same range, same L1 stop test against the FP64
  * residual of the starting fi) by FP64 iterative refinement around an FP32
@@ -172,8 +186,8 @@ typedef struct {
 } cap_flow;
 
 /* Settings the drivers read from modules parameters, boundc and time_mod.
- * solver is calcp's isol: 1 sipsol, 3 iccg, 4 pcgsip, 5 cgstab_sip (the
- * reference's setting); PISO and PIMPLE use pcgsip.  solver 6 (no
+ * solver is calcp's isol: 1 sipsol, 2 cgstab, 3 iccg, 4 pcgsip, 5 cgstab_sip
+ * (the reference's setting); PISO and PIMPLE use pcgsip.  solver 6 (no
  * counterpart in the reference) selects cap_pcgsip_mixed (inner_rtol 3e-4,
  * at most 2 nsw FP32 inner iterations) in all three drivers.  lbw..lbt are
  * device arrays indexed as in cap_bc (outbc reads them when lout is set). */

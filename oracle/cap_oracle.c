@@ -1618,6 +1618,7 @@ static void drv_solve(int ni, int nj, int nk, orc_flow *F, double *x, int solver
     orc_sys s = drv_sys(F);
     switch (solver) {
     case 1: orc_sipsol(ni, nj, nk, x, &s, alfa, sor, nsw, rep); break;
+    case 2: orc_cgstab(ni, nj, nk, x, &s, sor, nsw, rep); break;
     case 3: orc_iccg(ni, nj, nk, x, &s, sor, nsw, rep); break;
     case 4: orc_pcgsip(ni, nj, nk, x, &s, alfa, sor, nsw, rep); break;
     default: orc_cgstab_sip(ni, nj, nk, x, &s, alfa, sor, nsw, rep); break;
@@ -2080,4 +2081,195 @@ void orc_intfac(int ni, int nj, int nk, int idew, int idns, int idtb, const doub
                 const double bx = xc[e] - xc[p], by = yc[e] - yc[p], bz = zc[e] - zc[p] + small;
                 lambda[p] = sqrt(ax * ax + ay * ay + az * az) / (sqrt(bx * bx + by * by + bz * bz) + small);
             }
+}
+
+/* -------------------------------------------------- cgstab, pcg-jacobi -- */
+/*
+ * BiCGStab with the diagonal ILU(0) preconditioner (cgstab.f90:1-284):
+ * d = 1/(ap - aw d(i-1) ae(i-1) - as d(j-1) an(j-1) - ab d(k-1) at(k-1)) on
+ * k = 2..nkm (78-86); each application z = (r + aw z(i-1) + as z(j-1) +
+ * ab z(k-1)) d, z /= (d + small), z = (z + ae z(i+1) + an z(j+1) +
+ * at z(k+1)) d.  The second application's backward sweep covers only
+ * k = 3..nkmm (212), so the planes k = 2 and nkm keep the scaled forward
+ * values there.  The local arrays are zero outside the range (the reference
+ * leaves them uninitialised); res keeps the caller's values outside it.
+ */
+void orc_cgstab(int ni, int nj, int nk, double *fi, orc_sys *s, double sor, int nsw,
+                orc_report *rep)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkm = nk - 1, nkmm = nk - 2;
+    const size_t n = (size_t)nij * nk;
+    double *reso = calloc(n, sizeof(double)), *pk = calloc(n, sizeof(double)),
+           *uk = calloc(n, sizeof(double)), *zk = calloc(n, sizeof(double)),
+           *vk = calloc(n, sizeof(double)), *d = calloc(n, sizeof(double));
+    double *res = s->res;
+#define LOOP_F for (int k = 2; k <= nkm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j)
+    double res0 = 0.0;
+    psum res0_ps = PS0;
+    LOOP_F {                                                   /* 42-53 */
+        size_t c = IX(i, j, k);
+        res[c] = resid(s, fi, c, nj, nij);
+        res0 = ps_add(&res0_ps, fabs(res[c]), k);
+    }
+    LOOP_F {                                                   /* 58-66 */
+        size_t c = IX(i, j, k);
+        d[c] = 1. / (s->ap[c] - s->aw[c] * d[c - nj] * s->ae[c - nj] -
+                     s->as[c] * d[c - 1] * s->an[c - 1] -
+                     s->ab[c] * d[c - nij] * s->at[c - nij]);
+    }
+    LOOP_F {                                                   /* 71-83 */
+        size_t c = IX(i, j, k);
+        reso[c] = res[c];
+    }
+    double alf = 1.0, beto = 1.0, gam = 1.0, resl = 0.0, resor = 0.0;
+    int l, conv = 0;
+    /* z = M^-1 r with the backward sweep on k = kb_hi..kb_lo (downwards) */
+#define ILU_APPLY(rhs, kb_hi, kb_lo)                                                         \
+    do {                                                                                     \
+        LOOP_F {                                                                             \
+            size_t c = IX(i, j, k);                                                          \
+            zk[c] = (rhs[c] + s->aw[c] * zk[c - nj] + s->as[c] * zk[c - 1] +                 \
+                     s->ab[c] * zk[c - nij]) * d[c];                                         \
+        }                                                                                    \
+        LOOP_F {                                                                             \
+            size_t c = IX(i, j, k);                                                          \
+            zk[c] = zk[c] / (d[c] + SMALL);                                                  \
+        }                                                                                    \
+        for (int k = (kb_hi); k >= (kb_lo); --k)                                             \
+            for (int i = nimm; i >= 3; --i)                                                  \
+                for (int j = njmm; j >= 3; --j) {                                            \
+                    size_t c = IX(i, j, k);                                                  \
+                    zk[c] = (zk[c] + s->ae[c] * zk[c + nj] + s->an[c] * zk[c + 1] +          \
+                             s->at[c] * zk[c + nij]) * d[c];                                 \
+                }                                                                            \
+    } while (0)
+#define APPLY_A(out)                                                                         \
+    LOOP_F {                                                                                 \
+        size_t c = IX(i, j, k);                                                              \
+        out[c] = s->ap[c] * zk[c] - s->ae[c] * zk[c + nj] - s->aw[c] * zk[c - nj] -          \
+                 s->an[c] * zk[c + 1] - s->as[c] * zk[c - 1] - s->at[c] * zk[c + nij] -      \
+                 s->ab[c] * zk[c - nij];                                                     \
+    }
+    for (l = 1; l <= nsw; ++l) {
+        double bet = 0.0;
+        psum bet_ps = PS0;                                      /* 94-104 */
+        LOOP_F { size_t c = IX(i, j, k); bet = ps_add(&bet_ps, res[c] * reso[c], k); }
+        double om = bet * gam / (alf * beto + SMALL);
+        beto = bet;
+        LOOP_F {                                               /* 108-115 */
+            size_t c = IX(i, j, k);
+            pk[c] = res[c] + om * (pk[c] - alf * uk[c]);
+        }
+        ILU_APPLY(pk, nkm, 2);                                 /* 119-148 */
+        APPLY_A(uk);                                           /* 152-162 */
+        double ukreso = 0.0;
+        psum ukreso_ps = PS0;                                   /* 166-175 */
+        LOOP_F { size_t c = IX(i, j, k); ukreso = ps_add(&ukreso_ps, uk[c] * reso[c], k); }
+        gam = bet / ukreso;
+        LOOP_F {                                               /* 179-187 */
+            size_t c = IX(i, j, k);
+            fi[c] = fi[c] + gam * zk[c];
+            res[c] = res[c] - gam * uk[c];
+        }
+        ILU_APPLY(res, nkmm, 3);                               /* 191-220: the backward sweep of 212 is k = nkmm..3 */
+        APPLY_A(vk);                                           /* 224-234 */
+        double vres = 0.0, vv = 0.0;
+        psum vres_ps = PS0, vv_ps = PS0;                       /* 238-250 */
+        LOOP_F {
+            size_t c = IX(i, j, k);
+            vres = ps_add(&vres_ps, vk[c] * res[c], k);
+            vv = ps_add(&vv_ps, vk[c] * vk[c], k);
+        }
+        alf = vres / (vv + SMALL);
+        resl = 0.0;
+        psum resl_ps = PS0;                                    /* 254-264 */
+        LOOP_F {
+            size_t c = IX(i, j, k);
+            fi[c] = fi[c] + alf * zk[c];
+            res[c] = res[c] - alf * vk[c];
+            resl = ps_add(&resl_ps, fabs(res[c]), k);
+        }
+        if (l == 1) resor = res0;                              /* 268 */
+        double rsm = resl / (resor + SMALL);
+        if (rsm < sor) { conv = 1; break; }                    /* 271 */
+    }
+#undef ILU_APPLY
+#undef APPLY_A
+#undef LOOP_F
+    rep->resor = res0;
+    rep->resl = resl;
+    rep->iters = l;
+    rep->converged = conv;
+    free(reso); free(pk); free(uk); free(zk); free(vk); free(d);
+}
+
+/*
+ * CG with the Jacobi (diagonal) preconditioner (pcg-jacobi.f90:3-155,
+ * subroutine Diagonally_Preconditioned_CG): range k, i, j = 3..n?mm;
+ * pk, zk and the whole of res start at zero (33-37); z = r / (ap + small)
+ * (divided, 66); s0 = 1e20 as a REAL*4 literal.
+ */
+void orc_pcg_jacobi(int ni, int nj, int nk, double *fi, orc_sys *s, double sor, int nsw,
+                    orc_report *rep)
+{
+    const int nij = ni * nj;
+    const int nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const size_t n = (size_t)nij * nk;
+    double *pk = calloc(n, sizeof(double)), *zk = calloc(n, sizeof(double));
+    double *res = s->res;
+    memset(res, 0, n * sizeof(double));                        /* 41-48 */
+#define LOOP_F for (int k = 3; k <= nkmm; ++k) for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j)
+    double res0 = 0.0;
+    psum res0_ps = PS0;
+    LOOP_F {                                                   /* 52-64 */
+        size_t c = IX(i, j, k);
+        res[c] = resid(s, fi, c, nj, nij);
+        res0 = ps_add(&res0_ps, fabs(res[c]), k);
+    }
+    double s0 = S0INIT, resl = 0.0, resor = 0.0;
+    int l, conv = 0;
+    for (l = 1; l <= nsw; ++l) {
+        double sk = 0.0;
+        psum sk_ps = PS0;
+        LOOP_F {                                               /* 77-88 */
+            size_t c = IX(i, j, k);
+            zk[c] = res[c] / (s->ap[c] + SMALL);
+            sk = ps_add(&sk_ps, res[c] * zk[c], k);
+        }
+        double bet = sk / s0;                                  /* 92 */
+        LOOP_F {                                               /* 96-103 */
+            size_t c = IX(i, j, k);
+            pk[c] = zk[c] + bet * pk[c];
+        }
+        double pkapk = 0.0;
+        psum pkapk_ps = PS0;
+        LOOP_F {                                               /* 107-118 */
+            size_t c = IX(i, j, k);
+            zk[c] = s->ap[c] * pk[c] - s->ae[c] * pk[c + nj] - s->aw[c] * pk[c - nj] -
+                    s->an[c] * pk[c + 1] - s->as[c] * pk[c - 1] - s->at[c] * pk[c + nij] -
+                    s->ab[c] * pk[c - nij];
+            pkapk = ps_add(&pkapk_ps, pk[c] * zk[c], k);
+        }
+        double alf = sk / pkapk;                               /* 120 */
+        resl = 0.0;
+        psum resl_ps = PS0;
+        LOOP_F {                                               /* 124-134 */
+            size_t c = IX(i, j, k);
+            fi[c] = fi[c] + alf * pk[c];
+            res[c] = res[c] - alf * zk[c];
+            resl = ps_add(&resl_ps, fabs(res[c]), k);
+        }
+        s0 = sk;                                               /* 136 */
+        if (l == 1) resor = res0;                              /* 140 */
+        double rsm = resl / (resor + SMALL);
+        if (rsm < sor) { conv = 1; break; }                    /* 143 */
+    }
+#undef LOOP_F
+    rep->resor = res0;
+    rep->resl = resl;
+    rep->iters = l;
+    rep->converged = conv;
+    free(pk);
+    free(zk);
 }

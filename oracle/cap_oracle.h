@@ -245,6 +245,12 @@ void orc_find_intersection_point(const double *p1, const double *p2, const doubl
 void orc_intfac(int ni, int nj, int nk, int idew, int idns, int idtb, const double *x,
                 const double *y, const double *z, const double *xc, const double *yc,
                 const double *zc, double *lambda);
+/* cgstab.f90 (BiCGStab, diagonal ILU(0); k = 2..nkm) and pcg-jacobi.f90
+ * (Diagonally_Preconditioned_CG; k = 3..nkmm) */
+void orc_cgstab(int ni, int nj, int nk, double *fi, orc_sys *s, double sor, int nsw,
+                orc_report *rep);
+void orc_pcg_jacobi(int ni, int nj, int nk, double *fi, orc_sys *s, double sor, int nsw,
+                    orc_report *rep);
 #ifdef __cplusplus
 }
 #endif

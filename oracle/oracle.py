@@ -107,6 +107,8 @@ def lib():
             getattr(L, fn).argtypes = [i, i, i, dp, C.POINTER(_Sys), d, d, i,
                                        C.POINTER(_Report)]
         L.orc_iccg.argtypes = [i, i, i, dp, C.POINTER(_Sys), d, i, C.POINTER(_Report)]
+        L.orc_cgstab.argtypes = [i, i, i, dp, C.POINTER(_Sys), d, i, C.POINTER(_Report)]
+        L.orc_pcg_jacobi.argtypes = [i, i, i, dp, C.POINTER(_Sys), d, i, C.POINTER(_Report)]
         L.orc_sip_factor.argtypes = [i, i, i, i, i, C.POINTER(_Sys), d]
         L.orc_sip_apply.argtypes = [i, i, i, i, i, C.POINTER(_Sys), dp, dp]
         L.orc_spmv.argtypes = [i, i, i, i, i, C.POINTER(_Sys), dp, dp]
@@ -202,6 +204,16 @@ def cgstab_sip(s, fi, alfa=ALFA, sor=1e-8, nsw=1000):
 
 def iccg(s, fi, sor=1e-8, nsw=1000):
     return _solve(lib().orc_iccg, s, fi, sor, nsw)
+
+
+def cgstab(s, fi, sor=1e-8, nsw=1000):
+    """cgstab.f90: BiCGStab with the diagonal ILU(0) preconditioner"""
+    return _solve(lib().orc_cgstab, s, fi, sor, nsw)
+
+
+def pcg_jacobi(s, fi, sor=1e-8, nsw=1000):
+    """pcg-jacobi.f90: CG with the Jacobi preconditioner"""
+    return _solve(lib().orc_pcg_jacobi, s, fi, sor, nsw)
 
 
 class plane_sums:
@@ -484,7 +496,7 @@ def run_ref(mode, dims, arrays, nsw=0, alfa=ALFA, sor=1e-8, driver=REF_DRIVER):
         log = open(flog).read()
     if mode in ("continuity", "fip"):
         return [raw.copy()], None, log
-    nout = {"sipsol": 9, "cgstab_sip": 9, "pcgsip": 9, "iccg": 9, "grad_gauss": 3,
+    nout = {"sipsol": 9, "cgstab_sip": 9, "pcgsip": 9, "iccg": 9, "cgstab": 9, "pcgjac": 9, "grad_gauss": 3,
             "bpres": 1, "fluxmass": 5, "piso_assemble": 5, "assemble_p": 5, "fluxmc": 5,
             "fluxuvw": 8, "ghost": 7, "outbc": 15, "cbc": 5, "les": 1, "intfac": 3}[mode]
     outs = [raw[i * n:(i + 1) * n].copy() for i in range(nout)]

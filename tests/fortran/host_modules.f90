@@ -8,7 +8,7 @@
 ! provides them in modules_allocatable.f90 (types:1-13, parameters:15-109,
 ! indexes:111-132, geometry:147-159, coef:161-170, coefb:172-180,
 ! variables:274-298, title_mod:312-322, gradients:389-398, bc:208-244,
-! inlet values in boundc:246-256) and setind.f90:3-44; this file restates
+! inlet:380-387) and setind.f90:3-44; this file restates
 ! only the entities the drop-ins and the test driver use, so that the test
 ! host is built from this repository's sources alone.
 !-----------------------------------------------------------------------
@@ -33,6 +33,10 @@ module parameters
   integer :: icorr, ipcorr
   integer, dimension(nphi) :: nsw
   real(prec), dimension(nphi) :: sor, resor
+  ! convection schemes and deferred-correction blending of fluxuvw
+  logical :: lcds, lluds, lquds, lsmart, lavl, lmuscl, lumist, lgamma, lcds4
+  logical :: flux_limiter
+  real(dp), dimension(nphi) :: gds
 end module parameters
 
 module indexes
@@ -105,11 +109,11 @@ module bc
   integer, dimension(:), allocatable :: lbb, lbt
 end module bc
 
-module boundc
+module inlet
   use types
   implicit none
   real(prec) :: uin, vin, win, tein, edin, tin
-end module boundc
+end module inlet
 
 ! Index helpers of the structured grid: the derived extents and the plane
 ! and row offsets li(i) = (i-1) nj, lk(k) = (k-1) ni nj of the padded

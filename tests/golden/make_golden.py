@@ -116,6 +116,10 @@ SOLVER_CASES = [
     ("cgstab_general_12x10x9", "cgstab_sip", "general", (12, 10, 9), 104, 400, 0.92, True),
     ("cgstab_lognormal_20", "cgstab_sip", "lognormal", (20, 20, 20), 105, 400, 0.92, True),
     ("cgstab_lognormal_36", "cgstab_sip", "lognormal", (36, 36, 36), 106, 400, 0.92, False),
+    ("bicgilu_general_12x10x9", "cgstab", "general", (12, 10, 9), 107, 15, 0.0, True),
+    ("bicgilu_lognormal_20", "cgstab", "lognormal", (20, 20, 20), 108, 400, 0.0, True),
+    ("pcgjac_general_12x10x9", "pcgjac", "general", (12, 10, 9), 109, 40, 0.0, True),
+    ("pcgjac_lognormal_20", "pcgjac", "lognormal", (20, 20, 20), 110, 600, 0.0, True),
 ]
 ASSEMBLY_CASES = [
     ("fluxmass", (12, 10, 9), 301), ("fluxmass", (20, 17, 11), 302),

@@ -98,14 +98,18 @@ def check_reports(ro, rg, what):
 
 
 @pytest.mark.parametrize("solver,alfa,ncorr,npcor", [(4, 0.0, 2, 2), (1, 0.92, 1, 1),
-                                                     (3, 0.0, 1, 2), (5, 0.92, 2, 1)])
+                                                     (3, 0.0, 1, 2), (5, 0.92, 2, 1),
+                                                     (2, 0.0, 2, 1)])
 def test_calcp_bitwise(gpu, O, solver, alfa, ncorr, npcor):
     what = f"calcp solver {solver} ncorr {ncorr} npcor {npcor}"
     print(what, flush=True)
     fo, fg, ro, rg = run_pair(gpu, O, DIMS, 11, "calcp", solver=solver, alfa=alfa, ncorr=ncorr,
                               npcor=npcor)
     print("  iterations", [r["iters"] for r in rg[0]], "continuity", rg[1][-1], flush=True)
-    assert all(r["converged"] for r in rg[0])
+    # cgstab (isol 2) diverges on this state, in the oracle (and the
+    # reference) as on the GPU: its ghost-plane rows (ap = 1e3, k = 2 and nkm)
+    # are in its range; the check is that both do the same
+    assert all(r["converged"] for r in rg[0]) or solver == 2
     check_reports(ro, rg, what)
     check_state(fo, fg, what)
 

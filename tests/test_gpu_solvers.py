@@ -52,17 +52,19 @@ def rel(a, b):
 
 
 def check_solve(name, rep, rep_ref, fi, fi_ref, coef, s_ref, bitwise_fi=False, sens=0.0,
-                acc=None):
+                acc=None, spread=0):
     """GPU result vs the oracle in the reference's summation (rep_ref, fi_ref,
     s_ref) and, for the Krylov solvers, vs the oracle with compensated sums
-    (acc = (report, system, fi)), which must be bit-identical."""
+    (acc = (report, system, fi)), which must be bit-identical.  spread: how
+    many iterations the oracle itself moves between summation orders (the
+    iteration count may differ from the reference order's by that much)."""
     tol = max(1e-10, 4.0 * sens)
     print(f"{name}: gpu iters {rep['iters']} cpu iters {rep_ref['iters']} "
           f"fi rel {rel(fi, fi_ref):.2e} (tol {tol:.1e}, cpu order sensitivity {sens:.1e}) "
           f"resor {rep['resor']:.6e}/{rep_ref['resor']:.6e}"
           + ("" if acc is None else
              f"; vs compensated-sum oracle: iters {acc[0]['iters']} fi rel {rel(fi, acc[2]):.2e}"))
-    assert abs(rep["iters"] - rep_ref["iters"]) <= 1
+    assert abs(rep["iters"] - rep_ref["iters"]) <= max(1, spread)
     assert rep["resor"] == pytest.approx(rep_ref["resor"], rel=1e-12)
     if rep["iters"] == rep_ref["iters"]:
         if bitwise_fi:
@@ -70,7 +72,7 @@ def check_solve(name, rep, rep_ref, fi, fi_ref, coef, s_ref, bitwise_fi=False, s
         else:
             assert rel(fi, fi_ref) <= tol
     else:
-        assert rel(fi, fi_ref) <= 1e-6
+        assert rel(fi, fi_ref) <= max(1e-6, 4.0 * sens)
     for f in B_FIELDS:
         if getattr(coef, f) is not None:
             np.testing.assert_array_equal(getattr(coef, f), getattr(s_ref, f), err_msg=f)
@@ -180,6 +182,61 @@ def test_cgstab_sip(gpu, O, dims, kind):
     sens = O.order_sensitivity(O.cgstab_sip, s, fi0, sor=1e-8, nsw=400)
     acc = O.accurate_sums(O.cgstab_sip, s, fi0, sor=1e-8, nsw=400)
     check_solve("cgstab_sip/" + kind, rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens, acc=acc)
+
+
+@pytest.mark.parametrize("dims", SMALL_DIMS)
+@pytest.mark.parametrize("kind", ["general", "lognormal"])
+def test_cgstab_ilu(gpu, O, dims, kind):
+    """cgstab.f90 (BiCGStab, diagonal ILU(0), second backward sweep on
+    k = 3..nkmm only); the general systems give the range's end planes rows
+    of their own, where the two backward sweeps differ"""
+    if kind == "general":
+        s, fi0 = general_system(O, dims, 17)
+        nsw = 12
+    else:
+        s, fi0 = lognormal_system(O, dims, 17)
+        nsw = 400
+    print(f"cgstab {kind} {dims}", flush=True)
+    s_ref, fi_ref = s.copy(), fi0.copy()
+    rep_ref = O.cgstab(s_ref, fi_ref, sor=1e-8, nsw=nsw)
+    coef, fi = coef_of(gpu, s), fi0.copy()
+    with gpu.Context(*dims) as ctx:
+        rep = ctx.cgstab(fi, coef, sor=1e-8, nsw=nsw)
+    if kind == "lognormal":
+        assert rep["converged"]
+    sens = O.order_sensitivity(O.cgstab, s, fi0, sor=1e-8, nsw=nsw)
+    acc = O.accurate_sums(O.cgstab, s, fi0, sor=1e-8, nsw=nsw)
+    # BiCGStab with this preconditioner moves by several iterations between
+    # summation orders on the CPU alone (e.g. 141 / 147 at 52 x 70 x 30)
+    with O.plane_sums(1):
+        it_plane = O.cgstab(s.copy(), fi0.copy(), sor=1e-8, nsw=nsw)["iters"]
+    spread = max(abs(it_plane - rep_ref["iters"]), abs(acc[0]["iters"] - rep_ref["iters"]))
+    check_solve("cgstab/" + kind, rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens, acc=acc,
+                spread=spread)
+
+
+@pytest.mark.parametrize("dims", SMALL_DIMS)
+@pytest.mark.parametrize("kind", ["general", "lognormal"])
+def test_pcg_jacobi(gpu, O, dims, kind):
+    """pcg-jacobi.f90 (Diagonally_Preconditioned_CG): res comes back zero
+    outside the range, as the reference leaves it"""
+    if kind == "general":
+        s, fi0 = general_system(O, dims, 19)
+        nsw = 30
+    else:
+        s, fi0 = lognormal_system(O, dims, 19)
+        nsw = 6000
+    print(f"pcg_jacobi {kind} {dims}", flush=True)
+    s_ref, fi_ref = s.copy(), fi0.copy()
+    rep_ref = O.pcg_jacobi(s_ref, fi_ref, sor=1e-8, nsw=nsw)
+    coef, fi = coef_of(gpu, s), fi0.copy()
+    with gpu.Context(*dims) as ctx:
+        rep = ctx.pcg_jacobi(fi, coef, sor=1e-8, nsw=nsw)
+    if kind == "lognormal":
+        assert rep["converged"]
+    sens = O.order_sensitivity(O.pcg_jacobi, s, fi0, sor=1e-8, nsw=nsw)
+    acc = O.accurate_sums(O.pcg_jacobi, s, fi0, sor=1e-8, nsw=nsw)
+    check_solve("pcg_jacobi/" + kind, rep, rep_ref, fi, fi_ref, coef, s_ref, sens=sens, acc=acc)
 
 
 def test_device_pointers(gpu, O):

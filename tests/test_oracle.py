@@ -28,7 +28,15 @@ import make_golden as MG  # noqa: E402
 
 # --------------------------------------------------------------- golden
 SOLVER_FILES = sorted(glob.glob(os.path.join(GOLD, "sipsol_*.npz")) +
-                      glob.glob(os.path.join(GOLD, "cgstab_*.npz")))
+                      glob.glob(os.path.join(GOLD, "cgstab_*.npz")) +
+                      glob.glob(os.path.join(GOLD, "bicgilu_*.npz")) +
+                      glob.glob(os.path.join(GOLD, "pcgjac_*.npz")))
+
+
+def _solver_fn(O, routine):
+    """(oracle function, keyword arguments) of a reference solver routine"""
+    return {"sipsol": (O.sipsol, ("alfa",)), "cgstab_sip": (O.cgstab_sip, ("alfa",)),
+            "cgstab": (O.cgstab, ()), "pcgjac": (O.pcg_jacobi, ())}[routine]
 
 
 @pytest.mark.parametrize("path", SOLVER_FILES, ids=os.path.basename)
@@ -36,8 +44,9 @@ def test_solver_golden(O, path):
     g = np.load(path)
     dims = tuple(int(x) for x in g["dims"])
     s, fi = MG.solver_inputs(str(g["kind"]), dims, int(g["seed"]))
-    fn = O.sipsol if str(g["routine"]) == "sipsol" else O.cgstab_sip
-    rep = fn(s, fi, alfa=float(g["alfa"]), sor=1e-8, nsw=int(g["nsw"]))
+    fn, keys = _solver_fn(O, str(g["routine"]))
+    kw = {"alfa": float(g["alfa"])} if keys else {}
+    rep = fn(s, fi, sor=1e-8, nsw=int(g["nsw"]), **kw)
     assert rep["resor"] == float(g["resor"])
     assert rep["iters"] == int(g["iters"])
     got = dict(fi=fi, res=s.res, **{f: getattr(s, f) for f in B_FIELDS})
@@ -71,7 +80,9 @@ def test_bpres_golden(O):
                                                     "ref_driver")),
                     reason="reference build only exists where the reference sources are")
 @pytest.mark.parametrize("routine,kind,seed", [("sipsol", "general", 7), ("cgstab_sip", "general", 8),
-                                               ("cgstab_sip", "lognormal", 9)])
+                                               ("cgstab_sip", "lognormal", 9),
+                                               ("cgstab", "lognormal", 10), ("cgstab", "general", 11),
+                                               ("pcgjac", "lognormal", 12), ("pcgjac", "general", 13)])
 def test_against_reference_build_fresh(O, routine, kind, seed):
     """Fresh seeds straight through the reference build (not just the fixtures)."""
     dims = (14, 21, 11)
@@ -79,7 +90,7 @@ def test_against_reference_build_fresh(O, routine, kind, seed):
     arrays = [fi, s.ae, s.aw, s.an, s.as_, s.at, s.ab, s.ap, s.su, s.res] + \
              [getattr(s, f) for f in B_FIELDS]
     outs, resor, log = O.run_ref(routine, dims, [a.copy() for a in arrays], nsw=40)
-    fn = O.sipsol if routine == "sipsol" else O.cgstab_sip
+    fn, _ = _solver_fn(O, routine)
     rep = fn(s, fi, sor=1e-8, nsw=40)
     assert rep["resor"] == resor and rep["iters"] == O.parse_log(log)["iters"]
     for a, b in zip(outs, [fi, s.res] + [getattr(s, f) for f in B_FIELDS]):
