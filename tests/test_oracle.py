@@ -230,6 +230,8 @@ def test_iccg_against_dense_ic(O, dims):
     nit = 6
     fi = np.zeros(s.n)
     rep = O.iccg(s, fi, sor=0.0, nsw=nit)
+    fi_long = np.zeros(s.n)
+    rep_long = O.iccg(ref.copy(), fi_long, sor=1e-13, nsw=500)
     idx = interior_index(dims, 2, dims[2] - 1)
     A = dense_A(ref, dims, idx)
     m = len(idx)
@@ -241,8 +243,13 @@ def test_iccg_against_dense_ic(O, dims):
     Dinv = np.diag(1.0 / d)
     AU = np.triu(A, 1)   # the row's own upper couplings (A is not symmetric at the pinned cell)
     M = (Dinv + AL) @ np.diag(d) @ (Dinv + AU)
-    _, hist = textbook_pcg(A, ref.su[idx], np.zeros(m), lambda r: np.linalg.solve(M, r), nit)
+    x, hist = textbook_pcg(A, ref.su[idx], np.zeros(m), lambda r: np.linalg.solve(M, r), nit)
     assert rep["resl"] == pytest.approx(hist[-1], rel=1e-9)
+    np.testing.assert_allclose(fi[idx], x, rtol=0, atol=1e-10 * np.max(np.abs(x)))
+    # and, solved to 1e-13, the solution of the dense system
+    assert rep_long["converged"]
+    xs = np.linalg.solve(A, ref.su[idx])
+    np.testing.assert_allclose(fi_long[idx], xs, rtol=0, atol=1e-10 * np.max(np.abs(xs)))
 
 
 def test_spmv_dense(O):

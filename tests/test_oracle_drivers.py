@@ -245,3 +245,167 @@ def test_calcuvw_crank_nicolson_halves_the_coefficients(O):
         np.testing.assert_array_equal(fb[k], 0.5 * fa[k], err_msg=k)
     # the Crank-Nicolson time term joins the diagonal source of each equation
     assert np.all(grid(spb, UVW_DIMS)[2:-2, 2:-2, 2:-2] > grid(spa, UVW_DIMS)[2:-2, 2:-2, 2:-2])
+
+
+# ------------------------------------------- calcuvw's sources, closed forms
+def uniform_uvw(O, n=8, nz_int=3, **kw):
+    """A uniform n^3 grid (h = 1/n, outward face areas h^2) with every field
+    constant or zero: zero mass fluxes, pp = 0, rho = 1, mu = 0.02, steady.
+    Each term of calcuvw's sources can then be isolated exactly.  nz_int:
+    the module integer nz that the wall distance is projected on (calcuvw.f90:
+    187, 277 use nx, ny, nz where a normal is meant)."""
+    import configcase as CC
+    import gridcase as GC
+    w = GC.uniform_widths(n)
+    g, dims = CC._geometry(O, w, w, w)
+    f = FC.flow_case(dims, 1)
+    f.update(g)
+    N = dims[0] * dims[1] * dims[2]
+    for q in ("u", "v", "w", "pp", "f1", "f2", "f3", "uo", "vo", "wo", "uoo", "voo", "woo",
+              "su", "sv", "sw", "sp", "ae", "aw", "an", "as", "at", "ab", "ap"):
+        f[q] = np.zeros(N)
+    f["den"] = np.ones(N)
+    f["vis"] = np.full(N, 0.02)
+    f["t"] = np.ones(N)
+    o = FC.opts(O.DrvOpts, alfa=0.92, const_mflux=0, **kw)
+    q = O.UvwOpts()
+    q.schemes, q.gds = 1, 1.0
+    for m in range(3):
+        q.urfr[m], q.urfm[m], q.sor[m], q.nsw[m] = 1 / 0.7, 0.3, 0.2, 1
+    q.nx, q.ny, q.nz = 1, 1, nz_int
+    return f, dims, o, q, 1.0 / n
+
+
+def _run_uvw(O, f, dims, o, q):
+    spv = np.zeros_like(f["u"])
+    O.calcuvw(dims, f, spv, o, q)
+    return spv
+
+
+def test_calcuvw_wall_shear_closed_form(O):
+    """Constant v, w on a uniform grid, no flux, no pressure gradient.  The
+    wall terms of calcuvw.f90:161-337 act on the first cells above the bottom
+    and below the top wall (k = 3, nkmm): sp and spv gain vsol = mu A / dpb
+    with dpb = h/2 (nothing else adds to them in a steady run), and the
+    sources gain vsol phi - cf phi_t with cf = mu A / dnw, dnw = (h/2) nz
+    (the module integer nz, as written).  fluxuvw also acts near the walls
+    here (the ghost cells' partial Gauss sums enter its explicit diffusion),
+    so cf is isolated by running twice with nz = 3 and nz = 5: the tangential
+    v moves by -mu A V (2/h)(1/3 - 1/5), the wall-normal w (phi_t = 0) not at
+    all, and nothing else anywhere."""
+    runs = []
+    for nz_int in (3, 5):
+        f, dims, o, q, h = uniform_uvw(O, nz_int=nz_int)
+        V, W = 0.7, -0.4
+        f["v"][:] = V
+        f["w"][:] = W
+        w_in = f["w"].copy()
+        spv = _run_uvw(O, f, dims, o, q)
+        runs.append((f, spv, w_in))
+    ni, nj, nk = dims
+    mu, A = 0.02, h * h
+    vsol = mu * A / (0.5 * h)
+    (fa, spva, w_in), (fb, spvb, _) = runs
+    I = (slice(2, ni - 2), slice(2, nj - 2))
+    for k in (2, nk - 3):           # 0-based planes k = 3 and nkmm
+        np.testing.assert_allclose(grid(fa["sp"], dims)[k][I], vsol, rtol=1e-13)
+        np.testing.assert_allclose(grid(spva, dims)[k][I], vsol, rtol=1e-13)
+        dsv = grid(fa["sv"], dims)[k][I] - grid(fb["sv"], dims)[k][I]
+        np.testing.assert_allclose(dsv, -mu * A * V * (2.0 / h) * (1.0 / 3 - 1.0 / 5), rtol=1e-9)
+    for k in range(3, nk - 3):
+        assert np.all(grid(fa["sp"], dims)[k][I] == 0.0)
+    np.testing.assert_array_equal(fa["sw"], fb["sw"])
+    wall = np.zeros((nk, ni, nj), bool)
+    wall[2] = wall[nk - 3] = True
+    assert np.all(grid(fa["sv"], dims)[~wall] == grid(fb["sv"], dims)[~wall])
+    # away from every ghost layer, nothing acts on the constant field
+    C = (slice(3, nk - 3), slice(3, ni - 3), slice(3, nj - 3))
+    assert np.all(grid(fa["sv"], dims)[C] == 0.0) and np.all(grid(fa["sw"], dims)[C] == 0.0)
+    # the under-relaxation source of the last (w) equation, calcuvw.f90:458:
+    # su = sw + urfm ap w_old, ap already relaxed by urfr
+    R = (slice(1, nk - 1), slice(2, ni - 2), slice(2, nj - 2))
+    want = grid(fa["sw"], dims) + 0.3 * grid(fa["ap"], dims) * grid(w_in, dims)
+    np.testing.assert_array_equal(grid(fa["su"], dims)[R], want[R])
+
+
+def test_calcuvw_pressure_gradient_closed_form(O):
+    """A linear pressure correction pp = a x + b y + c z, zero velocity: the
+    momentum sources are -grad pp vol (calcuvw.f90:60-62), with the Gauss
+    gradient exact for a linear field on the uniform grid's interior."""
+    f, dims, o, q, h = uniform_uvw(O)
+    a, b, c = 0.3, -1.7, 2.2
+    f["pp"][:] = a * f["xc"] + b * f["yc"] + c * f["zc"]
+    _run_uvw(O, f, dims, o, q)
+    ni, nj, nk = dims
+    I = (slice(2, nk - 2), slice(2, ni - 2), slice(2, nj - 2))
+    vol = h ** 3
+    np.testing.assert_allclose(grid(f["sv"], dims)[I], -b * vol, rtol=1e-12)
+    np.testing.assert_allclose(grid(f["sw"], dims)[I], -c * vol, rtol=1e-12)
+
+
+def test_calcuvw_time_and_buoyancy_closed_forms(O):
+    """BDF2 (calcuvw.f90:86-98): s += rho vol/dt ((1 + btime) v_o - btime/2 v_oo)
+    and ap += rho vol/dt (1 + btime/2); Boussinesq buoyancy (74-80):
+    s -= g beta rho_ref (t - t_ref) vol.  Zero velocity, so no wall term."""
+    dt, bt, beta, tref, gy, gz = 0.05, 1.0, 3e-3, 0.5, -2.0, -9.81
+    f, dims, o, q, h = uniform_uvw(O, bdf=1, btime=bt, timestep=dt, buoy=1, boussinesq=1,
+                                   beta=beta, tref=tref, gravx=0.0, gravy=gy, gravz=gz, densit=1.3)
+    rng = np.random.default_rng(4)
+    N = f["u"].size
+    f["vo"][:] = rng.normal(size=N)
+    f["voo"][:] = rng.normal(size=N)
+    f["wo"][:] = rng.normal(size=N)
+    f["t"][:] = rng.uniform(0, 1, N)
+    vo, voo, wo, t = (f[k].copy() for k in ("vo", "voo", "wo", "t"))
+    spv = _run_uvw(O, f, dims, o, q)
+    ni, nj, nk = dims
+    I = (slice(2, nk - 2), slice(2, ni - 2), slice(2, nj - 2))
+    vol = h ** 3
+    at = vol / dt
+    heat = beta * 1.3 * (grid(t, dims) - tref) * vol
+    want_v = at * ((1 + bt) * grid(vo, dims) - 0.5 * bt * grid(voo, dims)) - gy * heat
+    want_w = at * ((1 + bt) * grid(wo, dims)) - gz * heat
+    np.testing.assert_allclose(grid(f["sv"], dims)[I], want_v[I], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(grid(f["sw"], dims)[I], want_w[I], rtol=1e-12, atol=1e-15)
+    J = (slice(3, nk - 3), slice(2, ni - 2), slice(2, nj - 2))     # no wall cells
+    np.testing.assert_allclose(grid(spv, dims)[J], at * (1 + 0.5 * bt), rtol=1e-14)
+
+
+def test_pimple_pressure_update_closed_form(O):
+    """PIMPLE_multiple_correction.f90 solves for the pressure itself, from
+    p = 0 (line 57): after one corrector p = p + urf (pp - p) = urf pp on the
+    interior (275-283), so pp* = p / urf must solve the assembled system
+    left in ae..ap, su (reference-cell row included: pp*(ref) = p_ref = 0) to
+    the requested residual; the velocities take -apu grad(pp) vol with the
+    gradient the routine leaves in gradp (276-278); and pp = p on return."""
+    f = FC.flow_case(DIMS, 17)
+    ipr = FC.ref_cell(DIMS, 4, 5, 4)
+    urf = 0.4
+    o = FC.opts(O.DrvOpts, ncorr=1, npcor=1, ijkpr=ipr, nsw=2000, sor=1e-10, urf=urf)
+    pp_in, u_in, v_in = f["pp"].copy(), f["u"].copy(), f["v"].copy()
+    reps, _ = O.piso(DIMS, f, o, pimple=True)
+    assert reps[0]["converged"]
+    ni, nj, nk = DIMS
+    m = np.zeros((nk, ni, nj), bool)
+    m[2:-2, 2:-2, 2:-2] = True
+    m = m.ravel()
+    x = pp_in.copy()                         # ghost values as the solve saw them
+    x[m] = f["p"][m] / urf
+    s = O.System(DIMS)
+    for q in ("ae", "aw", "an", "at", "ab", "ap", "su"):
+        getattr(s, q)[:] = f[q]
+    s.as_[:] = f["as"]
+    r = f["su"] - O.spmv(s, 3, nk - 2, x)
+    r0 = f["su"] - O.spmv(s, 3, nk - 2, pp_in)
+    assert np.abs(r[m]).sum() < 1e-8 * np.abs(r0[m]).sum()
+    assert f["p"][ipr - 1] == 0.0
+    np.testing.assert_array_equal(f["pp"], f["p"])
+    gp = f["gradp"].reshape(-1, 3)
+    np.testing.assert_array_equal(f["u"][m], (u_in - f["apu"] * gp[:, 0] * f["vol"])[m])
+    np.testing.assert_array_equal(f["v"][m], (v_in - f["apv"] * gp[:, 1] * f["vol"])[m])
+    # and the update is not p = pp, nor p += urf pp from a non-zero start:
+    # the recovered pp* would not solve the system then
+    bad = pp_in.copy()
+    bad[m] = f["p"][m]
+    rb = f["su"] - O.spmv(s, 3, nk - 2, bad)
+    assert np.abs(rb[m]).sum() > 1e-3 * np.abs(r0[m]).sum()
