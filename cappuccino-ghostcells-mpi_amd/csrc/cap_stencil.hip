@@ -1192,6 +1192,82 @@ extern "C" int cap_update_ghostcells(cap_ctx *ctx, double *const *var, const dou
     return CAP_OK;
 }
 
+// update_values_at_ghostcells.f90:3-148, the periodic variant: one thread
+// per ghost cell of the i planes (2, nim), the j planes (2, njm) on
+// k = 3..nkmm, and of the two wall planes k = 2 and nkm.  Every ghost value
+// is read from interior cells (or, vis at the walls, from the ghost cell
+// itself), so the cells are independent.
+struct PerIO {
+    double *v[6];          // u, v, w, te, ed, vis
+    const double *fz;
+};
+
+__global__ __launch_bounds__(256) void k_ghost_periodic(int ni, int nj, int nk, PerIO g)
+{
+    const int nij = ni * nj, nis = ni - 4, njs = nj - 4, nks = nk - 4;
+    const size_t ni_cnt = 2 * (size_t)njs * nks, nj_cnt = 2 * (size_t)nis * nks;
+    const size_t nk_cnt = 2 * (size_t)nis * njs, total = ni_cnt + nj_cnt + nk_cnt;
+    auto ix = [&](int i, int j, int k) {
+        return (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+    };
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (size_t)gridDim.x * 256) {
+        if (t < ni_cnt) {                       // i planes (39-66)
+            const int side = (int)(t / ((size_t)njs * nks));
+            const size_t r = t % ((size_t)njs * nks);
+            const int j = 3 + (int)(r % njs), k = 3 + (int)(r / njs);
+            const size_t dst = side ? ix(ni - 1, j, k) : ix(2, j, k);
+            const size_t src = side ? ix(3, j, k) : ix(ni - 2, j, k);
+            for (int q = 0; q < 6; ++q) g.v[q][dst] = g.v[q][src];
+        } else if (t < ni_cnt + nj_cnt) {       // j planes (68-94)
+            const size_t u = t - ni_cnt;
+            const int side = (int)(u / ((size_t)nis * nks));
+            const size_t r = u % ((size_t)nis * nks);
+            const int i = 3 + (int)(r % nis), k = 3 + (int)(r / nis);
+            const size_t dst = side ? ix(i, nj - 1, k) : ix(i, 2, k);
+            const size_t src = side ? ix(i, 3, k) : ix(i, nj - 2, k);
+            for (int q = 0; q < 6; ++q) g.v[q][dst] = g.v[q][src];
+        } else {                                // walls at k (96-145)
+            const size_t u = t - ni_cnt - nj_cnt;
+            const int top = (int)(u / ((size_t)nis * njs));
+            const size_t r = u % ((size_t)nis * njs);
+            const int j = 3 + (int)(r % njs), i = 3 + (int)(r / njs);
+            const size_t gc = top ? ix(i, j, nk - 1) : ix(i, j, 2);
+            const size_t p = top ? ix(i, j, nk - 2) : ix(i, j, 3);
+            double fxp, fxe;
+            if (top) { fxe = g.fz[p]; fxp = 1.0 - fxe; }
+            else { fxp = g.fz[gc]; fxe = 1.0 - fxp; }
+            for (int q = 0; q < 4; ++q) g.v[q][gc] = -g.v[q][p] * fxp / fxe;
+            g.v[4][gc] = g.v[4][p];
+            g.v[5][gc] = (g.v[5][gc] - g.v[5][p] * fxp) / fxe;
+        }
+    }
+}
+
+extern "C" int cap_update_ghostcells_periodic(cap_ctx *ctx, double *const *var, const double *fz)
+{
+    if (!ctx || !var || !fz || ctx->ni < 7 || ctx->nj < 7 || ctx->nk < 7) return CAP_ERR_ARG;
+    const int qs[6] = {0, 1, 2, 3, 4, 6};       // u, v, w, te, ed, vis of var[7]; t untouched
+    for (int q = 0; q < 6; ++q) if (!var[qs[q]]) return CAP_ERR_ARG;
+    const int ni = ctx->ni, nj = ctx->nj, nk = ctx->nk;
+    PerIO g;
+    int rc;
+    for (int q = 0; q < 6; ++q) {
+        const double *t;
+        if ((rc = stage(ctx, q, var[qs[q]], &t))) return rc;
+        g.v[q] = (double *)t;
+    }
+    if ((rc = stage(ctx, 6, fz, &g.fz))) return rc;
+    const size_t total = 2 * ((size_t)(nj - 4) * (nk - 4) + (size_t)(ni - 4) * (nk - 4) +
+                              (size_t)(ni - 4) * (nj - 4));
+    hipLaunchKernelGGL(k_ghost_periodic, dim3(blocks_for(total)), dim3(256), 0, ctx->stream, ni, nj,
+                       nk, g);
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < 6; ++q)
+        if ((rc = unstage(ctx, var[qs[q]], g.v[q]))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
 extern "C" int cap_courant(cap_ctx *ctx, const double *f1, const double *f2, const double *f3,
                            const double *vol, double *su, double *timestep, double *time,
                            int conumfix, double conumfixvalue, double *conum)

@@ -268,6 +268,12 @@ module cap_hip
       type(c_ptr), intent(in) :: arrays(19), lb(6)
       integer(c_int), intent(out) :: flags(9)
     end function
+    integer(c_int) function cap_update_ghostcells_periodic(ctx, var, fz) &
+        bind(c, name='cap_update_ghostcells_periodic')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, fz
+      type(c_ptr), intent(in) :: var(7)
+    end function
     integer(c_int) function cap_update_ghostcells(ctx, var, geom, bc) &
         bind(c, name='cap_update_ghostcells')
       import :: c_ptr, c_int, cap_bc

@@ -42,7 +42,7 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya",
            "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les", "cap_courant_noalloc",
            "cap_face_areas", "cap_intfac", "cap_grid_dims", "cap_read_grid", "cap_cgstab",
-           "cap_pcg_jacobi")
+           "cap_pcg_jacobi", "cap_update_ghostcells_periodic")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -150,6 +150,7 @@ def load():
     L.cap_continuity.argtypes = [vp, vp, vp, vp, C.POINTER(d)]
     L.cap_fluxuvw.argtypes = [vp, F, vp, i, i, i] + [vp] * 13 + [i, d]
     L.cap_update_ghostcells.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(_Bc)]
+    L.cap_update_ghostcells_periodic.argtypes = [vp, C.POINTER(vp), vp]
     L.cap_courant.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(d), C.POINTER(d), i, d,
                               C.POINTER(d)]
     L.cap_courant_noalloc.argtypes = L.cap_courant.argtypes
@@ -396,6 +397,14 @@ class Context:
                    for a, n in zip(lb, self._lb_sizes())], *[float(x) for x in inl])
         rc = self._lib.cap_update_ghostcells(self._h, V, G, C.byref(bc))
         self._check(rc, "cap_update_ghostcells", allow=(CAP_OK,))
+
+    def update_ghostcells_periodic(self, var, fz):
+        """var: [u, v, w, te, ed, t, vis] (t untouched); the periodic variant
+        update_values_at_ghostcells.f90 (periodic i and j, walls at k)."""
+        V = (C.c_void_p * 7)(*[self._f(a, q) for a, q in
+                                zip(var, ("u", "v", "w", "te", "ed", "t", "vis"))])
+        rc = self._lib.cap_update_ghostcells_periodic(self._h, V, self._f(fz, "fz"))
+        self._check(rc, "cap_update_ghostcells_periodic", allow=(CAP_OK,))
 
     def courant(self, f1, f2, f3, vol, su, timestep, time, conumfix=False, conumfixvalue=0.5,
                 noalloc=False):

@@ -352,6 +352,15 @@ typedef struct {
 int cap_update_ghostcells(cap_ctx *ctx, double *const *var, const double *const *geom,
                           const cap_bc *bc);
 
+/* Replaces the periodic `call update_values_at_ghostcells`
+ * (update_values_at_ghostcells.f90:3-148, the channel's variant): the ghost
+ * planes i = 2, nim and j = 2, njm take u, v, w, te, ed, vis of the opposite
+ * interior planes (k = 3..nkmm); at the walls k = 2 and nkm, u, v, w, te =
+ * -phi_P f_P / f_E, ed copied, vis = (vis_g - vis_P f_P) / f_E, with the
+ * wall face's interpolation factor fz.  var = {u, v, w, te, ed, t, vis} as
+ * cap_update_ghostcells takes it; t is not touched. */
+int cap_update_ghostcells_periodic(cap_ctx *ctx, double *const *var, const double *fz);
+
 /* Convection schemes of cap_fluxuvw: the reference's module logicals
  * lcds, lcds4, lluds, lquds, flux_limiter and the limiter choice lsmart,
  * lavl, lmuscl, lumist, lgamma (modules_allocatable.f90:58, 90), applied in

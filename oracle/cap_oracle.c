@@ -2273,3 +2273,54 @@ void orc_pcg_jacobi(int ni, int nj, int nk, double *fi, orc_sys *s, double sor, 
     free(pk);
     free(zk);
 }
+
+/* ------------------------------------------------ periodic ghost cells -- */
+/*
+ * update_values_at_ghostcells.f90:3-148 (the channel's periodic variant):
+ * ghost planes i = 2 / nim take the values of i = nimm / 3 (39-66), j = 2 /
+ * njm those of j = njmm / 3 (68-94), both on k = 3..nkmm, for u, v, w, te,
+ * ed, vis; walls at k = 2 and nkm (96-145): u, v, w, te = -phi_P f_P / f_E,
+ * ed copied, vis = (vis_g - vis_P f_P) / f_E with the face's interpolation
+ * factor fz.  var = {u, v, w, te, ed, t, vis}; t is not touched.
+ */
+void orc_ghostcells_periodic(int ni, int nj, int nk, double *const *var, const double *fz)
+{
+    const int nij = ni * nj, nim = ni - 1, njm = nj - 1, nimm = ni - 2, njmm = nj - 2;
+    const int nkm = nk - 1, nkmm = nk - 2;
+    double *u = var[0], *v = var[1], *w = var[2], *te = var[3], *ed = var[4], *vis = var[6];
+    double *cp[6] = {u, v, w, te, ed, vis};
+    for (int k = 3; k <= nkmm; ++k)
+        for (int j = 3; j <= njmm; ++j) {
+            const size_t gw = IX(2, j, k), p = IX(3, j, k), e = IX(nimm, j, k), ge = IX(nim, j, k);
+            for (int q = 0; q < 6; ++q) cp[q][gw] = cp[q][e];
+            for (int q = 0; q < 6; ++q) cp[q][ge] = cp[q][p];
+        }
+    for (int k = 3; k <= nkmm; ++k)
+        for (int i = 3; i <= nimm; ++i) {
+            const size_t gw = IX(i, 2, k), p = IX(i, 3, k), e = IX(i, njmm, k), ge = IX(i, njm, k);
+            for (int q = 0; q < 6; ++q) cp[q][gw] = cp[q][e];
+            for (int q = 0; q < 6; ++q) cp[q][ge] = cp[q][p];
+        }
+    for (int i = 3; i <= nimm; ++i)
+        for (int j = 3; j <= njmm; ++j) {
+            const size_t g = IX(i, j, 2), p = IX(i, j, 3);
+            const double fxp = fz[g], fxe = 1.0 - fxp;
+            u[g] = -u[p] * fxp / fxe;
+            v[g] = -v[p] * fxp / fxe;
+            w[g] = -w[p] * fxp / fxe;
+            te[g] = -te[p] * fxp / fxe;
+            ed[g] = ed[p];
+            vis[g] = (vis[g] - vis[p] * fxp) / fxe;
+        }
+    for (int i = 3; i <= nimm; ++i)
+        for (int j = 3; j <= njmm; ++j) {
+            const size_t g = IX(i, j, nkm), p = IX(i, j, nkmm);
+            const double fxe = fz[p], fxp = 1.0 - fxe;
+            u[g] = -u[p] * fxp / fxe;
+            v[g] = -v[p] * fxp / fxe;
+            w[g] = -w[p] * fxp / fxe;
+            te[g] = -te[p] * fxp / fxe;
+            ed[g] = ed[p];
+            vis[g] = (vis[g] - vis[p] * fxp) / fxe;
+        }
+}

@@ -251,6 +251,9 @@ void orc_cgstab(int ni, int nj, int nk, double *fi, orc_sys *s, double sor, int 
                 orc_report *rep);
 void orc_pcg_jacobi(int ni, int nj, int nk, double *fi, orc_sys *s, double sor, int nsw,
                     orc_report *rep);
+/* update_values_at_ghostcells.f90 (periodic i and j, walls at k); var =
+ * {u, v, w, te, ed, t, vis}, t untouched */
+void orc_ghostcells_periodic(int ni, int nj, int nk, double *const *var, const double *fz);
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+# the same driver linked with update_values_at_ghostcells.f90 (periodic variant)
+REF_DRIVER_PERIODIC = os.path.join(HERE, "_ref", "ref_driver_periodic")
 DROPIN_DRIVER = os.path.join(os.path.dirname(HERE), "tests", "fortran", "_build", "dropin_driver")
 
 SYS_FIELDS = ("ae", "aw", "an", "as_", "at", "ab", "ap", "su", "res",
@@ -129,6 +131,7 @@ def lib():
         L.orc_face_areas.argtypes = [i, i, i, dp, dp, dp, C.POINTER(dp)]
         ip = C.POINTER(C.c_int)
         L.orc_ghostcells.argtypes = [i, i, i, C.POINTER(dp), C.POINTER(dp)] + [ip] * 6 + [dp]
+        L.orc_ghostcells_periodic.argtypes = [i, i, i, C.POINTER(dp), dp]
         L.orc_find_intersection_point.argtypes = [dp] * 6
         L.orc_intfac.argtypes = [i, i, i, i, i, i] + [dp] * 7
         _lib = L
@@ -333,6 +336,14 @@ def ghostcells(dims, var, geo, lb, inl):
         assert a.dtype == np.int32 and a.flags.c_contiguous
     inl_ = np.ascontiguousarray(inl, dtype=np.float64)
     lib().orc_ghostcells(*dims, V, G, *[a.ctypes.data_as(ip) for a in lb], _p(inl_))
+
+
+def ghostcells_periodic(dims, var, fz):
+    """update_values_at_ghostcells.f90 (periodic i, j; walls at k) in place on
+    var = [u, v, w, te, ed, t, vis] (t untouched)."""
+    dp = C.POINTER(C.c_double)
+    V = (dp * 7)(*[_p(a) for a in var])
+    lib().orc_ghostcells_periodic(*dims, V, _p(fz))
 
 
 AR_FIELDS = ("ar1x", "ar1y", "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z")

@@ -164,6 +164,9 @@ def ghost_ref_arrays(dims, var, geo, lb, inl):
 
 
 GHOST_CASES = [((12, 10, 9), 601), ((20, 17, 11), 602)]
+# the periodic variant (update_values_at_ghostcells.f90) through the driver
+# linked with it
+GHOST_PERIODIC_CASES = [((12, 10, 9), 611), ((20, 17, 11), 612)]
 FLUXUVW_CASES = [((12, 10, 9), 501, "cds", 1.0), ((12, 10, 9), 502, "luds", 1.0),
                  ((12, 10, 9), 503, "quick", 1.0), ((12, 10, 9), 504, "cds4", 0.7),
                  ((12, 10, 9), 505, "smart", 1.0), ((12, 10, 9), 506, "gamma", 1.0),
@@ -296,7 +299,21 @@ def lognormal_digests():
     print("lognormal_digests.json: written")
 
 
+def ghost_periodic_cases():
+    for dims, seed in GHOST_PERIODIC_CASES:
+        var, geo, lb, inl = ghost_inputs(dims, seed)
+        outs, _, _ = O.run_ref("ghost", dims, ghost_ref_arrays(dims, var, geo, lb, inl),
+                               driver=O.REF_DRIVER_PERIODIC)
+        rec = dict(dims=np.array(dims), seed=seed, **{q: a for q, a in zip(GHOST_VARS, outs)})
+        name = f"ghost_periodic_{dims[0]}x{dims[1]}x{dims[2]}"
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print(f"{name}: written")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--ghost-periodic":
+        ghost_periodic_cases()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--lognormal":
         lognormal_digests()         # oracle only: needs no reference build
         return
@@ -346,6 +363,7 @@ def main():
         name = f"ghost_{dims[0]}x{dims[1]}x{dims[2]}"
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
         print(f"{name}: written")
+    ghost_periodic_cases()
     for dims, seed, scheme, gds in FLUXUVW_CASES:
         f = fields(dims, seed)
         rec = dict(dims=np.array(dims), seed=seed, flags=UVW_SCHEMES[scheme], gds=gds)
