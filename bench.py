@@ -61,14 +61,20 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-iters", type=int, default=24,
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
-    ap.add_argument("--others", default="cgstab_sip,iccg,mixed,learned,unet,piso,piso_mixed,host",
+    ap.add_argument("--others",
+                    default="cgstab_sip,iccg,cgstab,pcg_jacobi,mixed,learned,unet,channel,"
+                            "channel_mixed,host",
                     help="comma list of extra solves to time once each (learned: the U-Net "
                          "guess, trained on solver-generated 64^3 pairs, then the mixed and the "
                          "FP64 solve from it to the same absolute residual; unet: the learned "
-                         "guess's forward pass and training step on 128x96x64; piso: time "
-                         "steps of LES + calcuvw + PISO on the 256^3 channel; piso_mixed: the "
-                         "same with the mixed-precision p' solve; host: the same "
-                         "p' solve through host arrays, PCIe copies included)")
+                         "guess's forward pass and training step on 128x96x64; channel: time "
+                         "steps (Courant, LES, calcuvw, PISO) of the 384x256x192 channel with "
+                         "the FP64 p' solve; channel_mixed: the same with the mixed-precision "
+                         "p' solve; host: the same p' solve through host arrays, PCIe copies "
+                         "included)")
+    ap.add_argument("--channel-steps", type=int, default=100,
+                    help="time steps of the 384x256x192 channel timed with the mixed-precision "
+                         "p' solve (the FP64 comparison runs a tenth of them)")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: 3 PCG-SIP iterations, then "
                          "a 1 GiB device copy as the byte-counter calibration; prints one line")
@@ -152,6 +158,10 @@ def main():
             r = ctx.cgstab_sip(fi, coef, alfa=0.92, sor=1e-8, nsw=20000)
         elif other == "iccg":
             r = ctx.iccg(fi, coef, sor=1e-8, nsw=20000)
+        elif other == "cgstab":
+            r = ctx.cgstab(fi, coef, sor=1e-8, nsw=20000)
+        elif other == "pcg_jacobi":
+            r = ctx.pcg_jacobi(fi, coef, sor=1e-8, nsw=40000)
         elif other == "mixed":
             r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=3e-4)
         elif other == "learned":
@@ -161,11 +171,11 @@ def main():
         elif other == "unet":
             solvers["unet"] = _bench_unet(cap, torch)
             continue
-        elif other == "piso":
-            solvers["piso_step"] = _bench_piso(cap, torch, n)
+        elif other == "channel":
+            solvers["channel_fp64"] = _bench_channel(cap, torch, max(1, args.channel_steps // 10), 4)
             continue
-        elif other == "piso_mixed":
-            solvers["piso_step_mixed"] = _bench_piso(cap, torch, n, solver=6)
+        elif other == "channel_mixed":
+            solvers["channel_mixed"] = _bench_channel(cap, torch, args.channel_steps, 6)
             continue
         elif other == "host":
             solvers["pcgsip_host_pointers"] = _bench_host(cap, torch, arrs, dims)
@@ -403,124 +413,47 @@ def _bench_unet(cap, torch, nx=128, ny=96, nz=64, reps=5):
             "note": "host-side wall time per call including launch and synchronisation"}
 
 
-def _channel_state(torch, n, stretch=2.0, seed=7):
-    """Device-resident module state of a channel on n^3 cells: uniform in x
-    (the i index) and z (k), tanh-stretched in y (j, walls at both ends),
-    laminar parabolic u plus 10 % uniform noise, v = w = noise."""
-    dev, f64 = "cuda", torch.float64
-    g = torch.Generator(device=dev).manual_seed(seed)
-    N = n + 4
-
-    def axis(nodes):
-        h = torch.diff(nodes, prepend=nodes[:1] - (nodes[1] - nodes[0]))
-        c = nodes - 0.5 * h
-        nxt = torch.cat([c[1:], c[-1:] + h[-1:]])
-        fx = (nodes - c) / (nxt - c)
-        return nodes, c, h, fx
-
-    lin = torch.arange(N, dtype=f64, device=dev)
-    X = axis((lin - 1.5) / n * 6.283185307179586)
-    Z = axis((lin - 1.5) / n * 3.141592653589793)
-    xi = ((lin - 1.5) / n).clamp(-0.02, 1.02)
-    Y = axis(torch.tanh(stretch * (2 * xi - 1)) / torch.tanh(torch.tensor(stretch, dtype=f64)))
-    shape = (N, N, N)                       # (k, i, j)
-    b = lambda a, ax: a.view([-1 if d == ax else 1 for d in range(3)]).expand(shape)  # noqa: E731
-    f = {}
-    f["x"], f["xc"] = b(X[0], 1), b(X[1], 1)
-    f["y"], f["yc"] = b(Y[0], 2), b(Y[1], 2)
-    f["z"], f["zc"] = b(Z[0], 0), b(Z[1], 0)
-    f["fx"], f["fy"], f["fz"] = b(X[3], 1), b(Y[3], 2), b(Z[3], 0)
-    dx, dy, dz = b(X[2], 1), b(Y[2], 2), b(Z[2], 0)
-    f["vol"] = dx * dy * dz
-    f["ar1x"], f["ar2y"], f["ar3z"] = dy * dz, dx * dz, dx * dy
-    zero = torch.zeros(shape, dtype=f64, device=dev)
-    for q in ("ar1y", "ar1z", "ar2x", "ar2z", "ar3x", "ar3y"):
-        f[q] = zero
-    f["den"] = torch.ones(shape, dtype=f64, device=dev)
-    yy = b(Y[1], 2)
-    noise = lambda: 0.1 * (2 * torch.rand(shape, generator=g, dtype=f64, device=dev) - 1)  # noqa: E731
-    f["u"] = 1.5 * (1 - yy * yy).clamp(min=0) * (1 + noise())
-    f["v"], f["w"] = noise(), noise()
-    f["f1"] = f["den"] * f["u"] * f["ar1x"]
-    f["f2"], f["f3"] = f["den"] * f["v"] * f["ar2y"], f["den"] * f["w"] * f["ar3z"]
-    for q in ("p", "pp", "ae", "aw", "an", "as", "at", "ab", "su", "sv", "sw", "sp", "apu", "apv",
-              "apw", "res", "bb", "bs", "bw", "bp", "bn", "be", "bt", "he", "hw", "hn", "hs", "ht",
-              "hb", "te", "ed", "t"):
-        f[q] = zero.clone()
-    f["ap"] = torch.full(shape, 1e3, dtype=f64, device=dev)
-    f["vis"] = torch.full(shape, 1.0 / 5600.0, dtype=f64, device=dev)
-    for q in ("u", "v", "w"):
-        f[q + "o"], f[q + "oo"] = f[q].clone(), f[q].clone()
-    f = {q: a.contiguous().view(-1).clone() for q, a in f.items()}
-    for q in ("gradu", "gradv", "gradw", "gradp"):
-        f[q] = torch.zeros(3 * N ** 3, dtype=f64, device=dev)
-    wd = (1 - yy.abs()).clamp(min=1e-3)
-    return f, wd.contiguous().view(-1).clone()
-
-
-def _bench_piso(cap, torch, n, steps=2, solver=4):
-    """Time steps of the channel: LES viscosity (strain rate + calc_vis_les),
-    the momentum predictor (calcuvw: fluxuvw, wall shear, three sipsol) and
-    the pressure correction (PISO, 2 correctors, PCG-SIP alfa = 0, the last
-    to 1e-8; solver 6: the mixed-precision form), all device-resident.
-    Returns ms per step and per phase."""
+def _bench_channel(cap, torch, steps, solver, size=(384, 256, 192)):
+    """Time steps of the channel (cappuccino-ghostcells-mpi_amd/channel.py):
+    384 x 256 x 192 cells, periodic x and z, walls at y (the reference's k),
+    tanh stretching 2.0, Re_bulk 5600, Smagorinsky LES, dt from Co = 0.5,
+    CDS momentum with BDF2, PISO with 2 correctors (0.1, then 1e-8) and the
+    FP64 (solver 4) or mixed-precision (solver 6) p' solve; everything on the
+    device.  One untimed step first."""
+    import importlib.util
     import numpy as np
-    N = n + 4
-    f, wd = _channel_state(torch, n)
-    ctx = cap.Context(N, N, N, device_pointers=True)
-    spv = torch.zeros(N ** 3, dtype=torch.float64, device="cuda")
-    strain = torch.zeros_like(spv)
-    vort = torch.zeros_like(spv)
-    h = 6.283185307179586 / n
-    dt = 0.5 * h / 1.5                                   # Courant 0.5 at the centreline
-    o = cap.DrvOpts()
-    o.ncorr, o.npcor, o.solver, o.nsw = 2, 1, solver, 3000
-    o.urf, o.sor, o.alfa = 1.0, 1e-8, 0.0
-    o.ijkpr = 2 * N * N + 2 * N + 3                      # the cell (3, 3, 3)
-    o.const_mflux, o.bdf, o.timestep, o.btime, o.densit = 1, 1, dt, 1.0, 1.0
-    q = cap.UvwOpts()
-    q.schemes, q.gds = 1, 1.0                            # CDS, full deferred correction
-    for m in range(3):
-        q.urfr[m], q.urfm[m], q.sor[m], q.nsw[m] = 1.0, 0.0, float(np.float32(0.1)), 5
-    q.gradpcmf, q.nx, q.ny, q.nz = 0.0, N - 2, N - 2, N - 2
-    les = cap.LesOpts()
-    les.viscos, les.densit, les.cappa, les.elog, les.urfvis = 1.0 / 5600.0, 1.0, 0.41, 8.432, 1.0
-    les.nx, les.ny, les.nz = N - 2, N - 2, N - 2
-    phases = {"les": 0.0, "calcuvw": 0.0, "piso": 0.0}
+    spec = importlib.util.spec_from_file_location(
+        "cap_channel", os.path.join(ROOT, "cappuccino-ghostcells-mpi_amd", "channel.py"))
+    CH = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(CH)
+    nx, ny, nz = size
+    ch = CH.Channel(cap, torch, nx, ny, nz, solver=solver)
+    ch.step(timed=False)
     iters = []
-
-    def step(timed):
-        for a, b in (("uoo", "uo"), ("voo", "vo"), ("woo", "wo"), ("uo", "u"), ("vo", "v"), ("wo", "w")):
-            f[a].copy_(f[b])
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ctx.find_strain_rate(f, strain, vort)
-        ctx.calc_vis_les(f, strain, wd, les)
-        t1 = time.perf_counter()
-        ctx.calcuvw(f, spv, o, q)
-        t2 = time.perf_counter()
-        reps, cont = ctx.piso(f, o)
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        if timed:
-            phases["les"] += (t1 - t0) * 1e3
-            phases["calcuvw"] += (t2 - t1) * 1e3
-            phases["piso"] += (t3 - t2) * 1e3
-            iters.append([r["iters"] for r in reps])
-        return reps
-
-    step(False)
-    for _ in range(steps):
-        reps = step(True)
-    ctx.close()
-    per = {k: round(v / steps, 2) for k, v in phases.items()}
-    total = round(sum(per.values()), 2)
-    log(f"[bench] piso step {n}^3 (solver {solver}): {total} ms ({per}), p' solve iterations per "
-        f"corrector {iters}")
-    return {"ms_per_step": total, "phases_ms": per, "pcg_iters": iters, "steps": steps,
-            "case": f"{n}^3 channel (x, z uniform; y tanh-stretched 2.0), parabolic u + 10% noise, "
-                    "BDF2, CDS momentum, 2 PISO correctors (PCG-SIP alfa 0, sor 0.1 then 1e-8"
-                    + (", mixed-precision form" if solver == 6 else "") + "), LES viscosity each step"}
+    t = time.perf_counter()
+    for s in range(steps):
+        reps = ch.step()
+        iters.append([r["iters"] for r in reps])
+        if (s + 1) % 20 == 0:
+            log(f"[bench] channel step {s + 1}/{steps}")
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    per = {k: round(v, 2) for k, v in ch.phase_ms().items()}
+    it = np.array(iters)
+    out = {"grid": [nx, ny, nz], "padded": list(ch.dims), "steps": steps,
+           "ms_per_step": round(ms, 2), "steps_per_s": round(1e3 / ms, 3), "phases_ms": per,
+           "p_solve_iters_mean_per_corrector": [round(float(x), 1) for x in it.mean(axis=0)],
+           "continuity_last": [float(x) for x in ch.last["cont"][-1]],
+           "courant": ch.last["courant"], "dt": ch.last["dt"],
+           "bulk_velocity": round(ch.bulk_velocity(), 6),
+           "case": f"{nx}x{ny}x{nz} channel, periodic x and z, walls at y (reference k), tanh "
+                   "2.0, Re_bulk 5600, Smagorinsky LES, Co 0.5, CDS + BDF2 momentum, PISO 2 "
+                   "correctors, p' by " + ("mixed-precision PCG-SIP" if solver == 6 else
+                                           "PCG-SIP (alfa 0), FP64")}
+    log(f"[bench] channel (solver {solver}): {out['ms_per_step']} ms/step {per}, iterations "
+        f"{out['p_solve_iters_mean_per_corrector']}")
+    ch.close()
+    return out
 
 
 def _pmc_traffic(kernel, n):

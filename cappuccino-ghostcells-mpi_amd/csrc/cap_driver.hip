@@ -988,7 +988,10 @@ extern "C" int cap_calcuvw(cap_ctx *ctx, cap_flow *f, double *spv, const cap_drv
     if (o->bdf && (!f->uoo || !f->voo || !f->woo)) return CAP_ERR_ARG;
     if (o->buoy && o->boussinesq && !f->t) return CAP_ERR_ARG;
     const Dim d = dim_of(ctx);
-    if (d.ni - 1 > d.nk) return CAP_ERR_ARG;     // the top-wall plane lk(nimm) and its ghosts
+    // the top wall's cells: the plane lk(nimm) as written (it and its ghosts
+    // must lie in the array), or the plane top_k
+    if (q->top_k == 0 ? d.ni - 1 > d.nk : (q->top_k < 3 || q->top_k > d.nk - 2)) return CAP_ERR_ARG;
+    const int top_plane = q->top_k ? q->top_k : d.ni - 2;
     hipStream_t s = ctx->stream;
     RCHK(grad3(ctx, f, f->u, f->gradu));
     RCHK(grad3(ctx, f, f->v, f->gradv));
@@ -1006,7 +1009,7 @@ extern "C" int cap_calcuvw(cap_ctx *ctx, cap_flow *f, double *spv, const cap_drv
                      f->f3, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
     hipLaunchKernelGGL(k_uvw_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, io, 0, 3,
                        (double)q->nx, (double)q->ny, (double)q->nz);
-    hipLaunchKernelGGL(k_uvw_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, io, 1, d.ni - 2,
+    hipLaunchKernelGGL(k_uvw_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, io, 1, top_plane,
                        (double)q->nx, (double)q->ny, (double)q->nz);
     if (o->cn)
         hipLaunchKernelGGL(k_half6, dim3(grid_for((long)ctx->nstd)), dim3(256), 0, s, ctx->nstd, f->ae,
@@ -1051,7 +1054,8 @@ extern "C" int cap_calc_vis_les(cap_ctx *ctx, cap_flow *f, const double *strain,
         !f->den || !f->fz || !f->xc || !f->yc || !f->zc || !f->ar3x || !f->ar3y || !f->ar3z || !f->vis)
         return CAP_ERR_ARG;
     const Dim d = dim_of(ctx);
-    if (d.ni - 1 > d.nk) return CAP_ERR_ARG;     // the top-wall plane lk(nimm) and its ghosts
+    if (o->top_k == 0 ? d.ni - 1 > d.nk : (o->top_k < 3 || o->top_k > d.nk - 2)) return CAP_ERR_ARG;
+    const int top_plane = o->top_k ? o->top_k : d.ni - 2;
     LesIO q;
     q.u = f->u; q.v = f->v; q.w = f->w; q.vol = f->vol; q.den = f->den; q.fz = f->fz;
     q.xc = f->xc; q.yc = f->yc; q.zc = f->zc; q.ar3x = f->ar3x; q.ar3y = f->ar3y; q.ar3z = f->ar3z;
@@ -1061,7 +1065,7 @@ extern "C" int cap_calc_vis_les(cap_ctx *ctx, cap_flow *f, const double *strain,
     hipLaunchKernelGGL(k_les, dim3(grid_for(range2_cells(d))), dim3(256), 0, s, d, q);
     hipLaunchKernelGGL(k_les_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, q, 0, 3,
                        (double)o->nx, (double)o->ny, (double)o->nz);
-    hipLaunchKernelGGL(k_les_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, q, 1, d.ni - 2,
+    hipLaunchKernelGGL(k_les_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, q, 1, top_plane,
                        (double)o->nx, (double)o->ny, (double)o->nz);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));

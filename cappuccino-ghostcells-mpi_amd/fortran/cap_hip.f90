@@ -99,12 +99,14 @@ module cap_hip
     integer(c_int) :: nsw(3) = 5
     real(c_double) :: gradpcmf = 0
     integer(c_int) :: nx = 0, ny = 0, nz = 0
+    integer(c_int) :: top_k = 0      ! 0: the plane lk(nimm), as the reference writes it
   end type cap_uvw_opts
 
   ! calc_vis_les settings
   type, bind(c) :: cap_les_opts
     real(c_double) :: viscos = 0, densit = 1, cappa = 0.41d0, elog = 8.432d0, urfvis = 1
     integer(c_int) :: nx = 0, ny = 0, nz = 0
+    integer(c_int) :: top_k = 0
   end type cap_les_opts
 
   type, bind(c) :: cap_report

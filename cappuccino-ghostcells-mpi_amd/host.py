@@ -100,14 +100,15 @@ class UvwOpts(C.Structure):
     """calcuvw's settings beyond DrvOpts (cap_uvw_opts)."""
     _fields_ = [("schemes", C.c_int), ("gds", C.c_double), ("urfr", C.c_double * 3),
                 ("urfm", C.c_double * 3), ("sor", C.c_double * 3), ("nsw", C.c_int * 3),
-                ("gradpcmf", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
+                ("gradpcmf", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("top_k", C.c_int)]
 
 
 class LesOpts(C.Structure):
     """calc_vis_les settings (cap_les_opts)."""
     _fields_ = [("viscos", C.c_double), ("densit", C.c_double), ("cappa", C.c_double),
                 ("elog", C.c_double), ("urfvis", C.c_double), ("nx", C.c_int), ("ny", C.c_int),
-                ("nz", C.c_int)]
+                ("nz", C.c_int), ("top_k", C.c_int)]
 
 
 class _Report(C.Structure):

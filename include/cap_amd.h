@@ -226,7 +226,11 @@ int cap_gethbya(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o);
 /* calcuvw settings beyond cap_drv_opts (of which calcuvw reads const_mflux,
  * buoy, boussinesq, bdf, cn, timestep, btime, densit, beta, tref, grav*,
  * alfa): fluxuvw's scheme flags (CAP_UVW_*) and gds(iu); urfr, urfm, sor, nsw
- * of u, v, w; gradpcmf; the module integers nx, ny, nz of the wall loops. */
+ * of u, v, w; gradpcmf; the module integers nx, ny, nz of the wall loops.
+ * top_k (no counterpart in the reference): 0 takes the top wall's cells on
+ * the plane lk(nimm), as calcuvw.f90:254 writes it (the wall's first cells
+ * only when ni = nk; then ni <= nk + 1 is required); 3..nk-2 takes them on
+ * that plane instead (nk - 2: the wall's own first cells for any ni). */
 typedef struct {
     int schemes;
     double gds;
@@ -234,6 +238,7 @@ typedef struct {
     int nsw[3];
     double gradpcmf;
     int nx, ny, nz;
+    int top_k;
 } cap_uvw_opts;
 
 /* `call calcuvw` (calcuvw.f90:3-464, Gauss gradients, isol = 1): gradients of
@@ -255,13 +260,14 @@ int cap_find_strain_rate(cap_ctx *ctx, cap_flow *f, double *strain, double *vort
 typedef struct {
     double viscos, densit, cappa, elog, urfvis;   /* urfvis = urf(ivis) */
     int nx, ny, nz;                               /* module integers of the wall loops */
+    int top_k;                                    /* as cap_uvw_opts.top_k */
 } cap_les_opts;
 
 /* `call calc_vis_les` (calc_vis_les.f90:3-151): Smagorinsky viscosity with
  * van Driest damping (C_Les = 0.17, C_delta = 0.158) into f->vis on
  * k = 2..nkm, i, j = 3..n?mm, then the wall-function viscosity of the bottom
  * and top wall ghost cells (top wall on the plane lk(nimm), as written:
- * needs ni <= nk + 1).  strain and walldist are device arrays.  pow, exp
+ * needs ni <= nk + 1; or on o->top_k when that is set).  strain and walldist are device arrays.  pow, exp
  * and log are the device's: within a few ulp of the host's. */
 int cap_calc_vis_les(cap_ctx *ctx, cap_flow *f, const double *strain, const double *walldist,
                      const cap_les_opts *o);

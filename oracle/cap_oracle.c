@@ -1831,7 +1831,7 @@ void orc_calc_vis_les(int ni, int nj, int nk, const double *u, const double *v, 
                       const double *yc, const double *zc, const double *ar3x, const double *ar3y,
                       const double *ar3z, const double *strain, const double *walldist,
                       double *vis, double viscos, double densit, double cappa, double elog,
-                      double urfvis, int nx, int ny, int nz)
+                      double urfvis, int nx, int ny, int nz, int top_k)
 {
     const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
     /* default-real literals: C_delta = 0.158, C_Les = 0.17, 1./3. */
@@ -1855,9 +1855,11 @@ void orc_calc_vis_les(int ni, int nj, int nk, const double *u, const double *v, 
         les_wall(inbc, ingc, ingc, fz[ingc], u, v, w, xc, yc, zc, ar3x, ar3y, ar3z, vis, viscos,
                  densit, cappa, elog, dnx, dny, dnz);
     }
-    /* the top wall's cells are taken on the plane lk(nimm), as written */
+    /* the top wall's cells are taken on the plane lk(nimm), as written (or
+     * on top_k when set) */
+    const int ktop_les = top_k ? top_k : nimm;
     for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
-        const size_t inbc = IX(i, j, nimm), ingc = inbc + nij;
+        const size_t inbc = IX(i, j, ktop_les), ingc = inbc + nij;
         les_wall(inbc, ingc, inbc, 1.0 - fz[inbc], u, v, w, xc, yc, zc, ar3x, ar3y, ar3z, vis,
                  viscos, densit, cappa, elog, dnx, dny, dnz);
     }
@@ -1945,9 +1947,11 @@ void orc_calcuvw(int ni, int nj, int nk, orc_flow *F, double *spv, const orc_drv
         const size_t inbc = IX(i, j, 3), ingc = inbc - nij;
         uvw_wall(F, spv, inbc, ingc, ingc, F->fz[ingc], dnx, dny, dnz);
     }
-    /* the top wall's cells are taken on the plane lk(nimm), as written */
+    /* the top wall's cells are taken on the plane lk(nimm), as written (or
+     * on q->top_k when set) */
+    const int ktop = q->top_k ? q->top_k : nimm;
     for (int i = 3; i <= nimm; ++i) for (int j = 3; j <= njmm; ++j) {
-        const size_t inbc = IX(i, j, nimm), ingc = inbc + nij;
+        const size_t inbc = IX(i, j, ktop), ingc = inbc + nij;
         uvw_wall(F, spv, inbc, ingc, inbc, 1.0 - F->fz[inbc], dnx, dny, dnz);
     }
     if (o->cn)

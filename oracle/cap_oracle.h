@@ -214,7 +214,7 @@ void orc_calc_vis_les(int ni, int nj, int nk, const double *u, const double *v, 
                       const double *yc, const double *zc, const double *ar3x, const double *ar3y,
                       const double *ar3z, const double *strain, const double *walldist,
                       double *vis, double viscos, double densit, double cappa, double elog,
-                      double urfvis, int nx, int ny, int nz);
+                      double urfvis, int nx, int ny, int nz, int top_k);
 
 /* calcuvw settings beyond the driver settings */
 typedef struct {
@@ -225,6 +225,7 @@ typedef struct {
     int nsw[3];
     double gradpcmf;
     int nx, ny, nz;              /* module integers of the wall loops */
+    int top_k;                   /* top wall's plane: 0 = lk(nimm) as written */
 } orc_uvw_opts;
 
 /* calcuvw.f90:3-464 (Gauss gradients, isol = 1): momentum sources, fluxuvw

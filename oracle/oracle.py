@@ -76,7 +76,8 @@ class UvwOpts(C.Structure):
     """orc_uvw_opts: calcuvw's settings beyond the driver settings."""
     _fields_ = [("schemes", C.c_int), ("gds", C.c_double), ("urfr", C.c_double * 3),
                 ("urfm", C.c_double * 3), ("sor", C.c_double * 3), ("nsw", C.c_int * 3),
-                ("gradpcmf", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
+                ("gradpcmf", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("top_k", C.c_int)]
 
 
 _lib = None
@@ -126,7 +127,7 @@ def lib():
         L.orc_correct_bc.argtypes = [i, i, i, Fl]
         L.orc_gethbya.argtypes = [i, i, i, Fl, Op]
         L.orc_strain_rate.argtypes = [i, i, i, dp, dp, dp, dp, dp]
-        L.orc_calc_vis_les.argtypes = [i, i, i] + [dp] * 15 + [d, d, d, d, d, i, i, i]
+        L.orc_calc_vis_les.argtypes = [i, i, i] + [dp] * 15 + [d, d, d, d, d, i, i, i, i]
         L.orc_calcuvw.argtypes = [i, i, i, Fl, dp, Op, C.POINTER(UvwOpts), C.POINTER(_Report)]
         L.orc_face_areas.argtypes = [i, i, i, dp, dp, dp, C.POINTER(dp)]
         ip = C.POINTER(C.c_int)
@@ -469,7 +470,8 @@ def calc_vis_les(dims, f, vis, par):
     is updated in place; par: viscos, densit, cappa, elog, urf, nx, ny, nz"""
     lib().orc_calc_vis_les(*dims, *[_p(f[q]) for q in LES_IN], _p(f["strain"]),
                            _p(f["walldist"]), _p(vis), par["viscos"], par["densit"],
-                           par["cappa"], par["elog"], par["urf"], par["nx"], par["ny"], par["nz"])
+                           par["cappa"], par["elog"], par["urf"], par["nx"], par["ny"], par["nz"],
+                           par.get("top_k", 0))
 
 
 def calcuvw(dims, f, spv, opts, uvw):

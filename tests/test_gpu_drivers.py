@@ -215,6 +215,51 @@ def test_calcuvw_bitwise(gpu, O, schemes, opts):
     np.testing.assert_array_equal(spv_g.cpu().numpy(), spv_o)
 
 
+@pytest.mark.parametrize("dims", [(14, 12, 20), (22, 10, 16)])
+def test_top_wall_plane_option(gpu, O, dims):
+    """top_k = nkmm: the top wall's terms of calcuvw and calc_vis_les on the
+    wall's own first cells (the reference's lk(nimm) is that plane only when
+    ni = nk), bit-identical to the oracle with the same option."""
+    import torch
+    ni, nj, nk = dims
+    f = FC.flow_case(dims, 43)
+    rng = np.random.default_rng(44)
+    for q in ("f1", "f2", "f3"):
+        f[q][:] = 0.1 * rng.normal(size=f[q].size)
+    oo, go = FC.opts(O.DrvOpts, alfa=0.92, bdf=1), FC.opts(gpu.DrvOpts, alfa=0.92, bdf=1)
+    qo, qg = _uvw_opts(O.UvwOpts, dims, 1), _uvw_opts(gpu.UvwOpts, dims, 1)
+    qo.top_k = qg.top_k = nk - 2
+    fo = {k: a.copy() for k, a in f.items()}
+    spv_o = np.zeros_like(f["u"])
+    with O.plane_sums(2):
+        rep_o = O.calcuvw(dims, fo, spv_o, oo, qo)
+    fd = {k: torch.from_numpy(a.copy()).cuda() for k, a in f.items()}
+    spv_g = torch.zeros(f["u"].size, dtype=torch.float64, device="cuda")
+    with gpu.Context(*dims, device_pointers=True) as ctx:
+        rep_g = ctx.calcuvw(fd, spv_g, go, qg)
+        sys.path.insert(0, os.path.join(HERE, "golden"))
+        import make_golden as MG
+        lf, par = MG.les_inputs(dims, 45)
+        par["top_k"] = nk - 2
+        vis_o = lf["vis"].copy()
+        O.calc_vis_les(dims, lf, vis_o, par)
+        fl = {k: torch.from_numpy(lf[k].copy()).cuda() for k in O.LES_IN + ("vis",)}
+        les = gpu.LesOpts()
+        les.viscos, les.densit, les.cappa, les.elog, les.urfvis = (par[k] for k in ("viscos", "densit", "cappa", "elog", "urf"))
+        les.nx, les.ny, les.nz, les.top_k = par["nx"], par["ny"], par["nz"], nk - 2
+        ctx.calc_vis_les(fl, torch.from_numpy(lf["strain"]).cuda(), torch.from_numpy(lf["walldist"]).cuda(), les)
+        vis_g = fl["vis"].cpu().numpy()
+    for a, b in zip(rep_o, rep_g):
+        assert (a["iters"], a["resor"], a["resl"]) == (b["iters"], b["resor"], b["resl"])
+    check_state(fo, {k: t.cpu().numpy() for k, t in fd.items()}, f"calcuvw top_k {dims}")
+    np.testing.assert_array_equal(spv_g.cpu().numpy(), spv_o)
+    # the top wall's ghost plane nkm got its wall viscosity, within the device
+    # library's pow/exp/log
+    np.testing.assert_allclose(vis_g, vis_o, rtol=8e-16 * 8, atol=0)
+    top = vis_o.reshape(nk, ni, nj)[nk - 2, 2:-2, 2:-2]
+    assert not np.array_equal(top, lf["vis"].reshape(nk, ni, nj)[nk - 2, 2:-2, 2:-2])
+
+
 @pytest.mark.parametrize("dims", [(12, 10, 12), (18, 14, 18), (10, 13, 11)])
 def test_strain_and_les_viscosity(gpu, O, dims):
     import torch
