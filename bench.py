@@ -130,6 +130,7 @@ def main():
     torch.cuda.synchronize()
     ms_step = (time.perf_counter() - t1) * 1e3 / args.steps
     rep = reps[-1]
+    fi64 = fi.clone()      # the headline's FP64 solution, the mixed solve's reference
     log(f"[bench] pcgsip: {ms_step:.1f} ms/solve, iters {rep['iters']}, conv {rep['converged']}")
     iters = rep["iters"]
 
@@ -186,6 +187,9 @@ def main():
         dt = (time.perf_counter() - t) * 1e3
         solvers[other] = {"ms": round(dt, 2), "iters": r["iters"], "converged": r["converged"]}
         log(f"[bench] {other}: {dt:.1f} ms, iters {r['iters']}, conv {r['converged']}")
+        if other == "mixed":
+            solvers[other]["error_vs_fp64"] = _mixed_error(arrs, fi64, fi, dims)
+            log(f"[bench] mixed vs FP64: {json.dumps(solvers[other]['error_vs_fp64'])}")
 
     cpu = None
     if args.cpu_iters > 0:
@@ -277,6 +281,39 @@ def _l1_residual(arrs, x, dims):
          + v(arrs["an"])[c, c, c] * X[c, c, 3:-1] + v(arrs["as_"])[c, c, c] * X[c, c, 1:-3]
          + v(arrs["at"])[c, c, c] * X[3:-1, c, c] + v(arrs["ab"])[c, c, c] * X[1:-3, c, c])
     return float(r.abs().sum())
+
+
+def _mixed_error(arrs, x64, x, dims):
+    """Errors of a solution x against the FP64 solution x64 of the same
+    system: p' on the solve range, and the face mass-flux corrections the
+    solution makes, a_f (p'_nb - p'_P) on the east, north and top faces
+    between solve-range cells (calcp-multiple_correction_SIMPLE.f90:173-
+    190); max-norm and RMS, each relative to the same norm of the FP64
+    values."""
+    ni, nj, nk = dims
+    v = lambda a: a.view(nk, ni, nj)  # noqa: E731
+    c = slice(2, -2)
+    X, X64 = v(x), v(x64)
+    out = {}
+
+    def rel(d, ref):
+        return {"max": float(d.abs().max() / ref.abs().max()),
+                "rms": float(d.pow(2).mean().sqrt() / ref.pow(2).mean().sqrt())}
+    out["p"] = rel(X[c, c, c] - X64[c, c, c], X64[c, c, c])
+    lo, hi = slice(2, -3), slice(3, -2)
+    faces = {"east": (arrs["ae"], (c, lo, c), (c, hi, c)),
+             "north": (arrs["an"], (c, c, lo), (c, c, hi)),
+             "top": (arrs["at"], (lo, c, c), (hi, c, c))}
+    d_all, r_all = [], []
+    for name, (a, P, E) in faces.items():
+        A = v(a)[P]
+        f = A * (X[E] - X[P])
+        f64 = A * (X64[E] - X64[P])
+        d_all.append((f - f64).reshape(-1))
+        r_all.append(f64.reshape(-1))
+    import torch
+    out["face_flux_correction"] = rel(torch.cat(d_all), torch.cat(r_all))
+    return out
 
 
 def _bench_learned(cap, torch, ctx, coef, arrs, dims, fi, seed, r0, mixed, n_small=64,

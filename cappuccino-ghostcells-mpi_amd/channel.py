@@ -187,12 +187,17 @@ class Channel:
         return out
 
     # ---------------------------------------------------------- the step
-    def step(self, timed=True, guess=None):
-        """One time step; returns the PISO solver reports.  guess: optional
-        callable(channel) run right before the pressure correction (the
-        learned initial guess writes pp there)."""
+    def step(self, timed=True, net=None, warm=False):
+        """One time step; returns the PISO solver reports.  With net (a UNet
+        on this channel's context) or warm, the last corrector's p solve
+        starts from the learned guess or from the previous step's p instead
+        of the previous corrector's p, when that start has the lower L1
+        residual, and stops at the same absolute residual
+        (Context.piso_guess); self.last then also holds `accepted` and
+        `l1` (reference start, start used)."""
         torch = self.torch
         f, ctx = self.f, self.ctx
+        prev = f["p"].clone() if (net is not None or warm) else None
         for a, b in (("uoo", "uo"), ("voo", "vo"), ("woo", "wo"), ("uo", "u"), ("vo", "v"),
                      ("wo", "w")):
             f[a].copy_(f[b])
@@ -210,9 +215,12 @@ class Channel:
         ctx.calcuvw(f, self.spv, self.opts, self.uvw)
         self._ghosts()
         t3 = time.perf_counter()
-        if guess is not None:
-            guess(self)
-        reps, cont = ctx.piso(f, self.opts)
+        extra = {}
+        if prev is not None:
+            reps, cont, acc, l1 = ctx.piso_guess(f, self.opts, prev, net=net)
+            extra = dict(accepted=acc, l1=l1)
+        else:
+            reps, cont = ctx.piso(f, self.opts)
         self._ghosts()
         torch.cuda.synchronize()
         t4 = time.perf_counter()
@@ -220,8 +228,68 @@ class Channel:
             for k, v in zip(("courant", "les", "calcuvw", "piso"), (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
                 self.phases[k] += v * 1e3
             self.steps += 1
-        self.last = dict(reps=reps, cont=cont, courant=co, mean_courant=mean, dt=self.dt)
+        self.last = dict(reps=reps, cont=cont, courant=co, mean_courant=mean, dt=self.dt, **extra)
         return reps
+
+    # ------------------------------------------------- the learned guess
+    def snapshot(self):
+        """copies of every device field and the time-stepping scalars"""
+        return ({q: a.clone() for q, a in self.f.items()}, self.spv.clone(), self.strain.clone(),
+                self.vort.clone(), self.cosu.clone(), self.time, self.dt)
+
+    def restore(self, snap):
+        fields, spv, strain, vort, cosu, self.time, self.dt = snap
+        for q, a in fields.items():
+            self.f[q].copy_(a)
+        self.spv.copy_(spv)
+        self.strain.copy_(strain)
+        self.vort.copy_(vort)
+        self.cosu.copy_(cosu)
+
+    def pressure_system(self):
+        """the p system of the last corrector of the last step (PISO leaves
+        it in ae..ab, ap, su) as a Coef"""
+        f = self.f
+        return self.cap.Coef(ae=f["ae"], aw=f["aw"], an=f["an"], as_=f["as"], at=f["at"],
+                             ab=f["ab"], ap=f["ap"], su=f["su"])
+
+    def interior(self, a):
+        """the solve range (k, i, j = 3..n?mm) of a field, in the learned
+        guess's voxel order (j fastest, then i, then k)"""
+        ni, nj, nk = self.dims
+        return a.view(nk, ni, nj)[2:-2, 2:-2, 2:-2].reshape(-1)
+
+    def residual(self, x):
+        """r = su - M x on the solve range of the last corrector's system
+        (M x = ap x - sum a_nb x_nb), in the learned guess's voxel order"""
+        ni, nj, nk = self.dims
+        f = self.f
+        X = x.view(nk, ni, nj)
+        c = lambda q: f[q].view(nk, ni, nj)[2:-2, 2:-2, 2:-2]  # noqa: E731
+        r = (c("ae") * X[2:-2, 3:-1, 2:-2] + c("aw") * X[2:-2, 1:-3, 2:-2] +
+             c("an") * X[2:-2, 2:-2, 3:-1] + c("as") * X[2:-2, 2:-2, 1:-3] +
+             c("at") * X[3:-1, 2:-2, 2:-2] + c("ab") * X[1:-3, 2:-2, 2:-2] + c("su") -
+             c("ap") * X[2:-2, 2:-2, 2:-2])
+        return r.reshape(-1)
+
+    def pressure_pair(self, prev):
+        """one training pair of the learned guess from the step just taken:
+        the warm start w = prev on the solve range with the ghost values the
+        last corrector's solve held (those of p), inputs (r / rms r,
+        ap / mean ap, w / rms w) with r = su - M w (the channels
+        cap_unet_guess feeds the network in cap_piso_guess), target the
+        change p - w in units of its RMS; fp32, channel-major, on the device"""
+        torch = self.torch
+        rms = lambda a: a.pow(2).mean().sqrt().clamp(min=1e-300)  # noqa: E731
+        w = self.f["p"].clone()
+        ni, nj, nk = self.dims
+        w.view(nk, ni, nj)[2:-2, 2:-2, 2:-2] = prev.view(nk, ni, nj)[2:-2, 2:-2, 2:-2]
+        r = self.residual(w)
+        ap = self.interior(self.f["ap"])
+        pv = self.interior(w)
+        d = self.interior(self.f["p"]) - pv
+        x = torch.stack([r / rms(r), ap / ap.mean(), pv / rms(pv) if float(pv.abs().max()) > 0 else pv])
+        return x.float().reshape(-1).contiguous(), (d / rms(d)).float().contiguous()
 
     def phase_ms(self):
         s = max(self.steps, 1)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include "cap_amd.h"
 
 namespace cap { struct DevState; }
 
@@ -73,6 +74,17 @@ struct cap_ctx {
 };
 
 void cap_free_extra(cap_ctx *ctx);
+// a reference-layout array of nstd doubles kept in staging slot `slot`
+// (allocated on first use; null when that fails, with ctx->err set)
+double *cap_slot_buffer(cap_ctx *ctx, int slot);
+// The corrector loop of cap_piso.  Before the p solve of the last corrector
+// it calls hook (when not null), which may replace the solve's start x on the
+// solve range and scale the relative tolerance *sor; a negative return ends
+// the loop with that status.
+typedef int (*cap_start_hook)(void *user, cap_ctx *ctx, const cap_coef *c, double *x,
+                              double *sor);
+int cap_piso_run(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple, cap_report *reps,
+                 double *cont, cap_start_hook hook, void *user);
 // looks up this device's copy of the static scratch array of the
 // allocation-free entry points (once per device; the device must be current)
 int cap_scratch_setup(int device);

@@ -659,6 +659,12 @@ extern "C" int cap_calcp(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, cap_r
 extern "C" int cap_piso(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple,
                         cap_report *reps, double *cont)
 {
+    return cap_piso_run(ctx, f, o, pimple, reps, cont, nullptr, nullptr);
+}
+
+int cap_piso_run(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple, cap_report *reps,
+                 double *cont, cap_start_hook hook, void *user)
+{
     if (!dev_ok(ctx) || !f || !o || !reps || !cont || !flow_ok(f) || !f->he || !f->hw || !f->hn ||
         !f->hs || !f->ht || !f->hb || !f->be || !f->bn || !f->bt || !f->apu || !f->apv || !f->apw ||
         !f->den)
@@ -704,8 +710,12 @@ extern "C" int cap_piso(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pi
                                f->ab, f->ap, f->su, (const double *)f->p);
             HIPCHK(hipGetLastError());
             // sor(ip) = 0.1 on all but the last corrector: a default-real literal
-            const double sor = icorr != o->ncorr ? (double)0.1f : o->sor;
+            double sor = icorr != o->ncorr ? (double)0.1f : o->sor;
             cap_report *rep = &reps[(icorr - 1) * o->npcor + ipcorr - 1];
+            if (hook && icorr == o->ncorr && ipcorr == o->npcor) {
+                const int hr = hook(user, ctx, &C, x, &sor);
+                if (hr < 0) return hr;
+            }
             const int rc = solve(ctx, o->solver == 6 ? 6 : 4, x, &C, o, sor, rep);
             if (rc < 0) return rc;
             worst = std::max(worst, rc);

@@ -220,6 +220,50 @@ struct SlotWalk {
     }
 };
 
+// Layout moves (k_gather, k_scatter).  Block bid moves MOVE_SLOTS
+// consecutive slots [s0, s0 + ns) of one tile between the layouts through
+// LDS.  On the reference side a row of a tile (its 16 columns of one i, at
+// one plane) is 16 consecutive doubles, and its elements lie in 16
+// consecutive slots (slot = plane + a + b mod S); the block reads or writes
+// whole rows there (row step j: plane s0 - b - 15 + j) and whole slots on
+// the tiled side, so both sides are coalesced.  A row whose slots straddle
+// the ends of the block's range is shared with the neighbouring range.  The
+// hardware hands block b to XCD b mod 8, and XCD x takes the items
+// [x per, (x + 1) per) in order, so neighbouring ranges of a tile run on one
+// XCD one after the other and the shared rows' lines are still in its L2.
+constexpr int MOVE_SLOTS = 32;
+
+CAP_HD int move_grid(const Lay &L)
+{
+    const long W = (long)L.ntiles * ((L.S + MOVE_SLOTS - 1) / MOVE_SLOTS);
+    return (int)(8 * ((W + 7) / 8));
+}
+
+// the (tile, first slot) of block bid of the move_grid grid; false for the
+// blocks past the last item
+CAP_HD bool move_item(const Lay &L, int bid, int &tile, int &s0)
+{
+    const int nch = (L.S + MOVE_SLOTS - 1) / MOVE_SLOTS;
+    const long W = (long)L.ntiles * nch;
+    const long per = (W + 7) / 8;
+    const long r = bid >> 3;
+    const long w = (long)(bid & 7) * per + r;
+    if (r >= per || w >= W) return false;
+    tile = (int)(w / nch);
+    s0 = (int)(w % nch) * MOVE_SLOTS;
+    return true;
+}
+
+// row step j of lane (b, a): its element's slot s0 + r and plane kk; false
+// when the lane has no element of the range at that step
+CAP_HD bool move_step(const Lay &L, int s0, int ns, int b, int a, int j, int &r, int &kk)
+{
+    r = j - (TW - 1) + a;
+    if (r < 0 || r >= ns) return false;
+    kk = posmod(s0 + r - a - b, L.S);
+    return true;
+}
+
 // Decompose an element index into (ci, cj, kk) (ring columns included);
 // returns false for slots of tile columns beyond the solve box.
 CAP_HD bool decode(const Lay &L, size_t e, int &ci, int &cj, int &kk)

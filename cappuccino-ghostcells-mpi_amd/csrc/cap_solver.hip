@@ -28,10 +28,33 @@ using namespace cap;
 #define SPIN_LIMIT (1 << 20)
 
 // ============================================================ layout moves
+// One block per MOVE_SLOTS slots of a tile, staged through LDS (the schedule
+// is in cap_layout.h: move_grid, move_item, move_step).
+
+// reference -> tiled layout: every element (0 for tile columns beyond the
+// solve box), the ring columns included
 __global__ __launch_bounds__(BLOCK) void k_gather(Lay L, const double *__restrict__ src,
                                                   double *__restrict__ dst)
 {
-    for (size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x; e < L.n; e += (size_t)gridDim.x * BLOCK) {
+    __shared__ double buf[MOVE_SLOTS * TCOL];
+    int tile, s0;
+    if (move_item(L, blockIdx.x, tile, s0)) {
+        const int ti = tile / L.nTJ, tj = tile % L.nTJ;
+        const int lane = threadIdx.x, b = lane / TW, a = lane % TW;
+        const int ci = ti * TW + b, cj = tj * TW + a;
+        const bool valid = col_valid(L, ci, cj);
+        const int ns = min(MOVE_SLOTS, L.S - s0);
+        for (int j = 0; j < MOVE_SLOTS + TW - 1; ++j) {
+            int r, kk;
+            if (move_step(L, s0, ns, b, a, j, r, kk))
+                buf[r * TCOL + lane] = valid ? src[std_index(L, ci, cj, kk)] : 0.0;
+        }
+        __syncthreads();
+        const size_t tb = tile_base(L, ti, tj);
+        for (int r = 0; r < ns; ++r) dst[elem_at(tb, s0 + r, lane)] = buf[r * TCOL + lane];
+    }
+    for (size_t e = L.nmain + (size_t)blockIdx.x * BLOCK + threadIdx.x; e < L.n;
+         e += (size_t)gridDim.x * BLOCK) {
         int ci, cj, kk;
         double v = 0.0;
         if (decode(L, e, ci, cj, kk)) v = src[std_index(L, ci, cj, kk)];
@@ -43,9 +66,22 @@ __global__ __launch_bounds__(BLOCK) void k_gather(Lay L, const double *__restric
 __global__ __launch_bounds__(BLOCK) void k_scatter(Lay L, const double *__restrict__ src,
                                                    double *__restrict__ dst)
 {
-    for_cells(L, [&](const Cell &c) {
-        dst[std_index(L, c.ti * TW + c.b, c.tj * TW + c.a, c.kk)] = src[c.e];
-    });
+    __shared__ double buf[MOVE_SLOTS * TCOL];
+    int tile, s0;
+    if (!move_item(L, blockIdx.x, tile, s0)) return;
+    const int ti = tile / L.nTJ, tj = tile % L.nTJ;
+    const int lane = threadIdx.x, b = lane / TW, a = lane % TW;
+    const int ci = ti * TW + b, cj = tj * TW + a;
+    const int ns = min(MOVE_SLOTS, L.S - s0);
+    const size_t tb = tile_base(L, ti, tj);
+    for (int r = 0; r < ns; ++r) buf[r * TCOL + lane] = src[elem_at(tb, s0 + r, lane)];
+    __syncthreads();
+    if (!col_valid(L, ci, cj)) return;
+    for (int j = 0; j < MOVE_SLOTS + TW - 1; ++j) {
+        int r, kk;
+        if (move_step(L, s0, ns, b, a, j, r, kk) && kk >= 1 && kk <= L.nks)
+            dst[std_index(L, ci, cj, kk)] = buf[r * TCOL + lane];
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_fill(double *p, size_t n, double v)
@@ -812,25 +848,25 @@ static int prepare(cap_ctx *ctx, const Lay &L, double *fi, const cap_coef *c, So
         if (c->bw) HIPCHK(hipMemcpyAsync(ctx->S[S_BW], c->bw, ctx->nstd * 8, hipMemcpyHostToDevice, ctx->stream));
         if (c->bp) HIPCHK(hipMemcpyAsync(ctx->S[S_BP], c->bp, ctx->nstd * 8, hipMemcpyHostToDevice, ctx->stream));
     }
-    const int gb = grid_for(L);
+    const int gb = grid_for(L), gm = move_grid(L);
     hipStream_t s = ctx->stream;
     double *const *T = ctx->T;
     HIPCHK(hipEventRecord(ctx->ev0, s));
     HIPCHK(hipMemsetAsync(ctx->ticket, 0, 64, s));
-    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.fi, T[T_FI]);
+    hipLaunchKernelGGL(k_gather, dim3(gm), dim3(BLOCK), 0, s, L, (const double *)d.fi, T[T_FI]);
     const double *cs[8] = {d.ae, d.aw, d.an, d.as, d.at, d.ab, d.ap, d.su};
     const int ct[8] = {T_AE, T_AW, T_AN, T_AS, T_AT, T_AB, T_AP, T_SU};
     for (int v = 0; v < 8; ++v)
-        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, cs[v], T[ct[v]]);
+        hipLaunchKernelGGL(k_gather, dim3(gm), dim3(BLOCK), 0, s, L, cs[v], T[ct[v]]);
     // halo values of res and bn/be/bt are data for the recursions
     if (need_res_halo && d.res)
-        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.res, T[T_RES]);
+        hipLaunchKernelGGL(k_gather, dim3(gm), dim3(BLOCK), 0, s, L, (const double *)d.res, T[T_RES]);
     else
         hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_RES], L.n, 0.0);
     if (need_b_halo && d.bn) {
-        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.bn, T[T_BN]);
-        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.be, T[T_BE]);
-        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)d.bt, T[T_BT]);
+        hipLaunchKernelGGL(k_gather, dim3(gm), dim3(BLOCK), 0, s, L, (const double *)d.bn, T[T_BN]);
+        hipLaunchKernelGGL(k_gather, dim3(gm), dim3(BLOCK), 0, s, L, (const double *)d.be, T[T_BE]);
+        hipLaunchKernelGGL(k_gather, dim3(gm), dim3(BLOCK), 0, s, L, (const double *)d.bt, T[T_BT]);
     } else {
         hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_BN], L.n, 0.0);
         hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_BE], L.n, 0.0);
@@ -922,24 +958,24 @@ static void sip_bwd(cap_ctx *ctx, const Lay &L, double *z)
 static int writeback(cap_ctx *ctx, const Lay &L, double *fi_host, const cap_coef *c,
                      SolveIn &d, bool res_zero_first, bool factors)
 {
-    const int gb = grid_for(L);
+    const int gm = move_grid(L);
     hipStream_t s = ctx->stream;
     double *const *T = ctx->T;
     double *dsts[7] = {d.bb, d.bs, d.bw, d.bp, d.bn, d.be, d.bt};
     double *hs[7] = {c->bb, c->bs, c->bw, c->bp, c->bn, c->be, c->bt};
     const int ts[7] = {T_BB, T_BS, T_BW, T_BP, T_BN, T_BE, T_BT};
-    hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_FI], d.fi);
+    hipLaunchKernelGGL(k_scatter, dim3(gm), dim3(BLOCK), 0, s, L, (const double *)T[T_FI], d.fi);
     if (d.res) {
         if (res_zero_first) {
             int gs = (int)std::min<size_t>((ctx->nstd + BLOCK - 1) / BLOCK, 4096);
             hipLaunchKernelGGL(k_fill, dim3(gs), dim3(BLOCK), 0, s, d.res, ctx->nstd, 0.0);
         }
-        hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[T_RES], d.res);
+        hipLaunchKernelGGL(k_scatter, dim3(gm), dim3(BLOCK), 0, s, L, (const double *)T[T_RES], d.res);
     }
     if (factors)
         for (int v = 0; v < 7; ++v)
             if (dsts[v])
-                hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLOCK), 0, s, L, (const double *)T[ts[v]], dsts[v]);
+                hipLaunchKernelGGL(k_scatter, dim3(gm), dim3(BLOCK), 0, s, L, (const double *)T[ts[v]], dsts[v]);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev1, s));
     int rc = dev_out(ctx, fi_host, d.fi);

@@ -851,6 +851,26 @@ static int stage_n(cap_ctx *ctx, int slot, const double *p, int mult, const doub
     return CAP_OK;
 }
 
+double *cap_slot_buffer(cap_ctx *ctx, int slot)
+{
+    if (slot < 0 || slot >= CAP_NSTAGE) return nullptr;
+    const size_t bytes = ctx->nstd * sizeof(double);
+    if (ctx->Gsz[slot] < bytes) {
+        if (ctx->G[slot]) (void)hipFree(ctx->G[slot]);
+        ctx->G[slot] = nullptr;
+        ctx->Gsz[slot] = 0;
+        const hipError_t e = hipMalloc(&ctx->G[slot], bytes);
+        if (e != hipSuccess) {
+            ctx->G[slot] = nullptr;
+            snprintf(ctx->err, sizeof ctx->err, "hipMalloc of a staging array failed: %s",
+                     hipGetErrorString(e));
+            return nullptr;
+        }
+        ctx->Gsz[slot] = bytes;
+    }
+    return ctx->G[slot];
+}
+
 static int stage(cap_ctx *ctx, int slot, const double *p, const double **out)
 {
     return stage_n(ctx, slot, p, 1, out);

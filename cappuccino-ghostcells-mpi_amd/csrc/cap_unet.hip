@@ -1113,22 +1113,29 @@ extern "C" int cap_unet_train_step(cap_unet *n, const float *in, const float *ta
 
 // ---------------------------------------------------------- guess + solve
 namespace {
-int res_l1(cap_ctx *ctx, cap_unet *n, const cap_coef *c, const double *fi, double *out)
+// L1 residual of fi on the solve range (part: 2 doubles per block of the
+// grid_of(cells) grid, at most 32768)
+int res_l1_in(cap_ctx *ctx, double *part, const cap_coef *c, const double *fi, double *out)
 {
     const long cells = (long)(ctx->ni - 4) * (ctx->nj - 4) * (ctx->nk - 4);
     const int nb = grid_of(cells);
     hipLaunchKernelGGL(k_res_l1, dim3(nb), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj, ctx->nk,
                        (const double *)c->ae, (const double *)c->aw, (const double *)c->an,
                        (const double *)c->as, (const double *)c->at, (const double *)c->ab,
-                       (const double *)c->ap, (const double *)c->su, fi, n->resp);
+                       (const double *)c->ap, (const double *)c->su, fi, part);
     HCHK(hipGetLastError());
     std::vector<double> h(2 * nb);
-    HCHK(hipMemcpyAsync(h.data(), n->resp, 2 * nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipMemcpyAsync(h.data(), part, 2 * nb * 8, hipMemcpyDeviceToHost, ctx->stream));
     HCHK(hipStreamSynchronize(ctx->stream));
     long double s = 0.0L;
     for (int i = 0; i < 2 * nb; ++i) s += h[i];
     *out = (double)s;
     return CAP_OK;
+}
+
+int res_l1(cap_ctx *ctx, cap_unet *n, const cap_coef *c, const double *fi, double *out)
+{
+    return res_l1_in(ctx, n->resp, c, fi, out);
 }
 }  // namespace
 
@@ -1261,5 +1268,82 @@ extern "C" int cap_pcgsip_guess(cap_ctx *ctx, cap_unet *n, double *fi, const dou
     const double sor_eff = *accepted && rg > 0 ? sor * (r0 / rg) : sor;
     rc = cap_pcgsip(ctx, fi, c, alfa, sor_eff, nsw, rep);
     if (rc >= 0 && !*accepted && rep->status == CAP_OK) return CAP_GUESS_REJECTED;
+    return rc;
+}
+
+// ------------------------------------------------- PISO with a guessed start
+namespace {
+// dst = src on the solve range k, i, j = 3..n?mm
+__global__ void k_copy_range(int ni, int nj, int nk, const double *src, double *dst)
+{
+    const int nij = ni * nj;
+    const long n = (long)(ni - 4) * (nj - 4) * (nk - 4);
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int j = 3 + (int)(q % (nj - 4)), i = 3 + (int)((q / (nj - 4)) % (ni - 4)),
+                  k = 3 + (int)(q / ((long)(nj - 4) * (ni - 4)));
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        dst[c] = src[c];
+    }
+}
+
+struct GuessStart {
+    cap_unet *net;
+    const double *prev;
+    int accepted;
+    double l1_ref, l1_start;
+};
+
+// the start of the last corrector's p solve: the network's guess (with net)
+// or prev on the solve range (without), kept when it lowers the L1 residual
+// of the reference's start x; sor then scales by l1_ref / l1_start, so the
+// solve stops at the absolute residual the reference's start stops at
+int guess_start(void *user, cap_ctx *ctx, const cap_coef *c, double *x, double *sor)
+{
+    GuessStart *g = (GuessStart *)user;
+    double *keep = cap_slot_buffer(ctx, 46);
+    if (!keep) return CAP_ERR_NOMEM;
+    const size_t bytes = ctx->nstd * sizeof(double);
+    double r0 = 0.0, rg = 0.0;
+    UCHK(res_l1_in(ctx, ctx->partials, c, x, &r0));
+    HCHK(hipMemcpyAsync(keep, x, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    // the warm start: prev on the solve range, x's ghost values (the solve
+    // holds them fixed, and they enter the residual of the boundary cells)
+    const long cells = (long)(ctx->ni - 4) * (ctx->nj - 4) * (ctx->nk - 4);
+    if (g->net) {
+        double *pw = cap_slot_buffer(ctx, 45);
+        if (!pw) return CAP_ERR_NOMEM;
+        HCHK(hipMemcpyAsync(pw, x, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_copy_range, dim3(grid_of(cells)), dim3(256), 0, ctx->stream, ctx->ni,
+                           ctx->nj, ctx->nk, g->prev, pw);
+        HCHK(hipGetLastError());
+        UCHK(cap_unet_guess(ctx, g->net, x, pw, c, nullptr));
+    } else {
+        hipLaunchKernelGGL(k_copy_range, dim3(grid_of(cells)), dim3(256), 0, ctx->stream, ctx->ni,
+                           ctx->nj, ctx->nk, g->prev, x);
+        HCHK(hipGetLastError());
+    }
+    UCHK(res_l1_in(ctx, ctx->partials, c, x, &rg));
+    g->l1_ref = r0;
+    g->accepted = rg < r0;
+    g->l1_start = g->accepted ? rg : r0;
+    if (g->accepted) *sor = *sor * (r0 / rg);
+    else HCHK(hipMemcpyAsync(x, keep, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return CAP_OK;
+}
+}  // namespace
+
+extern "C" int cap_piso_guess(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple,
+                              cap_unet *net, const double *prev, cap_report *reps, double *cont,
+                              int *accepted, double *l1)
+{
+    if (!ctx || !prev || !accepted) return CAP_ERR_ARG;
+    if (!(ctx->flags & CAP_DEVICE_POINTERS)) return CAP_ERR_ARG;
+    if (net && (net->ctx != ctx || ctx->nj - 4 != net->nx || ctx->ni - 4 != net->ny ||
+                ctx->nk - 4 != net->nz))
+        return CAP_ERR_ARG;
+    GuessStart g = {net, prev, 0, 0.0, 0.0};
+    const int rc = cap_piso_run(ctx, f, o, pimple, reps, cont, guess_start, &g);
+    *accepted = g.accepted;
+    if (l1) { l1[0] = g.l1_ref; l1[1] = g.l1_start; }
     return rc;
 }

@@ -363,6 +363,18 @@ module cap_hip
       type(cap_report), intent(out) :: reps(*)
       real(c_double), intent(out) :: cont(*)
     end function
+    integer(c_int) function cap_piso_guess(ctx, f, o, pimple, net, prev, reps, cont, accepted, l1) &
+        bind(c, name='cap_piso_guess')
+      import :: c_ptr, c_int, c_double, cap_flow, cap_drv_opts, cap_report
+      type(c_ptr), value :: ctx, net, prev
+      type(cap_flow), intent(in) :: f
+      type(cap_drv_opts), intent(in) :: o
+      integer(c_int), value :: pimple
+      type(cap_report), intent(out) :: reps(*)
+      real(c_double), intent(out) :: cont(*)
+      integer(c_int), intent(out) :: accepted
+      real(c_double), intent(out) :: l1(2)
+    end function
     integer(c_int) function cap_outbc(ctx, f, o) bind(c, name='cap_outbc')
       import :: c_ptr, c_int, cap_flow, cap_drv_opts
       type(c_ptr), value :: ctx

@@ -42,7 +42,7 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya",
            "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les", "cap_courant_noalloc",
            "cap_face_areas", "cap_intfac", "cap_grid_dims", "cap_read_grid", "cap_cgstab",
-           "cap_pcg_jacobi", "cap_update_ghostcells_periodic")
+           "cap_pcg_jacobi", "cap_update_ghostcells_periodic", "cap_piso_guess")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -180,6 +180,8 @@ def load():
     L.cap_unet_guess.argtypes = [vp, vp, vp, vp, C.POINTER(_Coef), C.POINTER(d)]
     L.cap_pcgsip_guess.argtypes = [vp, vp, vp, vp, C.POINTER(_Coef), d, d, i, C.POINTER(_Report),
                                    C.POINTER(i)]
+    L.cap_piso_guess.argtypes = [vp, Fl, O, i, vp, vp, C.POINTER(_Report), C.POINTER(d),
+                                 C.POINTER(i), C.POINTER(d)]
     _lib = L
     return L
 
@@ -463,6 +465,27 @@ class Context:
         rc = self._lib.cap_piso(self._h, C.byref(st), C.byref(opts), 1 if pimple else 0, reps, cont)
         self._check(rc, "cap_piso")
         return self._reports(reps, n), [tuple(cont[3 * q:3 * q + 3]) for q in range(opts.ncorr)]
+
+    def piso_guess(self, f, opts, prev, net=None, pimple=False):
+        """piso with the last corrector's p solve started from the learned
+        guess (net: a UNet on this context) or from prev, the previous time
+        step's p, on the solve range (net None), when that start lowers the
+        L1 residual of the reference's start; the solve stops at the absolute
+        residual the reference's start stops at.  Returns (reports,
+        continuity, accepted, (L1 of the reference's start, of the start
+        used))."""
+        st = self._flow(f)
+        n = opts.ncorr * opts.npcor
+        reps = (_Report * n)()
+        cont = (C.c_double * (3 * opts.ncorr))()
+        acc = C.c_int()
+        l1 = (C.c_double * 2)()
+        rc = self._lib.cap_piso_guess(self._h, C.byref(st), C.byref(opts), 1 if pimple else 0,
+                                      net._h if net is not None else None, self._f(prev, "prev"),
+                                      reps, cont, C.byref(acc), l1)
+        self._check(rc, "cap_piso_guess")
+        return (self._reports(reps, n), [tuple(cont[3 * q:3 * q + 3]) for q in range(opts.ncorr)],
+                bool(acc.value), (l1[0], l1[1]))
 
     def outbc(self, f, opts):
         st = self._flow(f)

@@ -316,6 +316,21 @@ int cap_pcgsip_guess(cap_ctx *ctx, cap_unet *net, double *fi, const double *prev
                      const cap_coef *c, double alfa, double sor, int nsw, cap_report *rep,
                      int *accepted);
 
+/* No counterpart in the reference: cap_piso (PISO_multiple_correction.f90)
+ * with the last corrector's p solve (PISO_multiple_correction.f90:141, where
+ * the learned guess's training pairs arise) started from a guess instead of
+ * the previous corrector's p.  The warm start w is prev (the previous time
+ * step's p) on the solve range with the ghost values of the previous
+ * corrector's p, which the solve holds fixed.  With net NULL the start is w;
+ * with net it is cap_unet_guess's guess for the previous corrector's p and
+ * w.  It is kept (*accepted = 1) only when it lowers the L1 residual of
+ * the reference's start, and sor is then scaled by the ratio of the two, so
+ * the solve stops at the absolute residual the reference's start stops at.
+ * l1 (may be NULL) = {L1 residual of the reference's start, of the start
+ * used}.  Device pointers only. */
+int cap_piso_guess(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple, cap_unet *net,
+                   const double *prev, cap_report *reps, double *cont, int *accepted, double *l1);
+
 /* The Courant-number block included in the time loop (CourantNo.h:10-81,
  * ltransient branch): su = sum of |face mass fluxes| f1..f3 of each cell on
  * k = 2..nkm, i, j = 3..n?mm (0 elsewhere), conum = {CoNum, meanCoNum}; with

@@ -6,6 +6,7 @@
 // standing in for the workgroup's LDS exchange.  Used by tests to check the
 // schedule and the layout against the oracle bit for bit without a GPU.
 #include "../cappuccino-ghostcells-mpi_amd/csrc/cap_wave.h"
+#include <algorithm>
 #include <vector>
 #include <cstring>
 #include <stdexcept>
@@ -117,6 +118,84 @@ void emu_scatter(int ni, int nj, int nk, int klo, int khi, const double *t, doub
         if (e < L.nmain && decode(L, e, ci, cj, kk) && kk >= 1 && kk <= L.nks)
             std_[std_index(L, ci, cj, kk)] = t[e];
     }
+}
+
+// k_gather / k_scatter block by block as the kernels run them (the LDS buffer
+// of each block, every block's rows and slots); returns the number of
+// accesses outside the arrays and of tiled elements not written exactly once
+long emu_gather_blocks(int ni, int nj, int nk, int klo, int khi, const double *std_, double *t)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    const size_t nstd = (size_t)ni * nj * nk;
+    std::vector<int> writes(L.n, 0);
+    std::vector<double> buf(MOVE_SLOTS * TCOL);
+    long bad = 0;
+    const int G = move_grid(L);
+    for (int bid = 0; bid < G; ++bid) {
+        int tile, s0;
+        if (move_item(L, bid, tile, s0)) {
+            const int ti = tile / L.nTJ, tj = tile % L.nTJ, ns = std::min(MOVE_SLOTS, L.S - s0);
+            for (int lane = 0; lane < TCOL; ++lane) {
+                const int b = lane / TW, a = lane % TW, ci = ti * TW + b, cj = tj * TW + a;
+                const bool valid = col_valid(L, ci, cj);
+                for (int j = 0; j < MOVE_SLOTS + TW - 1; ++j) {
+                    int r, kk;
+                    if (!move_step(L, s0, ns, b, a, j, r, kk)) continue;
+                    const size_t q = std_index(L, ci, cj, kk);
+                    if (valid && q >= nstd) { ++bad; continue; }
+                    buf[r * TCOL + lane] = valid ? std_[q] : 0.0;
+                }
+            }
+            const size_t tb = tile_base(L, ti, tj);
+            for (int lane = 0; lane < TCOL; ++lane)
+                for (int r = 0; r < ns; ++r) {
+                    const size_t e = elem_at(tb, s0 + r, lane);
+                    if (e >= L.nmain) { ++bad; continue; }
+                    t[e] = buf[r * TCOL + lane];
+                    ++writes[e];
+                }
+        }
+        for (size_t e = L.nmain + (size_t)bid * 256; e < L.n; e += (size_t)G * 256)
+            for (size_t q = e; q < e + 256 && q < L.n; ++q) {
+                int ci, cj, kk;
+                t[q] = decode(L, q, ci, cj, kk) ? std_[std_index(L, ci, cj, kk)] : 0.0;
+                ++writes[q];
+            }
+    }
+    for (size_t e = 0; e < L.n; ++e) bad += writes[e] != 1;
+    return bad;
+}
+
+long emu_scatter_blocks(int ni, int nj, int nk, int klo, int khi, const double *t, double *std_)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    const size_t nstd = (size_t)ni * nj * nk;
+    std::vector<int> writes(nstd, 0);
+    std::vector<double> buf(MOVE_SLOTS * TCOL);
+    long bad = 0;
+    const int G = move_grid(L);
+    for (int bid = 0; bid < G; ++bid) {
+        int tile, s0;
+        if (!move_item(L, bid, tile, s0)) continue;
+        const int ti = tile / L.nTJ, tj = tile % L.nTJ, ns = std::min(MOVE_SLOTS, L.S - s0);
+        const size_t tb = tile_base(L, ti, tj);
+        for (int lane = 0; lane < TCOL; ++lane)
+            for (int r = 0; r < ns; ++r) buf[r * TCOL + lane] = t[elem_at(tb, s0 + r, lane)];
+        for (int lane = 0; lane < TCOL; ++lane) {
+            const int b = lane / TW, a = lane % TW, ci = ti * TW + b, cj = tj * TW + a;
+            if (!col_valid(L, ci, cj)) continue;
+            for (int j = 0; j < MOVE_SLOTS + TW - 1; ++j) {
+                int r, kk;
+                if (!move_step(L, s0, ns, b, a, j, r, kk) || kk < 1 || kk > L.nks) continue;
+                const size_t q = std_index(L, ci, cj, kk);
+                if (q >= nstd) { ++bad; continue; }
+                std_[q] = buf[r * TCOL + lane];
+                ++writes[q];
+            }
+        }
+    }
+    for (size_t q = 0; q < nstd; ++q) bad += writes[q] > 1;
+    return bad;
 }
 
 // op: 0 sip factor, 1 sip fwd, 2 sip bwd+dot, 3 ic diag, 4 ic fwd, 5 ic bwd+dot,

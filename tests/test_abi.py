@@ -25,6 +25,15 @@ def test_library_exports_every_declared_symbol(cap):
     assert not missing, missing
 
 
+def test_every_entry_point_has_a_prototype(cap):
+    """ctypes passes a Python int without a prototype as a 32-bit C int, which
+    would cut a 64-bit device or host address: every entry point the binding
+    calls must have its argument types declared"""
+    lib = cap.load()
+    missing = [s for s in cap.EXPORTS if getattr(lib, s).argtypes is None]
+    assert not missing, missing
+
+
 def test_fortran_binding_names_match(cap):
     fpath = os.path.join(ROOT, "cappuccino-ghostcells-mpi_amd", "fortran", "cap_hip.f90")
     if not os.path.exists(fpath):

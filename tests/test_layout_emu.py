@@ -144,3 +144,29 @@ def test_sipsol_sweep_with_res_halo(E, O):
 def test_flat_kernel_walk(E, dims, klo, khi, nblocks):
     # the flat kernels' per-block slot walk and wrap-based neighbour indices
     assert E.emu_check_walk(*dims, klo, khi, nblocks) == 0
+
+
+MOVE_CASES = CASES + [((68, 132, 100), 3, 98), ((196, 388, 62), 3, 60), ((21, 17, 40), 2, 39)]
+
+
+@pytest.mark.parametrize("dims,klo,khi", MOVE_CASES)
+def test_block_moves(E, dims, klo, khi):
+    """k_gather / k_scatter block by block (LDS rows of MOVE_SLOTS slots, the
+    neighbouring-range rows shared): the same tiled vector and the same
+    solve-range cells as the element-wise moves, every tiled element written
+    once, every access inside the arrays"""
+    E.emu_gather_blocks.restype = C.c_long
+    E.emu_scatter_blocks.restype = C.c_long
+    em = Emu(E, dims, klo, khi)
+    n = dims[0] * dims[1] * dims[2]
+    src = np.arange(1, n + 1, dtype=np.float64)
+    ref = em.gather(src)
+    got = np.full(em.n, -7.0)
+    assert E.emu_gather_blocks(*dims, klo, khi, P(src), P(got)) == 0
+    np.testing.assert_array_equal(got, ref)
+    base = np.full(n, -3.0)
+    t = np.random.default_rng(1).normal(size=em.n)
+    want = em.scatter(t, base)
+    back = base.copy()
+    assert E.emu_scatter_blocks(*dims, klo, khi, P(t), P(back)) == 0
+    np.testing.assert_array_equal(back, want)
