@@ -63,15 +63,24 @@ def main():
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
     ap.add_argument("--others",
                     default="cgstab_sip,iccg,cgstab,pcg_jacobi,mixed,learned,unet,channel,"
-                            "channel_mixed,host",
+                            "channel_mixed,channel_mixed_warm,learned_channel,channel_mixed_learned,host",
                     help="comma list of extra solves to time once each (learned: the U-Net "
                          "guess, trained on solver-generated 64^3 pairs, then the mixed and the "
                          "FP64 solve from it to the same absolute residual; unet: the learned "
                          "guess's forward pass and training step on 128x96x64; channel: time "
                          "steps (Courant, LES, calcuvw, PISO) of the 384x256x192 channel with "
                          "the FP64 p' solve; channel_mixed: the same with the mixed-precision "
-                         "p' solve; host: the same p' solve through host arrays, PCIe copies "
-                         "included)")
+                         "p' solve; channel_mixed_warm: the same with the last corrector "
+                         "started from the previous step's p; learned_channel: the learned "
+                         "guess trained on and evaluated against the previous step's p on the "
+                         "128x96x64 channel; channel_mixed_learned: the 384x256x192 channel with "
+                         "the mixed p' solve and the last corrector started from that network's "
+                         "guess; host: the same p' solve through host arrays, PCIe "
+                         "copies included)")
+    ap.add_argument("--c4-pairs", type=int, default=600,
+                    help="training pairs of the learned guess on the 128x96x64 channel")
+    ap.add_argument("--c4-epochs", type=int, default=10,
+                    help="training epochs over those pairs")
     ap.add_argument("--channel-steps", type=int, default=100,
                     help="time steps of the 384x256x192 channel timed with the mixed-precision "
                          "p' solve (the FP64 comparison runs a tenth of them)")
@@ -151,6 +160,7 @@ def main():
                                      "ms_per_iter": round(ms_step / max(iters, 1), 4),
                                      "effective_GBps": round(ITER_BYTES * ncell * iters /
                                                              (ms_step * 1e-3) / 1e9, 1)}}
+    c4_params = None
     for other in [o for o in args.others.split(",") if o]:
         fi.zero_()
         torch.cuda.synchronize()
@@ -177,6 +187,21 @@ def main():
             continue
         elif other == "channel_mixed":
             solvers["channel_mixed"] = _bench_channel(cap, torch, args.channel_steps, 6)
+            continue
+        elif other == "channel_mixed_warm":
+            solvers["channel_mixed_warm"] = _bench_channel(cap, torch, args.channel_steps, 6,
+                                                           start="warm")
+            continue
+        elif other == "learned_channel":
+            solvers["learned_channel"], c4_params = _bench_learned_channel(
+                cap, torch, args.c4_pairs, args.c4_epochs)
+            continue
+        elif other == "channel_mixed_learned":
+            if c4_params is None:
+                log("[bench] channel_mixed_learned needs learned_channel before it; skipped")
+                continue
+            solvers["channel_mixed_learned"] = _bench_channel(cap, torch, args.channel_steps, 6,
+                                                              start="learned", params=c4_params)
             continue
         elif other == "host":
             solvers["pcgsip_host_pointers"] = _bench_host(cap, torch, arrs, dims)
@@ -237,7 +262,14 @@ def main():
         "plain_vs_learned_mixed": {
             "plain_fp64_ms": round(ms_step, 2),
             "learned_mixed_ms": solvers.get("learned", {}).get("mixed", {}).get("ms"),
-            "mixed_ms": solvers.get("mixed", {}).get("ms")},
+            "mixed_ms": solvers.get("mixed", {}).get("ms"),
+            # the learned guess on its training distribution: ms per time step
+            # of the 384x256x192 channel by the start of the last corrector
+            "channel_ms_per_step": {
+                "fp64": solvers.get("channel_fp64", {}).get("ms_per_step"),
+                "mixed": solvers.get("channel_mixed", {}).get("ms_per_step"),
+                "mixed_warm": solvers.get("channel_mixed_warm", {}).get("ms_per_step"),
+                "mixed_learned": solvers.get("channel_mixed_learned", {}).get("ms_per_step")}},
         "solvers": solvers,
         "cpu_baseline": cpu,
     }
@@ -450,27 +482,46 @@ def _bench_unet(cap, torch, nx=128, ny=96, nz=64, reps=5):
             "note": "host-side wall time per call including launch and synchronisation"}
 
 
-def _bench_channel(cap, torch, steps, solver, size=(384, 256, 192)):
+def _channel_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "cap_channel", os.path.join(ROOT, "cappuccino-ghostcells-mpi_amd", "channel.py"))
+    CH = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(CH)
+    return CH
+
+
+def _bench_channel(cap, torch, steps, solver, size=(384, 256, 192), start="reference",
+                   params=None):
     """Time steps of the channel (cappuccino-ghostcells-mpi_amd/channel.py):
     384 x 256 x 192 cells, periodic x and z, walls at y (the reference's k),
     tanh stretching 2.0, Re_bulk 5600, Smagorinsky LES, dt from Co = 0.5,
     CDS momentum with BDF2, PISO with 2 correctors (0.1, then 1e-8) and the
     FP64 (solver 4) or mixed-precision (solver 6) p' solve; everything on the
-    device.  One untimed step first."""
-    import importlib.util
+    device.  start "warm": the last corrector's solve starts from the
+    previous step's p, "learned": from the learned guess with the network
+    weights `params`, each when it lowers the L1 residual of the reference's
+    start, and stops at the absolute residual that start stops at.  One
+    untimed step first."""
     import numpy as np
-    spec = importlib.util.spec_from_file_location(
-        "cap_channel", os.path.join(ROOT, "cappuccino-ghostcells-mpi_amd", "channel.py"))
-    CH = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(CH)
+    CH = _channel_module()
     nx, ny, nz = size
     ch = CH.Channel(cap, torch, nx, ny, nz, solver=solver)
-    ch.step(timed=False)
-    iters = []
+    warm = start == "warm"
+    net = None
+    if start == "learned":
+        ni, nj, nk = ch.dims
+        net = cap.UNet(ch.ctx, nj - 4, ni - 4, nk - 4, seed=1)
+        net.set_params(params)
+    ch.step(timed=False, warm=warm, net=net)
+    iters, accepted, l1_ratio = [], [], []
     t = time.perf_counter()
     for s in range(steps):
-        reps = ch.step()
+        reps = ch.step(warm=warm, net=net)
         iters.append([r["iters"] for r in reps])
+        if warm or net is not None:
+            accepted.append(ch.last["accepted"])
+            l1_ratio.append(ch.last["l1"][1] / ch.last["l1"][0])
         if (s + 1) % 20 == 0:
             log(f"[bench] channel step {s + 1}/{steps}")
     torch.cuda.synchronize()
@@ -483,14 +534,100 @@ def _bench_channel(cap, torch, steps, solver, size=(384, 256, 192)):
            "continuity_last": [float(x) for x in ch.last["cont"][-1]],
            "courant": ch.last["courant"], "dt": ch.last["dt"],
            "bulk_velocity": round(ch.bulk_velocity(), 6),
+           "start": start,
            "case": f"{nx}x{ny}x{nz} channel, periodic x and z, walls at y (reference k), tanh "
                    "2.0, Re_bulk 5600, Smagorinsky LES, Co 0.5, CDS + BDF2 momentum, PISO 2 "
                    "correctors, p' by " + ("mixed-precision PCG-SIP" if solver == 6 else
-                                           "PCG-SIP (alfa 0), FP64")}
-    log(f"[bench] channel (solver {solver}): {out['ms_per_step']} ms/step {per}, iterations "
-        f"{out['p_solve_iters_mean_per_corrector']}")
+                                           "PCG-SIP (alfa 0), FP64") +
+                   ("; last corrector from the previous step's p" if warm else "")}
+    if accepted:
+        out["start_accepted_fraction"] = round(float(np.mean(accepted)), 3)
+        out["l1_start_over_reference_mean"] = round(float(np.mean(l1_ratio)), 5)
+    log(f"[bench] channel (solver {solver}, {start} start): {out['ms_per_step']} ms/step {per}, "
+        f"iterations {out['p_solve_iters_mean_per_corrector']}")
+    if net is not None:
+        net.close()
     ch.close()
     return out
+
+
+def _bench_learned_channel(cap, torch, pairs, epochs, evals=8, warmup=20):
+    """The learned guess on its own training distribution: the 128 x 96 x 64
+    channel (the 384 x 256 x 192 case's family).  After `warmup` steps, the
+    FP64 PISO steps give `pairs` (inputs from the system of the last
+    corrector and the previous step's p, target its p) that train the U-Net
+    on the device.  Then, on `evals` held-out steps, each step is taken from
+    the same state with the last corrector's solve started three ways: the
+    reference's start (the previous corrector's p), the previous step's p
+    (warm), and the learned guess -- all stopping at the absolute residual
+    the reference's start stops at -- with the FP64 and the mixed-precision
+    solve.  Times are host wall clock around each step, synchronised.
+    Returns the report and the trained weights."""
+    import numpy as np
+    CH = _channel_module()
+    ch = CH.Channel(cap, torch, 128, 96, 64, solver=4)
+    t0 = time.perf_counter()
+    for _ in range(warmup):
+        ch.step(timed=False)
+    X, T = [], []
+    for _ in range(pairs):
+        prev = ch.f["p"].clone()
+        ch.step(timed=False)
+        x, t = ch.pressure_pair(prev)
+        X.append(x)
+        T.append(t)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    ni, nj, nk = ch.dims
+    net = cap.UNet(ch.ctx, nj - 4, ni - 4, nk - 4, seed=1)
+    g = torch.Generator().manual_seed(0)
+    t0 = time.perf_counter()
+    losses = []
+    for ep in range(epochs):
+        lr = 5e-4 if ep < int(0.7 * epochs) else 1e-4
+        losses.append(float(np.mean([net.train_step(X[q], T[q], lr)
+                                     for q in torch.randperm(pairs, generator=g).tolist()])))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del X, T
+    params = net.get_params()
+    log(f"[bench] learned channel: {pairs} pairs in {t_data:.1f}s, {epochs} epochs in "
+        f"{t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    res = {k: {"iters": [], "ms": [], "l1_start_over_ref": []}
+           for k in ("reference", "warm", "learned", "mixed_reference", "mixed_warm", "mixed_learned")}
+    for _ in range(evals):
+        snap = ch.snapshot()
+        for solver, pre in ((4, ""), (6, "mixed_")):
+            ch.opts.solver = solver
+            for kind in ("reference", "warm", "learned"):
+                ch.restore(snap)
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                reps = ch.step(timed=False, net=net if kind == "learned" else None,
+                               warm=kind == "warm")
+                torch.cuda.synchronize()
+                r = res[pre + kind]
+                r["ms"].append((time.perf_counter() - t) * 1e3)
+                r["iters"].append(reps[-1]["iters"])
+                if kind != "reference":
+                    l1 = ch.last["l1"]
+                    r["l1_start_over_ref"].append(l1[1] / l1[0])
+                    assert ch.last["accepted"]
+        ch.opts.solver = 4
+        ch.restore(snap)
+        ch.step(timed=False)
+    net.close()
+    ch.close()
+    out = {"grid": [128, 96, 64], "pairs": pairs, "epochs": epochs, "data_s": round(t_data, 1),
+           "train_s": round(t_train, 1), "loss_first_epoch": round(losses[0], 4),
+           "loss_last_epoch": round(losses[-1], 4), "eval_steps": evals}
+    for k, r in res.items():
+        out[k] = {"iters_mean": round(float(np.mean(r["iters"])), 2),
+                  "step_ms_mean": round(float(np.mean(r["ms"])), 2)}
+        if r["l1_start_over_ref"]:
+            out[k]["l1_start_over_reference"] = round(float(np.mean(r["l1_start_over_ref"])), 5)
+    log(f"[bench] learned channel: {json.dumps(out)}")
+    return out, params
 
 
 def _pmc_traffic(kernel, n):

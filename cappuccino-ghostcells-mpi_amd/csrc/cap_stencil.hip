@@ -32,69 +32,130 @@ struct Geo {
     const double *fx, *fy, *fz, *a1x, *a1y, *a1z, *a2x, *a2y, *a2z, *a3x, *a3y, *a3z, *vol;
 };
 
-// Green-Gauss gradient in gather form (GRAD_GAUSS.f90:37-102).  The
+// Green-Gauss gradient (GRAD_GAUSS.f90:37-102) in gather form.  The
 // reference scatters face terms in loop order (k, i, j); the value that
 // reaches cell P is ((((0 - t_B) - e_W) - n_S) + own faces), B = k-1,
 // W = i-1, S = j-1, each term present only when that face-loop iteration is
-// in the loop range k=2..nkm, i=2..nimm, j=2..njmm.  The gather below adds
-// the same terms in the same order, then scales by volr = 1/vol on
-// k=2..nkm, i,j=3..n?mm (88-102).
-__global__ __launch_bounds__(256) void k_grad_gauss(int ni, int nj, int nk,
-                                                    const double *__restrict__ u, Geo g,
-                                                    double *__restrict__ dx,
-                                                    double *__restrict__ dy,
-                                                    double *__restrict__ dz, int stride)
+// in the loop range k=2..nkm, i=2..nimm, j=2..njmm; then it is scaled by
+// volr = 1/vol on k=2..nkm, i,j=3..n?mm (88-102).
+//
+// A block of GG_J (j) x GG_I (i) threads, one per column, walks the planes
+// of one k range.  Each cell forms the products of its own east, north and
+// top faces once.  Its west and south neighbours' east and north products
+// come through LDS (the block's first row and column form them again from
+// memory), and its bottom face's products are the previous plane's top
+// products, kept in registers; the first plane of the range forms them
+// again.  Each product is the one the reference's face loop forms, and the
+// terms are added in the reference's order, so the result is the same bit
+// for bit, while each array is read once per plane.
+constexpr int GG_J = 64, GG_I = 4;
+
+__global__ __launch_bounds__(GG_J * GG_I) void k_grad_gauss(int ni, int nj, int nk,
+                                                              const double *__restrict__ u, Geo g,
+                                                              double *__restrict__ dx,
+                                                              double *__restrict__ dy,
+                                                              double *__restrict__ dz, int stride,
+                                                              int kchunk)
 {
+    __shared__ double le[2][3][GG_I][GG_J], ln[2][3][GG_I][GG_J];
+    const int tx = threadIdx.x % GG_J, ty = threadIdx.x / GG_J;
+    const int j = 1 + (int)blockIdx.x * GG_J + tx, i = 1 + (int)blockIdx.y * GG_I + ty;
+    const int k0 = 1 + (int)blockIdx.z * kchunk, k1 = min(nk, k0 + kchunk - 1);
+    const bool col = i <= ni && j <= nj;
     const int nij = ni * nj;
-    const size_t n = (size_t)nij * nk;
     const int nimm = ni - 2, njmm = nj - 2, nkm = nk - 1;
-    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
-        const int k = (int)(c / nij) + 1;
-        const int rem = (int)(c % nij);
-        const int i = rem / nj + 1, j = rem % nj + 1;
-        auto in_face = [&](int ii, int jj, int kk) {
-            return kk >= 2 && kk <= nkm && ii >= 2 && ii <= nimm && jj >= 2 && jj <= njmm;
-        };
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        if (in_face(i, j, k - 1)) {            // top face of B
-            const size_t b = c - nij;
-            const double ut = u[c] * g.fz[b] + u[b] * (1.0 - g.fz[b]);
-            sx = sx - (ut * g.a3x[b]);
-            sy = sy - (ut * g.a3y[b]);
-            sz = sz - (ut * g.a3z[b]);
+    auto in_face = [&](int ii, int jj, int kk) {
+        return kk >= 2 && kk <= nkm && ii >= 2 && ii <= nimm && jj >= 2 && jj <= njmm;
+    };
+    double tb[3] = {0.0, 0.0, 0.0};    // the previous plane's top products
+    for (int k = k0; k <= k1; ++k) {
+        const int par = k & 1;
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        double pe[3] = {0.0, 0.0, 0.0}, pn[3] = {0.0, 0.0, 0.0}, pt[3] = {0.0, 0.0, 0.0};
+        const bool own = col && in_face(i, j, k);
+        if (own) {
+            const double uc = u[c];
+            const double ue = u[c + nj] * g.fx[c] + uc * (1.0 - g.fx[c]);
+            const double un = u[c + 1] * g.fy[c] + uc * (1.0 - g.fy[c]);
+            const double ut = u[c + nij] * g.fz[c] + uc * (1.0 - g.fz[c]);
+            pe[0] = ue * g.a1x[c]; pe[1] = ue * g.a1y[c]; pe[2] = ue * g.a1z[c];
+            pn[0] = un * g.a2x[c]; pn[1] = un * g.a2y[c]; pn[2] = un * g.a2z[c];
+            pt[0] = ut * g.a3x[c]; pt[1] = ut * g.a3y[c]; pt[2] = ut * g.a3z[c];
         }
-        if (in_face(i - 1, j, k)) {            // east face of W
-            const size_t w = c - nj;
-            const double ue = u[c] * g.fx[w] + u[w] * (1.0 - g.fx[w]);
-            sx = sx - (ue * g.a1x[w]);
-            sy = sy - (ue * g.a1y[w]);
-            sz = sz - (ue * g.a1z[w]);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            le[par][q][ty][tx] = pe[q];
+            ln[par][q][ty][tx] = pn[q];
         }
-        if (in_face(i, j - 1, k)) {            // north face of S
-            const size_t s = c - 1;
-            const double un = u[c] * g.fy[s] + u[s] * (1.0 - g.fy[s]);
-            sx = sx - (un * g.a2x[s]);
-            sy = sy - (un * g.a2y[s]);
-            sz = sz - (un * g.a2z[s]);
+        __syncthreads();
+        if (col) {
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            if (in_face(i, j, k - 1)) {            // top face of B
+                if (k == k0) {
+                    const size_t b = c - nij;
+                    const double ut = u[c] * g.fz[b] + u[b] * (1.0 - g.fz[b]);
+                    tb[0] = ut * g.a3x[b]; tb[1] = ut * g.a3y[b]; tb[2] = ut * g.a3z[b];
+                }
+                sx = sx - tb[0];
+                sy = sy - tb[1];
+                sz = sz - tb[2];
+            }
+            if (in_face(i - 1, j, k)) {            // east face of W
+                double w3[3];
+                if (ty > 0) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) w3[q] = le[par][q][ty - 1][tx];
+                } else {
+                    const size_t w = c - nj;
+                    const double ue = u[c] * g.fx[w] + u[w] * (1.0 - g.fx[w]);
+                    w3[0] = ue * g.a1x[w]; w3[1] = ue * g.a1y[w]; w3[2] = ue * g.a1z[w];
+                }
+                sx = sx - w3[0];
+                sy = sy - w3[1];
+                sz = sz - w3[2];
+            }
+            if (in_face(i, j - 1, k)) {            // north face of S
+                double s3[3];
+                if (tx > 0) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) s3[q] = ln[par][q][ty][tx - 1];
+                } else {
+                    const size_t sc = c - 1;
+                    const double un = u[c] * g.fy[sc] + u[sc] * (1.0 - g.fy[sc]);
+                    s3[0] = un * g.a2x[sc]; s3[1] = un * g.a2y[sc]; s3[2] = un * g.a2z[sc];
+                }
+                sx = sx - s3[0];
+                sy = sy - s3[1];
+                sz = sz - s3[2];
+            }
+            if (own) {                             // own east/north/top faces
+                sx = sx + (pe[0] + pn[0] + pt[0]);
+                sy = sy + (pe[1] + pn[1] + pt[1]);
+                sz = sz + (pe[2] + pn[2] + pt[2]);
+            }
+            if (k >= 2 && k <= nkm && i >= 3 && i <= nimm && j >= 3 && j <= njmm) {
+                const double volr = 1.0 / g.vol[c];
+                sx = sx * volr;
+                sy = sy * volr;
+                sz = sz * volr;
+            }
+            dx[c * stride] = sx;
+            dy[c * stride] = sy;
+            dz[c * stride] = sz;
         }
-        if (in_face(i, j, k)) {                // own east/north/top faces
-            const double ue = u[c + nj] * g.fx[c] + u[c] * (1.0 - g.fx[c]);
-            const double un = u[c + 1] * g.fy[c] + u[c] * (1.0 - g.fy[c]);
-            const double ut = u[c + nij] * g.fz[c] + u[c] * (1.0 - g.fz[c]);
-            sx = sx + (ue * g.a1x[c] + un * g.a2x[c] + ut * g.a3x[c]);
-            sy = sy + (ue * g.a1y[c] + un * g.a2y[c] + ut * g.a3y[c]);
-            sz = sz + (ue * g.a1z[c] + un * g.a2z[c] + ut * g.a3z[c]);
-        }
-        if (k >= 2 && k <= nkm && i >= 3 && i <= nimm && j >= 3 && j <= njmm) {
-            const double volr = 1.0 / g.vol[c];
-            sx = sx * volr;
-            sy = sy * volr;
-            sz = sz * volr;
-        }
-        dx[c * stride] = sx;
-        dy[c * stride] = sy;
-        dz[c * stride] = sz;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) tb[q] = pt[q];
     }
+}
+
+// grid of k_grad_gauss: the columns in blocks, the planes in ranges so
+// that there are about 2048 blocks
+static void grad_gauss_grid(int ni, int nj, int nk, dim3 &grid, int &kchunk)
+{
+    const int bx = (nj + GG_J - 1) / GG_J, by = (ni + GG_I - 1) / GG_I;
+    const int nkc = std::max(1, std::min(nk, (2048 + bx * by - 1) / (bx * by)));
+    kchunk = (nk + nkc - 1) / nkc;
+    grid = dim3(bx, by, (nk + kchunk - 1) / kchunk);
 }
 
 // bpres.f90:29-79 -- one thread per ghost cell of the six planes
@@ -903,9 +964,11 @@ extern "C" int cap_grad_gauss(cap_ctx *ctx, const double *u, const double *const
         oy = ctx->G[15];
         zbuf = ctx->G[16];
     }
-    int blocks = (int)std::min<size_t>((ctx->nstd + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_grad_gauss, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj,
-                       ctx->nk, du, g, ox, oy, zbuf, 1);
+    dim3 grid;
+    int kchunk;
+    grad_gauss_grid(ctx->ni, ctx->nj, ctx->nk, grid, kchunk);
+    hipLaunchKernelGGL(k_grad_gauss, grid, dim3(GG_J * GG_I), 0, ctx->stream, ctx->ni, ctx->nj,
+                       ctx->nk, du, g, ox, oy, zbuf, 1, kchunk);
     HIPCHK(hipGetLastError());
     if (!(ctx->flags & CAP_DEVICE_POINTERS)) {
         HIPCHK(hipMemcpyAsync(dudx, ox, ctx->nstd * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -922,9 +985,11 @@ int cap_grad_gauss_aos(cap_ctx *ctx, const double *u, const double *const *geom,
 {
     Geo g = {geom[0], geom[1], geom[2], geom[3], geom[4], geom[5], geom[6], geom[7], geom[8],
              geom[9], geom[10], geom[11], geom[12]};
-    int blocks = (int)std::min<size_t>((ctx->nstd + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_grad_gauss, dim3(blocks), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj,
-                       ctx->nk, u, g, g3, g3 + 1, g3 + 2, 3);
+    dim3 grid;
+    int kchunk;
+    grad_gauss_grid(ctx->ni, ctx->nj, ctx->nk, grid, kchunk);
+    hipLaunchKernelGGL(k_grad_gauss, grid, dim3(GG_J * GG_I), 0, ctx->stream, ctx->ni, ctx->nj,
+                       ctx->nk, u, g, g3, g3 + 1, g3 + 2, 3, kchunk);
     HIPCHK(hipGetLastError());
     return CAP_OK;
 }

@@ -72,3 +72,70 @@ def test_channel_steps(gpu, solver):
         assert ch.last["cont"][-1][0] < 2e-3 * tot
     finally:
         ch.close()
+
+
+def test_warm_and_learned_starts(gpu):
+    """The last corrector's p solve (PISO_multiple_correction.f90:141) of
+    the 128 x 96 x 64 channel from three starts, on the same state: the
+    reference's (the previous corrector's p), the previous step's p (warm)
+    and the learned guess, the U-Net trained on pairs of the channel's own
+    earlier steps.  Each start is accepted only when it lowers the L1
+    residual; the solve then stops at the absolute residual the reference's
+    start stops at.  The warm start saves iterations; the network's guess
+    lowers the start's residual below the warm start's."""
+    import torch
+    CH = _channel_mod()
+    ch = CH.Channel(gpu, torch, 128, 96, 64, solver=4)
+    try:
+        for _ in range(10):
+            ch.step(timed=False)
+        X, T = [], []
+        for _ in range(300):
+            prev = ch.f["p"].clone()
+            ch.step(timed=False)
+            x, t = ch.pressure_pair(prev)
+            X.append(x)
+            T.append(t)
+        ni, nj, nk = ch.dims
+        net = gpu.UNet(ch.ctx, nj - 4, ni - 4, nk - 4, seed=1)
+        g = torch.Generator().manual_seed(0)
+        for ep in range(8):
+            lr = 5e-4 if ep < 6 else 1e-4
+            loss = np.mean([net.train_step(X[q], T[q], lr)
+                            for q in torch.randperm(len(X), generator=g).tolist()])
+        print(f"training loss, last epoch {loss:.4f}", flush=True)
+        del X, T
+        its = {"reference": [], "warm": [], "learned": []}
+        for s in range(3):
+            snap = ch.snapshot()
+            l1s = {}
+            for kind in its:
+                ch.restore(snap)
+                reps = ch.step(timed=False, net=net if kind == "learned" else None,
+                               warm=kind == "warm")
+                r = reps[-1]
+                assert r["converged"]
+                its[kind].append(r["iters"])
+                if kind == "reference":
+                    l1_ref = r["resor"]
+                    resl_ref = r["resl"]
+                else:
+                    assert ch.last["accepted"]
+                    ref, start = ch.last["l1"]
+                    assert ref == pytest.approx(l1_ref, rel=1e-9)
+                    l1s[kind] = start
+                    # the same absolute stopping point as the reference's start
+                    assert r["resl"] < 1e-8 * l1_ref * 1.01
+                    assert r["resor"] == pytest.approx(start, rel=1e-9)
+            print(f"step {s}: iterations {[its[k][-1] for k in its]}, L1 start / reference "
+                  f"warm {l1s['warm'] / l1_ref:.4f} learned {l1s['learned'] / l1_ref:.4f} "
+                  f"(reference stop {resl_ref:.3e})", flush=True)
+            assert l1s["warm"] < 0.2 * l1_ref
+            assert l1s["learned"] < l1s["warm"]
+            ch.restore(snap)
+            ch.step(timed=False)
+        assert np.mean(its["warm"]) < np.mean(its["reference"])
+        assert np.mean(its["learned"]) <= np.mean(its["warm"]) + 1
+        net.close()
+    finally:
+        ch.close()
