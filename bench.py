@@ -63,7 +63,8 @@ def main():
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
     ap.add_argument("--others",
                     default="cgstab_sip,iccg,cgstab,pcg_jacobi,mixed,learned,unet,channel,"
-                            "channel_mixed,channel_mixed_warm,learned_channel,channel_mixed_learned,host",
+                            "channel_mixed,channel_mixed_warm,learned_channel,channel_mixed_learned,"
+                            "channel_fp32_flux,host",
                     help="comma list of extra solves to time once each (learned: the U-Net "
                          "guess, trained on solver-generated 64^3 pairs, then the mixed and the "
                          "FP64 solve from it to the same absolute residual; unet: the learned "
@@ -75,8 +76,9 @@ def main():
                          "guess trained on and evaluated against the previous step's p on the "
                          "128x96x64 channel; channel_mixed_learned: the 384x256x192 channel with "
                          "the mixed p' solve and the last corrector started from that network's "
-                         "guess; host: the same p' solve through host arrays, PCIe "
-                         "copies included)")
+                         "guess; channel_fp32_flux: 10 steps of the 384x256x192 channel with the "
+                         "momentum fluxes' face arithmetic in FP32 against FP64; host: the same "
+                         "p' solve through host arrays, PCIe copies included)")
     ap.add_argument("--c4-pairs", type=int, default=600,
                     help="training pairs of the learned guess on the 128x96x64 channel")
     ap.add_argument("--c4-epochs", type=int, default=10,
@@ -192,6 +194,9 @@ def main():
             solvers["channel_mixed_warm"] = _bench_channel(cap, torch, args.channel_steps, 6,
                                                            start="warm")
             continue
+        elif other == "channel_fp32_flux":
+            solvers["channel_fp32_flux"] = _bench_fp32_flux(cap, torch, 10)
+            continue
         elif other == "learned_channel":
             solvers["learned_channel"], c4_params = _bench_learned_channel(
                 cap, torch, args.c4_pairs, args.c4_epochs)
@@ -214,7 +219,17 @@ def main():
         log(f"[bench] {other}: {dt:.1f} ms, iters {r['iters']}, conv {r['converged']}")
         if other == "mixed":
             solvers[other]["error_vs_fp64"] = _mixed_error(arrs, fi64, fi, dims)
-            log(f"[bench] mixed vs FP64: {json.dumps(solvers[other]['error_vs_fp64'])}")
+            # what the 1e-8 stop itself leaves: both solutions against an FP64
+            # solve to 1e-12 of the same system
+            tight = torch.zeros_like(fi)
+            rt = ctx.pcgsip(tight, coef, alfa=0.0, sor=1e-12, nsw=40000)
+            solvers[other]["error_vs_fp64_1e-12"] = _mixed_error(arrs, tight, fi, dims)
+            solvers[other]["fp64_error_vs_fp64_1e-12"] = _mixed_error(arrs, tight, fi64, dims)
+            solvers[other]["fp64_1e-12_iters"] = rt["iters"]
+            log(f"[bench] mixed vs FP64: {json.dumps(solvers[other]['error_vs_fp64'])}; against "
+                f"FP64 to 1e-12 ({rt['iters']} iterations): mixed "
+                f"{json.dumps(solvers[other]['error_vs_fp64_1e-12'])}, FP64 to 1e-8 "
+                f"{json.dumps(solvers[other]['fp64_error_vs_fp64_1e-12'])}")
 
     cpu = None
     if args.cpu_iters > 0:
@@ -548,6 +563,28 @@ def _bench_channel(cap, torch, steps, solver, size=(384, 256, 192), start="refer
     if net is not None:
         net.close()
     ch.close()
+    return out
+
+
+def _bench_fp32_flux(cap, torch, steps, size=(384, 256, 192)):
+    """The channel's momentum assembly with fluxuvw's face arithmetic in
+    FP32 (CAP_UVW_FP32) against FP64: `steps` steps of each from the same
+    start (mixed p' solve); calcuvw's ms per step and the largest velocity
+    difference at the end, relative to the bulk velocity (1)."""
+    CH = _channel_module()
+    res, vel = {}, []
+    for fp32 in (False, True):
+        ch = CH.Channel(cap, torch, *size, solver=6, fp32_flux=fp32)
+        ch.step(timed=False)
+        for _ in range(steps):
+            ch.step()
+        res["fp32" if fp32 else "fp64"] = {k: round(v, 2) for k, v in ch.phase_ms().items()}
+        vel.append({q: ch.f[q].clone() for q in ("u", "v", "w")})
+        ch.close()
+    diff = max(float((vel[0][q] - vel[1][q]).abs().max()) for q in ("u", "v", "w"))
+    out = {"grid": list(size), "steps": steps, "phases_ms_fp64_flux": res["fp64"],
+           "phases_ms_fp32_flux": res["fp32"], "max_velocity_difference": float(f"{diff:.3e}")}
+    log(f"[bench] fp32 momentum fluxes: {json.dumps(out)}")
     return out
 
 

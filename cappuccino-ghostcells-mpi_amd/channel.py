@@ -57,10 +57,11 @@ class Channel:
 
     cap: the loaded package; torch: the torch module (device memory);
     nx, ny, nz: interior cells (streamwise, wall-normal, spanwise);
-    solver: 4 (PCG-SIP, alfa 0) or 6 (its mixed-precision form)."""
+    solver: 4 (PCG-SIP, alfa 0) or 6 (its mixed-precision form);
+    fp32_flux: the momentum face fluxes' arithmetic in FP32 (CAP_UVW_FP32)."""
 
     def __init__(self, cap, torch, nx, ny, nz, stretch=2.0, re_bulk=5600.0, seed=7,
-                 solver=6, courant=0.5, lx=2 * math.pi, lz=math.pi, les=True):
+                 solver=6, courant=0.5, lx=2 * math.pi, lz=math.pi, les=True, fp32_flux=False):
         self.cap, self.torch = cap, torch
         self.nx, self.ny, self.nz = nx, ny, nz
         ni, nj, nk = nz + 4, nx + 4, ny + 4
@@ -135,7 +136,8 @@ class Channel:
         o.const_mflux, o.bdf, o.btime, o.densit = 1, 1, 1.0, 1.0
         self.opts = o
         q = cap.UvwOpts()
-        q.schemes, q.gds = 1, 1.0                          # CDS, full deferred correction
+        # CDS, full deferred correction (with CAP_UVW_FP32 = 1024: FP32 face arithmetic)
+        q.schemes, q.gds = 1 | (1024 if fp32_flux else 0), 1.0
         for m in range(3):
             q.urfr[m], q.urfm[m], q.sor[m], q.nsw[m] = 1.0, 0.0, float(np.float32(0.1)), 5
         # the laminar driving force d p / d x = -3 nu (for u = 1.5 (1 - y^2))

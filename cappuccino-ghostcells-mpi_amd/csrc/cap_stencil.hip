@@ -645,6 +645,81 @@ __global__ __launch_bounds__(256) void k_fluxuvw(int ni, int nj, int nk, Fld f, 
     }
 }
 
+// fluxuvw's CDS branch (the same faces and terms as k_fluxuvw with
+// lcds) with the face arithmetic in FP32: the geometric differences (xpn,
+// the face centre's offset from the interpolation point) are formed in FP64
+// and rounded once; velocities, gradients, viscosity and the mass flux are
+// rounded at the load; the face's contribution is added to su, sv, sw in FP64.
+__global__ __launch_bounds__(256) void k_fluxuvw32(int ni, int nj, int nk, Fld f, FaceDir D,
+                                                   UvwOut o, float gam, int kst, int ist, int jst)
+{
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const int NJ = njmm - jst + 1, NI = nimm - ist + 1, NK = nkmm - kst + 1;
+    const size_t nf = (size_t)NK * NI * NJ;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nf; q += (size_t)gridDim.x * 256) {
+        const int k = kst + (int)(q / ((size_t)NI * NJ));
+        const int r = (int)(q % ((size_t)NI * NJ));
+        const int i = ist + r / NJ, j = jst + r % NJ;
+        const size_t inp = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        const size_t ine = inp + D.idew, ins = inp - D.idns;
+        const size_t inb = inp - D.idtb, inbs = inb - D.idns;
+        const double fxe64 = D.fif[inp], fxp64 = 1.0 - fxe64;
+        const float fxe = (float)fxe64, fxp = 1.0f - fxe;
+        const float xpn = (float)(f.xc[ine] - f.xc[inp]), ypn = (float)(f.yc[ine] - f.yc[inp]),
+                    zpn = (float)(f.zc[ine] - f.zc[inp]);
+        const float dpn = sqrtf(xpn * xpn + ypn * ypn + zpn * zpn);
+        const float arx = (float)D.arvx[inp], ary = (float)D.arvy[inp], arz = (float)D.arvz[inp];
+        const float are = sqrtf(arx * arx + ary * ary + arz * arz);
+        const float vole = xpn * arx + ypn * ary + zpn * arz;
+        const float game = (float)o.vis[inp] * fxp + (float)o.vis[ine] * fxe;
+        const double xf = 0.25 * (f.x[inp] + f.x[ins] + f.x[inb] + f.x[inbs]);
+        const double yf = 0.25 * (f.y[inp] + f.y[ins] + f.y[inb] + f.y[inbs]);
+        const double zf = 0.25 * (f.z[inp] + f.z[ins] + f.z[inb] + f.z[inbs]);
+        const float dxf = (float)(xf - (f.xc[inp] * fxp64 + f.xc[ine] * fxe64));
+        const float dyf = (float)(yf - (f.yc[inp] * fxp64 + f.yc[ine] * fxe64));
+        const float dzf = (float)(zf - (f.zc[inp] * fxp64 + f.zc[ine] * fxe64));
+        const double *vel[3] = {f.u, f.v, f.w};
+        const double *grd[3] = {f.gu, f.gv, f.gw};
+        float dxi[3][3], dxii[3][3], vp[3], ve[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double *g = grd[c];
+            vp[c] = (float)vel[c][inp];
+            ve[c] = (float)vel[c][ine];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) dxi[c][m] = (float)G3(g, inp, m) * fxp + (float)G3(g, ine, m) * fxe;
+            const float dd = ve[c] - vp[c] - dxi[c][0] * xpn - dxi[c][1] * ypn - dxi[c][2] * zpn;
+            dxii[c][0] = dxi[c][0] + arx / vole * dd;
+            dxii[c][1] = dxi[c][1] + ary / vole * dd;
+            dxii[c][2] = dxi[c][2] + arz / vole * dd;
+        }
+        float fde[3], fdi[3];
+        fde[0] = game * ((dxii[0][0] + dxii[0][0]) * arx + (dxii[0][1] + dxii[1][0]) * ary +
+                         (dxii[0][2] + dxii[2][0]) * arz);
+        fde[1] = game * ((dxii[0][1] + dxii[1][0]) * arx + (dxii[1][1] + dxii[1][1]) * ary +
+                         (dxii[1][2] + dxii[2][1]) * arz);
+        fde[2] = game * ((dxii[0][2] + dxii[2][0]) * arx + (dxii[2][1] + dxii[1][2]) * ary +
+                         (dxii[2][2] + dxii[2][2]) * arz);
+        const float de = game * are / dpn;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fdi[c] = de * (dxi[c][0] * xpn + dxi[c][1] * ypn + dxi[c][2] * zpn);
+        const float flcf = (float)o.fcf[inp];
+        const float ce = flcf < 0.0f ? flcf : 0.0f, cp = flcf > 0.0f ? flcf : 0.0f;
+        o.acfe[inp] = (double)(de - ce);
+        o.acfw[ine] = (double)(de + cp);
+        double *srcs[3] = {o.su, o.sv, o.sw}, *scr[3] = {o.bp, o.bt, o.bb};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float fuds = cp * vp[c] + ce * ve[c];
+            const float fh = flcf * (vp[c] * fxp + ve[c] * fxe +
+                                     (dxi[c][0] * dxf + dxi[c][1] * dyf + dxi[c][2] * dzf));
+            const float t = gam * (fuds - fh) + fde[c] - fdi[c];
+            srcs[c][inp] = srcs[c][inp] + (double)t;
+            scr[c][ine] = (double)(-t);
+        }
+    }
+}
+
 // second loop (632-644): the east cell's share, k=2..nkmm, i=2..nimm, j=2..njmm
 __global__ __launch_bounds__(256) void k_fluxuvw_east(int ni, int nj, int nk, int idew, UvwOut o)
 {
@@ -1183,7 +1258,9 @@ extern "C" int cap_fluxuvw(cap_ctx *ctx, const cap_fields *F, const double *vis,
 {
     if (!ctx || !F || !vis || !arvx || !arvy || !arvz || !fif || !acfe || !acfw || !fcf || !su ||
         !sv || !sw || !bp || !bt || !bb || !dir_ok(ctx, idew, idns, idtb) ||
-        (schemes & ~CAP_UVW_ALL) || ctx->ni < 6 || ctx->nj < 6 || ctx->nk < 6)
+        (schemes & ~(CAP_UVW_ALL | CAP_UVW_FP32)) ||
+        ((schemes & CAP_UVW_FP32) && schemes != (CAP_UVW_FP32 | CAP_UVW_CDS)) || ctx->ni < 6 ||
+        ctx->nj < 6 || ctx->nk < 6)
         return CAP_ERR_ARG;
     Fld f;
     FaceDir D;
@@ -1203,8 +1280,12 @@ extern "C" int cap_fluxuvw(cap_ctx *ctx, const cap_fields *F, const double *vis,
     const int dir = idew == ctx->nj ? 1 : (idew == 1 ? 2 : 3);
     const int kst = dir == 3 ? 2 : 3, ist = dir == 1 ? 2 : 3, jst = dir == 2 ? 2 : 3;
     const size_t nf = (size_t)(ctx->nk - 2 - kst + 1) * (ctx->ni - 2 - ist + 1) * (ctx->nj - 2 - jst + 1);
-    hipLaunchKernelGGL(k_fluxuvw, dim3(blocks_for(nf)), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj,
-                       ctx->nk, f, D, o, schemes, gds, dir, kst, ist, jst);
+    if (schemes & CAP_UVW_FP32)
+        hipLaunchKernelGGL(k_fluxuvw32, dim3(blocks_for(nf)), dim3(256), 0, ctx->stream, ctx->ni,
+                           ctx->nj, ctx->nk, f, D, o, (float)gds, kst, ist, jst);
+    else
+        hipLaunchKernelGGL(k_fluxuvw, dim3(blocks_for(nf)), dim3(256), 0, ctx->stream, ctx->ni,
+                           ctx->nj, ctx->nk, f, D, o, schemes, gds, dir, kst, ist, jst);
     const size_t ne = (size_t)(ctx->nk - 3) * (ctx->ni - 3) * (ctx->nj - 3);
     hipLaunchKernelGGL(k_fluxuvw_east, dim3(blocks_for(ne)), dim3(256), 0, ctx->stream, ctx->ni,
                        ctx->nj, ctx->nk, idew, o);

@@ -397,6 +397,11 @@ int cap_update_ghostcells_periodic(cap_ctx *ctx, double *const *var, const doubl
 #define CAP_UVW_UMIST    256
 #define CAP_UVW_GAMMA    512
 #define CAP_UVW_ALL      1023
+/* No counterpart in the reference: with CAP_UVW_CDS alone, the face
+ * arithmetic of fluxuvw in FP32 (geometry differences formed in FP64 and
+ * rounded once; velocities, gradients, viscosity and mass flux rounded at
+ * the load) and each face's contribution added to su, sv, sw in FP64. */
+#define CAP_UVW_FP32     1024
 
 /* Replaces `call fluxuvw(idew, idns, idtb, arvx, arvy, arvz, fif, acfe, acfw,
  * fcf)` (fluxuvw-overrelaxed-correction.f90:3-659, called for the three

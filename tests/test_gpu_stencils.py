@@ -77,6 +77,51 @@ def test_fluxuvw_bitwise(gpu, O, dims, scheme):
                 np.testing.assert_array_equal(got[q], ref[q], err_msg=f"{scheme} {q} dir {d}")
 
 
+@pytest.mark.parametrize("dims", DIMS)
+def test_fluxuvw_fp32_close_to_fp64(gpu, dims):
+    """fluxuvw's CDS branch with its face arithmetic in FP32 (CAP_UVW_FP32)
+    against the FP64 kernel on the same fields.  The bound is set by the
+    fields' own conditioning: the FP64 kernel run on the inputs rounded to
+    FP32 moves each output by some amount (large where a random face's
+    xpn . S nearly vanishes); the FP32 kernel must stay within 16 times that
+    plus 1e-6 of the array's largest value.  Other schemes with
+    CAP_UVW_FP32 are refused."""
+    f, flds = fields_for(gpu, dims, 81)
+    cds, fp32, gds = MG.UVW_SCHEMES["cds"], 1024, 0.8
+    r32 = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731
+    flds_r = {q: r32(a) for q, a in flds.items()}
+
+    def run(fl, vis, fcf0, flags):
+        acfe, acfw, fcf, bp = [a.copy() for a in f["out0"]]
+        fcf[:] = fcf0
+        su, sv, sw = f["su"].copy(), f["sv"].copy(), f["sw"].copy()
+        bt, bb = f["bt0"].copy(), f["bb0"].copy()
+        ctx.fluxuvw(fl, d, vis, acfe, acfw, fcf, su, sv, sw, bp, bt, bb, flags, gds)
+        return dict(acfe=acfe, acfw=acfw, su=su - f["su"], sv=sv - f["sv"], sw=sw - f["sw"],
+                    bp=bp, bt=bt, bb=bb)
+    with gpu.Context(*dims) as ctx:
+        for d in range(3):
+            fcf0 = f["out0"][2]
+            ref = run(flds, f["vis"], fcf0, cds)
+            rnd = run(flds_r, r32(f["vis"]), r32(fcf0), cds)
+            got = run(flds, f["vis"], fcf0, cds | fp32)
+            for q in MG.UVW_OUT:
+                scale = np.max(np.abs(ref[q]))
+                if q in ("bp", "bt", "bb"):     # scratch, cleared after use
+                    assert scale == 0 and not np.any(got[q]), q
+                    continue
+                cond = np.max(np.abs(rnd[q] - ref[q]))
+                err = np.max(np.abs(got[q] - ref[q]))
+                print(f"dir {d} {q}: FP32 error {err / scale:.2e}, input-rounding effect "
+                      f"{cond / scale:.2e} (of the largest value)")
+                assert err <= 16 * cond + 1e-6 * scale, f"{q} dir {d}"
+        acfe, acfw, fcf, bp = [a.copy() for a in f["out0"]]
+        with pytest.raises(gpu.CapError):
+            ctx.fluxuvw(flds, 0, f["vis"], acfe, acfw, fcf, f["su"].copy(), f["sv"].copy(),
+                        f["sw"].copy(), bp, f["bt0"].copy(), f["bb0"].copy(),
+                        MG.UVW_SCHEMES["quick"] | fp32, gds)
+
+
 @pytest.mark.parametrize("dims", DIMS + [(36, 36, 36)])
 @pytest.mark.parametrize("types", ["mixed", "inlet", "outlet", "symmetry", "wall"])
 def test_ghostcells_bitwise(gpu, O, dims, types):
