@@ -62,10 +62,11 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=24,
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
     ap.add_argument("--others",
-                    default="cgstab_sip,iccg,cgstab,pcg_jacobi,mixed,learned,unet,channel,"
+                    default="cgstab_sip,iccg,cgstab,pcg_jacobi,mixed,fcg,learned,unet,channel,"
                             "channel_mixed,channel_mixed_warm,learned_channel,channel_mixed_learned,"
                             "channel_fp32_flux,host",
-                    help="comma list of extra solves to time once each (learned: the U-Net "
+                    help="comma list of extra solves to time once each (fcg: PCG-SIP with "
+                         "the preconditioner in FP32; learned: the U-Net "
                          "guess, trained on solver-generated 64^3 pairs, then the mixed and the "
                          "FP64 solve from it to the same absolute residual; unet: the learned "
                          "guess's forward pass and training step on 128x96x64; channel: time "
@@ -163,6 +164,7 @@ def main():
                                      "effective_GBps": round(ITER_BYTES * ncell * iters /
                                                              (ms_step * 1e-3) / 1e9, 1)}}
     c4_params = None
+    tight, tight_iters, fp64_err = None, None, None
     for other in [o for o in args.others.split(",") if o]:
         fi.zero_()
         torch.cuda.synchronize()
@@ -177,6 +179,8 @@ def main():
             r = ctx.pcg_jacobi(fi, coef, sor=1e-8, nsw=40000)
         elif other == "mixed":
             r = ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=3e-4)
+        elif other == "fcg":
+            r = ctx.pcgsip_fcg(fi, coef, alfa=0.0, sor=1e-8, nsw=20000)
         elif other == "learned":
             solvers["learned"] = _bench_learned(cap, torch, ctx, coef, arrs, dims, fi, args.seed,
                                                 rep["resor"], solvers.get("mixed"))
@@ -217,19 +221,22 @@ def main():
         dt = (time.perf_counter() - t) * 1e3
         solvers[other] = {"ms": round(dt, 2), "iters": r["iters"], "converged": r["converged"]}
         log(f"[bench] {other}: {dt:.1f} ms, iters {r['iters']}, conv {r['converged']}")
-        if other == "mixed":
+        if other in ("mixed", "fcg"):
             solvers[other]["error_vs_fp64"] = _mixed_error(arrs, fi64, fi, dims)
             # what the 1e-8 stop itself leaves: both solutions against an FP64
-            # solve to 1e-12 of the same system
-            tight = torch.zeros_like(fi)
-            rt = ctx.pcgsip(tight, coef, alfa=0.0, sor=1e-12, nsw=40000)
+            # solve to 1e-12 of the same system (solved once)
+            if tight is None:
+                tight = torch.zeros_like(fi)
+                rt = ctx.pcgsip(tight, coef, alfa=0.0, sor=1e-12, nsw=40000)
+                tight_iters = rt["iters"]
+                fp64_err = _mixed_error(arrs, tight, fi64, dims)
             solvers[other]["error_vs_fp64_1e-12"] = _mixed_error(arrs, tight, fi, dims)
-            solvers[other]["fp64_error_vs_fp64_1e-12"] = _mixed_error(arrs, tight, fi64, dims)
-            solvers[other]["fp64_1e-12_iters"] = rt["iters"]
-            log(f"[bench] mixed vs FP64: {json.dumps(solvers[other]['error_vs_fp64'])}; against "
-                f"FP64 to 1e-12 ({rt['iters']} iterations): mixed "
+            solvers[other]["fp64_error_vs_fp64_1e-12"] = fp64_err
+            solvers[other]["fp64_1e-12_iters"] = tight_iters
+            log(f"[bench] {other} vs FP64: {json.dumps(solvers[other]['error_vs_fp64'])}; against "
+                f"FP64 to 1e-12 ({tight_iters} iterations): {other} "
                 f"{json.dumps(solvers[other]['error_vs_fp64_1e-12'])}, FP64 to 1e-8 "
-                f"{json.dumps(solvers[other]['fp64_error_vs_fp64_1e-12'])}")
+                f"{json.dumps(fp64_err)}")
 
     cpu = None
     if args.cpu_iters > 0:

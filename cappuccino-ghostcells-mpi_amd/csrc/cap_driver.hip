@@ -491,6 +491,8 @@ int solve(cap_ctx *ctx, int solver, double *x, const cap_coef *c, const cap_drv_
     // refinement around FP32 PCG-SIP.  Its inner solves restart at each
     // refinement step, so they may take twice the FP64 iteration limit.
     case 6: return cap_pcgsip_mixed(ctx, x, c, o->alfa, sor, 2 * o->nsw, 3e-4, rep);
+    // 7, no counterpart either: pcgsip with its preconditioner in FP32
+    case 7: return cap_pcgsip_fcg(ctx, x, c, o->alfa, sor, o->nsw, rep);
     default: return CAP_ERR_ARG;
     }
 }
@@ -597,7 +599,7 @@ extern "C" int cap_calcp(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, cap_r
         !f->gradw || !f->apu || !f->apv || !f->apw || !f->den)
         return CAP_ERR_ARG;
     if (o->ncorr < 1 || o->npcor < 1 || o->ijkpr < 1 || (size_t)o->ijkpr > ctx->nstd) return CAP_ERR_ARG;
-    if (o->solver < 1 || o->solver > 6) return CAP_ERR_ARG;
+    if (o->solver < 1 || o->solver > 7) return CAP_ERR_ARG;
     const Dim d = dim_of(ctx);
     const size_t n = ctx->nstd;
     const cap_fields F = fields_of(f);
@@ -716,7 +718,8 @@ int cap_piso_run(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple, c
                 const int hr = hook(user, ctx, &C, x, &sor);
                 if (hr < 0) return hr;
             }
-            const int rc = solve(ctx, o->solver == 6 ? 6 : 4, x, &C, o, sor, rep);
+            const int rc = solve(ctx, o->solver == 6 || o->solver == 7 ? o->solver : 4, x, &C, o,
+                                 sor, rep);
             if (rc < 0) return rc;
             worst = std::max(worst, rc);
             RCHK(cap_bpres(ctx, x, f->fx, f->fy, f->fz));

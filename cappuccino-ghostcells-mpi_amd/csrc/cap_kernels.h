@@ -43,6 +43,7 @@ enum Fin {
     F_RAW,         // keep the sums in red[]
     F_BIC_RESL_BET,  // F_BIC_RESL on v[0]; unless that stops, F_BIC_BET on v[1]
     F_JAC_RESL_SK,   // F_PCG_RESL on v[0]; unless that stops, F_PCG_SK on v[1]
+    F_FCG_SK,        // flexible PCG: sk = (r, z), bet = ((r, z) - (r, z_old)) / s0
 };
 
 __device__ __forceinline__ void fin_resl(DevState *st, double resl)
@@ -104,6 +105,9 @@ __device__ __forceinline__ void finalize(DevState *st, const double *v)
             st->sk = v[1];
             st->bet = v[1] / st->s0;
         }
+    } else if constexpr (FIN == F_FCG_SK) {
+        st->sk = v[0];
+        st->bet = (v[0] - v[1]) / st->s0;
     } else if constexpr (FIN == F_BIC_RESL_BET) {
         // the stop test, then (when the solve goes on) the next iteration's
         // bet, om, beto, which its k_dot would compute from the same residual

@@ -1736,6 +1736,144 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
 }
 
 
+// ============================================ flexible PCG, FP32 preconditioner
+// The previous iteration's update of the flexible PCG (FP64): fi += alf p,
+// r -= alf w, sum|r| and the stop test (pcg-sip.f90:167-188), and the FP32
+// copy of r that the next preconditioner application reads.
+__global__ __launch_bounds__(BLOCK) void k_fcg_res(Lay L, double *__restrict__ fi,
+                                                   double *__restrict__ res,
+                                                   const double *__restrict__ p,
+                                                   const double *__restrict__ w,
+                                                   float *__restrict__ r32, DevState *st,
+                                                   double *partials, unsigned *ticket)
+{
+    if (st->done) return;
+    const double a = st->alf;
+    Acc acc[1];
+    for_cells(L, [&](const Cell &c) {
+        fi[c.e] = fi[c.e] + a * p[c.e];
+        const double r = res[c.e] - a * w[c.e];
+        res[c.e] = r;
+        r32[c.e] = (float)r;
+        acc[0] += fabs(r);
+    });
+    grid_reduce<1, F_PCG_RESL>(acc, blockIdx.x, gridDim.x, partials, ticket, st);
+}
+
+// p = z + bet p with z the FP32 preconditioned residual
+__global__ __launch_bounds__(BLOCK) void k_pupdate_z32(Lay L, const float *__restrict__ z,
+                                                       double *__restrict__ p, DevState *st)
+{
+    if (st->done) return;
+    const double bet = st->bet;
+    for_cells(L, [&](const Cell &c) { p[c.e] = (double)z[c.e] + bet * p[c.e]; });
+}
+
+// PCG-SIP (pcg-sip.f90) with the preconditioner applied in FP32: the SIP
+// factors come from the FP64 factorisation rounded to FP32, the forward and
+// backward sweeps run on the FP32 copy of the residual, and everything else
+// -- SpMV, residual, solution, scalar products, the stop test -- stays FP64.
+// The FP32 preconditioner is not exactly the same linear operator from one
+// iteration to the next, so beta is the flexible (Polak-Ribiere) form
+// ((r, z) - (r, z_old)) / (r_old, z_old), with z_old kept in a second FP32
+// vector; with an exact preconditioner (r, z_old) vanishes and this is the
+// reference's beta.  Unlike the refinement form (cap_pcgsip_mixed) the Krylov
+// space is never restarted, so the solution's error stays at the FP64 solve's.
+// A system that needs more than 4 (nis + njs + nks) iterations is too
+// ill-conditioned for the FP32 preconditioner (its rounding then stalls the
+// iteration): the solve goes on in FP64, cap_pcgsip from the current fi to
+// the same absolute residual.
+extern "C" int cap_pcgsip_fcg(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
+                              double sor, int nsw, cap_report *rep)
+{
+    if (!ctx) return CAP_ERR_ARG;
+    const Lay L = lay_for(ctx, 3, ctx->nk - 2);
+    SolveIn d;
+    int rc = prepare(ctx, L, fi, c, d, false, true);
+    if (rc) return rc;
+    if ((rc = ensure_f32(ctx, L.n))) return rc;
+    const int cap32 = 4 * (L.nis + L.njs + L.nks);
+    const int nsw32 = std::min(nsw, cap32);
+    if ((rc = init_state(ctx, sor, nsw32))) return rc;
+    sym_check(ctx, L);
+    double *const *T = ctx->T;
+    float *const *F = ctx->F;
+    const int gb = grid_for(L);
+    const int gl = (int)std::min<size_t>((L.n + BLOCK - 1) / BLOCK, 8192);
+    hipStream_t s = ctx->stream;
+    const Coef7 A = coef7(ctx);
+    hipLaunchKernelGGL(k_fill, dim3(gb), dim3(BLOCK), 0, s, T[T_PK], L.n, 0.0);
+    hipLaunchKernelGGL((k_resid<F_RES0>), dim3(gb), dim3(BLOCK), 0, s, L, A,
+                       (const double *)T[T_SU], (const double *)T[T_FI], T[T_RES], ctx->st,
+                       ctx->partials, ctx->ticket);
+    sip_factor(ctx, L, alfa);
+    const int c64[7] = {T_BB, T_BW, T_BS, T_BP, T_BN, T_BE, T_BT};
+    const int c32[7] = {F_BB, F_BW, F_BS, F_BP, F_BN, F_BE, F_BT};
+    for (int v = 0; v < 7; ++v)
+        hipLaunchKernelGGL(k_to_f32, dim3(gl), dim3(BLOCK), 0, s, (const double *)T[c64[v]], F[c32[v]], L.n);
+    hipLaunchKernelGGL(k_pack4<float>, dim3(gl), dim3(BLOCK), 0, s, L.n, (const float *)F[F_BB],
+                       (const float *)F[F_BW], (const float *)F[F_BS], (const float *)F[F_BP], ctx->pkf);
+    hipLaunchKernelGGL(k_to_f32, dim3(gl), dim3(BLOCK), 0, s, (const double *)T[T_RES], F[F_R], L.n);
+    hipLaunchKernelGGL(k_fillf, dim3(gl), dim3(BLOCK), 0, s, F[F_Z], L.n);
+    hipLaunchKernelGGL(k_fillf, dim3(gl), dim3(BLOCK), 0, s, F[F_X], L.n);
+    float *zn = F[F_Z], *zo = F[F_X];
+    int queued = 0;
+    auto one = [&]() {
+        if (queued++ > 0)
+            hipLaunchKernelGGL(k_fcg_res, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI], T[T_RES],
+                               (const double *)T[T_PK], (const double *)T[T_W], F[F_R], ctx->st,
+                               ctx->partials, ctx->ticket);
+        WaveIO<OpSipFwdT<float>> fw;
+        fw.in[0] = F[F_R]; fw.in[1] = F[F_BB]; fw.in[2] = F[F_BW]; fw.in[3] = F[F_BS];
+        fw.in[4] = F[F_BP];
+        fw.dep[0] = zn;
+        fw.xo[0] = nullptr;
+        fw.pk = ctx->pkf;
+        launch_wave<OpSipFwdT<float>, true, F_NONE>(ctx, L, OpSipFwdT<float>(), fw);
+        WaveIO<OpSipBwdDot2T<float>> bw;
+        bw.in[0] = zn; bw.in[1] = F[F_BN]; bw.in[2] = F[F_BE]; bw.in[3] = F[F_BT];
+        bw.in[4] = F[F_R]; bw.in[5] = zo;
+        bw.dep[0] = zn;
+        bw.xo[0] = nullptr;
+        launch_wave<OpSipBwdDot2T<float>, false, F_FCG_SK>(ctx, L, OpSipBwdDot2T<float>(), bw);
+        hipLaunchKernelGGL(k_pupdate_z32, dim3(gb), dim3(BLOCK), 0, s, L, (const float *)zn,
+                           T[T_PK], ctx->st);
+        hipLaunchKernelGGL((k_spmv<double, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L, A,
+                           (const double *)T[T_PK], T[T_W], (const double *)nullptr, ctx->st,
+                           ctx->partials, ctx->ticket, ctx->asym);
+        std::swap(zn, zo);
+    };
+    DevState h;
+    if ((rc = iterate(ctx, nsw32, one, h))) return rc;
+    if (queued > 0 && !h.err) {
+        // the loop ran out: the last iteration's update (a no-op when a
+        // residual update has stopped the solve)
+        hipLaunchKernelGGL(k_fcg_res, dim3(gb), dim3(BLOCK), 0, s, L, T[T_FI], T[T_RES],
+                           (const double *)T[T_PK], (const double *)T[T_W], F[F_R], ctx->st,
+                           ctx->partials, ctx->ticket);
+        HIPCHK(hipGetLastError());
+    }
+    if ((rc = read_state(ctx, h))) return rc;
+    if ((rc = writeback(ctx, L, fi, c, d, true, true))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    cap_report r32;
+    rc = finish_report(ctx, h, &r32, ms);
+    if (rc == CAP_MAXITER && nsw32 < nsw && r32.resl == r32.resl && r32.resl > 0.0) {
+        // too ill-conditioned for the FP32 preconditioner: FP64 from here
+        cap_report r64;
+        rc = cap_pcgsip(ctx, fi, c, alfa, sor * (r32.resor + (double)1e-30f) / r32.resl,
+                        nsw - h.l, &r64);
+        if (rc < 0) return rc;
+        r32.resl = r64.resl;
+        r32.iters = h.l + r64.iters;
+        r32.status = r64.status;
+        r32.ms += r64.ms;
+    }
+    if (rep) *rep = r32;
+    return r32.status;
+}
+
 // ================================================================ timing
 // Measurement hook (no counterpart in the reference).  Re-launches each
 // kernel of one PCG-SIP iteration `reps` times on the context's stream, on

@@ -211,6 +211,22 @@ struct OpSipBwdDotT {
 };
 using OpSipBwdDot = OpSipBwdDotT<double>;
 
+// The backward substitution of OpSipBwdDotT with a second product against
+// the previous preconditioned residual z_old, for the flexible PCG's
+// (Polak-Ribiere) beta.  x = {z, bn, be, bt, res, z_old}
+template <class V>
+struct OpSipBwdDot2T {
+    using T = V;
+    static constexpr int NIN = 6, ND = 1, NX = 0, NR = 2;
+    CAP_HD static void compute(const V *x, const V *dk, const V *di, const V *dj, V *d, V *,
+                               double *r)
+    {
+        d[0] = x[0] - x[1] * dj[0] - x[2] * di[0] - x[3] * dk[0];
+        r[0] = (double)x[4] * (double)d[0];
+        r[1] = (double)x[4] * (double)x[5];
+    }
+};
+
 // SIP backward substitution without a product, cgstab_sip.f90:148-149
 // x = {z, bn, be, bt}
 struct OpSipBwd {

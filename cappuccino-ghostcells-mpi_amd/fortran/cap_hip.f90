@@ -19,7 +19,7 @@ module cap_hip
   integer(c_int), parameter :: CAP_UVW_CDS = 1, CAP_UVW_CDS4 = 2, CAP_UVW_LUDS = 4, &
                                CAP_UVW_QUICK = 8, CAP_UVW_LIMITER = 16, CAP_UVW_SMART = 32, &
                                CAP_UVW_AVL = 64, CAP_UVW_MUSCL = 128, CAP_UVW_UMIST = 256, &
-                               CAP_UVW_GAMMA = 512
+                               CAP_UVW_GAMMA = 512, CAP_UVW_FP32 = 1024
 
   ! module `coef` + `coefb` arrays (modules_allocatable.f90:161-180)
   type, bind(c) :: cap_coef
@@ -180,6 +180,15 @@ module cap_hip
       type(c_ptr), value :: ctx, fi
       type(cap_coef), intent(in) :: c
       real(c_double), value :: alfa, sor, inner_rtol
+      integer(c_int), value :: nsw
+      type(cap_report), intent(out) :: rep
+    end function
+    integer(c_int) function cap_pcgsip_fcg(ctx, fi, c, alfa, sor, nsw, rep) &
+        bind(c, name='cap_pcgsip_fcg')
+      import :: c_ptr, c_int, c_double, cap_coef, cap_report
+      type(c_ptr), value :: ctx, fi
+      type(cap_coef), intent(in) :: c
+      real(c_double), value :: alfa, sor
       integer(c_int), value :: nsw
       type(cap_report), intent(out) :: rep
     end function

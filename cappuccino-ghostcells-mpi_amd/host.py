@@ -42,7 +42,8 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_calcp", "cap_piso", "cap_outbc", "cap_correct_bc", "cap_gethbya",
            "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les", "cap_courant_noalloc",
            "cap_face_areas", "cap_intfac", "cap_grid_dims", "cap_read_grid", "cap_cgstab",
-           "cap_pcg_jacobi", "cap_update_ghostcells_periodic", "cap_piso_guess")
+           "cap_pcg_jacobi", "cap_update_ghostcells_periodic", "cap_piso_guess",
+           "cap_pcgsip_fcg")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -138,6 +139,7 @@ def load():
     L.cap_cgstab.argtypes = [vp, vp, C.POINTER(_Coef), d, i, C.POINTER(_Report)]
     L.cap_pcg_jacobi.argtypes = [vp, vp, C.POINTER(_Coef), d, i, C.POINTER(_Report)]
     L.cap_pcgsip_mixed.argtypes = [vp, vp, C.POINTER(_Coef), d, d, i, d, C.POINTER(_Report)]
+    L.cap_pcgsip_fcg.argtypes = [vp, vp, C.POINTER(_Coef), d, d, i, C.POINTER(_Report)]
     L.cap_grad_gauss.argtypes = [vp, vp, C.POINTER(vp), vp, vp, vp]
     L.cap_bpres.argtypes = [vp, vp, vp, vp, vp]
     L.cap_build_lognormal.argtypes = [vp, C.c_ulonglong, d, C.POINTER(_Coef)]
@@ -327,6 +329,11 @@ class Context:
 
     def pcgsip_mixed(self, fi, coef, alfa=0.92, sor=1e-8, nsw=5000, inner_rtol=1e-3):
         return self._solve(self._lib.cap_pcgsip_mixed, fi, coef, alfa, sor, nsw, inner_rtol)
+
+    def pcgsip_fcg(self, fi, coef, alfa=0.92, sor=1e-8, nsw=1000):
+        """PCG-SIP with the preconditioner in FP32 (flexible beta), FP64
+        otherwise (cap_pcgsip_fcg)"""
+        return self._solve(self._lib.cap_pcgsip_fcg, fi, coef, alfa, sor, nsw)
 
     # --- stencils -----------------------------------------------------------
     def grad_gauss(self, u, geom, dudx, dudy, dudz):

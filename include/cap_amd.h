@@ -118,6 +118,20 @@ int cap_pcg_jacobi(cap_ctx *ctx, double *fi, const cap_coef *c, double sor, int 
 int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa,
                      double sor, int nsw, double inner_rtol, cap_report *rep);
 
+/* No counterpart in the reference: PCG-SIP with the preconditioner in FP32.
+ * The SIP factors (alfa as in cap_pcgsip) are formed in FP64 and rounded to
+ * FP32; the forward and backward sweeps run on an FP32 copy of the residual;
+ * the SpMV, residual, solution, scalar products and the stop test (L1
+ * residual relative to the start, as cap_pcgsip) stay FP64.  beta is the
+ * flexible (Polak-Ribiere) form, which the varying rounding of an FP32
+ * preconditioner needs.  The Krylov space is never restarted, so the
+ * solution's error stays at the FP64 solve's.  After 4 (nis + njs + nks)
+ * iterations without convergence (a system too ill-conditioned for the FP32
+ * preconditioner) the solve goes on as cap_pcgsip from the current fi, to
+ * the same absolute residual; the report counts both parts. */
+int cap_pcgsip_fcg(cap_ctx *ctx, double *fi, const cap_coef *c, double alfa, double sor,
+                   int nsw, cap_report *rep);
+
 /* Replaces `call grad_gauss(u, dudx, dudy, dudz)` (GRAD_GAUSS.f90:1-105).
  * geom holds fx, fy, fz, ar1x, ar1y, ar1z, ar2x, ar2y, ar2z, ar3x, ar3y,
  * ar3z, vol in this order (module `geometry`). */
@@ -189,7 +203,8 @@ typedef struct {
  * solver is calcp's isol: 1 sipsol, 2 cgstab, 3 iccg, 4 pcgsip, 5 cgstab_sip
  * (the reference's setting); PISO and PIMPLE use pcgsip.  solver 6 (no
  * counterpart in the reference) selects cap_pcgsip_mixed (inner_rtol 3e-4,
- * at most 2 nsw FP32 inner iterations) in all three drivers.  lbw..lbt are
+ * at most 2 nsw FP32 inner iterations) in all three drivers, solver 7 (no
+ * counterpart either) cap_pcgsip_fcg.  lbw..lbt are
  * device arrays indexed as in cap_bc (outbc reads them when lout is set). */
 typedef struct {
     int ncorr, npcor, solver, nsw;

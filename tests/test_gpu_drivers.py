@@ -119,23 +119,24 @@ def test_calcp_bitwise(gpu, O, solver, alfa, ncorr, npcor):
 # and the mixed solve ends in FP64.  Its first corrector stops at 0.1 of its
 # residual, where the two solvers stop at different points, so the fields
 # differ at that level.
+@pytest.mark.parametrize("solver", [6, 7])
 @pytest.mark.parametrize("kind,opts,bound", [("calcp", dict(ncorr=2, npcor=2), 1e-6),
                                              ("piso", dict(ncorr=1, npcor=1, bdf=1, nsw=2000), 1e-6),
                                              ("piso", dict(ncorr=2, npcor=1, bdf=1, nsw=2000), 5e-2)])
-def test_mixed_precision_drivers(gpu, O, kind, opts, bound):
-    """calcp and PISO with the mixed-precision p' solve (solver 6: FP32
-    PCG-SIP inside FP64 refinement) against the oracle's FP64 pcgsip.  Every
-    solve converges to the same stop test; the errors of p', p, the corrected
-    velocities and the face mass fluxes are printed and stay small (the p'
-    solves stop at 1e-8 of their initial L1 residual, or 0.1 in PISO's first
-    corrector)."""
+def test_mixed_precision_drivers(gpu, O, kind, opts, bound, solver):
+    """calcp and PISO with the mixed-precision p' solves (solver 6: FP32
+    PCG-SIP inside FP64 refinement; solver 7: PCG-SIP with the preconditioner
+    in FP32) against the oracle's FP64 pcgsip.  Every solve converges to the
+    same stop test; the errors of p', p, the corrected velocities and the
+    face mass fluxes are printed and stay small (the p' solves stop at 1e-8
+    of their initial L1 residual, or 0.1 in PISO's first corrector)."""
     kw = dict(solver=4, alfa=0.0, nsw=300)
     kw.update(opts)
-    fo, fg, ro, rg = run_pair(gpu, O, DIMS, 13, kind, gpu_over=dict(solver=6), **kw)
+    fo, fg, ro, rg = run_pair(gpu, O, DIMS, 13, kind, gpu_over=dict(solver=solver), **kw)
     errs = {}
     for q in ("pp", "p", "u", "v", "w", "f1", "f2", "f3"):
         errs[q] = float(np.max(np.abs(fg[q] - fo[q])) / max(np.max(np.abs(fo[q])), 1e-300))
-    print(f"{kind} {opts} mixed vs fp64: iterations {[r['iters'] for r in rg[0]]} vs "
+    print(f"{kind} {opts} solver {solver} vs fp64: iterations {[r['iters'] for r in rg[0]]} vs "
           f"{[r['iters'] for r in ro[0]]}; max rel errors " +
           ", ".join(f"{q} {e:.1e}" for q, e in errs.items()), flush=True)
     assert all(r["converged"] for r in ro[0]) and all(r["converged"] for r in rg[0])

@@ -282,6 +282,36 @@ def test_mixed_precision(gpu, O, dims):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("dims", [(36, 36, 36), (52, 70, 30)])
+def test_flexible_pcg_fp32_preconditioner(gpu, O, dims):
+    """PCG-SIP with its preconditioner in FP32 and a flexible beta
+    (cap_pcgsip_fcg): it stops on the reference's test with the true FP64
+    residual below it, takes about the FP64 iteration count (within 10 %),
+    and its solution is as close to the converged solution (the oracle's
+    FP64 solve to 1e-13) as the FP64 solve to the same 1e-8 is, give or take
+    a factor 10: no restarts, so no smooth error is left behind."""
+    s, fi0 = lognormal_system(O, dims, 31)
+    s_ref, fi_ref = s.copy(), fi0.copy()
+    rep_ref = O.pcgsip(s_ref, fi_ref, alfa=0.0, sor=1e-8, nsw=2000)
+    s_t, fi_t = s.copy(), fi0.copy()
+    O.pcgsip(s_t, fi_t, alfa=0.0, sor=1e-13, nsw=6000)
+    coef, fi = coef_of(gpu, s), fi0.copy()
+    with gpu.Context(*dims) as ctx:
+        rep = ctx.pcgsip_fcg(fi, coef, alfa=0.0, sor=1e-8, nsw=4000)
+    mask = np.zeros(s.n, bool)
+    mask.reshape(dims[2], dims[0], dims[1])[2:-2, 2:-2, 2:-2] = True
+    r = (s.su - O.spmv(s, 3, dims[2] - 2, fi))[mask]
+    ratio = np.sum(np.abs(r)) / np.sum(np.abs(s.su[mask]))
+    err = rel(fi[mask], fi_t[mask])
+    err64 = rel(fi_ref[mask], fi_t[mask])
+    print(f"fcg {dims}: {rep['iters']} iterations (fp64 pcg {rep_ref['iters']}), true L1 ratio "
+          f"{ratio:.2e}, max error vs the converged solution {err:.2e} (FP64 to 1e-8: {err64:.2e})")
+    assert rep["converged"]
+    assert ratio < 1.01e-8
+    assert abs(rep["iters"] - rep_ref["iters"]) <= 0.1 * rep_ref["iters"]
+    assert err <= 10 * err64 + 1e-12
+
+
 def random_geom(dims, seed):
     rng = np.random.default_rng(seed)
     n = dims[0] * dims[1] * dims[2]
@@ -350,6 +380,7 @@ def test_full_size_256_properties(gpu):
         cases = (("pcgsip", 2, 2e-8, lambda: ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=4000)),
                  ("mixed", 2, 2e-8, lambda: ctx.pcgsip_mixed(fi, coef, alfa=0.0, sor=1e-8, nsw=8000,
                                                            inner_rtol=3e-4)),
+                 ("fcg", 2, 2e-8, lambda: ctx.pcgsip_fcg(fi, coef, alfa=0.0, sor=1e-8, nsw=4000)),
                  ("iccg", 1, 2e-8, lambda: ctx.iccg(fi, coef, sor=1e-8, nsw=4000)),
                  ("cgstab_sip", 1, 1e-7, lambda: ctx.cgstab_sip(fi, coef, alfa=0.92, sor=1e-8, nsw=4000)))
         for name, kr, bound, solve in cases:
