@@ -308,4 +308,56 @@ __device__ __forceinline__ void for_cells(const Lay &L, F &&f)
     }
 }
 
+// for_cells with the memory access of each cell split from its arithmetic:
+// load(cell, valid) reads what the cell needs into a value of type T (a lane
+// with no cell at that slot, valid false, reads only its own element, which
+// is always in the vector), compute(cell, values) does the rest.  The walk
+// issues the loads of the lane's next slot before it computes the current
+// one, alternating two value buffers (no copy between them, so no wait for a
+// load before its cell is computed), and each wave keeps two slots' loads in
+// flight.  The cells are computed in for_cells' order.
+template <class T, class LD, class CP>
+__device__ __forceinline__ void for_cells_pipelined(const Lay &L, LD &&load, CP &&compute)
+{
+    const int c = blockIdx.x & 7, G = (int)(gridDim.x >> 3);
+    const int s0 = c * L.S / 8, s1 = (c + 1) * L.S / 8;
+    if (G == 0 || s0 >= s1) return;
+    const int lane = threadIdx.x;
+    for (int t = (int)(blockIdx.x >> 3); t < L.ntiles; t += G) {
+        const size_t lo = (size_t)t * L.S + s0;
+        SlotWalk w;
+        w.start(L, lo, lane);
+        // the cell at slot s of the walk (s <= s1: past the range it is the
+        // tile's next slot, or slot 0 of the next tile or ring, still an
+        // element of the vector)
+        auto cell_here = [&](int s) {
+            Cell cl;
+            cl.e = elem_ts(lo + (size_t)(s - s0), w.slot, lane);
+            cl.ti = w.ti; cl.tj = w.tj; cl.slot = w.slot; cl.b = w.b; cl.a = w.a; cl.kk = w.kk;
+            cl.in = true;
+            return cl;
+        };
+        Cell ca = cell_here(s0);
+        bool ha = w.cell(L);
+        T va = load(ca, ha);
+        w.next(L);
+        for (int s = s0 + 1;; s += 2) {
+            const bool in_b = s < s1;
+            const Cell cb = cell_here(s);
+            const bool hb = in_b && w.cell(L);
+            const T vb = load(cb, hb);
+            if (in_b) w.next(L);
+            if (ha) compute(ca, va);
+            if (!in_b) break;
+            const bool in_a = s + 1 < s1;
+            ca = cell_here(s + 1);
+            ha = in_a && w.cell(L);
+            va = load(ca, ha);
+            if (in_a) w.next(L);
+            if (hb) compute(cb, vb);
+            if (!in_a) break;
+        }
+    }
+}
+
 }  // namespace cap

@@ -151,20 +151,31 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(Lay L, Coef7T<V> A, const V *__r
 {
     if (st->done) return;
     Acc acc[MODE == 2 ? 2 : 1];
+    // a cell's operands: p of the cell and its six neighbours, the seven
+    // coefficients, y (MODE 1 and 2)
+    struct Ops { V pc, pe, pw, pn, ps, pt, pb, ap, ae, aw, an, as, at, ab, y; };
     auto run = [&](auto symc) {
         constexpr bool SYM = decltype(symc)::value;
-        for_cells(L, [&](const Cell &c) {
-            Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
+        for_cells_pipelined<Ops>(L, [&](const Cell &c, bool valid) {
             const size_t e = c.e;
-            const V aw = SYM ? A.ae[nb.w] : A.aw[e];
-            const V as = SYM ? A.an[nb.s] : A.as[e];
-            const V ab = SYM ? A.at[nb.bt] : A.ab[e];
-            V v = A.ap[e] * p[e] - A.ae[e] * p[nb.e] - aw * p[nb.w] - A.an[e] * p[nb.n] -
-                  as * p[nb.s] - A.at[e] * p[nb.t] - ab * p[nb.bt];
-            z[e] = v;
-            if constexpr (MODE == 0) acc[0] += (double)p[e] * (double)v;
-            if constexpr (MODE == 1) acc[0] += (double)v * (double)y[e];
-            if constexpr (MODE == 2) { acc[0] += (double)v * (double)y[e]; acc[1] += (double)v * (double)v; }
+            const Nb nb = valid ? neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk)
+                                : Nb{e, e, e, e, e, e};
+            Ops o;
+            o.pc = p[e]; o.pe = p[nb.e]; o.pw = p[nb.w]; o.pn = p[nb.n]; o.ps = p[nb.s];
+            o.pt = p[nb.t]; o.pb = p[nb.bt];
+            o.ap = A.ap[e]; o.ae = A.ae[e]; o.an = A.an[e]; o.at = A.at[e];
+            o.aw = SYM ? A.ae[nb.w] : A.aw[e];
+            o.as = SYM ? A.an[nb.s] : A.as[e];
+            o.ab = SYM ? A.at[nb.bt] : A.ab[e];
+            o.y = MODE == 0 ? V(0) : y[e];
+            return o;
+        }, [&](const Cell &c, const Ops &o) {
+            V v = o.ap * o.pc - o.ae * o.pe - o.aw * o.pw - o.an * o.pn - o.as * o.ps -
+                  o.at * o.pt - o.ab * o.pb;
+            z[c.e] = v;
+            if constexpr (MODE == 0) acc[0] += (double)o.pc * (double)v;
+            if constexpr (MODE == 1) acc[0] += (double)v * (double)o.y;
+            if constexpr (MODE == 2) { acc[0] += (double)v * (double)o.y; acc[1] += (double)v * (double)v; }
         });
     };
     if (asym && asym[blockIdx.x] == 0)

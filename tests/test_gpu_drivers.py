@@ -143,6 +143,58 @@ def test_mixed_precision_drivers(gpu, O, kind, opts, bound, solver):
     assert max(errs.values()) < bound, errs
 
 
+@pytest.mark.parametrize("solver", ["mixed", "fcg"])
+def test_fp64_finish_of_the_mixed_solves(gpu, O, solver):
+    """A log-normal p' system whose face coefficients span e^(+-15) (the
+    benchmark's family at sigma 5, 20^3 cells): FP32 cannot take it to 1e-8,
+    so both mixed-precision solves hand the rest over to the FP64 PCG-SIP from
+    their current fi.  From fi = 0: the solve converges; the FP64 L1 residual
+    of the returned fi, recomputed on the CPU, is at the requested level and
+    close to the report's resl (the solvers stop on their recursively updated
+    residual, as the FP64 solve does, and may drift from the true one by twice
+    what the FP64 solve's does); the report's resor is res0; and the report
+    counts both phases: more iterations than the flexible form's FP32 part may
+    take (4 (nis + njs + nks)), more than FP64 alone for the refinement form,
+    whose FP32 cycles come first."""
+    dims, sigma = (24, 24, 24), 5.0
+    s = O.build_lognormal(dims, 11, sigma)
+    ni, nj, nk = dims
+
+    def system():
+        return gpu.Coef(**{q: getattr(s, q).copy() for q in
+                           ("ae", "aw", "an", "as_", "at", "ab", "ap", "su")})
+
+    mask = np.zeros(s.n, bool)
+    mask.reshape(nk, ni, nj)[2:-2, 2:-2, 2:-2] = True
+
+    def l1(x):
+        return float(np.abs((s.su - O.spmv(s, 3, nk - 2, x))[mask]).sum())
+    res0 = l1(np.zeros(s.n))
+    with gpu.Context(*dims) as ctx:
+        fi64 = np.zeros(s.n)
+        r64 = ctx.pcgsip(fi64, system(), alfa=0.0, sor=1e-8, nsw=20000)
+        fi = np.zeros(s.n)
+        if solver == "mixed":
+            rep = ctx.pcgsip_mixed(fi, system(), alfa=0.0, sor=1e-8, nsw=40000, inner_rtol=3e-4)
+        else:
+            rep = ctx.pcgsip_fcg(fi, system(), alfa=0.0, sor=1e-8, nsw=20000)
+    true, true64 = l1(fi), l1(fi64)
+    cap32 = 4 * 3 * (ni - 4)
+    print(f"sigma {sigma} {solver}: {rep['iters']} iterations (FP64 alone {r64['iters']}), resor "
+          f"{rep['resor']:.6e} (CPU {res0:.6e}), resl {rep['resl']:.6e} (CPU {true:.6e}; FP64 "
+          f"alone: reported {r64['resl']:.6e}, CPU {true64:.6e})", flush=True)
+    assert r64["converged"] and rep["converged"]
+    assert rep["resor"] == pytest.approx(res0, rel=1e-9)
+    assert rep["resl"] <= 1e-8 * rep["resor"]
+    # at this conditioning the recursively updated residual drifts from the
+    # true one by about a tenth in FP64 alone; the mixed solves may drift as
+    # much again, no more
+    drift = max(0.05, 2 * abs(true64 - r64["resl"]) / r64["resl"])
+    assert abs(true - rep["resl"]) <= drift * rep["resl"]
+    assert true <= (1 + drift) * 1e-8 * res0
+    assert rep["iters"] > (cap32 if solver == "fcg" else r64["iters"])
+
+
 def test_calcp_with_outlet(gpu, O):
     fo, fg, ro, rg = run_pair(gpu, O, DIMS, 12, "calcp", outlet=True, solver=5, alfa=0.92)
     check_reports(ro, rg, "calcp outlet")
