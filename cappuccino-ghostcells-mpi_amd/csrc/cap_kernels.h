@@ -308,14 +308,45 @@ __device__ __forceinline__ void for_cells(const Lay &L, F &&f)
     }
 }
 
-// for_cells with the memory access of each cell split from its arithmetic:
-// load(cell, valid) reads what the cell needs into a value of type T (a lane
-// with no cell at that slot, valid false, reads only its own element, which
-// is always in the vector), compute(cell, values) does the rest.  The walk
-// issues the loads of the lane's next slot before it computes the current
-// one, alternating two value buffers (no copy between them, so no wait for a
-// load before its cell is computed), and each wave keeps two slots' loads in
-// flight.  The cells are computed in for_cells' order.
+// The SpMV's walk: blocks as in for_cells (block b, eighth b % 8 of the
+// slots of tiles b/8, b/8 + G, ...), each tile's range rotated by
+// skew_slot0 (TileWalk), so that the edge rows a stencil reads from the
+// neighbouring tiles are the ones those tiles' blocks work on at the same
+// step and are still in the XCD's L2.  A block visits the same cells as in
+// for_cells, in another order, so kernels that share per-block results (the
+// symmetry flags of k_symcheck) must both walk this way.
+template <class F>
+__device__ __forceinline__ void for_cells_skewed(const Lay &L, F &&f)
+{
+    const int c = blockIdx.x & 7, G = (int)(gridDim.x >> 3);
+    const int s0 = c * L.S / 8, s1 = (c + 1) * L.S / 8;
+    if (G == 0 || s0 >= s1) return;
+    const int lane = threadIdx.x;
+    for (int t = (int)(blockIdx.x >> 3); t < L.ntiles; t += G) {
+        TileWalk w;
+        w.start(L, t, skew_slot0(L, t, s0), lane);
+        for (int s = s0; s < s1; ++s) {
+            if (w.cell(L)) {
+                Cell cl;
+                cl.e = w.elem(lane);
+                cl.ti = w.ti; cl.tj = w.tj; cl.slot = w.slot; cl.b = w.b; cl.a = w.a; cl.kk = w.kk;
+                cl.in = true;
+                f(cl);
+            }
+            w.next(L);
+        }
+    }
+}
+
+// for_cells_skewed with the memory access of each cell split from its
+// arithmetic: load(cell, valid) reads what the cell needs into a value of
+// type T (a lane with no cell at that slot, valid false, reads only its own
+// element, which is always in the vector), compute(cell, values) does the
+// rest.  The walk issues the loads of the lane's next slot before it
+// computes the current one, alternating two value buffers (no copy between
+// them, so no wait for a load before its cell is computed), and each wave
+// keeps two slots' loads in flight.  The cells are computed in
+// for_cells_skewed's order.
 template <class T, class LD, class CP>
 __device__ __forceinline__ void for_cells_pipelined(const Lay &L, LD &&load, CP &&compute)
 {
@@ -324,36 +355,34 @@ __device__ __forceinline__ void for_cells_pipelined(const Lay &L, LD &&load, CP 
     if (G == 0 || s0 >= s1) return;
     const int lane = threadIdx.x;
     for (int t = (int)(blockIdx.x >> 3); t < L.ntiles; t += G) {
-        const size_t lo = (size_t)t * L.S + s0;
-        SlotWalk w;
-        w.start(L, lo, lane);
-        // the cell at slot s of the walk (s <= s1: past the range it is the
-        // tile's next slot, or slot 0 of the next tile or ring, still an
-        // element of the vector)
-        auto cell_here = [&](int s) {
+        TileWalk w;
+        w.start(L, t, skew_slot0(L, t, s0), lane);
+        // the cell at the walk's current slot (past the range it is the
+        // tile's next slot, still an element of the vector)
+        auto cell_here = [&]() {
             Cell cl;
-            cl.e = elem_ts(lo + (size_t)(s - s0), w.slot, lane);
+            cl.e = w.elem(lane);
             cl.ti = w.ti; cl.tj = w.tj; cl.slot = w.slot; cl.b = w.b; cl.a = w.a; cl.kk = w.kk;
             cl.in = true;
             return cl;
         };
-        Cell ca = cell_here(s0);
+        Cell ca = cell_here();
         bool ha = w.cell(L);
         T va = load(ca, ha);
         w.next(L);
         for (int s = s0 + 1;; s += 2) {
             const bool in_b = s < s1;
-            const Cell cb = cell_here(s);
+            const Cell cb = cell_here();
             const bool hb = in_b && w.cell(L);
             const T vb = load(cb, hb);
-            if (in_b) w.next(L);
+            w.next(L);
             if (ha) compute(ca, va);
             if (!in_b) break;
             const bool in_a = s + 1 < s1;
-            ca = cell_here(s + 1);
+            ca = cell_here();
             ha = in_a && w.cell(L);
             va = load(ca, ha);
-            if (in_a) w.next(L);
+            w.next(L);
             if (hb) compute(cb, vb);
             if (!in_a) break;
         }

@@ -220,6 +220,45 @@ struct SlotWalk {
     }
 };
 
+// Walk over a range of one tile's slots that wraps from slot S-1 to slot 0
+// of the same tile (the SpMV's skewed walk).  Cell (b, a) of slot s lies at
+// plane (s - a - b) mod S, so the j-side neighbour of an a = 15 cell is the
+// next tile's a = 0 cell in slot s - 15 and the i-side neighbour of a b = 15
+// cell is in slot s - 15 of the tile below; the i-1 / j-1 sides read slot
+// s + 15.  Starting tile (ti, tj) at slot s0 - 15 (ti + tj), every block of a
+// stencil reads its neighbour tiles' edge rows in the slots that their own
+// blocks, started at the same time, work on at the same step.
+struct TileWalk {
+    int a, b, slot, ti, tj, kk;
+    bool colv;
+    size_t base;
+    CAP_HD void start(const Lay &L, int tile, int slot0, int lane)
+    {
+        a = lane % TW;
+        b = lane / TW;
+        ti = tile / L.nTJ;
+        tj = tile % L.nTJ;
+        slot = slot0;
+        base = (size_t)tile * (size_t)L.S * TCOL;
+        colv = col_valid(L, ti * TW + b, tj * TW + a);
+        kk = posmod(slot - a - b, L.S);
+    }
+    CAP_HD bool cell(const Lay &L) const { return colv && kk >= 1 && kk <= L.nks; }
+    CAP_HD size_t elem(int lane) const { return base + (size_t)slot * TCOL + lane; }
+    CAP_HD void next(const Lay &L)
+    {
+        if (++kk == L.S) kk = 0;
+        if (++slot == L.S) slot = 0;
+    }
+};
+
+// first slot of the skewed walk of tile `tile` over its range starting at s0
+CAP_HD int skew_slot0(const Lay &L, int tile, int s0)
+{
+    const int ti = tile / L.nTJ, tj = tile % L.nTJ;
+    return posmod(s0 - (TW - 1) * (ti + tj), L.S);
+}
+
 // Layout moves (k_gather, k_scatter).  Block bid moves MOVE_SLOTS
 // consecutive slots [s0, s0 + ns) of one tile between the layouts through
 // LDS.  On the reference side a row of a tile (its 16 columns of one i, at

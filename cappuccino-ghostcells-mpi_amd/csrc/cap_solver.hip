@@ -129,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_symcheck(Lay L, Coef7 A, int *asym)
 {
     auto bits = [](double x) { return __double_as_longlong(x); };
     bool differ = false;
-    for_cells(L, [&](const Cell &c) {
+    for_cells_skewed(L, [&](const Cell &c) {
         Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
         const size_t e = c.e;
         differ |= bits(A.aw[e]) != bits(A.ae[nb.w]) || bits(A.as[e]) != bits(A.an[nb.s]) ||
@@ -590,33 +590,44 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
             }
         }
         const V *const pkp = io.pk;
-        auto fetch_in = [&](int t, V (&x)[NIN]) {
-            const size_t e = elem_at(tb, step_slot<FWD>(L, t), lane);
+        // Addresses are a uniform row pointer (the tile's slot, in scalar
+        // registers) plus the lane's 32-bit offset, so that the loads and
+        // stores take the scalar-base form and no 64-bit address is formed
+        // per lane.
+        const unsigned ul = (unsigned)lane;
+        auto fetch_row = [&](size_t ro, V (&x)[NIN]) {
             if constexpr (PKN > 0) {
-                x[0] = inp[0][e];
+                x[0] = (inp[0] + ro)[ul];
                 V p4[4];
-                load4(pkp + 4 * e, p4);
+                load4(pkp + 4 * ro + 4 * ul, p4);
 #pragma unroll
                 for (int q = 1; q <= PKN; ++q) x[q] = p4[q - 1];
 #pragma unroll
-                for (int q = PKN + 1; q < NIN; ++q) x[q] = inp[q][e];
+                for (int q = PKN + 1; q < NIN; ++q) x[q] = (inp[q] + ro)[ul];
             } else {
 #pragma unroll
-                for (int q = 0; q < NIN; ++q) x[q] = inp[q][e];
+                for (int q = 0; q < NIN; ++q) x[q] = (inp[q] + ro)[ul];
             }
+        };
+        // the slot of the next step (one further in the sweep's direction)
+        auto advance = [&](int sl) {
+            return FWD ? (sl + 1 == L.S ? 0 : sl + 1) : (sl == 0 ? L.S - 1 : sl - 1);
         };
         // Steps at which every lane of a full tile has a solve cell: there
         // the step kinds, halo values and dummy stores drop out.
         const bool full = L.nis - ti * TW >= TW && L.njs - tj * TW >= TW;
         const int t_lo = 2 * (TW - 1) + 1, t_hi = L.nks;
 #pragma unroll
-        for (int d = 0; d < U; ++d) fetch_in(d, inb[d]);
+        for (int d = 0; d < U; ++d) fetch_row(tb + (size_t)step_slot<FWD>(L, d) * TCOL, inb[d]);
+        // slots of step t and of step t + U
+        int sl_t = step_slot<FWD>(L, 0), sl_u = step_slot<FWD>(L, U);
         __syncthreads();   // step-0 inputs from outside the tile are in mbuf
         for (int t0 = 0; t0 < NSTP; t0 += U) {
 #pragma unroll
             for (int d = 0; d < U; ++d) {
                 const int t = t0 + d;
-                const size_t e = elem_at(tb, step_slot<FWD>(L, t), lane);
+                const size_t ro = tb + (size_t)sl_t * TCOL;
+                const size_t e = ro + ul;
                 const int rp = (t + 1) & 1, wp = t & 1;
                 V dk[ND], di[ND], dj[ND], out[ND];
 #pragma unroll
@@ -633,12 +644,12 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                     for (int q = 0; q < ND; ++q) lds[wp][q][lane] = out[q];
 #pragma unroll
                     for (int q = 0; q < ND; ++q) {
-                        depp[q][e] = out[q];
-                        if (li.prod_a) coh_store(msgp + msg_index(L, ND, 0, q, tile, t, li.b), out[q]);
-                        if (li.prod_b) coh_store(msgp + msg_index(L, ND, 1, q, tile, t, li.a), out[q]);
+                        (depp[q] + ro)[ul] = out[q];
+                        if (li.prod_a) coh_store(msgp + msg_index(L, ND, 0, q, tile, t, 0) + li.b, out[q]);
+                        if (li.prod_b) coh_store(msgp + msg_index(L, ND, 1, q, tile, t, 0) + li.a, out[q]);
                     }
 #pragma unroll
-                    for (int q = 0; q < Op::NX; ++q) xop[q][e] = xv[q];
+                    for (int q = 0; q < Op::NX; ++q) (xop[q] + ro)[ul] = xv[q];
 #pragma unroll
                     for (int q = 0; q < Op::NR; ++q) red[q] += r[q];
                 } else {
@@ -665,7 +676,9 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                 }
 #pragma unroll
                 for (int q = 0; q < ND; ++q) prev[q] = out[q];
-                fetch_in(t + U, inb[d]);
+                fetch_row(tb + (size_t)sl_u * TCOL, inb[d]);
+                sl_t = advance(sl_t);
+                sl_u = advance(sl_u);
                 __syncthreads();
             }
         }
@@ -686,7 +699,8 @@ static int grid_for(const Lay &L)
 // grid of the SpMV (and of its symmetry check, whose per-block flags it
 // reads): fewer, longer blocks, so that the p and coefficient lines of a
 // block's previous planes, which its stencil reads again, are still in the
-// XCD's L2 (measured at 256^3: 0.242 ms with 2048 blocks, 0.227 with 512)
+// XCD's L2 (measured at 256^3 with the skewed walk: 0.215 ms with 512
+// blocks, 0.217 with 1024, 0.222 with 2048)
 static int grid_spmv(const Lay &L)
 {
     return 8 * std::min(L.ntiles, 64);

@@ -266,6 +266,60 @@ int emu_check_walk(int ni, int nj, int nk, int klo, int khi, int nblocks)
     return bad;
 }
 
+// The SpMV's skewed walk (for_cells_skewed: block b, eighth b % 8 of the
+// slots of tiles b/8, b/8 + G, ..., each tile's range rotated by
+// skew_slot0): every solve cell visited once, with the plane and neighbours
+// of decode() / neighbours(), and every neighbour in another tile lying in
+// the slot that tile's block of the same eighth visits at the same step.
+// Returns the number of mismatches.
+int emu_check_skewed_walk(int ni, int nj, int nk, int klo, int khi, int nblocks)
+{
+    Lay L = make_lay(ni, nj, nk, 3, ni - 2, 3, nj - 2, klo, khi);
+    std::vector<int> seen(L.nmain, 0);
+    int bad = 0;
+    const int G = nblocks >> 3;
+    for (int blk = 0; blk < nblocks; ++blk) {
+        const int c = blk & 7;
+        const int s0 = c * L.S / 8, s1 = (c + 1) * L.S / 8;
+        if (G == 0 || s0 >= s1) continue;
+        for (int t = blk >> 3; t < L.ntiles; t += G) {
+            for (int lane = 0; lane < TCOL; ++lane) {
+                TileWalk w;
+                w.start(L, t, skew_slot0(L, t, s0), lane);
+                for (int s = s0; s < s1; ++s) {
+                    const size_t e = w.elem(lane);
+                    int ci, cj, kk;
+                    const bool valid = decode(L, e, ci, cj, kk);
+                    const bool in = valid && kk >= 1 && kk <= L.nks;
+                    if (w.cell(L) != in) ++bad;
+                    if (w.cell(L)) {
+                        seen[e]++;
+                        if (w.kk != kk || w.ti * TW + w.b != ci || w.tj * TW + w.a != cj) ++bad;
+                        Nb r = neighbours(L, w.ti, w.tj, w.slot, w.b, w.a, w.kk);
+                        Nb f = neighbours_at(L, w.ti, w.tj, e, w.slot, w.b, w.a, w.kk);
+                        if (r.e != f.e || r.w != f.w || r.n != f.n || r.s != f.s || r.t != f.t ||
+                            r.bt != f.bt) ++bad;
+                        for (size_t q : {r.e, r.w, r.n, r.s}) {
+                            if (q >= L.nmain) continue;   // ring column
+                            const int tq = (int)(q / ((size_t)L.S * TCOL));
+                            if (tq == t) continue;
+                            const int sq = (int)((q / TCOL) % (size_t)L.S);
+                            if (posmod(sq - skew_slot0(L, tq, s0), L.S) != s - s0) ++bad;
+                        }
+                    }
+                    w.next(L);
+                }
+            }
+        }
+    }
+    for (size_t e = 0; e < L.nmain; ++e) {
+        int ci, cj, kk;
+        const bool in = decode(L, e, ci, cj, kk) && kk >= 1 && kk <= L.nks;
+        if (seen[e] != (in ? 1 : 0)) ++bad;
+    }
+    return bad;
+}
+
 // z = A p on interior cells with the operand order of pcg-sip.f90:147-149
 void emu_spmv(int ni, int nj, int nk, int klo, int khi, const double *const *A,
               const double *p, double *z)

@@ -146,6 +146,14 @@ def test_flat_kernel_walk(E, dims, klo, khi, nblocks):
     assert E.emu_check_walk(*dims, klo, khi, nblocks) == 0
 
 
+@pytest.mark.parametrize("dims,klo,khi", CASES + [((68, 52, 12), 3, 10), ((100, 132, 6), 2, 5)])
+@pytest.mark.parametrize("nblocks", [8, 56, 512, 2048])
+def test_skewed_walk(E, dims, klo, khi, nblocks):
+    # the SpMV's rotated per-tile walk: coverage, planes, neighbours, and the
+    # neighbour tiles' edge rows visited at the same step
+    assert E.emu_check_skewed_walk(*dims, klo, khi, nblocks) == 0
+
+
 MOVE_CASES = CASES + [((68, 132, 100), 3, 98), ((196, 388, 62), 3, 60), ((21, 17, 40), 2, 39)]
 
 
