@@ -106,7 +106,7 @@ __global__ __launch_bounds__(BLOCK) void k_resid(Lay L, Coef7 A, const double *_
 {
     if (st->done) return;
     Acc acc[1];
-    for_cells(L, [&](const Cell &c) {
+    for_cells_skewed(L, [&](const Cell &c) {
         Nb nb = neighbours_at(L, c.ti, c.tj, c.e, c.slot, c.b, c.a, c.kk);
         const size_t e = c.e;
         double v = A.ae[e] * fi[nb.e] + A.aw[e] * fi[nb.w] + A.an[e] * fi[nb.n] +
@@ -815,16 +815,14 @@ static Coef7 coef7(cap_ctx *ctx)
     return A;
 }
 
-// flags the SpMV blocks of this solve's layout that must read aw, as, ab: for
-// the FP64 SpMV's grid in asym, and with f32 also for the FP32 SpMV's grid
-// (the flat kernels' one, which suits four-byte values better) in asym32
-static void sym_check(cap_ctx *ctx, const Lay &L, bool f32 = false)
+// flags the SpMV blocks of this solve's layout that must read aw, as, ab
+// (asym, one flag a block of the SpMV grid).  The FP32 SpMV runs on the same
+// grid and reads the same flags: two doubles that are equal bit for bit
+// round to equal floats.
+static void sym_check(cap_ctx *ctx, const Lay &L)
 {
     hipLaunchKernelGGL(k_symcheck, dim3(grid_spmv(L)), dim3(BLOCK), 0, ctx->stream, L, coef7(ctx),
                        ctx->asym);
-    if (f32)
-        hipLaunchKernelGGL(k_symcheck, dim3(grid_for(L)), dim3(BLOCK), 0, ctx->stream, L, coef7(ctx),
-                           ctx->asym + MAX_RED_BLOCKS);
 }
 
 // Everything a solve needs in device memory, in both layouts.
@@ -1597,7 +1595,7 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
     if (rc) return rc;
     if ((rc = ensure_f32(ctx, L.n))) return rc;
     if ((rc = init_state(ctx, sor, nsw))) return rc;
-    sym_check(ctx, L, true);
+    sym_check(ctx, L);
     double *const *T = ctx->T;
     float *const *F = ctx->F;
     const int gb = grid_for(L);
@@ -1683,9 +1681,9 @@ extern "C" int cap_pcgsip_mixed(cap_ctx *ctx, double *fi, const cap_coef *c, dou
             else
                 hipLaunchKernelGGL(k_pupdate_fi<float>, dim3(gb), dim3(BLOCK), 0, s, L,
                                    (const float *)F[F_Z], F[F_P], F[F_X], st2);
-            hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s, L,
+            hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s, L,
                                A32, (const float *)F[F_P], F[F_W], (const float *)nullptr, st2,
-                               ctx->partials, ctx->ticket, ctx->asym + MAX_RED_BLOCKS);
+                               ctx->partials, ctx->ticket, ctx->asym);
         };
         int batch = 4;
         for (int l = 0; l < hi.nsw;) {
@@ -1912,7 +1910,7 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
     const Lay L = lay_for(ctx, 3, ctx->nk - 2);
     int rc;
     if ((rc = init_state(ctx, 0.0, 1 << 30))) return rc;
-    sym_check(ctx, L, true);
+    sym_check(ctx, L);
     HIPCHK(hipMemsetAsync(ctx->ticket, 0, 64, ctx->stream));
     double *const *T = ctx->T;
     float *const *F = ctx->F;
@@ -1973,9 +1971,9 @@ extern "C" int cap_time_kernels(cap_ctx *ctx, int fp32, int reps, double *ms)
                     hipLaunchKernelGGL(k_pupdate<float>, dim3(gb), dim3(BLOCK), 0, s, L,
                                        (const float *)F[F_Z], F[F_P], ctx->st);
                 if (q == 3)
-                    hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(gb), dim3(BLOCK), 0, s,
+                    hipLaunchKernelGGL((k_spmv<float, 0, F_PCG_PKAPK>), dim3(grid_spmv(L)), dim3(BLOCK), 0, s,
                                        L, A32, (const float *)F[F_P], F[F_W],
-                                       (const float *)nullptr, ctx->st, ctx->partials, ctx->ticket, ctx->asym + MAX_RED_BLOCKS);
+                                       (const float *)nullptr, ctx->st, ctx->partials, ctx->ticket, ctx->asym);
                 if (q == 4)
                     hipLaunchKernelGGL((k_update<float, 0, F_PCG_RESL>), dim3(gb), dim3(BLOCK), 0,
                                        s, L, F[F_X], F[F_R], (const float *)F[F_P],
