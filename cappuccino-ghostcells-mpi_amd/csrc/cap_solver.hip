@@ -318,8 +318,9 @@ struct TakesAlf : std::false_type {};
 template <class Op>
 struct TakesAlf<Op, std::void_t<decltype(Op::TAKES_ALF)>> : std::true_type {};
 
-// steps of inputs a compute lane keeps in flight: 16 for one chained value
-// and up to five inputs, else 8 (register budget), unless the op says
+// depth U of a compute lane's input ring (U-1 steps of inputs in flight): 16
+// for one chained value and up to five inputs, else 8 (register budget),
+// unless the op says
 template <class Op, class = void>
 struct LookAhead {
     static constexpr int v = Op::ND == 1 && Op::NIN <= 5 ? 16 : 8;
@@ -414,8 +415,9 @@ __device__ __forceinline__ V msg_wait(const V *p, V v, DevState *st)
 // One workgroup = one tile (16x16 columns, all planes) marching through the
 // tile's steps, in five waves:
 //  * waves 0-3 (256 lanes, one per column) compute.  Their inputs for step
-//    t+U are loaded at step t (a register ring of depth U), so loads stay in
-//    flight across steps; values cross lanes through LDS, one barrier a step.
+//    t+U-1 are loaded at the start of step t (a register ring of depth U), so
+//    loads stay in flight across steps; values cross lanes through LDS, one
+//    barrier a step.
 //  * wave 4 carries the values that come from outside the tile: it reads the
 //    neighbouring tiles' messages (or the ring columns at the box edge) DM
 //    steps ahead and hands them to the compute waves through LDS, so a late
@@ -617,10 +619,13 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
         // the step kinds, halo values and dummy stores drop out.
         const bool full = L.nis - ti * TW >= TW && L.njs - tj * TW >= TW;
         const int t_lo = 2 * (TW - 1) + 1, t_hi = L.nks;
+        // Inputs are loaded U-1 steps ahead, at the start of a step (right
+        // after its LDS reads, so the loads issue while those are in flight),
+        // into the ring slot the previous step has just consumed.
 #pragma unroll
-        for (int d = 0; d < U; ++d) fetch_row(tb + (size_t)step_slot<FWD>(L, d) * TCOL, inb[d]);
-        // slots of step t and of step t + U
-        int sl_t = step_slot<FWD>(L, 0), sl_u = step_slot<FWD>(L, U);
+        for (int d = 0; d < U - 1; ++d) fetch_row(tb + (size_t)step_slot<FWD>(L, d) * TCOL, inb[d]);
+        // slots of step t and of step t + U - 1
+        int sl_t = step_slot<FWD>(L, 0), sl_u = step_slot<FWD>(L, U - 1);
         __syncthreads();   // step-0 inputs from outside the tile are in mbuf
         for (int t0 = 0; t0 < NSTP; t0 += U) {
 #pragma unroll
@@ -636,6 +641,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                     dj[q] = sa_ext ? mbuf[wp][q][0][li.b] : lds[rp][q][li.nb_a];
                     di[q] = sb_ext ? mbuf[wp][q][1][li.a] : lds[rp][q][li.nb_b];
                 }
+                __builtin_amdgcn_sched_barrier(0);
+                fetch_row(tb + (size_t)sl_u * TCOL, inb[(d + U - 1) % U]);
                 V xv[NXS];
                 double r[NRS];
                 op.compute(inb[d], dk, di, dj, out, xv, r);
@@ -644,12 +651,13 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                     for (int q = 0; q < ND; ++q) lds[wp][q][lane] = out[q];
 #pragma unroll
                     for (int q = 0; q < ND; ++q) {
-                        (depp[q] + ro)[ul] = out[q];
+                        // streaming stores: the next sweep reads them from HBM
+                        __builtin_nontemporal_store(out[q], &(depp[q] + ro)[ul]);
                         if (li.prod_a) coh_store(msgp + msg_index(L, ND, 0, q, tile, t, 0) + li.b, out[q]);
                         if (li.prod_b) coh_store(msgp + msg_index(L, ND, 1, q, tile, t, 0) + li.a, out[q]);
                     }
 #pragma unroll
-                    for (int q = 0; q < Op::NX; ++q) (xop[q] + ro)[ul] = xv[q];
+                    for (int q = 0; q < Op::NX; ++q) __builtin_nontemporal_store(xv[q], &(xop[q] + ro)[ul]);
 #pragma unroll
                     for (int q = 0; q < Op::NR; ++q) red[q] += r[q];
                 } else {
@@ -676,7 +684,6 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_wave(Lay L, Op op, WaveIO<Op> io
                 }
 #pragma unroll
                 for (int q = 0; q < ND; ++q) prev[q] = out[q];
-                fetch_row(tb + (size_t)sl_u * TCOL, inb[d]);
                 sl_t = advance(sl_t);
                 sl_u = advance(sl_u);
                 __syncthreads();
