@@ -90,3 +90,13 @@ int cap_piso_run(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple, c
 int cap_scratch_setup(int device);
 // grad_gauss into gradu(3, nxyz) layout (device pointers; queued, not synchronised)
 int cap_grad_gauss_aos(cap_ctx *ctx, const double *u, const double *const *geom, double *g3);
+// The east, north and top directions of fluxmass, PISO_assemble_pressure_eq
+// and fluxuvw (CDS scheme alone) in one pass, leaving every array as the
+// three per-direction calls do (device pointers; queued, not synchronised).
+// a6 = {ae, aw, an, as, at, ab}; f3 = {f1, f2, f3}; bp3 = {be, bn, bt};
+// s3 = {su, sv, sw}; scr3 = fluxuvw's scratch {bp, bt, bb}.
+int cap_fluxmass3_q(cap_ctx *ctx, const cap_fields *F, double *const *a6, double *const *f3);
+int cap_piso_assemble3_q(cap_ctx *ctx, const cap_fields *F, double *const *a6, double *const *f3,
+                         int ipc, double *const *bp3, double *su);
+int cap_fluxuvw3_q(cap_ctx *ctx, const cap_fields *F, const double *vis, double *const *a6,
+                   const double *const *f3, double *const *s3, double *const *scr3, double gds);

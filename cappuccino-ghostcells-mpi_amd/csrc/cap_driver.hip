@@ -472,6 +472,19 @@ cap_coef coef_of(const cap_flow *f)
     return c;
 }
 
+// {ae, aw, an, as, at, ab} and {f1, f2, f3} of the flow state
+double *const *a6_of(const cap_flow *f, double *a6[6])
+{
+    double *const a[6] = {f->ae, f->aw, f->an, f->as, f->at, f->ab};
+    for (int q = 0; q < 6; ++q) a6[q] = a[q];
+    return a6;
+}
+double *const *f3_of(const cap_flow *f, double *f3[3])
+{
+    f3[0] = f->f1; f3[1] = f->f2; f3[2] = f->f3;
+    return f3;
+}
+
 int grad3(cap_ctx *ctx, const cap_flow *f, const double *u, double *g3)
 {
     const double *gb[13];
@@ -613,9 +626,9 @@ extern "C" int cap_calcp(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, cap_r
         RCHK(grad3(ctx, f, f->u, f->gradu));
         RCHK(grad3(ctx, f, f->v, f->gradv));
         RCHK(grad3(ctx, f, f->w, f->gradw));
-        RCHK(cap_fluxmass(ctx, &F, d.nj, 1, d.nij, f->ar1x, f->ar1y, f->ar1z, f->fx, f->ae, f->aw, f->f1));
-        RCHK(cap_fluxmass(ctx, &F, 1, d.nij, d.nj, f->ar2x, f->ar2y, f->ar2z, f->fy, f->an, f->as, f->f2));
-        RCHK(cap_fluxmass(ctx, &F, d.nij, d.nj, 1, f->ar3x, f->ar3y, f->ar3z, f->fz, f->at, f->ab, f->f3));
+        // fluxmass for east, north and top (calcp-multiple_correction_SIMPLE.f90:50-60)
+        double *a6[6], *f3[3];
+        RCHK(cap_fluxmass3_q(ctx, &F, a6_of(f, a6), f3_of(f, f3)));
         if (!o->const_mflux) RCHK(cap_outbc(ctx, f, o));
         hipLaunchKernelGGL(k_su_ap, dim3(grid_for(nr)), dim3(256), 0, ctx->stream, d, 1, 1,
                            (const double *)f->f1, (const double *)f->f2, (const double *)f->f3,
@@ -696,12 +709,10 @@ int cap_piso_run(cap_ctx *ctx, cap_flow *f, const cap_drv_opts *o, int pimple, c
         if (!pimple) RCHK(cap_gethbya(ctx, f, o));
         for (int ipcorr = 1; ipcorr <= o->npcor; ++ipcorr) {
             const cap_fields F = fields_of(f);
-            RCHK(cap_piso_assemble_pressure_eq(ctx, &F, d.nj, 1, d.nij, f->ar1x, f->ar1y, f->ar1z, f->fx,
-                                               f->ae, f->aw, f->f1, ipcorr, f->be, f->su));
-            RCHK(cap_piso_assemble_pressure_eq(ctx, &F, 1, d.nij, d.nj, f->ar2x, f->ar2y, f->ar2z, f->fy,
-                                               f->an, f->as, f->f2, ipcorr, f->bn, f->su));
-            RCHK(cap_piso_assemble_pressure_eq(ctx, &F, d.nij, d.nj, 1, f->ar3x, f->ar3y, f->ar3z, f->fz,
-                                               f->at, f->ab, f->f3, ipcorr, f->bt, f->su));
+            // piso_assemble_pressure_eq for east, north and top, bp = be, bn, bt
+            double *a6[6], *f3[3];
+            double *const bp3[3] = {f->be, f->bn, f->bt};
+            RCHK(cap_piso_assemble3_q(ctx, &F, a6_of(f, a6), f3_of(f, f3), ipcorr, bp3, f->su));
             if (ipcorr == 1 && !o->const_mflux) RCHK(cap_outbc(ctx, f, o));
             hipLaunchKernelGGL(k_su_ap, dim3(grid_for(nr)), dim3(256), 0, s, d, ipcorr == 1 ? 1 : 0, 1,
                                (const double *)f->f1, (const double *)f->f2, (const double *)f->f3,
@@ -1014,12 +1025,18 @@ extern "C" int cap_calcuvw(cap_ctx *ctx, cap_flow *f, double *spv, const cap_drv
     hipLaunchKernelGGL(k_uvw_src, dim3(grid_for(range2_cells(d))), dim3(256), 0, s, d, io);
     HIPCHK(hipGetLastError());
     const cap_fields F = fields_of(f);
-    RCHK(cap_fluxuvw(ctx, &F, f->vis, d.nj, 1, d.nij, f->ar1x, f->ar1y, f->ar1z, f->fx, f->ae, f->aw,
-                     f->f1, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
-    RCHK(cap_fluxuvw(ctx, &F, f->vis, 1, d.nij, d.nj, f->ar2x, f->ar2y, f->ar2z, f->fy, f->an, f->as,
-                     f->f2, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
-    RCHK(cap_fluxuvw(ctx, &F, f->vis, d.nij, d.nj, 1, f->ar3x, f->ar3y, f->ar3z, f->fz, f->at, f->ab,
-                     f->f3, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
+    if (q->schemes == CAP_UVW_CDS) {   // the three directions in one pass
+        double *a6[6], *f3[3];
+        double *const s3[3] = {f->su, f->sv, f->sw}, *const scr3[3] = {f->bp, f->bt, f->bb};
+        RCHK(cap_fluxuvw3_q(ctx, &F, f->vis, a6_of(f, a6), f3_of(f, f3), s3, scr3, q->gds));
+    } else {
+        RCHK(cap_fluxuvw(ctx, &F, f->vis, d.nj, 1, d.nij, f->ar1x, f->ar1y, f->ar1z, f->fx, f->ae,
+                         f->aw, f->f1, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
+        RCHK(cap_fluxuvw(ctx, &F, f->vis, 1, d.nij, d.nj, f->ar2x, f->ar2y, f->ar2z, f->fy, f->an,
+                         f->as, f->f2, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
+        RCHK(cap_fluxuvw(ctx, &F, f->vis, d.nij, d.nj, 1, f->ar3x, f->ar3y, f->ar3z, f->fz, f->at,
+                         f->ab, f->f3, f->su, f->sv, f->sw, f->bp, f->bt, f->bb, q->schemes, q->gds));
+    }
     hipLaunchKernelGGL(k_uvw_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, io, 0, 3,
                        (double)q->nx, (double)q->ny, (double)q->nz);
     hipLaunchKernelGGL(k_uvw_wall, dim3(grid_for(plane_cells(d))), dim3(256), 0, s, d, io, 1, top_plane,

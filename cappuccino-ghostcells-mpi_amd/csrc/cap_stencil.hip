@@ -254,6 +254,60 @@ __global__ __launch_bounds__(256) void k_zero_st(double *p, size_t n)
         p[i] = 0.0;
 }
 
+// fluxmass2.f90:93-188: the face east of cell p (in the direction D):
+// its coefficient a (acfe(p) = acfw(e) = a) and mass flux fc
+__device__ __forceinline__ void face_fluxmass(const Fld &f, const FaceDir &D, size_t p, double &a,
+                                              double &fc)
+{
+    const size_t e = p + D.idew, s = p - D.idns, b = p - D.idtb, bs = b - D.idns;
+    const double fxe = D.fif[p], fxp = 1.0 - fxe;
+    const double xf = 0.25 * (f.x[p] + f.x[s] + f.x[b] + f.x[bs]);
+    const double yf = 0.25 * (f.y[p] + f.y[s] + f.y[b] + f.y[bs]);
+    const double zf = 0.25 * (f.z[p] + f.z[s] + f.z[b] + f.z[bs]);
+    const double xi = f.xc[p] * fxp + f.xc[e] * fxe;
+    const double yi = f.yc[p] * fxp + f.yc[e] * fxe;
+    const double zi = f.zc[p] * fxp + f.zc[e] * fxe;
+    const double xpn = f.xc[e] - f.xc[p], ypn = f.yc[e] - f.yc[p], zpn = f.zc[e] - f.zc[p];
+    const double arx = D.arvx[p], ary = D.arvy[p], arz = D.arvz[p];
+    const double are = sqrt(arx * arx + ary * ary + arz * arz);
+    const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+    const double sfdpnr = 1.0 / (arx * xpn * nxx + ary * ypn * nyy + arz * zpn * nzz);
+    const double dene = f.den[p] * fxp + f.den[e] * fxe;
+    const double smdpn = (arx * arx + ary * ary + arz * arz) * sfdpnr;
+    a = dene * (fxp * f.vol[p] * f.apu[p] + fxe * f.vol[e] * f.apu[e]) * smdpn;
+    double duxi = G3(f.gu, p, 0) * fxp + G3(f.gu, e, 0) * fxe;
+    double duyi = G3(f.gu, p, 1) * fxp + G3(f.gu, e, 1) * fxe;
+    double duzi = G3(f.gu, p, 2) * fxp + G3(f.gu, e, 2) * fxe;
+    const double ui = f.u[p] * fxp + f.u[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+    duxi = G3(f.gv, p, 0) * fxp + G3(f.gv, e, 0) * fxe;
+    duyi = G3(f.gv, p, 1) * fxp + G3(f.gv, e, 1) * fxe;
+    duzi = G3(f.gv, p, 2) * fxp + G3(f.gv, e, 2) * fxe;
+    const double vi = f.v[p] * fxp + f.v[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+    duxi = G3(f.gw, p, 0) * fxp + G3(f.gw, e, 0) * fxe;
+    duyi = G3(f.gw, p, 1) * fxp + G3(f.gw, e, 1) * fxe;
+    duzi = G3(f.gw, p, 2) * fxp + G3(f.gw, e, 2) * fxe;
+    const double wi = f.w[p] * fxp + f.w[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+    const double cu = fxe * f.vol[e] * f.apu[e] + fxp * f.vol[p] * f.apu[p];
+    const double cv = fxe * f.vol[e] * f.apv[e] + fxp * f.vol[p] * f.apv[p];
+    const double cw = fxe * f.vol[e] * f.apw[e] + fxp * f.vol[p] * f.apw[p];
+    const double dpxi = cu * (fxe * G3(f.gp, e, 0) + fxp * G3(f.gp, p, 0)) * xpn * nxx;
+    const double dpyi = cv * (fxe * G3(f.gp, e, 1) + fxp * G3(f.gp, p, 1)) * ypn * nyy;
+    const double dpzi = cw * (fxe * G3(f.gp, e, 2) + fxp * G3(f.gp, p, 2)) * zpn * nzz;
+    double xpp = xf - (xf - f.xc[p]) * nxx, ypp = yf - (yf - f.yc[p]) * nyy, zpp = zf - (zf - f.zc[p]) * nzz;
+    double xep = xf - (xf - f.xc[e]) * nxx, yep = yf - (yf - f.yc[e]) * nyy, zep = zf - (zf - f.zc[e]) * nzz;
+    xpp = xpp - f.xc[p]; ypp = ypp - f.yc[p]; zpp = zpp - f.zc[p];
+    xep = xep - f.xc[e]; yep = yep - f.yc[e]; zep = zep - f.zc[e];
+    // the reference's correction term, signs as written (fluxmass2.f90:171-173)
+    const double dpe = (f.p[e] - f.p[p]) +
+                       (G3(f.gp, e, 0) * xep + G3(f.gp, e, 1) * yep + G3(f.gp, e, 2) * zep -
+                        G3(f.gp, p, 0) * xpp + G3(f.gp, p, 1) * ypp + G3(f.gp, p, 2) * zpp);
+    const double dpex = cu * (arx * sfdpnr) * dpe;
+    const double dpey = cv * (ary * sfdpnr) * dpe;
+    const double dpez = cw * (arz * sfdpnr) * dpe;
+    const double ue = ui - dpex + dpxi, ve = vi - dpey + dpyi, we = wi - dpez + dpzi;
+    fc = dene * (ue * arx + ve * ary + we * arz);
+}
+
 // fluxmass2.f90:41-192
 __global__ __launch_bounds__(256) void k_fluxmass(int ni, int nj, int nk, Fld f, FaceDir D,
                                                   double *__restrict__ acfe,
@@ -264,55 +318,63 @@ __global__ __launch_bounds__(256) void k_fluxmass(int ni, int nj, int nk, Fld f,
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nf; q += (size_t)gridDim.x * 256) {
         size_t p;
         if (!face_of(ni, nj, 2, nk - 1, q, p)) continue;
-        const size_t e = p + D.idew, s = p - D.idns, b = p - D.idtb, bs = b - D.idns;
-        const double fxe = D.fif[p], fxp = 1.0 - fxe;
-        const double xf = 0.25 * (f.x[p] + f.x[s] + f.x[b] + f.x[bs]);
-        const double yf = 0.25 * (f.y[p] + f.y[s] + f.y[b] + f.y[bs]);
-        const double zf = 0.25 * (f.z[p] + f.z[s] + f.z[b] + f.z[bs]);
-        const double xi = f.xc[p] * fxp + f.xc[e] * fxe;
-        const double yi = f.yc[p] * fxp + f.yc[e] * fxe;
-        const double zi = f.zc[p] * fxp + f.zc[e] * fxe;
-        const double xpn = f.xc[e] - f.xc[p], ypn = f.yc[e] - f.yc[p], zpn = f.zc[e] - f.zc[p];
-        const double arx = D.arvx[p], ary = D.arvy[p], arz = D.arvz[p];
-        const double are = sqrt(arx * arx + ary * ary + arz * arz);
-        const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
-        const double sfdpnr = 1.0 / (arx * xpn * nxx + ary * ypn * nyy + arz * zpn * nzz);
-        const double dene = f.den[p] * fxp + f.den[e] * fxe;
-        const double smdpn = (arx * arx + ary * ary + arz * arz) * sfdpnr;
-        const double a = dene * (fxp * f.vol[p] * f.apu[p] + fxe * f.vol[e] * f.apu[e]) * smdpn;
+        double a, fc;
+        face_fluxmass(f, D, p, a, fc);
         acfe[p] = a;
-        acfw[e] = a;
+        acfw[p + D.idew] = a;
+        fcf[p] = fc;
+    }
+}
+
+// PISO_assemble_pressure_eq.f90:58-239: the face east of cell p (in the
+// direction D): its coefficient a (acfe(p) = acfw(e) = a) and, when ipc > 0,
+// the source suadd of the non-orthogonal correction and the mass flux fc
+__device__ __forceinline__ void face_piso(const Fld &f, const FaceDir &D, int ipc, size_t p,
+                                          double &a, double &suadd, double &fc)
+{
+    const size_t e = p + D.idew, s = p - D.idns, b = p - D.idtb, bs = b - D.idns;
+    const double fxe = D.fif[p], fxp = 1.0 - fxe;
+    const double xf = 0.25 * (f.x[p] + f.x[s] + f.x[b] + f.x[bs]);
+    const double yf = 0.25 * (f.y[p] + f.y[s] + f.y[b] + f.y[bs]);
+    const double zf = 0.25 * (f.z[p] + f.z[s] + f.z[b] + f.z[bs]);
+    const double xi = f.xc[p] * fxp + f.xc[e] * fxe;
+    const double yi = f.yc[p] * fxp + f.yc[e] * fxe;
+    const double zi = f.zc[p] * fxp + f.zc[e] * fxe;
+    const double xpn = f.xc[e] - f.xc[p], ypn = f.yc[e] - f.yc[p], zpn = f.zc[e] - f.zc[p];
+    const double dpn = sqrt(xpn * xpn + ypn * ypn + zpn * zpn);
+    const double ixi1 = xpn / dpn, ixi2 = ypn / dpn, ixi3 = zpn / dpn;
+    const double arx = D.arvx[p], ary = D.arvy[p], arz = D.arvz[p];
+    const double are = sqrt(arx * arx + ary * ary + arz * arz);
+    const double vole = xpn * arx + ypn * ary + zpn * arz;
+    const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+    const double costn = nxx * ixi1 + nyy * ixi2 + nzz * ixi3;
+    const double d2x = xpn * costn, d2y = ypn * costn, d2z = zpn * costn;
+    const double dene = f.den[p] * fxp + f.den[e] * fxe;
+    const double game = dene * (fxp * f.vol[p] * f.apu[p] + fxe * f.vol[e] * f.apu[e]);
+    a = game * are / dpn;
+    if (ipc > 0) {
+        const double dfixi = G3(f.gp, p, 0) * fxp + G3(f.gp, e, 0) * fxe;
+        const double dfiyi = G3(f.gp, p, 1) * fxp + G3(f.gp, e, 1) * fxe;
+        const double dfizi = G3(f.gp, p, 2) * fxp + G3(f.gp, e, 2) * fxe;
+        const double dfixii = dfixi * costn + arx / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+        const double dfiyii = dfiyi * costn + ary / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+        const double dfizii = dfizi * costn + arz / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
+        const double fdfie = game * (dfixii * arx + dfiyii * ary + dfizii * arz);
+        const double fdfii = game * are / dpn * (dfixi * xpn + dfiyi * ypn + dfizi * zpn);
+        suadd = fdfie - fdfii;
         double duxi = G3(f.gu, p, 0) * fxp + G3(f.gu, e, 0) * fxe;
         double duyi = G3(f.gu, p, 1) * fxp + G3(f.gu, e, 1) * fxe;
         double duzi = G3(f.gu, p, 2) * fxp + G3(f.gu, e, 2) * fxe;
-        const double ui = f.u[p] * fxp + f.u[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+        const double ui = f.u[p] * fxp + f.u[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
         duxi = G3(f.gv, p, 0) * fxp + G3(f.gv, e, 0) * fxe;
         duyi = G3(f.gv, p, 1) * fxp + G3(f.gv, e, 1) * fxe;
         duzi = G3(f.gv, p, 2) * fxp + G3(f.gv, e, 2) * fxe;
-        const double vi = f.v[p] * fxp + f.v[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
+        const double vi = f.v[p] * fxp + f.v[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
         duxi = G3(f.gw, p, 0) * fxp + G3(f.gw, e, 0) * fxe;
         duyi = G3(f.gw, p, 1) * fxp + G3(f.gw, e, 1) * fxe;
         duzi = G3(f.gw, p, 2) * fxp + G3(f.gw, e, 2) * fxe;
-        const double wi = f.w[p] * fxp + f.w[e] * fxe + (duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi));
-        const double cu = fxe * f.vol[e] * f.apu[e] + fxp * f.vol[p] * f.apu[p];
-        const double cv = fxe * f.vol[e] * f.apv[e] + fxp * f.vol[p] * f.apv[p];
-        const double cw = fxe * f.vol[e] * f.apw[e] + fxp * f.vol[p] * f.apw[p];
-        const double dpxi = cu * (fxe * G3(f.gp, e, 0) + fxp * G3(f.gp, p, 0)) * xpn * nxx;
-        const double dpyi = cv * (fxe * G3(f.gp, e, 1) + fxp * G3(f.gp, p, 1)) * ypn * nyy;
-        const double dpzi = cw * (fxe * G3(f.gp, e, 2) + fxp * G3(f.gp, p, 2)) * zpn * nzz;
-        double xpp = xf - (xf - f.xc[p]) * nxx, ypp = yf - (yf - f.yc[p]) * nyy, zpp = zf - (zf - f.zc[p]) * nzz;
-        double xep = xf - (xf - f.xc[e]) * nxx, yep = yf - (yf - f.yc[e]) * nyy, zep = zf - (zf - f.zc[e]) * nzz;
-        xpp = xpp - f.xc[p]; ypp = ypp - f.yc[p]; zpp = zpp - f.zc[p];
-        xep = xep - f.xc[e]; yep = yep - f.yc[e]; zep = zep - f.zc[e];
-        // the reference's correction term, signs as written (fluxmass2.f90:171-173)
-        const double dpe = (f.p[e] - f.p[p]) +
-                           (G3(f.gp, e, 0) * xep + G3(f.gp, e, 1) * yep + G3(f.gp, e, 2) * zep -
-                            G3(f.gp, p, 0) * xpp + G3(f.gp, p, 1) * ypp + G3(f.gp, p, 2) * zpp);
-        const double dpex = cu * (arx * sfdpnr) * dpe;
-        const double dpey = cv * (ary * sfdpnr) * dpe;
-        const double dpez = cw * (arz * sfdpnr) * dpe;
-        const double ue = ui - dpex + dpxi, ve = vi - dpey + dpyi, we = wi - dpez + dpzi;
-        fcf[p] = dene * (ue * arx + ve * ary + we * arz);
+        const double wi = f.w[p] * fxp + f.w[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
+        fc = dene * (ui * arx + vi * ary + wi * arz);
     }
 }
 
@@ -328,53 +390,14 @@ __global__ __launch_bounds__(256) void k_piso_assemble(int ni, int nj, int nk, F
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nf; q += (size_t)gridDim.x * 256) {
         size_t p;
         if (!face_of(ni, nj, 2, nk - 2, q, p)) continue;
-        const size_t e = p + D.idew, s = p - D.idns, b = p - D.idtb, bs = b - D.idns;
-        const double fxe = D.fif[p], fxp = 1.0 - fxe;
-        const double xf = 0.25 * (f.x[p] + f.x[s] + f.x[b] + f.x[bs]);
-        const double yf = 0.25 * (f.y[p] + f.y[s] + f.y[b] + f.y[bs]);
-        const double zf = 0.25 * (f.z[p] + f.z[s] + f.z[b] + f.z[bs]);
-        const double xi = f.xc[p] * fxp + f.xc[e] * fxe;
-        const double yi = f.yc[p] * fxp + f.yc[e] * fxe;
-        const double zi = f.zc[p] * fxp + f.zc[e] * fxe;
-        const double xpn = f.xc[e] - f.xc[p], ypn = f.yc[e] - f.yc[p], zpn = f.zc[e] - f.zc[p];
-        const double dpn = sqrt(xpn * xpn + ypn * ypn + zpn * zpn);
-        const double ixi1 = xpn / dpn, ixi2 = ypn / dpn, ixi3 = zpn / dpn;
-        const double arx = D.arvx[p], ary = D.arvy[p], arz = D.arvz[p];
-        const double are = sqrt(arx * arx + ary * ary + arz * arz);
-        const double vole = xpn * arx + ypn * ary + zpn * arz;
-        const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
-        const double costn = nxx * ixi1 + nyy * ixi2 + nzz * ixi3;
-        const double d2x = xpn * costn, d2y = ypn * costn, d2z = zpn * costn;
-        const double dene = f.den[p] * fxp + f.den[e] * fxe;
-        const double game = dene * (fxp * f.vol[p] * f.apu[p] + fxe * f.vol[e] * f.apu[e]);
-        const double a = game * are / dpn;
+        double a, suadd = 0.0, fc = 0.0;
+        face_piso(f, D, ipc, p, a, suadd, fc);
         acfe[p] = a;
-        acfw[e] = a;
+        acfw[p + D.idew] = a;
         if (ipc > 0) {
-            const double dfixi = G3(f.gp, p, 0) * fxp + G3(f.gp, e, 0) * fxe;
-            const double dfiyi = G3(f.gp, p, 1) * fxp + G3(f.gp, e, 1) * fxe;
-            const double dfizi = G3(f.gp, p, 2) * fxp + G3(f.gp, e, 2) * fxe;
-            const double dfixii = dfixi * costn + arx / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
-            const double dfiyii = dfiyi * costn + ary / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
-            const double dfizii = dfizi * costn + arz / vole * (f.p[e] - f.p[p] - (dfixi * d2x + dfiyi * d2y + dfizi * d2z));
-            const double fdfie = game * (dfixii * arx + dfiyii * ary + dfizii * arz);
-            const double fdfii = game * are / dpn * (dfixi * xpn + dfiyi * ypn + dfizi * zpn);
-            const double suadd = fdfie - fdfii;
             su[p] = su[p] + suadd;
             bp[p] = suadd;
-            double duxi = G3(f.gu, p, 0) * fxp + G3(f.gu, e, 0) * fxe;
-            double duyi = G3(f.gu, p, 1) * fxp + G3(f.gu, e, 1) * fxe;
-            double duzi = G3(f.gu, p, 2) * fxp + G3(f.gu, e, 2) * fxe;
-            const double ui = f.u[p] * fxp + f.u[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
-            duxi = G3(f.gv, p, 0) * fxp + G3(f.gv, e, 0) * fxe;
-            duyi = G3(f.gv, p, 1) * fxp + G3(f.gv, e, 1) * fxe;
-            duzi = G3(f.gv, p, 2) * fxp + G3(f.gv, e, 2) * fxe;
-            const double vi = f.v[p] * fxp + f.v[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
-            duxi = G3(f.gw, p, 0) * fxp + G3(f.gw, e, 0) * fxe;
-            duyi = G3(f.gw, p, 1) * fxp + G3(f.gw, e, 1) * fxe;
-            duzi = G3(f.gw, p, 2) * fxp + G3(f.gw, e, 2) * fxe;
-            const double wi = f.w[p] * fxp + f.w[e] * fxe + duxi * (xf - xi) + duyi * (yf - yi) + duzi * (zf - zi);
-            fcf[p] = dene * (ui * arx + vi * ary + wi * arz);
+            fcf[p] = fc;
         }
     }
 }
@@ -734,6 +757,255 @@ __global__ __launch_bounds__(256) void k_fluxuvw_east(int ni, int nj, int nk, in
         o.su[p] = o.su[p] + o.bp[p];
         o.sv[p] = o.sv[p] + o.bt[p];
         o.sw[p] = o.sw[p] + o.bb[p];
+    }
+}
+
+// ------------------------------------------- three face directions in one pass
+// The drivers assemble the east, north and top faces with three calls of a
+// per-direction routine (each a face loop, then for PISO and fluxuvw a loop
+// that moves a share to the neighbour cell).  The fused kernels give every
+// array the value those three calls leave in it: a thread owns one cell c,
+// evaluates the faces the per-direction loops evaluate at c (for fluxuvw
+// also at its west, south and bottom neighbours), writes what the loops write
+// at c and at its east, north and top neighbours' coefficient entries, and
+// applies the per-direction updates of c's sources in the calls' order.
+// Each formula is the per-direction kernels' own (the face_* functions), so
+// every value is bit-identical.  A block is 256 consecutive (i, j) columns,
+// j fastest; it marches F3_KSEG planes, so the plane a face reads above c is
+// the one the block reads next.
+constexpr int F3_KSEG = 8;
+
+struct Out3 {
+    double *a[6];           // ae, aw, an, as, at, ab
+    const double *fcr[3];   // fluxuvw: the mass fluxes f1, f2, f3 it reads
+    double *fc[3];          // fluxmass / PISO: the mass fluxes f1, f2, f3 they write
+    double *bp[3];          // PISO: be, bn, bt; fluxuvw: its scratch bp, bt, bb
+    double *su[3];          // PISO: su; fluxuvw: su, sv, sw
+    const double *vis;
+};
+
+// direction q: 0 east (i + 1), 1 north (j + 1), 2 top (k + 1), as the drivers
+// pass them
+__device__ __forceinline__ FaceDir dir_of(const Fld &f, int q, int nj, int nij)
+{
+    FaceDir D;
+    if (q == 0) {
+        D.idew = nj; D.idns = 1; D.idtb = nij;
+        D.arvx = f.ar1x; D.arvy = f.ar1y; D.arvz = f.ar1z; D.fif = f.fx;
+    } else if (q == 1) {
+        D.idew = 1; D.idns = nij; D.idtb = nj;
+        D.arvx = f.ar2x; D.arvy = f.ar2y; D.arvz = f.ar2z; D.fif = f.fy;
+    } else {
+        D.idew = nij; D.idns = nj; D.idtb = 1;
+        D.arvx = f.ar3x; D.arvy = f.ar3y; D.arvz = f.ar3z; D.fif = f.fz;
+    }
+    return D;
+}
+
+struct Box {
+    int i0, i1, j0, j1, k0, k1;
+    __device__ __forceinline__ bool has(int i, int j, int k) const
+    {
+        return i >= i0 && i <= i1 && j >= j0 && j <= j1 && k >= k0 && k <= k1;
+    }
+};
+
+// (i, j) of this thread's column, 1-based; false past the plane
+__device__ __forceinline__ bool f3_column(int ni, int nj, int &i, int &j)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= (long)ni * nj) return false;
+    i = (int)(q / nj) + 1;
+    j = (int)(q % nj) + 1;
+    return true;
+}
+
+// fluxmass2.f90:41-192 for the three directions: faces k=2..nkm, i=2..nimm,
+// j=2..njmm
+__global__ __launch_bounds__(256) void k_fluxmass3(int ni, int nj, int nk, Fld f, Out3 o)
+{
+    int i, j;
+    if (!f3_column(ni, nj, i, j)) return;
+    const int nij = ni * nj;
+    const Box F = {2, ni - 2, 2, nj - 2, 2, nk - 1};
+    const int k0 = 1 + (int)blockIdx.y * F3_KSEG, k1 = min(k0 + F3_KSEG - 1, nk);
+    for (int k = k0; k <= k1; ++k) {
+        if (!F.has(i, j, k)) continue;
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const FaceDir D = dir_of(f, q, nj, nij);
+            double a, fc;
+            face_fluxmass(f, D, c, a, fc);
+            o.a[2 * q][c] = a;
+            o.a[2 * q + 1][c + D.idew] = a;
+            o.fc[q][c] = fc;
+        }
+    }
+}
+
+// PISO_assemble_pressure_eq.f90 for the three directions: bp zeroed, the face
+// loop (k=2..nkmm, i=2..nimm, j=2..njmm), then su(c) = su(c) - bp(c) for
+// c - idew in k,i,j = 3..nkmm, 3..nimm, 3..njmm (241-253), direction by
+// direction.  bp(c) there is c's own face's suadd (zero outside the face
+// range, where su is left as it is).
+__global__ __launch_bounds__(256) void k_piso3(int ni, int nj, int nk, Fld f, Out3 o, int ipc)
+{
+    int i, j;
+    if (!f3_column(ni, nj, i, j)) return;
+    const int nij = ni * nj;
+    const Box F = {2, ni - 2, 2, nj - 2, 2, nk - 2};
+    const Box C = {3, ni - 2, 3, nj - 2, 3, nk - 2};
+    const int k0 = 1 + (int)blockIdx.y * F3_KSEG, k1 = min(k0 + F3_KSEG - 1, nk);
+    for (int k = k0; k <= k1; ++k) {
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        if (!F.has(i, j, k)) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) o.bp[q][c] = 0.0;
+            continue;
+        }
+        double s = ipc > 0 ? o.su[0][c] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const FaceDir D = dir_of(f, q, nj, nij);
+            double a, suadd = 0.0, fc = 0.0;
+            face_piso(f, D, ipc, c, a, suadd, fc);
+            o.a[2 * q][c] = a;
+            o.a[2 * q + 1][c + D.idew] = a;
+            o.bp[q][c] = suadd;
+            if (ipc > 0) {
+                o.fc[q][c] = fc;
+                s = s + suadd;
+                if (C.has(i - (q == 0), j - (q == 1), k - (q == 2))) s = s - suadd;
+            }
+        }
+        if (ipc > 0) o.su[0][c] = s;
+    }
+}
+
+// fluxuvw-overrelaxed-correction.f90 with the CDS scheme alone: the face
+// east of cell p (direction D), as k_fluxuvw evaluates it with lcds: the
+// coefficient terms de, ce, cp and, per velocity component, the deferred
+// correction g = gam*(fuds - fh) and the explicit / implicit diffusion
+// fde, fdi (su(p) += g + fde - fdi; the east cell's share -g - fde + fdi)
+__device__ __forceinline__ void face_uvw_cds(const Fld &f, const FaceDir &D, const double *vis,
+                                             const double *fcf, size_t inp, double gam,
+                                             double &de, double &ce, double &cp, double (&g)[3],
+                                             double (&fde)[3], double (&fdi)[3])
+{
+    const size_t ine = inp + D.idew, ins = inp - D.idns;
+    const size_t inb = inp - D.idtb, inbs = inb - D.idns;
+    const double fxe = D.fif[inp], fxp = 1.0 - fxe;
+    const double xpn = f.xc[ine] - f.xc[inp], ypn = f.yc[ine] - f.yc[inp], zpn = f.zc[ine] - f.zc[inp];
+    const double dpn = sqrt(xpn * xpn + ypn * ypn + zpn * zpn);
+    const double arx = D.arvx[inp], ary = D.arvy[inp], arz = D.arvz[inp];
+    const double are = sqrt(arx * arx + ary * ary + arz * arz);
+    const double costn = 1.0;
+    const double vole = xpn * arx + ypn * ary + zpn * arz;
+    const double d1x = costn, d1y = costn, d1z = costn;
+    const double d2x = xpn * costn, d2y = ypn * costn, d2z = zpn * costn;
+    const double game = vis[inp] * fxp + vis[ine] * fxe;
+    const double xf = 0.25 * (f.x[inp] + f.x[ins] + f.x[inb] + f.x[inbs]);
+    const double yf = 0.25 * (f.y[inp] + f.y[ins] + f.y[inb] + f.y[inbs]);
+    const double zf = 0.25 * (f.z[inp] + f.z[ins] + f.z[inb] + f.z[inbs]);
+    const double xi = f.xc[inp] * fxp + f.xc[ine] * fxe;
+    const double yi = f.yc[inp] * fxp + f.yc[ine] * fxe;
+    const double zi = f.zc[inp] * fxp + f.zc[ine] * fxe;
+    const double *vel[3] = {f.u, f.v, f.w};
+    const double *grd[3] = {f.gu, f.gv, f.gw};
+    double dxi[3][3], dxii[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double *gr = grd[c], *vq = vel[c];
+        dxi[c][0] = G3(gr, inp, 0) * fxp + G3(gr, ine, 0) * fxe;
+        dxi[c][1] = G3(gr, inp, 1) * fxp + G3(gr, ine, 1) * fxe;
+        dxi[c][2] = G3(gr, inp, 2) * fxp + G3(gr, ine, 2) * fxe;
+        const double dd = vq[ine] - vq[inp] - dxi[c][0] * d2x - dxi[c][1] * d2y - dxi[c][2] * d2z;
+        dxii[c][0] = dxi[c][0] * d1x + arx / vole * dd;
+        dxii[c][1] = dxi[c][1] * d1y + ary / vole * dd;
+        dxii[c][2] = dxi[c][2] * d1z + arz / vole * dd;
+    }
+    const double duxii = dxii[0][0], duyii = dxii[0][1], duzii = dxii[0][2];
+    const double dvxii = dxii[1][0], dvyii = dxii[1][1], dvzii = dxii[1][2];
+    const double dwxii = dxii[2][0], dwyii = dxii[2][1], dwzii = dxii[2][2];
+    fde[0] = game * ((duxii + duxii) * arx + (duyii + dvxii) * ary + (duzii + dwxii) * arz);
+    fde[1] = game * ((duyii + dvxii) * arx + (dvyii + dvyii) * ary + (dvzii + dwyii) * arz);
+    fde[2] = game * ((duzii + dwxii) * arx + (dwyii + dvzii) * ary + (dwzii + dwzii) * arz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        fdi[c] = game * are / dpn * (dxi[c][0] * xpn + dxi[c][1] * ypn + dxi[c][2] * zpn);
+    de = game * are / dpn;
+    const double flcf = fcf[inp];
+    ce = flcf < 0.0 ? flcf : 0.0;
+    cp = flcf > 0.0 ? flcf : 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double fuds = cp * vel[c][inp] + ce * vel[c][ine];
+        const double ue = vel[c][inp] * fxp + vel[c][ine] * fxe +
+                          (dxi[c][0] * (xf - xi) + dxi[c][1] * (yf - yi) + dxi[c][2] * (zf - zi));
+        const double fh = flcf * ue;
+        g[c] = gam * (fuds - fh);
+    }
+}
+
+// fluxuvw (CDS) for the three directions.  Direction q's face loop covers
+// k=kst..nkmm, i=ist..nimm, j=jst..njmm with the starts of 99-118; the east
+// loop (632-644) adds the scratch bp/bt/bb of c to su/sv/sw(c) for c - idew in
+// k,i,j = 2..nkmm, 2..nimm, 2..njmm, and the call ends by zeroing the
+// scratch.  So the east loop adds the share the west/south/bottom face wrote
+// at c, or, where that face is outside the face loop, the scratch's value:
+// what the caller left in it for the east call, zero for the later calls.
+__global__ __launch_bounds__(256) void k_uvw3(int ni, int nj, int nk, Fld f, Out3 o, double gam)
+{
+    int i, j;
+    if (!f3_column(ni, nj, i, j)) return;
+    const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
+    const Box G = {2, nimm, 2, njmm, 2, nkmm};
+    const Box F[3] = {{2, nimm, 3, njmm, 3, nkmm}, {3, nimm, 2, njmm, 3, nkmm},
+                      {3, nimm, 3, njmm, 2, nkmm}};
+    const int k0 = 1 + (int)blockIdx.y * F3_KSEG, k1 = min(k0 + F3_KSEG - 1, nk);
+    for (int k = k0; k <= k1; ++k) {
+        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+        double scr0[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            scr0[m] = o.bp[m][c];
+            o.bp[m][c] = 0.0;
+        }
+        bool touched = false;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            touched |= F[q].has(i, j, k) || G.has(i - (q == 0), j - (q == 1), k - (q == 2));
+        if (!touched) continue;
+        double s[3] = {o.su[0][c], o.su[1][c], o.su[2][c]};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const FaceDir D = dir_of(f, q, nj, nij);
+            double de, ce, cp, g[3], fde[3], fdi[3];
+            if (F[q].has(i, j, k)) {
+                face_uvw_cds(f, D, o.vis, o.fcr[q], c, gam, de, ce, cp, g, fde, fdi);
+                o.a[2 * q][c] = de - ce;
+#pragma unroll
+                for (int m = 0; m < 3; ++m) s[m] = s[m] + g[m] + fde[m] - fdi[m];
+            }
+            const int iw = i - (q == 0), jw = j - (q == 1), kw = k - (q == 2);
+            if (G.has(iw, jw, kw)) {
+                double sh[3];
+                if (F[q].has(iw, jw, kw)) {
+                    face_uvw_cds(f, D, o.vis, o.fcr[q], c - D.idew, gam, de, ce, cp, g, fde, fdi);
+                    o.a[2 * q + 1][c] = de + cp;
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) sh[m] = -g[m] - fde[m] + fdi[m];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) sh[m] = q == 0 ? scr0[m] : 0.0;
+                }
+#pragma unroll
+                for (int m = 0; m < 3; ++m) s[m] = s[m] + sh[m];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) o.su[m][c] = s[m];
     }
 }
 
@@ -1141,6 +1413,112 @@ static int out3(cap_ctx *ctx, double *a, double *b, double *c, double **da, doub
 }
 
 static int blocks_for(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
+
+// ------------------------------------ the three directions fused (drivers)
+// Device pointers only; queued on the context's stream, not synchronised.
+static dim3 f3_grid(const cap_ctx *ctx)
+{
+    const long cols = (long)ctx->ni * ctx->nj;
+    return dim3((unsigned)((cols + 255) / 256), (unsigned)((ctx->nk + F3_KSEG - 1) / F3_KSEG));
+}
+
+int cap_fluxmass3_q(cap_ctx *ctx, const cap_fields *F, double *const *a6, double *const *f3)
+{
+    if (!(ctx->flags & CAP_DEVICE_POINTERS)) return CAP_ERR_ARG;
+    Fld f;
+    int rc;
+    if ((rc = stage_fields(ctx, F, f))) return rc;
+    Out3 o = {};
+    for (int q = 0; q < 6; ++q) o.a[q] = a6[q];
+    for (int q = 0; q < 3; ++q) o.fc[q] = f3[q];
+    hipLaunchKernelGGL(k_fluxmass3, f3_grid(ctx), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj,
+                       ctx->nk, f, o);
+    HIPCHK(hipGetLastError());
+    return CAP_OK;
+}
+
+int cap_piso_assemble3_q(cap_ctx *ctx, const cap_fields *F, double *const *a6, double *const *f3,
+                         int ipc, double *const *bp3, double *su)
+{
+    if (!(ctx->flags & CAP_DEVICE_POINTERS)) return CAP_ERR_ARG;
+    Fld f;
+    int rc;
+    if ((rc = stage_fields(ctx, F, f))) return rc;
+    Out3 o = {};
+    for (int q = 0; q < 6; ++q) o.a[q] = a6[q];
+    for (int q = 0; q < 3; ++q) { o.fc[q] = f3[q]; o.bp[q] = bp3[q]; }
+    o.su[0] = su;
+    hipLaunchKernelGGL(k_piso3, f3_grid(ctx), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj, ctx->nk,
+                       f, o, ipc);
+    HIPCHK(hipGetLastError());
+    return CAP_OK;
+}
+
+int cap_fluxuvw3_q(cap_ctx *ctx, const cap_fields *F, const double *vis, double *const *a6,
+                   const double *const *f3, double *const *s3, double *const *scr3, double gds)
+{
+    if (!(ctx->flags & CAP_DEVICE_POINTERS) || ctx->ni < 6 || ctx->nj < 6 || ctx->nk < 6)
+        return CAP_ERR_ARG;
+    Fld f;
+    int rc;
+    if ((rc = stage_fields(ctx, F, f))) return rc;
+    Out3 o = {};
+    for (int q = 0; q < 6; ++q) o.a[q] = a6[q];
+    for (int q = 0; q < 3; ++q) { o.fcr[q] = f3[q]; o.su[q] = s3[q]; o.bp[q] = scr3[q]; }
+    o.vis = vis;
+    hipLaunchKernelGGL(k_uvw3, f3_grid(ctx), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj, ctx->nk,
+                       f, o, gds);
+    HIPCHK(hipGetLastError());
+    return CAP_OK;
+}
+
+extern "C" int cap_fluxmass3(cap_ctx *ctx, const cap_fields *F, double *ae, double *aw,
+                             double *an, double *as, double *at, double *ab, double *f1,
+                             double *f2, double *f3)
+{
+    if (!ctx || !F || !ae || !aw || !an || !as || !at || !ab || !f1 || !f2 || !f3)
+        return CAP_ERR_ARG;
+    double *const a6[6] = {ae, aw, an, as, at, ab}, *const fc[3] = {f1, f2, f3};
+    int rc;
+    if ((rc = cap_fluxmass3_q(ctx, F, a6, fc))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_piso_assemble_pressure_eq3(cap_ctx *ctx, const cap_fields *F, double *ae,
+                                              double *aw, double *an, double *as, double *at,
+                                              double *ab, double *f1, double *f2, double *f3,
+                                              int ipc, double *be, double *bn, double *bt,
+                                              double *su)
+{
+    if (!ctx || !F || !ae || !aw || !an || !as || !at || !ab || !f1 || !f2 || !f3 || !be ||
+        !bn || !bt || !su)
+        return CAP_ERR_ARG;
+    double *const a6[6] = {ae, aw, an, as, at, ab}, *const fc[3] = {f1, f2, f3};
+    double *const bp3[3] = {be, bn, bt};
+    int rc;
+    if ((rc = cap_piso_assemble3_q(ctx, F, a6, fc, ipc, bp3, su))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
+
+extern "C" int cap_fluxuvw3(cap_ctx *ctx, const cap_fields *F, const double *vis, double *ae,
+                            double *aw, double *an, double *as, double *at, double *ab,
+                            const double *f1, const double *f2, const double *f3, double *su,
+                            double *sv, double *sw, double *bp, double *bt, double *bb,
+                            int schemes, double gds)
+{
+    if (!ctx || !F || !vis || !ae || !aw || !an || !as || !at || !ab || !f1 || !f2 || !f3 ||
+        !su || !sv || !sw || !bp || !bt || !bb || schemes != CAP_UVW_CDS)
+        return CAP_ERR_ARG;
+    double *const a6[6] = {ae, aw, an, as, at, ab};
+    const double *const fc[3] = {f1, f2, f3};
+    double *const s3[3] = {su, sv, sw}, *const scr3[3] = {bp, bt, bb};
+    int rc;
+    if ((rc = cap_fluxuvw3_q(ctx, F, vis, a6, fc, s3, scr3, gds))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return CAP_OK;
+}
 
 extern "C" int cap_fluxmass(cap_ctx *ctx, const cap_fields *F, int idew, int idns, int idtb,
                             const double *arvx, const double *arvy, const double *arvz,

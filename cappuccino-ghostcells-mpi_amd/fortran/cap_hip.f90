@@ -237,6 +237,28 @@ module cap_hip
       integer(c_int), value :: idew, idns, idtb, schemes
       real(c_double), value :: gds
     end function
+    ! the drivers' three per-direction calls in one pass (device pointers)
+    integer(c_int) function cap_fluxmass3(ctx, f, ae, aw, an, as, at, ab, f1, f2, f3) &
+        bind(c, name='cap_fluxmass3')
+      import :: c_ptr, c_int, cap_fields
+      type(c_ptr), value :: ctx, ae, aw, an, as, at, ab, f1, f2, f3
+      type(cap_fields), intent(in) :: f
+    end function
+    integer(c_int) function cap_piso_assemble_pressure_eq3(ctx, f, ae, aw, an, as, at, ab, &
+        f1, f2, f3, ipc, be, bn, bt, su) bind(c, name='cap_piso_assemble_pressure_eq3')
+      import :: c_ptr, c_int, cap_fields
+      type(c_ptr), value :: ctx, ae, aw, an, as, at, ab, f1, f2, f3, be, bn, bt, su
+      type(cap_fields), intent(in) :: f
+      integer(c_int), value :: ipc
+    end function
+    integer(c_int) function cap_fluxuvw3(ctx, f, vis, ae, aw, an, as, at, ab, f1, f2, f3, &
+        su, sv, sw, bp, bt, bb, schemes, gds) bind(c, name='cap_fluxuvw3')
+      import :: c_ptr, c_int, c_double, cap_fields
+      type(c_ptr), value :: ctx, vis, ae, aw, an, as, at, ab, f1, f2, f3, su, sv, sw, bp, bt, bb
+      type(cap_fields), intent(in) :: f
+      integer(c_int), value :: schemes
+      real(c_double), value :: gds
+    end function
     integer(c_int) function cap_courant(ctx, f1, f2, f3, vol, su, timestep, time, conumfix, &
         conumfixvalue, conum) bind(c, name='cap_courant')
       import :: c_ptr, c_int, c_double

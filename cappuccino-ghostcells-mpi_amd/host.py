@@ -43,7 +43,8 @@ EXPORTS = ("cap_init", "cap_finalize", "cap_last_error", "cap_sipsol", "cap_pcgs
            "cap_calcuvw", "cap_find_strain_rate", "cap_calc_vis_les", "cap_courant_noalloc",
            "cap_face_areas", "cap_intfac", "cap_grid_dims", "cap_read_grid", "cap_cgstab",
            "cap_pcg_jacobi", "cap_update_ghostcells_periodic", "cap_piso_guess",
-           "cap_pcgsip_fcg")
+           "cap_pcgsip_fcg", "cap_fluxmass3", "cap_piso_assemble_pressure_eq3",
+           "cap_fluxuvw3")
 FIELD_NAMES = ("x", "y", "z", "xc", "yc", "zc", "vol", "fx", "fy", "fz", "ar1x", "ar1y",
                "ar1z", "ar2x", "ar2y", "ar2z", "ar3x", "ar3y", "ar3z", "den", "apu", "apv",
                "apw", "u", "v", "w", "p", "gradu", "gradv", "gradw", "gradp")
@@ -152,6 +153,9 @@ def load():
     L.cap_fluxmc.argtypes = [vp, F, i, i, i, vp, vp]
     L.cap_continuity.argtypes = [vp, vp, vp, vp, C.POINTER(d)]
     L.cap_fluxuvw.argtypes = [vp, F, vp, i, i, i] + [vp] * 13 + [i, d]
+    L.cap_fluxmass3.argtypes = [vp, F] + [vp] * 9
+    L.cap_piso_assemble_pressure_eq3.argtypes = [vp, F] + [vp] * 9 + [i] + [vp] * 4
+    L.cap_fluxuvw3.argtypes = [vp, F] + [vp] * 16 + [i, d]
     L.cap_update_ghostcells.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(_Bc)]
     L.cap_update_ghostcells_periodic.argtypes = [vp, C.POINTER(vp), vp]
     L.cap_courant.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(d), C.POINTER(d), i, d,
@@ -396,6 +400,31 @@ class Context:
             self._f(f[("fx", "fy", "fz")[d]]), self._f(acfe), self._f(acfw), self._f(fcf), self._f(su),
             self._f(sv), self._f(sw), self._f(bp), self._f(bt), self._f(bb), int(schemes), float(gds))
         self._check(rc, "cap_fluxuvw", allow=(CAP_OK,))
+
+    def fluxmass3(self, f, a6, f3):
+        """The three directions of fluxmass in one pass: a6 = [ae, aw, an, as,
+        at, ab], f3 = [f1, f2, f3] (device arrays)."""
+        st = self._fields(f)
+        rc = self._lib.cap_fluxmass3(self._h, C.byref(st), *map(self._f, a6), *map(self._f, f3))
+        self._check(rc, "cap_fluxmass3", allow=(CAP_OK,))
+
+    def piso_assemble_pressure_eq3(self, f, a6, f3, ipc, bp3, su):
+        """The three directions of piso_assemble_pressure_eq in one pass; bp3 =
+        [be, bn, bt]."""
+        st = self._fields(f)
+        rc = self._lib.cap_piso_assemble_pressure_eq3(
+            self._h, C.byref(st), *map(self._f, a6), *map(self._f, f3), int(ipc),
+            *map(self._f, bp3), self._f(su))
+        self._check(rc, "cap_piso_assemble_pressure_eq3", allow=(CAP_OK,))
+
+    def fluxuvw3(self, f, vis, a6, f3, s3, scr3, schemes, gds):
+        """calcuvw's three fluxuvw calls (CDS) in one pass: s3 = [su, sv, sw],
+        scr3 = the scratch [bp, bt, bb]."""
+        st = self._fields(f)
+        rc = self._lib.cap_fluxuvw3(self._h, C.byref(st), self._f(vis), *map(self._f, a6),
+                                    *map(self._f, f3), *map(self._f, s3), *map(self._f, scr3),
+                                    int(schemes), float(gds))
+        self._check(rc, "cap_fluxuvw3", allow=(CAP_OK,))
 
     def update_ghostcells(self, var, geom, lb, inl):
         """var: [u, v, w, te, ed, t, vis] (updated in place); geom: [fx, fy, fz,

@@ -178,6 +178,22 @@ int cap_assemble_pressure_eq(cap_ctx *ctx, const cap_fields *f, int idew, int id
 int cap_fluxmc(cap_ctx *ctx, const cap_fields *f, int idew, int idns, int idtb,
                const double *fif, double *fmcor);
 
+/* The drivers' three calls in one pass (no single counterpart in the
+ * reference, which calls these routines once per direction: east with
+ * (idew,idns,idtb) = (nj,1,nij) and fx, ar1*; north (1,nij,nj), fy, ar2*; top
+ * (nij,nj,1), fz, ar3*).  Each leaves every array bit for bit as the three
+ * calls do: cap_fluxmass3 = cap_fluxmass for the three directions
+ * (calcp-multiple_correction_SIMPLE.f90:50-60), cap_piso_assemble_pressure_eq3
+ * = cap_piso_assemble_pressure_eq for them with bp = be, bn, bt
+ * (PISO_multiple_correction.f90).  Device pointers only (CAP_DEVICE_POINTERS
+ * contexts); other contexts get CAP_ERR_ARG. */
+int cap_fluxmass3(cap_ctx *ctx, const cap_fields *f, double *ae, double *aw, double *an,
+                  double *as, double *at, double *ab, double *f1, double *f2, double *f3);
+int cap_piso_assemble_pressure_eq3(cap_ctx *ctx, const cap_fields *f, double *ae, double *aw,
+                                   double *an, double *as, double *at, double *ab, double *f1,
+                                   double *f2, double *f3, int ipc, double *be, double *bn,
+                                   double *bt, double *su);
+
 /* ---- Pressure-correction drivers ---------------------------------------
  * Module state of calcp / PISO / PIMPLE (modules geometry, variables, coef,
  * coefb, hcoef, gradients of modules_allocatable.f90), all device (HBM)
@@ -430,6 +446,16 @@ int cap_fluxuvw(cap_ctx *ctx, const cap_fields *f, const double *vis, int idew, 
                 const double *fif, double *acfe, double *acfw, const double *fcf, double *su,
                 double *sv, double *sw, double *bp, double *bt, double *bb, int schemes,
                 double gds);
+
+/* calcuvw.f90:100-140's three cap_fluxuvw calls (east, north, top, with f1,
+ * f2, f3 and ae/aw, an/as, at/ab) in one pass, for the CDS scheme alone
+ * (schemes must be CAP_UVW_CDS); every array ends bit for bit as after the
+ * three calls, bp, bt, bb zeroed.  No single counterpart in the reference.
+ * Device pointers only. */
+int cap_fluxuvw3(cap_ctx *ctx, const cap_fields *f, const double *vis, double *ae, double *aw,
+                 double *an, double *as, double *at, double *ab, const double *f1,
+                 const double *f2, const double *f3, double *su, double *sv, double *sw,
+                 double *bp, double *bt, double *bb, int schemes, double gds);
 
 /* The continuity-error sums of continuityErrors.h:55-81 over k=2..nkm,
  * i,j=3..n?mm: sums[0] = global (signed), sums[1] = sum of |local|.  FP64,

@@ -20,8 +20,12 @@ dp = C.POINTER(C.c_double)
 
 @pytest.fixture(scope="module")
 def E():
+    # built under a per-process name and moved into place, so that parallel
+    # test workers never load a library another one is still writing
+    tmp = f"{LIB}.{os.getpid()}"
     subprocess.run(["g++", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c++17",
-                    "-o", LIB, SRC], check=True)
+                    "-o", tmp, SRC], check=True)
+    os.replace(tmp, LIB)
     L = C.CDLL(LIB)
     L.emu_wave.restype = C.c_double
     L.emu_wave.argtypes = [C.c_int] * 6 + [C.c_double, C.POINTER(dp), C.POINTER(dp),
