@@ -62,14 +62,16 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=24,
                     help="PCG-SIP iterations of the CPU baseline sample (0: skip)")
     ap.add_argument("--others",
-                    default="cgstab_sip,iccg,cgstab,pcg_jacobi,mixed,fcg,learned,unet,channel,"
+                    default="cgstab_sip,iccg,cgstab,pcg_jacobi,mixed,fcg,learned,unet,stencils,channel,"
                             "channel_mixed,channel_mixed_warm,learned_channel,channel_mixed_learned,"
                             "channel_fp32_flux,host",
                     help="comma list of extra solves to time once each (fcg: PCG-SIP with "
                          "the preconditioner in FP32; learned: the U-Net "
                          "guess, trained on solver-generated 64^3 pairs, then the mixed and the "
                          "FP64 solve from it to the same absolute residual; unet: the learned "
-                         "guess's forward pass and training step on 128x96x64; channel: time "
+                         "guess's forward pass and training step on 128x96x64; stencils: the fused "
+                         "three-direction assembly passes against the per-direction calls and "
+                         "one Gauss gradient on a 256^3 channel state; channel: time "
                          "steps (Courant, LES, calcuvw, PISO) of the 384x256x192 channel with "
                          "the FP64 p' solve; channel_mixed: the same with the mixed-precision "
                          "p' solve; channel_mixed_warm: the same with the last corrector "
@@ -90,6 +92,9 @@ def main():
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: 3 PCG-SIP iterations, then "
                          "a 1 GiB device copy as the byte-counter calibration; prints one line")
+    ap.add_argument("--pmc-stencils", action="store_true",
+                    help="with --pmc-sample: also the stencil leg once (one call of each pass "
+                         "on the 256^3 channel state) before the copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -123,6 +128,8 @@ def main():
     if args.pmc_sample:
         fi.zero_()
         r = ctx.pcgsip(fi, coef, alfa=0.0, sor=1e-8, nsw=3)
+        if args.pmc_stencils:
+            _bench_stencils(cap, torch, reps=1)
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -187,6 +194,9 @@ def main():
             continue
         elif other == "unet":
             solvers["unet"] = _bench_unet(cap, torch)
+            continue
+        elif other == "stencils":
+            solvers["stencils"] = _bench_stencils(cap, torch)
             continue
         elif other == "channel":
             solvers["channel_fp64"] = _bench_channel(cap, torch, max(1, args.channel_steps // 10), 4)
@@ -502,6 +512,81 @@ def _bench_unet(cap, torch, nx=128, ny=96, nz=64, reps=5):
             "forward_TFLOPs": round(tf, 2), "mfma_frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
             "train_step_ms": round(trn, 3),
             "note": "host-side wall time per call including launch and synchronisation"}
+
+
+# bytes per interior cell of the fused three-direction passes (the distinct
+# arrays one pass reads and writes, 8 B each; fluxmass3 equals the 384 B/cell
+# of fluxmass2.f90's three directions) and of one Gauss gradient
+STENCIL_BYTES = {"fluxmass": 384, "piso_assemble": 408, "fluxuvw": 416, "grad_gauss": 136}
+
+
+def _bench_stencils(cap, torch, n=256, reps=5):
+    """The p' assembly stencils on an n^3 channel-geometry state (periodic x,
+    z, tanh-stretched y, after one time step): the drivers' fused
+    three-direction passes against the three per-direction entry points they
+    replace, and one Gauss gradient.  ms per call (host wall time around
+    synchronised calls) and GB/s over each fused pass's distinct arrays."""
+    CH = _channel_module()
+    ch = CH.Channel(cap, torch, n, n, n, solver=6)
+    try:
+        ch.step(timed=False)
+        ctx, f = ch.ctx, ch.f
+        F = {q: f[q] for q in cap.host.FIELD_NAMES}
+        a6 = [f[q] for q in ("ae", "aw", "an", "as", "at", "ab")]
+        f3 = [f[q] for q in ("f1", "f2", "f3")]
+        scr = [torch.zeros_like(f["u"]) for _ in range(3)]
+        bp3 = [f[q] for q in ("be", "bn", "bt")]
+        su3 = [f[q] for q in ("su", "sv", "sw")]
+        dirs = (("ae", "aw", "f1", "be"), ("an", "as", "f2", "bn"), ("at", "ab", "f3", "bt"))
+        geom = {q: f[q] for q in cap.host.GEOM_FIELDS}
+        g3 = [torch.empty_like(f["u"]) for _ in range(3)]
+
+        def per_dir(kind):
+            for d, (ae, aw, fc, bp) in enumerate(dirs):
+                if kind == "fluxmass":
+                    ctx.fluxmass(F, d, f[ae], f[aw], f[fc])
+                elif kind == "piso_assemble":
+                    ctx.piso_assemble_pressure_eq(F, d, f[ae], f[aw], f[fc], 1, f[bp], f["su"])
+                else:
+                    ctx.fluxuvw(F, d, f["vis"], f[ae], f[aw], f[fc], *su3, *scr, 1, 1.0)
+
+        def fused(kind):
+            if kind == "fluxmass":
+                ctx.fluxmass3(F, a6, f3)
+            elif kind == "piso_assemble":
+                ctx.piso_assemble_pressure_eq3(F, a6, f3, 1, bp3, f["su"])
+            else:
+                ctx.fluxuvw3(F, f["vis"], a6, f3, su3, scr, 1, 1.0)
+
+        def timed(fn, *a):
+            fn(*a)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn(*a)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / reps
+
+        ncell = n ** 3
+        res = {"grid": [n, n, n], "reps": reps}
+        for kind in ("fluxmass", "piso_assemble", "fluxuvw"):
+            t3, t1 = timed(per_dir, kind), timed(fused, kind)
+            gbs = STENCIL_BYTES[kind] * ncell / (t1 * 1e-3) / 1e9
+            res[kind] = {"three_calls_ms": round(t3, 3), "fused_ms": round(t1, 3),
+                         "fused_GBps": round(gbs, 1), "fused_frac": round(gbs / HBM_PEAK_GBS, 3),
+                         "bytes_per_cell": STENCIL_BYTES[kind]}
+            log(f"[bench] {kind} {n}^3: three calls {t3:.3f} ms, fused {t1:.3f} ms "
+                f"({gbs:.0f} GB/s over {STENCIL_BYTES[kind]} B/cell)")
+        tg = timed(ctx.grad_gauss, f["u"], geom, *g3)
+        gbs = STENCIL_BYTES["grad_gauss"] * ncell / (tg * 1e-3) / 1e9
+        res["grad_gauss"] = {"ms": round(tg, 3), "GBps": round(gbs, 1),
+                             "frac": round(gbs / HBM_PEAK_GBS, 3), "bytes_per_cell": 136}
+        log(f"[bench] grad_gauss {n}^3: {tg:.3f} ms ({gbs:.0f} GB/s)")
+        res["note"] = ("host wall time per synchronised call; each per-direction entry point "
+                       "synchronises its stream once")
+        return res
+    finally:
+        ch.close()
 
 
 def _channel_module():

@@ -763,24 +763,25 @@ __global__ __launch_bounds__(256) void k_fluxuvw_east(int ni, int nj, int nk, in
 // ------------------------------------------- three face directions in one pass
 // The drivers assemble the east, north and top faces with three calls of a
 // per-direction routine (each a face loop, then for PISO and fluxuvw a loop
-// that moves a share to the neighbour cell).  The fused kernels give every
-// array the value those three calls leave in it: a thread owns one cell c,
-// evaluates the faces the per-direction loops evaluate at c (for fluxuvw
-// also at its west, south and bottom neighbours), writes what the loops write
-// at c and at its east, north and top neighbours' coefficient entries, and
-// applies the per-direction updates of c's sources in the calls' order.
-// Each formula is the per-direction kernels' own (the face_* functions), so
-// every value is bit-identical.  A block is 256 consecutive (i, j) columns,
-// j fastest; it marches F3_KSEG planes, so the plane a face reads above c is
-// the one the block reads next.
+// that moves a share to the neighbour cell).  The fused passes give every
+// array the value those three calls leave in it, with the per-direction
+// kernels' own face formulas (the face_* functions), so every value is
+// bit-identical:
+//  * fluxmass and PISO: one face per thread, the three directions in one grid
+//    with the blocks of one run of faces on one XCD (k_fluxmass3x,
+//    k_piso3x), then for PISO a pass over the cells for su (k_piso3_su);
+//  * fluxuvw: its su/sv/sw take six terms a cell in a fixed order, so a
+//    thread owns a cell c (k_uvw3), evaluates its east/north/top faces and
+//    its west/south/bottom neighbours' (whose shares reach c through the
+//    second loop), and applies the terms in the calls' order.  A block is 256
+//    consecutive (i, j) columns, j fastest, marching F3_KSEG planes.
 constexpr int F3_KSEG = 8;
 
 struct Out3 {
     double *a[6];           // ae, aw, an, as, at, ab
-    const double *fcr[3];   // fluxuvw: the mass fluxes f1, f2, f3 it reads
-    double *fc[3];          // fluxmass / PISO: the mass fluxes f1, f2, f3 they write
-    double *bp[3];          // PISO: be, bn, bt; fluxuvw: its scratch bp, bt, bb
-    double *su[3];          // PISO: su; fluxuvw: su, sv, sw
+    const double *fcr[3];   // the mass fluxes f1, f2, f3
+    double *bp[3];          // the scratch bp, bt, bb
+    double *su[3];          // su, sv, sw
     const double *vis;
 };
 
@@ -810,76 +811,121 @@ struct Box {
     }
 };
 
-// (i, j) of this thread's column, 1-based; false past the plane
-__device__ __forceinline__ bool f3_column(int ni, int nj, int &i, int &j)
+// (i, j) of this thread's column, 1-based, and the first plane k0 of its
+// segment; false past the plane.  Block b runs on XCD b % 8; each XCD takes
+// a contiguous eighth of the column blocks, so the blocks an XCD runs one
+// after the other hold neighbouring rows of columns (a face reads the row
+// beside its cell), all of one plane segment.
+__device__ __forceinline__ bool f3_column(int ni, int nj, int &i, int &j, int &k0)
 {
-    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    const int ncb = (int)(((long)ni * nj + 255) / 256), per = (ncb + 7) / 8;
+    const int b = (int)blockIdx.x, w = b >> 3;
+    const int cbl = (b & 7) * per + w % per;
+    k0 = 1 + (w / per) * F3_KSEG;
+    if (cbl >= ncb) return false;
+    const long q = (long)cbl * 256 + threadIdx.x;
     if (q >= (long)ni * nj) return false;
     i = (int)(q / nj) + 1;
     j = (int)(q % nj) + 1;
     return true;
 }
 
-// fluxmass2.f90:41-192 for the three directions: faces k=2..nkm, i=2..nimm,
-// j=2..njmm
-__global__ __launch_bounds__(256) void k_fluxmass3(int ni, int nj, int nk, Fld f, Out3 o)
+// One face per thread, the three directions in one grid: block b runs on XCD
+// b % 8 (blocks are dispatched to the XCDs in turn), and the three
+// directions of one run of 256 faces take blocks b, b + 8, b + 16 of the same
+// XCD, so the cell values the three faces share are read from HBM once and
+// from that XCD's L2 after.  Measured at 256^3 against two other mappings:
+// each XCD taking a contiguous eighth of the runs (so that the row beside a
+// face's cell is on the same XCD too) 2.49 against 2.27 ms for fluxmass;
+// tiles of 64 x 4 faces marching 8 planes 2.97 ms.  Per block: run cb of
+// the face loop, direction q.
+__device__ __forceinline__ bool f3x_item(int nruns, int &cb, int &q)
 {
-    int i, j;
-    if (!f3_column(ni, nj, i, j)) return;
-    const int nij = ni * nj;
-    const Box F = {2, ni - 2, 2, nj - 2, 2, nk - 1};
-    const int k0 = 1 + (int)blockIdx.y * F3_KSEG, k1 = min(k0 + F3_KSEG - 1, nk);
-    for (int k = k0; k <= k1; ++k) {
-        if (!F.has(i, j, k)) continue;
-        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const FaceDir D = dir_of(f, q, nj, nij);
-            double a, fc;
-            face_fluxmass(f, D, c, a, fc);
-            o.a[2 * q][c] = a;
-            o.a[2 * q + 1][c + D.idew] = a;
-            o.fc[q][c] = fc;
-        }
-    }
+    const int b = (int)blockIdx.x, xcd = b & 7, w = b >> 3;
+    q = w % 3;
+    cb = (w / 3) * 8 + xcd;
+    return cb < nruns;
+}
+static inline int f3x_blocks(int nruns) { return (nruns + 7) / 8 * 8 * 3; }
+
+__global__ __launch_bounds__(256) void k_fluxmass3x(int ni, int nj, int nk, Fld f,
+                                                    double *__restrict__ ae, double *__restrict__ aw,
+                                                    double *__restrict__ an, double *__restrict__ as,
+                                                    double *__restrict__ at, double *__restrict__ ab,
+                                                    double *__restrict__ f1, double *__restrict__ f2,
+                                                    double *__restrict__ f3)
+{
+    const size_t nf = (size_t)(nk - 2) * (ni - 3) * (nj - 3);
+    int cb, q;
+    if (!f3x_item((int)((nf + 255) / 256), cb, q)) return;
+    const size_t t = (size_t)cb * 256 + threadIdx.x;
+    size_t p;
+    if (t >= nf || !face_of(ni, nj, 2, nk - 1, t, p)) return;
+    const FaceDir D = dir_of(f, q, nj, ni * nj);
+    double a, fc;
+    face_fluxmass(f, D, p, a, fc);
+    double *const acfe = q == 0 ? ae : (q == 1 ? an : at);
+    double *const acfw = q == 0 ? aw : (q == 1 ? as : ab);
+    double *const fcf = q == 0 ? f1 : (q == 1 ? f2 : f3);
+    acfe[p] = a;
+    acfw[p + D.idew] = a;
+    fcf[p] = fc;
 }
 
-// PISO_assemble_pressure_eq.f90 for the three directions: bp zeroed, the face
-// loop (k=2..nkmm, i=2..nimm, j=2..njmm), then su(c) = su(c) - bp(c) for
-// c - idew in k,i,j = 3..nkmm, 3..nimm, 3..njmm (241-253), direction by
-// direction.  bp(c) there is c's own face's suadd (zero outside the face
-// range, where su is left as it is).
-__global__ __launch_bounds__(256) void k_piso3(int ni, int nj, int nk, Fld f, Out3 o, int ipc)
+// PISO's face loop for the three directions (bp = be, bn, bt written on the
+// face range); k_piso3_su then finishes each cell
+__global__ __launch_bounds__(256) void k_piso3x(int ni, int nj, int nk, Fld f, int ipc,
+                                                double *__restrict__ ae, double *__restrict__ aw,
+                                                double *__restrict__ an, double *__restrict__ as,
+                                                double *__restrict__ at, double *__restrict__ ab,
+                                                double *__restrict__ f1, double *__restrict__ f2,
+                                                double *__restrict__ f3, double *__restrict__ be,
+                                                double *__restrict__ bn, double *__restrict__ bt)
 {
-    int i, j;
-    if (!f3_column(ni, nj, i, j)) return;
-    const int nij = ni * nj;
+    const size_t nf = (size_t)(nk - 3) * (ni - 3) * (nj - 3);
+    int cb, q;
+    if (!f3x_item((int)((nf + 255) / 256), cb, q)) return;
+    const size_t t = (size_t)cb * 256 + threadIdx.x;
+    size_t p;
+    if (t >= nf || !face_of(ni, nj, 2, nk - 2, t, p)) return;
+    const FaceDir D = dir_of(f, q, nj, ni * nj);
+    double a, suadd = 0.0, fc = 0.0;
+    face_piso(f, D, ipc, p, a, suadd, fc);
+    double *const acfe = q == 0 ? ae : (q == 1 ? an : at);
+    double *const acfw = q == 0 ? aw : (q == 1 ? as : ab);
+    acfe[p] = a;
+    acfw[p + D.idew] = a;
+    (q == 0 ? be : (q == 1 ? bn : bt))[p] = suadd;
+    if (ipc > 0) (q == 0 ? f1 : (q == 1 ? f2 : f3))[p] = fc;
+}
+
+// each cell after k_piso3x: bp zero outside the face range; on it, su takes
+// the three directions' face source and (241-253) loses it again where
+// c - idew lies in k,i,j = 3..nkmm, 3..nimm, 3..njmm, in the calls' order
+__global__ __launch_bounds__(256) void k_piso3_su(int ni, int nj, int nk, int ipc,
+                                                  double *__restrict__ be, double *__restrict__ bn,
+                                                  double *__restrict__ bt, double *__restrict__ su)
+{
+    const size_t n = (size_t)ni * nj * nk;
     const Box F = {2, ni - 2, 2, nj - 2, 2, nk - 2};
     const Box C = {3, ni - 2, 3, nj - 2, 3, nk - 2};
-    const int k0 = 1 + (int)blockIdx.y * F3_KSEG, k1 = min(k0 + F3_KSEG - 1, nk);
-    for (int k = k0; k <= k1; ++k) {
-        const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
+        const int k = (int)(c / ((size_t)ni * nj)) + 1;
+        const int r = (int)(c % ((size_t)ni * nj));
+        const int i = r / nj + 1, j = r % nj + 1;
         if (!F.has(i, j, k)) {
-#pragma unroll
-            for (int q = 0; q < 3; ++q) o.bp[q][c] = 0.0;
+            be[c] = 0.0; bn[c] = 0.0; bt[c] = 0.0;
             continue;
         }
-        double s = ipc > 0 ? o.su[0][c] : 0.0;
+        if (ipc <= 0) continue;
+        const double b3[3] = {be[c], bn[c], bt[c]};
+        double s = su[c];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const FaceDir D = dir_of(f, q, nj, nij);
-            double a, suadd = 0.0, fc = 0.0;
-            face_piso(f, D, ipc, c, a, suadd, fc);
-            o.a[2 * q][c] = a;
-            o.a[2 * q + 1][c + D.idew] = a;
-            o.bp[q][c] = suadd;
-            if (ipc > 0) {
-                o.fc[q][c] = fc;
-                s = s + suadd;
-                if (C.has(i - (q == 0), j - (q == 1), k - (q == 2))) s = s - suadd;
-            }
+            s = s + b3[q];
+            if (C.has(i - (q == 0), j - (q == 1), k - (q == 2))) s = s - b3[q];
         }
-        if (ipc > 0) o.su[0][c] = s;
+        su[c] = s;
     }
 }
 
@@ -957,55 +1003,75 @@ __device__ __forceinline__ void face_uvw_cds(const Fld &f, const FaceDir &D, con
 // what the caller left in it for the east call, zero for the later calls.
 __global__ __launch_bounds__(256) void k_uvw3(int ni, int nj, int nk, Fld f, Out3 o, double gam)
 {
-    int i, j;
-    if (!f3_column(ni, nj, i, j)) return;
+    int i, j, k0;
+    if (!f3_column(ni, nj, i, j, k0)) return;
     const int nij = ni * nj, nimm = ni - 2, njmm = nj - 2, nkmm = nk - 2;
     const Box G = {2, nimm, 2, njmm, 2, nkmm};
     const Box F[3] = {{2, nimm, 3, njmm, 3, nkmm}, {3, nimm, 2, njmm, 3, nkmm},
                       {3, nimm, 3, njmm, 2, nkmm}};
-    const int k0 = 1 + (int)blockIdx.y * F3_KSEG, k1 = min(k0 + F3_KSEG - 1, nk);
+    const int k1 = min(k0 + F3_KSEG - 1, nk);
     for (int k = k0; k <= k1; ++k) {
         const size_t c = (size_t)(k - 1) * nij + (size_t)(i - 1) * nj + (size_t)(j - 1);
         double scr0[3];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            scr0[m] = o.bp[m][c];
-            o.bp[m][c] = 0.0;
-        }
+        for (int m = 0; m < 3; ++m) scr0[m] = o.bp[m][c];
         bool touched = false;
 #pragma unroll
         for (int q = 0; q < 3; ++q)
             touched |= F[q].has(i, j, k) || G.has(i - (q == 0), j - (q == 1), k - (q == 2));
-        if (!touched) continue;
+        if (!touched) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) o.bp[m][c] = 0.0;
+            continue;
+        }
+        // every face first (the own face's coefficient and source terms,
+        // the west/south/bottom face's coefficient and share), the stores
+        // after them
         double s[3] = {o.su[0][c], o.su[1][c], o.su[2][c]};
+        bool own[3], wst[3], shr[3];
+        double aown[3], awst[3], tg[3][3], tfe[3][3], tfi[3][3], sh[3][3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const FaceDir D = dir_of(f, q, nj, nij);
             double de, ce, cp, g[3], fde[3], fdi[3];
-            if (F[q].has(i, j, k)) {
+            own[q] = F[q].has(i, j, k);
+            if (own[q]) {
                 face_uvw_cds(f, D, o.vis, o.fcr[q], c, gam, de, ce, cp, g, fde, fdi);
-                o.a[2 * q][c] = de - ce;
+                aown[q] = de - ce;
 #pragma unroll
-                for (int m = 0; m < 3; ++m) s[m] = s[m] + g[m] + fde[m] - fdi[m];
+                for (int m = 0; m < 3; ++m) { tg[q][m] = g[m]; tfe[q][m] = fde[m]; tfi[q][m] = fdi[m]; }
             }
             const int iw = i - (q == 0), jw = j - (q == 1), kw = k - (q == 2);
-            if (G.has(iw, jw, kw)) {
-                double sh[3];
-                if (F[q].has(iw, jw, kw)) {
-                    face_uvw_cds(f, D, o.vis, o.fcr[q], c - D.idew, gam, de, ce, cp, g, fde, fdi);
-                    o.a[2 * q + 1][c] = de + cp;
+            shr[q] = G.has(iw, jw, kw);
+            wst[q] = F[q].has(iw, jw, kw);
+            if (wst[q]) {
+                face_uvw_cds(f, D, o.vis, o.fcr[q], c - D.idew, gam, de, ce, cp, g, fde, fdi);
+                awst[q] = de + cp;
 #pragma unroll
-                    for (int m = 0; m < 3; ++m) sh[m] = -g[m] - fde[m] + fdi[m];
-                } else {
+                for (int m = 0; m < 3; ++m) sh[q][m] = -g[m] - fde[m] + fdi[m];
+            } else {
 #pragma unroll
-                    for (int m = 0; m < 3; ++m) sh[m] = q == 0 ? scr0[m] : 0.0;
-                }
-#pragma unroll
-                for (int m = 0; m < 3; ++m) s[m] = s[m] + sh[m];
+                for (int m = 0; m < 3; ++m) sh[q][m] = q == 0 ? scr0[m] : 0.0;
             }
         }
 #pragma unroll
-        for (int m = 0; m < 3; ++m) o.su[m][c] = s[m];
+        for (int q = 0; q < 3; ++q) {
+            if (own[q]) {
+                o.a[2 * q][c] = aown[q];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) s[m] = s[m] + tg[q][m] + tfe[q][m] - tfi[q][m];
+            }
+            if (wst[q]) o.a[2 * q + 1][c] = awst[q];
+            if (shr[q]) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) s[m] = s[m] + sh[q][m];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            o.su[m][c] = s[m];
+            o.bp[m][c] = 0.0;
+        }
     }
 }
 
@@ -1418,8 +1484,8 @@ static int blocks_for(size_t n) { return (int)std::max<size_t>(1, std::min<size_
 // Device pointers only; queued on the context's stream, not synchronised.
 static dim3 f3_grid(const cap_ctx *ctx)
 {
-    const long cols = (long)ctx->ni * ctx->nj;
-    return dim3((unsigned)((cols + 255) / 256), (unsigned)((ctx->nk + F3_KSEG - 1) / F3_KSEG));
+    const int ncb = (int)(((long)ctx->ni * ctx->nj + 255) / 256), per = (ncb + 7) / 8;
+    return dim3((unsigned)(8 * per * ((ctx->nk + F3_KSEG - 1) / F3_KSEG)));
 }
 
 int cap_fluxmass3_q(cap_ctx *ctx, const cap_fields *F, double *const *a6, double *const *f3)
@@ -1428,11 +1494,10 @@ int cap_fluxmass3_q(cap_ctx *ctx, const cap_fields *F, double *const *a6, double
     Fld f;
     int rc;
     if ((rc = stage_fields(ctx, F, f))) return rc;
-    Out3 o = {};
-    for (int q = 0; q < 6; ++q) o.a[q] = a6[q];
-    for (int q = 0; q < 3; ++q) o.fc[q] = f3[q];
-    hipLaunchKernelGGL(k_fluxmass3, f3_grid(ctx), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj,
-                       ctx->nk, f, o);
+    const size_t nf = (size_t)(ctx->nk - 2) * (ctx->ni - 3) * (ctx->nj - 3);
+    hipLaunchKernelGGL(k_fluxmass3x, dim3(f3x_blocks((int)((nf + 255) / 256))), dim3(256), 0,
+                       ctx->stream, ctx->ni, ctx->nj, ctx->nk, f, a6[0], a6[1], a6[2], a6[3], a6[4],
+                       a6[5], f3[0], f3[1], f3[2]);
     HIPCHK(hipGetLastError());
     return CAP_OK;
 }
@@ -1444,12 +1509,12 @@ int cap_piso_assemble3_q(cap_ctx *ctx, const cap_fields *F, double *const *a6, d
     Fld f;
     int rc;
     if ((rc = stage_fields(ctx, F, f))) return rc;
-    Out3 o = {};
-    for (int q = 0; q < 6; ++q) o.a[q] = a6[q];
-    for (int q = 0; q < 3; ++q) { o.fc[q] = f3[q]; o.bp[q] = bp3[q]; }
-    o.su[0] = su;
-    hipLaunchKernelGGL(k_piso3, f3_grid(ctx), dim3(256), 0, ctx->stream, ctx->ni, ctx->nj, ctx->nk,
-                       f, o, ipc);
+    const size_t nf = (size_t)(ctx->nk - 3) * (ctx->ni - 3) * (ctx->nj - 3);
+    hipLaunchKernelGGL(k_piso3x, dim3(f3x_blocks((int)((nf + 255) / 256))), dim3(256), 0,
+                       ctx->stream, ctx->ni, ctx->nj, ctx->nk, f, ipc, a6[0], a6[1], a6[2], a6[3],
+                       a6[4], a6[5], f3[0], f3[1], f3[2], bp3[0], bp3[1], bp3[2]);
+    hipLaunchKernelGGL(k_piso3_su, dim3(blocks_for(ctx->nstd)), dim3(256), 0, ctx->stream, ctx->ni,
+                       ctx->nj, ctx->nk, ipc, bp3[0], bp3[1], bp3[2], su);
     HIPCHK(hipGetLastError());
     return CAP_OK;
 }
