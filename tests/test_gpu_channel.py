@@ -1,7 +1,9 @@
 """The channel (periodic x and z, walls at y mapped to the reference's k,
 tanh-stretched by 2.0, Re_bulk 5600) at the learned-guess configuration's
-size, 128 x 96 x 64, advanced by the library's routines on the device
-(cappuccino-ghostcells-mpi_amd/channel.py).  Size-independent properties:
+size, 128 x 96 x 64, and at the full-step configuration's, 384 x 256 x 192
+(two steps with the mixed-precision p' solve), advanced by the library's
+routines on the device (cappuccino-ghostcells-mpi_amd/channel.py).
+Size-independent properties:
 
 * the geometry producer's outward face vectors: +i, +j, +k;
 * the Courant-number step holds Co = 0.5 exactly;
@@ -31,19 +33,20 @@ def _channel_mod():
     return m
 
 
-@pytest.mark.parametrize("solver", [4, 6])
-def test_channel_steps(gpu, solver):
+@pytest.mark.parametrize("size,solver,steps", [((128, 96, 64), 4, 4), ((128, 96, 64), 6, 4),
+                                               ((384, 256, 192), 6, 2)])
+def test_channel_steps(gpu, size, solver, steps):
     import torch
     CH = _channel_mod()
-    print(f"channel 128x96x64 solver {solver}", flush=True)
-    ch = CH.Channel(gpu, torch, 128, 96, 64, solver=solver)
+    print(f"channel {size} solver {solver}", flush=True)
+    ch = CH.Channel(gpu, torch, *size, solver=solver)
     try:
         f = ch.f
         ni, nj, nk = ch.dims
         sh = (nk, ni, nj)
         assert float(f["ar1z"].min()) >= 0 and float(f["ar2x"].min()) >= 0
         assert float(f["ar3y"].min()) >= 0 and float(f["ar3y"].max()) > 0
-        for s in range(4):
+        for s in range(steps):
             reps = ch.step()
             L = ch.last
             print(f"  step {s}: Co {L['courant']:.6f} dt {L['dt']:.4e} iterations "

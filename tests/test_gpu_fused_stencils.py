@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 import flowcase as FC  # noqa: E402
 
-DIMS = [(12, 10, 9), (20, 17, 11), (37, 29, 23), (132, 100, 68)]
+DIMS = [(6, 7, 6), (12, 10, 9), (20, 17, 11), (37, 29, 23), (132, 100, 68)]
 OUT = ("ae", "aw", "an", "as", "at", "ab", "f1", "f2", "f3", "be", "bn", "bt", "su", "sv", "sw",
        "bp", "bb")   # "bt" is both PISO's top bp and fluxuvw's v scratch
 
