@@ -20,8 +20,9 @@
 //    taps x input channels in steps of 32.  Used forward, and with the
 //    weights mirrored in the taps and transposed in the channels for the
 //    gradient with respect to the input.
-//  * k_wgrad: weight and bias gradients on MFMA (K = voxels, staged in LDS
-//    transposed), one atomic add per weight and block of 4096 voxels.
+//  * k_wgrad: weight and bias gradients on MFMA (K = voxels of 16 x 4 x 4
+//    bricks, staged transposed in LDS once for all 27 taps), one partial sum
+//    per weight and block, added up in block order by k_wgrad_sum.
 //  * pooling, upsampling, gradient combination with the ReLU mask, loss,
 //    Adam.
 #include <hip/hip_runtime.h>
@@ -213,111 +214,255 @@ int conv_grid(int D, int H, int W, int bz)
 int conv_bz(int D, int H, int W) { return conv_grid(D, H, W, 4) >= 512 ? 4 : 1; }
 
 // dW[co][tap][ci] += sum_v G[v][co] X[v + off(tap)][ci] on the matrix cores
-// (C = A B with M = co, N = ci, K = voxels).  A block (4 waves) takes one tap
-// and WVB voxels in steps of 32 (one MFMA K step); each step stages G as
-// gt[co][32] and the shifted X (zero outside the grid) as xt[ci][32] in LDS,
-// so that a lane's 8 consecutive voxels of one channel are one 16-byte read.
-// Wave w accumulates the ci tiles w and w + 4 for all NCO co tiles; each
-// weight gets one atomic add per block.  Blocks of the centre tap also sum G
-// for the bias gradient db[co].
-constexpr int WVB = 4096;    // voxels per block
-constexpr int KST = 40;      // LDS row stride (bf16) of gt / xt
-constexpr int MAXCI = 112;   // largest input channel count of the network
+// (C = A B with M = co, N = ci, K = voxels).  The voxels go in bricks of
+// 16 x 4 x 4 (x fastest; K = the brick's 256 voxels in 8 steps of 32).  A
+// block (4 waves) takes one tile of 16 input channels and a strided set of
+// bricks; for each brick it stages G transposed (gt[co][voxel]) and the X
+// halo brick of its 16 channels transposed, in three copies shifted by
+// dx = -1, 0, +1 (xs[dx + 1][ci][z][y][x], x = 0..15), so that for every tap a
+// lane's 8 consecutive voxels of one channel are one aligned 16-byte read
+// (one copy read at 2-byte offsets, in the LDS's unaligned mode, took twice
+// as long).
+// Each brick is read from memory once for all 27 taps.  Wave w accumulates
+// the taps w, w + 4, ... for all NCO output tiles in registers across its
+// bricks, and writes its sums to its own slot of a partials buffer, which
+// k_wgrad_sum adds up in block order (no atomics, the same sums every run).
+// Wave 0 of the blocks of the first channel tile also sums G for the bias
+// gradient db[co], as G^T times a column of ones on the matrix cores.
+constexpr int WBX = 16, WBY = 4, WBZ = 4, WBV = WBX * WBY * WBZ;   // brick, 256 voxels
+constexpr int WHY = WBY + 2, WHZ = WBZ + 2, WHX = WBX + 2;          // halo extents
+constexpr int XCS = WHZ * WHY * WBX + 8;   // LDS stride (bf16) of one channel of one copy (584:
+                                           // 292 dwords, 4 mod 32, so 8 channels' reads miss each other's banks)
+constexpr int GST = WBV + 8;               // LDS stride (bf16) of one co row of gt
+constexpr int WTAPS = 7;                   // taps per wave (27 over 4 waves)
+
+struct WBricks {
+    int nbx, nby, nbz, nb;
+};
+__host__ __device__ inline WBricks wbricks(int D, int H, int W)
+{
+    WBricks r;
+    r.nbx = (W + WBX - 1) / WBX;
+    r.nby = (H + WBY - 1) / WBY;
+    r.nbz = (D + WBZ - 1) / WBZ;
+    r.nb = r.nbx * r.nby * r.nbz;
+    return r;
+}
+
+// keep the first n of the eight bf16 values of v (n <= 0: none)
+__device__ __forceinline__ uint4 keep8(uint4 v, int n)
+{
+    unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] &= (2 * k < n ? 0xFFFFu : 0u) | (2 * k + 1 < n ? 0xFFFF0000u : 0u);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ void put8(bf16_t *p, int stride, uint4 v)
+{
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        p[(2 * k) * stride] = (bf16_t)(w[k] & 0xFFFFu);
+        p[(2 * k + 1) * stride] = (bf16_t)(w[k] >> 16);
+    }
+}
+constexpr int WXI = (WHZ * WHY * WHX * 2 + 255) / 256;   // halo items (voxel, 8 channels) per thread
 
 template <int NCO>
 __global__ __launch_bounds__(256) void k_wgrad(int D, int H, int W, const bf16_t *__restrict__ X,
                                                int cx, int xoff, int cin,
                                                const bf16_t *__restrict__ G, int cg, int cout,
-                                               float *__restrict__ dW, float *__restrict__ db)
+                                               int ncit, int nbg, int want_bias,
+                                               float *__restrict__ part)
 {
-    __shared__ __attribute__((aligned(16))) bf16_t gt[NCO * 16 * KST];
-    __shared__ __attribute__((aligned(16))) bf16_t xt[MAXCI * KST];
-    const int tap = blockIdx.y;
-    int dz, dy, dx;
-    tap_off(tap, dz, dy, dx);
-    const int nvox = D * H * W;
-    const int vb0 = blockIdx.x * WVB;
-    const int vb1 = vb0 + WVB < nvox ? vb0 + WVB : nvox;
+    constexpr int NXI = WHZ * WHY * WHX * 2;          // halo items: voxel x two groups of 8 channels
+    constexpr int GIT = WBV * NCO * 2 / 256;          // G items (voxel, 8 channels) per thread
+    static_assert(WXI + GIT <= 32, "one validity bit per item in okm");
+    __shared__ __attribute__((aligned(16))) bf16_t gt[NCO * 16 * GST];
+    __shared__ __attribute__((aligned(16))) bf16_t xs[3 * 16 * XCS];
+    const WBricks B = wbricks(D, H, W);
+    const int cit = blockIdx.x % ncit, grp = blockIdx.x / ncit;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, kg = lane >> 4, col = lane & 15;
-    const int ncit = (cin + 15) / 16, nco16 = NCO * 16, nci16 = ncit * 16;
-    f32x4 acc[NCO][2];
+    const bool bias = want_bias && cit == 0;
+    // this wave's taps w, w + 4, ...; wave 3 has six, its seventh repeats
+    // tap 26 and is not added
+    int toff[WTAPS];
 #pragma unroll
-    for (int t = 0; t < NCO; ++t) acc[t][0] = acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-    float bsum = 0.f;
-    for (int v0 = vb0; v0 < vb1; v0 += 32) {
-        __syncthreads();   // the previous step's readers are done
-        // eight channels (16 bytes) per load; channels from cout / cin on are
-        // zero (cg, cx and xoff are multiples of 8, so a group that starts
-        // below cout / cin lies within its voxel's row)
-        for (int q = threadIdx.x; q < 32 * (nco16 / 8); q += 256) {
-            const int c0 = 8 * (q % (nco16 / 8)), k = q / (nco16 / 8), v = v0 + k;
-            bf16_t val[8];
+    for (int i = 0; i < WTAPS; ++i) {
+        const int tap = w + 4 * i < 27 ? w + 4 * i : 26;
+        int dz, dy, dx;
+        tap_off(tap, dz, dy, dx);
+        toff[i] = (dx + 1) * 16 * XCS + (dz * WHY + dy) * WBX;
+    }
+    f32x4 acc[WTAPS][NCO], accb[NCO];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) val[j] = 0;
-            if (c0 < cout && v < vb1) {
-                const uint4 raw = *reinterpret_cast<const uint4 *>(G + (long)v * cg + c0);
-                memcpy(val, &raw, 16);
+    for (int t = 0; t < NCO; ++t) {
+        accb[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (c0 + j >= cout) val[j] = 0;
-            }
+        for (int i = 0; i < WTAPS; ++i) acc[i][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    bf16x8 ones;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) gt[(c0 + j) * KST + k] = val[j];
+    for (int j = 0; j < 8; ++j) ones[j] = (__bf16)1.0f;
+
+    // a brick's values, read into registers one brick ahead of their use.
+    // Every lane loads (an item outside the grid or the channels reads the
+    // first element instead), so that the loads of a brick are in flight
+    // together; okm marks the items that hold data, and the zeros (outside
+    // the grid, from cin / cout on) are put in when the values go to LDS.
+    uint4 xr[WXI], gr[GIT];
+    unsigned okm = 0u;
+    auto load = [&](int br) {
+        const int bx = br % B.nbx, by = (br / B.nbx) % B.nby, bz = br / (B.nbx * B.nby);
+        const int x0 = bx * WBX, y0 = by * WBY, z0 = bz * WBZ;
+        okm = 0u;
+#pragma unroll
+        for (int i = 0; i < WXI; ++i) {
+            const int q = threadIdx.x + 256 * i;
+            const int hx = q % WHX, hy = (q / WHX) % WHY, hz = (q / (WHX * WHY)) % WHZ, g = q / (WHX * WHY * WHZ);
+            const int c0 = 16 * cit + 8 * g;
+            const int xx = x0 + hx - 1, yy = y0 + hy - 1, zz = z0 + hz - 1;
+            const bool ok = q < NXI && c0 < cin && xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D;
+            okm |= (unsigned)ok << i;
+            xr[i] = *reinterpret_cast<const uint4 *>(ok ? X + ((long)(zz * H + yy) * W + xx) * cx + xoff + c0 : X);
         }
-        for (int q = threadIdx.x; q < 32 * (nci16 / 8); q += 256) {
-            const int c0 = 8 * (q % (nci16 / 8)), k = q / (nci16 / 8), v = v0 + k;
-            bf16_t val[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) val[j] = 0;
-            if (c0 < cin && v < vb1) {
-                const int x = v % W, y = (v / W) % H, z = v / (W * H);
-                const int xx = x + dx, yy = y + dy, zz = z + dz;
-                if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D) {
-                    const uint4 raw =
-                        *reinterpret_cast<const uint4 *>(X + (long)((zz * H + yy) * W + xx) * cx + xoff + c0);
-                    memcpy(val, &raw, 16);
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        if (c0 + j >= cin) val[j] = 0;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) xt[(c0 + j) * KST + k] = val[j];
+        for (int i = 0; i < GIT; ++i) {
+            const int q = threadIdx.x + 256 * i;
+            const int v = q % WBV, c0 = 8 * (q / WBV);
+            const int x = x0 + v % WBX, y = y0 + (v / WBX) % WBY, z = z0 + v / (WBX * WBY);
+            const bool ok = c0 < cout && x < W && y < H && z < D;
+            okm |= (unsigned)ok << (WXI + i);
+            gr[i] = *reinterpret_cast<const uint4 *>(ok ? G + ((long)(z * H + y) * W + x) * cg + c0 : G);
         }
+    };
+    // registers -> LDS: G transposed (gt[co][voxel]); each halo voxel into
+    // the copies whose rows hold it (copy c at row position hx - c)
+    auto store = [&]() {
+        const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int i = 0; i < WXI; ++i) {
+            const int q = threadIdx.x + 256 * i;
+            if (q >= NXI) break;
+            const int hx = q % WHX, hy = (q / WHX) % WHY, hz = (q / (WHX * WHY)) % WHZ, g = q / (WHX * WHY * WHZ);
+            const uint4 val = (okm >> i) & 1u ? keep8(xr[i], cin - 16 * cit - 8 * g) : zero;
+            const int row = (hz * WHY + hy) * WBX;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int x = hx - c;
+                if (x >= 0 && x < WBX) put8(xs + (c * 16 + 8 * g) * XCS + row + x, XCS, val);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < GIT; ++i) {
+            const int q = threadIdx.x + 256 * i;
+            const int c0 = 8 * (q / WBV);
+            const uint4 val = (okm >> (WXI + i)) & 1u ? keep8(gr[i], cout - c0) : zero;
+            put8(gt + c0 * GST + q % WBV, GST, val);
+        }
+    };
+
+    int br = grp;
+    if (br < B.nb) load(br);
+    for (; br < B.nb; br += nbg) {
+        __syncthreads();   // the previous brick's readers are done
+        store();
         __syncthreads();
-        if (db && tap == 13 && (int)threadIdx.x < cout)
-            for (int k = 0; k < 32; ++k) bsum += bf2f(gt[threadIdx.x * KST + k]);
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int cit = w + 4 * c;
-            if (cit >= ncit) break;   // uniform across the wave
-            const uint4 rb = *reinterpret_cast<const uint4 *>(xt + (cit * 16 + col) * KST + 8 * kg);
-            bf16x8 bv;
-            memcpy(&bv, &rb, 16);
+        if (br + nbg < B.nb) load(br + nbg);   // in flight during this brick's products
+#pragma unroll 2
+        for (int st = 0; st < WBV / 32; ++st) {
+            // K step st: voxels 32 st .. 32 st + 31, i.e. rows r = 2 st and
+            // 2 st + 1 of the brick (r = 4 z + y); lane group kg takes row
+            // 2 st + kg / 2, x = 8 (kg % 2) .. + 7
+            bf16x8 av[NCO], bv[WTAPS];
 #pragma unroll
             for (int t = 0; t < NCO; ++t) {
-                const uint4 ra = *reinterpret_cast<const uint4 *>(gt + (t * 16 + col) * KST + 8 * kg);
-                bf16x8 av;
-                memcpy(&av, &ra, 16);
-                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t][c], 0, 0, 0);
+                const uint4 ra = *reinterpret_cast<const uint4 *>(gt + (t * 16 + col) * GST + 32 * st + 8 * kg);
+                memcpy(&av[t], &ra, 16);
             }
+            const int r = 2 * st + (kg >> 1), z = r / WBY, y = r % WBY;
+            const bf16_t *xb = xs + col * XCS + ((z + 1) * WHY + (y + 1)) * WBX + 8 * (kg & 1);
+#pragma unroll
+            for (int i = 0; i < WTAPS; ++i) {
+                const uint4 rb = *reinterpret_cast<const uint4 *>(xb + toff[i]);
+                memcpy(&bv[i], &rb, 16);
+            }
+            if (bias && w == 0) {   // db: G^T times a column of ones
+#pragma unroll
+                for (int t = 0; t < NCO; ++t)
+                    accb[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], ones, accb[t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < WTAPS; ++i)
+#pragma unroll
+                for (int t = 0; t < NCO; ++t)
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv[i], acc[i][t], 0, 0, 0);
         }
     }
-    // D[row][n]: lane holds rows 4*(lane/16) + j (co), column lane%16 (ci)
+    // D[row][n]: lane holds rows 4*(lane/16) + j (co), column lane%16 (ci);
+    // the block's sums go to its own slot of the partials
+    float *pw = part + (size_t)blockIdx.x * 27 * NCO * 256;
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const int cit = w + 4 * c;
-        if (cit >= ncit) break;
-        const int ci = cit * 16 + col;
+    for (int i = 0; i < WTAPS; ++i) {
+        const int tap = w + 4 * i;
+        if (tap >= 27) break;
 #pragma unroll
         for (int t = 0; t < NCO; ++t)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int co = t * 16 + 4 * kg + j;
-                if (co < cout && ci < cin) atomicAdd(&dW[((size_t)co * 27 + tap) * cin + ci], acc[t][c][j]);
-            }
+            for (int j = 0; j < 4; ++j) pw[((size_t)tap * NCO * 16 + t * 16 + 4 * kg + j) * 16 + col] = acc[i][t][j];
     }
-    if (db && tap == 13 && (int)threadIdx.x < cout) atomicAdd(&db[threadIdx.x], bsum);
+    if (bias && w == 0 && col == 0) {
+        float *pb = part + (size_t)ncit * nbg * 27 * NCO * 256 + (size_t)grp * NCO * 16;
+#pragma unroll
+        for (int t = 0; t < NCO; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) pb[t * 16 + 4 * kg + j] = accb[t][j];
+    }
+}
+
+// dW[co][tap][ci] += the blocks' partial sums, db[co] += the bias partials of
+// the first channel tile's blocks.  One block per (co, tap, channel tile) and
+// per 16 bias rows: 16 columns x 16 slices of the partials (slice k sums the
+// blocks k, k + 16, ... in order), then the slices in order; the same sums
+// every run.
+__global__ __launch_bounds__(256) void k_wgrad_sum(int cout, int cin, int nco16, int ncit, int nbg,
+                                                   const float *__restrict__ part, float *__restrict__ dW,
+                                                   float *__restrict__ db)
+{
+    __shared__ float red[16][17];
+    const int col = threadIdx.x & 15, sl = threadIdx.x >> 4;
+    const int nwb = cout * 27 * ncit;
+    const int b = blockIdx.x;
+    float sum = 0.f;
+    if (b < nwb) {
+        const int cit = b % ncit, tap = (b / ncit) % 27, co = b / (ncit * 27);
+        const float *p = part + ((size_t)tap * nco16 + co) * 16 + col;
+        const size_t gs = (size_t)27 * nco16 * 16;   // floats per block slot
+#pragma unroll 8
+        for (int g = sl; g < nbg; g += 16) sum += p[(size_t)(g * ncit + cit) * gs];
+        red[sl][col] = sum;
+        __syncthreads();
+        if (sl == 0) {
+            float s = 0.f;
+            for (int k = 0; k < 16; ++k) s += red[k][col];
+            const int ci = cit * 16 + col;
+            if (ci < cin) dW[((size_t)co * 27 + tap) * cin + ci] += s;
+        }
+    } else {
+        const int co = (b - nwb) * 16 + col;
+        const float *pb = part + (size_t)ncit * nbg * 27 * nco16 * 16;
+        if (co < cout) {
+#pragma unroll 8
+            for (int g = sl; g < nbg; g += 16) sum += pb[(size_t)g * nco16 + co];
+        }
+        red[sl][col] = sum;
+        __syncthreads();
+        if (sl == 0 && co < cout && db) {
+            float s = 0.f;
+            for (int k = 0; k < 16; ++k) s += red[k][col];
+            db[co] += s;
+        }
+    }
 }
 
 // 2x2x2 average pooling: out[Dc][Hc][Wc][c] from in (stride ci, offset ioff)
@@ -681,6 +826,7 @@ struct cap_unet {
         *gp2, *g4, *g3, *gp1, *g2, *g1;
     float *in3, *tgt;                  // staging of caller inputs (fp32)
     double *resp;                      // reduction partials
+    float *wpart;                      // weight-gradient partials, one slot per block
     double *gpad, *gtmp;               // guess correction and a sweep buffer, reference layout
                                        // (allocated by the first guess)
     std::vector<void *> allocs;
@@ -757,24 +903,43 @@ void conv_bwd_launch(cap_unet *n, int l, const bf16_t *G, bf16_t *dX, int cdx, c
 #undef CONV_BWD
 }
 
+// the weight-gradient launch of layer l: ncit channel tiles x nbg brick
+// groups, about two blocks a CU (one for four output tiles, whose LDS holds
+// one block a CU); returns the floats of partials it writes
+size_t wgrad_shape(const cap_unet *n, int l, int &nco, int &ncit, int &nbg)
+{
+    const ConvL &c = n->L[l];
+    const int D = n->nz >> c.level, H = n->ny >> c.level, W = n->nx >> c.level;
+    nco = (c.cout + 15) / 16;
+    ncit = (c.cin + 15) / 16;
+    const WBricks B = wbricks(D, H, W);
+    const int target = nco <= 2 ? 512 : 256;
+    nbg = (target + ncit - 1) / ncit;
+    nbg = nbg < B.nb ? nbg : B.nb;
+    return (size_t)ncit * nbg * 27 * nco * 256 + (size_t)nbg * nco * 16;
+}
+
 void wgrad_launch(cap_unet *n, int l, const bf16_t *X, int cx, int xoff, const bf16_t *G)
 {
     const ConvL &c = n->L[l];
     const int lv = c.level;
     const int D = n->nz >> lv, H = n->ny >> lv, W = n->nx >> lv;
-    const long nvox = (long)D * H * W;
-    const dim3 grid((unsigned)((nvox + WVB - 1) / WVB), 27);
+    int nco, ncit, nbg;
+    (void)wgrad_shape(n, l, nco, ncit, nbg);
+    const dim3 grid((unsigned)(ncit * nbg));
     hipStream_t s = n->ctx->stream;
 #define WGRAD(NCO)                                                                                  \
     hipLaunchKernelGGL(k_wgrad<NCO>, grid, dim3(256), 0, s, D, H, W, X, cx, xoff, c.cin, G, c.coutp8, \
-                       c.cout, n->g + c.woff, n->g + c.boff)
-    switch ((c.cout + 15) / 16) {
+                       c.cout, ncit, nbg, 1, n->wpart)
+    switch (nco) {
     case 1: WGRAD(1); break;
     case 2: WGRAD(2); break;
     case 3: WGRAD(3); break;
     case 4: WGRAD(4); break;
     }
 #undef WGRAD
+    hipLaunchKernelGGL(k_wgrad_sum, dim3((unsigned)(c.cout * 27 * ncit + nco)), dim3(256), 0, s, c.cout,
+                       c.cin, nco * 16, ncit, nbg, (const float *)n->wpart, n->g + c.woff, n->g + c.boff);
 }
 
 int pack_weights(cap_unet *n)
@@ -992,6 +1157,15 @@ extern "C" int cap_unet_create(cap_ctx *ctx, int nx, int ny, int nz, unsigned lo
         (rc = A((void **)&n->g1, n0 * 16 * B)) || (rc = A((void **)&n->in3, n0 * 3 * 4)) ||
         (rc = A((void **)&n->tgt, n0 * 4)) || (rc = A((void **)&n->resp, 16384 * 2 * 8)))
         goto fail;
+    {
+        size_t wp = 0;
+        for (int l = 0; l < NL; ++l) {
+            int nco, ncit, nbg;
+            const size_t f = wgrad_shape(n, l, nco, ncit, nbg);
+            wp = f > wp ? f : wp;
+        }
+        if ((rc = A((void **)&n->wpart, wp * 4))) goto fail;
+    }
     {   // He initialisation (counter-based normal deviates), zero biases
         std::vector<float> h(off, 0.f);
         for (int l = 0; l < NL; ++l) {

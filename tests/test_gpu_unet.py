@@ -64,7 +64,25 @@ def test_forward_matches_oracle(small):
 
 def test_gradients_match_oracle(small):
     ctx, net, shape = small
-    rng = np.random.default_rng(2)
+    _check_gradients(net, shape, 2)
+
+
+def test_gradients_match_oracle_many_bricks(gpu):
+    """A grid whose full-resolution level has 640 weight-gradient bricks of
+    16 x 4 x 4 voxels, more than the blocks of one launch (so a block sums
+    several bricks), and whose second level ends in a partial brick."""
+    shape = (32, 64, 80)          # nz, ny, nx
+    nz, ny, nx = shape
+    with gpu.Context(ny + 4, nx + 4, nz + 4) as ctx:
+        net = gpu.UNet(ctx, nx, ny, nz, seed=5)
+        try:
+            _check_gradients(net, shape, 4)
+        finally:
+            net.close()
+
+
+def _check_gradients(net, shape, seed):
+    rng = np.random.default_rng(seed)
     p = U.init_params(8)
     for w, b in U.split(p):
         b[:] = rng.normal(0, 0.05, b.size)
