@@ -120,42 +120,45 @@ __global__ __launch_bounds__(64 * BZ) void k_conv(int D, int H, int W, const bf1
     for (int ch = 0; ch < nchunk; ++ch) {
         const int c0 = ch * CC;
         __syncthreads();   // the previous chunk's readers are done
-        // the halo brick, in batches of 4 loads per thread issued before their stores
-        constexpr int NQ = HV * (CC / 8), NB = 4;
+        // the halo brick, in batches of up to 12 loads per thread, all issued
+        // before their stores: every lane loads (the first element when its
+        // item lies outside the grid or beyond cinp) and zeros replace those
+        // values at the store
+        constexpr int NQ = HV * (CC / 8), NQT = (NQ + NTH - 1) / NTH, NB = NQT < 12 ? NQT : 12;
         for (int q0 = 0; q0 < NQ; q0 += NB * NTH) {
             uint4 raw[NB];
+            unsigned okb = 0u;
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
                 const int q = q0 + u * NTH + (int)threadIdx.x;
-                raw[u] = make_uint4(0u, 0u, 0u, 0u);
                 const int hv = q / (CC / 8), g = q % (CC / 8);
                 const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
                 const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
                 const int ci = c0 + 8 * g;
-                if (q < NQ && ci < cinp && gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D) {
-                    const long v = ((long)gz * H + gy) * W + gx;
-                    raw[u] = *reinterpret_cast<const uint4 *>(X + v * cx + xoff + ci);
-                }
+                const bool ok = q < NQ && ci < cinp && gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D;
+                okb |= (unsigned)ok << u;
+                raw[u] = *reinterpret_cast<const uint4 *>(ok ? X + (((long)gz * H + gy) * W + gx) * cx + xoff + ci : X);
             }
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
                 const int q = q0 + u * NTH + (int)threadIdx.x;
-                if (q < NQ) *reinterpret_cast<uint4 *>(lds + (q / (CC / 8)) * VST + 8 * (q % (CC / 8))) = raw[u];
+                if (q < NQ)
+                    *reinterpret_cast<uint4 *>(lds + (q / (CC / 8)) * VST + 8 * (q % (CC / 8))) =
+                        (okb >> u) & 1u ? raw[u] : make_uint4(0u, 0u, 0u, 0u);
             }
         }
         __syncthreads();
         const bool kin = c0 + 8 * kg < cinp;
         // the weights of tap + WP are loaded while tap's MFMAs run (a ring of
         // WP taps in registers).  Every lane loads, from column 0 of its row
-        // when its channel group lies beyond cinp, so the loads need no branch.
+        // when its channel group lies beyond cinp, so the loads need no branch;
+        // those lanes' activations in LDS are zero, so the weight they read
+        // (a finite one) adds nothing
         uint4 wring[WP][NT];
         auto wload = [&](int tap, uint4 (&dst)[NT]) {
             const int k = kin ? tap * cinp + c0 + 8 * kg : 0;
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const uint4 rb = *reinterpret_cast<const uint4 *>(Wt + (size_t)(t * 16 + mrow) * kp + k);
-                dst[t] = kin ? rb : make_uint4(0u, 0u, 0u, 0u);
-            }
+            for (int t = 0; t < NT; ++t) dst[t] = *reinterpret_cast<const uint4 *>(Wt + (size_t)(t * 16 + mrow) * kp + k);
         };
 #pragma unroll
         for (int q = 0; q < WP; ++q) wload(q, wring[q]);
@@ -179,24 +182,44 @@ __global__ __launch_bounds__(64 * BZ) void k_conv(int D, int H, int W, const bf1
             }
         }
     }
-    // D[row][n]: lane holds rows 4*(lane/16) + j (voxel x = x0 + row), column n = lane%16
+    // D[row][n]: lane holds rows 4*(lane/16) + j (voxel x = x0 + row), column n = lane%16.
+    // The bias and mask values are read for every lane (at clamped indices)
+    // before they are used, a row m at a time, so their loads overlap.
     const int z = z0 + w;
+    float bvv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + mrow;
+        bvv[t] = bias ? bias[co < cout ? co : cout - 1] : 0.f;
+    }
+    const int zc = z < D ? z : D - 1;
 #pragma unroll
     for (int m = 0; m < BY; ++m) {
-        const int y = y0 + m;
+        const int y = y0 + m, yc = y < H ? y : H - 1;
+        bf16_t mv[NT][4];
+        if (M) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int co = t * 16 + mrow, coc = co < cout ? co : cout - 1;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int x = x0 + 4 * kg + j, xc = x < W ? x : W - 1;
+                    mv[t][j] = M[(((long)zc * H + yc) * W + xc) * cm + moff + coc];
+                }
+            }
+        }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int co = t * 16 + mrow;
             if (co >= cout || y >= H || z >= D) continue;
-            const float bvv = bias ? bias[co] : 0.f;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int x = x0 + 4 * kg + j;
                 if (x >= W) continue;
                 const long v = ((long)z * H + y) * W + x;
-                float r = acc[m][t][j] + bvv;
+                float r = acc[m][t][j] + bvv[t];
                 if (relu) r = r > 0.f ? r : 0.f;
-                if (M) r = bf2f(M[v * cm + moff + co]) > 0.f ? r : 0.f;
+                if (M) r = bf2f(mv[t][j]) > 0.f ? r : 0.f;
                 if (yf) yf[v * cout + co] = r;
                 else Y[v * cy + yoff + co] = f2bf(r);
             }
