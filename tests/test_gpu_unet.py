@@ -114,6 +114,25 @@ def _check_gradients(net, shape, seed):
         o += n
 
 
+def test_gradients_are_the_same_every_call(small):
+    """The weight gradients are summed without atomics (one partial per
+    block, added in block order), so the same inputs give the same bits."""
+    ctx, net, shape = small
+    rng = np.random.default_rng(6)
+    net.set_params(U.init_params(10).astype(np.float32))
+    x3 = rng.normal(size=(3,) + shape).astype(np.float32).ravel()
+    t = rng.normal(size=shape).astype(np.float32).ravel()
+    g1 = np.zeros(U.num_params(), np.float32)
+    g2 = np.zeros(U.num_params(), np.float32)
+    l1 = net.gradients(x3, t, g1)
+    l2 = net.gradients(x3, t, g2)
+    assert l1 == l2
+    assert np.array_equal(g1, g2)
+    # not a trivial pass: most weights get a gradient (taps that see only
+    # the zero padding of the 2 x 2 x 3 coarsest level, and dead ReLUs, do not)
+    assert np.count_nonzero(g1) > 0.5 * g1.size
+
+
 def test_training_lowers_the_loss(small):
     ctx, net, shape = small
     rng = np.random.default_rng(4)
