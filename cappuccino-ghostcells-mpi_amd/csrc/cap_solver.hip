@@ -44,10 +44,24 @@ __global__ __launch_bounds__(BLOCK) void k_gather(Lay L, const double *__restric
         const int ci = ti * TW + b, cj = tj * TW + a;
         const bool valid = col_valid(L, ci, cj);
         const int ns = min(MOVE_SLOTS, L.S - s0);
-        for (int j = 0; j < MOVE_SLOTS + TW - 1; ++j) {
+        // all row steps' loads first (a lane without an element at a step
+        // reads element 0), then the LDS stores, so that the loads are in
+        // flight together
+        constexpr int NJ = MOVE_SLOTS + TW - 1;
+        static_assert(NJ <= 64, "one bit per row step in okm");
+        double v[NJ];
+        unsigned long long okm = 0ull;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
             int r, kk;
-            if (move_step(L, s0, ns, b, a, j, r, kk))
-                buf[r * TCOL + lane] = valid ? src[std_index(L, ci, cj, kk)] : 0.0;
+            const bool ok = move_step(L, s0, ns, b, a, j, r, kk) && valid;
+            okm |= (unsigned long long)ok << j;
+            v[j] = src[ok ? std_index(L, ci, cj, kk) : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            int r, kk;
+            if (move_step(L, s0, ns, b, a, j, r, kk)) buf[r * TCOL + lane] = (okm >> j) & 1ull ? v[j] : 0.0;
         }
         __syncthreads();
         const size_t tb = tile_base(L, ti, tj);
@@ -74,7 +88,12 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(Lay L, const double *__restri
     const int ci = ti * TW + b, cj = tj * TW + a;
     const int ns = min(MOVE_SLOTS, L.S - s0);
     const size_t tb = tile_base(L, ti, tj);
-    for (int r = 0; r < ns; ++r) buf[r * TCOL + lane] = src[elem_at(tb, s0 + r, lane)];
+    double v[MOVE_SLOTS];   // the loads first (slot s0 again past ns), then the LDS stores
+#pragma unroll
+    for (int r = 0; r < MOVE_SLOTS; ++r) v[r] = src[elem_at(tb, s0 + (r < ns ? r : 0), lane)];
+#pragma unroll
+    for (int r = 0; r < MOVE_SLOTS; ++r)
+        if (r < ns) buf[r * TCOL + lane] = v[r];
     __syncthreads();
     if (!col_valid(L, ci, cj)) return;
     for (int j = 0; j < MOVE_SLOTS + TW - 1; ++j) {
